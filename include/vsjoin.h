@@ -1,0 +1,247 @@
+/*
+ * vsjoin.h -- C-ABI of libvsjoin.so, the MI355X (gfx950) spatial-join library.
+ *
+ * The reference (vibeSpatial) has no native library: its hot path is Python
+ * that JIT-compiles CUDA strings (NVRTC) and calls CCCL/CuPy.  Each entry point
+ * below replaces one step of that path; the comment names the reference code
+ * it replaces (paths relative to the reference's src/vibespatial/).  The
+ * Python binding that calls these is vibespatial_amd/_lib.py (ctypes).
+ *
+ * Conventions
+ *   - Pointers are DEVICE pointers unless marked "host".  Buffers are owned by
+ *     the caller (the Python layer allocates them as torch ROCm tensors).
+ *   - Coordinates are fp64 structure-of-arrays.  Polygonal rows use
+ *     GeoArrow-style int32 offsets (geometry/buffers.py:140-250):
+ *       geom_offsets[row..row+1] -> parts, part_offsets[part..part+1] -> rings,
+ *       ring_offsets[ring..ring+1] -> coordinates; rings are closed
+ *     (first == last coordinate).  A polygon is a 1-part row; a row with no
+ *     parts or no coordinates is empty and never matches (like null).
+ *   - Points with a NaN coordinate are null/empty and never match.
+ *   - `stream` is a hipStream_t (void* here); all work is enqueued on it and
+ *     nothing synchronises the host, except where a function says so.
+ *   - Return value: VS_OK (0) or a negative VS_ERR_* code; the message is
+ *     available from vs_last_error() on the calling thread.
+ *   - No entry point allocates device memory: work buffers come from the
+ *     caller as one `workspace` of at least the size the matching
+ *     *_workspace_bytes() query reports.
+ *   - Output of variable size (pairs) is written into caller buffers of
+ *     `capacity` pairs.  d_result (device int64[4]) receives
+ *       [0] number of output pairs K, [1] number of pairs produced before the
+ *       capacity check, [2] bbox candidate pairs examined (point joins),
+ *       [3] sum over those candidates of the polygon's vertex count.
+ *     If result[1] > capacity the outputs are not valid: call again with
+ *     capacity >= result[1].
+ *   - Thread safety: calls are reentrant; concurrent calls must use distinct
+ *     workspaces.
+ */
+#ifndef VSJOIN_H
+#define VSJOIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VS_OK 0
+#define VS_ERR_INVALID (-1)
+#define VS_ERR_HIP (-2)
+
+#define VS_MAX_LEVELS 16
+
+/* Predicates, as predicate(left, right) with points on the left and
+ * polygons on the right; mapping of location codes 0 outside / 1 boundary /
+ * 2 interior follows predicates/point_relations.py:100-129. */
+#define VS_PRED_WITHIN 0     /* location == 2 */
+#define VS_PRED_INTERSECTS 1 /* location != 0 */
+#define VS_PRED_COVERED_BY 2 /* location != 0 */
+#define VS_PRED_TOUCHES 3    /* location == 1 */
+#define VS_PRED_CONTAINS 4   /* polygon (left) contains point (right): location == 2 */
+#define VS_PRED_COVERS 5     /* polygon (left) covers point (right): location != 0 */
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int vs_version(void);
+
+/* Copies the calling thread's last error message (NUL-terminated) into the
+ * host buffer; returns the message length. */
+int vs_last_error(char* buf, int64_t capacity);
+
+/* ---------------------------------------------------------------------
+ * Geometry bounds
+ * --------------------------------------------------------------------- */
+
+/* Per-row fp64 bbox {minx, miny, maxx, maxy}; NaN for empty rows.
+ * Replaces compute_geometry_bounds_device (kernels/core/geometry_analysis.py:648)
+ * and its kernels bounds_polygon/bounds_multipolygon
+ * (kernels/core/geometry_analysis_source.py:31-212). */
+int vs_polygon_bounds(const double* x, const double* y, const int32_t* geom_offsets,
+                      const int32_t* part_offsets, const int32_t* ring_offsets, int64_t n_rows,
+                      double* bounds, void* stream);
+
+/* Degenerate boxes of points; NaN for null points.  Replaces bounds_simple
+ * (kernels/core/geometry_analysis_source.py:31-72). */
+int vs_point_boxes(const double* px, const double* py, int64_t n, double* boxes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Packed Hilbert tree over boxes
+ * --------------------------------------------------------------------- */
+
+/* Level layout of a packed tree: level 0 is the root, the last level holds the
+ * items (boxes in Hilbert order); the children of node j at level l are nodes
+ * [j*fanout, (j+1)*fanout) of level l+1. */
+typedef struct {
+    int32_t fanout;
+    int32_t n_levels;
+    int64_t n_items;
+    int64_t n_nodes; /* total boxes over all levels, items included */
+    int64_t level_start[VS_MAX_LEVELS];
+    int64_t level_count[VS_MAX_LEVELS];
+} vs_tree_layout;
+
+/* Host-only: fills the layout for n_items and fanout (2..64). */
+int vs_tree_plan(int64_t n_items, int32_t fanout, vs_tree_layout* layout);
+
+int64_t vs_index_workspace_bytes(int64_t n_items);
+
+/* Builds the tree from per-item boxes: 64-bit Hilbert keys of box centres over
+ * the centre extent (computed on the device, no host fence), stable radix
+ * sort, items gathered in key order, parents bottom up.
+ *   node_boxes: 4 * layout->n_nodes doubles; item_ids: n_items original rows;
+ *   item_keys (optional, may be NULL): sorted keys.
+ * Replaces build_flat_spatial_index (spatial/indexing.py:1357-1533),
+ * morton_keys_from_bounds (spatial/indexing_kernels.py:20-49), CCCL sort_pairs
+ * (cuda/cccl_primitives.py:1054) and _build_packed_rtree
+ * (io/gpu_parse/indexing.py:262-400). */
+int vs_index_build(const double* bounds, int64_t n_items, const vs_tree_layout* layout,
+                   double* node_boxes, int32_t* item_ids, uint64_t* item_keys, void* workspace,
+                   int64_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Joins and queries
+ * --------------------------------------------------------------------- */
+
+int64_t vs_sjoin_points_polygons_workspace_bytes(int64_t n_points, int64_t capacity);
+
+/* sjoin(points, polygons, predicate): all (point, polygon) pairs with
+ * predicate(point, polygon) true, lexsorted, as int64 (left_index, right_index).
+ * The polygon tree comes from vs_index_build over the polygons' bounds.
+ * predicate: VS_PRED_WITHIN | VS_PRED_INTERSECTS | VS_PRED_COVERED_BY | VS_PRED_TOUCHES.
+ * Exact (no tolerance): boundary = the point lies on an edge.
+ * Replaces query_spatial_index's point-x-polygon route (spatial/query.py:247-1042):
+ * spatial_index_device_query (spatial/spatial_index_device.py:228-390), the
+ * point refine classify_point_predicates_indexed_device
+ * (predicates/point_relations.py:1040-1191) and the pair compaction
+ * (spatial/query_utils.py:867-868, spatial/query.py:143-153). */
+int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_points, const double* x,
+                             const double* y, const int32_t* geom_offsets, const int32_t* part_offsets,
+                             const int32_t* ring_offsets, int64_t n_polys, const vs_tree_layout* layout,
+                             const double* node_boxes, const int32_t* item_ids, int32_t predicate,
+                             int64_t capacity, int64_t* out_left, int64_t* out_right, int64_t* d_result,
+                             void* workspace, int64_t workspace_bytes, void* stream);
+
+int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity);
+
+/* Bounding-box query: every (query, tree item) pair whose boxes overlap
+ * (inclusive; NaN boxes never match), lexsorted, int64.  query_boxes: 4 doubles
+ * per query.  Replaces spatial_index_device_query's candidate output
+ * (spatial/spatial_index_device.py:228-390) and the bbox_overlap kernels
+ * (kernels/core/spatial_query_source.py:415-555). */
+int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_layout* layout,
+                   const double* node_boxes, const int32_t* item_ids, int64_t capacity, int64_t* out_left,
+                   int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
+                   void* stream);
+
+int64_t vs_refine_pairs_workspace_bytes(int64_t n_pairs);
+
+/* Exact refine of candidate pairs (left[i], right[i]) followed by an
+ * order-preserving compaction into out_left/out_right (capacity n_pairs);
+ * d_result[0] (device int64) receives the number kept.
+ *   mode 0: left rows index points (px, py), right rows polygons B;
+ *           keep = predicate(location of the point in the polygon), with
+ *           VS_PRED_WITHIN / VS_PRED_CONTAINS: interior, VS_PRED_TOUCHES:
+ *           boundary, otherwise: not outside.
+ *   mode 1: left rows polygons A, right rows polygons B; keep = intersects.
+ *   mode 2: left rows polygons B, right rows points (px, py); keep as mode 0
+ *           for the right point located in the left polygon (contains,
+ *           covers, contains_properly, touches, intersects of polygons over
+ *           points, api/sindex.py:144-503 with the swapped mapping of
+ *           predicates/point_relations.py:130-146).
+ * Replaces _filter_predicate_pairs_owned's device refine routes
+ * (spatial/query_utils.py:578-1147), compute_polygonal_intersects_gpu
+ * (predicates/polygon.py:849-996) and the boolean compaction
+ * (spatial/query_utils.py:867-868). */
+int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const int64_t* right, int64_t n_pairs,
+                    const double* px, const double* py, const double* ax, const double* ay,
+                    const int32_t* a_geom, const int32_t* a_part, const int32_t* a_ring, const double* bx,
+                    const double* by, const int32_t* b_geom, const int32_t* b_part, const int32_t* b_ring,
+                    int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                    int64_t workspace_bytes, void* stream);
+
+int64_t vs_lexsort_pairs_workspace_bytes(int64_t n_pairs);
+
+/* In-place lexsort of int64 pairs by (left, right); values must be in
+ * [0, 2^31).  Replaces cp.lexsort (spatial/query.py:105-115, 143-153). */
+int vs_lexsort_pairs(int64_t* left, int64_t* right, int64_t n_pairs, void* workspace, int64_t workspace_bytes,
+                     void* stream);
+
+/* ---------------------------------------------------------------------
+ * Primitives
+ * --------------------------------------------------------------------- */
+
+int64_t vs_scan_workspace_bytes(int64_t n);
+
+/* out[i] = sum(in[0..i)), *d_total = sum(in) (device int64).  In-place allowed.
+ * Replaces CCCL exclusive_sum (cuda/cccl_primitives.py:596). */
+int vs_exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* d_total,
+                          void* workspace, int64_t workspace_bytes, void* stream);
+
+/* key_bytes: 4 or 8. */
+int64_t vs_sort_workspace_bytes(int64_t n, int32_t key_bytes);
+
+/* Stable LSD radix sort of (key, value) in place by bits [begin_bit, end_bit).
+ * Replaces CCCL sort_pairs (cuda/cccl_primitives.py:1054). */
+int vs_sort_pairs_u64(uint64_t* keys, int32_t* vals, int64_t n, int32_t begin_bit, int32_t end_bit,
+                      void* workspace, int64_t workspace_bytes, void* stream);
+int vs_sort_pairs_u32(uint32_t* keys, int32_t* vals, int64_t n, int32_t begin_bit, int32_t end_bit,
+                      void* workspace, int64_t workspace_bytes, void* stream);
+
+int64_t vs_point_order_workspace_bytes(int64_t n_points);
+
+/* Hilbert order of points (keys over the point extent, bits_per_axis <= 16):
+ * perm[i] = original index of the i-th point in order; sorted_x/y gathered. */
+int vs_point_order(const double* px, const double* py, int64_t n_points, int32_t bits_per_axis,
+                   int32_t* perm, double* sorted_x, double* sorted_y, void* workspace,
+                   int64_t workspace_bytes, void* stream);
+
+/* Benchmark instrumentation (no reference counterpart): when enabled, the
+ * traversal kernel of the next join/query is bracketed by two HIP events
+ * recorded on the caller's stream; vs_last_kernel_ms waits for the second
+ * event and returns the elapsed milliseconds.  Process-wide, not reentrant. */
+int vs_set_kernel_timing(int32_t enable);
+int vs_last_kernel_ms(float* ms);
+
+/* ---------------------------------------------------------------------
+ * Host-side functions (no GPU needed): the device arithmetic compiled for the
+ * host, so unit tests can check it without a GPU.
+ * --------------------------------------------------------------------- */
+
+/* Exact orientation sign of (a, b, c) (orient2d.py:37-84 sign convention). */
+int vs_host_orient2d(double ax, double ay, double bx, double by, double cx, double cy);
+
+/* Location code (0/1/2) of (px, py) in polygonal row `row`; host pointers. */
+int vs_host_point_location(double px, double py, const double* x, const double* y,
+                           const int32_t* geom_offsets, const int32_t* part_offsets,
+                           const int32_t* ring_offsets, int64_t row);
+
+/* Exact closed-segment touch-or-cross (predicates/polygon_kernels.py:131-161). */
+int vs_host_segments_touch(double ax, double ay, double bx, double by, double cx, double cy,
+                           double dx, double dy);
+
+/* 64-bit Hilbert index of 32-bit grid coordinates. */
+uint64_t vs_host_hilbert64(uint32_t x, uint32_t y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VSJOIN_H */

@@ -1,0 +1,211 @@
+"""ctypes wrapper over liboracle.so (test infrastructure only).
+
+See oracle.c for what each function restates and how it is pinned.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+__all__ = [
+    "lib",
+    "build",
+    "orient2d",
+    "orient2d_exact",
+    "point_location",
+    "point_location_pairs",
+    "segments_touch",
+    "polygon_intersects",
+    "polygon_bounds",
+    "bbox_pairs",
+    "hilbert16",
+    "hilbert_textbook",
+    "sjoin_points_polygons",
+    "sjoin_polygons_polygons",
+    "box_query_points",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "liboracle.so")
+_lock = threading.RLock()
+_lib = None
+
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+_p = ctypes.c_void_p
+
+
+def build() -> str:
+    """Compile liboracle.so with make (gcc); returns its path."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
+    return _SO
+
+
+def lib():
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(_SO):
+                build()
+            L = ctypes.CDLL(_SO)
+            L.or_orient2d.argtypes = [_d] * 6
+            L.or_orient2d.restype = ctypes.c_int
+            L.or_orient2d_exact.argtypes = [_d] * 6
+            L.or_orient2d_exact.restype = ctypes.c_int
+            L.or_point_location.argtypes = [_d, _d, _p, _p, _p, _p, _p, _i64]
+            L.or_point_location.restype = ctypes.c_int
+            L.or_point_location_pairs.argtypes = [_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, ctypes.c_int]
+            L.or_point_location_pairs.restype = None
+            L.or_segments_touch.argtypes = [_d] * 8
+            L.or_segments_touch.restype = ctypes.c_int
+            L.or_polygon_intersects.argtypes = [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64]
+            L.or_polygon_intersects.restype = ctypes.c_int
+            L.or_polygon_bounds.argtypes = [_p, _p, _p, _p, _p, _i64, _p]
+            L.or_polygon_bounds.restype = None
+            L.or_bbox_pairs.argtypes = [_p, _i64, _p, _i64, _p, _p]
+            L.or_bbox_pairs.restype = _i64
+            L.or_hilbert16.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+            L.or_hilbert16.restype = ctypes.c_uint32
+            L.or_hilbert16_batch.argtypes = [_p, _p, _i64, _p]
+            L.or_hilbert16_batch.restype = None
+            L.or_hilbert_textbook.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
+            L.or_hilbert_textbook.restype = ctypes.c_uint64
+            L.or_sjoin_points_polygons.argtypes = [_p, _p, _i64, _p, _p, _p, _p, _p, _i64, ctypes.c_int, ctypes.c_int]
+            L.or_sjoin_points_polygons.restype = _i64
+            L.or_sjoin_polygons_polygons.argtypes = [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, ctypes.c_int]
+            L.or_sjoin_polygons_polygons.restype = _i64
+            L.or_box_query_points.argtypes = [_p, _p, _i64, _p, _i64, ctypes.c_int]
+            L.or_box_query_points.restype = _i64
+            L.or_join_take.argtypes = [_p, _p]
+            L.or_join_take.restype = None
+            _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def orient2d(ax, ay, bx, by, cx, cy) -> int:
+    return lib().or_orient2d(ax, ay, bx, by, cx, cy)
+
+
+def orient2d_exact(ax, ay, bx, by, cx, cy) -> int:
+    return lib().or_orient2d_exact(ax, ay, bx, by, cx, cy)
+
+
+class _Polys:
+    """Holds contiguous polygon buffers alive for a ctypes call."""
+
+    def __init__(self, polys):
+        self.x = _f64(polys.x)
+        self.y = _f64(polys.y)
+        self.g = _i32(polys.geom_offsets)
+        self.p = _i32(polys.part_offsets)
+        self.r = _i32(polys.ring_offsets)
+        self.n = len(self.g) - 1
+
+    def args(self):
+        return [_ptr(self.x), _ptr(self.y), _ptr(self.g), _ptr(self.p), _ptr(self.r)]
+
+
+def point_location(px: float, py: float, polys, row: int) -> int:
+    P = _Polys(polys)
+    return lib().or_point_location(px, py, *P.args(), row)
+
+
+def point_location_pairs(px, py, left, right, polys, nthreads: int = 0) -> np.ndarray:
+    P = _Polys(polys)
+    px, py = _f64(px), _f64(py)
+    li = np.ascontiguousarray(left, dtype=np.int64)
+    ri = np.ascontiguousarray(right, dtype=np.int64)
+    out = np.zeros(len(li), dtype=np.uint8)
+    lib().or_point_location_pairs(_ptr(px), _ptr(py), _ptr(li), _ptr(ri), len(li), *P.args(), _ptr(out), nthreads)
+    return out
+
+
+def segments_touch(a, b, c, d) -> bool:
+    return bool(lib().or_segments_touch(*a, *b, *c, *d))
+
+
+def polygon_intersects(A, ra: int, B, rb: int) -> bool:
+    PA, PB = _Polys(A), _Polys(B)
+    return bool(lib().or_polygon_intersects(*PA.args(), ra, *PB.args(), rb))
+
+
+def polygon_bounds(polys) -> np.ndarray:
+    P = _Polys(polys)
+    out = np.empty((P.n, 4), dtype=np.float64)
+    lib().or_polygon_bounds(*P.args(), P.n, _ptr(out))
+    return out
+
+
+def bbox_pairs(qbounds, tbounds):
+    qb = _f64(qbounds).reshape(-1, 4)
+    tb = _f64(tbounds).reshape(-1, 4)
+    n = lib().or_bbox_pairs(_ptr(qb), len(qb), _ptr(tb), len(tb), None, None)
+    lq = np.empty(n, dtype=np.int64)
+    lt = np.empty(n, dtype=np.int64)
+    lib().or_bbox_pairs(_ptr(qb), len(qb), _ptr(tb), len(tb), _ptr(lq), _ptr(lt))
+    return lq, lt
+
+
+def hilbert16(x, y) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    y = np.ascontiguousarray(y, dtype=np.uint32)
+    out = np.empty(len(x), dtype=np.uint32)
+    lib().or_hilbert16_batch(_ptr(x), _ptr(y), len(x), _ptr(out))
+    return out
+
+
+def hilbert_textbook(order: int, x: int, y: int) -> int:
+    return lib().or_hilbert_textbook(order, x, y)
+
+
+def _take(n: int):
+    left = np.empty(n, dtype=np.int64)
+    right = np.empty(n, dtype=np.int64)
+    lib().or_join_take(_ptr(left), _ptr(right))
+    return left, right
+
+
+_PRED = {"within": 0, "intersects": 1, "covered_by": 2, "touches": 3}
+
+
+def sjoin_points_polygons(px, py, polys, predicate: str = "within", nthreads: int = 0):
+    """Lexsorted (point_idx, poly_idx) int64 pairs satisfying predicate(point, polygon)."""
+    P = _Polys(polys)
+    px, py = _f64(px), _f64(py)
+    with _lock:
+        n = lib().or_sjoin_points_polygons(_ptr(px), _ptr(py), len(px), *P.args(), P.n, _PRED[predicate], nthreads)
+        return _take(n)
+
+
+def sjoin_polygons_polygons(A, B, nthreads: int = 0):
+    """Lexsorted (left_idx, right_idx) int64 pairs with intersects(A[l], B[r])."""
+    PA, PB = _Polys(A), _Polys(B)
+    with _lock:
+        n = lib().or_sjoin_polygons_polygons(*PA.args(), PA.n, *PB.args(), PB.n, nthreads)
+        return _take(n)
+
+
+def box_query_points(px, py, boxes, nthreads: int = 0):
+    """Lexsorted (box_idx, point_idx) pairs with the point inside the box (inclusive)."""
+    px, py = _f64(px), _f64(py)
+    b = _f64(boxes).reshape(-1, 4)
+    with _lock:
+        n = lib().or_box_query_points(_ptr(px), _ptr(py), len(px), _ptr(b), len(b), nthreads)
+        return _take(n)

@@ -1,0 +1,132 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol
+include/vsjoin.h declares, and its host-compiled arithmetic agrees with the
+oracle; layout and workspace queries behave."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+import oracle
+from oracle import fraction_ref as FR
+from vibespatial_amd import _lib
+from vibespatial_amd import synthetic as S
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "vsjoin.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing
+    # and the ctypes table binds exactly the declared set
+    assert sorted(_lib.SIGNATURES) == declared
+
+
+def test_version_and_error_channel():
+    L = _lib.lib()
+    assert L.vs_version() >= 100
+    lay = _lib.TreeLayout()
+    rc = L.vs_tree_plan(10, 1, ctypes.byref(lay))
+    assert rc == -1
+    assert "fanout" in _lib.last_error()
+    with pytest.raises(_lib.VsjoinError):
+        _lib.check(rc, "vs_tree_plan")
+
+
+def test_host_orient_matches_rationals():
+    L = _lib.lib()
+    rng = np.random.default_rng(11)
+    for _ in range(3000):
+        a = rng.uniform(-50, 50, 4)
+        t = rng.uniform(-1, 2)
+        c = a[:2] + t * (a[2:] - a[:2])
+        c[0] = np.nextafter(c[0], np.inf * rng.choice([-1, 1])) if rng.random() < 0.5 else c[0]
+        args = (*a, *c)
+        assert L.vs_host_orient2d(*args) == FR.orient(*args)
+
+
+def test_host_point_location_matches_oracle():
+    L = _lib.lib()
+    rng = np.random.default_rng(12)
+    m = 30
+    holes = rng.integers(0, 4, m)
+    nh = int(holes.sum())
+    P = S.star_polygons(rng, rng.uniform(0, 100, m), rng.uniform(0, 100, m), rng.uniform(5, 20, m),
+                        rng.integers(4, 40, m), holes=holes, hole_vertices=rng.integers(4, 9, nh),
+                        hole_frac=rng.uniform(0.1, 0.4, nh))
+    x, y = P.x, P.y
+    g, p, r = P.geom_offsets, P.part_offsets, P.ring_offsets
+    ptrs = [a.ctypes.data_as(ctypes.c_void_p) for a in (x, y, g, p, r)]
+    for q in range(m):
+        b = P.scalar(q).bounds
+        pts = list(zip(rng.uniform(b[0], b[2], 40), rng.uniform(b[1], b[3], 40)))
+        c0 = P.ring_offsets[P.part_offsets[P.geom_offsets[q]]]
+        pts += [(P.x[c0 + k], P.y[c0 + k]) for k in range(3)]
+        for px_, py_ in pts:
+            assert L.vs_host_point_location(px_, py_, *ptrs, q) == oracle.point_location(px_, py_, P, q)
+
+
+def test_host_segments_touch_matches_oracle():
+    L = _lib.lib()
+    rng = np.random.default_rng(13)
+    for _ in range(3000):
+        v = rng.integers(0, 4, 8).astype(float)
+        assert L.vs_host_segments_touch(*v) == int(oracle.segments_touch(v[0:2], v[2:4], v[4:6], v[6:8]))
+
+
+def test_host_hilbert64_extends_the_16_level_code():
+    L = _lib.lib()
+    rng = np.random.default_rng(14)
+    xs = rng.integers(0, 2**32, 2000, dtype=np.uint64)
+    ys = rng.integers(0, 2**32, 2000, dtype=np.uint64)
+    k64 = np.array([L.vs_host_hilbert64(int(a), int(b)) for a, b in zip(xs, ys)], dtype=np.uint64)
+    k16 = oracle.hilbert16((xs >> np.uint64(16)).astype(np.uint32), (ys >> np.uint64(16)).astype(np.uint32))
+    assert np.array_equal((k64 >> np.uint64(32)).astype(np.uint32), k16)
+    # a full 2^6 grid placed at the low bits: bijective and 4-adjacent
+    n = 64
+    gx, gy = np.meshgrid(np.arange(n), np.arange(n))
+    gx, gy = gx.ravel(), gy.ravel()
+    k = np.array([L.vs_host_hilbert64(int(a), int(b)) for a, b in zip(gx, gy)], dtype=np.uint64)
+    assert np.array_equal(np.sort(k), np.arange(n * n, dtype=np.uint64))
+    o = np.argsort(k)
+    assert np.all(np.abs(np.diff(gx[o])) + np.abs(np.diff(gy[o])) == 1)
+
+
+@pytest.mark.parametrize("n,fan", [(0, 16), (1, 16), (15, 16), (16, 16), (17, 16), (100_000, 16), (1000, 2)])
+def test_tree_plan_layout(n, fan):
+    L = _lib.lib()
+    lay = _lib.TreeLayout()
+    _lib.check(L.vs_tree_plan(n, fan, ctypes.byref(lay)), "vs_tree_plan")
+    nl = lay.n_levels
+    counts = [lay.level_count[i] for i in range(nl)]
+    starts = [lay.level_start[i] for i in range(nl)]
+    assert counts[-1] == n and counts[0] == (1 if n else 0)
+    for l in range(nl - 1):
+        assert counts[l] == -(-counts[l + 1] // fan)
+        assert starts[l + 1] == starts[l] + counts[l]
+    assert lay.n_nodes == sum(counts)
+
+
+def test_workspace_queries_are_positive_and_monotone():
+    L = _lib.lib()
+    for fn in ("vs_index_workspace_bytes", "vs_scan_workspace_bytes", "vs_point_order_workspace_bytes",
+               "vs_refine_pairs_workspace_bytes", "vs_lexsort_pairs_workspace_bytes"):
+        f = getattr(L, fn)
+        a, b = f(1000), f(10_000_000)
+        assert 0 < a <= b, fn
+    assert 0 < L.vs_sjoin_points_polygons_workspace_bytes(1000, 10) < L.vs_sjoin_points_polygons_workspace_bytes(
+        1000, 10**6)
+    assert L.vs_sort_workspace_bytes(10**6, 4) < L.vs_sort_workspace_bytes(10**6, 8)

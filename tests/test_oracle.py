@@ -1,0 +1,282 @@
+"""Pins the CPU oracle (test infrastructure) before it is trusted as the checker.
+
+Each oracle function is checked against something it does not share code
+with: exact rationals (fractions.Fraction) with an independent formulation,
+the textbook Hilbert algorithm, numpy brute force, the reference's literal
+vectors, and the GEOS counts recorded by the reference's upstream tests.
+"""
+
+from __future__ import annotations
+
+import os
+import zipfile
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, decode_geom, geom_parts
+
+import oracle
+from oracle import fraction_ref as FR
+from vibespatial_amd import synthetic as S
+from vibespatial_amd.geometry import PolygonArray
+from vibespatial_amd.io import read_shapefile_bytes
+
+
+# ---------------------------------------------------------------------------
+# orientation
+# ---------------------------------------------------------------------------
+def _near_collinear(rng, n, scale, offset=0.0):
+    a = rng.uniform(-scale, scale, (n, 4)) + offset
+    t = rng.uniform(-1.5, 2.5, n)
+    cx = a[:, 0] + t * (a[:, 2] - a[:, 0])
+    cy = a[:, 1] + t * (a[:, 3] - a[:, 1])
+    nudge = rng.integers(-2, 3, n)
+    cx = np.array([np.nextafter(v, np.inf) if k > 0 else (np.nextafter(v, -np.inf) if k < 0 else v)
+                   for v, k in zip(cx, nudge)])
+    return np.column_stack([a, cx, cy])
+
+
+@pytest.mark.parametrize("scale,offset", [(1.0, 0.0), (1e3, 0.0), (300.0, 5e5), (1e-4, 0.0), (1e6, 3e6)])
+def test_orient_exact_matches_rationals(scale, offset):
+    rng = np.random.default_rng(int(scale * 7 + offset) % 2**32)
+    trip = _near_collinear(rng, 3000, scale, offset)
+    for ax, ay, bx, by, cx, cy in trip:
+        e = FR.orient(ax, ay, bx, by, cx, cy)
+        assert oracle.orient2d_exact(ax, ay, bx, by, cx, cy) == e
+        assert oracle.orient2d(ax, ay, bx, by, cx, cy) == e
+
+
+def test_orient_exact_on_small_integer_grid():
+    # every triple on a 5x5 grid (lots of exact zeros), plus a half-ulp shift
+    g = [(float(x), float(y)) for x in range(5) for y in range(5)]
+    rng = np.random.default_rng(1)
+    for _ in range(4000):
+        a, b, c = (g[i] for i in rng.integers(0, len(g), 3))
+        assert oracle.orient2d_exact(*a, *b, *c) == FR.orient(*a, *b, *c)
+
+
+def test_orient_a_dropped_tail_is_detected():
+    # (1 + 2^-52) style inputs where the rounded determinant is 0 but the exact
+    # one is not: an implementation that drops the product tails answers 0.
+    ax, ay = 1.0 + 2.0**-52, 1.0 + 2.0**-52
+    bx, by = 2.0 + 2.0**-51, 2.0 + 2.0**-51 + 2.0**-51
+    cx, cy = 0.0, 0.0
+    e = FR.orient(ax, ay, bx, by, cx, cy)
+    assert e != 0
+    assert oracle.orient2d_exact(ax, ay, bx, by, cx, cy) == e
+
+
+# ---------------------------------------------------------------------------
+# segments and polygon intersects
+# ---------------------------------------------------------------------------
+def test_segments_touch_matches_rationals():
+    rng = np.random.default_rng(2)
+    for _ in range(6000):
+        pts = rng.integers(0, 4, (4, 2)).astype(float)
+        a, b, c, d = map(tuple, pts)
+        assert oracle.segments_touch(a, b, c, d) == FR.segments_touch(a, b, c, d), (a, b, c, d)
+
+
+def test_polygon_intersects_golden(golden):
+    for case in golden["polygon_intersects"]:
+        A = PolygonArray.from_geometries([decode_geom(g) for g in case["left"]])
+        B = PolygonArray.from_geometries([decode_geom(g) for g in case["right"]])
+        got = [oracle.polygon_intersects(A, i, B, j) for i, j in case["pairs"]]
+        assert got == case["expected"], case["source"]
+
+
+def test_polygon_intersects_is_symmetric_and_matches_rationals():
+    rng = np.random.default_rng(3)
+    cx, cy = rng.uniform(0, 20, 12), rng.uniform(0, 20, 12)
+    P = S.star_polygons(rng, cx, cy, rng.uniform(1, 4, 12), rng.integers(4, 9, 12))
+    for i in range(12):
+        for j in range(12):
+            a = oracle.polygon_intersects(P, i, P, j)
+            assert a == oracle.polygon_intersects(P, j, P, i)
+            # independent: any edge pair touches, or a vertex of one covers the other
+            gi, gj = geom_parts(P.scalar(i)), geom_parts(P.scalar(j))
+            edges = any(FR.segments_touch(r1[k - 1], r1[k], r2[m - 1], r2[m])
+                        for r1 in gi[0] for k in range(1, len(r1)) for r2 in gj[0] for m in range(1, len(r2)))
+            inside = FR.location_by_geometry(*gi[0][0][0], gj) != 0 or FR.location_by_geometry(*gj[0][0][0], gi) != 0
+            assert a == (edges or inside)
+
+
+# ---------------------------------------------------------------------------
+# point location
+# ---------------------------------------------------------------------------
+def test_point_location_matches_rationals_on_stars_with_holes():
+    rng = np.random.default_rng(4)
+    m = 25
+    P = S.star_polygons(rng, rng.uniform(0, 100, m), rng.uniform(0, 100, m), rng.uniform(5, 20, m),
+                        rng.integers(4, 30, m), holes=rng.integers(0, 4, m),
+                        hole_vertices=rng.integers(4, 9, 200)[: 0] if False else None, hole_frac=None) \
+        if False else None
+    holes = rng.integers(0, 4, m)
+    nh = int(holes.sum())
+    P = S.star_polygons(rng, rng.uniform(0, 100, m), rng.uniform(0, 100, m), rng.uniform(5, 20, m),
+                        rng.integers(4, 30, m), holes=holes, hole_vertices=rng.integers(4, 9, nh),
+                        hole_frac=rng.uniform(0.1, 0.4, nh))
+    bad = []
+    for q in range(m):
+        g = P.scalar(q)
+        parts = geom_parts(g)
+        b = g.bounds
+        pts = list(zip(rng.uniform(b[0], b[2], 25), rng.uniform(b[1], b[3], 25)))
+        for ring in g.rings:
+            pts += [tuple(v) for v in ring[:4]]
+            pts += [((ring[k][0] + ring[k + 1][0]) / 2, (ring[k][1] + ring[k + 1][1]) / 2) for k in range(4)]
+            pts += [(ring[0][0], np.nextafter(ring[0][1], np.inf)), (ring[1][0], ring[0][1])]
+        for x, y in pts:
+            if oracle.point_location(x, y, P, q) != FR.location_by_geometry(x, y, parts):
+                bad.append((q, x, y))
+    assert not bad
+
+
+def test_point_location_invariants():
+    # vertices are boundary, the star centre is interior, and translating by a
+    # power of two (exact in fp64) changes nothing
+    w = S.c1(n_points=10, n_polys=50)
+    P = w.polygons
+    rng = np.random.default_rng(5)
+    for q in range(len(P)):
+        g = P.scalar(q)
+        ring = g.rings[0]
+        for v in ring[:-1]:
+            assert oracle.point_location(v[0], v[1], P, q) == 1
+        b = g.bounds
+        xs = rng.uniform(b[0], b[2], 20)
+        ys = rng.uniform(b[1], b[3], 20)
+        shift = 1024.0
+        Q = PolygonArray(P.x + shift, P.y + shift, P.ring_offsets, P.part_offsets, P.geom_offsets)
+        for x, y in zip(xs, ys):
+            assert oracle.point_location(x, y, P, q) == oracle.point_location(x + shift, y + shift, Q, q)
+
+
+def test_point_location_golden_vectors(golden):
+    for case in golden["pip"]:
+        polys = [decode_geom(g) for g in case["polygons"]]
+        points = [decode_geom(g) for g in case["points"]]
+        for poly, p, exp in zip(polys, points, case["expected"]):
+            if poly is None or p is None:
+                assert exp is None, case["source"]
+                continue
+            if poly.is_empty or p.is_empty:
+                assert exp is False, case["source"]
+                continue
+            P = PolygonArray.from_geometries([poly])
+            covers = oracle.point_location(p.x, p.y, P, 0) != 0
+            assert covers == exp, case["source"]
+
+
+# ---------------------------------------------------------------------------
+# Hilbert and bbox pairs
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("order", [1, 2, 3, 5, 8])
+def test_hilbert16_is_the_hilbert_curve(order):
+    n = 1 << order
+    xs, ys = np.meshgrid(np.arange(n, dtype=np.uint32), np.arange(n, dtype=np.uint32))
+    xs, ys = xs.ravel(), ys.ravel()
+    h = oracle.hilbert16(xs << (16 - order), ys << (16 - order)) >> (32 - 2 * order)
+    if order <= 5:
+        tb = np.array([oracle.hilbert_textbook(order, int(x), int(y)) for x, y in zip(xs, ys)])
+        assert np.array_equal(h, tb)
+    # bijective onto [0, n^2) and consecutive codes are 4-neighbours
+    assert np.array_equal(np.sort(h), np.arange(n * n))
+    o = np.argsort(h)
+    step = np.abs(np.diff(xs[o].astype(int))) + np.abs(np.diff(ys[o].astype(int)))
+    assert np.all(step == 1)
+
+
+def test_bbox_pairs_match_numpy():
+    rng = np.random.default_rng(6)
+    q = rng.uniform(0, 10, (60, 2))
+    qb = np.hstack([q, q + rng.uniform(0, 2, (60, 2))])
+    t = rng.uniform(0, 10, (70, 2))
+    tb = np.hstack([t, t + rng.uniform(0, 2, (70, 2))])
+    tb[5] = np.nan
+    qb[3] = np.nan
+    lq, lt = oracle.bbox_pairs(qb, tb)
+    m = ((qb[:, None, 0] <= tb[None, :, 2]) & (qb[:, None, 2] >= tb[None, :, 0])
+         & (qb[:, None, 1] <= tb[None, :, 3]) & (qb[:, None, 3] >= tb[None, :, 1]))
+    el, er = np.nonzero(m)
+    assert np.array_equal(lq, el) and np.array_equal(lt, er)
+
+
+# ---------------------------------------------------------------------------
+# CPU join == brute force
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("predicate", ["within", "intersects", "touches"])
+def test_cpu_join_equals_bruteforce(predicate):
+    w = S.c1(n_points=3000, n_polys=60)
+    P = w.polygons
+    px, py = w.points.x.copy(), w.points.y.copy()
+    # put some points exactly on vertices and a NaN point
+    px[:40] = P.x[P.ring_offsets[:40]]
+    py[:40] = P.y[P.ring_offsets[:40]]
+    px[50] = np.nan
+    l, r = oracle.sjoin_points_polygons(px, py, P, predicate, nthreads=4)
+    pb = np.column_stack([px, py, px, py])
+    cl, cr = oracle.bbox_pairs(pb, P.bounds())
+    codes = oracle.point_location_pairs(px, py, cl, cr, P)
+    keep = {"within": codes == 2, "intersects": codes != 0, "touches": codes == 1}[predicate]
+    assert np.array_equal(l, cl[keep]) and np.array_equal(r, cr[keep])
+    assert len(l) > 0
+
+
+def test_cpu_polygon_join_equals_bruteforce():
+    rng = np.random.default_rng(7)
+    A = S.star_polygons(rng, rng.uniform(0, 50, 40), rng.uniform(0, 50, 40), rng.uniform(1, 5, 40),
+                        rng.integers(4, 12, 40))
+    B = S.star_polygons(rng, rng.uniform(0, 50, 30), rng.uniform(0, 50, 30), rng.uniform(1, 6, 30),
+                        rng.integers(4, 12, 30))
+    l, r = oracle.sjoin_polygons_polygons(A, B, nthreads=2)
+    cl, cr = oracle.bbox_pairs(A.bounds(), B.bounds())
+    keep = np.array([oracle.polygon_intersects(A, i, B, j) for i, j in zip(cl, cr)], dtype=bool)
+    assert np.array_equal(l, cl[keep]) and np.array_equal(r, cr[keep])
+
+
+def test_box_query_points_equals_bruteforce():
+    rng = np.random.default_rng(8)
+    px, py = rng.uniform(0, 100, 5000), rng.uniform(0, 100, 5000)
+    c = rng.uniform(0, 100, (300, 2))
+    boxes = np.hstack([c, c + rng.uniform(0, 8, (300, 2))])
+    l, r = oracle.box_query_points(px, py, boxes, nthreads=2)
+    cl, cr = oracle.bbox_pairs(boxes, np.column_stack([px, py, px, py]))
+    assert np.array_equal(l, cl) and np.array_equal(r, cr)
+
+
+# ---------------------------------------------------------------------------
+# GEOS counts recorded by the reference's upstream tests
+# ---------------------------------------------------------------------------
+def _nybb():
+    z = zipfile.ZipFile(os.path.join(GOLDEN, "upstream_data", "nybb_16a.zip"))
+    ga, attrs = read_shapefile_bytes(z.read("nybb.shp"), z.read("nybb.dbf"))
+    b = ga.polygons.bounds()
+    tb = [int(v) for v in (np.nanmin(b[:, 0]), np.nanmin(b[:, 1]), np.nanmax(b[:, 2]), np.nanmax(b[:, 3]))]
+    N = 20
+    xs = list(range(tb[0], tb[2], int((tb[2] - tb[0]) / N)))
+    ys = list(range(tb[1], tb[3], int((tb[3] - tb[1]) / N)))
+    pts = list(zip(xs, ys))
+    return ga, attrs, np.array([p[0] for p in pts], float), np.array([p[1] for p in pts], float)
+
+
+def test_oracle_reproduces_upstream_nybb_counts():
+    ga, attrs, px, py = _nybb()
+    assert len(px) == 21
+    l, r = oracle.sjoin_points_polygons(px, py, ga.polygons, "intersects")
+    assert len(l) == 11  # inner: shape (11, 8)
+    assert len(np.unique(l)) + (21 - len(np.unique(l))) == 21  # left: 21 rows
+    assert len(l) + (len(attrs) - len(np.unique(r))) == 12  # right: 12 rows
+    lw, rw = oracle.sjoin_points_polygons(px, py, ga.polygons, "within")
+    assert attrs.BoroName[rw[list(lw).index(1)]] == "Staten Island"
+
+
+def test_oracle_reproduces_upstream_world_cities_count():
+    d = os.path.join(GOLDEN, "upstream_data")
+    with open(os.path.join(d, "naturalearth_lowres", "naturalearth_lowres.shp"), "rb") as f:
+        world, _ = read_shapefile_bytes(f.read())
+    with open(os.path.join(d, "naturalearth_cities", "naturalearth_cities.shp"), "rb") as f:
+        cities, _ = read_shapefile_bytes(f.read())
+    l, r = oracle.sjoin_points_polygons(cities.points.x, cities.points.y, world.polygons, "intersects")
+    assert len(l) == 213

@@ -1,0 +1,29 @@
+"""vibespatial_amd -- MI355X-native spatial-join hot path of vibeSpatial.
+
+``sjoin(left, right, predicate=...)`` and ``GeoSeries.sindex.query`` over a
+Hilbert-packed spatial index, with exact point-in-polygon and polygon
+``intersects`` refinement, computed by hand-written gfx950 HIP kernels in
+``libvsjoin.so`` (C-ABI in ``include/vsjoin.h``, bound with ctypes).
+"""
+
+from .geometry import (  # noqa: F401
+    GeometryArray,
+    MultiPolygon,
+    Point,
+    PointArray,
+    Polygon,
+    PolygonArray,
+    box,
+    points_from_xy,
+)
+
+__version__ = "0.1.0"
+
+
+def __getattr__(name):
+    # the GPU-facing modules import torch and the library lazily
+    if name in ("sjoin", "GeoSeries", "GeoDataFrame", "SpatialIndex", "PREDICATES"):
+        from . import api
+
+        return getattr(api, name)
+    raise AttributeError(name)

@@ -1,0 +1,518 @@
+"""Public surface of the join path: GeoSeries, GeoDataFrame, SpatialIndex, sjoin.
+
+Signatures and argument checks follow the reference:
+  sjoin                src/vibespatial/api/tools/sjoin.py:37-192, checks :268-321
+  SpatialIndex.query   src/vibespatial/api/sindex.py:144-503, predicates :31-35,
+                       argument validation :122-142
+Output order: ``sindex.query`` returns (2, n) [query_idx; tree_idx] and ``sjoin``
+joins on (left_index, right_index); both are lexsorted here (the reference
+sorts only on request, ``query_utils.py:202-206``; a sorted result satisfies
+both contracts).
+
+Routing (all compute on the GPU through libvsjoin; host numpy only maps rows of
+mixed point/polygon columns and assembles frames):
+  point  x polygon  within / intersects / covered_by / touches -> fused join
+  polygon x point   contains / contains_properly / covers / intersects / touches
+                    -> bbox query + exact location refine (mode 2)
+  polygon x polygon intersects -> bbox query + exact edge/vertex refine (mode 1)
+  point  x point    intersects / within / contains / covers / covered_by /
+                    contains_properly -> bbox query (boxes are the points)
+  predicate None    -> bbox query
+Combinations whose answer is empty by definition (a point cannot contain a
+polygon, points cannot cross, ...) return no pairs, matching the reference's
+mapping (predicates/point_relations.py:100-146).  Polygon x polygon predicates
+other than intersects (full DE-9IM) and ``dwithin`` raise NotImplementedError.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import warnings
+
+import numpy as np
+import pandas as pd
+import torch
+
+from . import join as J
+from ._lib import PRED, check, lib
+from .device import device_points, device_polygons, ptr, require_gpu, stream
+from .geometry import GeometryArray, PointArray, PolygonArray, _Geom, as_geometry_array
+
+__all__ = ["GeoSeries", "GeoDataFrame", "SpatialIndex", "sjoin", "PREDICATES"]
+
+PREDICATES = {
+    None,
+    "intersects",
+    "within",
+    "contains",
+    "contains_properly",
+    "covers",
+    "covered_by",
+    "touches",
+    "crosses",
+    "overlaps",
+    "dwithin",
+}
+
+_EMPTY_PAIRS = {
+    # (query family, tree family): predicates that can never hold
+    ("point", "polygon"): {"contains", "contains_properly", "covers", "crosses", "overlaps"},
+    ("polygon", "point"): {"within", "covered_by", "crosses", "overlaps"},
+    ("point", "point"): {"touches", "crosses", "overlaps"},
+}
+
+
+# ---------------------------------------------------------------------------
+# pair helpers (device int64 tensors)
+# ---------------------------------------------------------------------------
+def _empty_pairs(dev):
+    e = torch.empty(0, dtype=torch.int64, device=dev)
+    return e, e.clone()
+
+
+def _refine(mode: int, predicate: str, left, right, points: PointArray | None, A: PolygonArray | None,
+            B: PolygonArray | None):
+    dev = require_gpu()
+    L = lib()
+    n = int(left.numel())
+    if n == 0:
+        return _empty_pairs(dev)
+    P = device_points(points) if points is not None else None
+    DA = device_polygons(A) if A is not None else None
+    DB = device_polygons(B) if B is not None else None
+    out_l = torch.empty(n, dtype=torch.int64, device=dev)
+    out_r = torch.empty(n, dtype=torch.int64, device=dev)
+    res = torch.zeros(4, dtype=torch.int64, device=dev)
+    ws = torch.empty(max(int(L.vs_refine_pairs_workspace_bytes(n)), 1), dtype=torch.uint8, device=dev)
+    none = [ptr(None)] * 5
+    a_args = DA.args() if DA is not None else none
+    b_args = DB.args() if DB is not None else none
+    check(
+        L.vs_refine_pairs(mode, PRED.get(predicate, 1), ptr(left), ptr(right), n,
+                          ptr(P.x if P else None), ptr(P.y if P else None), *a_args, *b_args,
+                          ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
+        "vs_refine_pairs",
+    )
+    k = int(res[0].item())
+    return out_l[:k], out_r[:k]
+
+
+def _family_bounds_device(ga: GeometryArray):
+    """(n_rows, 4) fp64 device bounds of a column (NaN for null rows)."""
+    dev = require_gpu()
+    if ga.is_points:
+        return J.point_boxes(device_points(ga.points))
+    if ga.is_polygons:
+        return J.polygon_bounds(device_polygons(ga.polygons))
+    out = torch.full((len(ga), 4), float("nan"), dtype=torch.float64, device=dev)
+    if ga.points is not None:
+        out[torch.as_tensor(ga.point_rows, device=dev)] = J.point_boxes(device_points(ga.points))
+    if ga.polygons is not None:
+        out[torch.as_tensor(ga.polygon_rows, device=dev)] = J.polygon_bounds(device_polygons(ga.polygons))
+    return out
+
+
+class _FamilyIndex:
+    """Packed trees over the tree column, one per family (built lazily)."""
+
+    def __init__(self, ga: GeometryArray):
+        self.ga = ga
+        self._trees = {}
+
+    def tree(self, family: str) -> J.DeviceTree:
+        t = self._trees.get(family)
+        if t is None:
+            if family == "point":
+                b = J.point_boxes(device_points(self.ga.points))
+            elif family == "polygon":
+                b = J.polygon_bounds(device_polygons(self.ga.polygons))
+            else:
+                b = _family_bounds_device(self.ga)
+            t = J.build_tree(b)
+            self._trees[family] = t
+        return t
+
+
+def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, predicate):
+    """Pairs (query family idx, tree family idx), lexsorted, device int64."""
+    dev = require_gpu()
+    if predicate in _EMPTY_PAIRS.get((q_fam, t_fam), ()):
+        return _empty_pairs(dev)
+    if predicate == "dwithin":
+        raise NotImplementedError("predicate='dwithin' is not implemented on this path")
+    tree = tindex.tree(t_fam)
+    if q_fam == "point" and t_fam == "polygon" and predicate is not None:
+        l, r, _ = J.sjoin_points_polygons(device_points(qarr), device_polygons(tarr), tree, predicate)
+        return l, r
+    qb = J.point_boxes(device_points(qarr)) if q_fam == "point" else J.polygon_bounds(device_polygons(qarr))
+    l, r, _ = J.query_boxes(qb, tree)
+    if predicate is None or (q_fam == "point" and t_fam == "point"):
+        return l, r
+    if q_fam == "polygon" and t_fam == "point":
+        pred = {"contains_properly": "contains"}.get(predicate, predicate)
+        return _refine(2, pred, l, r, tarr, None, qarr)
+    if q_fam == "polygon" and t_fam == "polygon":
+        if l.numel() == 0:
+            return l, r  # no bbox overlap: every predicate but disjoint is false
+        if predicate != "intersects":
+            raise NotImplementedError(f"polygon x polygon predicate {predicate!r} (full DE-9IM) is not implemented")
+        return _refine(1, predicate, l, r, None, qarr, tarr)
+    raise NotImplementedError(f"{q_fam} x {t_fam} predicate {predicate!r}")
+
+
+def query_pairs(query: GeometryArray, tree_ga: GeometryArray, tindex: _FamilyIndex, predicate):
+    """Lexsorted (query_row, tree_row) pairs as device int64 tensors."""
+    dev = require_gpu()
+    fams_q = [(f, a, rows) for f, a, rows in (("point", query.points, query.point_rows),
+                                               ("polygon", query.polygons, query.polygon_rows)) if a is not None]
+    fams_t = [(f, a, rows) for f, a, rows in (("point", tree_ga.points, tree_ga.point_rows),
+                                               ("polygon", tree_ga.polygons, tree_ga.polygon_rows)) if a is not None]
+    if len(fams_q) == 1 and len(fams_t) == 1:
+        (qf, qa, _), (tf, ta, _) = fams_q[0], fams_t[0]
+        return _pairs_family(qf, qa, tf, ta, tindex, predicate)
+    parts_l, parts_r = [], []
+    for qf, qa, qrows in fams_q:
+        for tf, ta, trows in fams_t:
+            l, r = _pairs_family(qf, qa, tf, ta, tindex, predicate)
+            parts_l.append(qrows[l.cpu().numpy()])
+            parts_r.append(trows[r.cpu().numpy()])
+    if not parts_l:
+        return _empty_pairs(dev)
+    lh = np.concatenate(parts_l)
+    rh = np.concatenate(parts_r)
+    o = np.lexsort((rh, lh))
+    return torch.as_tensor(lh[o], device=dev), torch.as_tensor(rh[o], device=dev)
+
+
+# ---------------------------------------------------------------------------
+# SpatialIndex
+# ---------------------------------------------------------------------------
+class SpatialIndex:
+    """Spatial index of a geometry column (packed Hilbert trees on the GPU)."""
+
+    def __init__(self, geometry):
+        self._ga = as_geometry_array(geometry)
+        self._index = _FamilyIndex(self._ga)
+
+    @property
+    def valid_query_predicates(self):
+        return set(PREDICATES)
+
+    @property
+    def size(self) -> int:
+        return int((self._ga.tags >= 0).sum())
+
+    def __len__(self) -> int:
+        return self.size
+
+    @property
+    def is_empty(self) -> bool:
+        return self.size == 0
+
+    def _check(self, predicate, distance):
+        if predicate not in self.valid_query_predicates:
+            raise ValueError(
+                f"Got predicate='{predicate}'; `predicate` must be one of {self.valid_query_predicates}"
+            )
+        if predicate == "dwithin":
+            if distance is None:
+                raise ValueError("'distance' parameter is required for 'dwithin' predicate")
+        elif distance is not None:
+            raise ValueError("'distance' parameter is only supported in combination with 'dwithin' predicate")
+
+    def query(self, geometry, predicate=None, sort=False, distance=None, output_format="indices",
+              return_device=False):
+        """predicate(geometry, tree_geometry) pairs; see api/sindex.py:144-503."""
+        self._check(predicate, distance)
+        if output_format not in ("indices", "dense", "sparse"):
+            raise ValueError(f"Invalid output_format: '{output_format}'. Use one of 'indices', 'sparse', 'dense'.")
+        if return_device and output_format != "indices":
+            raise ValueError("return_device=True requires output_format='indices'")
+        scalar = geometry is None or isinstance(geometry, _Geom)
+        if not scalar and not isinstance(geometry, (list, tuple, np.ndarray, GeometryArray, PointArray,
+                                                    PolygonArray, GeoSeries)):
+            raise TypeError(f"invalid geometry argument: {type(geometry).__name__}")
+        q = as_geometry_array(geometry)
+        if len(q) == 0 or self._ga.points is None and self._ga.polygons is None or (
+                q.points is None and q.polygons is None):
+            l = r = torch.empty(0, dtype=torch.int64, device=require_gpu())
+        else:
+            l, r = query_pairs(q, self._ga, self._index, predicate)
+        if return_device:
+            return r if scalar else torch.stack([l, r])
+        lh, rh = l.cpu().numpy(), r.cpu().numpy()
+        if output_format == "indices":
+            return rh if scalar else np.vstack([lh, rh])
+        n_tree = len(self._ga)
+        if scalar:
+            dense = np.zeros(n_tree, dtype=bool)
+            dense[rh] = True
+        else:
+            dense = np.zeros((n_tree, len(q)), dtype=bool)
+            dense[rh, lh] = True
+        if output_format == "dense":
+            return dense
+        from scipy import sparse
+
+        return sparse.coo_array(dense)
+
+    def intersection(self, coordinates):
+        """Tree rows whose bbox intersects a bounds tuple or a point (x, y)."""
+        if not isinstance(coordinates, tuple) or len(coordinates) not in (2, 4):
+            raise TypeError("Invalid coordinates, must be iterable in format (minx, miny, maxx, maxy) or (x, y)")
+        from .geometry import box as _box, Point as _Point
+
+        g = _Point(*coordinates) if len(coordinates) == 2 else _box(*coordinates)
+        return self.query(g)
+
+
+# ---------------------------------------------------------------------------
+# GeoSeries / GeoDataFrame (thin: attributes in pandas, geometry in SoA arrays)
+# ---------------------------------------------------------------------------
+class GeoSeries:
+    def __init__(self, data=None, index=None, crs=None, name=None):
+        self.values = as_geometry_array([] if data is None else data)
+        self.index = pd.RangeIndex(len(self.values)) if index is None else pd.Index(index)
+        self.crs = crs
+        self.name = name
+
+    def __len__(self) -> int:
+        return len(self.values)
+
+    def __getitem__(self, i):
+        return self.values[int(i)]
+
+    def __iter__(self):
+        for i in range(len(self)):
+            yield self.values[i]
+
+    @property
+    def sindex(self) -> SpatialIndex:
+        if self.values._sindex is None:
+            self.values._sindex = SpatialIndex(self.values)
+        return self.values._sindex
+
+    @property
+    def bounds(self) -> pd.DataFrame:
+        return pd.DataFrame(self.values.bounds(), columns=["minx", "miny", "maxx", "maxy"], index=self.index)
+
+    @property
+    def geom_type(self):
+        names = {0: "Point", 1: "Polygon", -1: None}
+        return pd.Series([names[int(t)] for t in self.values.tags], index=self.index)
+
+    def take(self, idx) -> "GeoSeries":
+        idx = np.asarray(idx, dtype=np.int64)
+        return GeoSeries([self.values[int(i)] if i >= 0 else None for i in idx], crs=self.crs, name=self.name)
+
+
+class GeoDataFrame:
+    """Attribute columns in a pandas DataFrame plus one active geometry column."""
+
+    def __init__(self, data=None, geometry=None, crs=None, index=None):
+        df = pd.DataFrame(data, index=index) if not isinstance(data, pd.DataFrame) else data.copy()
+        name = "geometry"
+        if isinstance(geometry, str):
+            name = geometry
+            geoms = df.pop(geometry)
+            geoms = list(geoms) if not isinstance(geoms, GeoSeries) else geoms
+        elif geometry is not None:
+            geoms = geometry
+        elif "geometry" in df.columns:
+            geoms = list(df.pop("geometry"))
+        else:
+            raise ValueError("GeoDataFrame needs a geometry column")
+        self._df = df
+        gs = geoms if isinstance(geoms, GeoSeries) else GeoSeries(geoms, crs=crs)
+        gs.index = df.index
+        self._geometry = gs
+        self._geom_name = name
+        self.crs = crs if crs is not None else gs.crs
+
+    # pandas-like surface
+    @property
+    def geometry(self) -> GeoSeries:
+        return self._geometry
+
+    @property
+    def sindex(self) -> SpatialIndex:
+        return self._geometry.sindex
+
+    @property
+    def index(self):
+        return self._df.index
+
+    @property
+    def columns(self):
+        cols = list(self._df.columns)
+        return pd.Index(cols[: self._geom_pos()] + [self._geom_name] + cols[self._geom_pos():])
+
+    def _geom_pos(self) -> int:
+        return getattr(self, "_geom_at", len(self._df.columns))
+
+    @property
+    def shape(self):
+        return (len(self._df), len(self._df.columns) + 1)
+
+    def __len__(self) -> int:
+        return len(self._df)
+
+    def __contains__(self, col) -> bool:
+        return col == self._geom_name or col in self._df.columns
+
+    def __getitem__(self, col):
+        if col == self._geom_name:
+            return self._geometry
+        return self._df[col]
+
+    def to_pandas(self) -> pd.DataFrame:
+        out = self._df.copy()
+        out.insert(self._geom_pos(), self._geom_name, list(self._geometry))
+        return out
+
+    @property
+    def loc(self):
+        return self.to_pandas().loc
+
+    @property
+    def iloc(self):
+        return self.to_pandas().iloc
+
+    def iterrows(self):
+        return self.to_pandas().iterrows()
+
+    def sjoin(self, df, *args, **kwargs):
+        return sjoin(self, df, *args, **kwargs)
+
+    def __repr__(self):
+        return repr(self.to_pandas())
+
+
+# ---------------------------------------------------------------------------
+# sjoin
+# ---------------------------------------------------------------------------
+def _maybe_make_list(obj):
+    if isinstance(obj, tuple):
+        return list(obj)
+    if obj is not None and not isinstance(obj, list):
+        return [obj]
+    return obj
+
+
+def _basic_checks(left_df, right_df, how, lsuffix, rsuffix, on_attribute=None):
+    if not isinstance(left_df, GeoDataFrame):
+        raise ValueError(f"'left_df' should be GeoDataFrame, got {type(left_df)}")
+    if not isinstance(right_df, GeoDataFrame):
+        raise ValueError(f"'right_df' should be GeoDataFrame, got {type(right_df)}")
+    allowed = ["left", "right", "inner", "outer"]
+    if how not in allowed:
+        raise ValueError(f'`how` was "{how}" but is expected to be in {allowed}')
+    if left_df.crs != right_df.crs:
+        warnings.warn("CRS mismatch between the CRS of left geometries and the CRS of right geometries.",
+                      UserWarning, stacklevel=3)
+    if on_attribute:
+        for attr in on_attribute:
+            if attr not in left_df and attr not in right_df:
+                raise ValueError(f"Expected column {attr} is missing from both of the dataframes.")
+            if attr not in left_df:
+                raise ValueError(f"Expected column {attr} is missing from the left dataframe.")
+            if attr not in right_df:
+                raise ValueError(f"Expected column {attr} is missing from the right dataframe.")
+            if attr in (left_df.geometry.name, right_df.geometry.name, left_df._geom_name, right_df._geom_name):
+                raise ValueError("Active geometry column cannot be used as an input for on_attribute parameter.")
+
+
+def sjoin_indices(left, right, predicate="intersects", return_device=False):
+    """(left_index, right_index) lexsorted pairs with predicate(left, right)."""
+    lg = left.geometry if isinstance(left, GeoDataFrame) else left
+    rg = right.geometry if isinstance(right, GeoDataFrame) else right
+    rg_series = rg if isinstance(rg, GeoSeries) else GeoSeries(rg)
+    res = rg_series.sindex.query(lg.values if isinstance(lg, GeoSeries) else lg, predicate=predicate,
+                                 return_device=return_device)
+    return res[0], res[1]
+
+
+def sjoin(left_df, right_df, how="inner", predicate="intersects", lsuffix="left", rsuffix="right",
+          distance=None, on_attribute=None, **kwargs):
+    """Spatial join of two GeoDataFrames (api/tools/sjoin.py:37-192)."""
+    if kwargs:
+        first = next(iter(kwargs.keys()))
+        raise TypeError(f"sjoin() got an unexpected keyword argument '{first}'")
+    on_attribute = _maybe_make_list(on_attribute)
+    _basic_checks(left_df, right_df, how, lsuffix, rsuffix, on_attribute)
+    tree = right_df.sindex
+    tree._check(predicate, distance)
+    res = tree.query(left_df.geometry.values, predicate=predicate, sort=True, distance=distance)
+    l_idx, r_idx = res[0], res[1]
+    if on_attribute:
+        keep = np.ones(len(l_idx), dtype=bool)
+        for attr in on_attribute:
+            keep &= left_df[attr].to_numpy()[l_idx] == right_df[attr].to_numpy()[r_idx]
+        l_idx, r_idx = l_idx[keep], r_idx[keep]
+    return _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribute)
+
+
+def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribute):
+    """GeoPandas-shaped frame for the join (api/_native_results.py:301-472)."""
+    ldf, rdf = left_df._df, right_df._df
+    on_attribute = on_attribute or []
+    index_left = f"index_{lsuffix}"
+    index_right = f"index_{rsuffix}"
+    lcols = list(ldf.columns)
+    rcols = [c for c in rdf.columns if c not in on_attribute]
+    overlap = (set(lcols) | {left_df._geom_name}) & set(rcols)
+    lren = {c: f"{c}_{lsuffix}" for c in overlap if c in lcols}
+    rren = {c: f"{c}_{rsuffix}" for c in overlap}
+    if how == "right":
+        l_idx, r_idx = l_idx.copy(), r_idx.copy()
+        o = np.lexsort((l_idx, r_idx))
+        l_idx, r_idx = l_idx[o], r_idx[o]
+        matched = np.zeros(len(rdf), dtype=bool)
+        matched[r_idx] = True
+        miss = np.flatnonzero(~matched)
+        r_all = np.concatenate([r_idx, miss])
+        l_all = np.concatenate([l_idx, np.full(len(miss), -1)])
+        o = np.argsort(r_all, kind="stable")
+        r_all, l_all = r_all[o], l_all[o]
+        left_part = ldf.rename(columns=lren).reset_index(drop=True).reindex(np.where(l_all >= 0, l_all, -1))
+        left_part.index = rdf.index[r_all]
+        out = pd.DataFrame(index=rdf.index[r_all])
+        out[index_left] = np.where(l_all >= 0, np.asarray(ldf.index)[np.maximum(l_all, 0)], np.nan) \
+            if len(l_all) else []
+        for c in lcols:
+            out[lren.get(c, c)] = left_part[lren.get(c, c)].to_numpy()
+        for c in rcols:
+            out[rren.get(c, c)] = rdf[c].to_numpy()[r_all]
+        geoms = right_df.geometry.take(r_all)
+        g = GeoDataFrame(out, geometry=geoms, crs=right_df.crs)
+        g._geom_name = right_df._geom_name
+        return g
+    if how in ("left", "outer"):
+        matched = np.zeros(len(ldf), dtype=bool)
+        matched[l_idx] = True
+        miss = np.flatnonzero(~matched)
+        l_all = np.concatenate([l_idx, miss])
+        r_all = np.concatenate([r_idx, np.full(len(miss), -1)])
+        o = np.argsort(l_all, kind="stable")
+        l_all, r_all = l_all[o], r_all[o]
+    else:
+        l_all, r_all = l_idx, r_idx
+    out = pd.DataFrame(index=ldf.index[l_all] if len(ldf.index) else None)
+    for c in lcols:
+        out[lren.get(c, c)] = ldf[c].to_numpy()[l_all]
+    valid = r_all >= 0
+    ridx = np.asarray(rdf.index)
+    out[index_right] = np.where(valid, ridx[np.maximum(r_all, 0)] if len(ridx) else 0, np.nan) \
+        if how != "inner" else (ridx[r_all] if len(r_all) else np.empty(0, dtype=ridx.dtype))
+    for c in rcols:
+        col = rdf[c].to_numpy()
+        vals = col[np.maximum(r_all, 0)] if len(col) else np.empty(len(r_all))
+        if how != "inner" and (~valid).any():
+            vals = vals.astype(object) if vals.dtype.kind not in "fc" else vals.astype(float)
+            vals[~valid] = np.nan
+        out[rren.get(c, c)] = vals
+    geoms = left_df.geometry.take(l_all)
+    g = GeoDataFrame(out, geometry=geoms, crs=left_df.crs)
+    g._geom_name = left_df._geom_name
+    g._geom_at = len(lcols)
+    return g

@@ -1,0 +1,425 @@
+// Index build: per-row bounds, centre extent, 64-bit Hilbert keys, packed tree.
+//
+// Replaces, on the sjoin path:
+//   bounds_polygon / bounds_multipolygon   src/vibespatial/kernels/core/geometry_analysis_source.py:31-212
+//   _device_total_bounds (D2H fence)       src/vibespatial/spatial/indexing.py:1241-1262
+//                                          (kept on the device here: no host round trip)
+//   morton_keys_from_bounds                src/vibespatial/spatial/indexing_kernels.py:20-49
+//   compute_hilbert_codes (32-bit)         src/vibespatial/io/gpu_parse/indexing_kernels.py:78-176
+//   _build_packed_rtree (fan-out 16)       src/vibespatial/io/gpu_parse/indexing.py:262-400
+//
+// Layout in HBM (all levels in one array of 32-byte boxes, root first, items
+// last, like the reference's root-first layout; the children of node j of level
+// l are nodes [j*F, j*F+F) of level l+1, so no child-pointer array is stored):
+//   node_boxes[level_start[l] + j] = {minx, miny, maxx, maxy}   (fp64)
+//   item_ids[k] = original row of the k-th item in Hilbert order
+// Empty/null rows have NaN boxes, key UINT64_MAX, sort last and never match.
+#include "vs_exact.h"
+#include "vs_internal.h"
+
+namespace vs {
+
+int sort_pairs_u64(uint64_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
+int sort_pairs_u32(uint32_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
+int64_t sort_workspace_bytes_u64(int64_t n);
+int64_t sort_workspace_bytes_u32(int64_t n);
+
+// One wave per polygonal row; lanes stride over the row's coordinates.
+__global__ __launch_bounds__(256) void polygon_bounds_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const int32_t* __restrict__ geom,
+    const int32_t* __restrict__ part, const int32_t* __restrict__ ring, int64_t n_rows,
+    double4* __restrict__ out) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n_rows) return;
+    const int32_t c0 = ring[part[geom[row]]], c1 = ring[part[geom[row + 1]]];
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (int32_t c = c0 + lane; c < c1; c += 64) {
+        const double xv = x[c], yv = y[c];
+        x0 = fmin(x0, xv); y0 = fmin(y0, yv);
+        x1 = fmax(x1, xv); y1 = fmax(y1, yv);
+    }
+    x0 = wave_min(x0); y0 = wave_min(y0);
+    x1 = wave_max(x1); y1 = wave_max(y1);
+    if (lane == 0) {
+        if (c1 <= c0 || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
+        out[row] = make_double4(x0, y0, x1, y1);
+    }
+}
+
+// Extent of box centres (or of points when boxes are degenerate): partial
+// reduction per block, then one block folds the partials.
+__global__ __launch_bounds__(256) void centre_extent_partial_kernel(const double4* __restrict__ b,
+                                                                   int64_t n,
+                                                                   double4* __restrict__ part) {
+    __shared__ double4 red[4];
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double4 v = b[i];
+        if (v.x == v.x) {
+            const double cx = 0.5 * (v.x + v.z), cy = 0.5 * (v.y + v.w);
+            x0 = fmin(x0, cx); y0 = fmin(y0, cy);
+            x1 = fmax(x1, cx); y1 = fmax(y1, cy);
+        }
+    }
+    x0 = wave_min(x0); y0 = wave_min(y0); x1 = wave_max(x1); y1 = wave_max(y1);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double4(x0, y0, x1, y1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double4 r = red[0];
+        for (int k = 1; k < 4; ++k) {
+            r.x = fmin(r.x, red[k].x); r.y = fmin(r.y, red[k].y);
+            r.z = fmax(r.z, red[k].z); r.w = fmax(r.w, red[k].w);
+        }
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void point_extent_partial_kernel(const double* __restrict__ px,
+                                                                  const double* __restrict__ py,
+                                                                  int64_t n,
+                                                                  double4* __restrict__ part) {
+    __shared__ double4 red[4];
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double a = px[i], c = py[i];
+        if (a == a && c == c) {
+            x0 = fmin(x0, a); y0 = fmin(y0, c);
+            x1 = fmax(x1, a); y1 = fmax(y1, c);
+        }
+    }
+    x0 = wave_min(x0); y0 = wave_min(y0); x1 = wave_max(x1); y1 = wave_max(y1);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double4(x0, y0, x1, y1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double4 r = red[0];
+        for (int k = 1; k < 4; ++k) {
+            r.x = fmin(r.x, red[k].x); r.y = fmin(r.y, red[k].y);
+            r.z = fmax(r.z, red[k].z); r.w = fmax(r.w, red[k].w);
+        }
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void extent_final_kernel(const double4* __restrict__ part, int nb,
+                                                          double4* __restrict__ ext) {
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        const double4 v = part[i];
+        x0 = fmin(x0, v.x); y0 = fmin(y0, v.y); x1 = fmax(x1, v.z); y1 = fmax(y1, v.w);
+    }
+    __shared__ double4 red[4];
+    x0 = wave_min(x0); y0 = wave_min(y0); x1 = wave_max(x1); y1 = wave_max(y1);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double4(x0, y0, x1, y1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double4 r = red[0];
+        for (int k = 1; k < 4; ++k) {
+            r.x = fmin(r.x, red[k].x); r.y = fmin(r.y, red[k].y);
+            r.z = fmax(r.z, red[k].z); r.w = fmax(r.w, red[k].w);
+        }
+        ext[0] = r;
+    }
+}
+
+constexpr int kExtentBlocks = 512;
+
+int64_t extent_workspace_bytes() { return sizeof(double4) * (kExtentBlocks + 1) + 256; }
+
+int centre_extent(const double4* b, int64_t n, double4* ext, void* ws, hipStream_t s) {
+    Carver c(ws);
+    double4* part = c.take<double4>(kExtentBlocks);
+    const int nb = (int)std::min<int64_t>(kExtentBlocks, std::max<int64_t>(1, ceil_div(n, 256)));
+    hipLaunchKernelGGL(centre_extent_partial_kernel, dim3(nb), dim3(256), 0, s, b, n, part);
+    VS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(extent_final_kernel, dim3(1), dim3(256), 0, s, part, nb, ext);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int point_extent(const double* px, const double* py, int64_t n, double4* ext, void* ws, hipStream_t s) {
+    Carver c(ws);
+    double4* part = c.take<double4>(kExtentBlocks);
+    const int nb = (int)std::min<int64_t>(kExtentBlocks, std::max<int64_t>(1, ceil_div(n, 256)));
+    hipLaunchKernelGGL(point_extent_partial_kernel, dim3(nb), dim3(256), 0, s, px, py, n, part);
+    VS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(extent_final_kernel, dim3(1), dim3(256), 0, s, part, nb, ext);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+struct KeyScale {
+    double x0, y0, sx, sy;
+};
+
+__device__ __forceinline__ KeyScale key_scale(const double4* ext, int bits) {
+    const double4 e = ext[0];
+    const double cells = (bits >= 32) ? 4294967295.0 : (double)((1ull << bits) - 1);
+    KeyScale k;
+    k.x0 = e.x;
+    k.y0 = e.y;
+    k.sx = (e.z > e.x) ? cells / (e.z - e.x) : 0.0;
+    k.sy = (e.w > e.y) ? cells / (e.w - e.y) : 0.0;
+    return k;
+}
+
+// Key of bits*2 bits: the Hilbert index of the box centre on a 2^bits grid over
+// the extent.  NaN boxes -> all ones (sort last).
+__global__ __launch_bounds__(256) void hilbert_keys_boxes_kernel(const double4* __restrict__ b,
+                                                                int64_t n,
+                                                                const double4* __restrict__ ext,
+                                                                int bits,
+                                                                uint64_t* __restrict__ keys,
+                                                                int32_t* __restrict__ ids) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const KeyScale ks = key_scale(ext, bits);
+    const double4 v = b[i];
+    uint64_t key = ~0ull;
+    if (v.x == v.x) {
+        const uint32_t qx = quantize(0.5 * (v.x + v.z), ks.x0, ks.sx, bits) << (32 - bits);
+        const uint32_t qy = quantize(0.5 * (v.y + v.w), ks.y0, ks.sy, bits) << (32 - bits);
+        key = hilbert64(qx, qy) >> (64 - 2 * bits);
+    }
+    keys[i] = key;
+    ids[i] = (int32_t)i;
+}
+
+template <class K>
+__global__ __launch_bounds__(256) void hilbert_keys_points_kernel(const double* __restrict__ px,
+                                                                 const double* __restrict__ py,
+                                                                 int64_t n,
+                                                                 const double4* __restrict__ ext,
+                                                                 int bits, K* __restrict__ keys,
+                                                                 int32_t* __restrict__ ids) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const KeyScale ks = key_scale(ext, bits);
+    const double a = px[i], c = py[i];
+    uint64_t key = ~0ull;
+    if (a == a && c == c) {
+        const uint32_t qx = quantize(a, ks.x0, ks.sx, bits) << (32 - bits);
+        const uint32_t qy = quantize(c, ks.y0, ks.sy, bits) << (32 - bits);
+        key = hilbert64(qx, qy) >> (64 - 2 * bits);
+    }
+    keys[i] = (K)key;
+    ids[i] = (int32_t)i;
+}
+
+// Items level: boxes gathered into Hilbert order.
+__global__ __launch_bounds__(256) void gather_boxes_kernel(const double4* __restrict__ b,
+                                                          const int32_t* __restrict__ order, int64_t n,
+                                                          double4* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = b[order[i]];
+}
+
+// Parent boxes = union of up to F child boxes (NaN children ignored; a parent
+// with no valid child is NaN).
+__global__ __launch_bounds__(256) void tree_level_kernel(const double4* __restrict__ child,
+                                                        int64_t n_child, double4* __restrict__ parent,
+                                                        int64_t n_parent, int fan) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n_parent) return;
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    const int64_t k1 = std::min<int64_t>((p + 1) * fan, n_child);
+    for (int64_t k = p * fan; k < k1; ++k) {
+        const double4 v = child[k];
+        if (v.x == v.x) {
+            x0 = fmin(x0, v.x); y0 = fmin(y0, v.y);
+            x1 = fmax(x1, v.z); y1 = fmax(y1, v.w);
+        }
+    }
+    if (!(x0 <= x1)) x0 = y0 = x1 = y1 = NAN;
+    parent[p] = make_double4(x0, y0, x1, y1);
+}
+
+// Point boxes (degenerate), NaN for null/empty points.
+__global__ __launch_bounds__(256) void point_boxes_kernel(const double* __restrict__ px,
+                                                         const double* __restrict__ py, int64_t n,
+                                                         double4* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double a = px[i], c = py[i];
+    if (!(a == a && c == c)) a = c = NAN;
+    out[i] = make_double4(a, c, a, c);
+}
+
+int polygon_bounds(const double* x, const double* y, const int32_t* geom, const int32_t* part,
+                   const int32_t* ring, int64_t n, double* bounds, hipStream_t s) {
+    if (n == 0) return VS_OK;
+    hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, s, x, y,
+                       geom, part, ring, n, reinterpret_cast<double4*>(bounds));
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int point_boxes(const double* px, const double* py, int64_t n, double* boxes, hipStream_t s) {
+    if (n == 0) return VS_OK;
+    hipLaunchKernelGGL(point_boxes_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py, n,
+                       reinterpret_cast<double4*>(boxes));
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int64_t index_workspace_bytes(int64_t n) {
+    Carver c(nullptr);
+    c.take<uint64_t>(n);
+    c.take<double4>(1);
+    c.take<char>(extent_workspace_bytes());
+    c.take<char>(sort_workspace_bytes_u64(n));
+    return c.bytes;
+}
+
+int index_build(const double* bounds, int64_t n, const vs_tree_layout* L, double* node_boxes,
+                int32_t* item_ids, uint64_t* keys_out, void* ws, int64_t ws_bytes, hipStream_t s) {
+    VS_REQUIRE(L && L->n_items == n && L->n_levels >= 1 && L->n_levels <= VS_MAX_LEVELS,
+               "vs_index_build: layout does not describe %lld items", (long long)n);
+    VS_REQUIRE(ws_bytes >= index_workspace_bytes(n), "vs_index_build: workspace too small");
+    if (n == 0) return VS_OK;
+    Carver c(ws);
+    uint64_t* keys = keys_out ? keys_out : c.take<uint64_t>(n);
+    if (keys_out) c.take<uint64_t>(n);
+    double4* ext = c.take<double4>(1);
+    void* ext_ws = c.take<char>(extent_workspace_bytes());
+    void* sort_ws = c.take<char>(sort_workspace_bytes_u64(n));
+    const double4* b = reinterpret_cast<const double4*>(bounds);
+    VS_TRY(centre_extent(b, n, ext, ext_ws, s));
+    hipLaunchKernelGGL(hilbert_keys_boxes_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, b,
+                       n, ext, 32, keys, item_ids);
+    VS_LAUNCH_CHECK();
+    VS_TRY(sort_pairs_u64(keys, item_ids, n, 0, 64, sort_ws, sort_workspace_bytes_u64(n), s));
+    double4* nb = reinterpret_cast<double4*>(node_boxes);
+    const int last = L->n_levels - 1;
+    hipLaunchKernelGGL(gather_boxes_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, b,
+                       item_ids, n, nb + L->level_start[last]);
+    VS_LAUNCH_CHECK();
+    for (int l = last - 1; l >= 0; --l) {
+        const int64_t np = L->level_count[l];
+        hipLaunchKernelGGL(tree_level_kernel, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0, s,
+                           nb + L->level_start[l + 1], L->level_count[l + 1], nb + L->level_start[l],
+                           np, L->fanout);
+        VS_LAUNCH_CHECK();
+    }
+    return VS_OK;
+}
+
+// Point ordering for coherent traversal: 16-level (32-bit) Hilbert keys over
+// the point extent, sorted; writes the permutation and the gathered coords.
+__global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ px,
+                                                           const double* __restrict__ py,
+                                                           const int32_t* __restrict__ perm,
+                                                           int64_t n, double* __restrict__ sx,
+                                                           double* __restrict__ sy) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int32_t j = perm[i];
+    sx[i] = px[j];
+    sy[i] = py[j];
+}
+
+int64_t point_order_workspace_bytes(int64_t n) {
+    Carver c(nullptr);
+    c.take<uint32_t>(n);
+    c.take<double4>(1);
+    c.take<char>(extent_workspace_bytes());
+    c.take<char>(sort_workspace_bytes_u32(n));
+    return c.bytes;
+}
+
+int point_order(const double* px, const double* py, int64_t n, int bits, int32_t* perm, double* sx,
+                double* sy, void* ws, hipStream_t s) {
+    if (n == 0) return VS_OK;
+    Carver c(ws);
+    uint32_t* keys = c.take<uint32_t>(n);
+    double4* ext = c.take<double4>(1);
+    void* ext_ws = c.take<char>(extent_workspace_bytes());
+    void* sort_ws = c.take<char>(sort_workspace_bytes_u32(n));
+    VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
+    hipLaunchKernelGGL(hilbert_keys_points_kernel<uint32_t>, dim3((unsigned)ceil_div(n, 256)), dim3(256),
+                       0, s, px, py, n, ext, 16, keys, perm);
+    VS_LAUNCH_CHECK();
+    VS_TRY(sort_pairs_u32(keys, perm, n, 32 - 2 * bits, 32, sort_ws, sort_workspace_bytes_u32(n), s));
+    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py,
+                       perm, n, sx, sy);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+}  // namespace vs
+
+extern "C" {
+
+int vs_tree_plan(int64_t n_items, int32_t fanout, vs_tree_layout* out) {
+    vs::clear_error();
+    VS_REQUIRE(out != nullptr, "vs_tree_plan: null layout");
+    VS_REQUIRE(fanout >= 2 && fanout <= 64, "vs_tree_plan: fanout %d not in [2, 64]", fanout);
+    VS_REQUIRE(n_items >= 0 && n_items < (int64_t)INT32_MAX, "vs_tree_plan: n_items out of range");
+    memset(out, 0, sizeof(*out));
+    int64_t counts[VS_MAX_LEVELS];
+    int nl = 0;
+    int64_t c = n_items;
+    counts[nl++] = c;
+    while (c > 1) {
+        c = (c + fanout - 1) / fanout;
+        VS_REQUIRE(nl < VS_MAX_LEVELS, "vs_tree_plan: too many levels");
+        counts[nl++] = c;
+    }
+    out->fanout = fanout;
+    out->n_items = n_items;
+    out->n_levels = nl;
+    int64_t total = 0;
+    for (int l = 0; l < nl; ++l) {
+        out->level_count[l] = counts[nl - 1 - l];
+        out->level_start[l] = total;
+        total += out->level_count[l];
+    }
+    out->n_nodes = total;
+    return VS_OK;
+}
+
+int vs_polygon_bounds(const double* x, const double* y, const int32_t* geom_offsets,
+                      const int32_t* part_offsets, const int32_t* ring_offsets, int64_t n_rows,
+                      double* bounds, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n_rows >= 0, "vs_polygon_bounds: negative n_rows");
+    return vs::polygon_bounds(x, y, geom_offsets, part_offsets, ring_offsets, n_rows, bounds,
+                              vs::stream_of(stream));
+}
+
+int vs_point_boxes(const double* px, const double* py, int64_t n, double* boxes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n >= 0, "vs_point_boxes: negative n");
+    return vs::point_boxes(px, py, n, boxes, vs::stream_of(stream));
+}
+
+int64_t vs_index_workspace_bytes(int64_t n_items) { return vs::index_workspace_bytes(n_items); }
+
+int vs_index_build(const double* bounds, int64_t n_items, const vs_tree_layout* layout,
+                   double* node_boxes, int32_t* item_ids, uint64_t* item_keys, void* workspace,
+                   int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    return vs::index_build(bounds, n_items, layout, node_boxes, item_ids, item_keys, workspace,
+                           workspace_bytes, vs::stream_of(stream));
+}
+
+int64_t vs_point_order_workspace_bytes(int64_t n_points) {
+    return vs::point_order_workspace_bytes(n_points);
+}
+
+int vs_point_order(const double* px, const double* py, int64_t n_points, int32_t bits_per_axis,
+                   int32_t* perm, double* sorted_x, double* sorted_y, void* workspace,
+                   int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(bits_per_axis >= 1 && bits_per_axis <= 16, "vs_point_order: bits_per_axis %d not in [1,16]",
+               bits_per_axis);
+    VS_REQUIRE(n_points >= 0 && n_points < (int64_t)INT32_MAX, "vs_point_order: n out of range");
+    VS_REQUIRE(workspace_bytes >= vs::point_order_workspace_bytes(n_points),
+               "vs_point_order: workspace too small");
+    return vs::point_order(px, py, n_points, bits_per_axis, perm, sorted_x, sorted_y, workspace,
+                           vs::stream_of(stream));
+}
+
+uint64_t vs_host_hilbert64(uint32_t x, uint32_t y) { return vs::hilbert64(x, y); }
+
+}  // extern "C"

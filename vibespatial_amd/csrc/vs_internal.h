@@ -1,0 +1,153 @@
+// Internal helpers shared by the vsjoin HIP sources (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <cstdarg>
+#include <cstdio>
+
+#include "../../include/vsjoin.h"
+
+namespace vs {
+
+// ---------------------------------------------------------------------------
+// Errors: every entry point returns VS_OK or a negative code and leaves a
+// thread-local message readable through vs_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define VS_CHECK_HIP(expr)                                                        \
+    do {                                                                          \
+        hipError_t e__ = (expr);                                                  \
+        if (e__ != hipSuccess) {                                                  \
+            ::vs::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e__), \
+                            __FILE__, __LINE__);                                  \
+            return VS_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define VS_REQUIRE(cond, ...)           \
+    do {                                \
+        if (!(cond)) {                  \
+            ::vs::set_error(__VA_ARGS__); \
+            return VS_ERR_INVALID;      \
+        }                               \
+    } while (0)
+
+#define VS_LAUNCH_CHECK()                                                          \
+    do {                                                                           \
+        hipError_t e__ = hipGetLastError();                                        \
+        if (e__ != hipSuccess) {                                                   \
+            ::vs::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e__), \
+                            __FILE__, __LINE__);                                   \
+            return VS_ERR_HIP;                                                     \
+        }                                                                          \
+    } while (0)
+
+#define VS_TRY(expr)             \
+    do {                         \
+        int rc__ = (expr);       \
+        if (rc__ != VS_OK) return rc__; \
+    } while (0)
+
+inline hipStream_t stream_of(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------
+// Workspace carving: the caller hands one device buffer; sub-buffers are
+// 256-byte aligned slices of it.  `bytes` accumulates the requirement so the
+// same code path answers workspace-size queries (base == nullptr).
+// ---------------------------------------------------------------------------
+struct Carver {
+    char* base;
+    int64_t bytes = 0;
+    explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+    template <class T>
+    T* take(int64_t count) {
+        int64_t off = (bytes + 255) & ~int64_t(255);
+        bytes = off + (count > 0 ? count : 1) * int64_t(sizeof(T));
+        return base ? reinterpret_cast<T*>(base + off) : nullptr;
+    }
+};
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// Device helpers for 64-wide wavefronts.
+// ---------------------------------------------------------------------------
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    union { double d; int i[2]; } u;
+    u.d = v;
+    u.i[0] = __builtin_amdgcn_readlane(u.i[0], lane);
+    u.i[1] = __builtin_amdgcn_readlane(u.i[1], lane);
+    return u.d;
+}
+
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ int readlane_i32(int v, int lane) {
+    return __builtin_amdgcn_readlane(v, lane);
+}
+
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = fmin(v, w);
+    }
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = fmax(v, w);
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i32(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Inclusive scan across the 64 lanes.
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    const int l = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        int w = __shfl_up(v, o, 64);
+        if (l >= o) v += w;
+    }
+    return v;
+}
+
+__device__ __forceinline__ long long wave_inclusive_scan_i64(long long v) {
+    const int l = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        long long w = __shfl_up(v, o, 64);
+        if (l >= o) v += w;
+    }
+    return v;
+}
+
+// Inclusive bbox overlap; NaN boxes never overlap anything.
+__device__ __forceinline__ bool box_overlap(double qx0, double qy0, double qx1, double qy1,
+                                            double tx0, double ty0, double tx1, double ty1) {
+    return qx0 <= tx1 && qx1 >= tx0 && qy0 <= ty1 && qy1 >= ty0;
+}
+
+}  // namespace vs

@@ -1,0 +1,560 @@
+// Spatial join: packed-tree traversal, exact refine, pair compaction.
+//
+// Replaces, on the sjoin path:
+//   spatial_index_device_query + Morton range count/scatter
+//       src/vibespatial/spatial/spatial_index_device.py:228-390, 761-1034
+//       src/vibespatial/kernels/core/spatial_query_source.py:1132-1253
+//   classify_point_predicates_indexed_device / point_in_polygon_prepared_part_y_index
+//       src/vibespatial/predicates/point_relations.py:1040-1191
+//       src/vibespatial/predicates/point_location_index_kernels.py:157-315
+//   compute_polygonal_intersects_gpu / polygon_polygon_intersects_from_owned
+//       src/vibespatial/predicates/polygon.py:849-996, polygon_kernels.py:1150-1197
+//   compaction d_left[keep] + count_scatter_total + lexsort
+//       src/vibespatial/spatial/query_utils.py:867-868, cuda/_runtime.py:2015-2038,
+//       src/vibespatial/spatial/query.py:143-153
+//
+// Point x polygon (the hot path) is one persistent kernel:
+//   * queries are points sorted by 16-level Hilbert key, 64 per wave;
+//   * each wave walks the packed tree depth first with one shared stack: a
+//     group of up to F sibling boxes is loaded by lanes 0..F-1 (the top levels
+//     come from LDS), tested against the union box of the wave's points and the
+//     overlapping ones are pushed;
+//   * at an item (polygon) the wave streams the polygon's coordinates once,
+//     64 at a time with coalesced loads, broadcasts each edge with readlane and
+//     every lane whose point is in the item's box runs the exact crossing /
+//     boundary step on it (fp64 with the stage-A filter, exact expansion
+//     fallback);
+//   * refined pairs are appended with one atomic per (wave, polygon); each lane
+//     counts its own point's pairs.
+// The finish pass (scan of per-point counts, scatter, per-point sort of the
+// polygon ids) produces pairs lexsorted by (point, polygon) as int64.
+// Waves are handed out from 8 queues, one per XCD, each over a contiguous
+// range of the Hilbert order, so an XCD's L2 sees a spatially compact slice;
+// an idle XCD steals from the others.  Every wave exits when all 8 are empty.
+#include "vs_exact.h"
+#include "vs_internal.h"
+
+namespace vs {
+
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, long long* d_total, void* ws,
+                       hipStream_t s);
+int64_t scan_workspace_bytes(int64_t n);
+int64_t point_order_workspace_bytes(int64_t n);
+int point_order(const double* px, const double* py, int64_t n, int bits, int32_t* perm, double* sx,
+                double* sy, void* ws, hipStream_t s);
+
+constexpr int kJoinThreads = 256;
+constexpr int kJoinWaves = kJoinThreads / 64;
+constexpr int kStackMax = 256;
+constexpr int kTopMax = 640;  // 20 KB of fp64 node boxes staged per block
+
+struct TreeView {
+    const double4* box;
+    const int32_t* item_ids;
+    int64_t level_start[VS_MAX_LEVELS];
+    int64_t level_count[VS_MAX_LEVELS];
+    int32_t n_levels;
+    int32_t fanout;
+    int64_t n_top;
+};
+
+struct PolyView {
+    const double* x;
+    const double* y;
+    const int32_t* geom;
+    const int32_t* part;
+    const int32_t* ring;
+};
+
+struct JoinCtl {
+    unsigned long long staged;  // pairs appended (may exceed capacity)
+    long long total;            // scan total of per-query counts
+    unsigned int queue[8];      // per-XCD wave queues
+    unsigned long long stat[4]; // [0] candidate pairs (bbox hits), [1] sum of their polygons' vertex counts
+};
+
+enum { kModePip = 0, kModeBox = 1 };
+
+__global__ void ctl_reset_kernel(JoinCtl* ctl) {
+    if (threadIdx.x == 0) {
+        ctl->staged = 0;
+        ctl->total = 0;
+        for (int i = 0; i < 8; ++i) ctl->queue[i] = 0;
+        for (int i = 0; i < 4; ++i) ctl->stat[i] = 0;
+    }
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int PRED>
+__device__ __forceinline__ bool pred_of_location(int loc) {
+    if (PRED == VS_PRED_WITHIN || PRED == VS_PRED_CONTAINS) return loc == 2;
+    if (PRED == VS_PRED_TOUCHES) return loc == 1;
+    return loc != 0;  // intersects, covered_by, covers
+}
+
+// Wave-cooperative exact location of each active lane's point in polygonal
+// row `row`; returns the predicate result per lane.
+template <int PRED>
+__device__ bool refine_pip_wave(double px, double py, bool active, int32_t row, const PolyView& P) {
+    const int lane = lane_id();
+    int loc = 0;
+    bool done = !active;
+    const int32_t p0 = P.geom[row], p1 = P.geom[row + 1];
+    for (int32_t pp = p0; pp < p1; ++pp) {
+        int parity = 0;
+        const int32_t r0 = P.part[pp], r1 = P.part[pp + 1];
+        for (int32_t rr = r0; rr < r1; ++rr) {
+            const int32_t c0 = P.ring[rr], c1 = P.ring[rr + 1];
+            double ax = 0.0, ay = 0.0;
+            for (int32_t base = c0; base < c1; base += 64) {
+                const int32_t c = base + lane;
+                const double xv = c < c1 ? P.x[c] : 0.0;
+                const double yv = c < c1 ? P.y[c] : 0.0;
+                const int jn = (c1 - base) < 64 ? (c1 - base) : 64;
+                for (int jj = 0; jj < jn; ++jj) {
+                    const double bx = readlane_f64(xv, jj);
+                    const double by = readlane_f64(yv, jj);
+                    if (base + jj > c0 && !done) {
+                        if (edge_step_f64(px, py, ax, ay, bx, by, parity)) {
+                            loc = 1;
+                            done = true;
+                        }
+                    }
+                    ax = bx;
+                    ay = by;
+                }
+                if (__ballot(!done) == 0) goto finished;
+            }
+        }
+        if (!done && parity) {
+            loc = 2;
+            done = true;
+        }
+        if (__ballot(!done) == 0) break;
+    }
+finished:
+    return active && pred_of_location<PRED>(loc);
+}
+
+// Persistent traversal kernel.  MODE kModePip: queries are points (qa, qb =
+// sorted x, y) with fused refine.  MODE kModeBox: queries are boxes given as
+// (qbox, perm) and every bbox pair is emitted.
+template <int MODE, int PRED>
+__global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
+    const double* __restrict__ qx, const double* __restrict__ qy, const double4* __restrict__ qbox,
+    const int32_t* __restrict__ perm, int64_t n, TreeView T, PolyView P, int2* __restrict__ stage,
+    int64_t capacity, JoinCtl* __restrict__ ctl, int32_t* __restrict__ counts) {
+    __shared__ double4 top[kTopMax];
+    __shared__ long long stack_s[kJoinWaves][kStackMax];
+    for (int i = threadIdx.x; i < T.n_top; i += kJoinThreads) top[i] = T.box[i];
+    __syncthreads();
+
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    long long* stack = stack_s[w];
+    const int64_t W = (n + 63) >> 6;
+    const int home = blockIdx.x & 7;
+    const int last = T.n_levels - 1;
+    const int F = T.fanout;
+    const uint64_t lt = lanemask_lt();
+
+    for (int xq = 0; xq < 8;) {
+        const int q = (home + xq) & 7;
+        const int64_t r0 = W * q / 8, r1 = W * (q + 1) / 8;
+        unsigned int t = 0;
+        if (lane == 0) t = atomicAdd(&ctl->queue[q], 1u);
+        t = __shfl(t, 0, 64);
+        if ((int64_t)t >= r1 - r0) {
+            ++xq;
+            continue;
+        }
+        const int64_t idx = (r0 + t) * 64 + lane;
+        const bool inr = idx < n;
+        double x0, y0, x1, y1;
+        int32_t orig = 0;
+        if (MODE == kModePip) {
+            x0 = inr ? qx[idx] : NAN;
+            y0 = inr ? qy[idx] : NAN;
+            x1 = x0;
+            y1 = y0;
+            orig = inr ? perm[idx] : 0;
+        } else {
+            orig = inr ? (perm ? perm[idx] : (int32_t)idx) : 0;
+            const double4 b = inr ? qbox[orig] : make_double4(NAN, NAN, NAN, NAN);
+            x0 = b.x; y0 = b.y; x1 = b.z; y1 = b.w;
+        }
+        const bool valid = inr && x0 == x0 && y0 == y0 && x1 == x1 && y1 == y1;
+        const double ux0 = wave_min(valid ? x0 : INFINITY), uy0 = wave_min(valid ? y0 : INFINITY);
+        const double ux1 = wave_max(valid ? x1 : -INFINITY), uy1 = wave_max(valid ? y1 : -INFINITY);
+        int cnt = 0;
+        if (__ballot(valid) != 0 && T.level_count[0] > 0) {
+            int sp = 1;
+            if (lane == 0) stack[0] = 0;  // group 0 of level 0 = the root
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            while (sp > 0) {
+                --sp;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const long long e = stack[sp];
+                const int l = (int)(e & 15);
+                const int64_t first = (e >> 4) * F;
+                const int64_t cl = T.level_count[l];
+                const int ng = (int)((cl - first) < F ? (cl - first) : F);
+                const int64_t g = T.level_start[l] + first + lane;
+                double4 b = make_double4(NAN, NAN, NAN, NAN);
+                bool ov = false;
+                if (lane < ng) {
+                    b = (g < T.n_top) ? top[g] : T.box[g];
+                    ov = box_overlap(ux0, uy0, ux1, uy1, b.x, b.y, b.z, b.w);
+                }
+                uint64_t m = __ballot(ov);
+                if (l == last) {
+                    while (m) {
+                        const int s = __ffsll((unsigned long long)m) - 1;
+                        m &= m - 1;
+                        const double bx0 = readlane_f64(b.x, s), by0 = readlane_f64(b.y, s);
+                        const double bx1 = readlane_f64(b.z, s), by1 = readlane_f64(b.w, s);
+                        const bool hit = valid && box_overlap(x0, y0, x1, y1, bx0, by0, bx1, by1);
+                        const uint64_t hm = __ballot(hit);
+                        if (!hm) continue;
+                        const int32_t row = T.item_ids[first + s];
+                        bool res;
+                        if (MODE == kModePip) {
+                            if (lane == 0) {
+                                const unsigned long long h = (unsigned long long)__popcll(hm);
+                                const int32_t nv = P.ring[P.part[P.geom[row + 1]]] - P.ring[P.part[P.geom[row]]];
+                                atomicAdd(&ctl->stat[0], h);
+                                atomicAdd(&ctl->stat[1], h * (unsigned long long)nv);
+                            }
+                            res = refine_pip_wave<PRED>(x0, y0, hit, row, P);
+                        } else {
+                            res = hit;
+                        }
+                        const uint64_t rm = __ballot(res);
+                        if (rm) {
+                            unsigned long long base = 0;
+                            if (lane == 0) base = atomicAdd(&ctl->staged, (unsigned long long)__popcll(rm));
+                            base = __shfl(base, 0, 64);
+                            if (res) {
+                                const unsigned long long slot = base + __popcll(rm & lt);
+                                if (slot < (unsigned long long)capacity) stage[slot] = make_int2(orig, row);
+                                ++cnt;
+                            }
+                        }
+                    }
+                } else {
+                    while (m) {
+                        const int s = 63 - __clzll((long long)m);
+                        m &= ~(1ull << s);
+                        if (lane == 0) stack[sp] = ((first + s) << 4) | (long long)(l + 1);
+                        ++sp;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                }
+            }
+        }
+        if (inr) counts[orig] = cnt;
+    }
+}
+
+// Scatter staged pairs into their lexsorted slots (order inside a query's run
+// is fixed afterwards by segment_sort_kernel).  Skipped when the staging buffer
+// overflowed: the caller retries with a larger capacity.
+__global__ __launch_bounds__(256) void scatter_pairs_kernel(const int2* __restrict__ stage,
+                                                           const JoinCtl* __restrict__ ctl,
+                                                           int64_t capacity,
+                                                           const int32_t* __restrict__ offsets,
+                                                           int32_t* __restrict__ cursor,
+                                                           int64_t* __restrict__ out_l,
+                                                           int64_t* __restrict__ out_r) {
+    const unsigned long long staged = ctl->staged;
+    if (staged > (unsigned long long)capacity) return;
+    for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < (int64_t)staged;
+         s += (int64_t)gridDim.x * 256) {
+        const int2 e = stage[s];
+        const int32_t pos = offsets[e.x] + atomicAdd(&cursor[e.x], 1);
+        out_l[pos] = e.x;
+        out_r[pos] = e.y;
+    }
+}
+
+// Insertion sort of each query's run of right indices (runs are short).
+__global__ __launch_bounds__(256) void segment_sort_kernel(const JoinCtl* __restrict__ ctl,
+                                                          int64_t capacity,
+                                                          const int32_t* __restrict__ offsets,
+                                                          const int32_t* __restrict__ counts, int64_t n,
+                                                          int64_t* __restrict__ out_r) {
+    if (ctl->staged > (unsigned long long)capacity) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int32_t c = counts[i];
+    if (c < 2) return;
+    int64_t* r = out_r + offsets[i];
+    for (int32_t a = 1; a < c; ++a) {
+        const int64_t v = r[a];
+        int32_t b = a - 1;
+        while (b >= 0 && r[b] > v) {
+            r[b + 1] = r[b];
+            --b;
+        }
+        r[b + 1] = v;
+    }
+}
+
+__global__ void publish_result_kernel(const JoinCtl* __restrict__ ctl, int64_t* __restrict__ result) {
+    if (threadIdx.x == 0) {
+        result[0] = ctl->total;
+        result[1] = (int64_t)ctl->staged;
+        result[2] = (int64_t)ctl->stat[0];
+        result[3] = (int64_t)ctl->stat[1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+TreeView make_tree_view(const vs_tree_layout* L, const double* node_boxes, const int32_t* item_ids) {
+    TreeView T;
+    memset(&T, 0, sizeof(T));
+    T.box = reinterpret_cast<const double4*>(node_boxes);
+    T.item_ids = item_ids;
+    T.n_levels = L->n_levels;
+    T.fanout = L->fanout;
+    for (int l = 0; l < L->n_levels; ++l) {
+        T.level_start[l] = L->level_start[l];
+        T.level_count[l] = L->level_count[l];
+    }
+    // stage whole levels (never the items level) while they fit
+    int64_t top = 0;
+    for (int l = 0; l + 1 < L->n_levels; ++l) {
+        const int64_t end = L->level_start[l] + L->level_count[l];
+        if (end <= kTopMax) top = end;
+    }
+    T.n_top = top;
+    return T;
+}
+
+int check_layout(const vs_tree_layout* L) {
+    VS_REQUIRE(L != nullptr, "null tree layout");
+    VS_REQUIRE(L->n_levels >= 1 && L->n_levels <= VS_MAX_LEVELS, "tree layout: bad level count %d",
+               L->n_levels);
+    VS_REQUIRE(L->fanout >= 2 && L->fanout <= 64, "tree layout: bad fanout %d", L->fanout);
+    VS_REQUIRE((int64_t)L->n_levels * L->fanout <= kStackMax, "tree layout: too deep for the stack");
+    VS_REQUIRE(L->level_count[L->n_levels - 1] == L->n_items, "tree layout: item level mismatch");
+    return VS_OK;
+}
+
+struct JoinWs {
+    int32_t* perm;
+    double* sx;
+    double* sy;
+    void* order_ws;
+    int2* stage;
+    int32_t* counts;
+    int32_t* offsets;
+    int32_t* cursor;
+    void* scan_ws;
+    JoinCtl* ctl;
+};
+
+JoinWs carve_join_ws(void* base, int64_t nq, int64_t capacity, bool sorted_points, int64_t* bytes) {
+    Carver c(base);
+    JoinWs J;
+    J.perm = c.take<int32_t>(nq);
+    if (sorted_points) {
+        J.sx = c.take<double>(nq);
+        J.sy = c.take<double>(nq);
+        J.order_ws = c.take<char>(point_order_workspace_bytes(nq));
+    } else {
+        J.sx = J.sy = nullptr;
+        J.order_ws = nullptr;
+    }
+    J.stage = c.take<int2>(capacity);
+    J.counts = c.take<int32_t>(nq);
+    J.offsets = c.take<int32_t>(nq);
+    J.cursor = c.take<int32_t>(nq);
+    J.scan_ws = c.take<char>(scan_workspace_bytes(nq));
+    J.ctl = c.take<JoinCtl>(1);
+    if (bytes) *bytes = c.bytes;
+    return J;
+}
+
+int grid_for(int64_t n_waves) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
+            cus = p.multiProcessorCount;
+    }
+    const int64_t want = ceil_div(n_waves, kJoinWaves);
+    int64_t g = std::min<int64_t>(want, (int64_t)cus * 8);
+    g = std::max<int64_t>(g, 8);
+    return (int)((g + 7) / 8 * 8);
+}
+
+// Optional timing of the traversal kernel with HIP events recorded on the
+// launch stream (instrumentation for benchmarks; process-wide, not reentrant).
+static bool g_timing = false;
+static hipEvent_t g_ev[2];
+static bool g_ev_ready = false;
+
+int timing_begin(hipStream_t s) {
+    if (!g_timing) return VS_OK;
+    if (!g_ev_ready) {
+        VS_CHECK_HIP(hipEventCreate(&g_ev[0]));
+        VS_CHECK_HIP(hipEventCreate(&g_ev[1]));
+        g_ev_ready = true;
+    }
+    VS_CHECK_HIP(hipEventRecord(g_ev[0], s));
+    return VS_OK;
+}
+
+int timing_end(hipStream_t s) {
+    if (g_timing && g_ev_ready) VS_CHECK_HIP(hipEventRecord(g_ev[1], s));
+    return VS_OK;
+}
+
+template <int MODE>
+int launch_traverse(int pred, const double* qx, const double* qy, const double4* qbox,
+                    const int32_t* perm, int64_t n, const TreeView& T, const PolyView& P,
+                    const JoinWs& J, int64_t capacity, hipStream_t s) {
+    const int grid = grid_for(ceil_div(n, 64));
+    VS_TRY(timing_begin(s));
+#define VS_LAUNCH_T(PR)                                                                          \
+    hipLaunchKernelGGL((traverse_kernel<MODE, PR>), dim3(grid), dim3(kJoinThreads), 0, s, qx, qy, \
+                       qbox, perm, n, T, P, J.stage, capacity, J.ctl, J.counts)
+    switch (pred) {
+        case VS_PRED_WITHIN: VS_LAUNCH_T(VS_PRED_WITHIN); break;
+        case VS_PRED_INTERSECTS: VS_LAUNCH_T(VS_PRED_INTERSECTS); break;
+        case VS_PRED_COVERED_BY: VS_LAUNCH_T(VS_PRED_COVERED_BY); break;
+        case VS_PRED_TOUCHES: VS_LAUNCH_T(VS_PRED_TOUCHES); break;
+        default: VS_REQUIRE(false, "unsupported predicate %d for this join", pred);
+    }
+#undef VS_LAUNCH_T
+    VS_LAUNCH_CHECK();
+    VS_TRY(timing_end(s));
+    return VS_OK;
+}
+
+int finish_pairs(const JoinWs& J, int64_t nq, int64_t capacity, int64_t* out_l, int64_t* out_r,
+                 int64_t* d_result, hipStream_t s) {
+    VS_TRY(exclusive_scan_i32(J.counts, J.offsets, nq, &J.ctl->total, J.scan_ws, s));
+    if (nq > 0) {
+        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, J.cursor, nq, 0);
+        VS_LAUNCH_CHECK();
+    }
+    const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, 256), 65536));
+    hipLaunchKernelGGL(scatter_pairs_kernel, dim3(sgrid), dim3(256), 0, s, J.stage, J.ctl, capacity,
+                       J.offsets, J.cursor, out_l, out_r);
+    VS_LAUNCH_CHECK();
+    if (nq > 0) {
+        hipLaunchKernelGGL(segment_sort_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, J.ctl,
+                           capacity, J.offsets, J.counts, nq, out_r);
+        VS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(publish_result_kernel, dim3(1), dim3(64), 0, s, J.ctl, d_result);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+}  // namespace vs
+
+extern "C" {
+
+int vs_set_kernel_timing(int32_t enable) {
+    vs::clear_error();
+    vs::g_timing = enable != 0;
+    return VS_OK;
+}
+
+int vs_last_kernel_ms(float* ms) {
+    vs::clear_error();
+    VS_REQUIRE(ms != nullptr, "vs_last_kernel_ms: null output");
+    VS_REQUIRE(vs::g_timing && vs::g_ev_ready, "vs_last_kernel_ms: timing not enabled or nothing timed yet");
+    VS_CHECK_HIP(hipEventSynchronize(vs::g_ev[1]));
+    VS_CHECK_HIP(hipEventElapsedTime(ms, vs::g_ev[0], vs::g_ev[1]));
+    return VS_OK;
+}
+
+int64_t vs_sjoin_points_polygons_workspace_bytes(int64_t n_points, int64_t capacity) {
+    int64_t b = 0;
+    vs::carve_join_ws(nullptr, n_points, capacity, true, &b);
+    return b;
+}
+
+int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_points, const double* x,
+                             const double* y, const int32_t* geom_offsets, const int32_t* part_offsets,
+                             const int32_t* ring_offsets, int64_t n_polys, const vs_tree_layout* layout,
+                             const double* node_boxes, const int32_t* item_ids, int32_t predicate,
+                             int64_t capacity, int64_t* out_left, int64_t* out_right, int64_t* d_result,
+                             void* workspace, int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    VS_TRY(vs::check_layout(layout));
+    VS_REQUIRE(layout->n_items == n_polys, "vs_sjoin_points_polygons: layout has %lld items, %lld polygons",
+               (long long)layout->n_items, (long long)n_polys);
+    VS_REQUIRE(n_points >= 0 && n_points < (int64_t)INT32_MAX, "vs_sjoin_points_polygons: n_points out of range");
+    VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_sjoin_points_polygons: capacity out of range");
+    VS_REQUIRE(predicate == VS_PRED_WITHIN || predicate == VS_PRED_INTERSECTS ||
+                   predicate == VS_PRED_COVERED_BY || predicate == VS_PRED_TOUCHES,
+               "vs_sjoin_points_polygons: unsupported predicate %d", predicate);
+    VS_REQUIRE(d_result != nullptr, "vs_sjoin_points_polygons: null result buffer");
+    int64_t need = 0;
+    const vs::JoinWs J = vs::carve_join_ws(workspace, n_points, capacity, true, &need);
+    VS_REQUIRE(workspace_bytes >= need, "vs_sjoin_points_polygons: workspace %lld < %lld bytes",
+               (long long)workspace_bytes, (long long)need);
+    hipStream_t s = vs::stream_of(stream);
+    hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl);
+    VS_LAUNCH_CHECK();
+    VS_TRY(vs::point_order(px, py, n_points, 16, J.perm, J.sx, J.sy, J.order_ws, s));
+    const vs::TreeView T = vs::make_tree_view(layout, node_boxes, item_ids);
+    const vs::PolyView P = {x, y, geom_offsets, part_offsets, ring_offsets};
+    if (n_points > 0 && n_polys > 0)
+        VS_TRY(vs::launch_traverse<vs::kModePip>(predicate, J.sx, J.sy, nullptr, J.perm, n_points, T, P, J,
+                                                 capacity, s));
+    else if (n_points > 0) {
+        hipLaunchKernelGGL(vs::fill_i32_kernel, dim3((unsigned)vs::ceil_div(n_points, 256)), dim3(256), 0, s,
+                           J.counts, n_points, 0);
+        VS_LAUNCH_CHECK();
+    }
+    return vs::finish_pairs(J, n_points, capacity, out_left, out_right, d_result, s);
+}
+
+int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity) {
+    int64_t b = 0;
+    vs::carve_join_ws(nullptr, n_queries, capacity, false, &b);
+    return b;
+}
+
+int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_layout* layout,
+                   const double* node_boxes, const int32_t* item_ids, int64_t capacity, int64_t* out_left,
+                   int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
+                   void* stream) {
+    vs::clear_error();
+    VS_TRY(vs::check_layout(layout));
+    VS_REQUIRE(n_queries >= 0 && n_queries < (int64_t)INT32_MAX, "vs_query_boxes: n_queries out of range");
+    VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_query_boxes: capacity out of range");
+    VS_REQUIRE(d_result != nullptr, "vs_query_boxes: null result buffer");
+    int64_t need = 0;
+    const vs::JoinWs J = vs::carve_join_ws(workspace, n_queries, capacity, false, &need);
+    VS_REQUIRE(workspace_bytes >= need, "vs_query_boxes: workspace %lld < %lld bytes",
+               (long long)workspace_bytes, (long long)need);
+    hipStream_t s = vs::stream_of(stream);
+    hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl);
+    VS_LAUNCH_CHECK();
+    const vs::TreeView T = vs::make_tree_view(layout, node_boxes, item_ids);
+    const vs::PolyView P = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (n_queries > 0 && layout->n_items > 0)
+        VS_TRY(vs::launch_traverse<vs::kModeBox>(VS_PRED_INTERSECTS, nullptr, nullptr,
+                                                 reinterpret_cast<const double4*>(query_boxes), nullptr,
+                                                 n_queries, T, P, J, capacity, s));
+    else if (n_queries > 0) {
+        hipLaunchKernelGGL(vs::fill_i32_kernel, dim3((unsigned)vs::ceil_div(n_queries, 256)), dim3(256), 0, s,
+                           J.counts, n_queries, 0);
+        VS_LAUNCH_CHECK();
+    }
+    return vs::finish_pairs(J, n_queries, capacity, out_left, out_right, d_result, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+// Pair refine over candidate pairs, order-preserving compaction, pair lexsort.
+//
+// Used by the routes that are not the fused point-in-polygon join:
+//   * polygon x polygon intersects after a bbox query, one thread per pair:
+//     any edge pair touches or crosses (exact orientation + collinear
+//     containment), else a vertex of one located in the other
+//     (src/vibespatial/predicates/polygon_kernels.py:131-228);
+//   * point x polygon location for swapped predicates (contains / covers of a
+//     polygon over tree points), predicate mapping of
+//     src/vibespatial/predicates/point_relations.py:100-129.
+// Compaction keeps the input order, so lexsorted candidates give lexsorted
+// output (replaces d_left[keep], spatial/query_utils.py:867-868).
+#include "vs_exact.h"
+#include "vs_internal.h"
+
+namespace vs {
+
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, long long* d_total, void* ws,
+                       hipStream_t s);
+int64_t scan_workspace_bytes(int64_t n);
+int sort_pairs_u64(uint64_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
+int64_t sort_workspace_bytes_u64(int64_t n);
+
+struct Polys {
+    const double* x;
+    const double* y;
+    const int32_t* geom;
+    const int32_t* part;
+    const int32_t* ring;
+};
+
+__device__ __forceinline__ int loc_keep(int pred, int loc) {
+    switch (pred) {
+        case VS_PRED_WITHIN:
+        case VS_PRED_CONTAINS: return loc == 2;
+        case VS_PRED_TOUCHES: return loc == 1;
+        default: return loc != 0;
+    }
+}
+
+__device__ bool polygons_intersect(const Polys& A, int64_t ra, const Polys& B, int64_t rb) {
+    const int32_t ap0 = A.geom[ra], ap1 = A.geom[ra + 1];
+    const int32_t bp0 = B.geom[rb], bp1 = B.geom[rb + 1];
+    const int32_t ar0 = A.part[ap0], ar1 = A.part[ap1];
+    const int32_t br0 = B.part[bp0], br1 = B.part[bp1];
+    if (A.ring[ar1] <= A.ring[ar0] || B.ring[br1] <= B.ring[br0]) return false;
+    for (int32_t ar = ar0; ar < ar1; ++ar)
+        for (int32_t ai = A.ring[ar] + 1; ai < A.ring[ar + 1]; ++ai) {
+            const double ax0 = A.x[ai - 1], ay0 = A.y[ai - 1], ax1 = A.x[ai], ay1 = A.y[ai];
+            for (int32_t br = br0; br < br1; ++br)
+                for (int32_t bi = B.ring[br] + 1; bi < B.ring[br + 1]; ++bi)
+                    if (segments_touch(ax0, ay0, ax1, ay1, B.x[bi - 1], B.y[bi - 1], B.x[bi], B.y[bi]))
+                        return true;
+        }
+    for (int32_t ar = ar0; ar < ar1; ++ar) {
+        const int32_t c0 = A.ring[ar], c1 = A.ring[ar + 1];
+        const int32_t vl = c1 > c0 + 1 ? c1 - 1 : c1;
+        for (int32_t v = c0; v < vl; ++v)
+            if (point_location_f64(A.x[v], A.y[v], B.x, B.y, B.geom, B.part, B.ring, rb) != 0) return true;
+    }
+    for (int32_t br = br0; br < br1; ++br) {
+        const int32_t c0 = B.ring[br], c1 = B.ring[br + 1];
+        const int32_t vl = c1 > c0 + 1 ? c1 - 1 : c1;
+        for (int32_t v = c0; v < vl; ++v)
+            if (point_location_f64(B.x[v], B.y[v], A.x, A.y, A.geom, A.part, A.ring, ra) != 0) return true;
+    }
+    return false;
+}
+
+// mode 0: left = points (px, py), right = polygons B; keep = pred(location).
+// mode 1: left = polygons A, right = polygons B; keep = intersects.
+// mode 2: left = polygons B, right = points (px, py); keep = pred(location of
+//         the right point in the left polygon) -- contains / covers / touches /
+//         intersects of a query polygon over tree points.
+__global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, const int64_t* __restrict__ left,
+                                                          const int64_t* __restrict__ right, int64_t n,
+                                                          const double* __restrict__ px,
+                                                          const double* __restrict__ py, Polys A, Polys B,
+                                                          int32_t* __restrict__ keep) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t l = left[i], r = right[i];
+        int k;
+        if (mode == 0 || mode == 2) {
+            const int64_t pi = mode == 0 ? l : r, gi = mode == 0 ? r : l;
+            const double x = px[pi], y = py[pi];
+            const int loc = (x == x && y == y) ? point_location_f64(x, y, B.x, B.y, B.geom, B.part, B.ring, gi) : 0;
+            k = loc_keep(pred, loc);
+        } else {
+            k = polygons_intersect(A, l, B, r) ? 1 : 0;
+        }
+        keep[i] = k;
+    }
+}
+
+__global__ __launch_bounds__(256) void compact_pairs_kernel(const int64_t* __restrict__ left,
+                                                           const int64_t* __restrict__ right, int64_t n,
+                                                           const int32_t* __restrict__ keep,
+                                                           const int32_t* __restrict__ pos,
+                                                           int64_t* __restrict__ out_l,
+                                                           int64_t* __restrict__ out_r) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && keep[i]) {
+        out_l[pos[i]] = left[i];
+        out_r[pos[i]] = right[i];
+    }
+}
+
+__global__ void total_to_result_kernel(const long long* __restrict__ total, int64_t* __restrict__ result) {
+    if (threadIdx.x == 0) result[0] = *total;
+}
+
+int64_t refine_workspace_bytes(int64_t n) {
+    Carver c(nullptr);
+    c.take<int32_t>(n);
+    c.take<int32_t>(n);
+    c.take<long long>(1);
+    c.take<char>(scan_workspace_bytes(n));
+    return c.bytes;
+}
+
+__global__ __launch_bounds__(256) void pack_pairs_kernel(const int64_t* __restrict__ l, const int64_t* __restrict__ r,
+                                                        int64_t n, uint64_t* __restrict__ key,
+                                                        int32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        key[i] = ((uint64_t)l[i] << 32) | (uint64_t)(uint32_t)r[i];
+        idx[i] = (int32_t)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void unpack_pairs_kernel(const uint64_t* __restrict__ key, int64_t n,
+                                                          int64_t* __restrict__ l, int64_t* __restrict__ r) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        l[i] = (int64_t)(key[i] >> 32);
+        r[i] = (int64_t)(key[i] & 0xFFFFFFFFull);
+    }
+}
+
+int64_t lexsort_workspace_bytes(int64_t n) {
+    Carver c(nullptr);
+    c.take<uint64_t>(n);
+    c.take<int32_t>(n);
+    c.take<char>(sort_workspace_bytes_u64(n));
+    return c.bytes;
+}
+
+}  // namespace vs
+
+extern "C" {
+
+int64_t vs_refine_pairs_workspace_bytes(int64_t n_pairs) { return vs::refine_workspace_bytes(n_pairs); }
+
+int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const int64_t* right, int64_t n_pairs,
+                    const double* px, const double* py, const double* ax, const double* ay,
+                    const int32_t* a_geom, const int32_t* a_part, const int32_t* a_ring, const double* bx,
+                    const double* by, const int32_t* b_geom, const int32_t* b_part, const int32_t* b_ring,
+                    int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                    int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(mode >= 0 && mode <= 2, "vs_refine_pairs: mode %d not in {0, 1, 2}", mode);
+    VS_REQUIRE(n_pairs >= 0 && n_pairs < (int64_t)INT32_MAX, "vs_refine_pairs: n_pairs out of range");
+    VS_REQUIRE(workspace_bytes >= vs::refine_workspace_bytes(n_pairs), "vs_refine_pairs: workspace too small");
+    VS_REQUIRE(d_result != nullptr, "vs_refine_pairs: null result");
+    hipStream_t s = vs::stream_of(stream);
+    vs::Carver c(workspace);
+    int32_t* keep = c.take<int32_t>(n_pairs);
+    int32_t* pos = c.take<int32_t>(n_pairs);
+    long long* total = c.take<long long>(1);
+    void* scan_ws = c.take<char>(vs::scan_workspace_bytes(n_pairs));
+    const vs::Polys A = {ax, ay, a_geom, a_part, a_ring};
+    const vs::Polys B = {bx, by, b_geom, b_part, b_ring};
+    if (n_pairs > 0) {
+        const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
+        hipLaunchKernelGGL(vs::refine_pairs_kernel, dim3(grid), dim3(256), 0, s, mode, predicate, left, right,
+                           n_pairs, px, py, A, B, keep);
+        VS_LAUNCH_CHECK();
+    }
+    VS_TRY(vs::exclusive_scan_i32(keep, pos, n_pairs, total, scan_ws, s));
+    if (n_pairs > 0) {
+        hipLaunchKernelGGL(vs::compact_pairs_kernel, dim3((unsigned)vs::ceil_div(n_pairs, 256)), dim3(256), 0, s,
+                           left, right, n_pairs, keep, pos, out_left, out_right);
+        VS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(vs::total_to_result_kernel, dim3(1), dim3(64), 0, s, total, d_result);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int64_t vs_lexsort_pairs_workspace_bytes(int64_t n_pairs) { return vs::lexsort_workspace_bytes(n_pairs); }
+
+int vs_lexsort_pairs(int64_t* left, int64_t* right, int64_t n_pairs, void* workspace, int64_t workspace_bytes,
+                     void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n_pairs >= 0 && n_pairs < (int64_t)INT32_MAX, "vs_lexsort_pairs: n_pairs out of range");
+    VS_REQUIRE(workspace_bytes >= vs::lexsort_workspace_bytes(n_pairs), "vs_lexsort_pairs: workspace too small");
+    if (n_pairs < 2) return VS_OK;
+    hipStream_t s = vs::stream_of(stream);
+    vs::Carver c(workspace);
+    uint64_t* key = c.take<uint64_t>(n_pairs);
+    int32_t* idx = c.take<int32_t>(n_pairs);
+    void* sort_ws = c.take<char>(vs::sort_workspace_bytes_u64(n_pairs));
+    const unsigned g = (unsigned)vs::ceil_div(n_pairs, 256);
+    hipLaunchKernelGGL(vs::pack_pairs_kernel, dim3(g), dim3(256), 0, s, left, right, n_pairs, key, idx);
+    VS_LAUNCH_CHECK();
+    VS_TRY(vs::sort_pairs_u64(key, idx, n_pairs, 0, 64, sort_ws, vs::sort_workspace_bytes_u64(n_pairs), s));
+    hipLaunchKernelGGL(vs::unpack_pairs_kernel, dim3(g), dim3(256), 0, s, key, n_pairs, left, right);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,346 @@
+// Device-wide exclusive scan and LSD radix sort of (key, int32 value) pairs.
+//
+// Replaces the CCCL primitives the reference JITs for this path:
+//   exclusive_sum  src/vibespatial/cuda/cccl_primitives.py:596
+//   sort_pairs     src/vibespatial/cuda/cccl_primitives.py:1054 (used by
+//                  src/vibespatial/spatial/indexing.py:1296-1322)
+//
+// Design (gfx950, wave64):
+//   scan   tile = 256 threads x 16 items; reduce -> single-block scan of tile
+//          sums (int64) -> downsweep.  No decoupled look-back, so no inter-block
+//          spin waits.
+//   sort   8-bit digits, three launches per pass: LDS histogram per tile,
+//          device scan of the digit-major histogram, stable scatter.  The
+//          scatter ranks keys inside each wave with 8 ballots per item (peer
+//          mask of equal digits), combines waves through LDS counters, writes the
+//          tile into LDS in digit order and then streams it out so each digit
+//          run is written contiguously.
+#include "vs_internal.h"
+
+namespace vs {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Block exclusive scan of one int per thread (kScanThreads threads).
+__device__ __forceinline__ long long block_exclusive_scan_i64(long long v, long long* lds4,
+                                                              long long& total) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    long long inc = wave_inclusive_scan_i64(v);
+    if (l == 63) lds4[w] = inc;
+    __syncthreads();
+    long long off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kScanThreads / 64; ++k) {
+        const long long t = lds4[k];
+        off += (k < w) ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const int32_t* __restrict__ in,
+                                                                  int64_t n,
+                                                                  long long* __restrict__ part) {
+    __shared__ long long lds[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    long long s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t idx = base + (int64_t)i * kScanThreads + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    long long tot;
+    block_exclusive_scan_i64(s, lds, tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// One block of 1024 threads scans the tile sums in place (exclusive) and
+// writes the grand total.
+__global__ __launch_bounds__(1024) void scan_partials_kernel(long long* __restrict__ part, int64_t nb,
+                                                             long long* __restrict__ total_out) {
+    __shared__ long long lds[16];
+    __shared__ long long carry_s;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    for (int64_t b0 = 0; b0 < nb; b0 += 1024) {
+        const int64_t i = b0 + threadIdx.x;
+        const long long v = i < nb ? part[i] : 0;
+        const long long inc = wave_inclusive_scan_i64(v);
+        if (l == 63) lds[w] = inc;
+        __syncthreads();
+        long long off = 0, tot = 0;
+        for (int k = 0; k < 16; ++k) {
+            off += (k < w) ? lds[k] : 0;
+            tot += lds[k];
+        }
+        const long long carry = carry_s;
+        if (i < nb) part[i] = carry + off + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry_s;
+}
+
+// Downsweep: thread t owns 16 consecutive elements of the tile.
+__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const int32_t* __restrict__ in,
+                                                                int32_t* __restrict__ out, int64_t n,
+                                                                const long long* __restrict__ part) {
+    __shared__ long long lds[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    long long s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    long long tot;
+    long long run = block_exclusive_scan_i64(s, lds, tot) + part[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = (int32_t)run;
+        run += v[i];
+    }
+}
+
+int64_t scan_workspace_bytes(int64_t n) {
+    Carver c(nullptr);
+    c.take<long long>(ceil_div(n, kScanTile) + 1);
+    return c.bytes;
+}
+
+// out[i] = sum(in[0..i)); *d_total (device, int64) = sum(in).  Values must fit
+// int32 per element; the running sum is formed in int64 and narrowed on store
+// (callers check the total against INT32_MAX before trusting offsets).
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, long long* d_total, void* ws,
+                       hipStream_t s) {
+    const int64_t nb = ceil_div(n, kScanTile);
+    Carver c(ws);
+    long long* part = c.take<long long>(nb + 1);
+    if (n == 0) {
+        if (d_total) VS_CHECK_HIP(hipMemsetAsync(d_total, 0, sizeof(long long), s));
+        return VS_OK;
+    }
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part);
+    VS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(1024), 0, s, part, nb, d_total);
+    VS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n, part);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Radix sort
+// ---------------------------------------------------------------------------
+constexpr int kRsThreads = 256;
+constexpr int kRsWaves = kRsThreads / 64;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsThreads * kRsItems;
+constexpr int kRsBins = 256;
+
+template <class K>
+__global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const K* __restrict__ keys, int64_t n,
+                                                            int shift, int32_t* __restrict__ hist,
+                                                            int nblocks) {
+    __shared__ int cnt[kRsWaves][kRsBins];
+    const int w = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+#pragma unroll 4
+    for (int i = 0; i < kRsItems; ++i) {
+        const int64_t idx = base + (int64_t)i * kRsThreads + threadIdx.x;
+        if (idx < n) atomicAdd(&cnt[w][(int)((keys[idx] >> shift) & 255)], 1);
+    }
+    __syncthreads();
+    const int d = threadIdx.x;
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < kRsWaves; ++k) t += cnt[k][d];
+    hist[(int64_t)d * nblocks + blockIdx.x] = t;
+}
+
+template <class K>
+__global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
+    const K* __restrict__ keys_in, const int32_t* __restrict__ vals_in, K* __restrict__ keys_out,
+    int32_t* __restrict__ vals_out, int64_t n, int shift, const int32_t* __restrict__ hist_scan,
+    int nblocks) {
+    __shared__ int wave_cnt[kRsWaves][kRsBins];
+    __shared__ int digit_block_off[kRsBins];
+    __shared__ long long digit_global[kRsBins];
+    __shared__ long long lds4[4];
+    __shared__ K skeys[kRsTile];
+    __shared__ int32_t svals[kRsTile];
+
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < kRsWaves * kRsBins; i += kRsThreads) (&wave_cnt[0][0])[i] = 0;
+    digit_global[tid] = hist_scan[(int64_t)tid * nblocks + blockIdx.x];
+    __syncthreads();
+
+    const int64_t base = (int64_t)blockIdx.x * kRsTile + (int64_t)w * (64 * kRsItems);
+    const uint64_t lt = lanemask_lt();
+    K k[kRsItems];
+    int32_t v[kRsItems];
+    int rank[kRsItems];
+#pragma unroll
+    for (int i = 0; i < kRsItems; ++i) {
+        const int64_t idx = base + (int64_t)i * 64 + lane;
+        const bool valid = idx < n;
+        k[i] = valid ? keys_in[idx] : (K)0;
+        v[i] = valid ? vals_in[idx] : 0;
+        const int d = (int)((k[i] >> shift) & 255);
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        int cur = 0;
+        if (valid) cur = wave_cnt[w][d];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        rank[i] = cur + __popcll(m & lt);
+        if (valid && (m & lt) == 0) wave_cnt[w][d] = cur + __popcll(m);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: exclusive over waves, then exclusive over digits
+    {
+        const int d = tid;
+        int s = 0;
+#pragma unroll
+        for (int ww = 0; ww < kRsWaves; ++ww) {
+            const int c = wave_cnt[ww][d];
+            wave_cnt[ww][d] = s;
+            s += c;
+        }
+        long long tot;
+        const long long off = block_exclusive_scan_i64(s, lds4, tot);
+        digit_block_off[d] = (int)off;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRsItems; ++i) {
+        const int64_t idx = base + (int64_t)i * 64 + lane;
+        if (idx < n) {
+            const int d = (int)((k[i] >> shift) & 255);
+            const int pos = digit_block_off[d] + wave_cnt[w][d] + rank[i];
+            skeys[pos] = k[i];
+            svals[pos] = v[i];
+        }
+    }
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * kRsTile;
+    const int tile_n = (int)((n - t0) < kRsTile ? (n - t0) : kRsTile);
+    for (int p = tid; p < tile_n; p += kRsThreads) {
+        const K kk = skeys[p];
+        const int d = (int)((kk >> shift) & 255);
+        const int64_t g = digit_global[d] + (p - digit_block_off[d]);
+        keys_out[g] = kk;
+        vals_out[g] = svals[p];
+    }
+}
+
+template <class K>
+int64_t sort_workspace_bytes_t(int64_t n) {
+    const int64_t nb = ceil_div(n > 0 ? n : 1, kRsTile);
+    Carver c(nullptr);
+    c.take<K>(n);
+    c.take<int32_t>(n);
+    c.take<int32_t>(nb * kRsBins);
+    c.bytes += scan_workspace_bytes(nb * kRsBins) + 256;
+    return c.bytes;
+}
+
+// Sorts (keys, vals) in place by bits [begin_bit, end_bit) of the key, stable.
+template <class K>
+int sort_pairs_t(K* keys, int32_t* vals, int64_t n, int begin_bit, int end_bit, void* ws,
+                 int64_t ws_bytes, hipStream_t s) {
+    VS_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX, "sort_pairs: n=%lld out of range", (long long)n);
+    VS_REQUIRE(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit,
+               "sort_pairs: bad bit range [%d, %d)", begin_bit, end_bit);
+    VS_REQUIRE(ws_bytes >= sort_workspace_bytes_t<K>(n), "sort_pairs: workspace too small");
+    if (n <= 1 || begin_bit == end_bit) return VS_OK;
+    const int nb = (int)ceil_div(n, kRsTile);
+    Carver c(ws);
+    K* kalt = c.take<K>(n);
+    int32_t* valt = c.take<int32_t>(n);
+    int32_t* hist = c.take<int32_t>((int64_t)nb * kRsBins);
+    void* scan_ws = c.take<char>(scan_workspace_bytes((int64_t)nb * kRsBins));
+    K* ki = keys;
+    int32_t* vi = vals;
+    K* ko = kalt;
+    int32_t* vo = valt;
+    int passes = 0;
+    for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
+        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, n, shift, hist, nb);
+        VS_LAUNCH_CHECK();
+        VS_TRY(exclusive_scan_i32(hist, hist, (int64_t)nb * kRsBins, nullptr, scan_ws, s));
+        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n,
+                           shift, hist, nb);
+        VS_LAUNCH_CHECK();
+        K* tk = ki; ki = ko; ko = tk;
+        int32_t* tv = vi; vi = vo; vo = tv;
+    }
+    if (passes & 1) {
+        VS_CHECK_HIP(hipMemcpyAsync(keys, ki, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
+        VS_CHECK_HIP(hipMemcpyAsync(vals, vi, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+    }
+    return VS_OK;
+}
+
+int64_t sort_workspace_bytes_u64(int64_t n) { return sort_workspace_bytes_t<uint64_t>(n); }
+int64_t sort_workspace_bytes_u32(int64_t n) { return sort_workspace_bytes_t<uint32_t>(n); }
+int sort_pairs_u64(uint64_t* k, int32_t* v, int64_t n, int b0, int b1, void* ws, int64_t wb, hipStream_t s) {
+    return sort_pairs_t<uint64_t>(k, v, n, b0, b1, ws, wb, s);
+}
+int sort_pairs_u32(uint32_t* k, int32_t* v, int64_t n, int b0, int b1, void* ws, int64_t wb, hipStream_t s) {
+    return sort_pairs_t<uint32_t>(k, v, n, b0, b1, ws, wb, s);
+}
+
+}  // namespace vs
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int64_t vs_scan_workspace_bytes(int64_t n) { return vs::scan_workspace_bytes(n); }
+
+int vs_exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* d_total,
+                          void* workspace, int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n >= 0, "vs_exclusive_scan_i32: negative n");
+    VS_REQUIRE(workspace_bytes >= vs::scan_workspace_bytes(n), "vs_exclusive_scan_i32: workspace too small");
+    return vs::exclusive_scan_i32(in, out, n, reinterpret_cast<long long*>(d_total), workspace,
+                                  vs::stream_of(stream));
+}
+
+int64_t vs_sort_workspace_bytes(int64_t n, int32_t key_bytes) {
+    return key_bytes == 4 ? vs::sort_workspace_bytes_u32(n) : vs::sort_workspace_bytes_u64(n);
+}
+
+int vs_sort_pairs_u64(uint64_t* keys, int32_t* vals, int64_t n, int32_t begin_bit, int32_t end_bit,
+                      void* workspace, int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    return vs::sort_pairs_u64(keys, vals, n, begin_bit, end_bit, workspace, workspace_bytes,
+                              vs::stream_of(stream));
+}
+
+int vs_sort_pairs_u32(uint32_t* keys, int32_t* vals, int64_t n, int32_t begin_bit, int32_t end_bit,
+                      void* workspace, int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    return vs::sort_pairs_u32(keys, vals, n, begin_bit, end_bit, workspace, workspace_bytes,
+                              vs::stream_of(stream));
+}
+
+}  // extern "C"

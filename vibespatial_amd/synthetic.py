@@ -1,0 +1,192 @@
+"""Seeded synthetic workloads (NumPy PCG64 ``default_rng``), like the
+reference's generators (src/vibespatial/testing/synthetic.py:36-326).
+
+Every polygon produced here is valid (simple rings, holes strictly inside the
+shell and pairwise disjoint), so even-odd location with exact boundary
+detection is exactly GEOS's ``within`` / ``intersects``:
+
+* star polygons: vertex angles are sorted uniform draws on [0, 2*pi), redrawn
+  for any polygon whose largest angular gap (wrap-around included) is >= pi;
+  every edge then subtends less than pi at the centre, so the ring is
+  star-shaped about the centre and simple.
+* holes fit in 0.9 x the shell's inscribed radius about the centre: one hole
+  is concentric; 2-3 holes sit on a circle of half that radius in disjoint
+  angular sectors.
+
+Workloads:
+  c1: 100k points U[0,1000]^2 (seed 0); 1000 stars (seed 1): centre
+      U[0,1000]^2, R ~ U(5,30), n ~ U{8..64}, radius R*(1+U(-0.3,0.3)).
+  c2: 10M points from a 64-component isotropic Gaussian mixture (means
+      U[0,1e5]^2, sigma ~ U(200,3000), equal weights; seed 2); 100k stars
+      (seed 3): centre U[0,1e5]^2, R ~ U(50,300), vertices
+      clip(round(LogNormal(ln 24, 1.0)), 4, 10000), holes ~ U{0..3} with
+      radius U(0.1,0.4)*R (shrunk to fit), hole vertices U{4..16}.
+Sizes can be scaled down with n_points / n_polys for tests.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .geometry import PointArray, PolygonArray
+
+__all__ = ["star_polygons", "c1", "c2", "gmm_points", "Workload"]
+
+
+class Workload:
+    def __init__(self, name, points, polygons, predicate, **meta):
+        self.name = name
+        self.points = points
+        self.polygons = polygons
+        self.predicate = predicate
+        self.meta = meta
+
+
+def _sorted_angles(rng, counts: np.ndarray) -> np.ndarray:
+    """Concatenated per-polygon sorted angles; polygons whose largest angular
+    gap (wrap-around included) is >= pi are redrawn until none is left."""
+    counts = np.asarray(counts, dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    ang = np.empty(int(counts.sum()))
+    todo = np.arange(len(counts))
+    while len(todo):
+        c = counts[todo]
+        seg = np.repeat(np.arange(len(todo)), c)
+        a = rng.uniform(0.0, 2 * np.pi, int(c.sum()))
+        a = a[np.lexsort((a, seg))]
+        o = np.concatenate([[0], np.cumsum(c)[:-1]])
+        last = o + c - 1
+        gaps = np.empty_like(a)
+        gaps[:-1] = a[1:] - a[:-1]
+        gaps[last] = a[o] + 2 * np.pi - a[last]
+        ok = np.maximum.reduceat(gaps, o) < np.pi
+        dst = np.repeat(off[todo], c) + (np.arange(len(a)) - np.repeat(o, c))
+        keep = np.repeat(ok, c)
+        ang[dst[keep]] = a[keep]
+        todo = todo[~ok]
+    return ang
+
+
+def _inscribed_radius(cx, cy, x, y, counts):
+    """Per ring: min distance from its centre to its (closed) edges."""
+    off = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    seg = np.repeat(np.arange(len(counts)), counts)
+    nxt = np.arange(len(x)) + 1
+    last = off + counts - 1
+    nxt[last] = off
+    ax, ay = x - cx[seg], y - cy[seg]
+    bx, by = x[nxt] - cx[seg], y[nxt] - cy[seg]
+    dx, dy = bx - ax, by - ay
+    L2 = dx * dx + dy * dy
+    t = np.clip(-(ax * dx + ay * dy) / np.where(L2 > 0, L2, 1.0), 0.0, 1.0)
+    px, py = ax + t * dx, ay + t * dy
+    d = np.sqrt(px * px + py * py)
+    return np.minimum.reduceat(d, off)
+
+
+def star_rings(rng, cx, cy, base_r, counts, jitter=0.3):
+    """Vertices of star rings (not closed), concatenated; counts per ring."""
+    counts = np.asarray(counts, dtype=np.int64)
+    ang = _sorted_angles(rng, counts)
+    seg = np.repeat(np.arange(len(counts)), counts)
+    rad = base_r[seg] * (1.0 + rng.uniform(-jitter, jitter, len(ang)))
+    return cx[seg] + rad * np.cos(ang), cy[seg] + rad * np.sin(ang)
+
+
+def _assemble(rings_x, rings_y, ring_counts, rings_per_poly, reverse_mask=None) -> PolygonArray:
+    """Close each ring and build offsets (one part per polygon)."""
+    ring_counts = np.asarray(ring_counts, dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(ring_counts)])
+    nring = len(ring_counts)
+    closed_counts = ring_counts + 1
+    coff = np.concatenate([[0], np.cumsum(closed_counts)])
+    X = np.empty(int(coff[-1]))
+    Y = np.empty(int(coff[-1]))
+    seg = np.repeat(np.arange(nring), ring_counts)
+    local = np.arange(int(off[-1])) - off[seg]
+    if reverse_mask is not None:
+        rev = reverse_mask[seg]
+        local = np.where(rev, ring_counts[seg] - 1 - local, local)
+    X[coff[seg] + local] = rings_x
+    Y[coff[seg] + local] = rings_y
+    # closing vertex = first vertex of the (possibly reversed) ring
+    X[coff[1:] - 1] = X[coff[:-1]]
+    Y[coff[1:] - 1] = Y[coff[:-1]]
+    part_offsets = np.concatenate([[0], np.cumsum(rings_per_poly)])
+    geom_offsets = np.arange(len(rings_per_poly) + 1)
+    return PolygonArray(X, Y, coff, part_offsets, geom_offsets)
+
+
+def star_polygons(rng, centres_x, centres_y, R, n_vertices, holes=None, hole_vertices=None,
+                  hole_frac=None, jitter=0.3) -> PolygonArray:
+    """Star polygons with optional holes; returns a PolygonArray (shell CCW, holes CW)."""
+    m = len(R)
+    n_vertices = np.asarray(n_vertices, dtype=np.int64)
+    sx, sy = star_rings(rng, centres_x, centres_y, R, n_vertices, jitter)
+    if holes is None:
+        return _assemble(sx, sy, n_vertices, np.ones(m, dtype=np.int64))
+    holes = np.asarray(holes, dtype=np.int64)
+    r_in = _inscribed_radius(centres_x, centres_y, sx, sy, n_vertices)
+    hole_poly = np.repeat(np.arange(m), holes)
+    hk = np.concatenate([np.arange(h) for h in holes]) if len(hole_poly) else np.empty(0, np.int64)
+    hcount = holes[hole_poly]
+    phi0 = rng.uniform(0, 2 * np.pi, m)[hole_poly]
+    d = np.where(hcount == 1, 0.0, 0.5 * r_in[hole_poly])
+    fit = np.where(hcount == 1, 0.9 * r_in[hole_poly],
+                   0.9 * np.minimum(0.5 * r_in[hole_poly], d * np.sin(np.pi / np.maximum(hcount, 1))))
+    want = hole_frac * R[hole_poly]
+    base = np.minimum(want, fit / (1.0 + jitter))
+    ang = phi0 + 2 * np.pi * hk / np.maximum(hcount, 1)
+    hcx = centres_x[hole_poly] + d * np.cos(ang)
+    hcy = centres_y[hole_poly] + d * np.sin(ang)
+    hv = np.asarray(hole_vertices, dtype=np.int64)
+    hx, hy = star_rings(rng, hcx, hcy, base, hv, jitter)
+    # interleave rings: polygon i = shell i then its holes
+    ring_poly = np.concatenate([np.arange(m), hole_poly])
+    ring_is_hole = np.concatenate([np.zeros(m, bool), np.ones(len(hole_poly), bool)])
+    order = np.lexsort((ring_is_hole, ring_poly))
+    counts_all = np.concatenate([n_vertices, hv])
+    xs_rings = np.split(np.concatenate([sx, hx]), np.cumsum(counts_all)[:-1])
+    ys_rings = np.split(np.concatenate([sy, hy]), np.cumsum(counts_all)[:-1])
+    rx = np.concatenate([xs_rings[i] for i in order]) if len(order) else np.empty(0)
+    ry = np.concatenate([ys_rings[i] for i in order]) if len(order) else np.empty(0)
+    return _assemble(rx, ry, counts_all[order], 1 + holes, reverse_mask=ring_is_hole[order])
+
+
+def c1(n_points: int = 100_000, n_polys: int = 1_000) -> Workload:
+    rp = np.random.default_rng(0)
+    px = rp.uniform(0, 1000, n_points)
+    py = rp.uniform(0, 1000, n_points)
+    rg = np.random.default_rng(1)
+    cx = rg.uniform(0, 1000, n_polys)
+    cy = rg.uniform(0, 1000, n_polys)
+    R = rg.uniform(5, 30, n_polys)
+    nv = rg.integers(8, 65, n_polys)
+    polys = star_polygons(rg, cx, cy, R, nv)
+    return Workload("c1", PointArray(px, py), polys, "within")
+
+
+def gmm_points(rng, n, n_comp=64, extent=1e5, sigma=(200.0, 3000.0)):
+    mx = rng.uniform(0, extent, n_comp)
+    my = rng.uniform(0, extent, n_comp)
+    sg = rng.uniform(sigma[0], sigma[1], n_comp)
+    comp = rng.integers(0, n_comp, n)
+    x = mx[comp] + sg[comp] * rng.standard_normal(n)
+    y = my[comp] + sg[comp] * rng.standard_normal(n)
+    return x, y
+
+
+def c2(n_points: int = 10_000_000, n_polys: int = 100_000, extent: float = 1e5) -> Workload:
+    rp = np.random.default_rng(2)
+    x, y = gmm_points(rp, n_points, extent=extent)
+    rg = np.random.default_rng(3)
+    cx = rg.uniform(0, extent, n_polys)
+    cy = rg.uniform(0, extent, n_polys)
+    R = rg.uniform(50, 300, n_polys)
+    nv = np.clip(np.round(rg.lognormal(np.log(24.0), 1.0, n_polys)), 4, 10_000).astype(np.int64)
+    holes = rg.integers(0, 4, n_polys)
+    n_holes = int(holes.sum())
+    hole_vertices = rg.integers(4, 17, n_holes)
+    hole_frac = rg.uniform(0.1, 0.4, n_holes)
+    polys = star_polygons(rg, cx, cy, R, nv, holes=holes, hole_vertices=hole_vertices, hole_frac=hole_frac)
+    return Workload("c2", PointArray(x, y), polys, "within")
