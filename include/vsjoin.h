@@ -120,11 +120,14 @@ int vs_index_build(const double* bounds, int64_t n_items, const vs_tree_layout* 
  * Joins and queries
  * --------------------------------------------------------------------- */
 
-int64_t vs_sjoin_points_polygons_workspace_bytes(int64_t n_points, int64_t capacity);
+int64_t vs_sjoin_points_polygons_workspace_bytes(int64_t n_points, int64_t n_polys, int64_t n_coords,
+                                                 int64_t capacity);
 
 /* sjoin(points, polygons, predicate): all (point, polygon) pairs with
  * predicate(point, polygon) true, lexsorted, as int64 (left_index, right_index).
- * The polygon tree comes from vs_index_build over the polygons' bounds.
+ * The polygon tree comes from vs_index_build over the polygons' bounds;
+ * n_parts / n_rings / n_coords are the lengths of part_offsets - 1,
+ * ring_offsets - 1 and x (host values, used to size the refine layout).
  * predicate: VS_PRED_WITHIN | VS_PRED_INTERSECTS | VS_PRED_COVERED_BY | VS_PRED_TOUCHES.
  * Exact (no tolerance): boundary = the point lies on an edge.
  * Replaces query_spatial_index's point-x-polygon route (spatial/query.py:247-1042):
@@ -134,10 +137,11 @@ int64_t vs_sjoin_points_polygons_workspace_bytes(int64_t n_points, int64_t capac
  * (spatial/query_utils.py:867-868, spatial/query.py:143-153). */
 int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_points, const double* x,
                              const double* y, const int32_t* geom_offsets, const int32_t* part_offsets,
-                             const int32_t* ring_offsets, int64_t n_polys, const vs_tree_layout* layout,
-                             const double* node_boxes, const int32_t* item_ids, int32_t predicate,
-                             int64_t capacity, int64_t* out_left, int64_t* out_right, int64_t* d_result,
-                             void* workspace, int64_t workspace_bytes, void* stream);
+                             const int32_t* ring_offsets, int64_t n_polys, int64_t n_parts, int64_t n_rings,
+                             int64_t n_coords, const vs_tree_layout* layout, const double* node_boxes,
+                             const int32_t* item_ids, int32_t predicate, int64_t capacity, int64_t* out_left,
+                             int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
+                             void* stream);
 
 int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity);
 

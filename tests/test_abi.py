@@ -127,6 +127,6 @@ def test_workspace_queries_are_positive_and_monotone():
         f = getattr(L, fn)
         a, b = f(1000), f(10_000_000)
         assert 0 < a <= b, fn
-    assert 0 < L.vs_sjoin_points_polygons_workspace_bytes(1000, 10) < L.vs_sjoin_points_polygons_workspace_bytes(
-        1000, 10**6)
+    assert 0 < L.vs_sjoin_points_polygons_workspace_bytes(1000, 10, 100, 10) < \
+        L.vs_sjoin_points_polygons_workspace_bytes(1000, 10, 100, 10**6)
     assert L.vs_sort_workspace_bytes(10**6, 4) < L.vs_sort_workspace_bytes(10**6, 8)

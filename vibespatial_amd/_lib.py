@@ -62,10 +62,11 @@ SIGNATURES = {
     "vs_tree_plan": (ctypes.c_int, [_i64, _i32, _LP]),
     "vs_index_workspace_bytes": (_i64, [_i64]),
     "vs_index_build": (ctypes.c_int, [_p, _i64, _LP, _p, _p, _p, _p, _i64, _p]),
-    "vs_sjoin_points_polygons_workspace_bytes": (_i64, [_i64, _i64]),
+    "vs_sjoin_points_polygons_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
     "vs_sjoin_points_polygons": (
         ctypes.c_int,
-        [_p, _p, _i64, _p, _p, _p, _p, _p, _i64, _LP, _p, _p, _i32, _i64, _p, _p, _p, _p, _i64, _p],
+        [_p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _LP, _p, _p, _i32, _i64, _p, _p, _p, _p, _i64,
+         _p],
     ),
     "vs_query_boxes_workspace_bytes": (_i64, [_i64, _i64]),
     "vs_query_boxes": (ctypes.c_int, [_p, _i64, _LP, _p, _p, _i64, _p, _p, _p, _p, _i64, _p]),

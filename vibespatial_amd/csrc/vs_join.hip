@@ -58,13 +58,20 @@ struct TreeView {
     int64_t n_top;
 };
 
+// Refine layout of the polygons: coordinates in their original order, one
+// flag byte per coordinate (bit0: first coordinate of a ring, so no edge ends
+// there; bit1: first coordinate of a part, where the even-odd parity of the
+// previous part is settled) and, per item slot of the tree (Hilbert order), the
+// row's coordinate range.  One load of item_range replaces the
+// geom -> part -> ring offset chain.
 struct PolyView {
     const double* x;
     const double* y;
-    const int32_t* geom;
-    const int32_t* part;
-    const int32_t* ring;
+    const uint8_t* flags;
+    const int2* item_range;
 };
+
+constexpr int kOutBuf = 256;  // per-wave LDS buffer of refined pairs
 
 struct JoinCtl {
     unsigned long long staged;  // pairs appended (may exceed capacity)
@@ -96,47 +103,63 @@ __device__ __forceinline__ bool pred_of_location(int loc) {
     return loc != 0;  // intersects, covered_by, covers
 }
 
-// Wave-cooperative exact location of each active lane's point in polygonal
-// row `row`; returns the predicate result per lane.
+// Wave-cooperative exact location of each active lane's point in the polygon
+// whose coordinates are [c0, c1); returns the predicate result per lane.
+// The wave loads 64 coordinates at a time (coalesced), forms the edge ending at
+// each lane's coordinate, and keeps only edges that can change the answer of
+// some active lane: their y-range meets the region [rx0,rx1]x[ry0,ry1] holding
+// every active point and they are not entirely left of it (an edge left of p
+// neither crosses p's rightward ray nor contains p).  The kept edges are
+// broadcast one by one with readlane and every active lane runs the exact
+// crossing / boundary step on them.
 template <int PRED>
-__device__ bool refine_pip_wave(double px, double py, bool active, int32_t row, const PolyView& P) {
+__device__ bool refine_pip_wave(double px, double py, bool active, int32_t c0, int32_t c1, double rx0,
+                                double ry0, double ry1, const PolyView& P) {
     const int lane = lane_id();
     int loc = 0;
     bool done = !active;
-    const int32_t p0 = P.geom[row], p1 = P.geom[row + 1];
-    for (int32_t pp = p0; pp < p1; ++pp) {
-        int parity = 0;
-        const int32_t r0 = P.part[pp], r1 = P.part[pp + 1];
-        for (int32_t rr = r0; rr < r1; ++rr) {
-            const int32_t c0 = P.ring[rr], c1 = P.ring[rr + 1];
-            double ax = 0.0, ay = 0.0;
-            for (int32_t base = c0; base < c1; base += 64) {
-                const int32_t c = base + lane;
-                const double xv = c < c1 ? P.x[c] : 0.0;
-                const double yv = c < c1 ? P.y[c] : 0.0;
-                const int jn = (c1 - base) < 64 ? (c1 - base) : 64;
-                for (int jj = 0; jj < jn; ++jj) {
-                    const double bx = readlane_f64(xv, jj);
-                    const double by = readlane_f64(yv, jj);
-                    if (base + jj > c0 && !done) {
-                        if (edge_step_f64(px, py, ax, ay, bx, by, parity)) {
-                            loc = 1;
-                            done = true;
-                        }
-                    }
-                    ax = bx;
-                    ay = by;
-                }
-                if (__ballot(!done) == 0) goto finished;
-            }
+    int parity = 0;
+    double cx = 0.0, cy = 0.0;  // coordinate base-1 for the chunk's lane 0
+    for (int32_t base = c0; base < c1; base += 64) {
+        const int32_t c = base + lane;
+        const bool inr = c < c1;
+        const double bx = inr ? P.x[c] : 0.0;
+        const double by = inr ? P.y[c] : 0.0;
+        const int f = inr ? (int)P.flags[c] : 0;
+        double ax = __shfl_up(bx, 1, 64);
+        double ay = __shfl_up(by, 1, 64);
+        if (lane == 0) {
+            ax = cx;
+            ay = cy;
         }
-        if (!done && parity) {
-            loc = 2;
-            done = true;
+        const bool rel = inr && !(f & 1) && fmax(ay, by) >= ry0 && fmin(ay, by) <= ry1 && fmax(ax, bx) >= rx0;
+        const uint64_t m = __ballot(rel);
+        const uint64_t pm = __ballot(inr && (f & 2));
+        cx = readlane_f64(bx, 63);
+        cy = readlane_f64(by, 63);
+        uint64_t work = m | pm;
+        while (work) {
+            const int s = __ffsll((unsigned long long)work) - 1;
+            work &= work - 1;
+            if ((pm >> s) & 1) {  // a new part starts: settle the previous one
+                if (!done && parity) {
+                    loc = 2;
+                    done = true;
+                }
+                parity = 0;
+            }
+            if ((m >> s) & 1) {
+                const double eax = readlane_f64(ax, s), eay = readlane_f64(ay, s);
+                const double ebx = readlane_f64(bx, s), eby = readlane_f64(by, s);
+                if (!done && edge_step_f64(px, py, eax, eay, ebx, eby, parity)) {
+                    loc = 1;
+                    done = true;
+                }
+            }
         }
         if (__ballot(!done) == 0) break;
     }
-finished:
+    if (!done && parity) loc = 2;
     return active && pred_of_location<PRED>(loc);
 }
 
@@ -150,11 +173,29 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
     int64_t capacity, JoinCtl* __restrict__ ctl, int32_t* __restrict__ counts) {
     __shared__ double4 top[kTopMax];
     __shared__ long long stack_s[kJoinWaves][kStackMax];
+    __shared__ int2 outbuf_s[kJoinWaves][kOutBuf];
     for (int i = threadIdx.x; i < T.n_top; i += kJoinThreads) top[i] = T.box[i];
     __syncthreads();
 
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     long long* stack = stack_s[w];
+    int2* outbuf = outbuf_s[w];
+    int nbuf = 0;                          // wave-uniform fill of outbuf
+    unsigned long long st_cand = 0, st_cverts = 0;
+    // Appends the wave's pairs in outbuf to the stage with one atomic.
+    auto flush = [&]() {
+        if (nbuf == 0) return;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&ctl->staged, (unsigned long long)nbuf);
+        base = __shfl(base, 0, 64);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int i = lane; i < nbuf; i += 64) {
+            const unsigned long long slot = base + (unsigned long long)i;
+            if (slot < (unsigned long long)capacity) stage[slot] = outbuf[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        nbuf = 0;
+    };
     const int64_t W = (n + 63) >> 6;
     const int home = blockIdx.x & 7;
     const int last = T.n_levels - 1;
@@ -222,26 +263,25 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
                         const int32_t row = T.item_ids[first + s];
                         bool res;
                         if (MODE == kModePip) {
-                            if (lane == 0) {
-                                const unsigned long long h = (unsigned long long)__popcll(hm);
-                                const int32_t nv = P.ring[P.part[P.geom[row + 1]]] - P.ring[P.part[P.geom[row]]];
-                                atomicAdd(&ctl->stat[0], h);
-                                atomicAdd(&ctl->stat[1], h * (unsigned long long)nv);
-                            }
-                            res = refine_pip_wave<PRED>(x0, y0, hit, row, P);
+                            const int2 rng = P.item_range[first + s];
+                            const unsigned long long h = (unsigned long long)__popcll(hm);
+                            st_cand += h;
+                            st_cverts += h * (unsigned long long)(rng.y - rng.x);
+                            // every active point lies in (wave box) x (item box)
+                            res = refine_pip_wave<PRED>(x0, y0, hit, rng.x, rng.y, fmax(ux0, bx0), fmax(uy0, by0),
+                                                        fmin(uy1, by1), P);
                         } else {
                             res = hit;
                         }
                         const uint64_t rm = __ballot(res);
                         if (rm) {
-                            unsigned long long base = 0;
-                            if (lane == 0) base = atomicAdd(&ctl->staged, (unsigned long long)__popcll(rm));
-                            base = __shfl(base, 0, 64);
+                            const int k = __popcll(rm);
+                            if (nbuf + k > kOutBuf) flush();
                             if (res) {
-                                const unsigned long long slot = base + __popcll(rm & lt);
-                                if (slot < (unsigned long long)capacity) stage[slot] = make_int2(orig, row);
+                                outbuf[nbuf + __popcll(rm & lt)] = make_int2(orig, row);
                                 ++cnt;
                             }
+                            nbuf += k;
                         }
                     }
                 } else {
@@ -256,6 +296,11 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
             }
         }
         if (inr) counts[orig] = cnt;
+    }
+    flush();
+    if (MODE == kModePip && lane == 0 && st_cand) {
+        atomicAdd(&ctl->stat[0], st_cand);
+        atomicAdd(&ctl->stat[1], st_cverts);
     }
 }
 
@@ -312,6 +357,36 @@ __global__ void publish_result_kernel(const JoinCtl* __restrict__ ctl, int64_t* 
     }
 }
 
+// Refine-layout preparation (per join: O(V + R + P + M), tiny next to the join).
+__global__ void fill_u8_kernel(uint8_t* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void ring_start_flags_kernel(const int32_t* __restrict__ ring, int64_t n_rings,
+                                        uint8_t* __restrict__ flags) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r < n_rings && ring[r + 1] > ring[r]) flags[ring[r]] = 1;
+}
+
+__global__ void part_start_flags_kernel(const int32_t* __restrict__ part, const int32_t* __restrict__ ring,
+                                        int64_t n_parts, uint8_t* __restrict__ flags) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n_parts && part[p + 1] > part[p]) {
+        const int32_t c = ring[part[p]];
+        if (ring[part[p + 1]] > c) flags[c] = 3;
+    }
+}
+
+__global__ void item_range_kernel(const int32_t* __restrict__ item_ids, int64_t n_items,
+                                  const int32_t* __restrict__ geom, const int32_t* __restrict__ part,
+                                  const int32_t* __restrict__ ring, int2* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n_items) return;
+    const int32_t row = item_ids[k];
+    out[k] = make_int2(ring[part[geom[row]]], ring[part[geom[row + 1]]]);
+}
+
 // ---------------------------------------------------------------------------
 // Host orchestration
 // ---------------------------------------------------------------------------
@@ -347,6 +422,8 @@ int check_layout(const vs_tree_layout* L) {
 }
 
 struct JoinWs {
+    uint8_t* flags;
+    int2* item_range;
     int32_t* perm;
     double* sx;
     double* sy;
@@ -359,9 +436,12 @@ struct JoinWs {
     JoinCtl* ctl;
 };
 
-JoinWs carve_join_ws(void* base, int64_t nq, int64_t capacity, bool sorted_points, int64_t* bytes) {
+JoinWs carve_join_ws(void* base, int64_t nq, int64_t n_items, int64_t n_coords, int64_t capacity,
+                     bool sorted_points, int64_t* bytes) {
     Carver c(base);
     JoinWs J;
+    J.flags = sorted_points ? c.take<uint8_t>(n_coords) : nullptr;
+    J.item_range = sorted_points ? c.take<int2>(n_items) : nullptr;
     J.perm = c.take<int32_t>(nq);
     if (sorted_points) {
         J.sx = c.take<double>(nq);
@@ -384,9 +464,8 @@ JoinWs carve_join_ws(void* base, int64_t nq, int64_t capacity, bool sorted_point
 int grid_for(int64_t n_waves) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
-            cus = p.multiProcessorCount;
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
     }
     const int64_t want = ceil_div(n_waves, kJoinWaves);
     int64_t g = std::min<int64_t>(want, (int64_t)cus * 8);
@@ -478,38 +557,62 @@ int vs_last_kernel_ms(float* ms) {
     return VS_OK;
 }
 
-int64_t vs_sjoin_points_polygons_workspace_bytes(int64_t n_points, int64_t capacity) {
+int64_t vs_sjoin_points_polygons_workspace_bytes(int64_t n_points, int64_t n_polys, int64_t n_coords,
+                                                 int64_t capacity) {
     int64_t b = 0;
-    vs::carve_join_ws(nullptr, n_points, capacity, true, &b);
+    vs::carve_join_ws(nullptr, n_points, n_polys, n_coords, capacity, true, &b);
     return b;
 }
 
 int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_points, const double* x,
                              const double* y, const int32_t* geom_offsets, const int32_t* part_offsets,
-                             const int32_t* ring_offsets, int64_t n_polys, const vs_tree_layout* layout,
-                             const double* node_boxes, const int32_t* item_ids, int32_t predicate,
-                             int64_t capacity, int64_t* out_left, int64_t* out_right, int64_t* d_result,
-                             void* workspace, int64_t workspace_bytes, void* stream) {
+                             const int32_t* ring_offsets, int64_t n_polys, int64_t n_parts, int64_t n_rings,
+                             int64_t n_coords, const vs_tree_layout* layout, const double* node_boxes,
+                             const int32_t* item_ids, int32_t predicate, int64_t capacity, int64_t* out_left,
+                             int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
+                             void* stream) {
     vs::clear_error();
     VS_TRY(vs::check_layout(layout));
     VS_REQUIRE(layout->n_items == n_polys, "vs_sjoin_points_polygons: layout has %lld items, %lld polygons",
                (long long)layout->n_items, (long long)n_polys);
     VS_REQUIRE(n_points >= 0 && n_points < (int64_t)INT32_MAX, "vs_sjoin_points_polygons: n_points out of range");
+    VS_REQUIRE(n_parts >= 0 && n_rings >= 0 && n_coords >= 0 && n_coords < (int64_t)INT32_MAX,
+               "vs_sjoin_points_polygons: bad part/ring/coordinate counts");
     VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_sjoin_points_polygons: capacity out of range");
     VS_REQUIRE(predicate == VS_PRED_WITHIN || predicate == VS_PRED_INTERSECTS ||
                    predicate == VS_PRED_COVERED_BY || predicate == VS_PRED_TOUCHES,
                "vs_sjoin_points_polygons: unsupported predicate %d", predicate);
     VS_REQUIRE(d_result != nullptr, "vs_sjoin_points_polygons: null result buffer");
     int64_t need = 0;
-    const vs::JoinWs J = vs::carve_join_ws(workspace, n_points, capacity, true, &need);
+    const vs::JoinWs J = vs::carve_join_ws(workspace, n_points, n_polys, n_coords, capacity, true, &need);
     VS_REQUIRE(workspace_bytes >= need, "vs_sjoin_points_polygons: workspace %lld < %lld bytes",
                (long long)workspace_bytes, (long long)need);
     hipStream_t s = vs::stream_of(stream);
     hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl);
     VS_LAUNCH_CHECK();
+    if (n_coords > 0) {
+        hipLaunchKernelGGL(vs::fill_u8_kernel, dim3((unsigned)vs::ceil_div(n_coords, 256)), dim3(256), 0, s, J.flags,
+                           n_coords);
+        VS_LAUNCH_CHECK();
+    }
+    if (n_rings > 0) {
+        hipLaunchKernelGGL(vs::ring_start_flags_kernel, dim3((unsigned)vs::ceil_div(n_rings, 256)), dim3(256), 0, s,
+                           ring_offsets, n_rings, J.flags);
+        VS_LAUNCH_CHECK();
+    }
+    if (n_parts > 0) {
+        hipLaunchKernelGGL(vs::part_start_flags_kernel, dim3((unsigned)vs::ceil_div(n_parts, 256)), dim3(256), 0, s,
+                           part_offsets, ring_offsets, n_parts, J.flags);
+        VS_LAUNCH_CHECK();
+    }
+    if (n_polys > 0) {
+        hipLaunchKernelGGL(vs::item_range_kernel, dim3((unsigned)vs::ceil_div(n_polys, 256)), dim3(256), 0, s, item_ids,
+                           n_polys, geom_offsets, part_offsets, ring_offsets, J.item_range);
+        VS_LAUNCH_CHECK();
+    }
     VS_TRY(vs::point_order(px, py, n_points, 16, J.perm, J.sx, J.sy, J.order_ws, s));
     const vs::TreeView T = vs::make_tree_view(layout, node_boxes, item_ids);
-    const vs::PolyView P = {x, y, geom_offsets, part_offsets, ring_offsets};
+    const vs::PolyView P = {x, y, J.flags, J.item_range};
     if (n_points > 0 && n_polys > 0)
         VS_TRY(vs::launch_traverse<vs::kModePip>(predicate, J.sx, J.sy, nullptr, J.perm, n_points, T, P, J,
                                                  capacity, s));
@@ -523,7 +626,7 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
 
 int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity) {
     int64_t b = 0;
-    vs::carve_join_ws(nullptr, n_queries, capacity, false, &b);
+    vs::carve_join_ws(nullptr, n_queries, 0, 0, capacity, false, &b);
     return b;
 }
 
@@ -537,14 +640,14 @@ int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_l
     VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_query_boxes: capacity out of range");
     VS_REQUIRE(d_result != nullptr, "vs_query_boxes: null result buffer");
     int64_t need = 0;
-    const vs::JoinWs J = vs::carve_join_ws(workspace, n_queries, capacity, false, &need);
+    const vs::JoinWs J = vs::carve_join_ws(workspace, n_queries, 0, 0, capacity, false, &need);
     VS_REQUIRE(workspace_bytes >= need, "vs_query_boxes: workspace %lld < %lld bytes",
                (long long)workspace_bytes, (long long)need);
     hipStream_t s = vs::stream_of(stream);
     hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl);
     VS_LAUNCH_CHECK();
     const vs::TreeView T = vs::make_tree_view(layout, node_boxes, item_ids);
-    const vs::PolyView P = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    const vs::PolyView P = {nullptr, nullptr, nullptr, nullptr};
     if (n_queries > 0 && layout->n_items > 0)
         VS_TRY(vs::launch_traverse<vs::kModeBox>(VS_PRED_INTERSECTS, nullptr, nullptr,
                                                  reinterpret_cast<const double4*>(query_boxes), nullptr,

@@ -78,6 +78,18 @@ class DevicePolygons:
     def __len__(self) -> int:
         return int(self.geom_offsets.numel()) - 1
 
+    @property
+    def n_parts(self) -> int:
+        return int(self.part_offsets.numel()) - 1
+
+    @property
+    def n_rings(self) -> int:
+        return int(self.ring_offsets.numel()) - 1
+
+    @property
+    def n_coords(self) -> int:
+        return int(self.x.numel())
+
     @classmethod
     def from_host(cls, pa: PolygonArray) -> "DevicePolygons":
         return cls(

@@ -114,13 +114,14 @@ def sjoin_points_polygons(Q: DevicePoints, P: DevicePolygons, tree: DeviceTree, 
     cap = int(capacity) if capacity is not None else max(1024, 2 * n)
     retries = 0
     while True:
-        ws = _ws(L.vs_sjoin_points_polygons_workspace_bytes(n, cap), dev)
+        ws = _ws(L.vs_sjoin_points_polygons_workspace_bytes(n, len(P), P.n_coords, cap), dev)
         out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
         out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
         res = torch.zeros(4, dtype=torch.int64, device=dev)
         check(
             L.vs_sjoin_points_polygons(
-                ptr(Q.x), ptr(Q.y), n, *P.args(), len(P), ctypes.byref(tree.layout), ptr(tree.node_boxes),
+                ptr(Q.x), ptr(Q.y), n, *P.args(), len(P), P.n_parts, P.n_rings, P.n_coords,
+                ctypes.byref(tree.layout), ptr(tree.node_boxes),
                 ptr(tree.item_ids), PRED[predicate], cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws),
                 ws.numel(), stream(),
             ),
