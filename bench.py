@@ -1,4 +1,5 @@
-"""Benchmark: sjoin(points, polygons, predicate='within') refined pairs/sec.
+"""Benchmark: sjoin(points, polygons, predicate='within') refined pairs/sec,
+learned + FP32 form vs the plain fp64 form.
 
 Workload (one GPU): 10,000,000 points from a 64-component Gaussian mixture
 over [0, 1e5]^2 and 100,000 star polygons with 0-3 holes and lognormal vertex
@@ -10,12 +11,16 @@ the pairs.  Input H2D and output D2H are outside the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the traversal +
-refine kernel): algorithmic bytes per launch / its average duration measured
-with HIP events on the launch stream.  Algorithmic bytes per launch:
+Both forms run the same step boundaries; `value` is the learned + FP32 form,
+`config.plain_fp64` the plain form, `config.learned_vs_plain` their ratio.
+Prints one JSON line.  `roofline` is for the learned form's dominant kernel
+(learned_join_kernel): algorithmic bytes per launch / its average duration
+measured with HIP events on the launch stream.  Algorithmic bytes per launch:
+    16 N + sum_q (16 V_q + 32) + sum_{(p,q) in C} (8 + 8 V_q) + 32 E + 8 K
+and for the plain form's traversal kernel
     16 N + sum_q (16 V_q + 32) + sum_{(p,q) in C} (16 + 16 V_q) + 8 K
 (N points, V_q vertices of polygon q over all rings, C the exact inclusive
-bbox candidate pairs, K the refined pairs).  `cpu_baseline` times the CPU
+bbox candidate pairs, E escalated edges, K the refined pairs).  `cpu_baseline` times the CPU
 oracle's join (packed tree + exact location, OpenMP) on a bounded sample of
 the same workload on this host.
 """
@@ -48,7 +53,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-points", type=int, default=10_000_000)
     ap.add_argument("--n-polys", type=int, default=100_000)
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000,
+    ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="points in the CPU-baseline sample (0 disables it)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -75,41 +80,52 @@ def main(argv=None):
     P = DevicePolygons.from_host(w.polygons)
     torch.cuda.synchronize()
     L = _lib.lib()
-
-    def step(cap):
-        tree = J.build_tree(J.polygon_bounds(P))
-        return J.sjoin_points_polygons(Q, P, tree, "within", capacity=cap)
-
-    _log("warmup")
-    cap = None
-    st = None
-    for _ in range(max(args.warmup, 1)):
-        _, _, st = step(cap)
-        cap = st.pairs + 1024
-    _lib.check(L.vs_set_kernel_timing(1), "vs_set_kernel_timing")
-    kernel_ms = []
-    torch.cuda.synchronize()
-    _log(f"timing {args.steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _, _, st = step(cap)  # reads the 4-word result header: synchronises the stream
-        ms = _lib.ctypes.c_float()
-        _lib.check(L.vs_last_kernel_ms(_lib.ctypes.byref(ms)), "vs_last_kernel_ms")
-        kernel_ms.append(ms.value)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    _lib.check(L.vs_set_kernel_timing(0), "vs_set_kernel_timing")
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    K = st.pairs
-    value = K / (ms_per_step * 1e-3)
-
-    # algorithmic bytes of one traversal launch
     P_h = w.polygons
     V = np.diff(P_h.ring_offsets[P_h.part_offsets[P_h.geom_offsets]])
     n_pts = len(w.points)
-    alg = 16 * n_pts + int((16 * V + 32).sum()) + 16 * st.candidates + 16 * st.candidate_vertices + 8 * K
-    k_ms = statistics.mean(kernel_ms)
-    achieved = alg / (k_ms * 1e-3) / 1e9
+    poly_bytes = int((16 * V + 32).sum())
+
+    def step_plain(cap):
+        tree = J.build_tree(J.polygon_bounds(P))
+        return J.sjoin_points_polygons(Q, P, tree, "within", capacity=cap)
+
+    def step_learned(cap):
+        return J.sjoin_points_polygons_learned(Q, P, J.polygon_bounds(P), "within", capacity=cap)
+
+    def measure(step, name):
+        _log(f"{name}: warmup")
+        cap, st = None, None
+        for _ in range(max(args.warmup, 1)):
+            _, _, st = step(cap)
+            cap = st.pairs + 1024
+        _lib.check(L.vs_set_kernel_timing(1), "vs_set_kernel_timing")
+        kernel_ms = []
+        torch.cuda.synchronize()
+        _log(f"{name}: timing {args.steps} steps")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, _, st = step(cap)  # reads the result header: synchronises the stream
+            ms = _lib.ctypes.c_float()
+            _lib.check(L.vs_last_kernel_ms(_lib.ctypes.byref(ms)), "vs_last_kernel_ms")
+            kernel_ms.append(ms.value)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _lib.check(L.vs_set_kernel_timing(0), "vs_set_kernel_timing")
+        return st, (t1 - t0) * 1e3 / args.steps, statistics.mean(kernel_ms)
+
+    st_p, ms_p, kms_p = measure(step_plain, "plain")
+    st_l, ms_l, kms_l = measure(step_learned, "learned+fp32")
+    assert st_p.pairs == st_l.pairs and st_p.candidates == st_l.candidates, "forms disagree"
+    K = st_l.pairs
+    # algorithmic bytes per launch of each form's dominant kernel
+    alg_p = 16 * n_pts + poly_bytes + 16 * st_p.candidates + 16 * st_p.candidate_vertices + 8 * K
+    alg_l = 16 * n_pts + poly_bytes + 8 * st_l.candidates + 8 * st_l.candidate_vertices + 32 * st_l.escalated_edges \
+        + 8 * K
+    ach_l = alg_l / (kms_l * 1e-3) / 1e9
+    ach_p = alg_p / (kms_p * 1e-3) / 1e9
+    value = K / (ms_l * 1e-3)
+    value_p = K / (ms_p * 1e-3)
+    ms_per_step = ms_l
 
     cpu = None
     if args.cpu_sample > 0:
@@ -143,27 +159,35 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": f"{n_pts:,} GMM points x {len(P_h):,} star polygons (0-3 holes, lognormal vertices), "
                         "sjoin predicate=within",
-            "form": "plain-fp64 (packed Hilbert tree, wave traversal, fused exact PIP)",
+            "form": "learned+fp32 (learned rank index over Hilbert-sorted points, polygon-major FP32 refine "
+                    "with exact escalation)",
             "pairs": K,
-            "bbox_candidates": st.candidates,
-            "candidate_rate_per_s": st.candidates / (ms_per_step * 1e-3),
+            "bbox_candidates": st_l.candidates,
+            "candidate_rate_per_s": st_l.candidates / (ms_l * 1e-3),
             "polygon_vertices": int(V.sum()),
+            "escalated_edges": st_l.escalated_edges,
+            "escalation_fraction": st_l.escalation_fraction,
+            "lookup_probes": st_l.lookup_probes,
+            "mlp_leaf_error": st_l.mlp_leaf_error,
+            "plain_fp64": {"value": value_p, "ms_per_step": ms_p, "kernel_ms": kms_p,
+                           "roofline_frac": ach_p / HBM_PEAK_GBS},
+            "learned_vs_plain": value / value_p,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "traverse_kernel<0,0> (tree traversal + exact PIP refine)",
-            "achieved": achieved,
+            "kernel": "learned_join_kernel<0,0> (scan of rank ranges + FP32 refine)",
+            "achieved": ach_l,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": ach_l / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel_ms": k_ms,
-            "alg_bytes_per_launch": alg,
+            "kernel_ms": kms_l,
+            "alg_bytes_per_launch": alg_l,
         },
         "cpu_baseline": cpu,
     }

@@ -189,6 +189,78 @@ int vs_lexsort_pairs(int64_t* left, int64_t* right, int64_t n_pairs, void* works
                      void* stream);
 
 /* ---------------------------------------------------------------------
+ * Learned point index and the learned + FP32 join form
+ * (no reference counterpart; see the comment at the top of vs_learned.hip)
+ * --------------------------------------------------------------------- */
+
+/* Number of leaves of the piecewise-linear stage for n points and a leaf size
+ * (<= 0: 512 positions per leaf). */
+int32_t vs_learned_leaf_count(int64_t n_points, int32_t leaf_size);
+
+/* Bytes of the index buffer (keys, permutation, sorted x/y, extent, MLP
+ * parameters, training losses, leaf table, leaf errors, meta). */
+int64_t vs_learned_index_bytes(int64_t n_points, int32_t key_bytes, int32_t n_leaves);
+
+/* Byte offsets inside the index buffer of: keys, perm, sorted x, sorted y,
+ * extent (4 doubles), MLP params (97 floats: w1[32], b1[32], w2[32], b2),
+ * losses (2 floats: before / after training), leaf keys (n_leaves + 1),
+ * leaf errors (int32 n_leaves), meta (int32[4]: [0] max MLP leaf error). */
+int vs_learned_index_layout(int64_t n_points, int32_t key_bytes, int32_t n_leaves, int64_t* offsets10);
+
+/* Builds the index over points: Hilbert keys of key_bytes*4 bits per axis over
+ * the point extent, radix-sorted; the MLP is trained on the GPU for
+ * train_iters full-batch Adam steps (lr relative to each parameter's scale);
+ * leaf table and error bounds are recorded over every key. */
+int vs_learned_index_build(const double* px, const double* py, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
+                           int32_t train_iters, float lr, void* index, int64_t index_bytes, void* stream);
+
+/* Batch lower_bound of query keys in the index's sorted keys.  lookup_mode:
+ * 0 learned (MLP leaf window + leaf error window), 1 leaf table only (binary
+ * search over all leaves, then the leaf error window), 2 plain binary search.
+ * d_stat3 (device uint64[3], accumulated): probes, MLP-window fallbacks,
+ * error-window fallbacks.  Results are exact in every mode. */
+int vs_learned_lower_bound(const void* index, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
+                           const void* query_keys, int64_t m, int32_t lookup_mode, int64_t* out, uint64_t* d_stat3,
+                           void* stream);
+
+/* Full-batch MSE loss and gradient of MLP parameters on (u, t) data, computed
+ * by the device code the trainer uses (for gradient checks). */
+int vs_mlp_loss_grad(const float* params, const float* u, const float* t, int64_t n, float* grad_out, float* loss_out,
+                     void* stream);
+
+int64_t vs_learned_query_boxes_workspace_bytes(int64_t n_boxes, int64_t capacity);
+
+/* sindex.query(boxes, predicate='intersects') on a point tree: (box, point)
+ * pairs with the point inside the box (inclusive), lexsorted, through the
+ * learned index (cover cells -> rank ranges -> scan).  d_result: int64[16],
+ * layout as for the learned join.  Replaces _query_point_tree_box_index
+ * (spatial/query_box.py:760-863). */
+int vs_learned_query_boxes(const void* index, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
+                           const double* boxes, int64_t n_boxes, int32_t lookup_mode, int64_t capacity,
+                           int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                           int64_t workspace_bytes, void* stream);
+
+int64_t vs_sjoin_points_polygons_learned_workspace_bytes(int64_t n_points, int64_t n_polys, int64_t n_coords,
+                                                         int64_t capacity);
+
+/* The learned + FP32 form of vs_sjoin_points_polygons (same pairs, bit for
+ * bit): builds the learned index over the points (32-bit keys), covers each
+ * polygon's bounds (poly_bounds, from vs_polygon_bounds) with rank ranges and
+ * refines candidates polygon-major in FP32 with escalation to the exact fp64
+ * step.  d_result (device int64[16]): [0] K, [1] staged, [2] candidates,
+ * [3] candidate vertices, [4] fp32 orientation evaluations, [5] escalated
+ * edges, [6] candidate pairs with an escalation, [7] lookup probes,
+ * [8] MLP-window fallbacks, [9] error-window fallbacks, [10] positions
+ * scanned, [11] max MLP leaf error. */
+int vs_sjoin_points_polygons_learned(const double* px, const double* py, int64_t n_points, const double* x,
+                                     const double* y, const int32_t* geom_offsets, const int32_t* part_offsets,
+                                     const int32_t* ring_offsets, int64_t n_polys, int64_t n_parts, int64_t n_rings,
+                                     int64_t n_coords, const double* poly_bounds, int32_t predicate,
+                                     int32_t train_iters, int32_t lookup_mode, int64_t capacity, int64_t* out_left,
+                                     int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
+                                     void* stream);
+
+/* ---------------------------------------------------------------------
  * Primitives
  * --------------------------------------------------------------------- */
 

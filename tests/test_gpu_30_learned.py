@@ -1,0 +1,206 @@
+"""GPU tests of the learned point index and the learned + FP32 join form."""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import mlp_ref
+from vibespatial_amd import join as J
+from vibespatial_amd import synthetic as S
+from vibespatial_amd.device import DevicePoints, DevicePolygons
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, len(os.sched_getaffinity(0)))
+
+
+def _dp(px, py):
+    return DevicePoints(torch.from_numpy(np.ascontiguousarray(px, dtype=np.float64)).cuda(),
+                        torch.from_numpy(np.ascontiguousarray(py, dtype=np.float64)).cuda())
+
+
+def _zipf_points(n, rng, n_clusters=256, extent=1e5, sigma=500.0):
+    w = 1.0 / np.arange(1, n_clusters + 1) ** 1.1
+    w /= w.sum()
+    cx, cy = rng.uniform(0, extent, n_clusters), rng.uniform(0, extent, n_clusters)
+    c = rng.choice(n_clusters, n, p=w)
+    return cx[c] + sigma * rng.standard_normal(n), cy[c] + sigma * rng.standard_normal(n)
+
+
+# ---------------------------------------------------------------------------
+# MLP forward / backward kernels vs the CPU oracle
+# ---------------------------------------------------------------------------
+def test_mlp_gradients_match_cpu_oracle():
+    rng = np.random.default_rng(0)
+    p = np.concatenate([rng.uniform(0.5, 2.0, 32), -rng.uniform(0, 1, 32), rng.normal(0, 0.3, 32), [0.1]])
+    u = np.sort(rng.uniform(0, 1, 3000))
+    t = np.sqrt(u) + 0.01 * rng.standard_normal(3000)
+    loss, g = J.mlp_loss_grad(torch.tensor(p, dtype=torch.float32).cuda(), torch.tensor(u, dtype=torch.float32).cuda(),
+                              torch.tensor(t, dtype=torch.float32).cuda())
+    p32 = p.astype(np.float32).astype(np.float64)
+    u32 = u.astype(np.float32).astype(np.float64)
+    t32 = t.astype(np.float32).astype(np.float64)
+    ga = mlp_ref.grad(p32, u32, t32)
+    gfd = mlp_ref.grad_fd(p32, u32, t32)
+    # the CPU analytic gradient is pinned by finite differences ...
+    assert np.allclose(ga, gfd, rtol=1e-4, atol=1e-6 * np.abs(ga).max())
+    # ... and the device gradient (fp32 accumulation) agrees with it
+    gd = g.cpu().numpy().astype(np.float64)
+    assert np.allclose(gd, ga, rtol=2e-3, atol=2e-4 * np.abs(ga).max())
+    assert abs(loss - mlp_ref.loss(p32, u32, t32)) <= 1e-3 * mlp_ref.loss(p32, u32, t32) + 1e-7
+
+
+# ---------------------------------------------------------------------------
+# learned lower_bound
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("key_bytes", [4, 8])
+def test_learned_lower_bound_is_exact_in_every_mode(key_bytes):
+    rng = np.random.default_rng(key_bytes)
+    px, py = _zipf_points(300_000, rng)
+    px[:1000] = px[1000:2000]  # duplicate points -> duplicate keys
+    py[:1000] = py[1000:2000]
+    idx = J.LearnedPointIndex(_dp(px, py), key_bytes=key_bytes, train_iters=64)
+    kt = torch.int64 if key_bytes == 8 else torch.int32
+    npt = np.uint64 if key_bytes == 8 else np.uint32
+    keys = idx.keys.cpu().numpy().view(npt)
+    assert np.all(keys[:-1] <= keys[1:])
+    top = np.iinfo(npt).max
+    q = np.concatenate([
+        rng.integers(0, top, 20_000, dtype=np.uint64).astype(npt),
+        keys[rng.integers(0, len(keys), 20_000)],
+        keys[rng.integers(0, len(keys), 5_000)] + npt(1),
+        keys[rng.integers(1, len(keys), 5_000)] - npt(1),
+        np.array([0, 1, top, top - 1, keys[0], keys[-1]], dtype=npt),
+    ])
+    expect = np.searchsorted(keys, q, side="left")
+    qt = torch.from_numpy(q.view(np.int64 if key_bytes == 8 else np.int32)).cuda()
+    probes = {}
+    for mode in ("learned", "leaves", "binary"):
+        got, st = idx.lower_bound(qt, mode)
+        assert np.array_equal(got.cpu().numpy(), expect), mode
+        probes[mode] = st["probes"] / len(q)
+    print(f"probes per lookup ({key_bytes}-byte keys): {probes}")
+    # what the network gains: fewer probes than the leaf table alone and than
+    # a plain binary search over the keys
+    assert probes["learned"] < probes["leaves"] and probes["learned"] < probes["binary"]
+
+
+def test_training_improves_the_rank_model():
+    rng = np.random.default_rng(5)
+    px, py = _zipf_points(400_000, rng)
+    trained = J.LearnedPointIndex(_dp(px, py), key_bytes=8, train_iters=128)
+    init = J.LearnedPointIndex(_dp(px, py), key_bytes=8, train_iters=0)
+    l0, l1 = trained.losses.cpu().numpy()
+    assert l1 <= l0
+    assert trained.mlp_leaf_error <= init.mlp_leaf_error
+    keys = trained.keys
+    q = keys[torch.randint(0, keys.numel(), (50_000,), device=keys.device)]
+    _, st_t = trained.lower_bound(q, "learned")
+    _, st_i = init.lower_bound(q, "learned")
+    print(f"loss {l0:.3e} -> {l1:.3e}; MLP leaf error {init.mlp_leaf_error} -> {trained.mlp_leaf_error}; "
+          f"probes {st_i['probes'] / 5e4:.2f} -> {st_t['probes'] / 5e4:.2f}")
+    assert st_t["probes"] <= st_i["probes"]
+
+
+# ---------------------------------------------------------------------------
+# learned box query (point tree x boxes)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("key_bytes", [4, 8])
+def test_learned_query_boxes_matches_oracle(key_bytes):
+    rng = np.random.default_rng(10 + key_bytes)
+    px, py = _zipf_points(200_000, rng)
+    ci = rng.integers(0, len(px), 3000)
+    w = rng.lognormal(np.log(50.0), 1.0, 3000)
+    h = rng.lognormal(np.log(50.0), 1.0, 3000)
+    boxes = np.column_stack([px[ci] - w / 2, py[ci] - h / 2, px[ci] + w / 2, py[ci] + h / 2])
+    boxes[7] = np.nan
+    boxes[8] = [px[5], py[5], px[5], py[5]]  # degenerate box on a point
+    idx = J.LearnedPointIndex(_dp(px, py), key_bytes=key_bytes)
+    el, er = oracle.box_query_points(px, py, boxes, nthreads=NTHREADS)
+    bt = torch.from_numpy(boxes).cuda()
+    for mode in ("learned", "binary"):
+        l, r, st = idx.query_boxes(bt, mode)
+        assert np.array_equal(l.cpu().numpy(), el) and np.array_equal(r.cpu().numpy(), er), mode
+        assert st.pairs == len(el) and st.scanned >= len(el)
+
+
+# ---------------------------------------------------------------------------
+# learned + FP32 join == oracle, bit for bit
+# ---------------------------------------------------------------------------
+def _learned_join(px, py, polys, predicate, **kw):
+    P = DevicePolygons.from_host(polys)
+    l, r, st = J.sjoin_points_polygons_learned(_dp(px, py), P, J.polygon_bounds(P), predicate, **kw)
+    return l.cpu().numpy(), r.cpu().numpy(), st
+
+
+@pytest.mark.parametrize("predicate", ["within", "intersects", "touches"])
+def test_learned_join_c1_with_boundary_points(predicate):
+    w = S.c1()
+    rng = np.random.default_rng(1)
+    px, py = w.points.x.copy(), w.points.y.copy()
+    k = rng.integers(0, len(w.polygons.x), 1000)
+    px[:1000], py[:1000] = w.polygons.x[k], w.polygons.y[k]
+    px[1000:2000] = 0.5 * (w.polygons.x[k] + w.polygons.x[np.minimum(k + 1, len(k) - 1)])
+    py[1000:2000] = 0.5 * (w.polygons.y[k] + w.polygons.y[np.minimum(k + 1, len(k) - 1)])
+    px[5] = np.nan
+    l, r, st = _learned_join(px, py, w.polygons, predicate)
+    el, er = oracle.sjoin_points_polygons(px, py, w.polygons, predicate, nthreads=NTHREADS)
+    assert np.array_equal(l, el) and np.array_equal(r, er)
+    assert st.escalated_edges > 0  # vertex points tie in y and must escalate
+
+
+def _verify(px, py, P, rows):
+    return oracle.point_location_pairs(px, py, np.arange(len(px)), rows, P, nthreads=NTHREADS)
+
+
+@pytest.mark.parametrize("predicate", ["within", "intersects"])
+def test_learned_join_c3_utm_degenerate(predicate):
+    print(f"c3 1M x 100k {predicate}", flush=True)
+    w = S.c3(n_points=1_000_000, n_polys=100_000, verify=_verify)
+    l, r, st = _learned_join(w.points.x, w.points.y, w.polygons, predicate)
+    el, er = oracle.sjoin_points_polygons(w.points.x, w.points.y, w.polygons, predicate, nthreads=NTHREADS)
+    assert np.array_equal(l, el) and np.array_equal(r, er)
+    print(f"escalation: {st.escalated_edges} of {st.orient_tests + st.escalated_edges} orientation decisions "
+          f"({st.escalation_fraction:.3e}); pairs with an escalation {st.escalated_pairs} of {st.candidates}")
+    assert st.escalated_edges > 0
+
+
+def test_learned_join_matches_plain_on_c2_full_size():
+    print("c2 full learned", flush=True)
+    w = S.c2()
+    Q = DevicePoints.from_host(w.points)
+    P = DevicePolygons.from_host(w.polygons)
+    b = J.polygon_bounds(P)
+    l1, r1, st1 = J.sjoin_points_polygons_learned(Q, P, b, "within")
+    l0, r0, st0 = J.sjoin_points_polygons(Q, P, J.build_tree(b), "within")
+    assert torch.equal(l1, l0) and torch.equal(r1, r0)
+    assert st1.candidates == st0.candidates  # both see exactly the inclusive bbox candidate set
+    el, er = oracle.sjoin_points_polygons(w.points.x, w.points.y, w.polygons, "within", nthreads=NTHREADS)
+    assert np.array_equal(l1.cpu().numpy(), el) and np.array_equal(r1.cpu().numpy(), er)
+
+
+def test_learned_join_edge_cases():
+    w = S.c1(n_points=1000, n_polys=50)
+    P = w.polygons
+    assert len(_learned_join(np.empty(0), np.empty(0), P, "within")[0]) == 0
+    assert len(_learned_join(np.full(50, np.nan), np.full(50, np.nan), P, "within")[0]) == 0
+    one = np.array([w.polygons.x[0]]), np.array([w.polygons.y[0]])
+    assert len(_learned_join(*one, P, "within")[0]) == 0
+    assert len(_learned_join(*one, P, "intersects")[0]) >= 1
+    from vibespatial_amd.geometry import MultiPolygon, Polygon, PolygonArray, box
+
+    geoms = [None, box(0, 0, 4, 4), Polygon(),
+             MultiPolygon([box(10, 10, 12, 12), Polygon([(20, 20), (30, 20), (30, 30), (20, 30)],
+                                                        [[(22, 22), (28, 22), (28, 28), (22, 28)]])])]
+    PA = PolygonArray.from_geometries(geoms)
+    px = np.array([2, 4, 11, 25, 21, 22, 100, 0.0, 12, 20])
+    py = np.array([2, 2, 11, 25, 21, 25, 100, 0.0, 12, 30])
+    for pred in ("within", "intersects", "touches"):
+        l, r, _ = _learned_join(px, py, PA, pred)
+        el, er = oracle.sjoin_points_polygons(px, py, PA, pred)
+        assert np.array_equal(l, el) and np.array_equal(r, er), pred

@@ -77,6 +77,20 @@ SIGNATURES = {
     ),
     "vs_lexsort_pairs_workspace_bytes": (_i64, [_i64]),
     "vs_lexsort_pairs": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p]),
+    "vs_learned_leaf_count": (_i32, [_i64, _i32]),
+    "vs_learned_index_bytes": (_i64, [_i64, _i32, _i32]),
+    "vs_learned_index_layout": (ctypes.c_int, [_i64, _i32, _i32, _p]),
+    "vs_learned_index_build": (ctypes.c_int, [_p, _p, _i64, _i32, _i32, _i32, ctypes.c_float, _p, _i64, _p]),
+    "vs_learned_lower_bound": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _i64, _i32, _p, _p, _p]),
+    "vs_mlp_loss_grad": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _p]),
+    "vs_learned_query_boxes_workspace_bytes": (_i64, [_i64, _i64]),
+    "vs_learned_query_boxes": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _i64, _i32, _i64, _p, _p, _p, _p, _i64, _p]),
+    "vs_sjoin_points_polygons_learned_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
+    "vs_sjoin_points_polygons_learned": (
+        ctypes.c_int,
+        [_p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _p, _i32, _i32, _i32, _i64, _p, _p, _p, _p, _i64,
+         _p],
+    ),
     "vs_scan_workspace_bytes": (_i64, [_i64]),
     "vs_exclusive_scan_i32": (ctypes.c_int, [_p, _p, _i64, _p, _p, _i64, _p]),
     "vs_sort_workspace_bytes": (_i64, [_i64, _i32]),

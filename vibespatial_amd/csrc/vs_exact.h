@@ -202,8 +202,24 @@ VS_HD uint64_t hilbert64(uint32_t x, uint32_t y) {
 VS_HD uint32_t quantize(double v, double lo, double scale, int bits) {
     double q = (v - lo) * scale;
     const double top = (double)((bits >= 32) ? 4294967295.0 : (double)((1ull << bits) - 1));
-    q = q < 0.0 ? 0.0 : (q > top ? top : q);
+    q = q > 0.0 ? (q < top ? q : top) : 0.0;  // NaN (empty extent) -> 0
     return (uint32_t)q;
+}
+
+// Grid quantisation of the key space: `bits` per axis over an extent.
+struct KeyScale {
+    double x0, y0, sx, sy;
+};
+
+__device__ __forceinline__ KeyScale key_scale(const double4* ext, int bits) {
+    const double4 e = ext[0];
+    const double cells = (bits >= 32) ? 4294967295.0 : (double)((1ull << bits) - 1);
+    KeyScale k;
+    k.x0 = e.x;
+    k.y0 = e.y;
+    k.sx = (e.z > e.x) ? cells / (e.z - e.x) : 0.0;
+    k.sy = (e.w > e.y) ? cells / (e.w - e.y) : 0.0;
+    return k;
 }
 
 }  // namespace vs

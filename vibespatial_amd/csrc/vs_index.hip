@@ -148,20 +148,6 @@ int point_extent(const double* px, const double* py, int64_t n, double4* ext, vo
     return VS_OK;
 }
 
-struct KeyScale {
-    double x0, y0, sx, sy;
-};
-
-__device__ __forceinline__ KeyScale key_scale(const double4* ext, int bits) {
-    const double4 e = ext[0];
-    const double cells = (bits >= 32) ? 4294967295.0 : (double)((1ull << bits) - 1);
-    KeyScale k;
-    k.x0 = e.x;
-    k.y0 = e.y;
-    k.sx = (e.z > e.x) ? cells / (e.z - e.x) : 0.0;
-    k.sy = (e.w > e.y) ? cells / (e.w - e.y) : 0.0;
-    return k;
-}
 
 // Key of bits*2 bits: the Hilbert index of the box centre on a 2^bits grid over
 // the extent.  NaN boxes -> all ones (sort last).
@@ -325,6 +311,51 @@ int64_t point_order_workspace_bytes(int64_t n) {
     c.take<char>(extent_workspace_bytes());
     c.take<char>(sort_workspace_bytes_u32(n));
     return c.bytes;
+}
+
+// Point index for the learned form: keys of `bits` per axis (K holds 2*bits),
+// sorted, with the permutation, gathered coordinates and the extent the keys
+// were quantised over (device double4).
+template <class K>
+int64_t point_index_workspace_bytes_t(int64_t n) {
+    Carver c(nullptr);
+    c.take<char>(extent_workspace_bytes());
+    c.take<char>(sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n));
+    return c.bytes;
+}
+
+template <class K>
+int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, int32_t* perm, double* sx,
+                        double* sy, double4* ext, void* ws, hipStream_t s) {
+    Carver c(ws);
+    void* ext_ws = c.take<char>(extent_workspace_bytes());
+    const int64_t sb = sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n);
+    void* sort_ws = c.take<char>(sb);
+    VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
+    if (n == 0) return VS_OK;
+    const int bits = 4 * (int)sizeof(K);
+    hipLaunchKernelGGL(hilbert_keys_points_kernel<K>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py, n,
+                       ext, bits, keys, perm);
+    VS_LAUNCH_CHECK();
+    if (sizeof(K) == 4)
+        VS_TRY(sort_pairs_u32(reinterpret_cast<uint32_t*>(keys), perm, n, 0, 32, sort_ws, sb, s));
+    else
+        VS_TRY(sort_pairs_u64(reinterpret_cast<uint64_t*>(keys), perm, n, 0, 64, sort_ws, sb, s));
+    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py, perm, n,
+                       sx, sy);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int64_t point_index_workspace_bytes_u32(int64_t n) { return point_index_workspace_bytes_t<uint32_t>(n); }
+int64_t point_index_workspace_bytes_u64(int64_t n) { return point_index_workspace_bytes_t<uint64_t>(n); }
+int point_index_build_u32(const double* px, const double* py, int64_t n, uint32_t* keys, int32_t* perm, double* sx,
+                          double* sy, double4* ext, void* ws, hipStream_t s) {
+    return point_index_build_t<uint32_t>(px, py, n, keys, perm, sx, sy, ext, ws, s);
+}
+int point_index_build_u64(const double* px, const double* py, int64_t n, uint64_t* keys, int32_t* perm, double* sx,
+                          double* sy, double4* ext, void* ws, hipStream_t s) {
+    return point_index_build_t<uint64_t>(px, py, n, keys, perm, sx, sy, ext, ws, s);
 }
 
 int point_order(const double* px, const double* py, int64_t n, int bits, int32_t* perm, double* sx,

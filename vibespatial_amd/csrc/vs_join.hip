@@ -33,6 +33,7 @@
 // an idle XCD steals from the others.  Every wave exits when all 8 are empty.
 #include "vs_exact.h"
 #include "vs_internal.h"
+#include "vs_join_common.h"
 
 namespace vs {
 
@@ -73,12 +74,6 @@ struct PolyView {
 
 constexpr int kOutBuf = 256;  // per-wave LDS buffer of refined pairs
 
-struct JoinCtl {
-    unsigned long long staged;  // pairs appended (may exceed capacity)
-    long long total;            // scan total of per-query counts
-    unsigned int queue[8];      // per-XCD wave queues
-    unsigned long long stat[4]; // [0] candidate pairs (bbox hits), [1] sum of their polygons' vertex counts
-};
 
 enum { kModePip = 0, kModeBox = 1 };
 
@@ -88,6 +83,7 @@ __global__ void ctl_reset_kernel(JoinCtl* ctl) {
         ctl->total = 0;
         for (int i = 0; i < 8; ++i) ctl->queue[i] = 0;
         for (int i = 0; i < 4; ++i) ctl->stat[i] = 0;
+        ctl->escalated_pairs = 0;
     }
 }
 
@@ -495,6 +491,9 @@ int timing_end(hipStream_t s) {
     return VS_OK;
 }
 
+int timing_begin_public(hipStream_t s) { return timing_begin(s); }
+int timing_end_public(hipStream_t s) { return timing_end(s); }
+
 template <int MODE>
 int launch_traverse(int pred, const double* qx, const double* qy, const double4* qbox,
                     const int32_t* perm, int64_t n, const TreeView& T, const PolyView& P,
@@ -517,26 +516,61 @@ int launch_traverse(int pred, const double* qx, const double* qy, const double4*
     return VS_OK;
 }
 
-int finish_pairs(const JoinWs& J, int64_t nq, int64_t capacity, int64_t* out_l, int64_t* out_r,
-                 int64_t* d_result, hipStream_t s) {
-    VS_TRY(exclusive_scan_i32(J.counts, J.offsets, nq, &J.ctl->total, J.scan_ws, s));
+int finish_pairs_raw(const int2* stage, int32_t* counts, int32_t* offsets, int32_t* cursor, void* scan_ws,
+                     JoinCtl* ctl, int64_t nq, int64_t capacity, int64_t* out_l, int64_t* out_r, int64_t* d_result,
+                     hipStream_t s) {
+    VS_TRY(exclusive_scan_i32(counts, offsets, nq, &ctl->total, scan_ws, s));
     if (nq > 0) {
-        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, J.cursor, nq, 0);
+        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, cursor, nq, 0);
         VS_LAUNCH_CHECK();
     }
     const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, 256), 65536));
-    hipLaunchKernelGGL(scatter_pairs_kernel, dim3(sgrid), dim3(256), 0, s, J.stage, J.ctl, capacity,
-                       J.offsets, J.cursor, out_l, out_r);
+    hipLaunchKernelGGL(scatter_pairs_kernel, dim3(sgrid), dim3(256), 0, s, stage, ctl, capacity, offsets, cursor,
+                       out_l, out_r);
     VS_LAUNCH_CHECK();
     if (nq > 0) {
-        hipLaunchKernelGGL(segment_sort_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, J.ctl,
-                           capacity, J.offsets, J.counts, nq, out_r);
+        hipLaunchKernelGGL(segment_sort_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, ctl, capacity,
+                           offsets, counts, nq, out_r);
         VS_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(publish_result_kernel, dim3(1), dim3(64), 0, s, J.ctl, d_result);
+    hipLaunchKernelGGL(publish_result_kernel, dim3(1), dim3(64), 0, s, ctl, d_result);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }
+
+int finish_pairs(const JoinWs& J, int64_t nq, int64_t capacity, int64_t* out_l, int64_t* out_r,
+                 int64_t* d_result, hipStream_t s) {
+    return finish_pairs_raw(J.stage, J.counts, J.offsets, J.cursor, J.scan_ws, J.ctl, nq, capacity, out_l, out_r,
+                            d_result, s);
+}
+
+int build_refine_flags(const int32_t* part, const int32_t* ring, int64_t n_parts, int64_t n_rings, int64_t n_coords,
+                       uint8_t* flags, hipStream_t s) {
+    if (n_coords > 0) {
+        hipLaunchKernelGGL(fill_u8_kernel, dim3((unsigned)ceil_div(n_coords, 256)), dim3(256), 0, s, flags, n_coords);
+        VS_LAUNCH_CHECK();
+    }
+    if (n_rings > 0) {
+        hipLaunchKernelGGL(ring_start_flags_kernel, dim3((unsigned)ceil_div(n_rings, 256)), dim3(256), 0, s, ring,
+                           n_rings, flags);
+        VS_LAUNCH_CHECK();
+    }
+    if (n_parts > 0) {
+        hipLaunchKernelGGL(part_start_flags_kernel, dim3((unsigned)ceil_div(n_parts, 256)), dim3(256), 0, s, part,
+                           ring, n_parts, flags);
+        VS_LAUNCH_CHECK();
+    }
+    return VS_OK;
+}
+
+void ctl_reset(JoinCtl* ctl, hipStream_t s) { hipLaunchKernelGGL(ctl_reset_kernel, dim3(1), dim3(64), 0, s, ctl); }
+
+void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+int timing_begin_public(hipStream_t s);
+int timing_end_public(hipStream_t s);
 
 }  // namespace vs
 
@@ -590,21 +624,7 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
     hipStream_t s = vs::stream_of(stream);
     hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl);
     VS_LAUNCH_CHECK();
-    if (n_coords > 0) {
-        hipLaunchKernelGGL(vs::fill_u8_kernel, dim3((unsigned)vs::ceil_div(n_coords, 256)), dim3(256), 0, s, J.flags,
-                           n_coords);
-        VS_LAUNCH_CHECK();
-    }
-    if (n_rings > 0) {
-        hipLaunchKernelGGL(vs::ring_start_flags_kernel, dim3((unsigned)vs::ceil_div(n_rings, 256)), dim3(256), 0, s,
-                           ring_offsets, n_rings, J.flags);
-        VS_LAUNCH_CHECK();
-    }
-    if (n_parts > 0) {
-        hipLaunchKernelGGL(vs::part_start_flags_kernel, dim3((unsigned)vs::ceil_div(n_parts, 256)), dim3(256), 0, s,
-                           part_offsets, ring_offsets, n_parts, J.flags);
-        VS_LAUNCH_CHECK();
-    }
+    VS_TRY(vs::build_refine_flags(part_offsets, ring_offsets, n_parts, n_rings, n_coords, J.flags, s));
     if (n_polys > 0) {
         hipLaunchKernelGGL(vs::item_range_kernel, dim3((unsigned)vs::ceil_div(n_polys, 256)), dim3(256), 0, s, item_ids,
                            n_polys, geom_offsets, part_offsets, ring_offsets, J.item_range);

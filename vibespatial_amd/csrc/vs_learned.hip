@@ -1,0 +1,1095 @@
+// Learned point index and the polygon-major FP32 join.
+//
+// No counterpart in the reference: its sjoin index is a Morton range scan
+// (src/vibespatial/spatial/spatial_index_device.py:761-1034) and its point refine
+// admits only fp64 (src/vibespatial/predicates/point_relations.py:557-579).  This
+// file restates the same results with a different organisation:
+//
+// Point index.  Points sorted by Hilbert key (bits per axis = 4 * sizeof(K)).
+// A two-stage rank model over the sorted keys:
+//   stage 1  a 2-layer MLP (1 -> 32 ReLU -> 1, fp32) maps the key (as a unit
+//            float) to an estimate of its CDF, i.e. a leaf number;
+//   stage 2  S equal-count leaves; leaf s spans sorted positions
+//            [pos(s), pos(s+1)] with pos(s) = floor(s (n-1) / S) and predicts the
+//            position by linear interpolation between its end keys.
+// Recorded bounds: E = max over indexed keys of |MLP leaf - true leaf| and
+// err[s] = max over the leaf's keys of |prediction - position|.  A lower_bound
+// lookup searches leaves only inside [leaf - E, leaf + E] and positions only in
+// [pred - err, pred + err + 1]; both windows are verified by bracketing keys and
+// widened to the full range when a check fails, so the result is always the
+// exact lower_bound (the fallbacks are counted).
+// The MLP is initialised as the piecewise-linear interpolant of 33 quantile knots
+// and refined on the GPU by full-batch Adam with hand-written forward/backward
+// passes (mlp_grad); the best-loss parameters are kept.
+//
+// Query windows.  A box maps to <= 9 Hilbert cells of one level (<= 3 per axis),
+// merged into key ranges; each range becomes a rank range through two lookups.
+//
+// Polygon-major join (learned+FP32 form).  Work items are (polygon, chunk of 512
+// scanned positions) taken from one atomic queue; a 256-thread block scans its
+// positions with the exact fp64 point-in-box test, keeps the candidates in LDS,
+// stages the polygon's vertices in LDS as fp32 coordinates recentred on the
+// polygon's box centre, and runs the crossing-number / boundary rule per
+// candidate with these decisions:
+//   y ties in fp32 -> escalate; strict fp32 order is the true order (the
+//   recentring x -> fl32(fl64(x - c)) is monotone);
+//   orientation: |det_fp32| > E_q decides the sign, E_q a forward error bound
+//   covering both the input rounding and the fp32 arithmetic; otherwise escalate.
+// An escalated edge is evaluated by the exact fp64 step (stage-A filter, then
+// the exact expansion) on the original coordinates.
+#include "vs_exact.h"
+#include "vs_internal.h"
+#include "vs_join_common.h"
+
+namespace vs {
+
+int64_t point_index_workspace_bytes_u32(int64_t n);
+int64_t point_index_workspace_bytes_u64(int64_t n);
+int point_index_build_u32(const double* px, const double* py, int64_t n, uint32_t* keys, int32_t* perm, double* sx,
+                          double* sy, double4* ext, void* ws, hipStream_t s);
+int point_index_build_u64(const double* px, const double* py, int64_t n, uint64_t* keys, int32_t* perm, double* sx,
+                          double* sy, double4* ext, void* ws, hipStream_t s);
+int build_refine_flags(const int32_t* part, const int32_t* ring, int64_t n_parts, int64_t n_rings, int64_t n_coords,
+                       uint8_t* flags, hipStream_t s);
+
+constexpr int kHidden = 32;
+constexpr int kNParams = 3 * kHidden + 1;  // w1[32], b1[32], w2[32], b2
+constexpr int kSample = 2048;              // training keys (evenly spaced ranks)
+constexpr int kMaxRanges = 9;
+constexpr int kChunk = 512;                // scanned positions per work item
+constexpr int kLJThreads = 256;
+constexpr int kCandPerThread = kChunk / kLJThreads;
+constexpr int kVmax = 2048;                // vertices staged per batch
+
+// ---------------------------------------------------------------------------
+// MLP: forward, backward, training
+// ---------------------------------------------------------------------------
+template <class K>
+__device__ __forceinline__ float key_unit(K k) {
+    // monotone map of the key to [0, 1]
+    return sizeof(K) == 4 ? (float)k * 2.3283064365386963e-10f : (float)k * 5.421010862427522e-20f;
+}
+
+__device__ __forceinline__ float mlp_eval(const float* P, float u) {
+    float y = P[3 * kHidden];
+#pragma unroll
+    for (int h = 0; h < kHidden; ++h) {
+        const float z = P[h] * u + P[kHidden + h];
+        y += P[2 * kHidden + h] * fmaxf(z, 0.0f);
+    }
+    return y;
+}
+
+// Accumulates d(scale * (y - t)^2)/d(params) into g[0..96] and (y - t)^2 into loss.
+__device__ __forceinline__ void mlp_grad(const float* P, float u, float t, float scale, float* g, float& loss) {
+    float z[kHidden];
+    float y = P[3 * kHidden];
+#pragma unroll
+    for (int h = 0; h < kHidden; ++h) {
+        z[h] = P[h] * u + P[kHidden + h];
+        y += P[2 * kHidden + h] * fmaxf(z[h], 0.0f);
+    }
+    const float r = y - t;
+    loss += r * r;
+    const float dy = 2.0f * r * scale;
+    g[3 * kHidden] += dy;
+#pragma unroll
+    for (int h = 0; h < kHidden; ++h) {
+        const float a = fmaxf(z[h], 0.0f);
+        g[2 * kHidden + h] += dy * a;
+        const float dz = z[h] > 0.0f ? dy * P[2 * kHidden + h] : 0.0f;
+        g[h] += dz * u;
+        g[kHidden + h] += dz;
+    }
+}
+
+// Block-wide sum of g[0..kNParams] (+ loss at index kNParams) into out[].
+__device__ void block_sum_grads(float* g, float loss, float (*red)[kNParams + 1], float* out) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k <= kNParams; ++k) {
+        float v = k < kNParams ? g[k] : loss;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (l == 0) red[w][k] = v;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k <= kNParams; k += blockDim.x) {
+        float v = 0.0f;
+        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) v += red[ww][k];
+        out[k] = v;
+    }
+    __syncthreads();
+}
+
+// One block: sample, quantile-knot initialisation, `iters` full-batch Adam
+// steps; writes the best-loss parameters and [initial loss, best loss].
+template <class K>
+__global__ __launch_bounds__(256) void mlp_train_kernel(const K* __restrict__ keys, int64_t n, int iters, float lr,
+                                                       float* __restrict__ params_out, float* __restrict__ losses) {
+    __shared__ float su[kSample], st[kSample];
+    __shared__ float P[kNParams], M1[kNParams], M2[kNParams], SC[kNParams], BEST[kNParams];
+    __shared__ float red[4][kNParams + 1];
+    __shared__ float tot[kNParams + 1];
+    __shared__ float best_loss, init_loss;
+    const int tid = threadIdx.x;
+    const int S = n >= kSample ? kSample : (int)n;
+    for (int j = tid; j < S; j += blockDim.x) {
+        const int64_t i = S > 1 ? (int64_t)((n - 1) * (double)j / (double)(S - 1)) : 0;
+        su[j] = key_unit<K>(keys[i]);
+        st[j] = (float)((double)i / (double)n);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // knots q_h at sample ranks h/H, PL interpolant g(u) = t_0 + sum_h dS_h relu(u - q_h)
+        float prev_slope = 0.0f;
+        for (int h = 0; h < kHidden; ++h) {
+            const int j0 = (int)((int64_t)h * (S - 1) / kHidden);
+            const int j1 = (int)((int64_t)(h + 1) * (S - 1) / kHidden);
+            const float q0 = su[j0], q1 = su[j1];
+            const float slope = (st[j1] - st[j0]) / fmaxf(q1 - q0, 1.0e-7f);
+            P[h] = 1.0f;
+            P[kHidden + h] = -q0;
+            P[2 * kHidden + h] = slope - prev_slope;
+            prev_slope = slope;
+            SC[h] = 0.05f;
+            SC[kHidden + h] = fmaxf(q1 - q0, 1.0e-7f);
+            SC[2 * kHidden + h] = fmaxf(fabsf(slope), 1.0e-3f);
+        }
+        P[3 * kHidden] = st[0];
+        SC[3 * kHidden] = 1.0f / kHidden;
+        best_loss = INFINITY;
+    }
+    for (int k = tid; k < kNParams; k += blockDim.x) {
+        M1[k] = 0.0f;
+        M2[k] = 0.0f;
+    }
+    __syncthreads();
+    const float scale = 1.0f / (float)(S > 0 ? S : 1);
+    const float b1 = 0.9f, b2 = 0.999f;
+    float b1t = 1.0f, b2t = 1.0f;
+    for (int it = 0; it <= iters; ++it) {
+        float g[kNParams];
+#pragma unroll
+        for (int k = 0; k < kNParams; ++k) g[k] = 0.0f;
+        float loss = 0.0f;
+        for (int j = tid; j < S; j += blockDim.x) mlp_grad(P, su[j], st[j], scale, g, loss);
+        block_sum_grads(g, loss, red, tot);
+        if (tid == 0) {
+            const float L = tot[kNParams] * scale;
+            if (it == 0) init_loss = L;
+            if (L < best_loss) {
+                best_loss = L;
+                for (int k = 0; k < kNParams; ++k) BEST[k] = P[k];
+            }
+        }
+        __syncthreads();
+        if (it == iters) break;
+        b1t *= b1;
+        b2t *= b2;
+        for (int k = tid; k < kNParams; k += blockDim.x) {
+            const float gk = tot[k];
+            M1[k] = b1 * M1[k] + (1.0f - b1) * gk;
+            M2[k] = b2 * M2[k] + (1.0f - b2) * gk * gk;
+            const float mh = M1[k] / (1.0f - b1t), vh = M2[k] / (1.0f - b2t);
+            P[k] -= lr * SC[k] * mh / (sqrtf(vh) + 1.0e-12f);
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < kNParams; k += blockDim.x) params_out[k] = BEST[k];
+    if (tid == 0) {
+        losses[0] = init_loss;
+        losses[1] = best_loss;
+    }
+}
+
+// Full-batch loss and gradient of given parameters on given (u, t) data
+// (the same device code the trainer uses), for checking against a CPU oracle.
+__global__ __launch_bounds__(256) void mlp_loss_grad_kernel(const float* __restrict__ params,
+                                                           const float* __restrict__ u, const float* __restrict__ t,
+                                                           int64_t n, float* __restrict__ grad_out,
+                                                           float* __restrict__ loss_out) {
+    __shared__ float P[kNParams];
+    __shared__ float red[4][kNParams + 1];
+    __shared__ float tot[kNParams + 1];
+    for (int k = threadIdx.x; k < kNParams; k += blockDim.x) P[k] = params[k];
+    __syncthreads();
+    float g[kNParams];
+#pragma unroll
+    for (int k = 0; k < kNParams; ++k) g[k] = 0.0f;
+    float loss = 0.0f;
+    const float scale = 1.0f / (float)(n > 0 ? n : 1);
+    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) mlp_grad(P, u[j], t[j], scale, g, loss);
+    block_sum_grads(g, loss, red, tot);
+    for (int k = threadIdx.x; k < kNParams; k += blockDim.x) grad_out[k] = tot[k];
+    if (threadIdx.x == 0) loss_out[0] = tot[kNParams] * scale;
+}
+
+// ---------------------------------------------------------------------------
+// Leaf table, error bounds, lookups
+// ---------------------------------------------------------------------------
+template <class K>
+struct LearnedView {
+    const K* keys;
+    int64_t n;
+    const float* mlp;
+    const K* leaf_key;  // S + 1
+    const int32_t* leaf_err;
+    const int32_t* meta;  // [0] E (MLP leaf error)
+    int32_t S;
+    int32_t mode;  // 0 learned, 1 leaf table only (binary search over leaves), 2 plain binary search
+};
+
+__device__ __forceinline__ int64_t leaf_pos(int64_t s, int64_t n, int64_t S) { return s * (n - 1) / S; }
+
+template <class K>
+__device__ __forceinline__ int mlp_leaf(const float* P, K k, int S) {
+    const float g = mlp_eval(P, key_unit<K>(k));
+    const float f = g * (float)S;
+    if (!(f > 0.0f)) return 0;  // also NaN
+    if (f >= (float)(S - 1)) return S - 1;
+    return (int)f;
+}
+
+template <class K>
+__device__ __forceinline__ double plm_pred(K q, int s, const K* leaf_key, int64_t n, int S) {
+    const int64_t p0 = leaf_pos(s, n, S), p1 = leaf_pos(s + 1, n, S);
+    const K k0 = leaf_key[s], k1 = leaf_key[s + 1];
+    if (!(k1 > k0)) return (double)p0;
+    const double f = (double)(K)(q - k0) / (double)(K)(k1 - k0);
+    return (double)p0 + f * (double)(p1 - p0);
+}
+
+template <class K>
+__global__ __launch_bounds__(256) void leaf_keys_kernel(const K* __restrict__ keys, int64_t n, int S,
+                                                       K* __restrict__ leaf_key) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s <= S) leaf_key[s] = keys[leaf_pos(s, n, S)];
+}
+
+// One wave per leaf: err[s] = ceil(max |pred - i|) over i in [pos(s), pos(s+1)];
+// the MLP leaf error over the leaf's first occurrences goes to meta[0].
+template <class K>
+__global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ keys, int64_t n, int S,
+                                                      const K* __restrict__ leaf_key, const float* __restrict__ mlp,
+                                                      int32_t* __restrict__ leaf_err, int32_t* __restrict__ meta) {
+    __shared__ float P[kNParams];
+    for (int k = threadIdx.x; k < kNParams; k += blockDim.x) P[k] = mlp[k];
+    __syncthreads();
+    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= S) return;
+    const int64_t p0 = leaf_pos(s, n, S), p1 = leaf_pos(s + 1, n, S);
+    double emax = 0.0;
+    int mlp_e = 0;
+    for (int64_t i = p0 + lane; i <= p1; i += 64) {
+        const K k = keys[i];
+        const double e = fabs(plm_pred<K>(k, (int)s, leaf_key, n, S) - (double)i);
+        emax = fmax(emax, e);
+        if (i > p0 && (i == 0 || k != keys[i - 1])) {
+            // true leaf of a first occurrence at position i >= 1
+            const int64_t st = (i * (int64_t)S - 1) / (n - 1);
+            const int sp = mlp_leaf<K>(P, k, S);
+            const int d = (int)(sp > st ? sp - st : st - sp);
+            mlp_e = d > mlp_e ? d : mlp_e;
+        }
+    }
+    emax = wave_max(emax);
+    for (int o = 32; o > 0; o >>= 1) {
+        const int v = __shfl_xor(mlp_e, o, 64);
+        mlp_e = v > mlp_e ? v : mlp_e;
+    }
+    if (lane == 0) {
+        leaf_err[s] = (int32_t)ceil(emax);
+        atomicMax(&meta[0], mlp_e);
+    }
+}
+
+struct LookupStats {
+    unsigned long long probes;      // key / leaf-key reads in binary searches
+    unsigned long long leaf_fb;     // MLP leaf window failed its bracket check
+    unsigned long long win_fb;      // leaf error window failed its bracket check
+};
+
+// First position j with keys[j] >= q (n if none).
+template <class K>
+__device__ int64_t learned_lower_bound(K q, const LearnedView<K>& V, LookupStats& st) {
+    const int64_t n = V.n;
+    if (n == 0) return 0;
+    if (q <= V.keys[0]) return 0;
+    if (q > V.keys[n - 1]) return n;
+    int64_t a, b;
+    if (V.mode == 2) {
+        a = 1;
+        b = n - 1;
+    } else {
+        const int S = V.S;
+        int wl = 0, wh = S - 1;
+        if (V.mode == 0) {
+            const int E = V.meta[0];
+            const int sp = mlp_leaf<K>(V.mlp, q, S);
+            int l2 = sp - E < 0 ? 0 : sp - E, h2 = sp + E > S - 1 ? S - 1 : sp + E;
+            st.probes += 2;
+            if (V.leaf_key[l2] < q && (h2 == S - 1 || V.leaf_key[h2 + 1] >= q)) {
+                wl = l2;
+                wh = h2;
+            } else {
+                ++st.leaf_fb;
+            }
+        }
+        while (wl < wh) {  // last leaf whose first key is < q
+            const int mid = (wl + wh + 1) >> 1;
+            ++st.probes;
+            if (V.leaf_key[mid] < q) wl = mid;
+            else wh = mid - 1;
+        }
+        const int s = wl;
+        const int64_t lo = leaf_pos(s, n, S) + 1, hi = leaf_pos(s + 1, n, S);
+        const double p = plm_pred<K>(q, s, V.leaf_key, n, S);
+        const int32_t e = V.leaf_err[s];
+        a = (int64_t)ceil(p - (double)e);
+        b = (int64_t)floor(p + (double)e) + 1;
+        a = a < lo ? lo : a;
+        b = b > hi ? hi : b;
+        st.probes += 2;
+        if (!(a <= b && (a == lo || V.keys[a - 1] < q) && (b == hi || V.keys[b] >= q))) {
+            ++st.win_fb;
+            a = lo;
+            b = hi;
+        }
+    }
+    while (a < b) {
+        const int64_t mid = (a + b) >> 1;
+        ++st.probes;
+        if (V.keys[mid] >= q) b = mid;
+        else a = mid + 1;
+    }
+    return a;
+}
+
+// <= 9 Hilbert cells of one level covering the box, merged into inclusive key
+// ranges sorted by start.  Returns the number of ranges.
+__device__ int cover_ranges(double x0, double y0, double x1, double y1, const KeyScale& ks, int bits,
+                            uint64_t* lo, uint64_t* hi) {
+    if (!(x0 <= x1 && y0 <= y1)) return 0;
+    const uint32_t qx0 = quantize(x0, ks.x0, ks.sx, bits), qx1 = quantize(x1, ks.x0, ks.sx, bits);
+    const uint32_t qy0 = quantize(y0, ks.y0, ks.sy, bits), qy1 = quantize(y1, ks.y0, ks.sy, bits);
+    int l = 0;
+    while (l < bits && (((qx1 >> l) - (qx0 >> l)) > 2u || ((qy1 >> l) - (qy0 >> l)) > 2u)) ++l;
+    const int lev = bits - l;
+    int cnt = 0;
+    for (uint32_t a = qx0 >> l; a <= (qx1 >> l); ++a)
+        for (uint32_t c = qy0 >> l; c <= (qy1 >> l); ++c) {
+            const uint64_t code = lev > 0 ? (hilbert64(a << (32 - lev), c << (32 - lev)) >> (64 - 2 * lev)) : 0ull;
+            const uint64_t kl = 2 * l >= 64 ? 0ull : (code << (2 * l));
+            const uint64_t kh = kl | (2 * l >= 64 ? ~0ull : ((1ull << (2 * l)) - 1ull));
+            int k = cnt++;
+            while (k > 0 && lo[k - 1] > kl) {
+                lo[k] = lo[k - 1];
+                hi[k] = hi[k - 1];
+                --k;
+            }
+            lo[k] = kl;
+            hi[k] = kh;
+        }
+    int m = 0;
+    for (int k = 1; k < cnt; ++k) {
+        if (hi[m] != ~0ull && lo[k] == hi[m] + 1) hi[m] = hi[k];
+        else {
+            ++m;
+            lo[m] = lo[k];
+            hi[m] = hi[k];
+        }
+    }
+    return cnt ? m + 1 : 0;
+}
+
+// Per query box: rank ranges of its cover, total scan length and the number
+// of work items (chunks of kChunk positions).
+template <class K>
+__global__ __launch_bounds__(256) void cover_kernel(const double4* __restrict__ qbox, int64_t nq,
+                                                   const double4* __restrict__ ext, LearnedView<K> V,
+                                                   int2* __restrict__ ranges, int32_t* __restrict__ nranges,
+                                                   int32_t* __restrict__ n_items, unsigned long long* __restrict__ stat,
+                                                   const int32_t* __restrict__ geom, const int32_t* __restrict__ part,
+                                                   const int32_t* __restrict__ ring, int2* __restrict__ coord_range) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < nq && coord_range) coord_range[q] = make_int2(ring[part[geom[q]]], ring[part[geom[q + 1]]]);
+    LookupStats st = {0, 0, 0};
+    unsigned long long scan = 0;
+    if (q < nq) {
+        const int bits = 4 * (int)sizeof(K);
+        const KeyScale ks = key_scale(ext, bits);
+        const double4 b = qbox[q];
+        uint64_t lo[kMaxRanges], hi[kMaxRanges];
+        const int nr = cover_ranges(b.x, b.y, b.z, b.w, ks, bits, lo, hi);
+        int64_t total = 0;
+        for (int k = 0; k < nr; ++k) {
+            const int64_t r0 = learned_lower_bound<K>((K)lo[k], V, st);
+            const bool top = hi[k] >= (sizeof(K) == 4 ? 0xFFFFFFFFull : ~0ull);
+            const int64_t r1 = top ? V.n : learned_lower_bound<K>((K)(hi[k] + 1), V, st);
+            ranges[q * kMaxRanges + k] = make_int2((int32_t)r0, (int32_t)r1);
+            total += r1 - r0;
+        }
+        nranges[q] = nr;
+        n_items[q] = (int32_t)((total + kChunk - 1) / kChunk);
+        scan = (unsigned long long)total;
+    }
+    // block-reduce the counters: [0] lookups probes, [1] leaf fallbacks, [2] window fallbacks, [3] scanned
+    unsigned long long v[4] = {st.probes, st.leaf_fb, st.win_fb, scan};
+    for (int k = 0; k < 4; ++k) {
+        unsigned long long x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&stat[k], x);
+    }
+}
+
+template <class K>
+__global__ __launch_bounds__(256) void lookup_batch_kernel(const K* __restrict__ qk, int64_t m, LearnedView<K> V,
+                                                          int64_t* __restrict__ out,
+                                                          unsigned long long* __restrict__ stat) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    LookupStats st = {0, 0, 0};
+    if (i < m) out[i] = learned_lower_bound<K>(qk[i], V, st);
+    unsigned long long v[3] = {st.probes, st.leaf_fb, st.win_fb};
+    for (int k = 0; k < 3; ++k) {
+        unsigned long long x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&stat[k], x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 edge step with escalation
+// ---------------------------------------------------------------------------
+// Returns 1 when the edge must be re-evaluated exactly; otherwise the fp32
+// decision is exact and `parity` has been updated.
+__device__ __forceinline__ int edge_step_f32(float px, float py, float ax, float ay, float bx, float by, float E,
+                                             int& parity, unsigned& n_orient) {
+    if (ay == py || by == py) return 1;
+    const bool above_a = ay > py, above_b = by > py;
+    if (above_a == above_b) return 0;
+    if (px < fminf(ax, bx)) {
+        parity ^= 1;
+        return 0;
+    }
+    if (px > fmaxf(ax, bx)) return 0;
+    ++n_orient;
+    const float d = (ax - px) * (by - py) - (ay - py) * (bx - px);
+    if (d > E) {
+        if (above_b) parity ^= 1;
+        return 0;
+    }
+    if (d < -E) {
+        if (!above_b) parity ^= 1;
+        return 0;
+    }
+    return 1;
+}
+
+// Forward error bound of the fp32 determinant on coordinates recentred on the
+// polygon's box centre, all of magnitude <= M (vertices and, for candidates,
+// the point lie in the box).  Inputs a' = fl32(fl64(x - c)): |a' - (x - c)| <=
+// d*M + eta with d = 2^-24 + 2^-53 + 2^-77, eta = 2^-149 (subnormals).  With
+// e = 2(d*M + eta) per coordinate difference and g4 = (1 + 2^-24)^4 - 1 for the
+// four roundings on each product path:
+//   |det' - det| <= 2 g4 (2M + e)^2 + 8 M e + 2 e^2,
+// evaluated in fp64, inflated by 2^-40 and rounded up to fp32.
+__device__ __forceinline__ float fp32_det_bound(double M) {
+    const double eps = 5.9604644775390625e-08;  // 2^-24
+    const double d = eps + 1.1102230246251565e-16 + 6.617444900424222e-24;
+    const double eta = 1.401298464324817e-45;
+    const double e = 2.0 * (d * M + eta);
+    const double g4 = 4.0 * eps + 6.0 * eps * eps + 4.0 * eps * eps * eps + eps * eps * eps * eps;
+    double E = 2.0 * g4 * (2.0 * M + e) * (2.0 * M + e) + 8.0 * M * e + 2.0 * e * e;
+    E *= 1.0 + 9.094947017729282e-13;
+    float Ef = (float)E;
+    if ((double)Ef < E) Ef = nextafterf(Ef, INFINITY);
+    return Ef;
+}
+
+struct RefineView {
+    const double* x;
+    const double* y;
+    const int32_t* geom;
+    const int32_t* part;
+    const int32_t* ring;
+    const uint8_t* flags;
+};
+
+enum { kLModePip = 0, kLModeBox = 1 };
+
+template <int PRED>
+__device__ __forceinline__ bool loc_pred(int loc) {
+    if (PRED == VS_PRED_WITHIN) return loc == 2;
+    if (PRED == VS_PRED_TOUCHES) return loc == 1;
+    return loc != 0;
+}
+
+// Persistent polygon-major join.  MODE kLModePip: query q is polygon row q
+// (qbox = its bounds), output pairs (point, q).  MODE kLModeBox: query q is a
+// box, output pairs (q, point).
+template <int MODE, int PRED>
+__global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
+    int64_t nq, const int32_t* __restrict__ item_off, const long long* __restrict__ total_items,
+    const int32_t* __restrict__ item_q, int64_t item_cap, const int2* __restrict__ coord_range,
+    const int2* __restrict__ ranges, const int32_t* __restrict__ nranges, const double* __restrict__ sx,
+    const double* __restrict__ sy, const int32_t* __restrict__ perm, const double4* __restrict__ qbox,
+    RefineView R, int2* __restrict__ stage, int64_t capacity, JoinCtl* __restrict__ ctl,
+    int32_t* __restrict__ counts) {
+    __shared__ int s_item, s_q, s_chunk, s_ncand, s_nout;
+    __shared__ unsigned long long s_base;
+    __shared__ int2 s_rng[kMaxRanges];
+    __shared__ int s_nr;
+    __shared__ int32_t cand[kChunk];
+    __shared__ int2 outb[kChunk];
+    __shared__ float2 Vt[kVmax + 1];
+    __shared__ uint8_t Ft[kVmax + 1];
+    const int tid = threadIdx.x;
+    const long long T = *total_items;
+    unsigned long long st_cand = 0, st_cverts = 0, st_orient = 0, st_esc = 0, st_escpairs = 0;
+    for (;;) {
+        if (tid == 0) {
+            const int item = (int)atomicAdd(&ctl->queue[0], 1u);
+            s_item = item;
+            if ((long long)item < T) {
+                int64_t lo;
+                if (item < item_cap) {
+                    lo = item_q[item];
+                } else {  // beyond the map: last q with item_off[q] <= item
+                    lo = 0;
+                    int64_t hi = nq - 1;
+                    while (lo < hi) {
+                        const int64_t mid = (lo + hi + 1) >> 1;
+                        if (item_off[mid] <= item) lo = mid;
+                        else hi = mid - 1;
+                    }
+                }
+                s_q = (int)lo;
+                s_chunk = item - item_off[lo];
+                s_nr = nranges[lo];
+            }
+            s_ncand = 0;
+            s_nout = 0;
+        }
+        __syncthreads();
+        if ((long long)s_item >= T) break;
+        const int q = s_q;
+        if (tid < s_nr) s_rng[tid] = ranges[(int64_t)q * kMaxRanges + tid];
+        __syncthreads();
+        const double4 box = qbox[q];
+        // 1. scan this chunk of the cover's positions with the exact box test
+        for (int k = tid; k < kChunk; k += kLJThreads) {
+            int off = s_chunk * kChunk + k;
+            int r = 0;
+            while (r < s_nr && off >= s_rng[r].y - s_rng[r].x) {
+                off -= s_rng[r].y - s_rng[r].x;
+                ++r;
+            }
+            if (r < s_nr) {
+                const int32_t pos = s_rng[r].x + off;
+                const double px = sx[pos], py = sy[pos];
+                if (px >= box.x && px <= box.z && py >= box.y && py <= box.w) {
+                    const int slot = atomicAdd(&s_ncand, 1);
+                    cand[slot] = pos;
+                }
+            }
+        }
+        __syncthreads();
+        const int ncand = s_ncand;
+        if (MODE == kLModeBox) {
+            for (int k = tid; k < ncand; k += kLJThreads) outb[k] = make_int2(q, perm[cand[k]]);
+            if (tid == 0) {
+                if (ncand) atomicAdd(&counts[q], ncand);
+                s_nout = ncand;
+            }
+            __syncthreads();
+        } else if (ncand > 0) {
+            // 2. refine the candidates against the polygon, vertices staged in LDS
+            const int2 cr = coord_range[q];
+            const int32_t c0 = cr.x, c1 = cr.y;
+            const double cx = 0.5 * (box.x + box.z), cy = 0.5 * (box.y + box.w);
+            double M = fmax(fmax(box.z - cx, cx - box.x), fmax(box.w - cy, cy - box.y));
+            M *= 1.0 + 4.440892098500626e-16;
+            const float E = fp32_det_bound(M);
+            int loc[kCandPerThread], par[kCandPerThread];
+            bool done[kCandPerThread], esc[kCandPerThread];
+            float pu[kCandPerThread], pv[kCandPerThread];
+            double pxd[kCandPerThread], pyd[kCandPerThread];
+#pragma unroll
+            for (int j = 0; j < kCandPerThread; ++j) {
+                const int k = tid + j * kLJThreads;
+                loc[j] = 0;
+                par[j] = 0;
+                esc[j] = false;
+                done[j] = k >= ncand;
+                if (k < ncand) {
+                    pxd[j] = sx[cand[k]];
+                    pyd[j] = sy[cand[k]];
+                    pu[j] = (float)(pxd[j] - cx);
+                    pv[j] = (float)(pyd[j] - cy);
+                } else {
+                    pxd[j] = pyd[j] = 0.0;
+                    pu[j] = pv[j] = 0.0f;
+                }
+            }
+            unsigned n_orient = 0;
+            for (int32_t b0 = c0; b0 < c1; b0 += kVmax) {
+                const int32_t b1 = (b0 + kVmax < c1) ? b0 + kVmax : c1;
+                __syncthreads();
+                for (int32_t c = b0 - 1 + tid; c < b1; c += kLJThreads) {
+                    if (c < c0) continue;
+                    Vt[c - b0 + 1] = make_float2((float)(R.x[c] - cx), (float)(R.y[c] - cy));
+                    Ft[c - b0 + 1] = R.flags[c];
+                }
+                __syncthreads();
+                for (int32_t c = b0; c < b1; ++c) {
+                    const int f = Ft[c - b0 + 1];
+                    if (f & 2) {
+#pragma unroll
+                        for (int j = 0; j < kCandPerThread; ++j) {
+                            if (!done[j] && par[j]) {
+                                loc[j] = 2;
+                                done[j] = true;
+                            }
+                            par[j] = 0;
+                        }
+                    }
+                    if (f & 1) continue;
+                    const float2 a = Vt[c - b0], b = Vt[c - b0 + 1];
+#pragma unroll
+                    for (int j = 0; j < kCandPerThread; ++j) {
+                        if (done[j]) continue;
+                        if (edge_step_f32(pu[j], pv[j], a.x, a.y, b.x, b.y, E, par[j], n_orient)) {
+                            ++st_esc;
+                            esc[j] = true;
+                            if (edge_step_f64(pxd[j], pyd[j], R.x[c - 1], R.y[c - 1], R.x[c], R.y[c], par[j])) {
+                                loc[j] = 1;
+                                done[j] = true;
+                            }
+                        }
+                    }
+                }
+            }
+            st_orient += n_orient;
+#pragma unroll
+            for (int j = 0; j < kCandPerThread; ++j) {
+                const int k = tid + j * kLJThreads;
+                if (k >= ncand) continue;
+                if (!done[j] && par[j]) loc[j] = 2;
+                st_escpairs += esc[j] ? 1 : 0;
+                if (loc_pred<PRED>(loc[j])) {
+                    const int32_t orig = perm[cand[k]];
+                    const int slot = atomicAdd(&s_nout, 1);
+                    outb[slot] = make_int2(orig, q);
+                    atomicAdd(&counts[orig], 1);
+                }
+            }
+            if (tid == 0) {
+                st_cand += (unsigned long long)ncand;
+                st_cverts += (unsigned long long)ncand * (unsigned long long)(c1 - c0);
+            }
+            __syncthreads();
+        }
+        // 3. append this item's pairs to the stage with one atomic
+        if (tid == 0) s_base = s_nout ? atomicAdd(&ctl->staged, (unsigned long long)s_nout) : 0ull;
+        __syncthreads();
+        const int nout = s_nout;
+        for (int k = tid; k < nout; k += kLJThreads) {
+            const unsigned long long slot = s_base + (unsigned long long)k;
+            if (slot < (unsigned long long)capacity) stage[slot] = outb[k];
+        }
+        __syncthreads();
+    }
+    // counters: [0] candidates, [1] candidate vertices, [2] fp32 orientation
+    // evaluations, [3] escalated edges; pairs with an escalation -> queue[1]
+    unsigned long long v[5] = {st_cand, st_cverts, st_orient, st_esc, st_escpairs};
+    for (int k = 0; k < 5; ++k) {
+        unsigned long long x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((tid & 63) == 0 && x) {
+            if (k < 4) atomicAdd(&ctl->stat[k], x);
+            else atomicAdd(&ctl->escalated_pairs, x);
+        }
+    }
+}
+
+// item_q[k] = query owning item k, for the first `cap` items (items beyond
+// are located by binary search over item_off).
+__global__ __launch_bounds__(256) void item_map_kernel(const int32_t* __restrict__ item_off,
+                                                      const int32_t* __restrict__ n_items, int64_t nq,
+                                                      int32_t* __restrict__ item_q, int64_t cap) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    const int64_t o = item_off[q];
+    for (int64_t k = 0; k < n_items[q] && o + k < cap; ++k) item_q[o + k] = (int32_t)q;
+}
+
+__global__ void total_items_kernel(const int32_t* __restrict__ off, int64_t nq, const int32_t* __restrict__ cnt,
+                                   long long* __restrict__ total) {
+    if (threadIdx.x == 0) *total = nq > 0 ? (long long)off[nq - 1] + cnt[nq - 1] : 0;
+}
+
+__global__ void publish_learned_kernel(const JoinCtl* __restrict__ ctl, const unsigned long long* __restrict__ cst,
+                                       const int32_t* __restrict__ meta, int64_t* __restrict__ result) {
+    if (threadIdx.x == 0) {
+        result[4] = (int64_t)ctl->stat[2];
+        result[5] = (int64_t)ctl->stat[3];
+        result[6] = (int64_t)ctl->escalated_pairs;
+        result[7] = (int64_t)cst[0];   // lookup probes
+        result[8] = (int64_t)cst[1];   // MLP leaf-window fallbacks
+        result[9] = (int64_t)cst[2];   // leaf error-window fallbacks
+        result[10] = (int64_t)cst[3];  // positions scanned
+        result[11] = meta ? (int64_t)meta[0] : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+template <class K>
+struct IndexBufs {
+    K* keys;
+    int32_t* perm;
+    double* sx;
+    double* sy;
+    double4* ext;
+    float* mlp;
+    float* losses;
+    K* leaf_key;
+    int32_t* leaf_err;
+    int32_t* meta;
+    void* build_ws;
+};
+
+template <class K>
+IndexBufs<K> carve_index(Carver& c, int64_t n, int S) {
+    IndexBufs<K> B;
+    B.keys = c.take<K>(n);
+    B.perm = c.take<int32_t>(n);
+    B.sx = c.take<double>(n);
+    B.sy = c.take<double>(n);
+    B.ext = c.take<double4>(1);
+    B.mlp = c.take<float>(kNParams);
+    B.losses = c.take<float>(2);
+    B.leaf_key = c.take<K>(S + 1);
+    B.leaf_err = c.take<int32_t>(S);
+    B.meta = c.take<int32_t>(4);
+    B.build_ws = c.take<char>(sizeof(K) == 4 ? point_index_workspace_bytes_u32(n) : point_index_workspace_bytes_u64(n));
+    return B;
+}
+
+inline int leaves_for(int64_t n, int leaf_size) {
+    const int64_t S = n > 1 ? (n - 1 + leaf_size - 1) / leaf_size : 1;
+    return (int)(S < 1 ? 1 : S);
+}
+
+__global__ void zero_i32_kernel(int32_t* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+template <class K>
+int index_build_t(const double* px, const double* py, int64_t n, int S, int iters, float lr, const IndexBufs<K>& B,
+                  hipStream_t s) {
+    if (sizeof(K) == 4)
+        VS_TRY(point_index_build_u32(px, py, n, reinterpret_cast<uint32_t*>(B.keys), B.perm, B.sx, B.sy, B.ext,
+                                     B.build_ws, s));
+    else
+        VS_TRY(point_index_build_u64(px, py, n, reinterpret_cast<uint64_t*>(B.keys), B.perm, B.sx, B.sy, B.ext,
+                                     B.build_ws, s));
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, B.meta, 4);
+    VS_LAUNCH_CHECK();
+    if (n < 2) return VS_OK;
+    hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(256), 0, s, B.keys, n, iters, lr, B.mlp, B.losses);
+    VS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(leaf_keys_kernel<K>, dim3((unsigned)ceil_div(S + 1, 256)), dim3(256), 0, s, B.keys, n, S,
+                       B.leaf_key);
+    VS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(leaf_err_kernel<K>, dim3((unsigned)ceil_div(S, 4)), dim3(256), 0, s, B.keys, n, S, B.leaf_key,
+                       B.mlp, B.leaf_err, B.meta);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+template <class K>
+LearnedView<K> view_of(const IndexBufs<K>& B, int64_t n, int S, int mode) {
+    LearnedView<K> V;
+    V.keys = B.keys;
+    V.n = n;
+    V.mlp = B.mlp;
+    V.leaf_key = B.leaf_key;
+    V.leaf_err = B.leaf_err;
+    V.meta = B.meta;
+    V.S = S;
+    V.mode = n < 2 ? 2 : mode;
+    return V;
+}
+
+struct QueryBufs {
+    int32_t* item_q;
+    int64_t item_cap;
+    int2* coord_range;
+    int2* ranges;
+    int32_t* nranges;
+    int32_t* n_items;
+    int32_t* item_off;
+    long long* total_items;
+    unsigned long long* cstat;
+    void* scan_ws;
+    int2* stage;
+    int32_t* counts;
+    int32_t* offsets;
+    int32_t* cursor;
+    void* fin_scan_ws;
+    JoinCtl* ctl;
+};
+
+// Items map: room for 4 items per query plus one per kChunk/4 left rows covers
+// typical covers; items beyond the map fall back to a binary search.
+inline int64_t item_cap_for(int64_t nq, int64_t n_left) { return 4 * nq + (4 * n_left) / kChunk + 1024; }
+
+QueryBufs carve_query(Carver& c, int64_t nq, int64_t n_left, int64_t capacity) {
+    QueryBufs Q;
+    Q.item_cap = item_cap_for(nq, n_left);
+    Q.item_q = c.take<int32_t>(Q.item_cap);
+    Q.coord_range = c.take<int2>(nq);
+    Q.ranges = c.take<int2>(nq * kMaxRanges);
+    Q.nranges = c.take<int32_t>(nq);
+    Q.n_items = c.take<int32_t>(nq);
+    Q.item_off = c.take<int32_t>(nq);
+    Q.total_items = c.take<long long>(1);
+    Q.cstat = c.take<unsigned long long>(4);
+    Q.scan_ws = c.take<char>(scan_workspace_bytes(nq));
+    Q.stage = c.take<int2>(capacity);
+    Q.counts = c.take<int32_t>(n_left);
+    Q.offsets = c.take<int32_t>(n_left);
+    Q.cursor = c.take<int32_t>(n_left);
+    Q.fin_scan_ws = c.take<char>(scan_workspace_bytes(n_left));
+    Q.ctl = c.take<JoinCtl>(1);
+    return Q;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+template <class K>
+int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const LearnedView<K>& V,
+                    const IndexBufs<K>& B, const RefineView& R, int64_t n_left, int64_t capacity,
+                    const QueryBufs& Q, int64_t* out_l, int64_t* out_r, int64_t* d_result, hipStream_t s) {
+    ctl_reset(Q.ctl, s);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, Q.cstat, 4);
+    VS_LAUNCH_CHECK();
+    fill_i32(Q.counts, n_left, 0, s);
+    VS_LAUNCH_CHECK();
+    if (nq > 0) {
+        hipLaunchKernelGGL(cover_kernel<K>, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, qbox, nq, B.ext, V,
+                           Q.ranges, Q.nranges, Q.n_items, Q.cstat, R.geom, R.part, R.ring,
+                           mode == kLModePip ? Q.coord_range : nullptr);
+        VS_LAUNCH_CHECK();
+        VS_TRY(exclusive_scan_i32(Q.n_items, Q.item_off, nq, nullptr, Q.scan_ws, s));
+        hipLaunchKernelGGL(item_map_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, Q.item_off, Q.n_items,
+                           nq, Q.item_q, Q.item_cap);
+        VS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(total_items_kernel, dim3(1), dim3(64), 0, s, Q.item_off, nq, Q.n_items, Q.total_items);
+        VS_LAUNCH_CHECK();
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            int v = 0;
+            if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
+        }
+        const int grid = cus * 4;
+        VS_TRY(timing_begin_public(s));
+#define VS_LJ(M, P)                                                                                          \
+    hipLaunchKernelGGL((learned_join_kernel<M, P>), dim3(grid), dim3(kLJThreads), 0, s, nq, Q.item_off,       \
+                       Q.total_items, Q.item_q, Q.item_cap, Q.coord_range, Q.ranges, Q.nranges, B.sx, B.sy, B.perm, \
+                       qbox, R, Q.stage, capacity, Q.ctl, Q.counts)
+        if (mode == kLModeBox) VS_LJ(kLModeBox, VS_PRED_INTERSECTS);
+        else if (pred == VS_PRED_WITHIN) VS_LJ(kLModePip, VS_PRED_WITHIN);
+        else if (pred == VS_PRED_TOUCHES) VS_LJ(kLModePip, VS_PRED_TOUCHES);
+        else VS_LJ(kLModePip, VS_PRED_INTERSECTS);
+#undef VS_LJ
+        VS_LAUNCH_CHECK();
+        VS_TRY(timing_end_public(s));
+    }
+    VS_TRY(finish_pairs_raw(Q.stage, Q.counts, Q.offsets, Q.cursor, Q.fin_scan_ws, Q.ctl, n_left, capacity, out_l,
+                            out_r, d_result, s));
+    hipLaunchKernelGGL(publish_learned_kernel, dim3(1), dim3(64), 0, s, Q.ctl, Q.cstat, B.meta, d_result);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+}  // namespace vs
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int32_t vs_learned_leaf_count(int64_t n_points, int32_t leaf_size) {
+    return vs::leaves_for(n_points, leaf_size > 0 ? leaf_size : 512);
+}
+
+int64_t vs_learned_index_bytes(int64_t n_points, int32_t key_bytes, int32_t n_leaves) {
+    vs::Carver c(nullptr);
+    if (key_bytes == 4) vs::carve_index<uint32_t>(c, n_points, n_leaves);
+    else vs::carve_index<uint64_t>(c, n_points, n_leaves);
+    return c.bytes;
+}
+
+int vs_learned_index_build(const double* px, const double* py, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
+                           int32_t train_iters, float lr, void* index, int64_t index_bytes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(key_bytes == 4 || key_bytes == 8, "vs_learned_index_build: key_bytes must be 4 or 8");
+    VS_REQUIRE(n_points >= 0 && n_points < (int64_t)INT32_MAX, "vs_learned_index_build: n_points out of range");
+    VS_REQUIRE(n_leaves >= 1, "vs_learned_index_build: n_leaves must be >= 1");
+    VS_REQUIRE(train_iters >= 0 && train_iters <= 100000, "vs_learned_index_build: bad train_iters");
+    VS_REQUIRE(index_bytes >= vs_learned_index_bytes(n_points, key_bytes, n_leaves),
+               "vs_learned_index_build: index buffer too small");
+    vs::Carver c(index);
+    hipStream_t s = vs::stream_of(stream);
+    if (key_bytes == 4) {
+        const auto B = vs::carve_index<uint32_t>(c, n_points, n_leaves);
+        return vs::index_build_t<uint32_t>(px, py, n_points, n_leaves, train_iters, lr, B, s);
+    }
+    const auto B = vs::carve_index<uint64_t>(c, n_points, n_leaves);
+    return vs::index_build_t<uint64_t>(px, py, n_points, n_leaves, train_iters, lr, B, s);
+}
+
+// Offsets (bytes from the index base) of the index's arrays, for callers that
+// read them (tests, persistence): keys, perm, sorted x, sorted y, extent, mlp
+// params, losses, leaf keys, leaf errors, meta.
+int vs_learned_index_layout(int64_t n_points, int32_t key_bytes, int32_t n_leaves, int64_t* offsets10) {
+    vs::clear_error();
+    VS_REQUIRE(offsets10 != nullptr, "vs_learned_index_layout: null output");
+    VS_REQUIRE(key_bytes == 4 || key_bytes == 8, "vs_learned_index_layout: key_bytes must be 4 or 8");
+    char* base = reinterpret_cast<char*>(4096);  // any non-null base: offsets are relative
+    vs::Carver c(base);
+    if (key_bytes == 4) {
+        const auto B = vs::carve_index<uint32_t>(c, n_points, n_leaves);
+        const void* p[10] = {B.keys, B.perm, B.sx, B.sy, B.ext, B.mlp, B.losses, B.leaf_key, B.leaf_err, B.meta};
+        for (int k = 0; k < 10; ++k) offsets10[k] = (const char*)p[k] - base;
+    } else {
+        const auto B = vs::carve_index<uint64_t>(c, n_points, n_leaves);
+        const void* p[10] = {B.keys, B.perm, B.sx, B.sy, B.ext, B.mlp, B.losses, B.leaf_key, B.leaf_err, B.meta};
+        for (int k = 0; k < 10; ++k) offsets10[k] = (const char*)p[k] - base;
+    }
+    return VS_OK;
+}
+
+int vs_learned_lower_bound(const void* index, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
+                           const void* query_keys, int64_t m, int32_t lookup_mode, int64_t* out, uint64_t* d_stat3,
+                           void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(key_bytes == 4 || key_bytes == 8, "vs_learned_lower_bound: key_bytes must be 4 or 8");
+    VS_REQUIRE(lookup_mode >= 0 && lookup_mode <= 2, "vs_learned_lower_bound: lookup_mode in {0, 1, 2}");
+    vs::Carver c(const_cast<void*>(index));
+    hipStream_t s = vs::stream_of(stream);
+    if (m == 0) return VS_OK;
+    if (key_bytes == 4) {
+        const auto B = vs::carve_index<uint32_t>(c, n_points, n_leaves);
+        const auto V = vs::view_of<uint32_t>(B, n_points, n_leaves, lookup_mode);
+        hipLaunchKernelGGL(vs::lookup_batch_kernel<uint32_t>, dim3((unsigned)vs::ceil_div(m, 256)), dim3(256), 0, s,
+                           reinterpret_cast<const uint32_t*>(query_keys), m, V, out,
+                           reinterpret_cast<unsigned long long*>(d_stat3));
+    } else {
+        const auto B = vs::carve_index<uint64_t>(c, n_points, n_leaves);
+        const auto V = vs::view_of<uint64_t>(B, n_points, n_leaves, lookup_mode);
+        hipLaunchKernelGGL(vs::lookup_batch_kernel<uint64_t>, dim3((unsigned)vs::ceil_div(m, 256)), dim3(256), 0, s,
+                           reinterpret_cast<const uint64_t*>(query_keys), m, V, out,
+                           reinterpret_cast<unsigned long long*>(d_stat3));
+    }
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int vs_mlp_loss_grad(const float* params, const float* u, const float* t, int64_t n, float* grad_out, float* loss_out,
+                     void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n >= 0, "vs_mlp_loss_grad: negative n");
+    hipLaunchKernelGGL(vs::mlp_loss_grad_kernel, dim3(1), dim3(256), 0, vs::stream_of(stream), params, u, t, n,
+                       grad_out, loss_out);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int64_t vs_learned_query_boxes_workspace_bytes(int64_t n_boxes, int64_t capacity) {
+    vs::Carver c(nullptr);
+    vs::carve_query(c, n_boxes, n_boxes, capacity);
+    return c.bytes;
+}
+
+int vs_learned_query_boxes(const void* index, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
+                           const double* boxes, int64_t n_boxes, int32_t lookup_mode, int64_t capacity,
+                           int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                           int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(key_bytes == 4 || key_bytes == 8, "vs_learned_query_boxes: key_bytes must be 4 or 8");
+    VS_REQUIRE(n_boxes >= 0 && n_boxes < (int64_t)INT32_MAX, "vs_learned_query_boxes: n_boxes out of range");
+    VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_learned_query_boxes: capacity out of range");
+    VS_REQUIRE(lookup_mode >= 0 && lookup_mode <= 2, "vs_learned_query_boxes: lookup_mode in {0, 1, 2}");
+    VS_REQUIRE(workspace_bytes >= vs_learned_query_boxes_workspace_bytes(n_boxes, capacity),
+               "vs_learned_query_boxes: workspace too small");
+    VS_REQUIRE(d_result != nullptr, "vs_learned_query_boxes: null result");
+    vs::Carver ci(const_cast<void*>(index));
+    vs::Carver cw(workspace);
+    const vs::QueryBufs Q = vs::carve_query(cw, n_boxes, n_boxes, capacity);
+    const vs::RefineView R = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipStream_t s = vs::stream_of(stream);
+    const double4* qb = reinterpret_cast<const double4*>(boxes);
+    if (key_bytes == 4) {
+        const auto B = vs::carve_index<uint32_t>(ci, n_points, n_leaves);
+        return vs::learned_query_t<uint32_t>(vs::kLModeBox, VS_PRED_INTERSECTS, qb, n_boxes,
+                                             vs::view_of<uint32_t>(B, n_points, n_leaves, lookup_mode), B, R, n_boxes,
+                                             capacity, Q, out_left, out_right, d_result, s);
+    }
+    const auto B = vs::carve_index<uint64_t>(ci, n_points, n_leaves);
+    return vs::learned_query_t<uint64_t>(vs::kLModeBox, VS_PRED_INTERSECTS, qb, n_boxes,
+                                         vs::view_of<uint64_t>(B, n_points, n_leaves, lookup_mode), B, R, n_boxes,
+                                         capacity, Q, out_left, out_right, d_result, s);
+}
+
+int64_t vs_sjoin_points_polygons_learned_workspace_bytes(int64_t n_points, int64_t n_polys, int64_t n_coords,
+                                                         int64_t capacity) {
+    const int S = vs::leaves_for(n_points, 512);
+    vs::Carver c(nullptr);
+    vs::carve_index<uint32_t>(c, n_points, S);
+    c.take<uint8_t>(n_coords);
+    vs::carve_query(c, n_polys, n_points, capacity);
+    return c.bytes;
+}
+
+int vs_sjoin_points_polygons_learned(const double* px, const double* py, int64_t n_points, const double* x,
+                                     const double* y, const int32_t* geom_offsets, const int32_t* part_offsets,
+                                     const int32_t* ring_offsets, int64_t n_polys, int64_t n_parts, int64_t n_rings,
+                                     int64_t n_coords, const double* poly_bounds, int32_t predicate,
+                                     int32_t train_iters, int32_t lookup_mode, int64_t capacity, int64_t* out_left,
+                                     int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
+                                     void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n_points >= 0 && n_points < (int64_t)INT32_MAX, "learned join: n_points out of range");
+    VS_REQUIRE(n_polys >= 0 && n_polys < (int64_t)INT32_MAX, "learned join: n_polys out of range");
+    VS_REQUIRE(n_coords >= 0 && n_coords < (int64_t)INT32_MAX, "learned join: n_coords out of range");
+    VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "learned join: capacity out of range");
+    VS_REQUIRE(predicate == VS_PRED_WITHIN || predicate == VS_PRED_INTERSECTS || predicate == VS_PRED_COVERED_BY ||
+                   predicate == VS_PRED_TOUCHES,
+               "learned join: unsupported predicate %d", predicate);
+    VS_REQUIRE(lookup_mode >= 0 && lookup_mode <= 2, "learned join: lookup_mode in {0, 1, 2}");
+    VS_REQUIRE(train_iters >= 0 && train_iters <= 100000, "learned join: bad train_iters");
+    VS_REQUIRE(d_result != nullptr, "learned join: null result");
+    VS_REQUIRE(workspace_bytes >= vs_sjoin_points_polygons_learned_workspace_bytes(n_points, n_polys, n_coords, capacity),
+               "learned join: workspace too small");
+    const int S = vs::leaves_for(n_points, 512);
+    vs::Carver c(workspace);
+    const auto B = vs::carve_index<uint32_t>(c, n_points, S);
+    uint8_t* flags = c.take<uint8_t>(n_coords);
+    const vs::QueryBufs Q = vs::carve_query(c, n_polys, n_points, capacity);
+    hipStream_t s = vs::stream_of(stream);
+    VS_TRY(vs::index_build_t<uint32_t>(px, py, n_points, S, train_iters, 0.01f, B, s));
+    VS_TRY(vs::build_refine_flags(part_offsets, ring_offsets, n_parts, n_rings, n_coords, flags, s));
+    const vs::RefineView R = {x, y, geom_offsets, part_offsets, ring_offsets, flags};
+    return vs::learned_query_t<uint32_t>(vs::kLModePip, predicate, reinterpret_cast<const double4*>(poly_bounds),
+                                         n_polys, vs::view_of<uint32_t>(B, n_points, S, lookup_mode), B, R, n_points,
+                                         capacity, Q, out_left, out_right, d_result, s);
+}
+
+}  // extern "C"

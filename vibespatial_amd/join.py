@@ -163,3 +163,155 @@ def query_boxes(qboxes: torch.Tensor, tree: DeviceTree, capacity: int | None = N
 
 
 _ = _lib  # re-exported module for callers that need the raw binding
+
+
+# ---------------------------------------------------------------------------
+# learned + FP32 form
+# ---------------------------------------------------------------------------
+@dataclass
+class LearnedStats(JoinStats):
+    orient_tests: int = 0
+    escalated_edges: int = 0
+    escalated_pairs: int = 0
+    lookup_probes: int = 0
+    mlp_fallbacks: int = 0
+    window_fallbacks: int = 0
+    scanned: int = 0
+    mlp_leaf_error: int = 0
+
+    @property
+    def escalation_fraction(self) -> float:
+        return self.escalated_edges / max(1, self.orient_tests + self.escalated_edges)
+
+
+def _learned_stats(h, cap, retries) -> LearnedStats:
+    return LearnedStats(int(h[0]), int(h[1]), int(h[2]), cap, retries, int(h[3]), int(h[4]), int(h[5]), int(h[6]),
+                        int(h[7]), int(h[8]), int(h[9]), int(h[10]), int(h[11]))
+
+
+LOOKUP_MODES = {"learned": 0, "leaves": 1, "binary": 2}
+
+
+def sjoin_points_polygons_learned(Q: DevicePoints, P: DevicePolygons, bounds: torch.Tensor, predicate: str = "within",
+                                  capacity: int | None = None, train_iters: int = 64, lookup: str = "learned"):
+    """Learned + FP32 form of sjoin_points_polygons: same pairs, bit for bit."""
+    dev = require_gpu()
+    if predicate not in ("within", "intersects", "covered_by", "touches"):
+        raise NotImplementedError(f"point x polygon predicate {predicate!r}")
+    L = lib()
+    n = len(Q)
+    cap = int(capacity) if capacity is not None else max(1024, 2 * n)
+    retries = 0
+    while True:
+        ws = _ws(L.vs_sjoin_points_polygons_learned_workspace_bytes(n, len(P), P.n_coords, cap), dev)
+        out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+        out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+        res = torch.zeros(16, dtype=torch.int64, device=dev)
+        check(
+            L.vs_sjoin_points_polygons_learned(
+                ptr(Q.x), ptr(Q.y), n, *P.args(), len(P), P.n_parts, P.n_rings, P.n_coords, ptr(bounds),
+                PRED[predicate], train_iters, LOOKUP_MODES[lookup], cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws),
+                ws.numel(), stream(),
+            ),
+            "vs_sjoin_points_polygons_learned",
+        )
+        h = res.cpu().tolist()
+        if h[1] <= cap:
+            return out_l[: h[0]], out_r[: h[0]], _learned_stats(h, cap, retries)
+        if h[1] >= 2**31 - 1:
+            raise OverflowError(f"{h[1]} pairs exceed the int32 pair limit")
+        cap = int(h[1])
+        retries += 1
+
+
+class LearnedPointIndex:
+    """Learned rank index over Hilbert-sorted points (device buffers)."""
+
+    NAMES = ("keys", "perm", "sx", "sy", "extent", "mlp", "losses", "leaf_keys", "leaf_err", "meta")
+
+    def __init__(self, Q: DevicePoints, key_bytes: int = 8, leaf_size: int = 512, train_iters: int = 64,
+                 lr: float = 0.01):
+        dev = require_gpu()
+        L = lib()
+        self.n = len(Q)
+        self.key_bytes = key_bytes
+        self.n_leaves = int(L.vs_learned_leaf_count(self.n, leaf_size))
+        nbytes = int(L.vs_learned_index_bytes(self.n, key_bytes, self.n_leaves))
+        self.buf = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+        check(L.vs_learned_index_build(ptr(Q.x), ptr(Q.y), self.n, key_bytes, self.n_leaves, train_iters, lr,
+                                       ptr(self.buf), self.buf.numel(), stream()), "vs_learned_index_build")
+        off = (ctypes.c_int64 * 10)()
+        check(L.vs_learned_index_layout(self.n, key_bytes, self.n_leaves, off), "vs_learned_index_layout")
+        self.offsets = dict(zip(self.NAMES, list(off)))
+
+    def _view(self, name, dtype, count):
+        o = self.offsets[name]
+        itemsize = torch.empty(0, dtype=dtype).element_size()
+        return self.buf[o: o + count * itemsize].view(dtype)
+
+    @property
+    def keys(self) -> torch.Tensor:
+        return self._view("keys", torch.int64 if self.key_bytes == 8 else torch.int32, self.n)
+
+    @property
+    def perm(self) -> torch.Tensor:
+        return self._view("perm", torch.int32, self.n)
+
+    @property
+    def mlp_params(self) -> torch.Tensor:
+        return self._view("mlp", torch.float32, 97)
+
+    @property
+    def losses(self) -> torch.Tensor:
+        return self._view("losses", torch.float32, 2)
+
+    @property
+    def mlp_leaf_error(self) -> int:
+        return int(self._view("meta", torch.int32, 4)[0].item())
+
+    @property
+    def leaf_err(self) -> torch.Tensor:
+        return self._view("leaf_err", torch.int32, self.n_leaves)
+
+    def lower_bound(self, qkeys: torch.Tensor, lookup: str = "learned"):
+        dev = require_gpu()
+        m = int(qkeys.numel())
+        out = torch.empty(max(m, 1), dtype=torch.int64, device=dev)
+        st = torch.zeros(3, dtype=torch.int64, device=dev)
+        check(lib().vs_learned_lower_bound(ptr(self.buf), self.n, self.key_bytes, self.n_leaves, ptr(qkeys), m,
+                                           LOOKUP_MODES[lookup], ptr(out), ptr(st), stream()), "vs_learned_lower_bound")
+        s = st.cpu().tolist()
+        return out[:m], {"probes": s[0], "mlp_fallbacks": s[1], "window_fallbacks": s[2]}
+
+    def query_boxes(self, boxes: torch.Tensor, lookup: str = "learned", capacity: int | None = None):
+        """(box, point) pairs with the point in the box (inclusive), lexsorted."""
+        dev = require_gpu()
+        L = lib()
+        nb = int(boxes.shape[0])
+        cap = int(capacity) if capacity is not None else max(1024, 4 * nb)
+        retries = 0
+        while True:
+            ws = _ws(L.vs_learned_query_boxes_workspace_bytes(nb, cap), dev)
+            out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+            out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+            res = torch.zeros(16, dtype=torch.int64, device=dev)
+            check(L.vs_learned_query_boxes(ptr(self.buf), self.n, self.key_bytes, self.n_leaves, ptr(boxes), nb,
+                                           LOOKUP_MODES[lookup], cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws),
+                                           ws.numel(), stream()), "vs_learned_query_boxes")
+            h = res.cpu().tolist()
+            if h[1] <= cap:
+                return out_l[: h[0]], out_r[: h[0]], _learned_stats(h, cap, retries)
+            if h[1] >= 2**31 - 1:
+                raise OverflowError(f"{h[1]} pairs exceed the int32 pair limit")
+            cap = int(h[1])
+            retries += 1
+
+
+def mlp_loss_grad(params: torch.Tensor, u: torch.Tensor, t: torch.Tensor):
+    """(loss, grad[97]) of the rank MLP on (u, t), from the device kernels."""
+    dev = require_gpu()
+    g = torch.empty(97, dtype=torch.float32, device=dev)
+    loss = torch.empty(1, dtype=torch.float32, device=dev)
+    check(lib().vs_mlp_loss_grad(ptr(params), ptr(u), ptr(t), int(u.numel()), ptr(g), ptr(loss), stream()),
+          "vs_mlp_loss_grad")
+    return float(loss.item()), g

@@ -30,7 +30,7 @@ import numpy as np
 
 from .geometry import PointArray, PolygonArray
 
-__all__ = ["star_polygons", "c1", "c2", "gmm_points", "Workload"]
+__all__ = ["star_polygons", "c1", "c2", "c3", "gmm_points", "shared_edge_triangles", "Workload"]
 
 
 class Workload:
@@ -190,3 +190,74 @@ def c2(n_points: int = 10_000_000, n_polys: int = 100_000, extent: float = 1e5) 
     hole_frac = rg.uniform(0.1, 0.4, n_holes)
     polys = star_polygons(rg, cx, cy, R, nv, holes=holes, hole_vertices=hole_vertices, hole_frac=hole_frac)
     return Workload("c2", PointArray(x, y), polys, "within")
+
+
+def shared_edge_triangles(rng, polys: PolygonArray, owners: np.ndarray, height: float = 0.5):
+    """For each owner polygon, a triangle sharing one of its shell edges
+    exactly (same two vertices, reversed) with its apex on the outer side."""
+    tris = []
+    for q in owners:
+        c0 = int(polys.ring_offsets[polys.part_offsets[polys.geom_offsets[q]]])
+        c1 = int(polys.ring_offsets[polys.part_offsets[polys.geom_offsets[q]] + 1])
+        k = int(rng.integers(c0 + 1, c1))
+        ax, ay, bx, by = polys.x[k - 1], polys.y[k - 1], polys.x[k], polys.y[k]
+        # shell rings are counter-clockwise: the outer side is to the right of a->b
+        nx, ny = (by - ay), -(bx - ax)
+        mx, my = 0.5 * (ax + bx), 0.5 * (ay + by)
+        tris.append([(bx, by), (ax, ay), (mx + height * nx, my + height * ny), (bx, by)])
+    return tris
+
+
+def c3(n_points: int = 10_000_000, n_polys: int = 100_000, shift=(5.0e5, 3.0e6), on_edge_frac: float = 0.05,
+       shared_frac: float = 0.01, verify=None) -> Workload:
+    """C2's layout shifted to UTM-scale offsets with exact degeneracies:
+    on_edge_frac of the points sit on polygon vertices or on edge midpoints
+    that lie exactly on the edge, and shared_frac of the polygons are replaced
+    by triangles sharing an edge exactly with another polygon.
+
+    `verify(px, py, polys, rows) -> location codes` (the test oracle) keeps
+    only midpoints that are exactly on their edge; without it only vertices
+    are used."""
+    from .geometry import Polygon
+
+    base = c2(n_points=n_points, n_polys=n_polys)
+    P = base.polygons
+    X = P.x + shift[0]
+    Y = P.y + shift[1]
+    P = PolygonArray(X, Y, P.ring_offsets, P.part_offsets, P.geom_offsets)
+    rng = np.random.default_rng(4)
+    n_shared = int(round(shared_frac * n_polys))
+    if n_shared:
+        owners = rng.choice(n_polys - n_shared, n_shared, replace=False)
+        tris = shared_edge_triangles(rng, P, owners)
+        tail = PolygonArray.from_geometries([Polygon(t) for t in tris])
+        head_coords = int(P.ring_offsets[P.part_offsets[P.geom_offsets[n_polys - n_shared]]])
+        head_rings = int(P.part_offsets[P.geom_offsets[n_polys - n_shared]])
+        head_parts = int(P.geom_offsets[n_polys - n_shared])
+        x = np.concatenate([P.x[:head_coords], tail.x])
+        y = np.concatenate([P.y[:head_coords], tail.y])
+        ring = np.concatenate([P.ring_offsets[:head_rings + 1], tail.ring_offsets[1:] + head_coords])
+        part = np.concatenate([P.part_offsets[:head_parts + 1], tail.part_offsets[1:] + head_rings])
+        geom = np.concatenate([P.geom_offsets[:n_polys - n_shared + 1], tail.geom_offsets[1:] + head_parts])
+        P = PolygonArray(x, y, ring, part, geom)
+    px = base.points.x + shift[0]
+    py = base.points.y + shift[1]
+    n_on = int(round(on_edge_frac * n_points))
+    if n_on:
+        idx = rng.choice(n_points, n_on, replace=False)
+        rows = rng.integers(0, len(P), n_on)
+        c0 = P.ring_offsets[P.part_offsets[P.geom_offsets[rows]]]
+        c1 = P.ring_offsets[P.part_offsets[P.geom_offsets[rows + 1]]]
+        k = c0 + 1 + (rng.random(n_on) * (c1 - c0 - 1)).astype(np.int64)
+        k = np.minimum(k, c1 - 1)
+        vx, vy = P.x[k], P.y[k]
+        mx, my = 0.5 * (P.x[k - 1] + P.x[k]), 0.5 * (P.y[k - 1] + P.y[k])
+        use_mid = rng.random(n_on) < 0.5
+        if verify is not None:
+            codes = verify(np.where(use_mid, mx, vx), np.where(use_mid, my, vy), P, rows)
+            use_mid &= codes == 1
+        else:
+            use_mid[:] = False
+        px[idx] = np.where(use_mid, mx, vx)
+        py[idx] = np.where(use_mid, my, vy)
+    return Workload("c3", PointArray(px, py), P, "within", on_edge=n_on, shared=n_shared)
