@@ -173,6 +173,7 @@ def main(argv=None):
             "escalated_edges": st_l.escalated_edges,
             "escalation_fraction": st_l.escalation_fraction,
             "lookup_probes": st_l.lookup_probes,
+            "positions_scanned": st_l.scanned,
             "mlp_leaf_error": st_l.mlp_leaf_error,
             "plain_fp64": {"value": value_p, "ms_per_step": ms_p, "kernel_ms": kms_p,
                            "roofline_frac": ach_p / HBM_PEAK_GBS},

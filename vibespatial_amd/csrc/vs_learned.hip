@@ -51,15 +51,18 @@ int point_index_build_u64(const double* px, const double* py, int64_t n, uint64_
                           double* sy, double4* ext, void* ws, hipStream_t s);
 int build_refine_flags(const int32_t* part, const int32_t* ring, int64_t n_parts, int64_t n_rings, int64_t n_coords,
                        uint8_t* flags, hipStream_t s);
+int sort_pairs_u32(uint32_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
+int64_t sort_workspace_bytes_u32(int64_t n);
 
 constexpr int kHidden = 32;
 constexpr int kNParams = 3 * kHidden + 1;  // w1[32], b1[32], w2[32], b2
 constexpr int kSample = 2048;              // training keys (evenly spaced ranks)
 constexpr int kMaxRanges = 9;
-constexpr int kChunk = 512;                // scanned positions per work item
+constexpr int kChunk = 512;                // positions scanned per round
+constexpr int kItemRounds = 8;             // rounds per work item (items span kChunk * kItemRounds positions)
+constexpr int kItemPos = kChunk * kItemRounds;
 constexpr int kLJThreads = 256;
 constexpr int kCandPerThread = kChunk / kLJThreads;
-constexpr int kVmax = 2048;                // vertices staged per batch
 
 // ---------------------------------------------------------------------------
 // MLP: forward, backward, training
@@ -411,9 +414,10 @@ __global__ __launch_bounds__(256) void cover_kernel(const double4* __restrict__ 
                                                    int2* __restrict__ ranges, int32_t* __restrict__ nranges,
                                                    int32_t* __restrict__ n_items, unsigned long long* __restrict__ stat,
                                                    const int32_t* __restrict__ geom, const int32_t* __restrict__ part,
-                                                   const int32_t* __restrict__ ring, int2* __restrict__ coord_range) {
+                                                   const int32_t* __restrict__ ring, int4* __restrict__ poly_info) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q < nq && coord_range) coord_range[q] = make_int2(ring[part[geom[q]]], ring[part[geom[q + 1]]]);
+    if (q < nq && poly_info)
+        poly_info[q] = make_int4(ring[part[geom[q]]], ring[part[geom[q + 1]]], geom[q + 1] - geom[q], 0);
     LookupStats st = {0, 0, 0};
     unsigned long long scan = 0;
     if (q < nq) {
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(256) void cover_kernel(const double4* __restrict__ 
             total += r1 - r0;
         }
         nranges[q] = nr;
-        n_items[q] = (int32_t)((total + kChunk - 1) / kChunk);
+        n_items[q] = (int32_t)((total + kItemPos - 1) / kItemPos);
         scan = (unsigned long long)total;
     }
     // block-reduce the counters: [0] lookups probes, [1] leaf fallbacks, [2] window fallbacks, [3] scanned
@@ -525,28 +529,120 @@ __device__ __forceinline__ bool loc_pred(int loc) {
     return loc != 0;
 }
 
+constexpr int kEdgeCap = 512;  // edges staged in LDS per batch
+constexpr int kOutCap = 2048;  // per-block output buffer (pairs)
+
+// Runs every staged edge against this thread's candidates.
+__device__ __forceinline__ void refine_edges(const float4* __restrict__ Et, const int32_t* __restrict__ Ec, int ne,
+                                             const RefineView& R, const float* pu, const float* pv, const double* pxd,
+                                             const double* pyd, float E, int* loc, int* par, bool* done, bool* esc,
+                                             unsigned& n_orient, unsigned long long& n_esc) {
+#pragma unroll 4
+    for (int e = 0; e < ne; ++e) {
+        const float4 ed = Et[e];
+#pragma unroll
+        for (int j = 0; j < kCandPerThread; ++j) {
+            if (done[j]) continue;
+            if (edge_step_f32(pu[j], pv[j], ed.x, ed.y, ed.z, ed.w, E, par[j], n_orient)) {
+                ++n_esc;
+                esc[j] = true;
+                const int32_t c = Ec[e];
+                if (edge_step_f64(pxd[j], pyd[j], R.x[c - 1], R.y[c - 1], R.x[c], R.y[c], par[j])) {
+                    loc[j] = 1;
+                    done[j] = true;
+                }
+            }
+        }
+    }
+}
+
+// Stages the edges (c-1, c), c in (lo, hi), of a single-part polygon that can
+// affect a point in [xmn, inf) x [ymn, ymx] (all of them when the bounds are
+// infinite) into Et/Ec in coordinate order, running refine_edges whenever the
+// buffer fills.  Block-uniform; returns the number left staged.
+__device__ int stage_edges(int32_t lo, int32_t hi, double xmn, double ymn, double ymx, double cx, double cy,
+                           const RefineView& R, float4* Et, int32_t* Ec, int* s_wsum, int ne, const float* pu,
+                           const float* pv, const double* pxd, const double* pyd, float E, int* loc, int* par,
+                           bool* done, bool* esc, unsigned& n_orient, unsigned long long& n_esc) {
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    for (int32_t base = lo; base < hi; base += kLJThreads) {
+        const int32_t c = base + tid;
+        bool rel = false;
+        double ax = 0, ay = 0, bx = 0, by = 0;
+        if (c > lo && c < hi && !(R.flags[c] & 1)) {
+            ax = R.x[c - 1];
+            ay = R.y[c - 1];
+            bx = R.x[c];
+            by = R.y[c];
+            rel = fmax(ay, by) >= ymn && fmin(ay, by) <= ymx && fmax(ax, bx) >= xmn;
+        }
+        const int inc = wave_inclusive_scan(rel ? 1 : 0);
+        if (lane == 63) s_wsum[wid] = inc;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w2 = 0; w2 < kLJThreads / 64; ++w2) {
+            woff += w2 < wid ? s_wsum[w2] : 0;
+            tot += s_wsum[w2];
+        }
+        if (ne + tot > kEdgeCap) {
+            refine_edges(Et, Ec, ne, R, pu, pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
+            ne = 0;
+            __syncthreads();
+        }
+        if (rel) {
+            const int slot = ne + woff + inc - 1;
+            Et[slot] = make_float4((float)(ax - cx), (float)(ay - cy), (float)(bx - cx), (float)(by - cy));
+            Ec[slot] = c;
+        }
+        ne += tot;
+        __syncthreads();
+    }
+    return ne;
+}
+
 // Persistent polygon-major join.  MODE kLModePip: query q is polygon row q
 // (qbox = its bounds), output pairs (point, q).  MODE kLModeBox: query q is a
-// box, output pairs (q, point).
+// box, output pairs (q, point).  Items (up to kItemPos scanned positions of one
+// query) are taken in LPT order (queries sorted by decreasing estimated cost)
+// and processed in rounds of kChunk positions.  A small single-part polygon's
+// edges are staged in LDS once per item; larger ones are staged per round,
+// keeping only edges that can affect the round's candidates; multi-part
+// polygons stream every edge in order to settle each part's parity.
 template <int MODE, int PRED>
 __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
     int64_t nq, const int32_t* __restrict__ item_off, const long long* __restrict__ total_items,
-    const int32_t* __restrict__ item_q, int64_t item_cap, const int2* __restrict__ coord_range,
-    const int2* __restrict__ ranges, const int32_t* __restrict__ nranges, const double* __restrict__ sx,
-    const double* __restrict__ sy, const int32_t* __restrict__ perm, const double4* __restrict__ qbox,
-    RefineView R, int2* __restrict__ stage, int64_t capacity, JoinCtl* __restrict__ ctl,
-    int32_t* __restrict__ counts) {
-    __shared__ int s_item, s_q, s_chunk, s_ncand, s_nout;
+    const int32_t* __restrict__ item_q, int64_t item_cap, const int32_t* __restrict__ order,
+    const int4* __restrict__ poly_info, const int2* __restrict__ ranges, const int32_t* __restrict__ nranges,
+    const double* __restrict__ sx, const double* __restrict__ sy, const int32_t* __restrict__ perm,
+    const double4* __restrict__ qbox, RefineView R, int2* __restrict__ stage, int64_t capacity,
+    JoinCtl* __restrict__ ctl, int32_t* __restrict__ counts) {
+    __shared__ int s_item, s_q, s_chunk, s_ncand, s_nout, s_nr;
     __shared__ unsigned long long s_base;
     __shared__ int2 s_rng[kMaxRanges];
-    __shared__ int s_nr;
     __shared__ int32_t cand[kChunk];
-    __shared__ int2 outb[kChunk];
-    __shared__ float2 Vt[kVmax + 1];
-    __shared__ uint8_t Ft[kVmax + 1];
-    const int tid = threadIdx.x;
+    __shared__ int2 outb[kOutCap];
+    __shared__ float4 Et[kEdgeCap];
+    __shared__ int32_t Ec[kEdgeCap];
+    __shared__ int s_wsum[kLJThreads / 64];
+    __shared__ double s_ext[kLJThreads / 64][3];
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
     const long long T = *total_items;
     unsigned long long st_cand = 0, st_cverts = 0, st_orient = 0, st_esc = 0, st_escpairs = 0;
+    if (tid == 0) s_nout = 0;
+    // Appends the block's buffered pairs to the stage with one atomic.
+    auto flush = [&]() {
+        __syncthreads();
+        if (tid == 0) s_base = s_nout ? atomicAdd(&ctl->staged, (unsigned long long)s_nout) : 0ull;
+        __syncthreads();
+        const int nout = s_nout;
+        for (int k = tid; k < nout; k += kLJThreads) {
+            const unsigned long long slot = s_base + (unsigned long long)k;
+            if (slot < (unsigned long long)capacity) stage[slot] = outb[k];
+        }
+        __syncthreads();
+        if (tid == 0) s_nout = 0;
+        __syncthreads();
+    };
     for (;;) {
         if (tid == 0) {
             const int item = (int)atomicAdd(&ctl->queue[0], 1u);
@@ -555,7 +651,7 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
                 int64_t lo;
                 if (item < item_cap) {
                     lo = item_q[item];
-                } else {  // beyond the map: last q with item_off[q] <= item
+                } else {  // beyond the map: last sorted index with item_off <= item
                     lo = 0;
                     int64_t hi = nq - 1;
                     while (lo < hi) {
@@ -564,12 +660,11 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
                         else hi = mid - 1;
                     }
                 }
-                s_q = (int)lo;
+                const int q = order[lo];
+                s_q = q;
                 s_chunk = item - item_off[lo];
-                s_nr = nranges[lo];
+                s_nr = nranges[q];
             }
-            s_ncand = 0;
-            s_nout = 0;
         }
         __syncthreads();
         if ((long long)s_item >= T) break;
@@ -577,44 +672,70 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
         if (tid < s_nr) s_rng[tid] = ranges[(int64_t)q * kMaxRanges + tid];
         __syncthreads();
         const double4 box = qbox[q];
-        // 1. scan this chunk of the cover's positions with the exact box test
-        for (int k = tid; k < kChunk; k += kLJThreads) {
-            int off = s_chunk * kChunk + k;
-            int r = 0;
-            while (r < s_nr && off >= s_rng[r].y - s_rng[r].x) {
-                off -= s_rng[r].y - s_rng[r].x;
-                ++r;
-            }
-            if (r < s_nr) {
-                const int32_t pos = s_rng[r].x + off;
-                const double px = sx[pos], py = sy[pos];
-                if (px >= box.x && px <= box.z && py >= box.y && py <= box.w) {
-                    const int slot = atomicAdd(&s_ncand, 1);
-                    cand[slot] = pos;
-                }
-            }
-        }
-        __syncthreads();
-        const int ncand = s_ncand;
-        if (MODE == kLModeBox) {
-            for (int k = tid; k < ncand; k += kLJThreads) outb[k] = make_int2(q, perm[cand[k]]);
-            if (tid == 0) {
-                if (ncand) atomicAdd(&counts[q], ncand);
-                s_nout = ncand;
-            }
-            __syncthreads();
-        } else if (ncand > 0) {
-            // 2. refine the candidates against the polygon, vertices staged in LDS
-            const int2 cr = coord_range[q];
-            const int32_t c0 = cr.x, c1 = cr.y;
-            const double cx = 0.5 * (box.x + box.z), cy = 0.5 * (box.y + box.w);
+        int total = 0;
+        for (int r = 0; r < s_nr; ++r) total += s_rng[r].y - s_rng[r].x;
+        const int first = s_chunk * kItemPos;
+        const int last = (first + kItemPos < total) ? first + kItemPos : total;
+        int4 info = make_int4(0, 0, 0, 0);
+        double cx = 0, cy = 0;
+        float E = 0.0f;
+        int ne_fixed = -1;  // edges staged once for the whole item (small single-part polygons)
+        if (MODE == kLModePip) {
+            info = poly_info[q];
+            cx = 0.5 * (box.x + box.z);
+            cy = 0.5 * (box.y + box.w);
             double M = fmax(fmax(box.z - cx, cx - box.x), fmax(box.w - cy, cy - box.y));
             M *= 1.0 + 4.440892098500626e-16;
-            const float E = fp32_det_bound(M);
+            E = fp32_det_bound(M);
+        }
+        if (tid == 0) s_ncand = 0;
+        __syncthreads();
+        for (int r0 = first; r0 < last; r0 += kLJThreads) {
+            // 1. scan kLJThreads positions with the exact box test; candidates
+            // accumulate until a full block's worth is buffered
+            {
+                const int off = r0 + tid;
+                if (off < last) {
+                    int o2 = off, r = 0;
+                    while (r < s_nr && o2 >= s_rng[r].y - s_rng[r].x) {
+                        o2 -= s_rng[r].y - s_rng[r].x;
+                        ++r;
+                    }
+                    if (r < s_nr) {
+                        const int32_t pos = s_rng[r].x + o2;
+                        const double px = sx[pos], py = sy[pos];
+                        if (px >= box.x && px <= box.z && py >= box.y && py <= box.w) {
+                            const int slot = atomicAdd(&s_ncand, 1);
+                            cand[slot] = pos;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const int ncand = s_ncand;
+            __syncthreads();  // every wave has read s_ncand before it can change again
+            if (ncand < kLJThreads && r0 + kLJThreads < last) continue;  // keep accumulating
+            if (ncand == 0) continue;
+            if (s_nout + ncand > kOutCap) flush();
+            if (MODE == kLModeBox) {
+                const int base = s_nout;
+                for (int k = tid; k < ncand; k += kLJThreads) outb[base + k] = make_int2(q, perm[cand[k]]);
+                __syncthreads();
+                if (tid == 0) {
+                    atomicAdd(&counts[q], ncand);
+                    s_nout = base + ncand;
+                    s_ncand = 0;
+                }
+                __syncthreads();
+                continue;
+            }
+            // 2. refine the round's candidates against the polygon
+            const int32_t c0 = info.x, c1 = info.y;
             int loc[kCandPerThread], par[kCandPerThread];
             bool done[kCandPerThread], esc[kCandPerThread];
             float pu[kCandPerThread], pv[kCandPerThread];
             double pxd[kCandPerThread], pyd[kCandPerThread];
+            double exmin = INFINITY, eymin = INFINITY, eymax = -INFINITY;
 #pragma unroll
             for (int j = 0; j < kCandPerThread; ++j) {
                 const int k = tid + j * kLJThreads;
@@ -627,50 +748,87 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
                     pyd[j] = sy[cand[k]];
                     pu[j] = (float)(pxd[j] - cx);
                     pv[j] = (float)(pyd[j] - cy);
+                    exmin = fmin(exmin, pxd[j]);
+                    eymin = fmin(eymin, pyd[j]);
+                    eymax = fmax(eymax, pyd[j]);
                 } else {
                     pxd[j] = pyd[j] = 0.0;
                     pu[j] = pv[j] = 0.0f;
                 }
             }
             unsigned n_orient = 0;
-            for (int32_t b0 = c0; b0 < c1; b0 += kVmax) {
-                const int32_t b1 = (b0 + kVmax < c1) ? b0 + kVmax : c1;
-                __syncthreads();
-                for (int32_t c = b0 - 1 + tid; c < b1; c += kLJThreads) {
-                    if (c < c0) continue;
-                    Vt[c - b0 + 1] = make_float2((float)(R.x[c] - cx), (float)(R.y[c] - cy));
-                    Ft[c - b0 + 1] = R.flags[c];
+            unsigned long long n_esc = 0;
+            if (info.z == 1 && c1 - c0 <= kEdgeCap + 1) {
+                if (ne_fixed < 0)  // stage every edge once for the item
+                    ne_fixed = stage_edges(c0, c1, -INFINITY, -INFINITY, INFINITY, cx, cy, R, Et, Ec, s_wsum, 0, pu,
+                                           pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
+                refine_edges(Et, Ec, ne_fixed, R, pu, pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
+            } else if (info.z == 1) {
+                exmin = wave_min(exmin);
+                eymin = wave_min(eymin);
+                eymax = wave_max(eymax);
+                if (lane == 0) {
+                    s_ext[wid][0] = exmin;
+                    s_ext[wid][1] = eymin;
+                    s_ext[wid][2] = eymax;
                 }
                 __syncthreads();
-                for (int32_t c = b0; c < b1; ++c) {
-                    const int f = Ft[c - b0 + 1];
-                    if (f & 2) {
+                double xmn = s_ext[0][0], ymn = s_ext[0][1], ymx = s_ext[0][2];
+                for (int w2 = 1; w2 < kLJThreads / 64; ++w2) {
+                    xmn = fmin(xmn, s_ext[w2][0]);
+                    ymn = fmin(ymn, s_ext[w2][1]);
+                    ymx = fmax(ymx, s_ext[w2][2]);
+                }
+                __syncthreads();
+                const int ne = stage_edges(c0, c1, xmn, ymn, ymx, cx, cy, R, Et, Ec, s_wsum, 0, pu, pv, pxd, pyd, E,
+                                           loc, par, done, esc, n_orient, n_esc);
+                refine_edges(Et, Ec, ne, R, pu, pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
+            } else {
+                float2* Vt = reinterpret_cast<float2*>(Et);
+                uint8_t* Ft = reinterpret_cast<uint8_t*>(Ec);
+                constexpr int kV = kEdgeCap;  // vertices per batch: 8 B + 1 B each fit the edge buffers
+                for (int32_t b0 = c0; b0 < c1; b0 += kV) {
+                    const int32_t b1 = (b0 + kV < c1) ? b0 + kV : c1;
+                    __syncthreads();
+                    for (int32_t c = b0 - 1 + tid; c < b1; c += kLJThreads) {
+                        if (c < c0) continue;
+                        Vt[c - b0 + 1] = make_float2((float)(R.x[c] - cx), (float)(R.y[c] - cy));
+                        Ft[c - b0 + 1] = R.flags[c];
+                    }
+                    __syncthreads();
+                    for (int32_t c = b0; c < b1; ++c) {
+                        const int f = Ft[c - b0 + 1];
+                        if (f & 2) {
+#pragma unroll
+                            for (int j = 0; j < kCandPerThread; ++j) {
+                                if (!done[j] && par[j]) {
+                                    loc[j] = 2;
+                                    done[j] = true;
+                                }
+                                par[j] = 0;
+                            }
+                        }
+                        if (f & 1) continue;
+                        const float2 a = Vt[c - b0], b = Vt[c - b0 + 1];
 #pragma unroll
                         for (int j = 0; j < kCandPerThread; ++j) {
-                            if (!done[j] && par[j]) {
-                                loc[j] = 2;
-                                done[j] = true;
-                            }
-                            par[j] = 0;
-                        }
-                    }
-                    if (f & 1) continue;
-                    const float2 a = Vt[c - b0], b = Vt[c - b0 + 1];
-#pragma unroll
-                    for (int j = 0; j < kCandPerThread; ++j) {
-                        if (done[j]) continue;
-                        if (edge_step_f32(pu[j], pv[j], a.x, a.y, b.x, b.y, E, par[j], n_orient)) {
-                            ++st_esc;
-                            esc[j] = true;
-                            if (edge_step_f64(pxd[j], pyd[j], R.x[c - 1], R.y[c - 1], R.x[c], R.y[c], par[j])) {
-                                loc[j] = 1;
-                                done[j] = true;
+                            if (done[j]) continue;
+                            if (edge_step_f32(pu[j], pv[j], a.x, a.y, b.x, b.y, E, par[j], n_orient)) {
+                                ++n_esc;
+                                esc[j] = true;
+                                if (edge_step_f64(pxd[j], pyd[j], R.x[c - 1], R.y[c - 1], R.x[c], R.y[c], par[j])) {
+                                    loc[j] = 1;
+                                    done[j] = true;
+                                }
                             }
                         }
                     }
                 }
+                __syncthreads();
+                ne_fixed = -1;  // the edge buffers were reused as vertex buffers
             }
             st_orient += n_orient;
+            st_esc += n_esc;
 #pragma unroll
             for (int j = 0; j < kCandPerThread; ++j) {
                 const int k = tid + j * kLJThreads;
@@ -689,28 +847,43 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
                 st_cverts += (unsigned long long)ncand * (unsigned long long)(c1 - c0);
             }
             __syncthreads();
-        }
-        // 3. append this item's pairs to the stage with one atomic
-        if (tid == 0) s_base = s_nout ? atomicAdd(&ctl->staged, (unsigned long long)s_nout) : 0ull;
-        __syncthreads();
-        const int nout = s_nout;
-        for (int k = tid; k < nout; k += kLJThreads) {
-            const unsigned long long slot = s_base + (unsigned long long)k;
-            if (slot < (unsigned long long)capacity) stage[slot] = outb[k];
+            if (tid == 0) s_ncand = 0;
+            __syncthreads();
         }
         __syncthreads();
     }
+    flush();
     // counters: [0] candidates, [1] candidate vertices, [2] fp32 orientation
-    // evaluations, [3] escalated edges; pairs with an escalation -> queue[1]
+    // evaluations, [3] escalated edges; candidate pairs with an escalation
     unsigned long long v[5] = {st_cand, st_cverts, st_orient, st_esc, st_escpairs};
     for (int k = 0; k < 5; ++k) {
         unsigned long long x = v[k];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if ((tid & 63) == 0 && x) {
+        if (lane == 0 && x) {
             if (k < 4) atomicAdd(&ctl->stat[k], x);
             else atomicAdd(&ctl->escalated_pairs, x);
         }
     }
+}
+
+// LPT key: queries with larger estimated work (items x vertices) sort first.
+__global__ __launch_bounds__(256) void cost_key_kernel(const int32_t* __restrict__ n_items,
+                                                      const int4* __restrict__ poly_info, int64_t nq,
+                                                      uint32_t* __restrict__ key, int32_t* __restrict__ idx) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    const double v = poly_info ? (double)(poly_info[q].y - poly_info[q].x) : 1.0;
+    const double cost = (double)n_items[q] * (v > 1.0 ? v : 1.0);
+    const int lg = (int)(8.0 * log2(1.0 + cost));
+    key[q] = (uint32_t)(255 - (lg > 255 ? 255 : lg));
+    idx[q] = (int32_t)q;
+}
+
+__global__ __launch_bounds__(256) void gather_counts_kernel(const int32_t* __restrict__ n_items,
+                                                           const int32_t* __restrict__ order, int64_t nq,
+                                                           int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < nq) out[i] = n_items[order[i]];
 }
 
 // item_q[k] = query owning item k, for the first `cap` items (items beyond
@@ -827,7 +1000,11 @@ LearnedView<K> view_of(const IndexBufs<K>& B, int64_t n, int S, int mode) {
 struct QueryBufs {
     int32_t* item_q;
     int64_t item_cap;
-    int2* coord_range;
+    int4* poly_info;
+    uint32_t* cost_key;
+    int32_t* order;
+    int32_t* n_items_sorted;
+    void* sort_ws;
     int2* ranges;
     int32_t* nranges;
     int32_t* n_items;
@@ -845,13 +1022,17 @@ struct QueryBufs {
 
 // Items map: room for 4 items per query plus one per kChunk/4 left rows covers
 // typical covers; items beyond the map fall back to a binary search.
-inline int64_t item_cap_for(int64_t nq, int64_t n_left) { return 4 * nq + (4 * n_left) / kChunk + 1024; }
+inline int64_t item_cap_for(int64_t nq, int64_t n_left) { return 2 * nq + (8 * n_left) / kItemPos + 1024; }
 
 QueryBufs carve_query(Carver& c, int64_t nq, int64_t n_left, int64_t capacity) {
     QueryBufs Q;
     Q.item_cap = item_cap_for(nq, n_left);
     Q.item_q = c.take<int32_t>(Q.item_cap);
-    Q.coord_range = c.take<int2>(nq);
+    Q.poly_info = c.take<int4>(nq);
+    Q.cost_key = c.take<uint32_t>(nq);
+    Q.order = c.take<int32_t>(nq);
+    Q.n_items_sorted = c.take<int32_t>(nq);
+    Q.sort_ws = c.take<char>(sort_workspace_bytes_u32(nq));
     Q.ranges = c.take<int2>(nq * kMaxRanges);
     Q.nranges = c.take<int32_t>(nq);
     Q.n_items = c.take<int32_t>(nq);
@@ -882,15 +1063,23 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
     fill_i32(Q.counts, n_left, 0, s);
     VS_LAUNCH_CHECK();
     if (nq > 0) {
-        hipLaunchKernelGGL(cover_kernel<K>, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, qbox, nq, B.ext, V,
-                           Q.ranges, Q.nranges, Q.n_items, Q.cstat, R.geom, R.part, R.ring,
-                           mode == kLModePip ? Q.coord_range : nullptr);
+        const unsigned gq = (unsigned)ceil_div(nq, 256);
+        int4* pinfo = mode == kLModePip ? Q.poly_info : nullptr;
+        hipLaunchKernelGGL(cover_kernel<K>, dim3(gq), dim3(256), 0, s, qbox, nq, B.ext, V, Q.ranges, Q.nranges,
+                           Q.n_items, Q.cstat, R.geom, R.part, R.ring, pinfo);
         VS_LAUNCH_CHECK();
-        VS_TRY(exclusive_scan_i32(Q.n_items, Q.item_off, nq, nullptr, Q.scan_ws, s));
-        hipLaunchKernelGGL(item_map_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, Q.item_off, Q.n_items,
-                           nq, Q.item_q, Q.item_cap);
+        // LPT order of the queries, then item offsets in that order
+        hipLaunchKernelGGL(cost_key_kernel, dim3(gq), dim3(256), 0, s, Q.n_items, pinfo, nq, Q.cost_key, Q.order);
         VS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(total_items_kernel, dim3(1), dim3(64), 0, s, Q.item_off, nq, Q.n_items, Q.total_items);
+        VS_TRY(sort_pairs_u32(Q.cost_key, Q.order, nq, 0, 8, Q.sort_ws, sort_workspace_bytes_u32(nq), s));
+        hipLaunchKernelGGL(gather_counts_kernel, dim3(gq), dim3(256), 0, s, Q.n_items, Q.order, nq, Q.n_items_sorted);
+        VS_LAUNCH_CHECK();
+        VS_TRY(exclusive_scan_i32(Q.n_items_sorted, Q.item_off, nq, nullptr, Q.scan_ws, s));
+        hipLaunchKernelGGL(item_map_kernel, dim3(gq), dim3(256), 0, s, Q.item_off, Q.n_items_sorted, nq, Q.item_q,
+                           Q.item_cap);
+        VS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(total_items_kernel, dim3(1), dim3(64), 0, s, Q.item_off, nq, Q.n_items_sorted,
+                           Q.total_items);
         VS_LAUNCH_CHECK();
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {
@@ -901,8 +1090,8 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
         VS_TRY(timing_begin_public(s));
 #define VS_LJ(M, P)                                                                                          \
     hipLaunchKernelGGL((learned_join_kernel<M, P>), dim3(grid), dim3(kLJThreads), 0, s, nq, Q.item_off,       \
-                       Q.total_items, Q.item_q, Q.item_cap, Q.coord_range, Q.ranges, Q.nranges, B.sx, B.sy, B.perm, \
-                       qbox, R, Q.stage, capacity, Q.ctl, Q.counts)
+                       Q.total_items, Q.item_q, Q.item_cap, Q.order, Q.poly_info, Q.ranges, Q.nranges, B.sx, B.sy,  \
+                       B.perm, qbox, R, Q.stage, capacity, Q.ctl, Q.counts)
         if (mode == kLModeBox) VS_LJ(kLModeBox, VS_PRED_INTERSECTS);
         else if (pred == VS_PRED_WITHIN) VS_LJ(kLModePip, VS_PRED_WITHIN);
         else if (pred == VS_PRED_TOUCHES) VS_LJ(kLModePip, VS_PRED_TOUCHES);

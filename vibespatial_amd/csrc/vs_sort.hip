@@ -147,8 +147,10 @@ constexpr int kRsBins = 256;
 
 template <class K>
 __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const K* __restrict__ keys, int64_t n,
+                                                            const unsigned long long* __restrict__ d_n,
                                                             int shift, int32_t* __restrict__ hist,
                                                             int nblocks) {
+    if (d_n && (int64_t)*d_n < n) n = (int64_t)*d_n;
     __shared__ int cnt[kRsWaves][kRsBins];
     const int w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads) (&cnt[0][0])[i] = 0;
@@ -170,8 +172,9 @@ __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const K* __restrict
 template <class K>
 __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
     const K* __restrict__ keys_in, const int32_t* __restrict__ vals_in, K* __restrict__ keys_out,
-    int32_t* __restrict__ vals_out, int64_t n, int shift, const int32_t* __restrict__ hist_scan,
-    int nblocks) {
+    int32_t* __restrict__ vals_out, int64_t n, const unsigned long long* __restrict__ d_n, int shift,
+    const int32_t* __restrict__ hist_scan, int nblocks) {
+    if (d_n && (int64_t)*d_n < n) n = (int64_t)*d_n;
     __shared__ int wave_cnt[kRsWaves][kRsBins];
     __shared__ int digit_block_off[kRsBins];
     __shared__ long long digit_global[kRsBins];
@@ -262,9 +265,11 @@ int64_t sort_workspace_bytes_t(int64_t n) {
 }
 
 // Sorts (keys, vals) in place by bits [begin_bit, end_bit) of the key, stable.
+// n: element capacity; d_n (device, optional): the number of valid leading
+// elements, read by the kernels (min(*d_n, n) are sorted; no host sync).
 template <class K>
-int sort_pairs_t(K* keys, int32_t* vals, int64_t n, int begin_bit, int end_bit, void* ws,
-                 int64_t ws_bytes, hipStream_t s) {
+int sort_pairs_t(K* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit, int end_bit,
+                 void* ws, int64_t ws_bytes, hipStream_t s) {
     VS_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX, "sort_pairs: n=%lld out of range", (long long)n);
     VS_REQUIRE(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit,
                "sort_pairs: bad bit range [%d, %d)", begin_bit, end_bit);
@@ -282,10 +287,10 @@ int sort_pairs_t(K* keys, int32_t* vals, int64_t n, int begin_bit, int end_bit, 
     int32_t* vo = valt;
     int passes = 0;
     for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
-        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, n, shift, hist, nb);
+        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, n, d_n, shift, hist, nb);
         VS_LAUNCH_CHECK();
         VS_TRY(exclusive_scan_i32(hist, hist, (int64_t)nb * kRsBins, nullptr, scan_ws, s));
-        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n,
+        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n, d_n,
                            shift, hist, nb);
         VS_LAUNCH_CHECK();
         K* tk = ki; ki = ko; ko = tk;
@@ -301,10 +306,14 @@ int sort_pairs_t(K* keys, int32_t* vals, int64_t n, int begin_bit, int end_bit, 
 int64_t sort_workspace_bytes_u64(int64_t n) { return sort_workspace_bytes_t<uint64_t>(n); }
 int64_t sort_workspace_bytes_u32(int64_t n) { return sort_workspace_bytes_t<uint32_t>(n); }
 int sort_pairs_u64(uint64_t* k, int32_t* v, int64_t n, int b0, int b1, void* ws, int64_t wb, hipStream_t s) {
-    return sort_pairs_t<uint64_t>(k, v, n, b0, b1, ws, wb, s);
+    return sort_pairs_t<uint64_t>(k, v, n, nullptr, b0, b1, ws, wb, s);
 }
 int sort_pairs_u32(uint32_t* k, int32_t* v, int64_t n, int b0, int b1, void* ws, int64_t wb, hipStream_t s) {
-    return sort_pairs_t<uint32_t>(k, v, n, b0, b1, ws, wb, s);
+    return sort_pairs_t<uint32_t>(k, v, n, nullptr, b0, b1, ws, wb, s);
+}
+int sort_pairs_u32_devn(uint32_t* k, int32_t* v, int64_t n, const unsigned long long* d_n, int b0, int b1, void* ws,
+                        int64_t wb, hipStream_t s) {
+    return sort_pairs_t<uint32_t>(k, v, n, d_n, b0, b1, ws, wb, s);
 }
 
 }  // namespace vs
