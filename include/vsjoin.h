@@ -146,14 +146,16 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
 int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity);
 
 /* Bounding-box query: every (query, tree item) pair whose boxes overlap
- * (inclusive; NaN boxes never match), lexsorted, int64.  query_boxes: 4 doubles
- * per query.  Replaces spatial_index_device_query's candidate output
- * (spatial/spatial_index_device.py:228-390) and the bbox_overlap kernels
- * (kernels/core/spatial_query_source.py:415-555). */
+ * (inclusive; NaN boxes never match), int64.  query_boxes: 4 doubles per
+ * query.  sort != 0: lexsorted (query, item); sort == 0: grouped by query in
+ * ascending order, items within a query in unspecified order (the reference's
+ * sort=False, spatial/query.py:143-153).  Replaces spatial_index_device_query's
+ * candidate output (spatial/spatial_index_device.py:228-390) and the
+ * bbox_overlap kernels (kernels/core/spatial_query_source.py:415-555). */
 int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_layout* layout,
-                   const double* node_boxes, const int32_t* item_ids, int64_t capacity, int64_t* out_left,
-                   int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
-                   void* stream);
+                   const double* node_boxes, const int32_t* item_ids, int64_t capacity, int32_t sort,
+                   int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                   int64_t workspace_bytes, void* stream);
 
 int64_t vs_refine_pairs_workspace_bytes(int64_t n_pairs);
 
@@ -231,14 +233,14 @@ int vs_mlp_loss_grad(const float* params, const float* u, const float* t, int64_
 int64_t vs_learned_query_boxes_workspace_bytes(int64_t n_boxes, int64_t capacity);
 
 /* sindex.query(boxes, predicate='intersects') on a point tree: (box, point)
- * pairs with the point inside the box (inclusive), lexsorted, through the
- * learned index (cover cells -> rank ranges -> scan).  d_result: int64[16],
- * layout as for the learned join.  Replaces _query_point_tree_box_index
- * (spatial/query_box.py:760-863). */
+ * pairs with the point inside the box (inclusive) through the learned index
+ * (cover cells -> rank ranges -> scan); `sort` as for vs_query_boxes.
+ * d_result: int64[16], layout as for the learned join.  Replaces
+ * _query_point_tree_box_index (spatial/query_box.py:760-863). */
 int vs_learned_query_boxes(const void* index, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
                            const double* boxes, int64_t n_boxes, int32_t lookup_mode, int64_t capacity,
-                           int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
-                           int64_t workspace_bytes, void* stream);
+                           int32_t sort, int64_t* out_left, int64_t* out_right, int64_t* d_result,
+                           void* workspace, int64_t workspace_bytes, void* stream);
 
 int64_t vs_sjoin_points_polygons_learned_workspace_bytes(int64_t n_points, int64_t n_polys, int64_t n_coords,
                                                          int64_t capacity);
@@ -259,6 +261,32 @@ int vs_sjoin_points_polygons_learned(const double* px, const double* py, int64_t
                                      int32_t train_iters, int32_t lookup_mode, int64_t capacity, int64_t* out_left,
                                      int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
                                      void* stream);
+
+/* ---------------------------------------------------------------------
+ * Batched polygon x polygon intersects
+ * --------------------------------------------------------------------- */
+
+int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t capacity);
+
+/* Many independent sjoin(left_j, right_j, predicate='intersects') jobs in one
+ * fixed launch sequence (a forest of packed trees, one persistent traversal,
+ * one exact refine, one finish).  Job j owns left rows [left_job_offsets[j],
+ * left_job_offsets[j+1]) of A and right rows [right_job_offsets[j],
+ * right_job_offsets[j+1]) of B (device int64 arrays of n_jobs + 1 entries,
+ * non-decreasing); only pairs inside one job are tested.  Output: global row
+ * pairs lexsorted, so each job's pairs are contiguous and lexsorted.
+ * tile: bits 0-7 traversal waves per block (1-4, 0 = 4), bits 8-23 refine
+ * block size (multiple of 64 up to 256, 0 = 256).  d_result (device int64[4]):
+ * [0] pairs, [1] bbox candidates staged (> capacity: retry with that
+ * capacity), [2] bbox candidates.  Replaces, per job, the reference's
+ * polygon x polygon sjoin path (spatial/spatial_index_device.py:228-390,
+ * predicates/polygon.py:849-996, spatial/query.py:143-153). */
+int vs_batched_intersects(const double* ax, const double* ay, const int32_t* a_geom, const int32_t* a_part,
+                          const int32_t* a_ring, int64_t n_left, const double* bx, const double* by,
+                          const int32_t* b_geom, const int32_t* b_part, const int32_t* b_ring, int64_t n_right,
+                          const int64_t* left_job_offsets, const int64_t* right_job_offsets, int64_t n_jobs,
+                          int32_t tile, int64_t capacity, int64_t* out_left, int64_t* out_right, int64_t* d_result,
+                          void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
  * Primitives

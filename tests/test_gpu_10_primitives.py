@@ -140,3 +140,26 @@ def test_query_boxes_capacity_retry():
     assert st.retries == 1 and st.pairs == 6000
     el, er = oracle.bbox_pairs(qb, tb)
     assert np.array_equal(l.cpu().numpy(), el) and np.array_equal(r.cpu().numpy(), er)
+
+
+@pytest.mark.parametrize("n_t", [30_000, 70_001])
+def test_query_boxes_long_runs_are_sorted(n_t):
+    """Queries with many hits: runs of one left id longer than the short-run
+    insertion sort (LDS bitonic sort) and longer than one LDS tile (chunk sort +
+    merge passes), next to short runs; and a single query (one run holds every
+    pair)."""
+    print(f"query_boxes long runs n_t={n_t}", flush=True)
+    rng = np.random.default_rng(n_t)
+    t = rng.uniform(0, 1000, (n_t, 2))
+    tb = np.hstack([t, t + rng.uniform(0, 3, (n_t, 2))])
+    tree = J.build_tree(torch.from_numpy(tb).to(_dev()))
+    q = rng.uniform(0, 1000, (300, 2))
+    qb = np.hstack([q, q + rng.uniform(0, 20, (300, 2))])
+    qb[10] = [-1, -1, 2000, 2000]  # every tree box
+    qb[11] = [0, 0, 300, 300]  # ~9% of them
+    qb[12] = [0, 0, 60, 60]  # a few hundred
+    qb[13] = [0, 0, 20, 40]  # tens
+    for sel in (slice(None), slice(10, 11)):
+        l, r, st = J.query_boxes(torch.from_numpy(qb[sel]).to(_dev()), tree)
+        el, er = oracle.bbox_pairs(qb[sel], tb)
+        assert np.array_equal(l.cpu().numpy(), el) and np.array_equal(r.cpu().numpy(), er)

@@ -129,6 +129,18 @@ def test_learned_query_boxes_matches_oracle(key_bytes):
         assert st.pairs == len(el) and st.scanned >= len(el)
 
 
+def test_learned_query_boxes_long_runs():
+    rng = np.random.default_rng(21)
+    px, py = _zipf_points(120_000, rng)
+    idx = J.LearnedPointIndex(_dp(px, py), key_bytes=4)
+    boxes = np.array([[px.min() - 1, py.min() - 1, px.max() + 1, py.max() + 1],  # every point
+                      [np.percentile(px, 40), np.percentile(py, 40), np.percentile(px, 60), np.percentile(py, 60)],
+                      [px[0] - 5, py[0] - 5, px[0] + 5, py[0] + 5]])
+    el, er = oracle.box_query_points(px, py, boxes, nthreads=NTHREADS)
+    l, r, _ = idx.query_boxes(torch.from_numpy(boxes).cuda(), "learned")
+    assert np.array_equal(l.cpu().numpy(), el) and np.array_equal(r.cpu().numpy(), er)
+
+
 # ---------------------------------------------------------------------------
 # learned + FP32 join == oracle, bit for bit
 # ---------------------------------------------------------------------------
@@ -204,3 +216,36 @@ def test_learned_join_edge_cases():
         l, r, _ = _learned_join(px, py, PA, pred)
         el, er = oracle.sjoin_points_polygons(px, py, PA, pred)
         assert np.array_equal(l, el) and np.array_equal(r, er), pred
+
+
+# ---------------------------------------------------------------------------
+# point-tree box query workload (Zipf clusters, 64-bit keys): learned index ==
+# packed-tree traversal == brute force
+# ---------------------------------------------------------------------------
+def test_c4_shape_learned_equals_tree_and_bruteforce():
+    print("c4-shaped 4M points x 20k boxes", flush=True)
+    w = S.c4(n_points=4_000_000, n_boxes=20_000)
+    px, py = w.points.x, w.points.y
+    boxes = w.meta["boxes"]
+    Q = _dp(px, py)
+    bt = torch.from_numpy(boxes).cuda()
+    idx = J.LearnedPointIndex(Q, key_bytes=8)
+    l1, r1, st1 = idx.query_boxes(bt, "learned", sort=False)
+    lg = l1.cpu().numpy()
+    assert np.all(lg[:-1] <= lg[1:])  # grouped by box
+    pts = torch.stack([Q.x, Q.y, Q.x, Q.y], dim=1).contiguous()
+    tree = J.build_tree(pts)
+    l0, r0, st0 = J.query_boxes(bt, tree, sort=False)
+    assert st0.pairs == st1.pairs
+    ls, rs, _ = idx.query_boxes(bt, "learned", sort=True)
+    key0 = l0 * len(px) + r0
+    key1 = l1 * len(px) + r1
+    keys = ls * len(px) + rs
+    assert torch.equal(torch.sort(key0).values, keys) and torch.equal(torch.sort(key1).values, keys)
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(len(boxes), 400, replace=False))
+    el, er = oracle.box_query_points(px, py, boxes[sample], nthreads=NTHREADS)
+    lsn, rsn = ls.cpu().numpy(), rs.cpu().numpy()
+    m = np.isin(lsn, sample)
+    assert np.array_equal(lsn[m], sample[el]) and np.array_equal(rsn[m], er)
+    print(f"hits {st1.pairs}, scanned {st1.scanned}, probes/lookup-range {st1.lookup_probes}")

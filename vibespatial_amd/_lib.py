@@ -69,7 +69,7 @@ SIGNATURES = {
          _p],
     ),
     "vs_query_boxes_workspace_bytes": (_i64, [_i64, _i64]),
-    "vs_query_boxes": (ctypes.c_int, [_p, _i64, _LP, _p, _p, _i64, _p, _p, _p, _p, _i64, _p]),
+    "vs_query_boxes": (ctypes.c_int, [_p, _i64, _LP, _p, _p, _i64, _i32, _p, _p, _p, _p, _i64, _p]),
     "vs_refine_pairs_workspace_bytes": (_i64, [_i64]),
     "vs_refine_pairs": (
         ctypes.c_int,
@@ -84,13 +84,17 @@ SIGNATURES = {
     "vs_learned_lower_bound": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _i64, _i32, _p, _p, _p]),
     "vs_mlp_loss_grad": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _p]),
     "vs_learned_query_boxes_workspace_bytes": (_i64, [_i64, _i64]),
-    "vs_learned_query_boxes": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _i64, _i32, _i64, _p, _p, _p, _p, _i64, _p]),
+    "vs_learned_query_boxes": (ctypes.c_int,
+                               [_p, _i64, _i32, _i32, _p, _i64, _i32, _i64, _i32, _p, _p, _p, _p, _i64, _p]),
     "vs_sjoin_points_polygons_learned_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
     "vs_sjoin_points_polygons_learned": (
         ctypes.c_int,
         [_p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _p, _i32, _i32, _i32, _i64, _p, _p, _p, _p, _i64,
          _p],
     ),
+    "vs_batched_intersects_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
+    "vs_batched_intersects": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _p, _p, _i64, _i32,
+                                             _i64, _p, _p, _p, _p, _i64, _p]),
     "vs_scan_workspace_bytes": (_i64, [_i64]),
     "vs_exclusive_scan_i32": (ctypes.c_int, [_p, _p, _i64, _p, _p, _i64, _p]),
     "vs_sort_workspace_bytes": (_i64, [_i64, _i32]),

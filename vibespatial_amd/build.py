@@ -20,7 +20,10 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvsjoin.so")
 ARCH = os.environ.get("VSJOIN_ARCH", "gfx950")
 
-SOURCES = ["vs_api.hip", "vs_sort.hip", "vs_index.hip", "vs_join.hip", "vs_refine.hip", "vs_learned.hip"]
+SOURCES = [
+    "vs_api.hip", "vs_sort.hip", "vs_index.hip", "vs_join.hip", "vs_finish.hip", "vs_refine.hip", "vs_learned.hip",
+    "vs_batch.hip",
+]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -1,0 +1,470 @@
+// Batched polygon x polygon intersects: many independent sjoin jobs in one
+// fixed sequence of launches, whatever the number of jobs.
+//
+// Replaces, per job, the reference's sjoin(left, right, predicate='intersects')
+// path: bbox candidates from the device index (spatial/spatial_index_device.py:
+// 228-390) refined by compute_polygonal_intersects_gpu (predicates/polygon.py:
+// 849-996, polygon_kernels.py:163-228), compacted and lexsorted
+// (spatial/query.py:143-153).  Job j pairs left rows [left_job[j],
+// left_job[j+1]) with right rows [right_job[j], right_job[j+1]) only.
+//
+//   1. bounds of every left and right row;
+//   2. right rows sorted by (job, Hilbert key of the bbox centre over the
+//      extent of all right centres); a packed tree per job (fan-out 16, root
+//      first, items last) is built by one block per job into one forest array;
+//   3. left rows sorted the same way; every job is cut into waves of 64
+//      consecutive sorted left rows; a persistent kernel hands waves out from
+//      one queue and each wave walks its own job's tree with a shared LDS
+//      stack, staging bbox candidates;
+//   4. one thread per candidate runs the exact intersects refine (edges of
+//      the pair, then one vertex per ring); survivors go to the finish pass
+//      (radix sort by left row, sort of right rows per left row).
+// Tile shape (`tile`): waves per block of the traversal and the refine block
+// size, chosen by the host autotuner.
+#include "vs_join_common.h"
+#include "vs_polys.h"
+
+namespace vs {
+
+int polygon_bounds(const double* x, const double* y, const int32_t* geom, const int32_t* part, const int32_t* ring,
+                   int64_t n, double* out, hipStream_t s);
+int centre_extent(const double4* b, int64_t n, double4* ext, void* ws, hipStream_t s);
+int64_t extent_workspace_bytes();
+int sort_pairs_u64(uint64_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
+int64_t sort_workspace_bytes_u64(int64_t n);
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, long long* d_total, void* ws, hipStream_t s);
+int64_t scan_workspace_bytes(int64_t n);
+
+constexpr int kFF = 16;          // forest fan-out
+constexpr int kFMaxLevels = 10;  // 16^9 > 2^31 items
+constexpr int kFStack = kFMaxLevels * kFF;
+constexpr int kFOut = 128;       // per-wave LDS candidate buffer
+
+// Level sizes of a packed tree over n items, root first; returns the level count.
+__host__ __device__ inline int forest_levels(int64_t n, int64_t* size) {
+    if (n <= 0) return 0;
+    int64_t tmp[kFMaxLevels];
+    int d = 0;
+    int64_t m = n;
+    for (;;) {
+        tmp[d++] = m;
+        if (m == 1) break;
+        m = (m + kFF - 1) / kFF;
+    }
+    for (int l = 0; l < d; ++l) size[l] = tmp[d - 1 - l];
+    return d;
+}
+
+__device__ __forceinline__ int64_t job_of(const int64_t* __restrict__ off, int64_t n_jobs, int64_t row) {
+    // last job j with off[j] <= row (empty jobs share their offset with the next)
+    int64_t lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int64_t m = (lo + hi + 1) >> 1;
+        if (off[m] <= row) lo = m;
+        else hi = m - 1;
+    }
+    return lo;
+}
+
+// key = job << 32 | 16-level Hilbert code of the bbox centre (NaN -> last).
+__global__ __launch_bounds__(256) void job_keys_kernel(const double4* __restrict__ b, int64_t n,
+                                                      const int64_t* __restrict__ job_off, int64_t n_jobs,
+                                                      const double4* __restrict__ ext, uint64_t* __restrict__ keys,
+                                                      int32_t* __restrict__ ids) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const KeyScale ks = key_scale(ext, 16);
+    const double4 v = b[i];
+    uint64_t h = 0xFFFFFFFFull;
+    if (v.x == v.x && v.y == v.y && v.z == v.z && v.w == v.w) {
+        const uint32_t qx = quantize(0.5 * (v.x + v.z), ks.x0, ks.sx, 16) << 16;
+        const uint32_t qy = quantize(0.5 * (v.y + v.w), ks.y0, ks.sy, 16) << 16;
+        h = hilbert64(qx, qy) >> 32;
+    }
+    keys[i] = ((uint64_t)job_of(job_off, n_jobs, i) << 32) | h;
+    ids[i] = (int32_t)i;
+}
+
+// Nodes per job tree and waves per job.
+__global__ __launch_bounds__(256) void job_sizes_kernel(const int64_t* __restrict__ left_job,
+                                                       const int64_t* __restrict__ right_job, int64_t n_jobs,
+                                                       int32_t* __restrict__ nodes, int32_t* __restrict__ waves) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n_jobs) return;
+    int64_t size[kFMaxLevels];
+    const int d = forest_levels(right_job[j + 1] - right_job[j], size);
+    int64_t t = 0;
+    for (int l = 0; l < d; ++l) t += size[l];
+    nodes[j] = (int32_t)t;
+    waves[j] = (int32_t)((left_job[j + 1] - left_job[j] + 63) / 64);
+}
+
+// One block per job: item boxes in sorted order, then each parent level.
+__global__ __launch_bounds__(256) void forest_build_kernel(const double4* __restrict__ rbox,
+                                                          const int32_t* __restrict__ rperm,
+                                                          const int64_t* __restrict__ right_job,
+                                                          const int32_t* __restrict__ tree_off, int64_t n_jobs,
+                                                          double4* __restrict__ fbox) {
+    for (int64_t j = blockIdx.x; j < n_jobs; j += gridDim.x) {
+        const int64_t r0 = right_job[j], R = right_job[j + 1] - r0;
+        int64_t size[kFMaxLevels];
+        const int d = forest_levels(R, size);
+        if (d == 0) continue;
+        int64_t start[kFMaxLevels];
+        int64_t acc = tree_off[j];
+        for (int l = 0; l < d; ++l) {
+            start[l] = acc;
+            acc += size[l];
+        }
+        for (int64_t k = threadIdx.x; k < R; k += blockDim.x) fbox[start[d - 1] + k] = rbox[rperm[r0 + k]];
+        __syncthreads();
+        for (int l = d - 2; l >= 0; --l) {
+            for (int64_t k = threadIdx.x; k < size[l]; k += blockDim.x) {
+                double4 u = make_double4(NAN, NAN, NAN, NAN);
+                const int64_t c0 = k * kFF, c1 = (c0 + kFF < size[l + 1]) ? c0 + kFF : size[l + 1];
+                for (int64_t c = c0; c < c1; ++c) {
+                    const double4 v = fbox[start[l + 1] + c];
+                    u.x = fmin(u.x, v.x);
+                    u.y = fmin(u.y, v.y);
+                    u.z = fmax(u.z, v.z);
+                    u.w = fmax(u.w, v.w);
+                }
+                fbox[start[l] + k] = u;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Persistent: waves take (job, 64 sorted left rows) tasks from ctl->queue[0].
+__global__ __launch_bounds__(256) void forest_query_kernel(const double4* __restrict__ lbox,
+                                                          const int32_t* __restrict__ lperm,
+                                                          const int64_t* __restrict__ left_job,
+                                                          const int64_t* __restrict__ right_job, int64_t n_jobs,
+                                                          const int32_t* __restrict__ wave_off,
+                                                          const long long* __restrict__ total_waves,
+                                                          const double4* __restrict__ fbox,
+                                                          const int32_t* __restrict__ tree_off,
+                                                          const int32_t* __restrict__ rperm,
+                                                          uint32_t* __restrict__ cand_l, int32_t* __restrict__ cand_r,
+                                                          int64_t capacity, JoinCtl* __restrict__ ctl) {
+    __shared__ long long stack_s[4][kFStack];
+    __shared__ int2 outbuf_s[4][kFOut];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    long long* stack = stack_s[w];
+    int2* outbuf = outbuf_s[w];
+    const uint64_t lt = lanemask_lt();
+    const long long W = *total_waves;
+    int nbuf = 0;
+    auto flush = [&]() {
+        if (nbuf == 0) return;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&ctl->staged, (unsigned long long)nbuf);
+        base = __shfl(base, 0, 64);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int i = lane; i < nbuf; i += 64) {
+            const unsigned long long slot = base + (unsigned long long)i;
+            if (slot < (unsigned long long)capacity) {
+                const int2 e = outbuf[i];
+                cand_l[slot] = (uint32_t)e.x;
+                cand_r[slot] = e.y;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        nbuf = 0;
+    };
+    for (;;) {
+        unsigned int t = 0;
+        if (lane == 0) t = atomicAdd(&ctl->queue[0], 1u);
+        t = __shfl(t, 0, 64);
+        if ((long long)t >= W) break;
+        // job of wave t: last j with wave_off[j] <= t
+        int64_t lo = 0, hi = n_jobs - 1;
+        while (lo < hi) {
+            const int64_t m = (lo + hi + 1) >> 1;
+            if (wave_off[m] <= (int32_t)t) lo = m;
+            else hi = m - 1;
+        }
+        const int64_t j = lo;
+        const int64_t s0 = left_job[j] + 64 * (int64_t)(t - (unsigned)wave_off[j]);
+        const int64_t s1 = (s0 + 64 < left_job[j + 1]) ? s0 + 64 : left_job[j + 1];
+        const int64_t idx = s0 + lane;
+        const bool inr = idx < s1;
+        const int32_t orig = inr ? lperm[idx] : 0;
+        const double4 b = inr ? lbox[orig] : make_double4(NAN, NAN, NAN, NAN);
+        const bool valid = inr && b.x == b.x && b.y == b.y && b.z == b.z && b.w == b.w;
+        const double ux0 = wave_min(valid ? b.x : INFINITY), uy0 = wave_min(valid ? b.y : INFINITY);
+        const double ux1 = wave_max(valid ? b.z : -INFINITY), uy1 = wave_max(valid ? b.w : -INFINITY);
+        const int64_t r0 = right_job[j];
+        int64_t size[kFMaxLevels];
+        const int d = forest_levels(right_job[j + 1] - r0, size);
+        if (d == 0 || __ballot(valid) == 0) continue;
+        int64_t start[kFMaxLevels];
+        {
+            int64_t acc = tree_off[j];
+            for (int l = 0; l < d; ++l) {
+                start[l] = acc;
+                acc += size[l];
+            }
+        }
+        int sp = 1;
+        if (lane == 0) stack[0] = 0;  // group 0 of level 0: the root
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        while (sp > 0) {
+            --sp;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const long long e = stack[sp];
+            const int l = (int)(e & 15);
+            const int64_t first = (e >> 4) * kFF;
+            int64_t cl = 0, sl = 0;
+            for (int q = 0; q < d; ++q)  // wave-uniform select (no dynamic register indexing)
+                if (q == l) {
+                    cl = size[q];
+                    sl = start[q];
+                }
+            const int ng = (int)((cl - first) < kFF ? (cl - first) : kFF);
+            double4 nb = make_double4(NAN, NAN, NAN, NAN);
+            bool ov = false;
+            if (lane < ng) {
+                nb = fbox[sl + first + lane];
+                ov = box_overlap(ux0, uy0, ux1, uy1, nb.x, nb.y, nb.z, nb.w);
+            }
+            uint64_t m = __ballot(ov);
+            if (l == d - 1) {
+                while (m) {
+                    const int s = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const double bx0 = readlane_f64(nb.x, s), by0 = readlane_f64(nb.y, s);
+                    const double bx1 = readlane_f64(nb.z, s), by1 = readlane_f64(nb.w, s);
+                    const bool hit = valid && box_overlap(b.x, b.y, b.z, b.w, bx0, by0, bx1, by1);
+                    const uint64_t hm = __ballot(hit);
+                    if (!hm) continue;
+                    const int k = __popcll(hm);
+                    if (nbuf + k > kFOut) flush();
+                    if (hit) outbuf[nbuf + __popcll(hm & lt)] = make_int2(orig, rperm[r0 + first + s]);
+                    nbuf += k;
+                }
+            } else {
+                while (m) {
+                    const int s = 63 - __clzll((long long)m);
+                    m &= ~(1ull << s);
+                    if (lane == 0) stack[sp] = ((first + s) << 4) | (long long)(l + 1);
+                    ++sp;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            }
+        }
+    }
+    flush();
+}
+
+// One thread per candidate: exact intersects; survivors appended to the
+// finish stage with one atomic per wave.
+__global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __restrict__ cand_l,
+                                                          const int32_t* __restrict__ cand_r,
+                                                          const JoinCtl* __restrict__ cctl, int64_t ccap, Polys A,
+                                                          Polys B, uint32_t* __restrict__ out_l,
+                                                          int32_t* __restrict__ out_r, int64_t capacity,
+                                                          JoinCtl* __restrict__ ctl) {
+    const unsigned long long nc = cctl->staged;
+    if (nc > (unsigned long long)ccap) return;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < (int64_t)nc; i0 += stride) {
+        const int64_t i = i0 + lane;
+        bool keep = false;
+        uint32_t l = 0;
+        int32_t r = 0;
+        if (i < (int64_t)nc) {
+            l = cand_l[i];
+            r = cand_r[i];
+            keep = polygons_intersect(A, l, B, r);
+        }
+        const uint64_t km = __ballot(keep);
+        if (!km) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&ctl->staged, (unsigned long long)__popcll(km));
+        base = __shfl(base, 0, 64);
+        if (keep) {
+            const unsigned long long slot = base + (unsigned long long)__popcll(km & lt);
+            if (slot < (unsigned long long)capacity) {
+                out_l[slot] = l;
+                out_r[slot] = r;
+            }
+        }
+    }
+}
+
+// d_result: [0] pairs K, [1] bbox candidates staged (the stage that bounds
+// `capacity`: survivors are a subset), [2] bbox candidates, [3] 0.
+__global__ void publish_batch_kernel(const JoinCtl* __restrict__ cctl, int64_t* __restrict__ result) {
+    if (threadIdx.x == 0) {
+        result[1] = (int64_t)cctl->staged;
+        result[2] = (int64_t)cctl->staged;
+        result[3] = 0;
+    }
+}
+
+struct BatchWs {
+    double4* lbox;
+    double4* rbox;
+    uint64_t* lkey;
+    int32_t* lperm;
+    uint64_t* rkey;
+    int32_t* rperm;
+    void* sort_ws;
+    double4* ext;
+    void* ext_ws;
+    int32_t* nodes;
+    int32_t* tree_off;
+    int32_t* waves;
+    int32_t* wave_off;
+    long long* total_nodes;
+    long long* total_waves;
+    void* scan_ws;
+    double4* fbox;
+    uint32_t* cand_l;
+    int32_t* cand_r;
+    JoinCtl* cctl;
+    JoinCtl* ctl;
+    void* fin_ws;
+    uint32_t* stage_l;
+    int32_t* stage_r;
+};
+
+// Forest nodes are bounded by the items and their parents: < n_right * 16/15 + levels * jobs.
+static int64_t forest_node_bound(int64_t n_right, int64_t n_jobs) {
+    return n_right + n_right / (kFF - 1) + 1 + (int64_t)kFMaxLevels * n_jobs;
+}
+
+static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t capacity,
+                           int64_t* bytes) {
+    Carver c(base);
+    BatchWs W;
+    W.lbox = c.take<double4>(n_left);
+    W.rbox = c.take<double4>(n_right);
+    W.lkey = c.take<uint64_t>(n_left);
+    W.lperm = c.take<int32_t>(n_left);
+    W.rkey = c.take<uint64_t>(n_right);
+    W.rperm = c.take<int32_t>(n_right);
+    const int64_t nmax = n_left > n_right ? n_left : n_right;
+    W.sort_ws = c.take<char>(sort_workspace_bytes_u64(nmax));
+    W.ext = c.take<double4>(1);
+    W.ext_ws = c.take<char>(extent_workspace_bytes());
+    W.nodes = c.take<int32_t>(n_jobs);
+    W.tree_off = c.take<int32_t>(n_jobs);
+    W.waves = c.take<int32_t>(n_jobs);
+    W.wave_off = c.take<int32_t>(n_jobs);
+    W.total_nodes = c.take<long long>(1);
+    W.total_waves = c.take<long long>(1);
+    W.scan_ws = c.take<char>(scan_workspace_bytes(n_jobs));
+    W.fbox = c.take<double4>(forest_node_bound(n_right, n_jobs));
+    W.cand_l = c.take<uint32_t>(capacity);
+    W.cand_r = c.take<int32_t>(capacity);
+    W.cctl = c.take<JoinCtl>(1);
+    W.ctl = c.take<JoinCtl>(1);
+    W.fin_ws = c.take<char>(finish_workspace_bytes(capacity));
+    W.stage_l = nullptr;
+    W.stage_r = nullptr;
+    if (base) finish_stage(W.fin_ws, capacity, &W.stage_l, &W.stage_r);
+    if (bytes) *bytes = c.bytes;
+    return W;
+}
+
+static int bits_for_jobs(int64_t n) {
+    int b = 0;
+    while (b < 31 && ((int64_t)1 << b) < n) ++b;
+    return b;
+}
+
+int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n_right, const int64_t* left_job,
+                       const int64_t* right_job, int64_t n_jobs, int tile, int64_t capacity, int64_t* out_l,
+                       int64_t* out_r, int64_t* d_result, const BatchWs& W, hipStream_t s) {
+    ctl_reset(W.cctl, s);
+    ctl_reset(W.ctl, s);
+    const int waves_per_block = (tile & 0xFF) ? (tile & 0xFF) : 4;
+    const int refine_threads = ((tile >> 8) & 0xFFFF) ? ((tile >> 8) & 0xFFFF) : 256;
+    VS_REQUIRE(waves_per_block >= 1 && waves_per_block <= 4, "batched join: tile waves per block in [1, 4]");
+    VS_REQUIRE(refine_threads % 64 == 0 && refine_threads <= 256, "batched join: refine block in {64,..,256}");
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
+    }
+    if (n_jobs > 0 && n_left > 0 && n_right > 0) {
+        VS_TRY(polygon_bounds(A.x, A.y, A.geom, A.part, A.ring, n_left, reinterpret_cast<double*>(W.lbox), s));
+        VS_TRY(polygon_bounds(B.x, B.y, B.geom, B.part, B.ring, n_right, reinterpret_cast<double*>(W.rbox), s));
+        const int jb = bits_for_jobs(n_jobs);
+        // right rows: (job, Hilbert) order, forest
+        VS_TRY(centre_extent(W.rbox, n_right, W.ext, W.ext_ws, s));
+        hipLaunchKernelGGL(job_keys_kernel, dim3((unsigned)ceil_div(n_right, 256)), dim3(256), 0, s, W.rbox, n_right,
+                           right_job, n_jobs, W.ext, W.rkey, W.rperm);
+        VS_LAUNCH_CHECK();
+        VS_TRY(sort_pairs_u64(W.rkey, W.rperm, n_right, 0, 32 + jb, W.sort_ws, sort_workspace_bytes_u64(n_right), s));
+        hipLaunchKernelGGL(job_sizes_kernel, dim3((unsigned)ceil_div(n_jobs, 256)), dim3(256), 0, s, left_job,
+                           right_job, n_jobs, W.nodes, W.waves);
+        VS_LAUNCH_CHECK();
+        VS_TRY(exclusive_scan_i32(W.nodes, W.tree_off, n_jobs, W.total_nodes, W.scan_ws, s));
+        VS_TRY(exclusive_scan_i32(W.waves, W.wave_off, n_jobs, W.total_waves, W.scan_ws, s));
+        hipLaunchKernelGGL(forest_build_kernel, dim3((unsigned)std::min<int64_t>(n_jobs, 65535)), dim3(256), 0, s,
+                           W.rbox, W.rperm, right_job, W.tree_off, n_jobs, W.fbox);
+        VS_LAUNCH_CHECK();
+        // left rows: (job, Hilbert) order, then the traversal
+        VS_TRY(centre_extent(W.lbox, n_left, W.ext, W.ext_ws, s));
+        hipLaunchKernelGGL(job_keys_kernel, dim3((unsigned)ceil_div(n_left, 256)), dim3(256), 0, s, W.lbox, n_left,
+                           left_job, n_jobs, W.ext, W.lkey, W.lperm);
+        VS_LAUNCH_CHECK();
+        VS_TRY(sort_pairs_u64(W.lkey, W.lperm, n_left, 0, 32 + jb, W.sort_ws, sort_workspace_bytes_u64(n_left), s));
+        VS_TRY(timing_begin_public(s));
+        hipLaunchKernelGGL(forest_query_kernel, dim3((unsigned)(cus * 8)), dim3(64 * waves_per_block), 0, s, W.lbox,
+                           W.lperm, left_job, right_job, n_jobs, W.wave_off, W.total_waves, W.fbox, W.tree_off,
+                           W.rperm, W.cand_l, W.cand_r, capacity, W.cctl);
+        VS_LAUNCH_CHECK();
+        const unsigned rgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, refine_threads),
+                                                                                  (int64_t)cus * 16));
+        hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads), 0, s, W.cand_l, W.cand_r, W.cctl,
+                           capacity, A, B, W.stage_l, W.stage_r, capacity, W.ctl);
+        VS_LAUNCH_CHECK();
+        VS_TRY(timing_end_public(s));
+    }
+    VS_TRY(finish_pairs(W.fin_ws, W.ctl, n_left, capacity, true, out_l, out_r, d_result, s));
+    hipLaunchKernelGGL(publish_batch_kernel, dim3(1), dim3(64), 0, s, W.cctl, d_result);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+}  // namespace vs
+
+extern "C" {
+
+int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t capacity) {
+    int64_t b = 0;
+    vs::carve_batch(nullptr, n_jobs, n_left, n_right, capacity, &b);
+    return b;
+}
+
+int vs_batched_intersects(const double* ax, const double* ay, const int32_t* a_geom, const int32_t* a_part,
+                          const int32_t* a_ring, int64_t n_left, const double* bx, const double* by,
+                          const int32_t* b_geom, const int32_t* b_part, const int32_t* b_ring, int64_t n_right,
+                          const int64_t* left_job_offsets, const int64_t* right_job_offsets, int64_t n_jobs,
+                          int32_t tile, int64_t capacity, int64_t* out_left, int64_t* out_right, int64_t* d_result,
+                          void* workspace, int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n_jobs >= 0 && n_jobs < (int64_t)INT32_MAX, "vs_batched_intersects: n_jobs out of range");
+    VS_REQUIRE(n_left >= 0 && n_left < (int64_t)INT32_MAX && n_right >= 0 && n_right < (int64_t)INT32_MAX,
+               "vs_batched_intersects: row counts out of range");
+    VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_batched_intersects: capacity out of range");
+    VS_REQUIRE(d_result != nullptr, "vs_batched_intersects: null result");
+    VS_REQUIRE(n_jobs == 0 || (left_job_offsets && right_job_offsets), "vs_batched_intersects: null job offsets");
+    int64_t need = 0;
+    const vs::BatchWs W = vs::carve_batch(workspace, n_jobs, n_left, n_right, capacity, &need);
+    VS_REQUIRE(workspace_bytes >= need, "vs_batched_intersects: workspace %lld < %lld bytes",
+               (long long)workspace_bytes, (long long)need);
+    const vs::Polys A = {ax, ay, a_geom, a_part, a_ring};
+    const vs::Polys B = {bx, by, b_geom, b_part, b_ring};
+    return vs::batched_intersects(A, n_left, B, n_right, left_job_offsets, right_job_offsets, n_jobs, tile, capacity,
+                                  out_left, out_right, d_result, W, vs::stream_of(stream));
+}
+
+}  // extern "C"

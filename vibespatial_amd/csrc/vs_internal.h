@@ -74,6 +74,19 @@ struct Carver {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Radix sort of (key, int32 value) pairs (vs_sort.hip).  The no-copy form
+// leaves the result in either the inputs or the workspace's alternates.
+template <class K>
+struct SortResult {
+    K* keys;
+    int32_t* vals;
+    K* alt_keys;
+    int32_t* alt_vals;
+};
+int64_t sort_workspace_bytes_u32(int64_t n);
+int sort_pairs_u32_dev(uint32_t* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit,
+                       int end_bit, void* ws, int64_t ws_bytes, hipStream_t s, SortResult<uint32_t>* res);
+
 // ---------------------------------------------------------------------------
 // Device helpers for 64-wide wavefronts.
 // ---------------------------------------------------------------------------

@@ -24,10 +24,10 @@
 //     every lane whose point is in the item's box runs the exact crossing /
 //     boundary step on it (fp64 with the stage-A filter, exact expansion
 //     fallback);
-//   * refined pairs are appended with one atomic per (wave, polygon); each lane
-//     counts its own point's pairs.
-// The finish pass (scan of per-point counts, scatter, per-point sort of the
-// polygon ids) produces pairs lexsorted by (point, polygon) as int64.
+//   * refined pairs are buffered per wave in LDS and appended to the stage
+//     with one atomic per full buffer.
+// The finish pass (vs_finish.hip: radix sort by point id, sort of each
+// point's polygon ids) produces pairs lexsorted by (point, polygon) as int64.
 // Waves are handed out from 8 queues, one per XCD, each over a contiguous
 // range of the Hilbert order, so an XCD's L2 sees a spatially compact slice;
 // an idle XCD steals from the others.  Every wave exits when all 8 are empty.
@@ -80,7 +80,7 @@ enum { kModePip = 0, kModeBox = 1 };
 __global__ void ctl_reset_kernel(JoinCtl* ctl) {
     if (threadIdx.x == 0) {
         ctl->staged = 0;
-        ctl->total = 0;
+        ctl->n_long = 0;
         for (int i = 0; i < 8; ++i) ctl->queue[i] = 0;
         for (int i = 0; i < 4; ++i) ctl->stat[i] = 0;
         ctl->escalated_pairs = 0;
@@ -165,8 +165,8 @@ __device__ bool refine_pip_wave(double px, double py, bool active, int32_t c0, i
 template <int MODE, int PRED>
 __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
     const double* __restrict__ qx, const double* __restrict__ qy, const double4* __restrict__ qbox,
-    const int32_t* __restrict__ perm, int64_t n, TreeView T, PolyView P, int2* __restrict__ stage,
-    int64_t capacity, JoinCtl* __restrict__ ctl, int32_t* __restrict__ counts) {
+    const int32_t* __restrict__ perm, int64_t n, TreeView T, PolyView P, uint32_t* __restrict__ stage_l,
+    int32_t* __restrict__ stage_r, int64_t capacity, JoinCtl* __restrict__ ctl) {
     __shared__ double4 top[kTopMax];
     __shared__ long long stack_s[kJoinWaves][kStackMax];
     __shared__ int2 outbuf_s[kJoinWaves][kOutBuf];
@@ -187,7 +187,11 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int i = lane; i < nbuf; i += 64) {
             const unsigned long long slot = base + (unsigned long long)i;
-            if (slot < (unsigned long long)capacity) stage[slot] = outbuf[i];
+            if (slot < (unsigned long long)capacity) {
+                const int2 e = outbuf[i];
+                stage_l[slot] = (uint32_t)e.x;
+                stage_r[slot] = e.y;
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         nbuf = 0;
@@ -226,7 +230,6 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
         const bool valid = inr && x0 == x0 && y0 == y0 && x1 == x1 && y1 == y1;
         const double ux0 = wave_min(valid ? x0 : INFINITY), uy0 = wave_min(valid ? y0 : INFINITY);
         const double ux1 = wave_max(valid ? x1 : -INFINITY), uy1 = wave_max(valid ? y1 : -INFINITY);
-        int cnt = 0;
         if (__ballot(valid) != 0 && T.level_count[0] > 0) {
             int sp = 1;
             if (lane == 0) stack[0] = 0;  // group 0 of level 0 = the root
@@ -273,10 +276,7 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
                         if (rm) {
                             const int k = __popcll(rm);
                             if (nbuf + k > kOutBuf) flush();
-                            if (res) {
-                                outbuf[nbuf + __popcll(rm & lt)] = make_int2(orig, row);
-                                ++cnt;
-                            }
+                            if (res) outbuf[nbuf + __popcll(rm & lt)] = make_int2(orig, row);
                             nbuf += k;
                         }
                     }
@@ -291,65 +291,11 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
                 }
             }
         }
-        if (inr) counts[orig] = cnt;
     }
     flush();
     if (MODE == kModePip && lane == 0 && st_cand) {
         atomicAdd(&ctl->stat[0], st_cand);
         atomicAdd(&ctl->stat[1], st_cverts);
-    }
-}
-
-// Scatter staged pairs into their lexsorted slots (order inside a query's run
-// is fixed afterwards by segment_sort_kernel).  Skipped when the staging buffer
-// overflowed: the caller retries with a larger capacity.
-__global__ __launch_bounds__(256) void scatter_pairs_kernel(const int2* __restrict__ stage,
-                                                           const JoinCtl* __restrict__ ctl,
-                                                           int64_t capacity,
-                                                           const int32_t* __restrict__ offsets,
-                                                           int32_t* __restrict__ cursor,
-                                                           int64_t* __restrict__ out_l,
-                                                           int64_t* __restrict__ out_r) {
-    const unsigned long long staged = ctl->staged;
-    if (staged > (unsigned long long)capacity) return;
-    for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < (int64_t)staged;
-         s += (int64_t)gridDim.x * 256) {
-        const int2 e = stage[s];
-        const int32_t pos = offsets[e.x] + atomicAdd(&cursor[e.x], 1);
-        out_l[pos] = e.x;
-        out_r[pos] = e.y;
-    }
-}
-
-// Insertion sort of each query's run of right indices (runs are short).
-__global__ __launch_bounds__(256) void segment_sort_kernel(const JoinCtl* __restrict__ ctl,
-                                                          int64_t capacity,
-                                                          const int32_t* __restrict__ offsets,
-                                                          const int32_t* __restrict__ counts, int64_t n,
-                                                          int64_t* __restrict__ out_r) {
-    if (ctl->staged > (unsigned long long)capacity) return;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int32_t c = counts[i];
-    if (c < 2) return;
-    int64_t* r = out_r + offsets[i];
-    for (int32_t a = 1; a < c; ++a) {
-        const int64_t v = r[a];
-        int32_t b = a - 1;
-        while (b >= 0 && r[b] > v) {
-            r[b + 1] = r[b];
-            --b;
-        }
-        r[b + 1] = v;
-    }
-}
-
-__global__ void publish_result_kernel(const JoinCtl* __restrict__ ctl, int64_t* __restrict__ result) {
-    if (threadIdx.x == 0) {
-        result[0] = ctl->total;
-        result[1] = (int64_t)ctl->staged;
-        result[2] = (int64_t)ctl->stat[0];
-        result[3] = (int64_t)ctl->stat[1];
     }
 }
 
@@ -424,12 +370,10 @@ struct JoinWs {
     double* sx;
     double* sy;
     void* order_ws;
-    int2* stage;
-    int32_t* counts;
-    int32_t* offsets;
-    int32_t* cursor;
-    void* scan_ws;
     JoinCtl* ctl;
+    void* fin_ws;
+    uint32_t* stage_l;
+    int32_t* stage_r;
 };
 
 JoinWs carve_join_ws(void* base, int64_t nq, int64_t n_items, int64_t n_coords, int64_t capacity,
@@ -447,12 +391,11 @@ JoinWs carve_join_ws(void* base, int64_t nq, int64_t n_items, int64_t n_coords, 
         J.sx = J.sy = nullptr;
         J.order_ws = nullptr;
     }
-    J.stage = c.take<int2>(capacity);
-    J.counts = c.take<int32_t>(nq);
-    J.offsets = c.take<int32_t>(nq);
-    J.cursor = c.take<int32_t>(nq);
-    J.scan_ws = c.take<char>(scan_workspace_bytes(nq));
     J.ctl = c.take<JoinCtl>(1);
+    J.fin_ws = c.take<char>(finish_workspace_bytes(capacity));
+    J.stage_l = nullptr;
+    J.stage_r = nullptr;
+    if (base) finish_stage(J.fin_ws, capacity, &J.stage_l, &J.stage_r);
     if (bytes) *bytes = c.bytes;
     return J;
 }
@@ -502,7 +445,7 @@ int launch_traverse(int pred, const double* qx, const double* qy, const double4*
     VS_TRY(timing_begin(s));
 #define VS_LAUNCH_T(PR)                                                                          \
     hipLaunchKernelGGL((traverse_kernel<MODE, PR>), dim3(grid), dim3(kJoinThreads), 0, s, qx, qy, \
-                       qbox, perm, n, T, P, J.stage, capacity, J.ctl, J.counts)
+                       qbox, perm, n, T, P, J.stage_l, J.stage_r, capacity, J.ctl)
     switch (pred) {
         case VS_PRED_WITHIN: VS_LAUNCH_T(VS_PRED_WITHIN); break;
         case VS_PRED_INTERSECTS: VS_LAUNCH_T(VS_PRED_INTERSECTS); break;
@@ -514,34 +457,6 @@ int launch_traverse(int pred, const double* qx, const double* qy, const double4*
     VS_LAUNCH_CHECK();
     VS_TRY(timing_end(s));
     return VS_OK;
-}
-
-int finish_pairs_raw(const int2* stage, int32_t* counts, int32_t* offsets, int32_t* cursor, void* scan_ws,
-                     JoinCtl* ctl, int64_t nq, int64_t capacity, int64_t* out_l, int64_t* out_r, int64_t* d_result,
-                     hipStream_t s) {
-    VS_TRY(exclusive_scan_i32(counts, offsets, nq, &ctl->total, scan_ws, s));
-    if (nq > 0) {
-        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, cursor, nq, 0);
-        VS_LAUNCH_CHECK();
-    }
-    const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, 256), 65536));
-    hipLaunchKernelGGL(scatter_pairs_kernel, dim3(sgrid), dim3(256), 0, s, stage, ctl, capacity, offsets, cursor,
-                       out_l, out_r);
-    VS_LAUNCH_CHECK();
-    if (nq > 0) {
-        hipLaunchKernelGGL(segment_sort_kernel, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, ctl, capacity,
-                           offsets, counts, nq, out_r);
-        VS_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(publish_result_kernel, dim3(1), dim3(64), 0, s, ctl, d_result);
-    VS_LAUNCH_CHECK();
-    return VS_OK;
-}
-
-int finish_pairs(const JoinWs& J, int64_t nq, int64_t capacity, int64_t* out_l, int64_t* out_r,
-                 int64_t* d_result, hipStream_t s) {
-    return finish_pairs_raw(J.stage, J.counts, J.offsets, J.cursor, J.scan_ws, J.ctl, nq, capacity, out_l, out_r,
-                            d_result, s);
 }
 
 int build_refine_flags(const int32_t* part, const int32_t* ring, int64_t n_parts, int64_t n_rings, int64_t n_coords,
@@ -636,12 +551,7 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
     if (n_points > 0 && n_polys > 0)
         VS_TRY(vs::launch_traverse<vs::kModePip>(predicate, J.sx, J.sy, nullptr, J.perm, n_points, T, P, J,
                                                  capacity, s));
-    else if (n_points > 0) {
-        hipLaunchKernelGGL(vs::fill_i32_kernel, dim3((unsigned)vs::ceil_div(n_points, 256)), dim3(256), 0, s,
-                           J.counts, n_points, 0);
-        VS_LAUNCH_CHECK();
-    }
-    return vs::finish_pairs(J, n_points, capacity, out_left, out_right, d_result, s);
+    return vs::finish_pairs(J.fin_ws, J.ctl, n_points, capacity, true, out_left, out_right, d_result, s);
 }
 
 int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity) {
@@ -651,9 +561,9 @@ int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity) {
 }
 
 int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_layout* layout,
-                   const double* node_boxes, const int32_t* item_ids, int64_t capacity, int64_t* out_left,
-                   int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
-                   void* stream) {
+                   const double* node_boxes, const int32_t* item_ids, int64_t capacity, int32_t sort,
+                   int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                   int64_t workspace_bytes, void* stream) {
     vs::clear_error();
     VS_TRY(vs::check_layout(layout));
     VS_REQUIRE(n_queries >= 0 && n_queries < (int64_t)INT32_MAX, "vs_query_boxes: n_queries out of range");
@@ -672,12 +582,7 @@ int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_l
         VS_TRY(vs::launch_traverse<vs::kModeBox>(VS_PRED_INTERSECTS, nullptr, nullptr,
                                                  reinterpret_cast<const double4*>(query_boxes), nullptr,
                                                  n_queries, T, P, J, capacity, s));
-    else if (n_queries > 0) {
-        hipLaunchKernelGGL(vs::fill_i32_kernel, dim3((unsigned)vs::ceil_div(n_queries, 256)), dim3(256), 0, s,
-                           J.counts, n_queries, 0);
-        VS_LAUNCH_CHECK();
-    }
-    return vs::finish_pairs(J, n_queries, capacity, out_left, out_right, d_result, s);
+    return vs::finish_pairs(J.fin_ws, J.ctl, n_queries, capacity, sort != 0, out_left, out_right, d_result, s);
 }
 
 }  // extern "C"

@@ -1,8 +1,10 @@
 // Shared by the join translation units: the control block every join keeps in
-// its workspace and the finish pass (per-left counts -> scan -> scatter ->
-// per-left sort of right indices) that turns staged pairs into lexsorted
-// int64 output.
+// its workspace and the finish pass (radix sort of the staged pairs by left
+// id, then a sort of the right ids of each left id) that turns staged pairs
+// into lexsorted int64 output.
 #pragma once
+
+#include <algorithm>
 
 #include "vs_internal.h"
 
@@ -10,7 +12,8 @@ namespace vs {
 
 struct JoinCtl {
     unsigned long long staged;   // pairs appended (may exceed capacity)
-    long long total;             // scan total of per-left counts
+    unsigned int n_long;         // finish: runs of one left id longer than the short-run limit
+    unsigned int pad;
     unsigned int queue[8];       // work queues (per XCD for point waves, [0] for items)
     unsigned long long stat[4];  // join-specific counters, published in d_result[2..5]
     unsigned long long escalated_pairs;
@@ -19,10 +22,15 @@ struct JoinCtl {
 void ctl_reset(JoinCtl* ctl, hipStream_t s);
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 
-// counts[nq] must hold the number of staged pairs of each left row.
-int finish_pairs_raw(const int2* stage, int32_t* counts, int32_t* offsets, int32_t* cursor, void* scan_ws,
-                     JoinCtl* ctl, int64_t nq, int64_t capacity, int64_t* out_l, int64_t* out_r, int64_t* d_result,
-                     hipStream_t s);
+// Finish (vs_finish.hip).  A join appends pairs to the two stage arrays that
+// finish_stage carves from the finish workspace (left ids as u32, right ids as
+// i32, `capacity` slots, ctl->staged the count); finish_pairs turns them into
+// int64 output sorted by left id (and by right id within a left id when
+// sort_right) and publishes d_result[0..3].
+int64_t finish_workspace_bytes(int64_t capacity);
+void finish_stage(void* fin_ws, int64_t capacity, uint32_t** stage_l, int32_t** stage_r);
+int finish_pairs(void* fin_ws, JoinCtl* ctl, int64_t n_left, int64_t capacity, bool sort_right, int64_t* out_l,
+                 int64_t* out_r, int64_t* d_result, hipStream_t s);
 
 // Per-coordinate flags of the refine layout: bit0 first coordinate of a ring,
 // bit1 first coordinate of a part.

@@ -83,40 +83,74 @@ __device__ __forceinline__ float mlp_eval(const float* P, float u) {
     return y;
 }
 
-// Accumulates d(scale * (y - t)^2)/d(params) into g[0..96] and (y - t)^2 into loss.
-__device__ __forceinline__ void mlp_grad(const float* P, float u, float t, float scale, float* g, float& loss) {
-    float z[kHidden];
-    float y = P[3 * kHidden];
-#pragma unroll
-    for (int h = 0; h < kHidden; ++h) {
-        z[h] = P[h] * u + P[kHidden + h];
-        y += P[2 * kHidden + h] * fmaxf(z[h], 0.0f);
-    }
-    const float r = y - t;
-    loss += r * r;
-    const float dy = 2.0f * r * scale;
-    g[3 * kHidden] += dy;
-#pragma unroll
-    for (int h = 0; h < kHidden; ++h) {
-        const float a = fmaxf(z[h], 0.0f);
-        g[2 * kHidden + h] += dy * a;
-        const float dz = z[h] > 0.0f ? dy * P[2 * kHidden + h] : 0.0f;
-        g[h] += dz * u;
-        g[kHidden + h] += dz;
-    }
-}
+// Gradient of the mean squared rank error, computed by one block in two
+// phases so that no thread holds all 97 partial derivatives:
+//   phase 1  thread-strided over samples: residual r_j = y_j - t_j into LDS,
+//            partial loss and output-bias gradient;
+//   phase 2  thread t owns hidden unit h = t mod 32 for the samples
+//            j = t div 32 (mod blocks/32): with z = w1_h u_j + b1_h > 0,
+//            dy = 2 r_j scale:  dL/dw2_h += dy z,  dL/dw1_h += dy w2_h u_j,
+//            dL/db1_h += dy w2_h.
+// Parameters: P[h] = w1, P[32 + h] = b1, P[64 + h] = w2, P[96] = b2.
+constexpr int kTrainThreads = 1024;
+constexpr int kTrainWaves = kTrainThreads / 64;
 
-// Block-wide sum of g[0..kNParams] (+ loss at index kNParams) into out[].
-__device__ void block_sum_grads(float* g, float loss, float (*red)[kNParams + 1], float* out) {
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k <= kNParams; ++k) {
-        float v = k < kNParams ? g[k] : loss;
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (l == 0) red[w][k] = v;
+struct GradPart {
+    float gw1 = 0.0f, gb1 = 0.0f, gw2 = 0.0f, gb2 = 0.0f, loss = 0.0f;
+};
+
+// Accumulates the contribution of samples (su, st)[0, S) (LDS) into gp; sr is
+// LDS scratch of S floats.  Called by every thread of the block.
+__device__ void mlp_grad_accumulate(const float* P, const float* su, const float* st, float* sr, int S, float scale,
+                                    GradPart& gp) {
+    const int tid = threadIdx.x;
+    for (int j = tid; j < S; j += blockDim.x) {
+        const float r = mlp_eval(P, su[j]) - st[j];
+        sr[j] = r;
+        gp.loss += r * r;
+        gp.gb2 += 2.0f * r * scale;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k <= kNParams; k += blockDim.x) {
+    const int h = tid & (kHidden - 1), grp = tid / kHidden, ngrp = blockDim.x / kHidden;
+    const float w1 = P[h], b1 = P[kHidden + h], w2 = P[2 * kHidden + h];
+    for (int j = grp; j < S; j += ngrp) {
+        const float u = su[j];
+        const float z = w1 * u + b1;
+        if (z > 0.0f) {
+            const float dy = 2.0f * sr[j] * scale;
+            gp.gw2 += dy * z;
+            const float dz = dy * w2;
+            gp.gw1 += dz * u;
+            gp.gb1 += dz;
+        }
+    }
+    __syncthreads();
+}
+
+// Block reduction of the per-thread parts into out[0..kNParams] (gradient,
+// then the summed squared residual at out[kNParams]).
+__device__ void mlp_grad_reduce(const GradPart& gp, float (*red)[kNParams + 1], float* out) {
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    float gw1 = gp.gw1, gb1 = gp.gb1, gw2 = gp.gw2, gb2 = gp.gb2, loss = gp.loss;
+    // lanes l and l ^ 32 hold the same hidden unit (two sample groups per wave)
+    gw1 += __shfl_xor(gw1, 32, 64);
+    gb1 += __shfl_xor(gb1, 32, 64);
+    gw2 += __shfl_xor(gw2, 32, 64);
+    for (int o = 32; o > 0; o >>= 1) {
+        gb2 += __shfl_xor(gb2, o, 64);
+        loss += __shfl_xor(loss, o, 64);
+    }
+    if (l < kHidden) {
+        red[w][l] = gw1;
+        red[w][kHidden + l] = gb1;
+        red[w][2 * kHidden + l] = gw2;
+    }
+    if (l == 0) {
+        red[w][3 * kHidden] = gb2;
+        red[w][kNParams] = loss;
+    }
+    __syncthreads();
+    for (int k = tid; k <= kNParams; k += blockDim.x) {
         float v = 0.0f;
         for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) v += red[ww][k];
         out[k] = v;
@@ -127,11 +161,12 @@ __device__ void block_sum_grads(float* g, float loss, float (*red)[kNParams + 1]
 // One block: sample, quantile-knot initialisation, `iters` full-batch Adam
 // steps; writes the best-loss parameters and [initial loss, best loss].
 template <class K>
-__global__ __launch_bounds__(256) void mlp_train_kernel(const K* __restrict__ keys, int64_t n, int iters, float lr,
-                                                       float* __restrict__ params_out, float* __restrict__ losses) {
-    __shared__ float su[kSample], st[kSample];
+__global__ __launch_bounds__(kTrainThreads) void mlp_train_kernel(const K* __restrict__ keys, int64_t n, int iters,
+                                                                 float lr, float* __restrict__ params_out,
+                                                                 float* __restrict__ losses) {
+    __shared__ float su[kSample], st[kSample], sr[kSample];
     __shared__ float P[kNParams], M1[kNParams], M2[kNParams], SC[kNParams], BEST[kNParams];
-    __shared__ float red[4][kNParams + 1];
+    __shared__ float red[kTrainWaves][kNParams + 1];
     __shared__ float tot[kNParams + 1];
     __shared__ float best_loss, init_loss;
     const int tid = threadIdx.x;
@@ -142,25 +177,30 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const K* __restrict__ ke
         st[j] = (float)((double)i / (double)n);
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < kHidden) {
         // knots q_h at sample ranks h/H, PL interpolant g(u) = t_0 + sum_h dS_h relu(u - q_h)
-        float prev_slope = 0.0f;
-        for (int h = 0; h < kHidden; ++h) {
-            const int j0 = (int)((int64_t)h * (S - 1) / kHidden);
-            const int j1 = (int)((int64_t)(h + 1) * (S - 1) / kHidden);
-            const float q0 = su[j0], q1 = su[j1];
-            const float slope = (st[j1] - st[j0]) / fmaxf(q1 - q0, 1.0e-7f);
-            P[h] = 1.0f;
-            P[kHidden + h] = -q0;
-            P[2 * kHidden + h] = slope - prev_slope;
-            prev_slope = slope;
-            SC[h] = 0.05f;
-            SC[kHidden + h] = fmaxf(q1 - q0, 1.0e-7f);
-            SC[2 * kHidden + h] = fmaxf(fabsf(slope), 1.0e-3f);
+        const int h = tid;
+        auto slope_of = [&](int hh) {
+            const int j0 = (int)((int64_t)hh * (S - 1) / kHidden);
+            const int j1 = (int)((int64_t)(hh + 1) * (S - 1) / kHidden);
+            return (st[j1] - st[j0]) / fmaxf(su[j1] - su[j0], 1.0e-7f);
+        };
+        const int j0 = (int)((int64_t)h * (S - 1) / kHidden);
+        const int j1 = (int)((int64_t)(h + 1) * (S - 1) / kHidden);
+        const float q0 = su[j0], q1 = su[j1];
+        const float slope = slope_of(h);
+        const float prev_slope = h > 0 ? slope_of(h - 1) : 0.0f;
+        P[h] = 1.0f;
+        P[kHidden + h] = -q0;
+        P[2 * kHidden + h] = slope - prev_slope;
+        SC[h] = 0.05f;
+        SC[kHidden + h] = fmaxf(q1 - q0, 1.0e-7f);
+        SC[2 * kHidden + h] = fmaxf(fabsf(slope), 1.0e-3f);
+        if (h == 0) {
+            P[3 * kHidden] = st[0];
+            SC[3 * kHidden] = 1.0f / kHidden;
+            best_loss = INFINITY;
         }
-        P[3 * kHidden] = st[0];
-        SC[3 * kHidden] = 1.0f / kHidden;
-        best_loss = INFINITY;
     }
     for (int k = tid; k < kNParams; k += blockDim.x) {
         M1[k] = 0.0f;
@@ -171,25 +211,22 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const K* __restrict__ ke
     const float b1 = 0.9f, b2 = 0.999f;
     float b1t = 1.0f, b2t = 1.0f;
     for (int it = 0; it <= iters; ++it) {
-        float g[kNParams];
-#pragma unroll
-        for (int k = 0; k < kNParams; ++k) g[k] = 0.0f;
-        float loss = 0.0f;
-        for (int j = tid; j < S; j += blockDim.x) mlp_grad(P, su[j], st[j], scale, g, loss);
-        block_sum_grads(g, loss, red, tot);
-        if (tid == 0) {
-            const float L = tot[kNParams] * scale;
-            if (it == 0) init_loss = L;
-            if (L < best_loss) {
-                best_loss = L;
-                for (int k = 0; k < kNParams; ++k) BEST[k] = P[k];
-            }
-        }
+        GradPart gp;
+        mlp_grad_accumulate(P, su, st, sr, S, scale, gp);
+        mlp_grad_reduce(gp, red, tot);
+        const float L = tot[kNParams] * scale;
+        const bool better = L < best_loss;  // uniform: every thread reads the same LDS values
+        if (tid < kNParams && better) BEST[tid] = P[tid];
         __syncthreads();
+        if (tid == 0) {
+            if (it == 0) init_loss = L;
+            if (better) best_loss = L;
+        }
         if (it == iters) break;
         b1t *= b1;
         b2t *= b2;
-        for (int k = tid; k < kNParams; k += blockDim.x) {
+        if (tid < kNParams) {
+            const int k = tid;
             const float gk = tot[k];
             M1[k] = b1 * M1[k] + (1.0f - b1) * gk;
             M2[k] = b2 * M2[k] + (1.0f - b2) * gk * gk;
@@ -198,6 +235,7 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const K* __restrict__ ke
         }
         __syncthreads();
     }
+    __syncthreads();
     for (int k = tid; k < kNParams; k += blockDim.x) params_out[k] = BEST[k];
     if (tid == 0) {
         losses[0] = init_loss;
@@ -207,22 +245,30 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const K* __restrict__ ke
 
 // Full-batch loss and gradient of given parameters on given (u, t) data
 // (the same device code the trainer uses), for checking against a CPU oracle.
-__global__ __launch_bounds__(256) void mlp_loss_grad_kernel(const float* __restrict__ params,
-                                                           const float* __restrict__ u, const float* __restrict__ t,
-                                                           int64_t n, float* __restrict__ grad_out,
-                                                           float* __restrict__ loss_out) {
+__global__ __launch_bounds__(kTrainThreads) void mlp_loss_grad_kernel(const float* __restrict__ params,
+                                                                     const float* __restrict__ u,
+                                                                     const float* __restrict__ t, int64_t n,
+                                                                     float* __restrict__ grad_out,
+                                                                     float* __restrict__ loss_out) {
+    __shared__ float su[kSample], st[kSample], sr[kSample];
     __shared__ float P[kNParams];
-    __shared__ float red[4][kNParams + 1];
+    __shared__ float red[kTrainWaves][kNParams + 1];
     __shared__ float tot[kNParams + 1];
     for (int k = threadIdx.x; k < kNParams; k += blockDim.x) P[k] = params[k];
-    __syncthreads();
-    float g[kNParams];
-#pragma unroll
-    for (int k = 0; k < kNParams; ++k) g[k] = 0.0f;
-    float loss = 0.0f;
     const float scale = 1.0f / (float)(n > 0 ? n : 1);
-    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) mlp_grad(P, u[j], t[j], scale, g, loss);
-    block_sum_grads(g, loss, red, tot);
+    GradPart gp;
+    for (int64_t c0 = 0; c0 < n; c0 += kSample) {
+        const int S = (n - c0) < kSample ? (int)(n - c0) : kSample;
+        __syncthreads();
+        for (int j = threadIdx.x; j < S; j += blockDim.x) {
+            su[j] = u[c0 + j];
+            st[j] = t[c0 + j];
+        }
+        __syncthreads();
+        mlp_grad_accumulate(P, su, st, sr, S, scale, gp);
+    }
+    __syncthreads();
+    mlp_grad_reduce(gp, red, tot);
     for (int k = threadIdx.x; k < kNParams; k += blockDim.x) grad_out[k] = tot[k];
     if (threadIdx.x == 0) loss_out[0] = tot[kNParams] * scale;
 }
@@ -253,10 +299,10 @@ __device__ __forceinline__ int mlp_leaf(const float* P, K k, int S) {
     return (int)f;
 }
 
+// Linear interpolation of the rank of q inside leaf s, which spans sorted
+// positions [p0, p1] and keys [k0, k1].
 template <class K>
-__device__ __forceinline__ double plm_pred(K q, int s, const K* leaf_key, int64_t n, int S) {
-    const int64_t p0 = leaf_pos(s, n, S), p1 = leaf_pos(s + 1, n, S);
-    const K k0 = leaf_key[s], k1 = leaf_key[s + 1];
+__device__ __forceinline__ double plm_pred(K q, K k0, K k1, int64_t p0, int64_t p1) {
     if (!(k1 > k0)) return (double)p0;
     const double f = (double)(K)(q - k0) / (double)(K)(k1 - k0);
     return (double)p0 + f * (double)(p1 - p0);
@@ -282,17 +328,18 @@ __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ key
     const int lane = threadIdx.x & 63;
     if (s >= S) return;
     const int64_t p0 = leaf_pos(s, n, S), p1 = leaf_pos(s + 1, n, S);
+    const K k0 = leaf_key[s], k1 = leaf_key[s + 1];
     double emax = 0.0;
     int mlp_e = 0;
     for (int64_t i = p0 + lane; i <= p1; i += 64) {
         const K k = keys[i];
-        const double e = fabs(plm_pred<K>(k, (int)s, leaf_key, n, S) - (double)i);
+        const double e = fabs(plm_pred<K>(k, k0, k1, p0, p1) - (double)i);
         emax = fmax(emax, e);
-        if (i > p0 && (i == 0 || k != keys[i - 1])) {
-            // true leaf of a first occurrence at position i >= 1
-            const int64_t st = (i * (int64_t)S - 1) / (n - 1);
+        if (i > p0 && k != keys[i - 1]) {
+            // a first occurrence at position i in (p0, p1]: the lookup of its key
+            // must land in leaf s (the last leaf whose first key is below it)
             const int sp = mlp_leaf<K>(P, k, S);
-            const int d = (int)(sp > st ? sp - st : st - sp);
+            const int d = sp > (int)s ? sp - (int)s : (int)s - sp;
             mlp_e = d > mlp_e ? d : mlp_e;
         }
     }
@@ -346,8 +393,8 @@ __device__ int64_t learned_lower_bound(K q, const LearnedView<K>& V, LookupStats
             else wh = mid - 1;
         }
         const int s = wl;
-        const int64_t lo = leaf_pos(s, n, S) + 1, hi = leaf_pos(s + 1, n, S);
-        const double p = plm_pred<K>(q, s, V.leaf_key, n, S);
+        const int64_t p0 = leaf_pos(s, n, S), hi = leaf_pos(s + 1, n, S), lo = p0 + 1;
+        const double p = plm_pred<K>(q, V.leaf_key[s], V.leaf_key[s + 1], p0, hi);
         const int32_t e = V.leaf_err[s];
         a = (int64_t)ceil(p - (double)e);
         b = (int64_t)floor(p + (double)e) + 1;
@@ -532,17 +579,99 @@ __device__ __forceinline__ bool loc_pred(int loc) {
 constexpr int kEdgeCap = 512;  // edges staged in LDS per batch
 constexpr int kOutCap = 2048;  // per-block output buffer (pairs)
 
-// Runs every staged edge against this thread's candidates.
-__device__ __forceinline__ void refine_edges(const float4* __restrict__ Et, const int32_t* __restrict__ Ec, int ne,
-                                             const RefineView& R, const float* pu, const float* pv, const double* pxd,
-                                             const double* pyd, float E, int* loc, int* par, bool* done, bool* esc,
-                                             unsigned& n_orient, unsigned long long& n_esc) {
-#pragma unroll 4
-    for (int e = 0; e < ne; ++e) {
-        const float4 ed = Et[e];
+// ---------------------------------------------------------------------------
+// y-slab lists of the staged edges.  The staged batch's y-range [ylo, yhi]
+// (recentred fp32) is cut into B equal slabs; edge e is listed in every slab
+// from slab(min(ay, by)) to slab(max(ay, by)).  slab() is monotone in y, so a
+// point whose fp32 y lies in an edge's closed fp32 y-range finds that edge in
+// its own slab; every other edge has both endpoints strictly above or below
+// the point and leaves its parity and boundary state unchanged
+// (edge_step_f32 returns 0 for it without escalating).  Parity is an XOR and
+// any boundary hit decides, so the order within a slab does not matter: the
+// binned refine makes exactly the decisions of the full edge loop.
+// ---------------------------------------------------------------------------
+constexpr int kMaxBins = 64;
+constexpr int kBinCap = 4096;  // slab entries per staged batch (u16 edge indices)
+
+struct Slabs {
+    int B;      // slab count (1: one slab holding every edge)
+    float ylo;  // fp32 y of the first slab's lower edge
+    float inv;  // slabs per unit of y
+};
+
+__device__ __forceinline__ int slab_of(float y, const Slabs& S) {
+    float f = (y - S.ylo) * S.inv;
+    f = fminf(fmaxf(f, 0.0f), (float)(S.B - 1));
+    return (int)f;
+}
+
+// Block-uniform.  Builds start[0..B] and list[] for the ne staged edges.
+__device__ Slabs build_slabs(const float4* __restrict__ Et, int ne, float ylo, float yhi, int* start, int* cur,
+                             uint16_t* list) {
+    const int tid = threadIdx.x;
+    Slabs S;
+    S.B = 1;
+    while (S.B < kMaxBins && S.B * 4 < ne) S.B <<= 1;
+    S.ylo = ylo;
+    S.inv = (yhi > ylo) ? (float)S.B / (yhi - ylo) : 0.0f;
+    if (!(S.inv > 0.0f) || !(S.inv < INFINITY)) {
+        S.B = 1;
+        S.inv = 0.0f;
+    }
+    if (tid <= kMaxBins) start[tid] = 0;
+    __syncthreads();
+    if (S.B > 1) {
+        for (int e = tid; e < ne; e += kLJThreads) {
+            const float4 ed = Et[e];
+            const int b0 = slab_of(fminf(ed.y, ed.w), S), b1 = slab_of(fmaxf(ed.y, ed.w), S);
+            for (int b = b0; b <= b1; ++b) atomicAdd(&start[b + 1], 1);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            int v = tid < S.B ? start[tid + 1] : 0;
+            v = wave_inclusive_scan(v);
+            if (tid < S.B) start[tid + 1] = v;
+        }
+        __syncthreads();
+        const int total = start[S.B];
+        __syncthreads();
+        if (total <= kBinCap) {
+            if (tid < S.B) cur[tid] = start[tid];
+            __syncthreads();
+            for (int e = tid; e < ne; e += kLJThreads) {
+                const float4 ed = Et[e];
+                const int b0 = slab_of(fminf(ed.y, ed.w), S), b1 = slab_of(fmaxf(ed.y, ed.w), S);
+                for (int b = b0; b <= b1; ++b) list[atomicAdd(&cur[b], 1)] = (uint16_t)e;
+            }
+            __syncthreads();
+            return S;
+        }
+        S.B = 1;  // too many long edges for the slab lists: one slab
+        S.inv = 0.0f;
+    }
+    if (tid == 0) {
+        start[0] = 0;
+        start[1] = ne;
+    }
+    for (int e = tid; e < ne; e += kLJThreads) list[e] = (uint16_t)e;
+    __syncthreads();
+    return S;
+}
+
+// Runs this thread's candidates against the edges of their own slabs.
+__device__ __forceinline__ void refine_slabs(const float4* __restrict__ Et, const int32_t* __restrict__ Ec,
+                                             const int* __restrict__ start, const uint16_t* __restrict__ list,
+                                             const Slabs& S, const RefineView& R, const float* pu, const float* pv,
+                                             const double* pxd, const double* pyd, float E, int* loc, int* par,
+                                             bool* done, bool* esc, unsigned& n_orient, unsigned long long& n_esc) {
 #pragma unroll
-        for (int j = 0; j < kCandPerThread; ++j) {
-            if (done[j]) continue;
+    for (int j = 0; j < kCandPerThread; ++j) {
+        if (done[j]) continue;
+        const int b = slab_of(pv[j], S);
+        const int k1 = start[b + 1];
+        for (int k = start[b]; k < k1; ++k) {
+            const int e = list[k];
+            const float4 ed = Et[e];
             if (edge_step_f32(pu[j], pv[j], ed.x, ed.y, ed.z, ed.w, E, par[j], n_orient)) {
                 ++n_esc;
                 esc[j] = true;
@@ -550,21 +679,35 @@ __device__ __forceinline__ void refine_edges(const float4* __restrict__ Et, cons
                 if (edge_step_f64(pxd[j], pyd[j], R.x[c - 1], R.y[c - 1], R.x[c], R.y[c], par[j])) {
                     loc[j] = 1;
                     done[j] = true;
+                    break;
                 }
             }
         }
     }
 }
 
+// LDS buffers of the staged edges and their slab lists.
+struct EdgeLds {
+    float4* Et;
+    int32_t* Ec;
+    int* start;
+    int* cur;
+    uint16_t* list;
+    int* wsum;
+};
+
 // Stages the edges (c-1, c), c in (lo, hi), of a single-part polygon that can
 // affect a point in [xmn, inf) x [ymn, ymx] (all of them when the bounds are
-// infinite) into Et/Ec in coordinate order, running refine_edges whenever the
-// buffer fills.  Block-uniform; returns the number left staged.
+// infinite) into Et/Ec in coordinate order, refining the candidates against a
+// full buffer before reusing it.  Block-uniform; returns the number left staged.
 __device__ int stage_edges(int32_t lo, int32_t hi, double xmn, double ymn, double ymx, double cx, double cy,
-                           const RefineView& R, float4* Et, int32_t* Ec, int* s_wsum, int ne, const float* pu,
-                           const float* pv, const double* pxd, const double* pyd, float E, int* loc, int* par,
-                           bool* done, bool* esc, unsigned& n_orient, unsigned long long& n_esc) {
+                           const RefineView& R, const EdgeLds& L, int ne, const float* pu, const float* pv,
+                           const double* pxd, const double* pyd, float E, int* loc, int* par, bool* done, bool* esc,
+                           unsigned& n_orient, unsigned long long& n_esc) {
     const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    float4* Et = L.Et;
+    int32_t* Ec = L.Ec;
+    int* s_wsum = L.wsum;
     for (int32_t base = lo; base < hi; base += kLJThreads) {
         const int32_t c = base + tid;
         bool rel = false;
@@ -585,7 +728,8 @@ __device__ int stage_edges(int32_t lo, int32_t hi, double xmn, double ymn, doubl
             tot += s_wsum[w2];
         }
         if (ne + tot > kEdgeCap) {
-            refine_edges(Et, Ec, ne, R, pu, pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
+            const Slabs S = build_slabs(Et, ne, (float)(ymn - cy), (float)(ymx - cy), L.start, L.cur, L.list);
+            refine_slabs(Et, Ec, L.start, L.list, S, R, pu, pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
             ne = 0;
             __syncthreads();
         }
@@ -614,8 +758,8 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
     const int32_t* __restrict__ item_q, int64_t item_cap, const int32_t* __restrict__ order,
     const int4* __restrict__ poly_info, const int2* __restrict__ ranges, const int32_t* __restrict__ nranges,
     const double* __restrict__ sx, const double* __restrict__ sy, const int32_t* __restrict__ perm,
-    const double4* __restrict__ qbox, RefineView R, int2* __restrict__ stage, int64_t capacity,
-    JoinCtl* __restrict__ ctl, int32_t* __restrict__ counts) {
+    const double4* __restrict__ qbox, RefineView R, uint32_t* __restrict__ stage_l, int32_t* __restrict__ stage_r,
+    int64_t capacity, JoinCtl* __restrict__ ctl) {
     __shared__ int s_item, s_q, s_chunk, s_ncand, s_nout, s_nr;
     __shared__ unsigned long long s_base;
     __shared__ int2 s_rng[kMaxRanges];
@@ -625,6 +769,9 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
     __shared__ int32_t Ec[kEdgeCap];
     __shared__ int s_wsum[kLJThreads / 64];
     __shared__ double s_ext[kLJThreads / 64][3];
+    __shared__ int s_bstart[kMaxBins + 1], s_bcur[kMaxBins];
+    __shared__ uint16_t s_blist[kBinCap];
+    const EdgeLds L = {Et, Ec, s_bstart, s_bcur, s_blist, s_wsum};
     const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
     const long long T = *total_items;
     unsigned long long st_cand = 0, st_cverts = 0, st_orient = 0, st_esc = 0, st_escpairs = 0;
@@ -637,7 +784,11 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
         const int nout = s_nout;
         for (int k = tid; k < nout; k += kLJThreads) {
             const unsigned long long slot = s_base + (unsigned long long)k;
-            if (slot < (unsigned long long)capacity) stage[slot] = outb[k];
+            if (slot < (unsigned long long)capacity) {
+                const int2 e = outb[k];
+                stage_l[slot] = (uint32_t)e.x;
+                stage_r[slot] = e.y;
+            }
         }
         __syncthreads();
         if (tid == 0) s_nout = 0;
@@ -680,6 +831,7 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
         double cx = 0, cy = 0;
         float E = 0.0f;
         int ne_fixed = -1;  // edges staged once for the whole item (small single-part polygons)
+        Slabs fixed_slabs = {1, 0.0f, 0.0f};
         if (MODE == kLModePip) {
             info = poly_info[q];
             cx = 0.5 * (box.x + box.z);
@@ -722,7 +874,6 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
                 for (int k = tid; k < ncand; k += kLJThreads) outb[base + k] = make_int2(q, perm[cand[k]]);
                 __syncthreads();
                 if (tid == 0) {
-                    atomicAdd(&counts[q], ncand);
                     s_nout = base + ncand;
                     s_ncand = 0;
                 }
@@ -759,10 +910,14 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
             unsigned n_orient = 0;
             unsigned long long n_esc = 0;
             if (info.z == 1 && c1 - c0 <= kEdgeCap + 1) {
-                if (ne_fixed < 0)  // stage every edge once for the item
-                    ne_fixed = stage_edges(c0, c1, -INFINITY, -INFINITY, INFINITY, cx, cy, R, Et, Ec, s_wsum, 0, pu,
-                                           pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
-                refine_edges(Et, Ec, ne_fixed, R, pu, pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
+                if (ne_fixed < 0) {  // stage every edge and its slabs once for the item
+                    ne_fixed = stage_edges(c0, c1, -INFINITY, -INFINITY, INFINITY, cx, cy, R, L, 0, pu, pv, pxd, pyd,
+                                           E, loc, par, done, esc, n_orient, n_esc);
+                    fixed_slabs = build_slabs(Et, ne_fixed, (float)(box.y - cy), (float)(box.w - cy), s_bstart,
+                                              s_bcur, s_blist);
+                }
+                refine_slabs(Et, Ec, s_bstart, s_blist, fixed_slabs, R, pu, pv, pxd, pyd, E, loc, par, done, esc,
+                             n_orient, n_esc);
             } else if (info.z == 1) {
                 exmin = wave_min(exmin);
                 eymin = wave_min(eymin);
@@ -780,9 +935,11 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
                     ymx = fmax(ymx, s_ext[w2][2]);
                 }
                 __syncthreads();
-                const int ne = stage_edges(c0, c1, xmn, ymn, ymx, cx, cy, R, Et, Ec, s_wsum, 0, pu, pv, pxd, pyd, E,
-                                           loc, par, done, esc, n_orient, n_esc);
-                refine_edges(Et, Ec, ne, R, pu, pv, pxd, pyd, E, loc, par, done, esc, n_orient, n_esc);
+                const int ne = stage_edges(c0, c1, xmn, ymn, ymx, cx, cy, R, L, 0, pu, pv, pxd, pyd, E, loc, par,
+                                           done, esc, n_orient, n_esc);
+                const Slabs S = build_slabs(Et, ne, (float)(ymn - cy), (float)(ymx - cy), s_bstart, s_bcur, s_blist);
+                refine_slabs(Et, Ec, s_bstart, s_blist, S, R, pu, pv, pxd, pyd, E, loc, par, done, esc, n_orient,
+                             n_esc);
             } else {
                 float2* Vt = reinterpret_cast<float2*>(Et);
                 uint8_t* Ft = reinterpret_cast<uint8_t*>(Ec);
@@ -839,7 +996,6 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
                     const int32_t orig = perm[cand[k]];
                     const int slot = atomicAdd(&s_nout, 1);
                     outb[slot] = make_int2(orig, q);
-                    atomicAdd(&counts[orig], 1);
                 }
             }
             if (tid == 0) {
@@ -972,7 +1128,7 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, B.meta, 4);
     VS_LAUNCH_CHECK();
     if (n < 2) return VS_OK;
-    hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(256), 0, s, B.keys, n, iters, lr, B.mlp, B.losses);
+    hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(vs::kTrainThreads), 0, s, B.keys, n, iters, lr, B.mlp, B.losses);
     VS_LAUNCH_CHECK();
     hipLaunchKernelGGL(leaf_keys_kernel<K>, dim3((unsigned)ceil_div(S + 1, 256)), dim3(256), 0, s, B.keys, n, S,
                        B.leaf_key);
@@ -1012,12 +1168,10 @@ struct QueryBufs {
     long long* total_items;
     unsigned long long* cstat;
     void* scan_ws;
-    int2* stage;
-    int32_t* counts;
-    int32_t* offsets;
-    int32_t* cursor;
-    void* fin_scan_ws;
     JoinCtl* ctl;
+    void* fin_ws;
+    uint32_t* stage_l;
+    int32_t* stage_r;
 };
 
 // Items map: room for 4 items per query plus one per kChunk/4 left rows covers
@@ -1040,12 +1194,11 @@ QueryBufs carve_query(Carver& c, int64_t nq, int64_t n_left, int64_t capacity) {
     Q.total_items = c.take<long long>(1);
     Q.cstat = c.take<unsigned long long>(4);
     Q.scan_ws = c.take<char>(scan_workspace_bytes(nq));
-    Q.stage = c.take<int2>(capacity);
-    Q.counts = c.take<int32_t>(n_left);
-    Q.offsets = c.take<int32_t>(n_left);
-    Q.cursor = c.take<int32_t>(n_left);
-    Q.fin_scan_ws = c.take<char>(scan_workspace_bytes(n_left));
     Q.ctl = c.take<JoinCtl>(1);
+    Q.fin_ws = c.take<char>(finish_workspace_bytes(capacity));
+    Q.stage_l = nullptr;
+    Q.stage_r = nullptr;
+    if (c.base) finish_stage(Q.fin_ws, capacity, &Q.stage_l, &Q.stage_r);
     return Q;
 }
 
@@ -1055,12 +1208,10 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 
 template <class K>
 int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const LearnedView<K>& V,
-                    const IndexBufs<K>& B, const RefineView& R, int64_t n_left, int64_t capacity,
+                    const IndexBufs<K>& B, const RefineView& R, int64_t n_left, int64_t capacity, bool sort,
                     const QueryBufs& Q, int64_t* out_l, int64_t* out_r, int64_t* d_result, hipStream_t s) {
     ctl_reset(Q.ctl, s);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, Q.cstat, 4);
-    VS_LAUNCH_CHECK();
-    fill_i32(Q.counts, n_left, 0, s);
     VS_LAUNCH_CHECK();
     if (nq > 0) {
         const unsigned gq = (unsigned)ceil_div(nq, 256);
@@ -1091,7 +1242,7 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
 #define VS_LJ(M, P)                                                                                          \
     hipLaunchKernelGGL((learned_join_kernel<M, P>), dim3(grid), dim3(kLJThreads), 0, s, nq, Q.item_off,       \
                        Q.total_items, Q.item_q, Q.item_cap, Q.order, Q.poly_info, Q.ranges, Q.nranges, B.sx, B.sy,  \
-                       B.perm, qbox, R, Q.stage, capacity, Q.ctl, Q.counts)
+                       B.perm, qbox, R, Q.stage_l, Q.stage_r, capacity, Q.ctl)
         if (mode == kLModeBox) VS_LJ(kLModeBox, VS_PRED_INTERSECTS);
         else if (pred == VS_PRED_WITHIN) VS_LJ(kLModePip, VS_PRED_WITHIN);
         else if (pred == VS_PRED_TOUCHES) VS_LJ(kLModePip, VS_PRED_TOUCHES);
@@ -1100,8 +1251,7 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
         VS_LAUNCH_CHECK();
         VS_TRY(timing_end_public(s));
     }
-    VS_TRY(finish_pairs_raw(Q.stage, Q.counts, Q.offsets, Q.cursor, Q.fin_scan_ws, Q.ctl, n_left, capacity, out_l,
-                            out_r, d_result, s));
+    VS_TRY(finish_pairs(Q.fin_ws, Q.ctl, n_left, capacity, sort, out_l, out_r, d_result, s));
     hipLaunchKernelGGL(publish_learned_kernel, dim3(1), dim3(64), 0, s, Q.ctl, Q.cstat, B.meta, d_result);
     VS_LAUNCH_CHECK();
     return VS_OK;
@@ -1195,7 +1345,7 @@ int vs_mlp_loss_grad(const float* params, const float* u, const float* t, int64_
                      void* stream) {
     vs::clear_error();
     VS_REQUIRE(n >= 0, "vs_mlp_loss_grad: negative n");
-    hipLaunchKernelGGL(vs::mlp_loss_grad_kernel, dim3(1), dim3(256), 0, vs::stream_of(stream), params, u, t, n,
+    hipLaunchKernelGGL(vs::mlp_loss_grad_kernel, dim3(1), dim3(vs::kTrainThreads), 0, vs::stream_of(stream), params, u, t, n,
                        grad_out, loss_out);
     VS_LAUNCH_CHECK();
     return VS_OK;
@@ -1209,8 +1359,8 @@ int64_t vs_learned_query_boxes_workspace_bytes(int64_t n_boxes, int64_t capacity
 
 int vs_learned_query_boxes(const void* index, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
                            const double* boxes, int64_t n_boxes, int32_t lookup_mode, int64_t capacity,
-                           int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
-                           int64_t workspace_bytes, void* stream) {
+                           int32_t sort, int64_t* out_left, int64_t* out_right, int64_t* d_result,
+                           void* workspace, int64_t workspace_bytes, void* stream) {
     vs::clear_error();
     VS_REQUIRE(key_bytes == 4 || key_bytes == 8, "vs_learned_query_boxes: key_bytes must be 4 or 8");
     VS_REQUIRE(n_boxes >= 0 && n_boxes < (int64_t)INT32_MAX, "vs_learned_query_boxes: n_boxes out of range");
@@ -1229,12 +1379,12 @@ int vs_learned_query_boxes(const void* index, int64_t n_points, int32_t key_byte
         const auto B = vs::carve_index<uint32_t>(ci, n_points, n_leaves);
         return vs::learned_query_t<uint32_t>(vs::kLModeBox, VS_PRED_INTERSECTS, qb, n_boxes,
                                              vs::view_of<uint32_t>(B, n_points, n_leaves, lookup_mode), B, R, n_boxes,
-                                             capacity, Q, out_left, out_right, d_result, s);
+                                             capacity, sort != 0, Q, out_left, out_right, d_result, s);
     }
     const auto B = vs::carve_index<uint64_t>(ci, n_points, n_leaves);
     return vs::learned_query_t<uint64_t>(vs::kLModeBox, VS_PRED_INTERSECTS, qb, n_boxes,
                                          vs::view_of<uint64_t>(B, n_points, n_leaves, lookup_mode), B, R, n_boxes,
-                                         capacity, Q, out_left, out_right, d_result, s);
+                                         capacity, sort != 0, Q, out_left, out_right, d_result, s);
 }
 
 int64_t vs_sjoin_points_polygons_learned_workspace_bytes(int64_t n_points, int64_t n_polys, int64_t n_coords,
@@ -1278,7 +1428,7 @@ int vs_sjoin_points_polygons_learned(const double* px, const double* py, int64_t
     const vs::RefineView R = {x, y, geom_offsets, part_offsets, ring_offsets, flags};
     return vs::learned_query_t<uint32_t>(vs::kLModePip, predicate, reinterpret_cast<const double4*>(poly_bounds),
                                          n_polys, vs::view_of<uint32_t>(B, n_points, S, lookup_mode), B, R, n_points,
-                                         capacity, Q, out_left, out_right, d_result, s);
+                                         capacity, true, Q, out_left, out_right, d_result, s);
 }
 
 }  // extern "C"

@@ -12,6 +12,7 @@
 // output (replaces d_left[keep], spatial/query_utils.py:867-868).
 #include "vs_exact.h"
 #include "vs_internal.h"
+#include "vs_polys.h"
 
 namespace vs {
 
@@ -21,14 +22,6 @@ int64_t scan_workspace_bytes(int64_t n);
 int sort_pairs_u64(uint64_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
 int64_t sort_workspace_bytes_u64(int64_t n);
 
-struct Polys {
-    const double* x;
-    const double* y;
-    const int32_t* geom;
-    const int32_t* part;
-    const int32_t* ring;
-};
-
 __device__ __forceinline__ int loc_keep(int pred, int loc) {
     switch (pred) {
         case VS_PRED_WITHIN:
@@ -36,35 +29,6 @@ __device__ __forceinline__ int loc_keep(int pred, int loc) {
         case VS_PRED_TOUCHES: return loc == 1;
         default: return loc != 0;
     }
-}
-
-__device__ bool polygons_intersect(const Polys& A, int64_t ra, const Polys& B, int64_t rb) {
-    const int32_t ap0 = A.geom[ra], ap1 = A.geom[ra + 1];
-    const int32_t bp0 = B.geom[rb], bp1 = B.geom[rb + 1];
-    const int32_t ar0 = A.part[ap0], ar1 = A.part[ap1];
-    const int32_t br0 = B.part[bp0], br1 = B.part[bp1];
-    if (A.ring[ar1] <= A.ring[ar0] || B.ring[br1] <= B.ring[br0]) return false;
-    for (int32_t ar = ar0; ar < ar1; ++ar)
-        for (int32_t ai = A.ring[ar] + 1; ai < A.ring[ar + 1]; ++ai) {
-            const double ax0 = A.x[ai - 1], ay0 = A.y[ai - 1], ax1 = A.x[ai], ay1 = A.y[ai];
-            for (int32_t br = br0; br < br1; ++br)
-                for (int32_t bi = B.ring[br] + 1; bi < B.ring[br + 1]; ++bi)
-                    if (segments_touch(ax0, ay0, ax1, ay1, B.x[bi - 1], B.y[bi - 1], B.x[bi], B.y[bi]))
-                        return true;
-        }
-    for (int32_t ar = ar0; ar < ar1; ++ar) {
-        const int32_t c0 = A.ring[ar], c1 = A.ring[ar + 1];
-        const int32_t vl = c1 > c0 + 1 ? c1 - 1 : c1;
-        for (int32_t v = c0; v < vl; ++v)
-            if (point_location_f64(A.x[v], A.y[v], B.x, B.y, B.geom, B.part, B.ring, rb) != 0) return true;
-    }
-    for (int32_t br = br0; br < br1; ++br) {
-        const int32_t c0 = B.ring[br], c1 = B.ring[br + 1];
-        const int32_t vl = c1 > c0 + 1 ? c1 - 1 : c1;
-        for (int32_t v = c0; v < vl; ++v)
-            if (point_location_f64(B.x[v], B.y[v], A.x, A.y, A.geom, A.part, A.ring, ra) != 0) return true;
-    }
-    return false;
 }
 
 // mode 0: left = points (px, py), right = polygons B; keep = pred(location).

@@ -145,12 +145,20 @@ constexpr int kRsItems = 16;
 constexpr int kRsTile = kRsThreads * kRsItems;
 constexpr int kRsBins = 256;
 
+// Valid element count: n, or the device count *d_n when given.  A device count
+// above n means the producer overflowed its buffer: nothing is sorted.
+__device__ __forceinline__ int64_t device_count(int64_t n, const unsigned long long* d_n) {
+    if (!d_n) return n;
+    const unsigned long long c = *d_n;
+    return c <= (unsigned long long)n ? (int64_t)c : 0;
+}
+
 template <class K>
 __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const K* __restrict__ keys, int64_t n,
                                                             const unsigned long long* __restrict__ d_n,
-                                                            int shift, int32_t* __restrict__ hist,
+                                                            int shift, int dmask, int32_t* __restrict__ hist,
                                                             int nblocks) {
-    if (d_n && (int64_t)*d_n < n) n = (int64_t)*d_n;
+    n = device_count(n, d_n);
     __shared__ int cnt[kRsWaves][kRsBins];
     const int w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads) (&cnt[0][0])[i] = 0;
@@ -159,7 +167,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const K* __restrict
 #pragma unroll 4
     for (int i = 0; i < kRsItems; ++i) {
         const int64_t idx = base + (int64_t)i * kRsThreads + threadIdx.x;
-        if (idx < n) atomicAdd(&cnt[w][(int)((keys[idx] >> shift) & 255)], 1);
+        if (idx < n) atomicAdd(&cnt[w][(int)((keys[idx] >> shift) & dmask)], 1);
     }
     __syncthreads();
     const int d = threadIdx.x;
@@ -173,8 +181,9 @@ template <class K>
 __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
     const K* __restrict__ keys_in, const int32_t* __restrict__ vals_in, K* __restrict__ keys_out,
     int32_t* __restrict__ vals_out, int64_t n, const unsigned long long* __restrict__ d_n, int shift,
-    const int32_t* __restrict__ hist_scan, int nblocks) {
-    if (d_n && (int64_t)*d_n < n) n = (int64_t)*d_n;
+    int dmask, const int32_t* __restrict__ hist_scan, int nblocks) {
+    n = device_count(n, d_n);
+    if ((int64_t)blockIdx.x * kRsTile >= n) return;  // whole tile past the valid count (block-uniform)
     __shared__ int wave_cnt[kRsWaves][kRsBins];
     __shared__ int digit_block_off[kRsBins];
     __shared__ long long digit_global[kRsBins];
@@ -198,7 +207,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
         const bool valid = idx < n;
         k[i] = valid ? keys_in[idx] : (K)0;
         v[i] = valid ? vals_in[idx] : 0;
-        const int d = (int)((k[i] >> shift) & 255);
+        const int d = (int)((k[i] >> shift) & dmask);
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -235,7 +244,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
     for (int i = 0; i < kRsItems; ++i) {
         const int64_t idx = base + (int64_t)i * 64 + lane;
         if (idx < n) {
-            const int d = (int)((k[i] >> shift) & 255);
+            const int d = (int)((k[i] >> shift) & dmask);
             const int pos = digit_block_off[d] + wave_cnt[w][d] + rank[i];
             skeys[pos] = k[i];
             svals[pos] = v[i];
@@ -246,7 +255,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
     const int tile_n = (int)((n - t0) < kRsTile ? (n - t0) : kRsTile);
     for (int p = tid; p < tile_n; p += kRsThreads) {
         const K kk = skeys[p];
-        const int d = (int)((kk >> shift) & 255);
+        const int d = (int)((kk >> shift) & dmask);
         const int64_t g = digit_global[d] + (p - digit_block_off[d]);
         keys_out[g] = kk;
         vals_out[g] = svals[p];
@@ -264,18 +273,18 @@ int64_t sort_workspace_bytes_t(int64_t n) {
     return c.bytes;
 }
 
-// Sorts (keys, vals) in place by bits [begin_bit, end_bit) of the key, stable.
-// n: element capacity; d_n (device, optional): the number of valid leading
-// elements, read by the kernels (min(*d_n, n) are sorted; no host sync).
+// Sorts (keys, vals) by bits [begin_bit, end_bit) of the key, stable, and
+// returns the buffers that hold the result (the inputs or the workspace's
+// alternates, by the parity of the pass count) and the other pair.
+// n: element capacity; d_n (device, optional): see device_count.
 template <class K>
-int sort_pairs_t(K* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit, int end_bit,
-                 void* ws, int64_t ws_bytes, hipStream_t s) {
+int sort_pairs_nocopy(K* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit, int end_bit,
+                      void* ws, int64_t ws_bytes, hipStream_t s, SortResult<K>* res) {
     VS_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX, "sort_pairs: n=%lld out of range", (long long)n);
     VS_REQUIRE(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit,
                "sort_pairs: bad bit range [%d, %d)", begin_bit, end_bit);
     VS_REQUIRE(ws_bytes >= sort_workspace_bytes_t<K>(n), "sort_pairs: workspace too small");
-    if (n <= 1 || begin_bit == end_bit) return VS_OK;
-    const int nb = (int)ceil_div(n, kRsTile);
+    const int nb = (int)ceil_div(n > 0 ? n : 1, kRsTile);
     Carver c(ws);
     K* kalt = c.take<K>(n);
     int32_t* valt = c.take<int32_t>(n);
@@ -285,20 +294,37 @@ int sort_pairs_t(K* keys, int32_t* vals, int64_t n, const unsigned long long* d_
     int32_t* vi = vals;
     K* ko = kalt;
     int32_t* vo = valt;
-    int passes = 0;
-    for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
-        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, n, d_n, shift, hist, nb);
-        VS_LAUNCH_CHECK();
-        VS_TRY(exclusive_scan_i32(hist, hist, (int64_t)nb * kRsBins, nullptr, scan_ws, s));
-        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n, d_n,
-                           shift, hist, nb);
-        VS_LAUNCH_CHECK();
-        K* tk = ki; ki = ko; ko = tk;
-        int32_t* tv = vi; vi = vo; vo = tv;
+    if (n > 1) {
+        for (int shift = begin_bit; shift < end_bit; shift += 8) {
+            const int width = end_bit - shift < 8 ? end_bit - shift : 8;
+            const int dmask = (1 << width) - 1;
+            hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, n, d_n, shift, dmask, hist,
+                               nb);
+            VS_LAUNCH_CHECK();
+            VS_TRY(exclusive_scan_i32(hist, hist, (int64_t)nb * kRsBins, nullptr, scan_ws, s));
+            hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n, d_n,
+                               shift, dmask, hist, nb);
+            VS_LAUNCH_CHECK();
+            K* tk = ki; ki = ko; ko = tk;
+            int32_t* tv = vi; vi = vo; vo = tv;
+        }
     }
-    if (passes & 1) {
-        VS_CHECK_HIP(hipMemcpyAsync(keys, ki, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
-        VS_CHECK_HIP(hipMemcpyAsync(vals, vi, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+    res->keys = ki;
+    res->vals = vi;
+    res->alt_keys = ko;
+    res->alt_vals = vo;
+    return VS_OK;
+}
+
+// In-place form: copies the result back when it ends in the alternates.
+template <class K>
+int sort_pairs_t(K* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit, int end_bit,
+                 void* ws, int64_t ws_bytes, hipStream_t s) {
+    SortResult<K> r;
+    VS_TRY(sort_pairs_nocopy<K>(keys, vals, n, d_n, begin_bit, end_bit, ws, ws_bytes, s, &r));
+    if (r.keys != keys) {
+        VS_CHECK_HIP(hipMemcpyAsync(keys, r.keys, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
+        VS_CHECK_HIP(hipMemcpyAsync(vals, r.vals, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
     }
     return VS_OK;
 }
@@ -311,9 +337,9 @@ int sort_pairs_u64(uint64_t* k, int32_t* v, int64_t n, int b0, int b1, void* ws,
 int sort_pairs_u32(uint32_t* k, int32_t* v, int64_t n, int b0, int b1, void* ws, int64_t wb, hipStream_t s) {
     return sort_pairs_t<uint32_t>(k, v, n, nullptr, b0, b1, ws, wb, s);
 }
-int sort_pairs_u32_devn(uint32_t* k, int32_t* v, int64_t n, const unsigned long long* d_n, int b0, int b1, void* ws,
-                        int64_t wb, hipStream_t s) {
-    return sort_pairs_t<uint32_t>(k, v, n, d_n, b0, b1, ws, wb, s);
+int sort_pairs_u32_dev(uint32_t* k, int32_t* v, int64_t n, const unsigned long long* d_n, int b0, int b1, void* ws,
+                       int64_t wb, hipStream_t s, SortResult<uint32_t>* res) {
+    return sort_pairs_nocopy<uint32_t>(k, v, n, d_n, b0, b1, ws, wb, s, res);
 }
 
 }  // namespace vs

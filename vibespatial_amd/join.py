@@ -19,6 +19,7 @@ turns into an error instead.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 from dataclasses import dataclass
 
 import torch
@@ -136,30 +137,65 @@ def sjoin_points_polygons(Q: DevicePoints, P: DevicePolygons, tree: DeviceTree, 
         retries += 1
 
 
-def query_boxes(qboxes: torch.Tensor, tree: DeviceTree, capacity: int | None = None):
-    """(left, right, stats): lexsorted (query, item) pairs whose boxes overlap."""
+PAIR_LIMIT = 2**31 - 2  # pairs one device call can hold (int32 slots)
+
+
+class _PairLimit(Exception):
+    pass
+
+
+def _split_boxes(run, boxes: torch.Tensor):
+    """run(boxes) -> (left, right, stats) or raises _PairLimit; splits the
+    query boxes in halves until every call fits PAIR_LIMIT and concatenates
+    (left ids offset by each half's start, so grouping and order carry over)."""
+    try:
+        return run(boxes)
+    except _PairLimit:
+        n = int(boxes.shape[0])
+        if n <= 1:
+            raise OverflowError("one query box has more hits than a device call can hold (2^31 - 2)") from None
+        h = n // 2
+        l0, r0, s0 = _split_boxes(run, boxes[:h])
+        l1, r1, s1 = _split_boxes(run, boxes[h:])
+        for f in dataclasses.fields(s0):  # counters add up; the model's error bound is a max
+            a, b = getattr(s0, f.name), getattr(s1, f.name)
+            setattr(s0, f.name, max(a, b) if f.name == "mlp_leaf_error" else a + b)
+        return torch.cat([l0, l1 + h]), torch.cat([r0, r1]), s0
+
+
+def query_boxes(qboxes: torch.Tensor, tree: DeviceTree, capacity: int | None = None, sort: bool = True):
+    """(left, right, stats): (query, item) pairs whose boxes overlap; lexsorted
+    when `sort`, else grouped by query with items in unspecified order.  Query
+    sets whose hits exceed one call's int32 slot limit are split."""
     dev = require_gpu()
     L = lib()
-    n = int(qboxes.shape[0])
-    cap = int(capacity) if capacity is not None else max(1024, 2 * n)
-    retries = 0
-    while True:
-        ws = _ws(L.vs_query_boxes_workspace_bytes(n, cap), dev)
-        out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
-        out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
-        res = torch.zeros(4, dtype=torch.int64, device=dev)
-        check(
-            L.vs_query_boxes(ptr(qboxes), n, ctypes.byref(tree.layout), ptr(tree.node_boxes), ptr(tree.item_ids),
-                             cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
-            "vs_query_boxes",
-        )
-        k, staged, cand, cverts = _read_result(res)
-        if staged <= cap:
-            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts)
-        if staged >= 2**31 - 1:
-            raise OverflowError(f"{staged} pairs exceed the int32 pair limit")
-        cap = staged
-        retries += 1
+
+    def run(qb):
+        n = int(qb.shape[0])
+        cap = int(capacity) if capacity is not None else max(1024, 2 * n)
+        cap = min(cap, PAIR_LIMIT)
+        retries = 0
+        while True:
+            ws = _ws(L.vs_query_boxes_workspace_bytes(n, cap), dev)
+            out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+            out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+            res = torch.zeros(4, dtype=torch.int64, device=dev)
+            check(
+                L.vs_query_boxes(ptr(qb), n, ctypes.byref(tree.layout), ptr(tree.node_boxes), ptr(tree.item_ids),
+                                 cap, int(bool(sort)), ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(),
+                                 stream()),
+                "vs_query_boxes",
+            )
+            k, staged, cand, cverts = _read_result(res)
+            if staged <= cap:
+                return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts)
+            if staged > PAIR_LIMIT:
+                raise _PairLimit()
+            del ws, out_l, out_r
+            cap = staged
+            retries += 1
+
+    return _split_boxes(run, qboxes)
 
 
 _ = _lib  # re-exported module for callers that need the raw binding
@@ -283,28 +319,36 @@ class LearnedPointIndex:
         s = st.cpu().tolist()
         return out[:m], {"probes": s[0], "mlp_fallbacks": s[1], "window_fallbacks": s[2]}
 
-    def query_boxes(self, boxes: torch.Tensor, lookup: str = "learned", capacity: int | None = None):
-        """(box, point) pairs with the point in the box (inclusive), lexsorted."""
+    def query_boxes(self, boxes: torch.Tensor, lookup: str = "learned", capacity: int | None = None,
+                    sort: bool = True):
+        """(box, point) pairs with the point in the box (inclusive); lexsorted
+        when `sort`, else grouped by box with points in unspecified order."""
         dev = require_gpu()
         L = lib()
-        nb = int(boxes.shape[0])
-        cap = int(capacity) if capacity is not None else max(1024, 4 * nb)
-        retries = 0
-        while True:
-            ws = _ws(L.vs_learned_query_boxes_workspace_bytes(nb, cap), dev)
-            out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
-            out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
-            res = torch.zeros(16, dtype=torch.int64, device=dev)
-            check(L.vs_learned_query_boxes(ptr(self.buf), self.n, self.key_bytes, self.n_leaves, ptr(boxes), nb,
-                                           LOOKUP_MODES[lookup], cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws),
-                                           ws.numel(), stream()), "vs_learned_query_boxes")
-            h = res.cpu().tolist()
-            if h[1] <= cap:
-                return out_l[: h[0]], out_r[: h[0]], _learned_stats(h, cap, retries)
-            if h[1] >= 2**31 - 1:
-                raise OverflowError(f"{h[1]} pairs exceed the int32 pair limit")
-            cap = int(h[1])
-            retries += 1
+
+        def run(bx):
+            nb = int(bx.shape[0])
+            cap = int(capacity) if capacity is not None else max(1024, 4 * nb)
+            cap = min(cap, PAIR_LIMIT)
+            retries = 0
+            while True:
+                ws = _ws(L.vs_learned_query_boxes_workspace_bytes(nb, cap), dev)
+                out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+                out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+                res = torch.zeros(16, dtype=torch.int64, device=dev)
+                check(L.vs_learned_query_boxes(ptr(self.buf), self.n, self.key_bytes, self.n_leaves, ptr(bx), nb,
+                                               LOOKUP_MODES[lookup], cap, int(bool(sort)), ptr(out_l), ptr(out_r),
+                                               ptr(res), ptr(ws), ws.numel(), stream()), "vs_learned_query_boxes")
+                h = res.cpu().tolist()
+                if h[1] <= cap:
+                    return out_l[: h[0]], out_r[: h[0]], _learned_stats(h, cap, retries)
+                if h[1] > PAIR_LIMIT:
+                    raise _PairLimit()
+                del ws, out_l, out_r
+                cap = int(h[1])
+                retries += 1
+
+        return _split_boxes(run, boxes)
 
 
 def mlp_loss_grad(params: torch.Tensor, u: torch.Tensor, t: torch.Tensor):
@@ -315,3 +359,134 @@ def mlp_loss_grad(params: torch.Tensor, u: torch.Tensor, t: torch.Tensor):
     check(lib().vs_mlp_loss_grad(ptr(params), ptr(u), ptr(t), int(u.numel()), ptr(g), ptr(loss), stream()),
           "vs_mlp_loss_grad")
     return float(loss.item()), g
+
+
+# ---------------------------------------------------------------------------
+# batched polygon x polygon intersects
+# ---------------------------------------------------------------------------
+def _encode_tile(waves_per_block: int, refine_threads: int) -> int:
+    return int(waves_per_block) | (int(refine_threads) << 8)
+
+
+TILE_CANDIDATES = [(w, t) for w in (1, 2, 4) for t in (64, 128, 256)]
+_TILE_CACHE: dict = {}
+
+
+def _tile_key(n_jobs, n_left, n_right):
+    # shape class: powers of two of the sizes (an autotune result carries over
+    # to batches of similar shape)
+    return (int(n_jobs).bit_length(), int(n_left).bit_length(), int(n_right).bit_length())
+
+
+def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Tensor, right_job: torch.Tensor,
+                       capacity: int | None = None, tile: tuple[int, int] | None = None):
+    """Many independent polygon x polygon sjoin(intersects) jobs in one launch
+    sequence.  left_job / right_job: device int64 row offsets (n_jobs + 1).
+    Returns (left, right, stats) with global row ids, lexsorted; job j's pairs
+    are the contiguous run with left in [left_job[j], left_job[j+1])."""
+    dev = require_gpu()
+    L = lib()
+    n_jobs = int(left_job.numel()) - 1
+    if int(right_job.numel()) - 1 != n_jobs:
+        raise ValueError("left_job and right_job must have the same number of jobs")
+    lj = left_job.to(device=dev, dtype=torch.int64).contiguous()
+    rj = right_job.to(device=dev, dtype=torch.int64).contiguous()
+    if tile is None:
+        tile = _TILE_CACHE.get(_tile_key(n_jobs, len(A), len(B)), (4, 256))
+    code = _encode_tile(*tile)
+    cap = int(capacity) if capacity is not None else max(1024, 4 * len(A))
+    cap = min(cap, PAIR_LIMIT)
+    retries = 0
+    while True:
+        ws = _ws(L.vs_batched_intersects_workspace_bytes(n_jobs, len(A), len(B), cap), dev)
+        out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+        out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+        res = torch.zeros(4, dtype=torch.int64, device=dev)
+        check(L.vs_batched_intersects(ptr(A.x), ptr(A.y), ptr(A.geom_offsets), ptr(A.part_offsets),
+                                      ptr(A.ring_offsets), len(A), ptr(B.x), ptr(B.y), ptr(B.geom_offsets),
+                                      ptr(B.part_offsets), ptr(B.ring_offsets), len(B), ptr(lj), ptr(rj), n_jobs,
+                                      code, cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
+              "vs_batched_intersects")
+        k, staged, cand, _ = _read_result(res)
+        if staged <= cap:
+            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, 0)
+        if staged > PAIR_LIMIT:
+            raise OverflowError(f"{staged} bbox candidates exceed the int32 slot limit of one call")
+        del ws, out_l, out_r
+        cap = staged
+        retries += 1
+
+
+def autotune_batched(A: DevicePolygons, B: DevicePolygons, left_job: torch.Tensor, right_job: torch.Tensor,
+                     reps: int = 2, candidates=None):
+    """Times every tile shape on this batch (HIP events on the current stream)
+    and caches the fastest for batches of the same shape class.  Returns
+    (best tile, {tile: ms})."""
+    require_gpu()
+    cands = list(candidates or TILE_CANDIDATES)
+    _, _, st = batched_intersects(A, B, left_job, right_job, tile=cands[0])
+    cap = st.staged + 1024
+    times = {}
+    for t in cands:
+        batched_intersects(A, B, left_job, right_job, capacity=cap, tile=t)  # warm
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            batched_intersects(A, B, left_job, right_job, capacity=cap, tile=t)
+        e1.record()
+        e1.synchronize()
+        times[t] = e0.elapsed_time(e1) / reps
+    best = min(times, key=times.get)
+    _TILE_CACHE[_tile_key(int(left_job.numel()) - 1, len(A), len(B))] = best
+    return best, times
+
+
+class BatchedIntersectsGraph:
+    """The batched join captured once in a HIP graph for fixed device inputs
+    (same tensors, same sizes): replay() re-runs every launch of the sequence
+    with one graph launch.  `capacity` bounds the bbox candidates; a replay
+    that overflows raises (capture again with a larger capacity)."""
+
+    def __init__(self, A: DevicePolygons, B: DevicePolygons, left_job: torch.Tensor, right_job: torch.Tensor,
+                 capacity: int, tile: tuple[int, int] | None = None):
+        dev = require_gpu()
+        L = lib()
+        self.n_jobs = int(left_job.numel()) - 1
+        self.lj = left_job.to(device=dev, dtype=torch.int64).contiguous()
+        self.rj = right_job.to(device=dev, dtype=torch.int64).contiguous()
+        self.A, self.B = A, B
+        self.cap = min(int(capacity), PAIR_LIMIT)
+        if tile is None:
+            tile = _TILE_CACHE.get(_tile_key(self.n_jobs, len(A), len(B)), (4, 256))
+        self.code = _encode_tile(*tile)
+        self.ws = _ws(L.vs_batched_intersects_workspace_bytes(self.n_jobs, len(A), len(B), self.cap), dev)
+        self.out_l = torch.empty(max(self.cap, 1), dtype=torch.int64, device=dev)
+        self.out_r = torch.empty(max(self.cap, 1), dtype=torch.int64, device=dev)
+        self.res = torch.zeros(4, dtype=torch.int64, device=dev)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self._launch()  # eager warm-up on the capture stream
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._launch()
+        torch.cuda.synchronize()
+
+    def _launch(self):
+        A, B, L = self.A, self.B, lib()
+        check(L.vs_batched_intersects(ptr(A.x), ptr(A.y), ptr(A.geom_offsets), ptr(A.part_offsets),
+                                      ptr(A.ring_offsets), len(A), ptr(B.x), ptr(B.y), ptr(B.geom_offsets),
+                                      ptr(B.part_offsets), ptr(B.ring_offsets), len(B), ptr(self.lj), ptr(self.rj),
+                                      self.n_jobs, self.code, self.cap, ptr(self.out_l), ptr(self.out_r),
+                                      ptr(self.res), ptr(self.ws), self.ws.numel(), stream()),
+              "vs_batched_intersects")
+
+    def replay(self):
+        """(left, right, stats) views of the graph's output buffers."""
+        self.graph.replay()
+        k, staged, cand, _ = _read_result(self.res)
+        if staged > self.cap:
+            raise OverflowError(f"{staged} bbox candidates exceed the captured capacity {self.cap}")
+        return self.out_l[:k], self.out_r[:k], JoinStats(k, staged, cand, self.cap, 0, 0)

@@ -21,6 +21,17 @@ Workloads:
       (seed 3): centre U[0,1e5]^2, R ~ U(50,300), vertices
       clip(round(LogNormal(ln 24, 1.0)), 4, 10000), holes ~ U{0..3} with
       radius U(0.1,0.4)*R (shrunk to fit), hole vertices U{4..16}.
+  c4: 50M points from 1024 clusters with Zipf(s=1.1) weights, N(0, 500 m)
+      about centres U[0,1e5]^2 (seed 4); 1M axis-aligned boxes (seed 5)
+      centred on fresh draws from the same mixture, width and height
+      independent LogNormal(ln 50, 1.0).
+  c5: 4096 polygon x polygon intersects jobs (seed 6): per job
+      L = clip(LogNormal(ln 2000, 1.0), 1e3, 2e5) left footprints (stars with
+      4-12 vertices, radius U(2.5, 20) m) and R = clip(LogNormal(ln 300, 1.0),
+      10, 5000) right parcels (stars, radius U(40, 70) m, vertices
+      clip(LogNormal(ln 40, 0.5), 4, 400)) on a job square of side
+      120 sqrt(R) m; jobs on a 64-wide grid of 2e4 m cells (or all at the
+      origin with overlap=True, so that only the job offsets separate them).
 Sizes can be scaled down with n_points / n_polys for tests.
 """
 
@@ -30,7 +41,8 @@ import numpy as np
 
 from .geometry import PointArray, PolygonArray
 
-__all__ = ["star_polygons", "c1", "c2", "c3", "gmm_points", "shared_edge_triangles", "Workload"]
+__all__ = ["star_polygons", "c1", "c2", "c3", "c4", "c5", "gmm_points", "zipf_points", "shared_edge_triangles",
+           "Workload"]
 
 
 class Workload:
@@ -67,6 +79,41 @@ def _sorted_angles(rng, counts: np.ndarray) -> np.ndarray:
     return ang
 
 
+def _sorted_angles_spacing(rng, counts: np.ndarray) -> np.ndarray:
+    """Same distribution as _sorted_angles without a sort: the order statistics
+    of n uniforms are the normalised partial sums of n + 1 exponential
+    spacings; the ring's largest angular gap is its largest spacing, the
+    wrap-around gap being E_1 + E_{n+1}.  Rings with a gap >= pi are redrawn."""
+    counts = np.asarray(counts, dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    ang = np.empty(int(counts.sum()))
+    todo = np.arange(len(counts))
+    while len(todo):
+        c = counts[todo]
+        e = rng.exponential(1.0, int(c.sum() + len(c)))
+        o1 = np.concatenate([[0], np.cumsum(c + 1)[:-1]])  # first spacing of each ring
+        tot = np.add.reduceat(e, o1)
+        cs = np.cumsum(e)
+        seg = np.repeat(np.arange(len(todo)), c + 1)
+        within = np.arange(len(e)) - o1[seg]
+        base = cs[o1] - e[o1]
+        u = (cs - base[seg]) / tot[seg]  # partial sums E_1..E_k over the ring
+        gap = e / tot[seg]
+        lastk = o1 + c  # the (n+1)-th spacing
+        wrap = (e[o1] + e[lastk]) / tot
+        inner = gap.copy()
+        inner[o1] = 0.0
+        inner[lastk] = 0.0
+        ok = np.maximum(np.maximum.reduceat(inner, o1), wrap) < 0.5
+        take = within < c[seg]  # the first n partial sums of each ring
+        dst = np.repeat(off[todo], c)
+        dst = dst + (np.arange(int(c.sum())) - np.repeat(np.concatenate([[0], np.cumsum(c)[:-1]]), c))
+        keep = np.repeat(ok, c)
+        ang[dst[keep]] = 2 * np.pi * u[take][keep]
+        todo = todo[~ok]
+    return ang
+
+
 def _inscribed_radius(cx, cy, x, y, counts):
     """Per ring: min distance from its centre to its (closed) edges."""
     off = np.concatenate([[0], np.cumsum(counts)[:-1]])
@@ -84,10 +131,10 @@ def _inscribed_radius(cx, cy, x, y, counts):
     return np.minimum.reduceat(d, off)
 
 
-def star_rings(rng, cx, cy, base_r, counts, jitter=0.3):
+def star_rings(rng, cx, cy, base_r, counts, jitter=0.3, spacing=False):
     """Vertices of star rings (not closed), concatenated; counts per ring."""
     counts = np.asarray(counts, dtype=np.int64)
-    ang = _sorted_angles(rng, counts)
+    ang = _sorted_angles_spacing(rng, counts) if spacing else _sorted_angles(rng, counts)
     seg = np.repeat(np.arange(len(counts)), counts)
     rad = base_r[seg] * (1.0 + rng.uniform(-jitter, jitter, len(ang)))
     return cx[seg] + rad * np.cos(ang), cy[seg] + rad * np.sin(ang)
@@ -118,11 +165,11 @@ def _assemble(rings_x, rings_y, ring_counts, rings_per_poly, reverse_mask=None) 
 
 
 def star_polygons(rng, centres_x, centres_y, R, n_vertices, holes=None, hole_vertices=None,
-                  hole_frac=None, jitter=0.3) -> PolygonArray:
+                  hole_frac=None, jitter=0.3, spacing=False) -> PolygonArray:
     """Star polygons with optional holes; returns a PolygonArray (shell CCW, holes CW)."""
     m = len(R)
     n_vertices = np.asarray(n_vertices, dtype=np.int64)
-    sx, sy = star_rings(rng, centres_x, centres_y, R, n_vertices, jitter)
+    sx, sy = star_rings(rng, centres_x, centres_y, R, n_vertices, jitter, spacing)
     if holes is None:
         return _assemble(sx, sy, n_vertices, np.ones(m, dtype=np.int64))
     holes = np.asarray(holes, dtype=np.int64)
@@ -190,6 +237,60 @@ def c2(n_points: int = 10_000_000, n_polys: int = 100_000, extent: float = 1e5) 
     hole_frac = rg.uniform(0.1, 0.4, n_holes)
     polys = star_polygons(rg, cx, cy, R, nv, holes=holes, hole_vertices=hole_vertices, hole_frac=hole_frac)
     return Workload("c2", PointArray(x, y), polys, "within")
+
+
+def _zipf_mixture(n_clusters=1024, s=1.1, extent=1e5, seed=4):
+    rng = np.random.default_rng(seed)
+    w = 1.0 / np.arange(1, n_clusters + 1) ** s
+    w /= w.sum()
+    return rng, w, rng.uniform(0, extent, n_clusters), rng.uniform(0, extent, n_clusters)
+
+
+def zipf_points(rng, n, w, cx, cy, sigma=500.0):
+    c = rng.choice(len(w), n, p=w)
+    return cx[c] + sigma * rng.standard_normal(n), cy[c] + sigma * rng.standard_normal(n)
+
+
+def c4(n_points: int = 50_000_000, n_boxes: int = 1_000_000):
+    """(points, boxes): the point-tree box query workload; boxes (n, 4) as
+    (xmin, ymin, xmax, ymax)."""
+    rng, w, cx, cy = _zipf_mixture()
+    x, y = zipf_points(rng, n_points, w, cx, cy)
+    rb = np.random.default_rng(5)
+    bx, by = zipf_points(rb, n_boxes, w, cx, cy)
+    bw = rb.lognormal(np.log(50.0), 1.0, n_boxes)
+    bh = rb.lognormal(np.log(50.0), 1.0, n_boxes)
+    boxes = np.column_stack([bx - bw / 2, by - bh / 2, bx + bw / 2, by + bh / 2])
+    return Workload("c4", PointArray(x, y), None, "intersects", boxes=boxes)
+
+
+def c5(n_jobs: int = 4096, left_median: float = 2000.0, right_median: float = 300.0, overlap: bool = False,
+       left_clip=(1_000, 200_000), right_clip=(10, 5_000)):
+    """(left, right, left_job, right_job): PolygonArrays of every job's rows
+    and int64 row offsets per job (n_jobs + 1)."""
+    rng = np.random.default_rng(6)
+    nl = np.clip(np.round(rng.lognormal(np.log(left_median), 1.0, n_jobs)), *left_clip).astype(np.int64)
+    nr = np.clip(np.round(rng.lognormal(np.log(right_median), 1.0, n_jobs)), *right_clip).astype(np.int64)
+    side = 120.0 * np.sqrt(nr)
+    ox = np.zeros(n_jobs) if overlap else (np.arange(n_jobs) % 64) * 2.0e4
+    oy = np.zeros(n_jobs) if overlap else (np.arange(n_jobs) // 64) * 2.0e4
+    jl = np.repeat(np.arange(n_jobs), nl)
+    jr = np.repeat(np.arange(n_jobs), nr)
+    # right parcels
+    rcx = ox[jr] + rng.uniform(0, 1, len(jr)) * side[jr]
+    rcy = oy[jr] + rng.uniform(0, 1, len(jr)) * side[jr]
+    rR = rng.uniform(40.0, 70.0, len(jr))
+    rnv = np.clip(np.round(rng.lognormal(np.log(40.0), 0.5, len(jr))), 4, 400).astype(np.int64)
+    right = star_polygons(rng, rcx, rcy, rR, rnv, spacing=True)
+    # left footprints
+    lcx = ox[jl] + rng.uniform(0, 1, len(jl)) * side[jl]
+    lcy = oy[jl] + rng.uniform(0, 1, len(jl)) * side[jl]
+    lR = rng.uniform(2.5, 20.0, len(jl))
+    lnv = rng.integers(4, 13, len(jl))
+    left = star_polygons(rng, lcx, lcy, lR, lnv, spacing=True)
+    lj = np.concatenate([[0], np.cumsum(nl)]).astype(np.int64)
+    rj = np.concatenate([[0], np.cumsum(nr)]).astype(np.int64)
+    return left, right, lj, rj
 
 
 def shared_edge_triangles(rng, polys: PolygonArray, owners: np.ndarray, height: float = 0.5):
