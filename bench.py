@@ -55,8 +55,12 @@ def main(argv=None):
     ap.add_argument("--n-polys", type=int, default=100_000)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="points in the CPU-baseline sample (0 disables it)")
-    ap.add_argument("--workload", choices=("c2", "c4"), default="c2",
-                    help="c2: the sjoin metric (default); c4: point-tree sindex.query, learned index vs tree")
+    ap.add_argument("--workload", choices=("c2", "c4", "c5"), default="c2",
+                    help="c2: the sjoin metric (default); c4: point-tree sindex.query, learned index vs tree; "
+                         "c5: 4096 batched polygon x polygon intersects jobs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device once (a kernel of known traffic for "
+                         "scaling hardware byte counters)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -77,7 +81,9 @@ def main(argv=None):
 
     torch.cuda.set_device(0)
     if args.workload == "c4":
-        return run_c4(args)
+        return _with_copy_probe(args, run_c4)
+    if args.workload == "c5":
+        return _with_copy_probe(args, run_c5)
     _log("generating workload")
     w = S.c2(n_points=args.n_points, n_polys=args.n_polys)
     Q = DevicePoints.from_host(w.points)
@@ -197,7 +203,28 @@ def main(argv=None):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    _copy_probe(args)
     return 0
+
+
+def _copy_probe(args):
+    if args.copy_probe <= 0:
+        return
+    import torch
+
+    n = args.copy_probe * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    _log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+
+
+def _with_copy_probe(args, fn):
+    rc = fn(args)
+    _copy_probe(args)
+    return rc
 
 
 def run_c4(args):
@@ -294,6 +321,90 @@ def run_c4(args):
         "roofline": {"bound": "hbm", "kernel": "whole step (build + query)", "achieved": alg / (ms_l * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms_l * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "alg_bytes_per_step": alg},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_c5(args):
+    """4,096 independent polygon x polygon sjoin(intersects) jobs
+    (vibespatial_amd.synthetic.c5) in one batched launch sequence: forest
+    build, persistent traversal, exact refine, finish.  A step = one batched
+    call over every job, inputs resident; the tile shape is autotuned first.
+    Algorithmic bytes per step: sum_q 16 V_q + sum_C 16 (V_a + V_b) + 8 K."""
+    import numpy as np
+    import torch
+
+    from vibespatial_amd import _lib
+    from vibespatial_amd import join as J
+    from vibespatial_amd.device import DevicePolygons
+    from vibespatial_amd import synthetic as S
+
+    _log("c5: generating 4096 jobs")
+    left, right, lj, rj = S.c5()
+    A, B = DevicePolygons.from_host(left), DevicePolygons.from_host(right)
+    tl, tr = torch.from_numpy(lj).cuda(), torch.from_numpy(rj).cuda()
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    _log("c5: autotune")
+    best, times = J.autotune_batched(A, B, tl, tr, reps=1)
+    _, _, st = J.batched_intersects(A, B, tl, tr)
+    cap = st.staged + 1024
+    for _ in range(max(args.warmup, 1)):
+        J.batched_intersects(A, B, tl, tr, capacity=cap)
+    _lib.check(L.vs_set_kernel_timing(1), "vs_set_kernel_timing")
+    kms = []
+    torch.cuda.synchronize()
+    _log(f"c5: timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, _, st = J.batched_intersects(A, B, tl, tr, capacity=cap)
+        ms = _lib.ctypes.c_float()
+        _lib.check(L.vs_last_kernel_ms(_lib.ctypes.byref(ms)), "vs_last_kernel_ms")
+        kms.append(ms.value)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    _lib.check(L.vs_set_kernel_timing(0), "vs_set_kernel_timing")
+    # graph replay of the same sequence
+    g = J.BatchedIntersectsGraph(A, B, tl, tr, capacity=cap, tile=best)
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, _, stg = g.replay()
+    torch.cuda.synchronize()
+    ms_graph = (time.perf_counter() - t0) * 1e3 / args.steps
+    assert stg.pairs == st.pairs
+    VA = np.diff(left.ring_offsets[left.part_offsets[left.geom_offsets]])
+    VB = np.diff(right.ring_offsets[right.part_offsets[right.geom_offsets]])
+    alg = 16 * int(VA.sum() + VB.sum()) + 8 * st.pairs  # + sum_C 16 (V_a + V_b): candidate list not kept
+    out = {
+        "metric": "batched sjoin(intersects) refined pairs/sec, 4096 polygon x polygon jobs",
+        "value": st.pairs / (ms_step * 1e-3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(lj) - 1} jobs, {len(left):,} left footprints x {len(right):,} right parcels "
+                        "(vibespatial_amd.synthetic.c5)",
+            "pairs": st.pairs,
+            "bbox_candidates": st.candidates,
+            "tile": list(best),
+            "autotune_ms": {f"{k[0]}x{k[1]}": v for k, v in times.items()},
+            "traverse_refine_kernel_ms": statistics.mean(kms),
+            "graph_replay_ms_per_step": ms_graph,
+        },
+        "roofline": {"bound": "hbm", "kernel": "whole step", "achieved": alg / (ms_step * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes_per_step_lower_bound": alg},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

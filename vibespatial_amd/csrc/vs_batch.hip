@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
         if (i < (int64_t)nc) {
             l = cand_l[i];
             r = cand_r[i];
-            keep = polygons_intersect(A, l, B, r);
+            keep = polygons_intersect_auto(A, l, B, r);
         }
         const uint64_t km = __ballot(keep);
         if (!km) continue;

@@ -24,26 +24,52 @@ int sort_pairs_u32(uint32_t*, int32_t*, int64_t, int, int, void*, int64_t, hipSt
 int64_t sort_workspace_bytes_u64(int64_t n);
 int64_t sort_workspace_bytes_u32(int64_t n);
 
-// One wave per polygonal row; lanes stride over the row's coordinates.
+// One row per lane: rows of at most 64 coordinates are reduced by their own
+// lane (rows are contiguous in memory, so the lanes of a wave read nearby
+// addresses); longer rows are then reduced by the whole wave, one at a time,
+// lanes striding over the coordinates.
+__device__ __forceinline__ void bounds_store(double4* out, int64_t row, bool empty, double x0, double y0, double x1,
+                                             double y1) {
+    if (empty || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
+    out[row] = make_double4(x0, y0, x1, y1);
+}
+
 __global__ __launch_bounds__(256) void polygon_bounds_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const int32_t* __restrict__ geom,
     const int32_t* __restrict__ part, const int32_t* __restrict__ ring, int64_t n_rows,
     double4* __restrict__ out) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n_rows) return;
-    const int32_t c0 = ring[part[geom[row]]], c1 = ring[part[geom[row + 1]]];
-    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
-    for (int32_t c = c0 + lane; c < c1; c += 64) {
-        const double xv = x[c], yv = y[c];
-        x0 = fmin(x0, xv); y0 = fmin(y0, yv);
-        x1 = fmax(x1, xv); y1 = fmax(y1, yv);
+    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool inr = row < n_rows;
+    int32_t c0 = 0, c1 = 0;
+    if (inr) {
+        c0 = ring[part[geom[row]]];
+        c1 = ring[part[geom[row + 1]]];
     }
-    x0 = wave_min(x0); y0 = wave_min(y0);
-    x1 = wave_max(x1); y1 = wave_max(y1);
-    if (lane == 0) {
-        if (c1 <= c0 || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
-        out[row] = make_double4(x0, y0, x1, y1);
+    const bool small = inr && c1 - c0 <= 64;
+    if (small) {
+        double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+        for (int32_t c = c0; c < c1; ++c) {
+            const double xv = x[c], yv = y[c];
+            x0 = fmin(x0, xv); y0 = fmin(y0, yv);
+            x1 = fmax(x1, xv); y1 = fmax(y1, yv);
+        }
+        bounds_store(out, row, c1 <= c0, x0, y0, x1, y1);
+    }
+    const int lane = threadIdx.x & 63;
+    uint64_t m = __ballot(inr && !small);
+    while (m) {
+        const int s = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        const int32_t r0 = readlane_i32(c0, s), r1 = readlane_i32(c1, s);
+        double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+        for (int32_t c = r0 + lane; c < r1; c += 64) {
+            const double xv = x[c], yv = y[c];
+            x0 = fmin(x0, xv); y0 = fmin(y0, yv);
+            x1 = fmax(x1, xv); y1 = fmax(y1, yv);
+        }
+        x0 = wave_min(x0); y0 = wave_min(y0);
+        x1 = wave_max(x1); y1 = wave_max(y1);
+        if (lane == s) bounds_store(out, row, false, x0, y0, x1, y1);
     }
 }
 
@@ -234,7 +260,7 @@ __global__ __launch_bounds__(256) void point_boxes_kernel(const double* __restri
 int polygon_bounds(const double* x, const double* y, const int32_t* geom, const int32_t* part,
                    const int32_t* ring, int64_t n, double* bounds, hipStream_t s) {
     if (n == 0) return VS_OK;
-    hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, s, x, y,
+    hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, x, y,
                        geom, part, ring, n, reinterpret_cast<double4*>(bounds));
     VS_LAUNCH_CHECK();
     return VS_OK;
@@ -290,18 +316,64 @@ int index_build(const double* bounds, int64_t n, const vs_tree_layout* L, double
     return VS_OK;
 }
 
+// Query-box ordering for coherent traversal: 16-level Hilbert keys of the box
+// centres over their extent, sorted (NaN boxes last); writes the permutation.
+int64_t box_order_workspace_bytes(int64_t n) {
+    Carver c(nullptr);
+    c.take<uint64_t>(n);
+    c.take<double4>(1);
+    c.take<char>(extent_workspace_bytes());
+    c.take<char>(sort_workspace_bytes_u64(n));
+    return c.bytes;
+}
+
+int box_order(const double4* b, int64_t n, int32_t* perm, void* ws, hipStream_t s) {
+    if (n == 0) return VS_OK;
+    Carver c(ws);
+    uint64_t* keys = c.take<uint64_t>(n);
+    double4* ext = c.take<double4>(1);
+    void* ext_ws = c.take<char>(extent_workspace_bytes());
+    void* sort_ws = c.take<char>(sort_workspace_bytes_u64(n));
+    VS_TRY(centre_extent(b, n, ext, ext_ws, s));
+    hipLaunchKernelGGL(hilbert_keys_boxes_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, b, n, ext, 16,
+                       keys, perm);
+    VS_LAUNCH_CHECK();
+    // NaN boxes carry all-ones keys: sort 33 bits so they stay after every code
+    VS_TRY(sort_pairs_u64(keys, perm, n, 0, 40, sort_ws, sort_workspace_bytes_u64(n), s));
+    return VS_OK;
+}
+
 // Point ordering for coherent traversal: 16-level (32-bit) Hilbert keys over
 // the point extent, sorted; writes the permutation and the gathered coords.
+// Four points per thread, every load issued before the first store, so each
+// thread keeps eight random reads in flight.
+constexpr int kGatherItems = 4;
 __global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ px,
                                                            const double* __restrict__ py,
                                                            const int32_t* __restrict__ perm,
                                                            int64_t n, double* __restrict__ sx,
                                                            double* __restrict__ sy) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int32_t j = perm[i];
-    sx[i] = px[j];
-    sy[i] = py[j];
+    const int64_t base = (int64_t)blockIdx.x * (256 * kGatherItems) + threadIdx.x;
+    int32_t j[kGatherItems];
+    double x[kGatherItems], y[kGatherItems];
+#pragma unroll
+    for (int b = 0; b < kGatherItems; ++b) {
+        const int64_t i = base + 256 * b;
+        j[b] = i < n ? perm[i] : -1;
+    }
+#pragma unroll
+    for (int b = 0; b < kGatherItems; ++b) {
+        x[b] = j[b] >= 0 ? px[j[b]] : 0.0;
+        y[b] = j[b] >= 0 ? py[j[b]] : 0.0;
+    }
+#pragma unroll
+    for (int b = 0; b < kGatherItems; ++b) {
+        const int64_t i = base + 256 * b;
+        if (i < n) {
+            sx[i] = x[b];
+            sy[i] = y[b];
+        }
+    }
 }
 
 int64_t point_order_workspace_bytes(int64_t n) {
@@ -341,8 +413,8 @@ int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, 
         VS_TRY(sort_pairs_u32(reinterpret_cast<uint32_t*>(keys), perm, n, 0, 32, sort_ws, sb, s));
     else
         VS_TRY(sort_pairs_u64(reinterpret_cast<uint64_t*>(keys), perm, n, 0, 64, sort_ws, sb, s));
-    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py, perm, n,
-                       sx, sy);
+    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256 * kGatherItems)), dim3(256), 0, s, px,
+                       py, perm, n, sx, sy);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }
@@ -371,8 +443,8 @@ int point_order(const double* px, const double* py, int64_t n, int bits, int32_t
                        0, s, px, py, n, ext, 16, keys, perm);
     VS_LAUNCH_CHECK();
     VS_TRY(sort_pairs_u32(keys, perm, n, 32 - 2 * bits, 32, sort_ws, sort_workspace_bytes_u32(n), s));
-    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py,
-                       perm, n, sx, sy);
+    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256 * kGatherItems)), dim3(256), 0, s, px,
+                       py, perm, n, sx, sy);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }

@@ -157,6 +157,41 @@ __device__ __forceinline__ long long wave_inclusive_scan_i64(long long v) {
     return v;
 }
 
+// Block-wide sums of N per-thread counters, then one atomicAdd per counter per
+// block (`red` is LDS of at least N * (blockDim.x / 64) entries).  Keeps
+// same-address atomics to one per block: per-wave atomics on one counter
+// serialise in L2 and cost more than the kernels they count.
+template <int N>
+__device__ __forceinline__ void block_add_counters(unsigned long long (&v)[N], unsigned long long* red,
+                                                   unsigned long long* dst) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        unsigned long long x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (l == 0) red[k * nw + w] = x;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < N) {
+        unsigned long long t = 0;
+        for (int ww = 0; ww < nw; ++ww) t += red[threadIdx.x * nw + ww];
+        if (t) atomicAdd(&dst[threadIdx.x], t);
+    }
+}
+
+// Grid for grid-stride kernels: enough blocks to fill the device, few
+// enough that per-block atomics stay cheap.
+inline unsigned stride_grid(int64_t n, int threads, int per_cu = 4) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
+    }
+    const int64_t want = (n + threads - 1) / threads;
+    const int64_t cap = (int64_t)cus * per_cu;
+    return (unsigned)(want < 1 ? 1 : (want < cap ? want : cap));
+}
+
 // Inclusive bbox overlap; NaN boxes never overlap anything.
 __device__ __forceinline__ bool box_overlap(double qx0, double qy0, double qx1, double qy1,
                                             double tx0, double ty0, double tx1, double ty1) {

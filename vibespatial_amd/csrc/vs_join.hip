@@ -41,6 +41,8 @@ int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, long long* d_
                        hipStream_t s);
 int64_t scan_workspace_bytes(int64_t n);
 int64_t point_order_workspace_bytes(int64_t n);
+int64_t box_order_workspace_bytes(int64_t n);
+int box_order(const double4* b, int64_t n, int32_t* perm, void* ws, hipStream_t s);
 int point_order(const double* px, const double* py, int64_t n, int bits, int32_t* perm, double* sx,
                 double* sy, void* ws, hipStream_t s);
 
@@ -389,7 +391,7 @@ JoinWs carve_join_ws(void* base, int64_t nq, int64_t n_items, int64_t n_coords, 
         J.order_ws = c.take<char>(point_order_workspace_bytes(nq));
     } else {
         J.sx = J.sy = nullptr;
-        J.order_ws = nullptr;
+        J.order_ws = c.take<char>(box_order_workspace_bytes(nq));
     }
     J.ctl = c.take<JoinCtl>(1);
     J.fin_ws = c.take<char>(finish_workspace_bytes(capacity));
@@ -578,10 +580,12 @@ int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_l
     VS_LAUNCH_CHECK();
     const vs::TreeView T = vs::make_tree_view(layout, node_boxes, item_ids);
     const vs::PolyView P = {nullptr, nullptr, nullptr, nullptr};
-    if (n_queries > 0 && layout->n_items > 0)
-        VS_TRY(vs::launch_traverse<vs::kModeBox>(VS_PRED_INTERSECTS, nullptr, nullptr,
-                                                 reinterpret_cast<const double4*>(query_boxes), nullptr,
-                                                 n_queries, T, P, J, capacity, s));
+    if (n_queries > 0 && layout->n_items > 0) {
+        const double4* qb = reinterpret_cast<const double4*>(query_boxes);
+        VS_TRY(vs::box_order(qb, n_queries, J.perm, J.order_ws, s));  // waves of nearby boxes
+        VS_TRY(vs::launch_traverse<vs::kModeBox>(VS_PRED_INTERSECTS, nullptr, nullptr, qb, J.perm, n_queries, T, P,
+                                                 J, capacity, s));
+    }
     return vs::finish_pairs(J.fin_ws, J.ctl, n_queries, capacity, sort != 0, out_left, out_right, d_result, s);
 }
 

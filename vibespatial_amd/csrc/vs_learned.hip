@@ -57,7 +57,7 @@ int64_t sort_workspace_bytes_u32(int64_t n);
 constexpr int kHidden = 32;
 constexpr int kNParams = 3 * kHidden + 1;  // w1[32], b1[32], w2[32], b2
 constexpr int kSample = 2048;              // training keys (evenly spaced ranks)
-constexpr int kMaxRanges = 9;
+constexpr int kMaxRanges = 16;             // cells of a query's cover (merged into at most this many ranges)
 constexpr int kChunk = 512;                // positions scanned per round
 constexpr int kItemRounds = 8;             // rounds per work item (items span kChunk * kItemRounds positions)
 constexpr int kItemPos = kChunk * kItemRounds;
@@ -290,13 +290,16 @@ struct LearnedView {
 
 __device__ __forceinline__ int64_t leaf_pos(int64_t s, int64_t n, int64_t S) { return s * (n - 1) / S; }
 
-template <class K>
-__device__ __forceinline__ int mlp_leaf(const float* P, K k, int S) {
-    const float g = mlp_eval(P, key_unit<K>(k));
+__device__ __forceinline__ int leaf_of_pred(float g, int S) {
     const float f = g * (float)S;
     if (!(f > 0.0f)) return 0;  // also NaN
     if (f >= (float)(S - 1)) return S - 1;
     return (int)f;
+}
+
+template <class K>
+__device__ __forceinline__ int mlp_leaf(const float* P, K k, int S) {
+    return leaf_of_pred(mlp_eval(P, key_unit<K>(k)), S);
 }
 
 // Linear interpolation of the rank of q inside leaf s, which spans sorted
@@ -324,33 +327,61 @@ __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ key
     __shared__ float P[kNParams];
     for (int k = threadIdx.x; k < kNParams; k += blockDim.x) P[k] = mlp[k];
     __syncthreads();
-    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ int red[4];
     const int lane = threadIdx.x & 63;
-    if (s >= S) return;
-    const int64_t p0 = leaf_pos(s, n, S), p1 = leaf_pos(s + 1, n, S);
-    const K k0 = leaf_key[s], k1 = leaf_key[s + 1];
-    double emax = 0.0;
-    int mlp_e = 0;
-    for (int64_t i = p0 + lane; i <= p1; i += 64) {
-        const K k = keys[i];
-        const double e = fabs(plm_pred<K>(k, k0, k1, p0, p1) - (double)i);
-        emax = fmax(emax, e);
-        if (i > p0 && k != keys[i - 1]) {
-            // a first occurrence at position i in (p0, p1]: the lookup of its key
-            // must land in leaf s (the last leaf whose first key is below it)
-            const int sp = mlp_leaf<K>(P, k, S);
-            const int d = sp > (int)s ? sp - (int)s : (int)s - sp;
-            mlp_e = d > mlp_e ? d : mlp_e;
+    int mlp_e = 0;  // over every leaf this thread visits
+    for (int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); s < S; s += (int64_t)gridDim.x * 4) {
+        const int64_t p0 = leaf_pos(s, n, S), p1 = leaf_pos(s + 1, n, S);
+        const K k0 = leaf_key[s], k1 = leaf_key[s + 1];
+        double emax = 0.0;
+        // kB keys per lane at a time: the network's weights are read once per
+        // hidden unit for all of them (same operation order as mlp_eval, so the
+        // predictions are bit-identical to the lookups')
+        constexpr int kB = 8;
+        for (int64_t base = p0; base <= p1; base += 64 * kB) {
+            float u[kB], y[kB];
+            bool first[kB];
+#pragma unroll
+            for (int b = 0; b < kB; ++b) {
+                const int64_t i = base + lane + 64 * b;
+                const bool valid = i <= p1;
+                const K k = valid ? keys[i] : (K)0;
+                first[b] = valid && i > p0 && k != keys[i - 1];
+                if (valid) emax = fmax(emax, fabs(plm_pred<K>(k, k0, k1, p0, p1) - (double)i));
+                u[b] = key_unit<K>(k);
+                y[b] = P[3 * kHidden];
+            }
+#pragma unroll 4
+            for (int h = 0; h < kHidden; ++h) {
+                const float w1 = P[h], b1 = P[kHidden + h], w2 = P[2 * kHidden + h];
+#pragma unroll
+                for (int b = 0; b < kB; ++b) {
+                    const float z = w1 * u[b] + b1;
+                    y[b] += w2 * fmaxf(z, 0.0f);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < kB; ++b) {
+                if (!first[b]) continue;
+                // a first occurrence in (p0, p1]: the lookup of its key must land
+                // in leaf s (the last leaf whose first key is below it)
+                const int sp = leaf_of_pred(y[b], S);
+                const int d = sp > (int)s ? sp - (int)s : (int)s - sp;
+                mlp_e = d > mlp_e ? d : mlp_e;
+            }
         }
+        emax = wave_max(emax);
+        if (lane == 0) leaf_err[s] = (int32_t)ceil(emax);
     }
-    emax = wave_max(emax);
     for (int o = 32; o > 0; o >>= 1) {
         const int v = __shfl_xor(mlp_e, o, 64);
         mlp_e = v > mlp_e ? v : mlp_e;
     }
-    if (lane == 0) {
-        leaf_err[s] = (int32_t)ceil(emax);
-        atomicMax(&meta[0], mlp_e);
+    if (lane == 0) red[threadIdx.x >> 6] = mlp_e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int m = max(max(red[0], red[1]), max(red[2], red[3]));
+        if (m > 0) atomicMax(&meta[0], m);
     }
 }
 
@@ -416,97 +447,148 @@ __device__ int64_t learned_lower_bound(K q, const LearnedView<K>& V, LookupStats
     return a;
 }
 
-// <= 9 Hilbert cells of one level covering the box, merged into inclusive key
-// ranges sorted by start.  Returns the number of ranges.
+// Cover of a box on the key grid: the finest level whose cells covering the
+// box number at most kMaxRanges (long thin boxes get thin covers; by level
+// bits-1 at most 2 x 2 remain), each cell a key range [code 4^l, (code+1) 4^l);
+// the codes are sorted (a bitonic network on registers) and consecutive ones
+// merged.  Writes (lo, hi) pairs to out and returns their count.
 __device__ int cover_ranges(double x0, double y0, double x1, double y1, const KeyScale& ks, int bits,
-                            uint64_t* lo, uint64_t* hi) {
+                            unsigned long long* __restrict__ out) {
     if (!(x0 <= x1 && y0 <= y1)) return 0;
     const uint32_t qx0 = quantize(x0, ks.x0, ks.sx, bits), qx1 = quantize(x1, ks.x0, ks.sx, bits);
     const uint32_t qy0 = quantize(y0, ks.y0, ks.sy, bits), qy1 = quantize(y1, ks.y0, ks.sy, bits);
     int l = 0;
-    while (l < bits && (((qx1 >> l) - (qx0 >> l)) > 2u || ((qy1 >> l) - (qy0 >> l)) > 2u)) ++l;
+    while (l < bits - 1 && (uint64_t)((qx1 >> l) - (qx0 >> l) + 1) * ((qy1 >> l) - (qy0 >> l) + 1) > kMaxRanges) ++l;
     const int lev = bits - l;
-    int cnt = 0;
-    for (uint32_t a = qx0 >> l; a <= (qx1 >> l); ++a)
-        for (uint32_t c = qy0 >> l; c <= (qy1 >> l); ++c) {
-            const uint64_t code = lev > 0 ? (hilbert64(a << (32 - lev), c << (32 - lev)) >> (64 - 2 * lev)) : 0ull;
-            const uint64_t kl = 2 * l >= 64 ? 0ull : (code << (2 * l));
-            const uint64_t kh = kl | (2 * l >= 64 ? ~0ull : ((1ull << (2 * l)) - 1ull));
-            int k = cnt++;
-            while (k > 0 && lo[k - 1] > kl) {
-                lo[k] = lo[k - 1];
-                hi[k] = hi[k - 1];
-                --k;
+    const uint32_t ax = qx0 >> l, ay = qy0 >> l;
+    const uint32_t nx = (qx1 >> l) - ax + 1, ncell = nx * ((qy1 >> l) - ay + 1);
+    uint64_t code[kMaxRanges];
+#pragma unroll
+    for (int t = 0; t < kMaxRanges; ++t) {
+        const uint32_t a = ax + (uint32_t)t % nx, c = ay + (uint32_t)t / nx;
+        code[t] = (uint32_t)t < ncell ? hilbert64(a << (32 - lev), c << (32 - lev)) >> (64 - 2 * lev) : ~0ull;
+    }
+#pragma unroll
+    for (int k = 2; k <= kMaxRanges; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < kMaxRanges; ++i) {
+                const int o = i ^ j;
+                if (o > i) {
+                    const bool asc = (i & k) == 0;
+                    const uint64_t a = code[i], b = code[o];
+                    if ((a > b) == asc) {
+                        code[i] = b;
+                        code[o] = a;
+                    }
+                }
             }
-            lo[k] = kl;
-            hi[k] = kh;
-        }
+    const uint64_t span = (1ull << (2 * l)) - 1ull;
     int m = 0;
-    for (int k = 1; k < cnt; ++k) {
-        if (hi[m] != ~0ull && lo[k] == hi[m] + 1) hi[m] = hi[k];
-        else {
+    uint64_t rlo = code[0], rhi = code[0];
+#pragma unroll
+    for (int t = 1; t < kMaxRanges; ++t) {
+        if ((uint32_t)t >= ncell) continue;
+        if (code[t] == rhi + 1) {
+            rhi = code[t];
+        } else {
+            out[2 * m] = rlo << (2 * l);
+            out[2 * m + 1] = (rhi << (2 * l)) | span;
             ++m;
-            lo[m] = lo[k];
-            hi[m] = hi[k];
+            rlo = rhi = code[t];
         }
     }
-    return cnt ? m + 1 : 0;
+    out[2 * m] = rlo << (2 * l);
+    out[2 * m + 1] = (rhi << (2 * l)) | span;
+    return m + 1;
 }
 
-// Per query box: rank ranges of its cover, total scan length and the number
-// of work items (chunks of kChunk positions).
+// Cover of each query box (and, for polygon queries, its coordinate range
+// and part count): up to kMaxRanges merged key ranges [lo, hi].
 template <class K>
-__global__ __launch_bounds__(256) void cover_kernel(const double4* __restrict__ qbox, int64_t nq,
-                                                   const double4* __restrict__ ext, LearnedView<K> V,
-                                                   int2* __restrict__ ranges, int32_t* __restrict__ nranges,
-                                                   int32_t* __restrict__ n_items, unsigned long long* __restrict__ stat,
-                                                   const int32_t* __restrict__ geom, const int32_t* __restrict__ part,
-                                                   const int32_t* __restrict__ ring, int4* __restrict__ poly_info) {
+__global__ __launch_bounds__(256) void cover_cells_kernel(const double4* __restrict__ qbox, int64_t nq,
+                                                         const double4* __restrict__ ext,
+                                                         unsigned long long* __restrict__ keyr,
+                                                         int32_t* __restrict__ nranges,
+                                                         const int32_t* __restrict__ geom,
+                                                         const int32_t* __restrict__ part,
+                                                         const int32_t* __restrict__ ring, int4* __restrict__ poly_info) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q < nq && poly_info)
-        poly_info[q] = make_int4(ring[part[geom[q]]], ring[part[geom[q + 1]]], geom[q + 1] - geom[q], 0);
+    if (q >= nq) return;
+    if (poly_info) poly_info[q] = make_int4(ring[part[geom[q]]], ring[part[geom[q + 1]]], geom[q + 1] - geom[q], 0);
+    const int bits = 4 * (int)sizeof(K);
+    const KeyScale ks = key_scale(ext, bits);
+    const double4 b = qbox[q];
+    nranges[q] = cover_ranges(b.x, b.y, b.z, b.w, ks, bits, keyr + q * kMaxRanges * 2);
+}
+
+// One thread per range end: rank of the range's first key (x) and of the key
+// after its last (y) through the learned lower bound.
+template <class K>
+__global__ __launch_bounds__(256) void cover_lookup_kernel(int64_t nq, const unsigned long long* __restrict__ keyr,
+                                                          const int32_t* __restrict__ nranges, LearnedView<K> V,
+                                                          int32_t* __restrict__ ranges,
+                                                          unsigned long long* __restrict__ stat) {
+    __shared__ float P[kNParams];
+    __shared__ unsigned long long red[3 * 4];
+    for (int k = threadIdx.x; k < kNParams; k += blockDim.x) P[k] = V.mlp[k];
+    __syncthreads();
+    LearnedView<K> Vl = V;
+    Vl.mlp = P;
     LookupStats st = {0, 0, 0};
-    unsigned long long scan = 0;
-    if (q < nq) {
-        const int bits = 4 * (int)sizeof(K);
-        const KeyScale ks = key_scale(ext, bits);
-        const double4 b = qbox[q];
-        uint64_t lo[kMaxRanges], hi[kMaxRanges];
-        const int nr = cover_ranges(b.x, b.y, b.z, b.w, ks, bits, lo, hi);
-        int64_t total = 0;
-        for (int k = 0; k < nr; ++k) {
-            const int64_t r0 = learned_lower_bound<K>((K)lo[k], V, st);
-            const bool top = hi[k] >= (sizeof(K) == 4 ? 0xFFFFFFFFull : ~0ull);
-            const int64_t r1 = top ? V.n : learned_lower_bound<K>((K)(hi[k] + 1), V, st);
-            ranges[q * kMaxRanges + k] = make_int2((int32_t)r0, (int32_t)r1);
-            total += r1 - r0;
+    const int64_t n_ends = nq * kMaxRanges * 2;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n_ends; t += (int64_t)gridDim.x * 256) {
+        const int64_t q = t / (kMaxRanges * 2);
+        const int k = (int)((t / 2) % kMaxRanges), end = (int)(t & 1);
+        if (k < nranges[q]) {
+            const uint64_t key = keyr[t];
+            int64_t r;
+            if (end == 0) r = learned_lower_bound<K>((K)key, Vl, st);
+            else if (key >= (sizeof(K) == 4 ? 0xFFFFFFFFull : ~0ull)) r = V.n;
+            else r = learned_lower_bound<K>((K)(key + 1), Vl, st);
+            ranges[t] = (int32_t)r;  // int2 {x, y} per range
         }
-        nranges[q] = nr;
+    }
+    unsigned long long v[3] = {st.probes, st.leaf_fb, st.win_fb};
+    block_add_counters<3>(v, red, stat);
+}
+
+// Per query: total scan length and its number of work items (kItemPos
+// positions each); stat[3] += positions scanned.
+__global__ __launch_bounds__(256) void cover_total_kernel(int64_t nq, const int2* __restrict__ ranges,
+                                                         const int32_t* __restrict__ nranges,
+                                                         int32_t* __restrict__ n_items,
+                                                         unsigned long long* __restrict__ stat) {
+    __shared__ unsigned long long red[4];
+    unsigned long long scan[1] = {0};
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < nq; q += (int64_t)gridDim.x * 256) {
+        int64_t total = 0;
+        for (int k = 0; k < nranges[q]; ++k) {
+            const int2 r = ranges[q * kMaxRanges + k];
+            total += r.y - r.x;
+        }
         n_items[q] = (int32_t)((total + kItemPos - 1) / kItemPos);
-        scan = (unsigned long long)total;
+        scan[0] += (unsigned long long)total;
     }
-    // block-reduce the counters: [0] lookups probes, [1] leaf fallbacks, [2] window fallbacks, [3] scanned
-    unsigned long long v[4] = {st.probes, st.leaf_fb, st.win_fb, scan};
-    for (int k = 0; k < 4; ++k) {
-        unsigned long long x = v[k];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&stat[k], x);
-    }
+    block_add_counters<1>(scan, red, stat + 3);
 }
 
 template <class K>
 __global__ __launch_bounds__(256) void lookup_batch_kernel(const K* __restrict__ qk, int64_t m, LearnedView<K> V,
                                                           int64_t* __restrict__ out,
                                                           unsigned long long* __restrict__ stat) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    __shared__ float P[kNParams];
+    __shared__ unsigned long long red[3 * 4];
+    for (int k = threadIdx.x; k < kNParams; k += blockDim.x) P[k] = V.mlp[k];
+    __syncthreads();
+    LearnedView<K> Vl = V;
+    Vl.mlp = P;
     LookupStats st = {0, 0, 0};
-    if (i < m) out[i] = learned_lower_bound<K>(qk[i], V, st);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256)
+        out[i] = learned_lower_bound<K>(qk[i], Vl, st);
     unsigned long long v[3] = {st.probes, st.leaf_fb, st.win_fb};
-    for (int k = 0; k < 3; ++k) {
-        unsigned long long x = v[k];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&stat[k], x);
-    }
+    block_add_counters<3>(v, red, stat);
 }
 
 // ---------------------------------------------------------------------------
@@ -577,6 +659,8 @@ __device__ __forceinline__ bool loc_pred(int loc) {
 }
 
 constexpr int kEdgeCap = 512;  // edges staged in LDS per batch
+constexpr int kPosPerThread = 4;                       // direct path: positions per thread per round
+constexpr int kRoundPos = kLJThreads * kPosPerThread;  // direct path: positions per round
 constexpr int kOutCap = 2048;  // per-block output buffer (pairs)
 
 // ---------------------------------------------------------------------------
@@ -686,6 +770,64 @@ __device__ __forceinline__ void refine_slabs(const float4* __restrict__ Et, cons
     }
 }
 
+// One candidate against the edges of its slab; returns its location code.
+__device__ __forceinline__ int refine_one(const float4* __restrict__ Et, const int32_t* __restrict__ Ec,
+                                         const int* __restrict__ start, const uint16_t* __restrict__ list,
+                                         const Slabs& S, const RefineView& R, float pu, float pv, double pxd,
+                                         double pyd, float E, bool& esc, unsigned& n_orient,
+                                         unsigned long long& n_esc) {
+    int par = 0;
+    const int b = slab_of(pv, S);
+    const int k1 = start[b + 1];
+    for (int k = start[b]; k < k1; ++k) {
+        const int e = list[k];
+        const float4 ed = Et[e];
+        if (edge_step_f32(pu, pv, ed.x, ed.y, ed.z, ed.w, E, par, n_orient)) {
+            ++n_esc;
+            esc = true;
+            const int32_t c = Ec[e];
+            if (edge_step_f64(pxd, pyd, R.x[c - 1], R.y[c - 1], R.x[c], R.y[c], par)) return 1;
+        }
+    }
+    return par ? 2 : 0;
+}
+
+// Stages every edge (c-1, c), c in (lo, hi), of a single-part polygon with at
+// most kEdgeCap edges into Et/Ec (recentred fp32 and coordinate index).
+// Block-uniform; returns the number staged.
+__device__ int stage_all_edges(int32_t lo, int32_t hi, double cx, double cy, const RefineView& R, float4* Et,
+                               int32_t* Ec, int* s_wsum) {
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    int ne = 0;
+    for (int32_t base = lo; base < hi; base += kLJThreads) {
+        const int32_t c = base + tid;
+        const bool rel = c > lo && c < hi && !(R.flags[c] & 1);
+        double ax = 0, ay = 0, bx = 0, by = 0;
+        if (rel) {
+            ax = R.x[c - 1];
+            ay = R.y[c - 1];
+            bx = R.x[c];
+            by = R.y[c];
+        }
+        const int inc = wave_inclusive_scan(rel ? 1 : 0);
+        if (lane == 63) s_wsum[wid] = inc;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w2 = 0; w2 < kLJThreads / 64; ++w2) {
+            woff += w2 < wid ? s_wsum[w2] : 0;
+            tot += s_wsum[w2];
+        }
+        if (rel) {
+            const int slot = ne + woff + inc - 1;
+            Et[slot] = make_float4((float)(ax - cx), (float)(ay - cy), (float)(bx - cx), (float)(by - cy));
+            Ec[slot] = c;
+        }
+        ne += tot;
+        __syncthreads();
+    }
+    return ne;
+}
+
 // LDS buffers of the staged edges and their slab lists.
 struct EdgeLds {
     float4* Et;
@@ -753,7 +895,9 @@ __device__ int stage_edges(int32_t lo, int32_t hi, double xmn, double ymn, doubl
 // keeping only edges that can affect the round's candidates; multi-part
 // polygons stream every edge in order to settle each part's parity.
 template <int MODE, int PRED>
-__global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
+// Four waves per SIMD (the register cap costs a few spills; the latency of one
+// block's item start-up is hidden by the others on the CU).
+__global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
     int64_t nq, const int32_t* __restrict__ item_off, const long long* __restrict__ total_items,
     const int32_t* __restrict__ item_q, int64_t item_cap, const int32_t* __restrict__ order,
     const int4* __restrict__ poly_info, const int2* __restrict__ ranges, const int32_t* __restrict__ nranges,
@@ -763,6 +907,7 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
     __shared__ int s_item, s_q, s_chunk, s_ncand, s_nout, s_nr;
     __shared__ unsigned long long s_base;
     __shared__ int2 s_rng[kMaxRanges];
+    __shared__ int s_pre[kMaxRanges + 1];  // exclusive prefix of the range lengths
     __shared__ int32_t cand[kChunk];
     __shared__ int2 outb[kOutCap];
     __shared__ float4 Et[kEdgeCap];
@@ -822,9 +967,18 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
         const int q = s_q;
         if (tid < s_nr) s_rng[tid] = ranges[(int64_t)q * kMaxRanges + tid];
         __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int r = 0; r < s_nr; ++r) {
+                s_pre[r] = acc;
+                acc += s_rng[r].y - s_rng[r].x;
+            }
+            s_pre[s_nr] = acc;
+        }
+        __syncthreads();
         const double4 box = qbox[q];
-        int total = 0;
-        for (int r = 0; r < s_nr; ++r) total += s_rng[r].y - s_rng[r].x;
+        const int nr = s_nr;
+        const int total = s_pre[nr];
         const int first = s_chunk * kItemPos;
         const int last = (first + kItemPos < total) ? first + kItemPos : total;
         int4 info = make_int4(0, 0, 0, 0);
@@ -840,6 +994,71 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
             M *= 1.0 + 4.440892098500626e-16;
             E = fp32_det_bound(M);
         }
+        const bool direct = MODE == kLModeBox || (info.z == 1 && info.y - info.x <= kEdgeCap + 1);
+        if (direct) {
+            // Box queries and small single-part polygons: every thread scans
+            // kPosPerThread positions per round and refines its own candidates
+            // at once against the item's slab lists (no candidate compaction).
+            Slabs S = {1, 0.0f, 0.0f};
+            if (MODE == kLModePip) {
+                const int ne = stage_all_edges(info.x, info.y, cx, cy, R, Et, Ec, s_wsum);
+                S = build_slabs(Et, ne, (float)(box.y - cy), (float)(box.w - cy), s_bstart, s_bcur, s_blist);
+            }
+            unsigned n_orient = 0;
+            unsigned long long n_esc = 0;
+            for (int r0 = first; r0 < last; r0 += kRoundPos) {
+                __syncthreads();  // s_nout is stable
+                if (s_nout > kOutCap - kRoundPos) flush();
+                int32_t pos[kPosPerThread];
+                double px[kPosPerThread], py[kPosPerThread];
+#pragma unroll
+                for (int j = 0; j < kPosPerThread; ++j) {
+                    const int off = r0 + tid + j * kLJThreads;
+                    pos[j] = -1;
+                    if (off < last) {
+                        int r = 0;  // last range with s_pre[r] <= off
+                        for (int step = kMaxRanges / 2; step > 0; step >>= 1)
+                            if (r + step < nr && s_pre[r + step] <= off) r += step;
+                        pos[j] = s_rng[r].x + (off - s_pre[r]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kPosPerThread; ++j) {
+                    px[j] = pos[j] >= 0 ? sx[pos[j]] : NAN;
+                    py[j] = pos[j] >= 0 ? sy[pos[j]] : NAN;
+                }
+#pragma unroll
+                for (int j = 0; j < kPosPerThread; ++j) {
+                    bool keep = px[j] >= box.x && px[j] <= box.z && py[j] >= box.y && py[j] <= box.w;
+                    if (MODE == kLModePip && keep) {
+                        ++st_cand;
+                        st_cverts += (unsigned long long)(info.y - info.x);
+                        bool esc = false;
+                        const int loc = refine_one(Et, Ec, s_bstart, s_blist, S, R, (float)(px[j] - cx),
+                                                   (float)(py[j] - cy), px[j], py[j], E, esc, n_orient, n_esc);
+                        st_escpairs += esc ? 1 : 0;
+                        keep = loc_pred<PRED>(loc);
+                    }
+                    // wave-aggregated append to the block buffer
+                    const uint64_t m = __ballot(keep);
+                    if (m) {
+                        const int leader = __ffsll((unsigned long long)m) - 1;
+                        int base = 0;
+                        if (lane == leader) base = atomicAdd(&s_nout, __popcll(m));
+                        base = __shfl(base, leader, 64);
+                        if (keep) {
+                            const int32_t orig = perm[pos[j]];
+                            outb[base + __popcll(m & lanemask_lt())] =
+                                MODE == kLModeBox ? make_int2(q, orig) : make_int2(orig, q);
+                        }
+                    }
+                }
+            }
+            st_orient += n_orient;
+            st_esc += n_esc;
+            __syncthreads();
+            continue;
+        }
         if (tid == 0) s_ncand = 0;
         __syncthreads();
         for (int r0 = first; r0 < last; r0 += kLJThreads) {
@@ -848,13 +1067,11 @@ __global__ __launch_bounds__(kLJThreads) void learned_join_kernel(
             {
                 const int off = r0 + tid;
                 if (off < last) {
-                    int o2 = off, r = 0;
-                    while (r < s_nr && o2 >= s_rng[r].y - s_rng[r].x) {
-                        o2 -= s_rng[r].y - s_rng[r].x;
-                        ++r;
-                    }
-                    if (r < s_nr) {
-                        const int32_t pos = s_rng[r].x + o2;
+                    int r = 0;  // last range with s_pre[r] <= off (off < total, so r < nr)
+                    for (int step = kMaxRanges / 2; step > 0; step >>= 1)
+                        if (r + step < nr && s_pre[r + step] <= off) r += step;
+                    {
+                        const int32_t pos = s_rng[r].x + (off - s_pre[r]);
                         const double px = sx[pos], py = sy[pos];
                         if (px >= box.x && px <= box.z && py >= box.y && py <= box.w) {
                             const int slot = atomicAdd(&s_ncand, 1);
@@ -1133,7 +1350,7 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
     hipLaunchKernelGGL(leaf_keys_kernel<K>, dim3((unsigned)ceil_div(S + 1, 256)), dim3(256), 0, s, B.keys, n, S,
                        B.leaf_key);
     VS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(leaf_err_kernel<K>, dim3((unsigned)ceil_div(S, 4)), dim3(256), 0, s, B.keys, n, S, B.leaf_key,
+    hipLaunchKernelGGL(leaf_err_kernel<K>, dim3(stride_grid(S, 4)), dim3(256), 0, s, B.keys, n, S, B.leaf_key,
                        B.mlp, B.leaf_err, B.meta);
     VS_LAUNCH_CHECK();
     return VS_OK;
@@ -1162,6 +1379,7 @@ struct QueryBufs {
     int32_t* n_items_sorted;
     void* sort_ws;
     int2* ranges;
+    unsigned long long* keyr;
     int32_t* nranges;
     int32_t* n_items;
     int32_t* item_off;
@@ -1188,6 +1406,7 @@ QueryBufs carve_query(Carver& c, int64_t nq, int64_t n_left, int64_t capacity) {
     Q.n_items_sorted = c.take<int32_t>(nq);
     Q.sort_ws = c.take<char>(sort_workspace_bytes_u32(nq));
     Q.ranges = c.take<int2>(nq * kMaxRanges);
+    Q.keyr = c.take<unsigned long long>(nq * kMaxRanges * 2);
     Q.nranges = c.take<int32_t>(nq);
     Q.n_items = c.take<int32_t>(nq);
     Q.item_off = c.take<int32_t>(nq);
@@ -1216,8 +1435,14 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
     if (nq > 0) {
         const unsigned gq = (unsigned)ceil_div(nq, 256);
         int4* pinfo = mode == kLModePip ? Q.poly_info : nullptr;
-        hipLaunchKernelGGL(cover_kernel<K>, dim3(gq), dim3(256), 0, s, qbox, nq, B.ext, V, Q.ranges, Q.nranges,
-                           Q.n_items, Q.cstat, R.geom, R.part, R.ring, pinfo);
+        hipLaunchKernelGGL(cover_cells_kernel<K>, dim3(gq), dim3(256), 0, s, qbox, nq, B.ext, Q.keyr, Q.nranges,
+                           R.geom, R.part, R.ring, pinfo);
+        VS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(cover_lookup_kernel<K>, dim3(stride_grid(nq * kMaxRanges * 2, 256)), dim3(256), 0, s,
+                           nq, Q.keyr, Q.nranges, V, reinterpret_cast<int32_t*>(Q.ranges), Q.cstat);
+        VS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(cover_total_kernel, dim3(stride_grid(nq, 256)), dim3(256), 0, s, nq, Q.ranges, Q.nranges,
+                           Q.n_items, Q.cstat);
         VS_LAUNCH_CHECK();
         // LPT order of the queries, then item offsets in that order
         hipLaunchKernelGGL(cost_key_kernel, dim3(gq), dim3(256), 0, s, Q.n_items, pinfo, nq, Q.cost_key, Q.order);
@@ -1327,13 +1552,13 @@ int vs_learned_lower_bound(const void* index, int64_t n_points, int32_t key_byte
     if (key_bytes == 4) {
         const auto B = vs::carve_index<uint32_t>(c, n_points, n_leaves);
         const auto V = vs::view_of<uint32_t>(B, n_points, n_leaves, lookup_mode);
-        hipLaunchKernelGGL(vs::lookup_batch_kernel<uint32_t>, dim3((unsigned)vs::ceil_div(m, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL(vs::lookup_batch_kernel<uint32_t>, dim3(vs::stride_grid(m, 256)), dim3(256), 0, s,
                            reinterpret_cast<const uint32_t*>(query_keys), m, V, out,
                            reinterpret_cast<unsigned long long*>(d_stat3));
     } else {
         const auto B = vs::carve_index<uint64_t>(c, n_points, n_leaves);
         const auto V = vs::view_of<uint64_t>(B, n_points, n_leaves, lookup_mode);
-        hipLaunchKernelGGL(vs::lookup_batch_kernel<uint64_t>, dim3((unsigned)vs::ceil_div(m, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL(vs::lookup_batch_kernel<uint64_t>, dim3(vs::stride_grid(m, 256)), dim3(256), 0, s,
                            reinterpret_cast<const uint64_t*>(query_keys), m, V, out,
                            reinterpret_cast<unsigned long long*>(d_stat3));
     }

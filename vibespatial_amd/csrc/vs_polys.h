@@ -53,4 +53,74 @@ __device__ inline bool polygons_intersect(const Polys& A, int64_t ra, const Poly
     return false;
 }
 
+// Same predicate when one side A is a single ring (one part, one ring), with
+// the loops turned around: the other polygon's edges are streamed once, each
+// first checked against A's bounds (exact min/max, so the check only skips
+// edges that cannot touch A) before A's edges are tested against it, and the
+// crossing step of A's first vertex rides along; then the first vertex of
+// each of B's rings is located in A unless it lies outside A's bounds.
+// Segment touch is symmetric and the location checks are those of
+// polygons_intersect, so the answer is the same boolean.
+__device__ inline bool single_ring_of(const Polys& P, int64_t r, int32_t& c0, int32_t& n) {
+    const int32_t p0 = P.geom[r];
+    if (P.geom[r + 1] - p0 != 1) return false;
+    const int32_t r0 = P.part[p0];
+    if (P.part[p0 + 1] - r0 != 1) return false;
+    c0 = P.ring[r0];
+    n = P.ring[r0 + 1] - c0;
+    return n >= 1;
+}
+
+__device__ inline bool polygons_intersect_ring(const Polys& A, int32_t a0, int32_t na, const Polys& B, int64_t rb) {
+    const int32_t bp0 = B.geom[rb], bp1 = B.geom[rb + 1];
+    if (B.ring[B.part[bp1]] <= B.ring[B.part[bp0]]) return false;
+    double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+    for (int32_t k = a0; k < a0 + na; ++k) {
+        mnx = fmin(mnx, A.x[k]);
+        mny = fmin(mny, A.y[k]);
+        mxx = fmax(mxx, A.x[k]);
+        mxy = fmax(mxy, A.y[k]);
+    }
+    const double fx = A.x[a0], fy = A.y[a0];
+    for (int32_t bp = bp0; bp < bp1; ++bp) {
+        int parity = 0;  // A's first vertex against this part of B
+        for (int32_t br = B.part[bp]; br < B.part[bp + 1]; ++br) {
+            const int32_t e0 = B.ring[br], e1 = B.ring[br + 1];
+            if (e1 <= e0) continue;
+            double px = B.x[e0], py = B.y[e0];
+            for (int32_t c = e0 + 1; c < e1; ++c) {
+                const double qx = B.x[c], qy = B.y[c];
+                if (fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy) {
+                    for (int32_t k = a0 + 1; k < a0 + na; ++k)
+                        if (segments_touch(A.x[k - 1], A.y[k - 1], A.x[k], A.y[k], px, py, qx, qy)) return true;
+                }
+                if (edge_step_f64(fx, fy, px, py, qx, qy, parity)) return true;  // on B's boundary
+                px = qx;
+                py = qy;
+            }
+        }
+        if (parity) return true;  // inside this part of B
+    }
+    for (int32_t br = B.part[bp0]; br < B.part[bp1]; ++br) {
+        const int32_t e0 = B.ring[br];
+        if (B.ring[br + 1] <= e0) continue;
+        const double px = B.x[e0], py = B.y[e0];
+        if (px < mnx || px > mxx || py < mny || py > mxy) continue;  // outside A's bounds: location 0
+        int parity = 0;
+        for (int32_t k = a0 + 1; k < a0 + na; ++k)
+            if (edge_step_f64(px, py, A.x[k - 1], A.y[k - 1], A.x[k], A.y[k], parity)) return true;
+        if (parity) return true;
+    }
+    return false;
+}
+
+// polygons_intersect, streaming the larger side when the smaller is one ring.
+__device__ inline bool polygons_intersect_auto(const Polys& A, int64_t ra, const Polys& B, int64_t rb) {
+    int32_t ca, na, cb, nb;
+    const bool sa = single_ring_of(A, ra, ca, na), sb = single_ring_of(B, rb, cb, nb);
+    if (sa && (!sb || na <= nb)) return polygons_intersect_ring(A, ca, na, B, rb);
+    if (sb) return polygons_intersect_ring(B, cb, nb, A, ra);
+    return polygons_intersect(A, ra, B, rb);
+}
+
 }  // namespace vs

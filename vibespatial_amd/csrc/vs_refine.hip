@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, c
             const int loc = (x == x && y == y) ? point_location_f64(x, y, B.x, B.y, B.geom, B.part, B.ring, gi) : 0;
             k = loc_keep(pred, loc);
         } else {
-            k = polygons_intersect(A, l, B, r) ? 1 : 0;
+            k = polygons_intersect_auto(A, l, B, r) ? 1 : 0;
         }
         keep[i] = k;
     }
