@@ -209,8 +209,9 @@ int64_t vs_learned_index_bytes(int64_t n_points, int32_t key_bytes, int32_t n_le
  * leaf errors (int32 n_leaves), meta (int32[4]: [0] max MLP leaf error). */
 int vs_learned_index_layout(int64_t n_points, int32_t key_bytes, int32_t n_leaves, int64_t* offsets10);
 
-/* Builds the index over points: Hilbert keys of key_bytes*4 bits per axis over
- * the point extent, radix-sorted; the MLP is trained on the GPU for
+/* Builds the index over points: Hilbert keys over the point extent, radix-
+ * sorted (key_bytes 8: a 2^32 x 2^32 grid; key_bytes 4: a 2^12 x 2^12 grid,
+ * left-aligned so the low 8 key bits are zero); the MLP is trained on the GPU for
  * train_iters full-batch Adam steps (lr relative to each parameter's scale);
  * leaf table and error bounds are recorded over every key. */
 int vs_learned_index_build(const double* px, const double* py, int64_t n_points, int32_t key_bytes, int32_t n_leaves,

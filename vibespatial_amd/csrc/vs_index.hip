@@ -197,12 +197,14 @@ __global__ __launch_bounds__(256) void hilbert_keys_boxes_kernel(const double4* 
     ids[i] = (int32_t)i;
 }
 
+// Keys of 2*bits bits (right-aligned); the low `zero_bits` bits are cleared, so
+// the key is the Hilbert index of a coarser grid, left-aligned.
 template <class K>
 __global__ __launch_bounds__(256) void hilbert_keys_points_kernel(const double* __restrict__ px,
                                                                  const double* __restrict__ py,
                                                                  int64_t n,
                                                                  const double4* __restrict__ ext,
-                                                                 int bits, K* __restrict__ keys,
+                                                                 int bits, int zero_bits, K* __restrict__ keys,
                                                                  int32_t* __restrict__ ids) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -214,6 +216,7 @@ __global__ __launch_bounds__(256) void hilbert_keys_points_kernel(const double* 
         const uint32_t qy = quantize(c, ks.y0, ks.sy, bits) << (32 - bits);
         key = hilbert64(qx, qy) >> (64 - 2 * bits);
     }
+    if (zero_bits > 0) key = (key >> zero_bits) << zero_bits;
     keys[i] = (K)key;
     ids[i] = (int32_t)i;
 }
@@ -406,13 +409,14 @@ int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, 
     VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
     if (n == 0) return VS_OK;
     const int bits = 4 * (int)sizeof(K);
+    const int zero_bits = 2 * (bits - point_index_grid_bits<K>());
     hipLaunchKernelGGL(hilbert_keys_points_kernel<K>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py, n,
-                       ext, bits, keys, perm);
+                       ext, bits, zero_bits, keys, perm);
     VS_LAUNCH_CHECK();
     if (sizeof(K) == 4)
-        VS_TRY(sort_pairs_u32(reinterpret_cast<uint32_t*>(keys), perm, n, 0, 32, sort_ws, sb, s));
+        VS_TRY(sort_pairs_u32(reinterpret_cast<uint32_t*>(keys), perm, n, zero_bits, 32, sort_ws, sb, s));
     else
-        VS_TRY(sort_pairs_u64(reinterpret_cast<uint64_t*>(keys), perm, n, 0, 64, sort_ws, sb, s));
+        VS_TRY(sort_pairs_u64(reinterpret_cast<uint64_t*>(keys), perm, n, zero_bits, 64, sort_ws, sb, s));
     hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256 * kGatherItems)), dim3(256), 0, s, px,
                        py, perm, n, sx, sy);
     VS_LAUNCH_CHECK();
@@ -440,7 +444,7 @@ int point_order(const double* px, const double* py, int64_t n, int bits, int32_t
     void* sort_ws = c.take<char>(sort_workspace_bytes_u32(n));
     VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
     hipLaunchKernelGGL(hilbert_keys_points_kernel<uint32_t>, dim3((unsigned)ceil_div(n, 256)), dim3(256),
-                       0, s, px, py, n, ext, 16, keys, perm);
+                       0, s, px, py, n, ext, 16, 0, keys, perm);
     VS_LAUNCH_CHECK();
     VS_TRY(sort_pairs_u32(keys, perm, n, 32 - 2 * bits, 32, sort_ws, sort_workspace_bytes_u32(n), s));
     hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256 * kGatherItems)), dim3(256), 0, s, px,

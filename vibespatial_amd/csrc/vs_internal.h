@@ -74,6 +74,14 @@ struct Carver {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Grid of the learned point index: keys are Hilbert indices of a 2^g x 2^g
+// grid over the point extent, left-aligned in K (low bits zero).  32-bit keys
+// use g = 12 (three 8-bit radix passes); 64-bit keys g = 32.
+template <class K>
+constexpr int point_index_grid_bits() {
+    return sizeof(K) == 4 ? 12 : 32;
+}
+
 // Radix sort of (key, int32 value) pairs (vs_sort.hip).  The no-copy form
 // leaves the result in either the inputs or the workspace's alternates.
 template <class K>

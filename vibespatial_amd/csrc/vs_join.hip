@@ -547,7 +547,7 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
                            n_polys, geom_offsets, part_offsets, ring_offsets, J.item_range);
         VS_LAUNCH_CHECK();
     }
-    VS_TRY(vs::point_order(px, py, n_points, 16, J.perm, J.sx, J.sy, J.order_ws, s));
+    VS_TRY(vs::point_order(px, py, n_points, 12, J.perm, J.sx, J.sy, J.order_ws, s));  // 3 radix passes
     const vs::TreeView T = vs::make_tree_view(layout, node_boxes, item_ids);
     const vs::PolyView P = {x, y, J.flags, J.item_range};
     if (n_points > 0 && n_polys > 0)

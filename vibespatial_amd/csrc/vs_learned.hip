@@ -75,7 +75,7 @@ __device__ __forceinline__ float key_unit(K k) {
 
 __device__ __forceinline__ float mlp_eval(const float* P, float u) {
     float y = P[3 * kHidden];
-#pragma unroll
+#pragma unroll 4
     for (int h = 0; h < kHidden; ++h) {
         const float z = P[h] * u + P[kHidden + h];
         y += P[2 * kHidden + h] * fmaxf(z, 0.0f);
@@ -447,17 +447,18 @@ __device__ int64_t learned_lower_bound(K q, const LearnedView<K>& V, LookupStats
     return a;
 }
 
-// Cover of a box on the key grid: the finest level whose cells covering the
-// box number at most kMaxRanges (long thin boxes get thin covers; by level
-// bits-1 at most 2 x 2 remain), each cell a key range [code 4^l, (code+1) 4^l);
+// Cover of a box on the key grid: the finest level, no finer than min_l (the
+// index grid), whose cells covering the box number at most kMaxRanges (long
+// thin boxes get thin covers; by level bits-1 at most 2 x 2 remain), each cell
+// a key range [code 4^l, (code+1) 4^l);
 // the codes are sorted (a bitonic network on registers) and consecutive ones
 // merged.  Writes (lo, hi) pairs to out and returns their count.
-__device__ int cover_ranges(double x0, double y0, double x1, double y1, const KeyScale& ks, int bits,
+__device__ int cover_ranges(double x0, double y0, double x1, double y1, const KeyScale& ks, int bits, int min_l,
                             unsigned long long* __restrict__ out) {
     if (!(x0 <= x1 && y0 <= y1)) return 0;
     const uint32_t qx0 = quantize(x0, ks.x0, ks.sx, bits), qx1 = quantize(x1, ks.x0, ks.sx, bits);
     const uint32_t qy0 = quantize(y0, ks.y0, ks.sy, bits), qy1 = quantize(y1, ks.y0, ks.sy, bits);
-    int l = 0;
+    int l = min_l;  // cells no finer than the index grid, so every range is a union of grid cells
     while (l < bits - 1 && (uint64_t)((qx1 >> l) - (qx0 >> l) + 1) * ((qy1 >> l) - (qy0 >> l) + 1) > kMaxRanges) ++l;
     const int lev = bits - l;
     const uint32_t ax = qx0 >> l, ay = qy0 >> l;
@@ -520,7 +521,8 @@ __global__ __launch_bounds__(256) void cover_cells_kernel(const double4* __restr
     const int bits = 4 * (int)sizeof(K);
     const KeyScale ks = key_scale(ext, bits);
     const double4 b = qbox[q];
-    nranges[q] = cover_ranges(b.x, b.y, b.z, b.w, ks, bits, keyr + q * kMaxRanges * 2);
+    nranges[q] = cover_ranges(b.x, b.y, b.z, b.w, ks, bits, bits - point_index_grid_bits<K>(),
+                              keyr + q * kMaxRanges * 2);
 }
 
 // One thread per range end: rank of the range's first key (x) and of the key
@@ -1438,7 +1440,7 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
         hipLaunchKernelGGL(cover_cells_kernel<K>, dim3(gq), dim3(256), 0, s, qbox, nq, B.ext, Q.keyr, Q.nranges,
                            R.geom, R.part, R.ring, pinfo);
         VS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(cover_lookup_kernel<K>, dim3(stride_grid(nq * kMaxRanges * 2, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL(cover_lookup_kernel<K>, dim3(stride_grid(nq * kMaxRanges * 2, 256, 8)), dim3(256), 0, s,
                            nq, Q.keyr, Q.nranges, V, reinterpret_cast<int32_t*>(Q.ranges), Q.cstat);
         VS_LAUNCH_CHECK();
         hipLaunchKernelGGL(cover_total_kernel, dim3(stride_grid(nq, 256)), dim3(256), 0, s, nq, Q.ranges, Q.nranges,
