@@ -24,7 +24,7 @@ int sort_pairs_u32(uint32_t*, int32_t*, int64_t, int, int, void*, int64_t, hipSt
 int64_t sort_workspace_bytes_u64(int64_t n);
 int64_t sort_workspace_bytes_u32(int64_t n);
 
-// One row per lane: rows of at most 64 coordinates are reduced by their own
+// One row per lane: rows of at most 16 coordinates are reduced by their own
 // lane (rows are contiguous in memory, so the lanes of a wave read nearby
 // addresses); longer rows are then reduced by the whole wave, one at a time,
 // lanes striding over the coordinates.
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void polygon_bounds_kernel(
         c0 = ring[part[geom[row]]];
         c1 = ring[part[geom[row + 1]]];
     }
-    const bool small = inr && c1 - c0 <= 64;
+    const bool small = inr && c1 - c0 <= 16;
     if (small) {
         double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
         for (int32_t c = c0; c < c1; ++c) {
@@ -399,6 +399,16 @@ int64_t point_index_workspace_bytes_t(int64_t n) {
     return c.bytes;
 }
 
+int gather_points(const double* px, const double* py, const int32_t* perm, int64_t n, double* sx, double* sy,
+                  hipStream_t s) {
+    if (n == 0) return VS_OK;
+    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256 * kGatherItems)), dim3(256), 0, s, px,
+                       py, perm, n, sx, sy);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+// Keys, sort and (when sx is not null) the gathered coordinates.
 template <class K>
 int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, int32_t* perm, double* sx,
                         double* sy, double4* ext, void* ws, hipStream_t s) {
@@ -417,9 +427,7 @@ int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, 
         VS_TRY(sort_pairs_u32(reinterpret_cast<uint32_t*>(keys), perm, n, zero_bits, 32, sort_ws, sb, s));
     else
         VS_TRY(sort_pairs_u64(reinterpret_cast<uint64_t*>(keys), perm, n, zero_bits, 64, sort_ws, sb, s));
-    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256 * kGatherItems)), dim3(256), 0, s, px,
-                       py, perm, n, sx, sy);
-    VS_LAUNCH_CHECK();
+    if (sx) VS_TRY(gather_points(px, py, perm, n, sx, sy, s));
     return VS_OK;
 }
 

@@ -37,6 +37,8 @@
 //   covering both the input rounding and the fp32 arithmetic; otherwise escalate.
 // An escalated edge is evaluated by the exact fp64 step (stage-A filter, then
 // the exact expansion) on the original coordinates.
+#include <mutex>
+
 #include "vs_exact.h"
 #include "vs_internal.h"
 #include "vs_join_common.h"
@@ -45,6 +47,8 @@ namespace vs {
 
 int64_t point_index_workspace_bytes_u32(int64_t n);
 int64_t point_index_workspace_bytes_u64(int64_t n);
+int gather_points(const double* px, const double* py, const int32_t* perm, int64_t n, double* sx, double* sy,
+                  hipStream_t s);
 int point_index_build_u32(const double* px, const double* py, int64_t n, uint32_t* keys, int32_t* perm, double* sx,
                           double* sy, double4* ext, void* ws, hipStream_t s);
 int point_index_build_u64(const double* px, const double* py, int64_t n, uint64_t* keys, int32_t* perm, double* sx,
@@ -1335,26 +1339,67 @@ __global__ void zero_i32_kernel(int32_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+// A second stream per device (default priority) for work that overlaps the
+// caller's stream inside one call, with a fork / join event pair.  Created on
+// first use and kept for the process.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+static int side_stream(SideStream** out) {
+    static std::mutex mu;
+    static SideStream per_dev[64];
+    int dev = 0;
+    VS_CHECK_HIP(hipGetDevice(&dev));
+    VS_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    SideStream& S = per_dev[dev];
+    if (!S.stream) {
+        VS_CHECK_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+        VS_CHECK_HIP(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+        VS_CHECK_HIP(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
+    }
+    *out = &S;
+    return VS_OK;
+}
+
+// Keys and their sort on the caller's stream; then the network's training,
+// the leaf table and the error pass (which need only the sorted keys) run on
+// the side stream while the caller's stream gathers the sorted coordinates.
+// The call joins both before returning.  Calls that share a device serialise
+// on the side stream's events, so one call must finish enqueueing before the
+// next starts (the library's entry points are not reentrant on a device).
 template <class K>
 int index_build_t(const double* px, const double* py, int64_t n, int S, int iters, float lr, const IndexBufs<K>& B,
                   hipStream_t s) {
     if (sizeof(K) == 4)
-        VS_TRY(point_index_build_u32(px, py, n, reinterpret_cast<uint32_t*>(B.keys), B.perm, B.sx, B.sy, B.ext,
+        VS_TRY(point_index_build_u32(px, py, n, reinterpret_cast<uint32_t*>(B.keys), B.perm, nullptr, nullptr, B.ext,
                                      B.build_ws, s));
     else
-        VS_TRY(point_index_build_u64(px, py, n, reinterpret_cast<uint64_t*>(B.keys), B.perm, B.sx, B.sy, B.ext,
+        VS_TRY(point_index_build_u64(px, py, n, reinterpret_cast<uint64_t*>(B.keys), B.perm, nullptr, nullptr, B.ext,
                                      B.build_ws, s));
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, B.meta, 4);
+    SideStream* side = nullptr;
+    VS_TRY(side_stream(&side));
+    VS_CHECK_HIP(hipEventRecord(side->fork, s));
+    VS_CHECK_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+    const hipStream_t t = side->stream;
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, t, B.meta, 4);
     VS_LAUNCH_CHECK();
-    if (n < 2) return VS_OK;
-    hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(vs::kTrainThreads), 0, s, B.keys, n, iters, lr, B.mlp, B.losses);
-    VS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(leaf_keys_kernel<K>, dim3((unsigned)ceil_div(S + 1, 256)), dim3(256), 0, s, B.keys, n, S,
-                       B.leaf_key);
-    VS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(leaf_err_kernel<K>, dim3(stride_grid(S, 4)), dim3(256), 0, s, B.keys, n, S, B.leaf_key,
-                       B.mlp, B.leaf_err, B.meta);
-    VS_LAUNCH_CHECK();
+    if (n >= 2) {
+        hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(vs::kTrainThreads), 0, t, B.keys, n, iters, lr, B.mlp,
+                           B.losses);
+        VS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(leaf_keys_kernel<K>, dim3((unsigned)ceil_div(S + 1, 256)), dim3(256), 0, t, B.keys, n, S,
+                           B.leaf_key);
+        VS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(leaf_err_kernel<K>, dim3(stride_grid(S, 4)), dim3(256), 0, t, B.keys, n, S, B.leaf_key,
+                           B.mlp, B.leaf_err, B.meta);
+        VS_LAUNCH_CHECK();
+    }
+    VS_TRY(gather_points(px, py, B.perm, n, B.sx, B.sy, s));
+    VS_CHECK_HIP(hipEventRecord(side->join, t));
+    VS_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
     return VS_OK;
 }
 
