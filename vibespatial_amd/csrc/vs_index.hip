@@ -24,52 +24,36 @@ int sort_pairs_u32(uint32_t*, int32_t*, int64_t, int, int, void*, int64_t, hipSt
 int64_t sort_workspace_bytes_u64(int64_t n);
 int64_t sort_workspace_bytes_u32(int64_t n);
 
-// One row per lane: rows of at most 16 coordinates are reduced by their own
-// lane (rows are contiguous in memory, so the lanes of a wave read nearby
-// addresses); longer rows are then reduced by the whole wave, one at a time,
-// lanes striding over the coordinates.
-__device__ __forceinline__ void bounds_store(double4* out, int64_t row, bool empty, double x0, double y0, double x1,
-                                             double y1) {
-    if (empty || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
-    out[row] = make_double4(x0, y0, x1, y1);
-}
-
+// Sixteen lanes per polygonal row, striding over its coordinates (rows are
+// contiguous, so a wave's four rows read one span of memory), reduced with
+// 16-wide shuffles.
 __global__ __launch_bounds__(256) void polygon_bounds_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const int32_t* __restrict__ geom,
     const int32_t* __restrict__ part, const int32_t* __restrict__ ring, int64_t n_rows,
     double4* __restrict__ out) {
-    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int sub = threadIdx.x & 15;
     const bool inr = row < n_rows;
     int32_t c0 = 0, c1 = 0;
     if (inr) {
         c0 = ring[part[geom[row]]];
         c1 = ring[part[geom[row + 1]]];
     }
-    const bool small = inr && c1 - c0 <= 16;
-    if (small) {
-        double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
-        for (int32_t c = c0; c < c1; ++c) {
-            const double xv = x[c], yv = y[c];
-            x0 = fmin(x0, xv); y0 = fmin(y0, yv);
-            x1 = fmax(x1, xv); y1 = fmax(y1, yv);
-        }
-        bounds_store(out, row, c1 <= c0, x0, y0, x1, y1);
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (int32_t c = c0 + sub; c < c1; c += 16) {
+        const double xv = x[c], yv = y[c];
+        x0 = fmin(x0, xv); y0 = fmin(y0, yv);
+        x1 = fmax(x1, xv); y1 = fmax(y1, yv);
     }
-    const int lane = threadIdx.x & 63;
-    uint64_t m = __ballot(inr && !small);
-    while (m) {
-        const int s = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        const int32_t r0 = readlane_i32(c0, s), r1 = readlane_i32(c1, s);
-        double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
-        for (int32_t c = r0 + lane; c < r1; c += 64) {
-            const double xv = x[c], yv = y[c];
-            x0 = fmin(x0, xv); y0 = fmin(y0, yv);
-            x1 = fmax(x1, xv); y1 = fmax(y1, yv);
-        }
-        x0 = wave_min(x0); y0 = wave_min(y0);
-        x1 = wave_max(x1); y1 = wave_max(y1);
-        if (lane == s) bounds_store(out, row, false, x0, y0, x1, y1);
+    for (int o = 8; o > 0; o >>= 1) {
+        x0 = fmin(x0, __shfl_xor(x0, o, 16));
+        y0 = fmin(y0, __shfl_xor(y0, o, 16));
+        x1 = fmax(x1, __shfl_xor(x1, o, 16));
+        y1 = fmax(y1, __shfl_xor(y1, o, 16));
+    }
+    if (inr && sub == 0) {
+        if (c1 <= c0 || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
+        out[row] = make_double4(x0, y0, x1, y1);
     }
 }
 
@@ -263,7 +247,7 @@ __global__ __launch_bounds__(256) void point_boxes_kernel(const double* __restri
 int polygon_bounds(const double* x, const double* y, const int32_t* geom, const int32_t* part,
                    const int32_t* ring, int64_t n, double* bounds, hipStream_t s) {
     if (n == 0) return VS_OK;
-    hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, x, y,
+    hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 16)), dim3(256), 0, s, x, y,
                        geom, part, ring, n, reinterpret_cast<double4*>(bounds));
     VS_LAUNCH_CHECK();
     return VS_OK;

@@ -173,6 +173,8 @@ __global__ __launch_bounds__(kTrainThreads) void mlp_train_kernel(const K* __res
     __shared__ float red[kTrainWaves][kNParams + 1];
     __shared__ float tot[kNParams + 1];
     __shared__ float best_loss, init_loss;
+    // one block beside the coordinate gather: raise its waves' issue priority
+    __builtin_amdgcn_s_setprio(3);
     const int tid = threadIdx.x;
     const int S = n >= kSample ? kSample : (int)n;
     for (int j = tid; j < S; j += blockDim.x) {
