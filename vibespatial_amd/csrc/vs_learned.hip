@@ -60,7 +60,7 @@ int64_t sort_workspace_bytes_u32(int64_t n);
 
 constexpr int kHidden = 32;
 constexpr int kNParams = 3 * kHidden + 1;  // w1[32], b1[32], w2[32], b2
-constexpr int kSample = 2048;              // training keys (evenly spaced ranks)
+constexpr int kSample = 1024;              // training keys (evenly spaced ranks)
 constexpr int kMaxRanges = 16;             // cells of a query's cover (merged into at most this many ranges)
 constexpr int kChunk = 512;                // positions scanned per round
 constexpr int kItemRounds = 8;             // rounds per work item (items span kChunk * kItemRounds positions)
