@@ -13,8 +13,8 @@
 //      extent of all right centres); a packed tree per job (fan-out 16, root
 //      first, items last) is built by one block per job into one forest array;
 //   3. left rows sorted the same way; every job is cut into waves of 64
-//      consecutive sorted left rows; a persistent kernel hands waves out from
-//      one queue and each wave walks its own job's tree with a shared LDS
+//      consecutive sorted left rows; a persistent kernel deals the waves out
+//      round-robin and each wave walks its own job's tree with a shared LDS
 //      stack, staging bbox candidates;
 //   4. one thread per candidate runs the exact intersects refine (edges of
 //      the pair, then one vertex per ring); survivors go to the finish pass
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void forest_build_kernel(const double4* __rest
     }
 }
 
-// Persistent: waves take (job, 64 sorted left rows) tasks from ctl->queue[0].
+// Persistent: waves take (job, 64 sorted left rows) tasks round-robin.
 __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __restrict__ lbox,
                                                           const int32_t* __restrict__ lperm,
                                                           const int64_t* __restrict__ left_job,
@@ -173,11 +173,11 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         nbuf = 0;
     };
-    for (;;) {
-        unsigned int t = 0;
-        if (lane == 0) t = atomicAdd(&ctl->queue[0], 1u);
-        t = __shfl(t, 0, 64);
-        if ((long long)t >= W) break;
+    // static round-robin of the wave tasks (a shared queue counter would
+    // serialise every task start in L2)
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long tt = (long long)blockIdx.x * (blockDim.x >> 6) + w; tt < W; tt += nw) {
+        const unsigned int t = (unsigned int)tt;
         // job of wave t: last j with wave_off[j] <= t
         int64_t lo = 0, hi = n_jobs - 1;
         while (lo < hi) {
