@@ -6,8 +6,8 @@ over [0, 1e5]^2 and 100,000 star polygons with 0-3 holes and lognormal vertex
 counts (vibespatial_amd.synthetic.c2, seeded).  A step is one full join from
 device-resident SoA inputs to device-resident lexsorted int64
 (left_index, right_index) pairs: polygon bounds, packed Hilbert tree build,
-point Hilbert ordering, traversal with fused exact refine, scan/scatter/sort of
-the pairs.  Input H2D and output D2H are outside the timed region.
+point Hilbert ordering, traversal with fused exact refine, radix sort of
+the staged pairs by left id and the sorted emit.  Input H2D and output D2H are outside the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -196,7 +196,7 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach_l / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": _measured_traffic(n_pts, len(P_h)),
             "kernel_ms": kms_l,
             "alg_bytes_per_launch": alg_l,
         },
@@ -205,6 +205,19 @@ def main(argv=None):
     print(json.dumps(out), flush=True)
     _copy_probe(args)
     return 0
+
+
+# HBM bytes per learned_join_kernel launch on the default workload, from two
+# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) scaled by
+# what the same counters read for a 1 GiB device-to-device copy in the same run
+# (FETCH_SIZE 524,317 KiB -> x2.0, WRITE_SIZE 1,048,576 KiB -> x1.0):
+#   (2.0 * 410,552 + 1.0 * 252,163) KiB = 1,099,025,408 bytes.
+# The summary with both commands is profiles/learned_join_kernel_10Mpts_100kpolys_pmc_traffic.md.
+_TRAFFIC_BYTES = {(10_000_000, 100_000): 1_099_025_408}
+
+
+def _measured_traffic(n_points, n_polys):
+    return _TRAFFIC_BYTES.get((n_points, n_polys))
 
 
 def _copy_probe(args):
