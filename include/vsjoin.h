@@ -247,7 +247,9 @@ int64_t vs_sjoin_points_polygons_learned_workspace_bytes(int64_t n_points, int64
                                                          int64_t capacity);
 
 /* The learned + FP32 form of vs_sjoin_points_polygons (same pairs, bit for
- * bit): builds the learned index over the points (32-bit keys), covers each
+ * bit): builds the learned index over the points (32-bit keys; the network
+ * gets train_iters Adam steps of 3e-4 relative to each parameter's scale
+ * after its quantile-knot start), covers each
  * polygon's bounds (poly_bounds, from vs_polygon_bounds) with rank ranges and
  * refines candidates polygon-major in FP32 with escalation to the exact fp64
  * step.  d_result (device int64[16]): [0] K, [1] staged, [2] candidates,

@@ -61,7 +61,7 @@ def test_radix_sort_partial_bits_is_stable():
     assert np.array_equal(vals.cpu().numpy(), order)
 
 
-@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 1_000_003])
+@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 1_000_003, 4_194_304, 5_000_001])
 def test_exclusive_scan(n):
     L = _lib.lib()
     rng = np.random.default_rng(n)

@@ -326,6 +326,9 @@ struct BatchWs {
     double4* fbox;
     uint32_t* cand_l;
     int32_t* cand_r;
+    uint32_t* cs_r;  // candidates sorted by right row
+    int32_t* cs_l;
+    void* csort_ws;
     JoinCtl* cctl;
     JoinCtl* ctl;
     void* fin_ws;
@@ -362,6 +365,9 @@ static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n
     W.fbox = c.take<double4>(forest_node_bound(n_right, n_jobs));
     W.cand_l = c.take<uint32_t>(capacity);
     W.cand_r = c.take<int32_t>(capacity);
+    W.cs_r = c.take<uint32_t>(capacity);
+    W.cs_l = c.take<int32_t>(capacity);
+    W.csort_ws = c.take<char>(sort_workspace_bytes_u32(capacity));
     W.cctl = c.take<JoinCtl>(1);
     W.ctl = c.take<JoinCtl>(1);
     W.fin_ws = c.take<char>(finish_workspace_bytes(capacity));
@@ -421,10 +427,17 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
                            W.lperm, left_job, right_job, n_jobs, W.wave_off, W.total_waves, W.fbox, W.tree_off,
                            W.rperm, W.cand_l, W.cand_r, capacity, W.cctl);
         VS_LAUNCH_CHECK();
+        // candidates grouped by right row: the lanes of a wave then share
+        // parcels (same coordinates, same edge count)
+        VS_TRY(sort_pairs_into_u32(reinterpret_cast<const uint32_t*>(W.cand_r),
+                                   reinterpret_cast<const int32_t*>(W.cand_l), W.cs_r, W.cs_l, capacity, 0,
+                                   bits_for_jobs(n_right), W.csort_ws, sort_workspace_bytes_u32(capacity), s,
+                                   &W.cctl->staged));
         const unsigned rgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, refine_threads),
                                                                                   (int64_t)cus * 16));
-        hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads), 0, s, W.cand_l, W.cand_r, W.cctl,
-                           capacity, A, B, W.stage_l, W.stage_r, capacity, W.ctl);
+        hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads), 0, s,
+                           reinterpret_cast<const uint32_t*>(W.cs_l), reinterpret_cast<const int32_t*>(W.cs_r),
+                           W.cctl, capacity, A, B, W.stage_l, W.stage_r, capacity, W.ctl);
         VS_LAUNCH_CHECK();
         VS_TRY(timing_end_public(s));
     }

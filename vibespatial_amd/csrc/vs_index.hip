@@ -308,6 +308,8 @@ int index_build(const double* bounds, int64_t n, const vs_tree_layout* L, double
 int64_t box_order_workspace_bytes(int64_t n) {
     Carver c(nullptr);
     c.take<uint64_t>(n);
+    c.take<uint64_t>(n);
+    c.take<int32_t>(n);
     c.take<double4>(1);
     c.take<char>(extent_workspace_bytes());
     c.take<char>(sort_workspace_bytes_u64(n));
@@ -318,47 +320,54 @@ int box_order(const double4* b, int64_t n, int32_t* perm, void* ws, hipStream_t 
     if (n == 0) return VS_OK;
     Carver c(ws);
     uint64_t* keys = c.take<uint64_t>(n);
+    uint64_t* keys_in = c.take<uint64_t>(n);
+    int32_t* ids_in = c.take<int32_t>(n);
     double4* ext = c.take<double4>(1);
     void* ext_ws = c.take<char>(extent_workspace_bytes());
     void* sort_ws = c.take<char>(sort_workspace_bytes_u64(n));
     VS_TRY(centre_extent(b, n, ext, ext_ws, s));
     hipLaunchKernelGGL(hilbert_keys_boxes_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, b, n, ext, 16,
-                       keys, perm);
+                       keys_in, ids_in);
     VS_LAUNCH_CHECK();
     // NaN boxes carry all-ones keys: sort 33 bits so they stay after every code
-    VS_TRY(sort_pairs_u64(keys, perm, n, 0, 40, sort_ws, sort_workspace_bytes_u64(n), s));
+    VS_TRY(sort_pairs_into_u64(keys_in, ids_in, keys, perm, n, 0, 40, sort_ws, sort_workspace_bytes_u64(n), s));
     return VS_OK;
 }
 
 // Point ordering for coherent traversal: 16-level (32-bit) Hilbert keys over
 // the point extent, sorted; writes the permutation and the gathered coords.
 // Four points per thread, every load issued before the first store, so each
-// thread keeps eight random reads in flight.
+// thread keeps eight random reads in flight; blocks stride over the tiles of
+// 1024 points, so the grid can be capped to leave compute units free for
+// work on another stream.
 constexpr int kGatherItems = 4;
+constexpr int kGatherTile = 256 * kGatherItems;
 __global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ px,
                                                            const double* __restrict__ py,
                                                            const int32_t* __restrict__ perm,
                                                            int64_t n, double* __restrict__ sx,
                                                            double* __restrict__ sy) {
-    const int64_t base = (int64_t)blockIdx.x * (256 * kGatherItems) + threadIdx.x;
-    int32_t j[kGatherItems];
-    double x[kGatherItems], y[kGatherItems];
+    for (int64_t t0 = (int64_t)blockIdx.x * kGatherTile; t0 < n; t0 += (int64_t)gridDim.x * kGatherTile) {
+        const int64_t base = t0 + threadIdx.x;
+        int32_t j[kGatherItems];
+        double x[kGatherItems], y[kGatherItems];
 #pragma unroll
-    for (int b = 0; b < kGatherItems; ++b) {
-        const int64_t i = base + 256 * b;
-        j[b] = i < n ? perm[i] : -1;
-    }
+        for (int b = 0; b < kGatherItems; ++b) {
+            const int64_t i = base + 256 * b;
+            j[b] = i < n ? perm[i] : -1;
+        }
 #pragma unroll
-    for (int b = 0; b < kGatherItems; ++b) {
-        x[b] = j[b] >= 0 ? px[j[b]] : 0.0;
-        y[b] = j[b] >= 0 ? py[j[b]] : 0.0;
-    }
+        for (int b = 0; b < kGatherItems; ++b) {
+            x[b] = j[b] >= 0 ? px[j[b]] : 0.0;
+            y[b] = j[b] >= 0 ? py[j[b]] : 0.0;
+        }
 #pragma unroll
-    for (int b = 0; b < kGatherItems; ++b) {
-        const int64_t i = base + 256 * b;
-        if (i < n) {
-            sx[i] = x[b];
-            sy[i] = y[b];
+        for (int b = 0; b < kGatherItems; ++b) {
+            const int64_t i = base + 256 * b;
+            if (i < n) {
+                sx[i] = x[b];
+                sy[i] = y[b];
+            }
         }
     }
 }
@@ -366,6 +375,8 @@ __global__ __launch_bounds__(256) void gather_points_kernel(const double* __rest
 int64_t point_order_workspace_bytes(int64_t n) {
     Carver c(nullptr);
     c.take<uint32_t>(n);
+    c.take<uint32_t>(n);
+    c.take<int32_t>(n);
     c.take<double4>(1);
     c.take<char>(extent_workspace_bytes());
     c.take<char>(sort_workspace_bytes_u32(n));
@@ -380,13 +391,15 @@ int64_t point_index_workspace_bytes_t(int64_t n) {
     Carver c(nullptr);
     c.take<char>(extent_workspace_bytes());
     c.take<char>(sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n));
+    c.take<K>(n);  // unsorted keys and ids
+    c.take<int32_t>(n);
     return c.bytes;
 }
 
 int gather_points(const double* px, const double* py, const int32_t* perm, int64_t n, double* sx, double* sy,
-                  hipStream_t s) {
+                  hipStream_t s, int blocks_per_cu) {
     if (n == 0) return VS_OK;
-    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256 * kGatherItems)), dim3(256), 0, s, px,
+    hipLaunchKernelGGL(gather_points_kernel, dim3(stride_grid(n, kGatherTile, blocks_per_cu)), dim3(256), 0, s, px,
                        py, perm, n, sx, sy);
     VS_LAUNCH_CHECK();
     return VS_OK;
@@ -400,17 +413,21 @@ int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, 
     void* ext_ws = c.take<char>(extent_workspace_bytes());
     const int64_t sb = sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n);
     void* sort_ws = c.take<char>(sb);
+    K* keys_in = c.take<K>(n);
+    int32_t* ids_in = c.take<int32_t>(n);
     VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
     if (n == 0) return VS_OK;
     const int bits = 4 * (int)sizeof(K);
     const int zero_bits = 2 * (bits - point_index_grid_bits<K>());
     hipLaunchKernelGGL(hilbert_keys_points_kernel<K>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py, n,
-                       ext, bits, zero_bits, keys, perm);
+                       ext, bits, zero_bits, keys_in, ids_in);
     VS_LAUNCH_CHECK();
     if (sizeof(K) == 4)
-        VS_TRY(sort_pairs_u32(reinterpret_cast<uint32_t*>(keys), perm, n, zero_bits, 32, sort_ws, sb, s));
+        VS_TRY(sort_pairs_into_u32(reinterpret_cast<const uint32_t*>(keys_in), ids_in,
+                                   reinterpret_cast<uint32_t*>(keys), perm, n, zero_bits, 32, sort_ws, sb, s));
     else
-        VS_TRY(sort_pairs_u64(reinterpret_cast<uint64_t*>(keys), perm, n, zero_bits, 64, sort_ws, sb, s));
+        VS_TRY(sort_pairs_into_u64(reinterpret_cast<const uint64_t*>(keys_in), ids_in,
+                                   reinterpret_cast<uint64_t*>(keys), perm, n, zero_bits, 64, sort_ws, sb, s));
     if (sx) VS_TRY(gather_points(px, py, perm, n, sx, sy, s));
     return VS_OK;
 }
@@ -431,16 +448,19 @@ int point_order(const double* px, const double* py, int64_t n, int bits, int32_t
     if (n == 0) return VS_OK;
     Carver c(ws);
     uint32_t* keys = c.take<uint32_t>(n);
+    uint32_t* keys_in = c.take<uint32_t>(n);
+    int32_t* ids_in = c.take<int32_t>(n);
     double4* ext = c.take<double4>(1);
     void* ext_ws = c.take<char>(extent_workspace_bytes());
     void* sort_ws = c.take<char>(sort_workspace_bytes_u32(n));
     VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
     hipLaunchKernelGGL(hilbert_keys_points_kernel<uint32_t>, dim3((unsigned)ceil_div(n, 256)), dim3(256),
-                       0, s, px, py, n, ext, 16, 0, keys, perm);
+                       0, s, px, py, n, ext, 16, 0, keys_in, ids_in);
     VS_LAUNCH_CHECK();
-    VS_TRY(sort_pairs_u32(keys, perm, n, 32 - 2 * bits, 32, sort_ws, sort_workspace_bytes_u32(n), s));
-    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, 256 * kGatherItems)), dim3(256), 0, s, px,
-                       py, perm, n, sx, sy);
+    VS_TRY(sort_pairs_into_u32(keys_in, ids_in, keys, perm, n, 32 - 2 * bits, 32, sort_ws,
+                               sort_workspace_bytes_u32(n), s));
+    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, kGatherTile)), dim3(256), 0, s, px, py,
+                       perm, n, sx, sy);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }

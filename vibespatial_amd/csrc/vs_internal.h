@@ -82,6 +82,11 @@ constexpr int point_index_grid_bits() {
     return sizeof(K) == 4 ? 12 : 32;
 }
 
+// sx[i] = px[perm[i]], sy[i] = py[perm[i]]; at most blocks_per_cu blocks of
+// 256 threads per compute unit (vs_index.hip).
+int gather_points(const double* px, const double* py, const int32_t* perm, int64_t n, double* sx, double* sy,
+                  hipStream_t s, int blocks_per_cu = 64);
+
 // Radix sort of (key, int32 value) pairs (vs_sort.hip).  The no-copy form
 // leaves the result in either the inputs or the workspace's alternates.
 template <class K>
@@ -92,6 +97,17 @@ struct SortResult {
     int32_t* alt_vals;
 };
 int64_t sort_workspace_bytes_u32(int64_t n);
+int64_t sort_workspace_bytes_u64(int64_t n);
+// Sorts (in_keys, in_vals) into (out_keys, out_vals) with no copy pass; the
+// inputs are only read and must not alias the outputs.  With d_n (device
+// count, optional) only the first *d_n of the n slots are sorted, and a count
+// above n sorts nothing.
+int sort_pairs_into_u32(const uint32_t* in_keys, const int32_t* in_vals, uint32_t* out_keys, int32_t* out_vals,
+                        int64_t n, int begin_bit, int end_bit, void* ws, int64_t ws_bytes, hipStream_t s,
+                        const unsigned long long* d_n = nullptr);
+int sort_pairs_into_u64(const uint64_t* in_keys, const int32_t* in_vals, uint64_t* out_keys, int32_t* out_vals,
+                        int64_t n, int begin_bit, int end_bit, void* ws, int64_t ws_bytes, hipStream_t s,
+                        const unsigned long long* d_n = nullptr);
 int sort_pairs_u32_dev(uint32_t* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit,
                        int end_bit, void* ws, int64_t ws_bytes, hipStream_t s, SortResult<uint32_t>* res);
 

@@ -79,7 +79,8 @@ constexpr int kOutBuf = 256;  // per-wave LDS buffer of refined pairs
 
 enum { kModePip = 0, kModeBox = 1 };
 
-__global__ void ctl_reset_kernel(JoinCtl* ctl) {
+// Clears the control block and, when given, n_extra further counters.
+__global__ void ctl_reset_kernel(JoinCtl* ctl, unsigned long long* extra, int n_extra) {
     if (threadIdx.x == 0) {
         ctl->staged = 0;
         ctl->n_long = 0;
@@ -87,6 +88,7 @@ __global__ void ctl_reset_kernel(JoinCtl* ctl) {
         for (int i = 0; i < 4; ++i) ctl->stat[i] = 0;
         ctl->escalated_pairs = 0;
     }
+    if (extra && (int)threadIdx.x < n_extra) extra[threadIdx.x] = 0;
 }
 
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
@@ -480,7 +482,9 @@ int build_refine_flags(const int32_t* part, const int32_t* ring, int64_t n_parts
     return VS_OK;
 }
 
-void ctl_reset(JoinCtl* ctl, hipStream_t s) { hipLaunchKernelGGL(ctl_reset_kernel, dim3(1), dim3(64), 0, s, ctl); }
+void ctl_reset(JoinCtl* ctl, hipStream_t s, unsigned long long* extra, int n_extra) {
+    hipLaunchKernelGGL(ctl_reset_kernel, dim3(1), dim3(64), 0, s, ctl, extra, n_extra);
+}
 
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
@@ -539,7 +543,7 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
     VS_REQUIRE(workspace_bytes >= need, "vs_sjoin_points_polygons: workspace %lld < %lld bytes",
                (long long)workspace_bytes, (long long)need);
     hipStream_t s = vs::stream_of(stream);
-    hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl);
+    hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl, nullptr, 0);
     VS_LAUNCH_CHECK();
     VS_TRY(vs::build_refine_flags(part_offsets, ring_offsets, n_parts, n_rings, n_coords, J.flags, s));
     if (n_polys > 0) {
@@ -576,7 +580,7 @@ int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_l
     VS_REQUIRE(workspace_bytes >= need, "vs_query_boxes: workspace %lld < %lld bytes",
                (long long)workspace_bytes, (long long)need);
     hipStream_t s = vs::stream_of(stream);
-    hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl);
+    hipLaunchKernelGGL(vs::ctl_reset_kernel, dim3(1), dim3(64), 0, s, J.ctl, nullptr, 0);
     VS_LAUNCH_CHECK();
     const vs::TreeView T = vs::make_tree_view(layout, node_boxes, item_ids);
     const vs::PolyView P = {nullptr, nullptr, nullptr, nullptr};

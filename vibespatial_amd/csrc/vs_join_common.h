@@ -19,7 +19,8 @@ struct JoinCtl {
     unsigned long long escalated_pairs;
 };
 
-void ctl_reset(JoinCtl* ctl, hipStream_t s);
+// Clears ctl and extra[0, n_extra) (n_extra <= 64) with one launch.
+void ctl_reset(JoinCtl* ctl, hipStream_t s, unsigned long long* extra = nullptr, int n_extra = 0);
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 
 // Finish (vs_finish.hip).  A join appends pairs to the two stage arrays that

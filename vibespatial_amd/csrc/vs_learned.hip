@@ -47,8 +47,6 @@ namespace vs {
 
 int64_t point_index_workspace_bytes_u32(int64_t n);
 int64_t point_index_workspace_bytes_u64(int64_t n);
-int gather_points(const double* px, const double* py, const int32_t* perm, int64_t n, double* sx, double* sy,
-                  hipStream_t s);
 int point_index_build_u32(const double* px, const double* py, int64_t n, uint32_t* keys, int32_t* perm, double* sx,
                           double* sy, double4* ext, void* ws, hipStream_t s);
 int point_index_build_u64(const double* px, const double* py, int64_t n, uint64_t* keys, int32_t* perm, double* sx,
@@ -66,6 +64,7 @@ constexpr int kChunk = 512;                // positions scanned per round
 constexpr int kItemRounds = 8;             // rounds per work item (items span kChunk * kItemRounds positions)
 constexpr int kItemPos = kChunk * kItemRounds;
 constexpr int kLJThreads = 256;
+constexpr int kClaim = 4;                  // work items taken per queue atomic
 constexpr int kCandPerThread = kChunk / kLJThreads;
 
 // ---------------------------------------------------------------------------
@@ -97,6 +96,9 @@ __device__ __forceinline__ float mlp_eval(const float* P, float u) {
 //            dL/db1_h += dy w2_h.
 // Parameters: P[h] = w1, P[32 + h] = b1, P[64 + h] = w2, P[96] = b2.
 constexpr int kTrainThreads = 1024;
+// Adam step (relative to each parameter's scale) of the join's index build:
+// larger steps leave the quantile-knot start best for the first tens of steps.
+constexpr float kJoinTrainLr = 3.0e-4f;
 constexpr int kTrainWaves = kTrainThreads / 64;
 
 struct GradPart {
@@ -167,7 +169,8 @@ __device__ void mlp_grad_reduce(const GradPart& gp, float (*red)[kNParams + 1], 
 template <class K>
 __global__ __launch_bounds__(kTrainThreads) void mlp_train_kernel(const K* __restrict__ keys, int64_t n, int iters,
                                                                  float lr, float* __restrict__ params_out,
-                                                                 float* __restrict__ losses) {
+                                                                 float* __restrict__ losses,
+                                                                 int32_t* __restrict__ meta) {
     __shared__ float su[kSample], st[kSample], sr[kSample];
     __shared__ float P[kNParams], M1[kNParams], M2[kNParams], SC[kNParams], BEST[kNParams];
     __shared__ float red[kTrainWaves][kNParams + 1];
@@ -176,6 +179,7 @@ __global__ __launch_bounds__(kTrainThreads) void mlp_train_kernel(const K* __res
     // one block beside the coordinate gather: raise its waves' issue priority
     __builtin_amdgcn_s_setprio(3);
     const int tid = threadIdx.x;
+    if (meta && tid < 4) meta[tid] = 0;  // the leaf error pass that follows takes maxima there
     const int S = n >= kSample ? kSample : (int)n;
     for (int j = tid; j < S; j += blockDim.x) {
         const int64_t i = S > 1 ? (int64_t)((n - 1) * (double)j / (double)(S - 1)) : 0;
@@ -317,18 +321,13 @@ __device__ __forceinline__ double plm_pred(K q, K k0, K k1, int64_t p0, int64_t 
     return (double)p0 + f * (double)(p1 - p0);
 }
 
-template <class K>
-__global__ __launch_bounds__(256) void leaf_keys_kernel(const K* __restrict__ keys, int64_t n, int S,
-                                                       K* __restrict__ leaf_key) {
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s <= S) leaf_key[s] = keys[leaf_pos(s, n, S)];
-}
-
-// One wave per leaf: err[s] = ceil(max |pred - i|) over i in [pos(s), pos(s+1)];
-// the MLP leaf error over the leaf's first occurrences goes to meta[0].
+// One wave per leaf: leaf_key[s] = keys[pos(s)] (and leaf_key[S] = keys[n - 1]),
+// err[s] = ceil(max |pred - i|) over i in [pos(s), pos(s+1)]; the MLP leaf
+// error over the leaf's first occurrences goes to meta[0] (cleared by the
+// trainer, which runs first).
 template <class K>
 __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ keys, int64_t n, int S,
-                                                      const K* __restrict__ leaf_key, const float* __restrict__ mlp,
+                                                      K* __restrict__ leaf_key, const float* __restrict__ mlp,
                                                       int32_t* __restrict__ leaf_err, int32_t* __restrict__ meta) {
     __shared__ float P[kNParams];
     for (int k = threadIdx.x; k < kNParams; k += blockDim.x) P[k] = mlp[k];
@@ -338,19 +337,25 @@ __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ key
     int mlp_e = 0;  // over every leaf this thread visits
     for (int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); s < S; s += (int64_t)gridDim.x * 4) {
         const int64_t p0 = leaf_pos(s, n, S), p1 = leaf_pos(s + 1, n, S);
-        const K k0 = leaf_key[s], k1 = leaf_key[s + 1];
+        const K k0 = keys[p0], k1 = keys[p1];
+        if (lane == 0) {
+            leaf_key[s] = k0;
+            if (s == S - 1) leaf_key[S] = k1;
+        }
         double emax = 0.0;
-        // kB keys per lane at a time: the network's weights are read once per
-        // hidden unit for all of them (same operation order as mlp_eval, so the
-        // predictions are bit-identical to the lookups')
+        // Positions [p0, p1) in passes of 64 kB keys (one pass for a 512-key
+        // leaf), then p1 itself from registers.  The network's weights are read
+        // once per hidden unit for kB keys (same operation order as mlp_eval,
+        // so the predictions are bit-identical to the lookups').
         constexpr int kB = 8;
-        for (int64_t base = p0; base <= p1; base += 64 * kB) {
+        const K kbefore = keys[p1 > p0 ? p1 - 1 : p0];
+        for (int64_t base = p0; base < p1; base += 64 * kB) {
             float u[kB], y[kB];
             bool first[kB];
 #pragma unroll
             for (int b = 0; b < kB; ++b) {
                 const int64_t i = base + lane + 64 * b;
-                const bool valid = i <= p1;
+                const bool valid = i < p1;
                 const K k = valid ? keys[i] : (K)0;
                 first[b] = valid && i > p0 && k != keys[i - 1];
                 if (valid) emax = fmax(emax, fabs(plm_pred<K>(k, k0, k1, p0, p1) - (double)i));
@@ -375,6 +380,13 @@ __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ key
                 const int d = sp > (int)s ? sp - (int)s : (int)s - sp;
                 mlp_e = d > mlp_e ? d : mlp_e;
             }
+        }
+        // position p1 (the same in every lane)
+        emax = fmax(emax, fabs(plm_pred<K>(k1, k0, k1, p0, p1) - (double)p1));
+        if (p1 > p0 && k1 != kbefore) {
+            const int sp = leaf_of_pred(mlp_eval(P, key_unit<K>(k1)), S);
+            const int d = sp > (int)s ? sp - (int)s : (int)s - sp;
+            mlp_e = d > mlp_e ? d : mlp_e;
         }
         emax = wave_max(emax);
         if (lane == 0) leaf_err[s] = (int32_t)ceil(emax);
@@ -563,10 +575,14 @@ __global__ __launch_bounds__(256) void cover_lookup_kernel(int64_t nq, const uns
 }
 
 // Per query: total scan length and its number of work items (kItemPos
-// positions each); stat[3] += positions scanned.
+// positions each); stat[3] += positions scanned.  Also the query's cost key
+// for the longest-first order: 255 - 8 log2(1 + items x vertices) (8 bits,
+// ascending = most expensive first) with idx[q] = q.
 __global__ __launch_bounds__(256) void cover_total_kernel(int64_t nq, const int2* __restrict__ ranges,
                                                          const int32_t* __restrict__ nranges,
-                                                         int32_t* __restrict__ n_items,
+                                                         const int4* __restrict__ poly_info,
+                                                         int32_t* __restrict__ n_items, uint32_t* __restrict__ key,
+                                                         int32_t* __restrict__ idx,
                                                          unsigned long long* __restrict__ stat) {
     __shared__ unsigned long long red[4];
     unsigned long long scan[1] = {0};
@@ -576,8 +592,14 @@ __global__ __launch_bounds__(256) void cover_total_kernel(int64_t nq, const int2
             const int2 r = ranges[q * kMaxRanges + k];
             total += r.y - r.x;
         }
-        n_items[q] = (int32_t)((total + kItemPos - 1) / kItemPos);
+        const int32_t items = (int32_t)((total + kItemPos - 1) / kItemPos);
+        n_items[q] = items;
         scan[0] += (unsigned long long)total;
+        const double v = poly_info ? (double)(poly_info[q].y - poly_info[q].x) : 1.0;
+        const double cost = (double)items * (v > 1.0 ? v : 1.0);
+        const int lg = (int)(8.0 * log2(1.0 + cost));
+        key[q] = (uint32_t)(255 - (lg > 255 ? 255 : lg));
+        idx[q] = (int32_t)q;
     }
     block_add_counters<1>(scan, red, stat + 3);
 }
@@ -947,9 +969,18 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
         if (tid == 0) s_nout = 0;
         __syncthreads();
     };
+    // Thread 0 claims kClaim consecutive items per queue atomic (the items are
+    // in descending cost order, so late claims are small): one same-address
+    // atomic per item would serialise in L2.
+    int claim_next = 0, claim_left = 0;
     for (;;) {
         if (tid == 0) {
-            const int item = (int)atomicAdd(&ctl->queue[0], 1u);
+            if (claim_left == 0) {
+                claim_next = (int)atomicAdd(&ctl->queue[0], (unsigned)kClaim);
+                claim_left = kClaim;
+            }
+            const int item = claim_next++;
+            --claim_left;
             s_item = item;
             if ((long long)item < T) {
                 int64_t lo;
@@ -1248,18 +1279,6 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
 }
 
 // LPT key: queries with larger estimated work (items x vertices) sort first.
-__global__ __launch_bounds__(256) void cost_key_kernel(const int32_t* __restrict__ n_items,
-                                                      const int4* __restrict__ poly_info, int64_t nq,
-                                                      uint32_t* __restrict__ key, int32_t* __restrict__ idx) {
-    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= nq) return;
-    const double v = poly_info ? (double)(poly_info[q].y - poly_info[q].x) : 1.0;
-    const double cost = (double)n_items[q] * (v > 1.0 ? v : 1.0);
-    const int lg = (int)(8.0 * log2(1.0 + cost));
-    key[q] = (uint32_t)(255 - (lg > 255 ? 255 : lg));
-    idx[q] = (int32_t)q;
-}
-
 __global__ __launch_bounds__(256) void gather_counts_kernel(const int32_t* __restrict__ n_items,
                                                            const int32_t* __restrict__ order, int64_t nq,
                                                            int32_t* __restrict__ out) {
@@ -1276,11 +1295,6 @@ __global__ __launch_bounds__(256) void item_map_kernel(const int32_t* __restrict
     if (q >= nq) return;
     const int64_t o = item_off[q];
     for (int64_t k = 0; k < n_items[q] && o + k < cap; ++k) item_q[o + k] = (int32_t)q;
-}
-
-__global__ void total_items_kernel(const int32_t* __restrict__ off, int64_t nq, const int32_t* __restrict__ cnt,
-                                   long long* __restrict__ total) {
-    if (threadIdx.x == 0) *total = nq > 0 ? (long long)off[nq - 1] + cnt[nq - 1] : 0;
 }
 
 __global__ void publish_learned_kernel(const JoinCtl* __restrict__ ctl, const unsigned long long* __restrict__ cst,
@@ -1386,20 +1400,19 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
     VS_CHECK_HIP(hipEventRecord(side->fork, s));
     VS_CHECK_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
     const hipStream_t t = side->stream;
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, t, B.meta, 4);
-    VS_LAUNCH_CHECK();
     if (n >= 2) {
         hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(vs::kTrainThreads), 0, t, B.keys, n, iters, lr, B.mlp,
-                           B.losses);
-        VS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(leaf_keys_kernel<K>, dim3((unsigned)ceil_div(S + 1, 256)), dim3(256), 0, t, B.keys, n, S,
-                           B.leaf_key);
+                           B.losses, B.meta);
         VS_LAUNCH_CHECK();
         hipLaunchKernelGGL(leaf_err_kernel<K>, dim3(stride_grid(S, 4)), dim3(256), 0, t, B.keys, n, S, B.leaf_key,
                            B.mlp, B.leaf_err, B.meta);
         VS_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, t, B.meta, 4);
+        VS_LAUNCH_CHECK();
     }
-    VS_TRY(gather_points(px, py, B.perm, n, B.sx, B.sy, s));
+    // four gather blocks per CU leave room for the side stream's kernels
+    VS_TRY(gather_points(px, py, B.perm, n, B.sx, B.sy, s, 4));
     VS_CHECK_HIP(hipEventRecord(side->join, t));
     VS_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
     return VS_OK;
@@ -1423,6 +1436,8 @@ struct QueryBufs {
     int32_t* item_q;
     int64_t item_cap;
     int4* poly_info;
+    uint32_t* cost_in;  // unsorted cost keys and query ids
+    int32_t* order_in;
     uint32_t* cost_key;
     int32_t* order;
     int32_t* n_items_sorted;
@@ -1450,6 +1465,8 @@ QueryBufs carve_query(Carver& c, int64_t nq, int64_t n_left, int64_t capacity) {
     Q.item_cap = item_cap_for(nq, n_left);
     Q.item_q = c.take<int32_t>(Q.item_cap);
     Q.poly_info = c.take<int4>(nq);
+    Q.cost_in = c.take<uint32_t>(nq);
+    Q.order_in = c.take<int32_t>(nq);
     Q.cost_key = c.take<uint32_t>(nq);
     Q.order = c.take<int32_t>(nq);
     Q.n_items_sorted = c.take<int32_t>(nq);
@@ -1470,16 +1487,11 @@ QueryBufs carve_query(Carver& c, int64_t nq, int64_t n_left, int64_t capacity) {
     return Q;
 }
 
-__global__ void zero_u64_kernel(unsigned long long* p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
-}
-
 template <class K>
 int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const LearnedView<K>& V,
                     const IndexBufs<K>& B, const RefineView& R, int64_t n_left, int64_t capacity, bool sort,
                     const QueryBufs& Q, int64_t* out_l, int64_t* out_r, int64_t* d_result, hipStream_t s) {
-    ctl_reset(Q.ctl, s);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, Q.cstat, 4);
+    ctl_reset(Q.ctl, s, Q.cstat, 4);
     VS_LAUNCH_CHECK();
     if (nq > 0) {
         const unsigned gq = (unsigned)ceil_div(nq, 256);
@@ -1491,20 +1503,16 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
                            nq, Q.keyr, Q.nranges, V, reinterpret_cast<int32_t*>(Q.ranges), Q.cstat);
         VS_LAUNCH_CHECK();
         hipLaunchKernelGGL(cover_total_kernel, dim3(stride_grid(nq, 256)), dim3(256), 0, s, nq, Q.ranges, Q.nranges,
-                           Q.n_items, Q.cstat);
+                           pinfo, Q.n_items, Q.cost_in, Q.order_in, Q.cstat);
         VS_LAUNCH_CHECK();
-        // LPT order of the queries, then item offsets in that order
-        hipLaunchKernelGGL(cost_key_kernel, dim3(gq), dim3(256), 0, s, Q.n_items, pinfo, nq, Q.cost_key, Q.order);
-        VS_LAUNCH_CHECK();
-        VS_TRY(sort_pairs_u32(Q.cost_key, Q.order, nq, 0, 8, Q.sort_ws, sort_workspace_bytes_u32(nq), s));
+        // longest-first order of the queries, then item offsets in that order
+        VS_TRY(sort_pairs_into_u32(Q.cost_in, Q.order_in, Q.cost_key, Q.order, nq, 0, 8, Q.sort_ws,
+                                   sort_workspace_bytes_u32(nq), s));
         hipLaunchKernelGGL(gather_counts_kernel, dim3(gq), dim3(256), 0, s, Q.n_items, Q.order, nq, Q.n_items_sorted);
         VS_LAUNCH_CHECK();
-        VS_TRY(exclusive_scan_i32(Q.n_items_sorted, Q.item_off, nq, nullptr, Q.scan_ws, s));
+        VS_TRY(exclusive_scan_i32(Q.n_items_sorted, Q.item_off, nq, Q.total_items, Q.scan_ws, s));
         hipLaunchKernelGGL(item_map_kernel, dim3(gq), dim3(256), 0, s, Q.item_off, Q.n_items_sorted, nq, Q.item_q,
                            Q.item_cap);
-        VS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(total_items_kernel, dim3(1), dim3(64), 0, s, Q.item_off, nq, Q.n_items_sorted,
-                           Q.total_items);
         VS_LAUNCH_CHECK();
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {
@@ -1697,7 +1705,7 @@ int vs_sjoin_points_polygons_learned(const double* px, const double* py, int64_t
     uint8_t* flags = c.take<uint8_t>(n_coords);
     const vs::QueryBufs Q = vs::carve_query(c, n_polys, n_points, capacity);
     hipStream_t s = vs::stream_of(stream);
-    VS_TRY(vs::index_build_t<uint32_t>(px, py, n_points, S, train_iters, 0.01f, B, s));
+    VS_TRY(vs::index_build_t<uint32_t>(px, py, n_points, S, train_iters, vs::kJoinTrainLr, B, s));
     VS_TRY(vs::build_refine_flags(part_offsets, ring_offsets, n_parts, n_rings, n_coords, flags, s));
     const vs::RefineView R = {x, y, geom_offsets, part_offsets, ring_offsets, flags};
     return vs::learned_query_t<uint32_t>(vs::kLModePip, predicate, reinterpret_cast<const double4*>(poly_bounds),

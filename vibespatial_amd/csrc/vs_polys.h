@@ -88,15 +88,27 @@ __device__ inline bool polygons_intersect_ring(const Polys& A, int32_t a0, int32
             const int32_t e0 = B.ring[br], e1 = B.ring[br + 1];
             if (e1 <= e0) continue;
             double px = B.x[e0], py = B.y[e0];
-            for (int32_t c = e0 + 1; c < e1; ++c) {
-                const double qx = B.x[c], qy = B.y[c];
-                if (fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy) {
-                    for (int32_t k = a0 + 1; k < a0 + na; ++k)
-                        if (segments_touch(A.x[k - 1], A.y[k - 1], A.x[k], A.y[k], px, py, qx, qy)) return true;
+            // four coordinates loaded together per step (memory-level parallelism)
+            for (int32_t c0 = e0 + 1; c0 < e1; c0 += 4) {
+                double qxs[4], qys[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int32_t c = c0 + u < e1 ? c0 + u : e1 - 1;
+                    qxs[u] = B.x[c];
+                    qys[u] = B.y[c];
                 }
-                if (edge_step_f64(fx, fy, px, py, qx, qy, parity)) return true;  // on B's boundary
-                px = qx;
-                py = qy;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (c0 + u >= e1) break;
+                    const double qx = qxs[u], qy = qys[u];
+                    if (fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy) {
+                        for (int32_t k = a0 + 1; k < a0 + na; ++k)
+                            if (segments_touch(A.x[k - 1], A.y[k - 1], A.x[k], A.y[k], px, py, qx, qy)) return true;
+                    }
+                    if (edge_step_f64(fx, fy, px, py, qx, qy, parity)) return true;  // on B's boundary
+                    px = qx;
+                    py = qy;
+                }
             }
         }
         if (parity) return true;  // inside this part of B

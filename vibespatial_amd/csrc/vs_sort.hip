@@ -109,6 +109,38 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const int32_t* 
     }
 }
 
+// Downsweep that forms its own tile prefix from the tile sums (no separate
+// pass over them): for at most kFusedScanTiles tiles each block sums the
+// sums of the tiles before it; the last block writes the grand total.
+constexpr int kFusedScanTiles = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void scan_down_fused_kernel(const int32_t* __restrict__ in,
+                                                                      int32_t* __restrict__ out, int64_t n,
+                                                                      const long long* __restrict__ part,
+                                                                      long long* __restrict__ total_out) {
+    __shared__ long long lds[4];
+    long long p = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += kScanThreads) p += part[t];
+    long long prefix;  // the block total of p, the same in every thread
+    block_exclusive_scan_i64(p, lds, prefix);
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    long long sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        sum += v[i];
+    }
+    long long tot;
+    long long run = block_exclusive_scan_i64(sum, lds, tot) + prefix;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = (int32_t)run;
+        run += v[i];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = prefix + tot;
+}
+
 int64_t scan_workspace_bytes(int64_t n) {
     Carver c(nullptr);
     c.take<long long>(ceil_div(n, kScanTile) + 1);
@@ -129,6 +161,12 @@ int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, long long* d_
     }
     hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part);
     VS_LAUNCH_CHECK();
+    if (nb <= kFusedScanTiles) {
+        hipLaunchKernelGGL(scan_down_fused_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n, part,
+                           d_total);
+        VS_LAUNCH_CHECK();
+        return VS_OK;
+    }
     hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(1024), 0, s, part, nb, d_total);
     VS_LAUNCH_CHECK();
     hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n, part);
@@ -316,6 +354,56 @@ int sort_pairs_nocopy(K* keys, int32_t* vals, int64_t n, const unsigned long lon
     return VS_OK;
 }
 
+// Sorts (in_keys, in_vals) into (out_keys, out_vals), stable, by bits
+// [begin_bit, end_bit), with no copy: with an odd pass count the first pass
+// writes the outputs, with an even one the workspace alternates, so the last
+// pass always lands in the outputs.  The inputs are only read; they must not
+// alias the outputs.  n: element capacity; d_n (device, optional): see
+// device_count (an overflowed count sorts nothing).
+template <class K>
+int sort_pairs_into(const K* in_keys, const int32_t* in_vals, K* out_keys, int32_t* out_vals, int64_t n,
+                    const unsigned long long* d_n, int begin_bit, int end_bit, void* ws, int64_t ws_bytes,
+                    hipStream_t s) {
+    VS_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX, "sort_pairs: n=%lld out of range", (long long)n);
+    VS_REQUIRE(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit,
+               "sort_pairs: bad bit range [%d, %d)", begin_bit, end_bit);
+    VS_REQUIRE(ws_bytes >= sort_workspace_bytes_t<K>(n), "sort_pairs: workspace too small");
+    VS_REQUIRE(in_keys != out_keys && in_vals != out_vals, "sort_pairs_into: inputs alias outputs");
+    const int passes = (end_bit - begin_bit + 7) / 8;
+    if (n <= 1 || passes == 0) {  // nothing to order: copy all n slots (a device count only limits the valid ones)
+        if (n > 0) {
+            VS_CHECK_HIP(hipMemcpyAsync(out_keys, in_keys, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
+            VS_CHECK_HIP(hipMemcpyAsync(out_vals, in_vals, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+        }
+        return VS_OK;
+    }
+    const int nb = (int)ceil_div(n, kRsTile);
+    Carver c(ws);
+    K* kalt = c.take<K>(n);
+    int32_t* valt = c.take<int32_t>(n);
+    int32_t* hist = c.take<int32_t>((int64_t)nb * kRsBins);
+    void* scan_ws = c.take<char>(scan_workspace_bytes((int64_t)nb * kRsBins));
+    const K* ki = in_keys;
+    const int32_t* vi = in_vals;
+    K* ko = (passes & 1) ? out_keys : kalt;
+    int32_t* vo = (passes & 1) ? out_vals : valt;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        const int width = end_bit - shift < 8 ? end_bit - shift : 8;
+        const int dmask = (1 << width) - 1;
+        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, n, d_n, shift, dmask, hist, nb);
+        VS_LAUNCH_CHECK();
+        VS_TRY(exclusive_scan_i32(hist, hist, (int64_t)nb * kRsBins, nullptr, scan_ws, s));
+        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n, d_n, shift,
+                           dmask, hist, nb);
+        VS_LAUNCH_CHECK();
+        ki = ko;
+        vi = vo;
+        ko = ko == out_keys ? kalt : out_keys;
+        vo = vo == out_vals ? valt : out_vals;
+    }
+    return VS_OK;
+}
+
 // In-place form: copies the result back when it ends in the alternates.
 template <class K>
 int sort_pairs_t(K* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit, int end_bit,
@@ -336,6 +424,14 @@ int sort_pairs_u64(uint64_t* k, int32_t* v, int64_t n, int b0, int b1, void* ws,
 }
 int sort_pairs_u32(uint32_t* k, int32_t* v, int64_t n, int b0, int b1, void* ws, int64_t wb, hipStream_t s) {
     return sort_pairs_t<uint32_t>(k, v, n, nullptr, b0, b1, ws, wb, s);
+}
+int sort_pairs_into_u32(const uint32_t* ik, const int32_t* iv, uint32_t* ok, int32_t* ov, int64_t n, int b0, int b1,
+                        void* ws, int64_t wb, hipStream_t s, const unsigned long long* d_n) {
+    return sort_pairs_into<uint32_t>(ik, iv, ok, ov, n, d_n, b0, b1, ws, wb, s);
+}
+int sort_pairs_into_u64(const uint64_t* ik, const int32_t* iv, uint64_t* ok, int32_t* ov, int64_t n, int b0, int b1,
+                        void* ws, int64_t wb, hipStream_t s, const unsigned long long* d_n) {
+    return sort_pairs_into<uint64_t>(ik, iv, ok, ov, n, d_n, b0, b1, ws, wb, s);
 }
 int sort_pairs_u32_dev(uint32_t* k, int32_t* v, int64_t n, const unsigned long long* d_n, int b0, int b1, void* ws,
                        int64_t wb, hipStream_t s, SortResult<uint32_t>* res) {

@@ -229,7 +229,7 @@ LOOKUP_MODES = {"learned": 0, "leaves": 1, "binary": 2}
 
 
 def sjoin_points_polygons_learned(Q: DevicePoints, P: DevicePolygons, bounds: torch.Tensor, predicate: str = "within",
-                                  capacity: int | None = None, train_iters: int = 8, lookup: str = "learned"):
+                                  capacity: int | None = None, train_iters: int = 16, lookup: str = "learned"):
     """Learned + FP32 form of sjoin_points_polygons: same pairs, bit for bit."""
     dev = require_gpu()
     if predicate not in ("within", "intersects", "covered_by", "touches"):
