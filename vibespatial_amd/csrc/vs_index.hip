@@ -24,36 +24,95 @@ int sort_pairs_u32(uint32_t*, int32_t*, int64_t, int, int, void*, int64_t, hipSt
 int64_t sort_workspace_bytes_u64(int64_t n);
 int64_t sort_workspace_bytes_u32(int64_t n);
 
-// Sixteen lanes per polygonal row, striding over its coordinates (rows are
-// contiguous, so a wave's four rows read one span of memory), reduced with
-// 16-wide shuffles.
+// One wave per 16 consecutive polygonal rows (rows are contiguous, so a wave
+// reads one span of memory).  Lanes 0..15 load the rows' coordinate ranges;
+// then, by the longest of the 16:
+//   <= 16 coordinates: four lanes per row, each loading up to four
+//     coordinates at once, reduced with 4-wide shuffles;
+//   longer: sixteen lanes per row, four rows at a time, 64 coordinates per
+//     pass (four loads in flight per lane), reduced with 16-wide shuffles.
+// Every load of a pass is issued before the first use.
+constexpr int kBoundsRows = 16;
 __global__ __launch_bounds__(256) void polygon_bounds_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const int32_t* __restrict__ geom,
     const int32_t* __restrict__ part, const int32_t* __restrict__ ring, int64_t n_rows,
     double4* __restrict__ out) {
-    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    const int sub = threadIdx.x & 15;
-    const bool inr = row < n_rows;
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kBoundsRows;
+    if (r0 >= n_rows) return;  // wave-uniform
     int32_t c0 = 0, c1 = 0;
-    if (inr) {
-        c0 = ring[part[geom[row]]];
-        c1 = ring[part[geom[row + 1]]];
+    if (lane < kBoundsRows && r0 + lane < n_rows) {
+        c0 = ring[part[geom[r0 + lane]]];
+        c1 = ring[part[geom[r0 + lane + 1]]];
     }
-    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
-    for (int32_t c = c0 + sub; c < c1; c += 16) {
-        const double xv = x[c], yv = y[c];
-        x0 = fmin(x0, xv); y0 = fmin(y0, yv);
-        x1 = fmax(x1, xv); y1 = fmax(y1, yv);
+    int longest = c1 - c0;
+    for (int o = 32; o > 0; o >>= 1) longest = max(longest, __shfl_xor(longest, o, 64));
+    if (longest <= 16) {
+        const int j = lane >> 2, sub = lane & 3;
+        const int32_t a0 = __shfl(c0, j, 64), a1 = __shfl(c1, j, 64);
+        double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+        double xv[4], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int32_t c = a0 + sub + 4 * u;
+            const bool ok = c < a1;
+            xv[u] = ok ? x[c] : INFINITY;
+            yv[u] = ok ? y[c] : INFINITY;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (a0 + sub + 4 * u < a1) {
+                x0 = fmin(x0, xv[u]); y0 = fmin(y0, yv[u]);
+                x1 = fmax(x1, xv[u]); y1 = fmax(y1, yv[u]);
+            }
+        }
+        for (int o = 2; o > 0; o >>= 1) {
+            x0 = fmin(x0, __shfl_xor(x0, o, 4));
+            y0 = fmin(y0, __shfl_xor(y0, o, 4));
+            x1 = fmax(x1, __shfl_xor(x1, o, 4));
+            y1 = fmax(y1, __shfl_xor(y1, o, 4));
+        }
+        const int64_t row = r0 + j;
+        if (sub == 0 && row < n_rows) {
+            if (a1 <= a0 || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
+            out[row] = make_double4(x0, y0, x1, y1);
+        }
+        return;
     }
-    for (int o = 8; o > 0; o >>= 1) {
-        x0 = fmin(x0, __shfl_xor(x0, o, 16));
-        y0 = fmin(y0, __shfl_xor(y0, o, 16));
-        x1 = fmax(x1, __shfl_xor(x1, o, 16));
-        y1 = fmax(y1, __shfl_xor(y1, o, 16));
-    }
-    if (inr && sub == 0) {
-        if (c1 <= c0 || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
-        out[row] = make_double4(x0, y0, x1, y1);
+    // sixteen lanes per row, four rows at a time, 64 coordinates per pass
+    const int g = lane >> 4, sub = lane & 15;
+    for (int j0 = 0; j0 < kBoundsRows; j0 += 4) {
+        const int j = j0 + g;
+        const int64_t row = r0 + j;
+        const int32_t a0 = __shfl(c0, j, 64), a1 = __shfl(c1, j, 64);
+        double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+        for (int32_t cb = a0; cb < a1; cb += 64) {
+            double xv[4], yv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int32_t c = cb + sub + 16 * u;
+                const bool ok = c < a1;
+                xv[u] = ok ? x[c] : INFINITY;
+                yv[u] = ok ? y[c] : INFINITY;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (cb + sub + 16 * u < a1) {
+                    x0 = fmin(x0, xv[u]); y0 = fmin(y0, yv[u]);
+                    x1 = fmax(x1, xv[u]); y1 = fmax(y1, yv[u]);
+                }
+            }
+        }
+        for (int o = 8; o > 0; o >>= 1) {
+            x0 = fmin(x0, __shfl_xor(x0, o, 16));
+            y0 = fmin(y0, __shfl_xor(y0, o, 16));
+            x1 = fmax(x1, __shfl_xor(x1, o, 16));
+            y1 = fmax(y1, __shfl_xor(y1, o, 16));
+        }
+        if (sub == 0 && row < n_rows) {
+            if (a1 <= a0 || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
+            out[row] = make_double4(x0, y0, x1, y1);
+        }
     }
 }
 
@@ -247,7 +306,7 @@ __global__ __launch_bounds__(256) void point_boxes_kernel(const double* __restri
 int polygon_bounds(const double* x, const double* y, const int32_t* geom, const int32_t* part,
                    const int32_t* ring, int64_t n, double* bounds, hipStream_t s) {
     if (n == 0) return VS_OK;
-    hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 16)), dim3(256), 0, s, x, y,
+    hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 4 * kBoundsRows)), dim3(256), 0, s, x, y,
                        geom, part, ring, n, reinterpret_cast<double4*>(bounds));
     VS_LAUNCH_CHECK();
     return VS_OK;
