@@ -208,12 +208,13 @@ def main(argv=None):
 
 
 # HBM bytes per learned_join_kernel launch on the default workload, from two
-# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) scaled by
-# what the same counters read for a 1 GiB device-to-device copy in the same run
-# (FETCH_SIZE 524,317 KiB -> x2.0, WRITE_SIZE 1,048,576 KiB -> x1.0):
-#   (2.0 * 410,552 + 1.0 * 252,163) KiB = 1,099,025,408 bytes.
+# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch, mean of the
+# warm-up and the timed dispatch) scaled by what the same counters read for a
+# 1 GiB device-to-device copy in the same run (FETCH_SIZE 524,318 KiB -> x2.0,
+# WRITE_SIZE 1,048,576 KiB -> x1.0):
+#   (2.0 * 415,112.5 + 1.0 * 294,977.65) KiB = 1,152,207,514 bytes.
 # The summary with both commands is profiles/learned_join_kernel_10Mpts_100kpolys_pmc_traffic.md.
-_TRAFFIC_BYTES = {(10_000_000, 100_000): 1_099_025_408}
+_TRAFFIC_BYTES = {(10_000_000, 100_000): 1_152_207_514}
 
 
 def _measured_traffic(n_points, n_polys):
