@@ -183,6 +183,32 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
                     int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
                     int64_t workspace_bytes, void* stream);
 
+/* out[i] = boxes[i] grown by distance[i] (per row, device fp64) on every
+ * side, each bound rounded one ulp outward so every point within the distance
+ * of the box lies in the grown box; NaN or negative distance -> NaN box.
+ * The distance-expanded candidate filter of the reference's dwithin query
+ * (spatial/spatial_index_device.py:398-430). */
+int vs_expand_boxes(const double* boxes, int64_t n, const double* distance, double* out, void* stream);
+
+/* Exact dwithin refine of candidate pairs (left[i], right[i]) with
+ * d = distance[left[i]] (device fp64 per left row), then an order-preserving
+ * compaction into out_left/out_right; d_result[0] = pairs kept.  Workspace:
+ * vs_refine_pairs_workspace_bytes(n_pairs).  Kinds: 0 = point rows (x[row],
+ * y[row]; offsets unused), 1 = polygonal rows (GeoArrow offsets).  A pair is
+ * kept when the exact distance between the geometries is <= d: points by
+ * d^2 - |p - q|^2 >= 0; a point and a polygon when the point is not outside
+ * it or within d of a boundary segment; two polygons when they intersect or
+ * a vertex of one is within d of a boundary segment of the other.  NaN
+ * coordinates, empty rows and NaN / negative d never match.  Replaces
+ * _dwithin_refine_gpu (spatial/nearest.py:2855-3016), which compares rounded
+ * fp64 distances. */
+int vs_refine_dwithin(int32_t left_kind, int32_t right_kind, const int64_t* left, const int64_t* right,
+                      int64_t n_pairs, const double* distance, const double* lx, const double* ly,
+                      const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring, const double* rx,
+                      const double* ry, const int32_t* r_geom, const int32_t* r_part, const int32_t* r_ring,
+                      int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+
 int64_t vs_lexsort_pairs_workspace_bytes(int64_t n_pairs);
 
 /* In-place lexsort of int64 pairs by (left, right); values must be in
@@ -343,6 +369,14 @@ int vs_host_point_location(double px, double py, const double* x, const double* 
 /* Exact closed-segment touch-or-cross (predicates/polygon_kernels.py:131-161). */
 int vs_host_segments_touch(double ax, double ay, double bx, double by, double cx, double cy,
                            double dx, double dy);
+
+/* Exact |p - q| <= d (1/0); NaN coordinates or a NaN / negative d give 0,
+ * d = +inf gives 1.  The device dwithin decision for point pairs. */
+int vs_host_point_point_within(double px, double py, double qx, double qy, double d);
+
+/* Exact: the distance from p to the closed segment a-b is <= d (1/0); same
+ * NaN / infinity rules.  The device decision for a point and one edge. */
+int vs_host_point_segment_within(double px, double py, double ax, double ay, double bx, double by, double d);
 
 /* 64-bit Hilbert index of 32-bit grid coordinates. */
 uint64_t vs_host_hilbert64(uint32_t x, uint32_t y);

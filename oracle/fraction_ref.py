@@ -9,6 +9,10 @@ Semantics followed (reference file:line):
   ring classify      src/vibespatial/cuda/device_functions/point_in_ring.py:148-188
   polygon location   src/vibespatial/predicates/point_relations_kernels.py:37-95
   segment touch      src/vibespatial/predicates/polygon_kernels.py:131-161
+  point-segment dist src/vibespatial/spatial/point_distance_kernels.py:17-39
+                     (closest point by the clamped projection parameter);
+                     dwithin = distance <= d (spatial/nearest.py:2855-3016),
+                     decided here on exact squared distances
 """
 
 from __future__ import annotations
@@ -20,6 +24,9 @@ __all__ = [
     "point_on_segment",
     "location_by_geometry",
     "segments_touch",
+    "dist2_point_point",
+    "dist2_point_segment",
+    "within_distance",
 ]
 
 
@@ -110,3 +117,31 @@ def segments_touch(a, b, c, d) -> bool:
         return max(min(p0, p1), min(q0, q1)) <= min(max(p0, p1), max(q0, q1))
 
     return overlap(ax, bx, cx, dx) and overlap(ay, by, cy, dy)
+
+
+def dist2_point_point(px, py, qx, qy) -> Fraction:
+    """Exact squared distance between two points."""
+    F = Fraction
+    dx, dy = F(px) - F(qx), F(py) - F(qy)
+    return dx * dx + dy * dy
+
+
+def dist2_point_segment(px, py, ax, ay, bx, by) -> Fraction:
+    """Exact squared distance from p to the closed segment a-b: the projection
+    parameter t = (p - a).(b - a) / |b - a|^2 clamped to [0, 1]."""
+    F = Fraction
+    ux, uy = F(bx) - F(ax), F(by) - F(ay)
+    wx, wy = F(px) - F(ax), F(py) - F(ay)
+    l2 = ux * ux + uy * uy
+    t = F(0) if l2 == 0 else min(max((ux * wx + uy * wy) / l2, F(0)), F(1))
+    ex, ey = wx - t * ux, wy - t * uy
+    return ex * ex + ey * ey
+
+
+def within_distance(d2: Fraction, d: float) -> bool:
+    """d2 <= d^2 for d >= 0 (NaN / negative d never matches)."""
+    if not (d >= 0.0):
+        return False
+    if d == float("inf"):
+        return True
+    return d2 <= Fraction(d) * Fraction(d)

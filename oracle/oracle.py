@@ -28,6 +28,8 @@ __all__ = [
     "sjoin_points_polygons",
     "sjoin_polygons_polygons",
     "box_query_points",
+    "dwithin_pair",
+    "dwithin_pairs",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -209,3 +211,73 @@ def box_query_points(px, py, boxes, nthreads: int = 0):
     with _lock:
         n = lib().or_box_query_points(_ptr(px), _ptr(py), len(px), _ptr(b), len(b), nthreads)
         return _take(n)
+
+
+def _rings_of(polys, row: int):
+    g, pt, r = polys.geom_offsets, polys.part_offsets, polys.ring_offsets
+    for ring in range(pt[g[row]], pt[g[row + 1]]):
+        yield int(r[ring]), int(r[ring + 1])
+
+
+def dwithin_pair(left_kind: int, right_kind: int, L, R, li: int, ri: int, d: float) -> bool:
+    """Exact dwithin of one pair (kinds: 0 points (x, y arrays), 1 polygons):
+    the exact distance between the geometries is <= d.  A point and a polygon:
+    location != 0 (C restatement) or a boundary segment within d; two
+    polygons: intersects (C restatement) or a vertex of one within d of a
+    boundary segment of the other.  Rationals; small inputs only."""
+    from . import fraction_ref as FR
+
+    if not (d >= 0.0):
+        return False
+    if left_kind == 1 and right_kind == 0:
+        return dwithin_pair(0, 1, R, L, ri, li, d)
+    if left_kind == 0:
+        px, py = float(L.x[li]), float(L.y[li])
+        if px != px or py != py:
+            return False
+        if right_kind == 0:
+            qx, qy = float(R.x[ri]), float(R.y[ri])
+            if qx != qx or qy != qy:
+                return False
+            return FR.within_distance(FR.dist2_point_point(px, py, qx, qy), d)
+        rings = list(_rings_of(R, ri))
+        if not any(e > b for b, e in rings):
+            return False
+        if d == float("inf") or point_location(px, py, R, ri) != 0:
+            return True
+        for b, e in rings:
+            for c in range(b + 1, e):
+                if FR.within_distance(FR.dist2_point_segment(px, py, R.x[c - 1], R.y[c - 1], R.x[c], R.y[c]), d):
+                    return True
+        return False
+    ra, rb = list(_rings_of(L, li)), list(_rings_of(R, ri))
+    if not any(e > b for b, e in ra) or not any(e > b for b, e in rb):
+        return False
+    if d == float("inf") or polygon_intersects(L, li, R, ri):
+        return True
+    for P, rp, Q, rq in ((L, ra, R, rb), (R, rb, L, ra)):
+        for b0, e0 in rp:
+            for v in range(b0, e0):
+                for b, e in rq:
+                    for c in range(b + 1, e):
+                        if FR.within_distance(FR.dist2_point_segment(P.x[v], P.y[v], Q.x[c - 1], Q.y[c - 1],
+                                                                     Q.x[c], Q.y[c]), d):
+                            return True
+    return False
+
+
+def dwithin_pairs(left_kind: int, right_kind: int, L, R, distance, left_bounds, right_bounds):
+    """Lexsorted (left, right) pairs within each left row's distance: boxes
+    grown by the distance (bbox brute force, inclusive), then dwithin_pair."""
+    qb = _f64(left_bounds).reshape(-1, 4).copy()
+    d = np.asarray(distance, dtype=np.float64)
+    with np.errstate(invalid="ignore"):
+        qb[:, 0] -= d
+        qb[:, 1] -= d
+        qb[:, 2] += d
+        qb[:, 3] += d
+    qb[~(d >= 0)] = np.nan
+    lq, lt = bbox_pairs(np.nextafter(qb, np.array([-np.inf, -np.inf, np.inf, np.inf])), right_bounds)
+    keep = np.array([dwithin_pair(left_kind, right_kind, L, R, int(a), int(b), float(d[a]))
+                     for a, b in zip(lq, lt)], dtype=bool)
+    return lq[keep], lt[keep]

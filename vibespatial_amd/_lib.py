@@ -75,6 +75,11 @@ SIGNATURES = {
         ctypes.c_int,
         [_i32, _i32, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p],
     ),
+    "vs_expand_boxes": (ctypes.c_int, [_p, _i64, _p, _p, _p]),
+    "vs_refine_dwithin": (
+        ctypes.c_int,
+        [_i32, _i32, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p],
+    ),
     "vs_lexsort_pairs_workspace_bytes": (_i64, [_i64]),
     "vs_lexsort_pairs": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p]),
     "vs_learned_leaf_count": (_i32, [_i64, _i32]),
@@ -107,6 +112,8 @@ SIGNATURES = {
     "vs_host_orient2d": (ctypes.c_int, [_d] * 6),
     "vs_host_point_location": (ctypes.c_int, [_d, _d, _p, _p, _p, _p, _p, _i64]),
     "vs_host_segments_touch": (ctypes.c_int, [_d] * 8),
+    "vs_host_point_point_within": (ctypes.c_int, [_d] * 5),
+    "vs_host_point_segment_within": (ctypes.c_int, [_d] * 7),
     "vs_host_hilbert64": (_u64, [_u32, _u32]),
 }
 

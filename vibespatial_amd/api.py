@@ -18,10 +18,12 @@ mixed point/polygon columns and assembles frames):
   point  x point    intersects / within / contains / covers / covered_by /
                     contains_properly -> bbox query (boxes are the points)
   predicate None    -> bbox query
+  dwithin (every family pair) -> query with boxes grown by each query row's
+                    distance, exact distance refine (vs_refine_dwithin)
 Combinations whose answer is empty by definition (a point cannot contain a
 polygon, points cannot cross, ...) return no pairs, matching the reference's
 mapping (predicates/point_relations.py:100-146).  Polygon x polygon predicates
-other than intersects (full DE-9IM) and ``dwithin`` raise NotImplementedError.
+other than intersects and dwithin (full DE-9IM) raise NotImplementedError.
 """
 
 from __future__ import annotations
@@ -133,14 +135,24 @@ class _FamilyIndex:
         return t
 
 
-def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, predicate):
-    """Pairs (query family idx, tree family idx), lexsorted, device int64."""
+def _dev_geom(fam: str, arr):
+    return device_points(arr) if fam == "point" else device_polygons(arr)
+
+
+def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, predicate, dist=None):
+    """Pairs (query family idx, tree family idx), lexsorted, device int64.
+    dist: per query-family row distances (fp64) for dwithin."""
     dev = require_gpu()
     if predicate in _EMPTY_PAIRS.get((q_fam, t_fam), ()):
         return _empty_pairs(dev)
-    if predicate == "dwithin":
-        raise NotImplementedError("predicate='dwithin' is not implemented on this path")
     tree = tindex.tree(t_fam)
+    if predicate == "dwithin":
+        # boxes grown by each row's distance (outward-rounded), then the exact
+        # distance refine (spatial/query.py:663-826 route)
+        d = torch.as_tensor(np.ascontiguousarray(dist, dtype=np.float64), device=dev)
+        qb = J.point_boxes(device_points(qarr)) if q_fam == "point" else J.polygon_bounds(device_polygons(qarr))
+        l, r, _ = J.query_boxes(J.expand_boxes(qb, d), tree)
+        return J.refine_dwithin(l, r, d, _dev_geom(q_fam, qarr), _dev_geom(t_fam, tarr))
     if q_fam == "point" and t_fam == "polygon" and predicate is not None:
         l, r, _ = J.sjoin_points_polygons(device_points(qarr), device_polygons(tarr), tree, predicate)
         return l, r
@@ -160,20 +172,31 @@ def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, pred
     raise NotImplementedError(f"{q_fam} x {t_fam} predicate {predicate!r}")
 
 
-def query_pairs(query: GeometryArray, tree_ga: GeometryArray, tindex: _FamilyIndex, predicate):
-    """Lexsorted (query_row, tree_row) pairs as device int64 tensors."""
+def _map_rows(idx: torch.Tensor, rows: np.ndarray) -> torch.Tensor:
+    """Family index -> column row (rows ascending, so order is kept)."""
+    if len(rows) == 0 or (rows[0] == 0 and rows[-1] == len(rows) - 1):
+        return idx  # identity
+    return torch.as_tensor(rows, device=idx.device)[idx]
+
+
+def query_pairs(query: GeometryArray, tree_ga: GeometryArray, tindex: _FamilyIndex, predicate, distance=None):
+    """Lexsorted (query_row, tree_row) pairs as device int64 tensors.
+    distance: per query row fp64 array (dwithin only)."""
     dev = require_gpu()
     fams_q = [(f, a, rows) for f, a, rows in (("point", query.points, query.point_rows),
                                                ("polygon", query.polygons, query.polygon_rows)) if a is not None]
     fams_t = [(f, a, rows) for f, a, rows in (("point", tree_ga.points, tree_ga.point_rows),
                                                ("polygon", tree_ga.polygons, tree_ga.polygon_rows)) if a is not None]
     if len(fams_q) == 1 and len(fams_t) == 1:
-        (qf, qa, _), (tf, ta, _) = fams_q[0], fams_t[0]
-        return _pairs_family(qf, qa, tf, ta, tindex, predicate)
+        (qf, qa, qrows), (tf, ta, trows) = fams_q[0], fams_t[0]
+        dist = None if distance is None else distance[qrows]
+        l, r = _pairs_family(qf, qa, tf, ta, tindex, predicate, dist)
+        return _map_rows(l, qrows), _map_rows(r, trows)
     parts_l, parts_r = [], []
     for qf, qa, qrows in fams_q:
         for tf, ta, trows in fams_t:
-            l, r = _pairs_family(qf, qa, tf, ta, tindex, predicate)
+            dist = None if distance is None else distance[qrows]
+            l, r = _pairs_family(qf, qa, tf, ta, tindex, predicate, dist)
             parts_l.append(qrows[l.cpu().numpy()])
             parts_r.append(trows[r.cpu().numpy()])
     if not parts_l:
@@ -237,7 +260,13 @@ class SpatialIndex:
                 q.points is None and q.polygons is None):
             l = r = torch.empty(0, dtype=torch.int64, device=require_gpu())
         else:
-            l, r = query_pairs(q, self._ga, self._index, predicate)
+            dist = None
+            if predicate == "dwithin":
+                try:
+                    dist = np.ascontiguousarray(np.broadcast_to(np.asarray(distance, dtype=np.float64), (len(q),)))
+                except ValueError as exc:
+                    raise ValueError("distance array must be broadcastable to the geometry input") from exc
+            l, r = query_pairs(q, self._ga, self._index, predicate, dist)
         if return_device:
             return r if scalar else torch.stack([l, r])
         lh, rh = l.cpu().numpy(), r.cpu().numpy()

@@ -1,4 +1,5 @@
 // Library-wide C-ABI pieces: version, thread-local errors, host-side hooks.
+#include "vs_distance.h"
 #include "vs_exact.h"
 #include "vs_internal.h"
 
@@ -40,6 +41,14 @@ int vs_host_point_location(double px, double py, const double* x, const double* 
                            const int32_t* ring_offsets, int64_t row) {
     if (!(px == px && py == py)) return 0;
     return vs::point_location_f64(px, py, x, y, geom_offsets, part_offsets, ring_offsets, row);
+}
+
+int vs_host_point_point_within(double px, double py, double qx, double qy, double d) {
+    return vs::point_point_within(px, py, qx, qy, d) ? 1 : 0;
+}
+
+int vs_host_point_segment_within(double px, double py, double ax, double ay, double bx, double by, double d) {
+    return vs::point_segment_within(px, py, ax, ay, bx, by, d) ? 1 : 0;
 }
 
 int vs_host_segments_touch(double ax, double ay, double bx, double by, double cx, double cy,

@@ -7,9 +7,12 @@
 //     (src/vibespatial/predicates/polygon_kernels.py:131-228);
 //   * point x polygon location for swapped predicates (contains / covers of a
 //     polygon over tree points), predicate mapping of
-//     src/vibespatial/predicates/point_relations.py:100-129.
+//     src/vibespatial/predicates/point_relations.py:100-129;
+//   * dwithin after a query with distance-expanded boxes, exact distance
+//     decisions (replaces _dwithin_refine_gpu, spatial/nearest.py:2855-3016).
 // Compaction keeps the input order, so lexsorted candidates give lexsorted
 // output (replaces d_left[keep], spatial/query_utils.py:867-868).
+#include "vs_distance.h"
 #include "vs_exact.h"
 #include "vs_internal.h"
 #include "vs_polys.h"
@@ -71,6 +74,43 @@ __global__ __launch_bounds__(256) void compact_pairs_kernel(const int64_t* __res
 
 __global__ void total_to_result_kernel(const long long* __restrict__ total, int64_t* __restrict__ result) {
     if (threadIdx.x == 0) result[0] = *total;
+}
+
+// Exact dwithin per pair; kinds: 0 point rows (x[row], y[row]), 1 polygonal
+// rows.  d = distance[left row].
+__global__ __launch_bounds__(256) void dwithin_pairs_kernel(int left_kind, int right_kind,
+                                                           const int64_t* __restrict__ left,
+                                                           const int64_t* __restrict__ right, int64_t n,
+                                                           const double* __restrict__ distance, Polys L, Polys R,
+                                                           int32_t* __restrict__ keep) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t l = left[i], r = right[i];
+        const double d = distance[l];
+        bool k;
+        if (left_kind == 0 && right_kind == 0) k = point_point_within(L.x[l], L.y[l], R.x[r], R.y[r], d);
+        else if (left_kind == 0) k = point_polygon_within(L.x[l], L.y[l], R, r, d);
+        else if (right_kind == 0) k = point_polygon_within(R.x[r], R.y[r], L, l, d);
+        else k = polygons_within(L, l, R, r, d);
+        keep[i] = k ? 1 : 0;
+    }
+}
+
+// Boxes grown by distance[i] on every side, each bound rounded one ulp
+// outward so the grown box holds every point within the distance; a NaN or
+// negative distance gives a NaN box (matches nothing).
+__global__ __launch_bounds__(256) void expand_boxes_kernel(const double4* __restrict__ b, int64_t n,
+                                                          const double* __restrict__ distance,
+                                                          double4* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double d = distance[i];
+    const double4 v = b[i];
+    if (!(d >= 0.0)) {
+        out[i] = make_double4(NAN, NAN, NAN, NAN);
+        return;
+    }
+    out[i] = make_double4(nextafter(v.x - d, -INFINITY), nextafter(v.y - d, -INFINITY),
+                          nextafter(v.z + d, INFINITY), nextafter(v.w + d, INFINITY));
 }
 
 int64_t refine_workspace_bytes(int64_t n) {
@@ -138,6 +178,56 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
         const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
         hipLaunchKernelGGL(vs::refine_pairs_kernel, dim3(grid), dim3(256), 0, s, mode, predicate, left, right,
                            n_pairs, px, py, A, B, keep);
+        VS_LAUNCH_CHECK();
+    }
+    VS_TRY(vs::exclusive_scan_i32(keep, pos, n_pairs, total, scan_ws, s));
+    if (n_pairs > 0) {
+        hipLaunchKernelGGL(vs::compact_pairs_kernel, dim3((unsigned)vs::ceil_div(n_pairs, 256)), dim3(256), 0, s,
+                           left, right, n_pairs, keep, pos, out_left, out_right);
+        VS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(vs::total_to_result_kernel, dim3(1), dim3(64), 0, s, total, d_result);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int vs_expand_boxes(const double* boxes, int64_t n, const double* distance, double* out, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n >= 0, "vs_expand_boxes: n < 0");
+    if (n == 0) return VS_OK;
+    VS_REQUIRE(boxes && distance && out, "vs_expand_boxes: null pointer");
+    hipStream_t s = vs::stream_of(stream);
+    hipLaunchKernelGGL(vs::expand_boxes_kernel, dim3((unsigned)vs::ceil_div(n, 256)), dim3(256), 0, s,
+                       reinterpret_cast<const double4*>(boxes), n, distance, reinterpret_cast<double4*>(out));
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int vs_refine_dwithin(int32_t left_kind, int32_t right_kind, const int64_t* left, const int64_t* right,
+                      int64_t n_pairs, const double* distance, const double* lx, const double* ly,
+                      const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring, const double* rx,
+                      const double* ry, const int32_t* r_geom, const int32_t* r_part, const int32_t* r_ring,
+                      int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                      int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE((left_kind == 0 || left_kind == 1) && (right_kind == 0 || right_kind == 1),
+               "vs_refine_dwithin: kinds must be 0 (points) or 1 (polygons)");
+    VS_REQUIRE(n_pairs >= 0 && n_pairs < (int64_t)INT32_MAX, "vs_refine_dwithin: n_pairs out of range");
+    VS_REQUIRE(workspace_bytes >= vs::refine_workspace_bytes(n_pairs), "vs_refine_dwithin: workspace too small");
+    VS_REQUIRE(d_result != nullptr, "vs_refine_dwithin: null result");
+    VS_REQUIRE(n_pairs == 0 || distance != nullptr, "vs_refine_dwithin: null distance");
+    hipStream_t s = vs::stream_of(stream);
+    vs::Carver c(workspace);
+    int32_t* keep = c.take<int32_t>(n_pairs);
+    int32_t* pos = c.take<int32_t>(n_pairs);
+    long long* total = c.take<long long>(1);
+    void* scan_ws = c.take<char>(vs::scan_workspace_bytes(n_pairs));
+    const vs::Polys L = {lx, ly, l_geom, l_part, l_ring};
+    const vs::Polys R = {rx, ry, r_geom, r_part, r_ring};
+    if (n_pairs > 0) {
+        const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
+        hipLaunchKernelGGL(vs::dwithin_pairs_kernel, dim3(grid), dim3(256), 0, s, left_kind, right_kind, left, right,
+                           n_pairs, distance, L, R, keep);
         VS_LAUNCH_CHECK();
     }
     VS_TRY(vs::exclusive_scan_i32(keep, pos, n_pairs, total, scan_ws, s));

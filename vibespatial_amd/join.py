@@ -97,6 +97,49 @@ def build_tree(bounds: torch.Tensor, fanout: int = FANOUT) -> DeviceTree:
     return DeviceTree(layout, node_boxes, item_ids, item_keys, bounds)
 
 
+def expand_boxes(boxes: torch.Tensor, distance: torch.Tensor) -> torch.Tensor:
+    """(n, 4) device boxes grown by a per-row fp64 distance on every side, each
+    bound rounded one ulp outward (vs_expand_boxes); NaN / negative distance
+    gives a NaN box."""
+    dev = require_gpu()
+    n = int(boxes.shape[0])
+    d = distance.to(device=dev, dtype=torch.float64).contiguous()
+    if d.numel() != n:
+        raise ValueError("one distance per box")
+    out = torch.empty_like(boxes)
+    check(lib().vs_expand_boxes(ptr(boxes), n, ptr(d), ptr(out), stream()), "vs_expand_boxes")
+    return out
+
+
+def refine_dwithin(left: torch.Tensor, right: torch.Tensor, distance: torch.Tensor, lgeom, rgeom):
+    """Exact dwithin of candidate pairs (vs_refine_dwithin); lgeom / rgeom are
+    DevicePoints or DevicePolygons; distance is per left row.  Keeps order."""
+    dev = require_gpu()
+    L = lib()
+    n = int(left.numel())
+    if n == 0:
+        e = torch.empty(0, dtype=torch.int64, device=dev)
+        return e, e.clone()
+
+    def kind_args(g):
+        if isinstance(g, DevicePoints):
+            return 0, [ptr(g.x), ptr(g.y), ptr(None), ptr(None), ptr(None)]
+        return 1, g.args()
+
+    lk, la = kind_args(lgeom)
+    rk, ra = kind_args(rgeom)
+    d = distance.to(device=dev, dtype=torch.float64).contiguous()
+    out_l = torch.empty(n, dtype=torch.int64, device=dev)
+    out_r = torch.empty(n, dtype=torch.int64, device=dev)
+    res = torch.zeros(4, dtype=torch.int64, device=dev)
+    ws = _ws(L.vs_refine_pairs_workspace_bytes(n), dev)
+    check(L.vs_refine_dwithin(lk, rk, ptr(left.contiguous()), ptr(right.contiguous()), n, ptr(d), *la, *ra,
+                              ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
+          "vs_refine_dwithin")
+    k = int(res[0].item())
+    return out_l[:k], out_r[:k]
+
+
 def _read_result(res: torch.Tensor):
     h = res.cpu().tolist()  # synchronises the current stream
     return int(h[0]), int(h[1]), int(h[2]), int(h[3])
