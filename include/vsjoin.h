@@ -190,10 +190,14 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
  * (spatial/spatial_index_device.py:398-430). */
 int vs_expand_boxes(const double* boxes, int64_t n, const double* distance, double* out, void* stream);
 
+int64_t vs_refine_dwithin_workspace_bytes(int64_t n_pairs);
+
 /* Exact dwithin refine of candidate pairs (left[i], right[i]) with
  * d = distance[left[i]] (device fp64 per left row), then an order-preserving
- * compaction into out_left/out_right; d_result[0] = pairs kept.  Workspace:
- * vs_refine_pairs_workspace_bytes(n_pairs).  Kinds: 0 = point rows (x[row],
+ * compaction into out_left/out_right; d_result[0] = pairs kept.  n_right:
+ * right row count (right ids < n_right); pairs are visited grouped by right
+ * row.  Workspace: vs_refine_dwithin_workspace_bytes(n_pairs).  Kinds: 0 =
+ * point rows (x[row],
  * y[row]; offsets unused), 1 = polygonal rows (GeoArrow offsets).  A pair is
  * kept when the exact distance between the geometries is <= d: points by
  * d^2 - |p - q|^2 >= 0; a point and a polygon when the point is not outside
@@ -203,7 +207,7 @@ int vs_expand_boxes(const double* boxes, int64_t n, const double* distance, doub
  * _dwithin_refine_gpu (spatial/nearest.py:2855-3016), which compares rounded
  * fp64 distances. */
 int vs_refine_dwithin(int32_t left_kind, int32_t right_kind, const int64_t* left, const int64_t* right,
-                      int64_t n_pairs, const double* distance, const double* lx, const double* ly,
+                      int64_t n_pairs, int64_t n_right, const double* distance, const double* lx, const double* ly,
                       const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring, const double* rx,
                       const double* ry, const int32_t* r_geom, const int32_t* r_part, const int32_t* r_ring,
                       int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,

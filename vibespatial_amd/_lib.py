@@ -76,9 +76,10 @@ SIGNATURES = {
         [_i32, _i32, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p],
     ),
     "vs_expand_boxes": (ctypes.c_int, [_p, _i64, _p, _p, _p]),
+    "vs_refine_dwithin_workspace_bytes": (_i64, [_i64]),
     "vs_refine_dwithin": (
         ctypes.c_int,
-        [_i32, _i32, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p],
+        [_i32, _i32, _p, _p, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p],
     ),
     "vs_lexsort_pairs_workspace_bytes": (_i64, [_i64]),
     "vs_lexsort_pairs": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p]),

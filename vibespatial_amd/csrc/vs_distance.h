@@ -221,17 +221,40 @@ VS_HD bool point_segment_within(double px, double py, double ax, double ay, doub
     return point_segment_within_fin(px, py, ax, ay, bx, by, d);
 }
 
-// Polygonal row `row` of P is within d of point p: p is not outside it, or
-// one of its boundary segments is within d.
+// Segment a-b cannot be within d of p: p lies more than d outside the
+// segment's box on one axis (gap measured in fp64; the limit carries a
+// relative margin above the rounding of one subtraction, so a rejected
+// segment is truly farther than d).
+VS_HD bool segment_far(double px, double py, double ax, double ay, double bx, double by, double lim) {
+    const double gx = fmax(fmin(ax, bx) - px, px - fmax(ax, bx));
+    const double gy = fmax(fmin(ay, by) - py, py - fmax(ay, by));
+    return gx > lim || gy > lim;
+}
+
+VS_HD double far_limit(double d) { return d * (1.0 + 8.881784197001252e-16); }  // d (1 + 2^-50)
+
+// Polygonal row `row` of P is within d of point p: a boundary segment within
+// d, or p inside a part (even-odd over the part's rings; a point on the
+// boundary is at distance 0 and found by either test).  One pass over the
+// edges; segments farther than d on an axis are skipped before the exact
+// distance test.
 __device__ inline bool point_polygon_within(double px, double py, const Polys& P, int64_t row, double d) {
     if (!(d >= 0.0) || px != px || py != py) return false;
-    const int32_t r0 = P.part[P.geom[row]], r1 = P.part[P.geom[row + 1]];
-    if (P.ring[r1] <= P.ring[r0]) return false;  // empty
+    const int32_t p0 = P.geom[row], p1 = P.geom[row + 1];
+    if (P.ring[P.part[p1]] <= P.ring[P.part[p0]]) return false;  // empty
     if (d == INFINITY) return true;
-    if (point_location_f64(px, py, P.x, P.y, P.geom, P.part, P.ring, row) != 0) return true;
-    for (int32_t r = r0; r < r1; ++r)
-        for (int32_t c = P.ring[r] + 1; c < P.ring[r + 1]; ++c)
-            if (point_segment_within(px, py, P.x[c - 1], P.y[c - 1], P.x[c], P.y[c], d)) return true;
+    const double lim = far_limit(d);
+    for (int32_t pp = p0; pp < p1; ++pp) {
+        int parity = 0;
+        for (int32_t r = P.part[pp]; r < P.part[pp + 1]; ++r)
+            for (int32_t c = P.ring[r] + 1; c < P.ring[r + 1]; ++c) {
+                const double ax = P.x[c - 1], ay = P.y[c - 1], bx = P.x[c], by = P.y[c];
+                if (!segment_far(px, py, ax, ay, bx, by, lim) && point_segment_within(px, py, ax, ay, bx, by, d))
+                    return true;
+                if (edge_step_f64(px, py, ax, ay, bx, by, parity)) return true;  // on the boundary
+            }
+        if (parity) return true;  // inside this part
+    }
     return false;
 }
 
@@ -245,17 +268,24 @@ __device__ inline bool polygons_within(const Polys& A, int64_t ra, const Polys& 
     if (A.ring[a1] <= A.ring[a0] || B.ring[b1] <= B.ring[b0]) return false;  // empty
     if (d == INFINITY) return true;
     if (polygons_intersect_auto(A, ra, B, rb)) return true;
+    const double lim = far_limit(d);
     for (int32_t va = A.ring[a0]; va < A.ring[a1]; ++va) {
         const double px = A.x[va], py = A.y[va];
         for (int32_t r = b0; r < b1; ++r)
-            for (int32_t c = B.ring[r] + 1; c < B.ring[r + 1]; ++c)
-                if (point_segment_within(px, py, B.x[c - 1], B.y[c - 1], B.x[c], B.y[c], d)) return true;
+            for (int32_t c = B.ring[r] + 1; c < B.ring[r + 1]; ++c) {
+                const double ax = B.x[c - 1], ay = B.y[c - 1], bx = B.x[c], by = B.y[c];
+                if (!segment_far(px, py, ax, ay, bx, by, lim) && point_segment_within(px, py, ax, ay, bx, by, d))
+                    return true;
+            }
     }
     for (int32_t vb = B.ring[b0]; vb < B.ring[b1]; ++vb) {
         const double px = B.x[vb], py = B.y[vb];
         for (int32_t r = a0; r < a1; ++r)
-            for (int32_t c = A.ring[r] + 1; c < A.ring[r + 1]; ++c)
-                if (point_segment_within(px, py, A.x[c - 1], A.y[c - 1], A.x[c], A.y[c], d)) return true;
+            for (int32_t c = A.ring[r] + 1; c < A.ring[r + 1]; ++c) {
+                const double ax = A.x[c - 1], ay = A.y[c - 1], bx = A.x[c], by = A.y[c];
+                if (!segment_far(px, py, ax, ay, bx, by, lim) && point_segment_within(px, py, ax, ay, bx, by, d))
+                    return true;
+            }
     }
     return false;
 }

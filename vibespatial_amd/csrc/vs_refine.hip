@@ -78,12 +78,16 @@ __global__ void total_to_result_kernel(const long long* __restrict__ total, int6
 
 // Exact dwithin per pair; kinds: 0 point rows (x[row], y[row]), 1 polygonal
 // rows.  d = distance[left row].
+// Pairs are visited in right-row order (order[t]), so the lanes of a wave
+// mostly share one right geometry: same coordinates, same loop lengths.
 __global__ __launch_bounds__(256) void dwithin_pairs_kernel(int left_kind, int right_kind,
                                                            const int64_t* __restrict__ left,
                                                            const int64_t* __restrict__ right, int64_t n,
+                                                           const int32_t* __restrict__ order,
                                                            const double* __restrict__ distance, Polys L, Polys R,
                                                            int32_t* __restrict__ keep) {
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = order[t];
         const int64_t l = left[i], r = right[i];
         const double d = distance[l];
         bool k;
@@ -93,6 +97,28 @@ __global__ __launch_bounds__(256) void dwithin_pairs_kernel(int left_kind, int r
         else k = polygons_within(L, l, R, r, d);
         keep[i] = k ? 1 : 0;
     }
+}
+
+__global__ __launch_bounds__(256) void right_keys_kernel(const int64_t* __restrict__ right, int64_t n,
+                                                        uint32_t* __restrict__ key, int32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        key[i] = (uint32_t)right[i];
+        idx[i] = (int32_t)i;
+    }
+}
+
+int64_t refine_workspace_bytes(int64_t n);
+
+int64_t dwithin_workspace_bytes(int64_t n) {
+    Carver c(nullptr);
+    c.take<char>(refine_workspace_bytes(n));
+    c.take<uint32_t>(n);
+    c.take<int32_t>(n);
+    c.take<uint32_t>(n);
+    c.take<int32_t>(n);
+    c.take<char>(sort_workspace_bytes_u32(n));
+    return c.bytes;
 }
 
 // Boxes grown by distance[i] on every side, each bound rounded one ulp
@@ -203,8 +229,10 @@ int vs_expand_boxes(const double* boxes, int64_t n, const double* distance, doub
     return VS_OK;
 }
 
+int64_t vs_refine_dwithin_workspace_bytes(int64_t n_pairs) { return vs::dwithin_workspace_bytes(n_pairs); }
+
 int vs_refine_dwithin(int32_t left_kind, int32_t right_kind, const int64_t* left, const int64_t* right,
-                      int64_t n_pairs, const double* distance, const double* lx, const double* ly,
+                      int64_t n_pairs, int64_t n_right, const double* distance, const double* lx, const double* ly,
                       const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring, const double* rx,
                       const double* ry, const int32_t* r_geom, const int32_t* r_part, const int32_t* r_ring,
                       int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
@@ -213,21 +241,36 @@ int vs_refine_dwithin(int32_t left_kind, int32_t right_kind, const int64_t* left
     VS_REQUIRE((left_kind == 0 || left_kind == 1) && (right_kind == 0 || right_kind == 1),
                "vs_refine_dwithin: kinds must be 0 (points) or 1 (polygons)");
     VS_REQUIRE(n_pairs >= 0 && n_pairs < (int64_t)INT32_MAX, "vs_refine_dwithin: n_pairs out of range");
-    VS_REQUIRE(workspace_bytes >= vs::refine_workspace_bytes(n_pairs), "vs_refine_dwithin: workspace too small");
+    VS_REQUIRE(n_right >= 0 && n_right <= (int64_t)INT32_MAX, "vs_refine_dwithin: n_right out of range");
+    VS_REQUIRE(workspace_bytes >= vs::dwithin_workspace_bytes(n_pairs), "vs_refine_dwithin: workspace too small");
     VS_REQUIRE(d_result != nullptr, "vs_refine_dwithin: null result");
     VS_REQUIRE(n_pairs == 0 || distance != nullptr, "vs_refine_dwithin: null distance");
     hipStream_t s = vs::stream_of(stream);
     vs::Carver c(workspace);
-    int32_t* keep = c.take<int32_t>(n_pairs);
-    int32_t* pos = c.take<int32_t>(n_pairs);
-    long long* total = c.take<long long>(1);
-    void* scan_ws = c.take<char>(vs::scan_workspace_bytes(n_pairs));
+    void* refine_ws = c.take<char>(vs::refine_workspace_bytes(n_pairs));
+    uint32_t* key_in = c.take<uint32_t>(n_pairs);
+    int32_t* idx_in = c.take<int32_t>(n_pairs);
+    uint32_t* key_out = c.take<uint32_t>(n_pairs);
+    int32_t* order = c.take<int32_t>(n_pairs);
+    void* sort_ws = c.take<char>(vs::sort_workspace_bytes_u32(n_pairs));
+    vs::Carver cr(refine_ws);
+    int32_t* keep = cr.take<int32_t>(n_pairs);
+    int32_t* pos = cr.take<int32_t>(n_pairs);
+    long long* total = cr.take<long long>(1);
+    void* scan_ws = cr.take<char>(vs::scan_workspace_bytes(n_pairs));
     const vs::Polys L = {lx, ly, l_geom, l_part, l_ring};
     const vs::Polys R = {rx, ry, r_geom, r_part, r_ring};
     if (n_pairs > 0) {
+        hipLaunchKernelGGL(vs::right_keys_kernel, dim3((unsigned)vs::ceil_div(n_pairs, 256)), dim3(256), 0, s, right,
+                           n_pairs, key_in, idx_in);
+        VS_LAUNCH_CHECK();
+        int bits = 0;
+        while (bits < 31 && ((int64_t)1 << bits) < n_right) ++bits;
+        VS_TRY(vs::sort_pairs_into_u32(key_in, idx_in, key_out, order, n_pairs, 0, bits, sort_ws,
+                                       vs::sort_workspace_bytes_u32(n_pairs), s));
         const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
         hipLaunchKernelGGL(vs::dwithin_pairs_kernel, dim3(grid), dim3(256), 0, s, left_kind, right_kind, left, right,
-                           n_pairs, distance, L, R, keep);
+                           n_pairs, order, distance, L, R, keep);
         VS_LAUNCH_CHECK();
     }
     VS_TRY(vs::exclusive_scan_i32(keep, pos, n_pairs, total, scan_ws, s));

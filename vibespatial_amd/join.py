@@ -132,9 +132,9 @@ def refine_dwithin(left: torch.Tensor, right: torch.Tensor, distance: torch.Tens
     out_l = torch.empty(n, dtype=torch.int64, device=dev)
     out_r = torch.empty(n, dtype=torch.int64, device=dev)
     res = torch.zeros(4, dtype=torch.int64, device=dev)
-    ws = _ws(L.vs_refine_pairs_workspace_bytes(n), dev)
-    check(L.vs_refine_dwithin(lk, rk, ptr(left.contiguous()), ptr(right.contiguous()), n, ptr(d), *la, *ra,
-                              ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
+    ws = _ws(L.vs_refine_dwithin_workspace_bytes(n), dev)
+    check(L.vs_refine_dwithin(lk, rk, ptr(left.contiguous()), ptr(right.contiguous()), n, len(rgeom), ptr(d),
+                              *la, *ra, ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
           "vs_refine_dwithin")
     k = int(res[0].item())
     return out_l[:k], out_r[:k]
