@@ -22,7 +22,9 @@ and for the plain form's traversal kernel
 (N points, V_q vertices of polygon q over all rings, C the exact inclusive
 bbox candidate pairs, E escalated edges, K the refined pairs).  `cpu_baseline` times the CPU
 oracle's join (packed tree + exact location, OpenMP) on a bounded sample of
-the same workload on this host.
+the same workload on this host: median of 3 runs on all usable cores (up to
+16) after a warm-up, plus `one_thread`, the same join on one thread over a
+tenth of the sample.
 """
 
 from __future__ import annotations
@@ -150,16 +152,30 @@ def main(argv=None):
         _log(f"cpu baseline on {ns} points, {cores} threads")
         px, py = w.points.x[:ns], w.points.y[:ns]
         oracle.sjoin_points_polygons(px[: ns // 10], py[: ns // 10], P_h, "within", nthreads=cores)  # warm-up
-        tc0 = time.perf_counter()
-        el, _ = oracle.sjoin_points_polygons(px, py, P_h, "within", nthreads=cores)
-        tc = time.perf_counter() - tc0
+        runs = []
+        for _ in range(3):  # median of 3 after the warm-up
+            tc0 = time.perf_counter()
+            el, _ = oracle.sjoin_points_polygons(px, py, P_h, "within", nthreads=cores)
+            runs.append(time.perf_counter() - tc0)
+        tc = statistics.median(runs)
+        # one thread on a tenth of the sample, median of 3
+        n1 = max(ns // 10, 1)
+        runs1 = []
+        for _ in range(3):
+            tc0 = time.perf_counter()
+            e1, _ = oracle.sjoin_points_polygons(px[:n1], py[:n1], P_h, "within", nthreads=1)
+            runs1.append(time.perf_counter() - tc0)
+        t1 = statistics.median(runs1)
         cpu = {
             "value": len(el) / tc,
             "unit": "pairs/s",
             "cores": cores,
             "kind": "port",
             "sample": f"first {ns:,} of the {n_pts:,} points x all {len(P_h):,} polygons, within "
-                      f"({len(el):,} pairs in {tc:.2f} s; tree build included)",
+                      f"({len(el):,} pairs, median of 3 runs {tc:.2f} s; tree build included)",
+            "one_thread": {"value": len(e1) / t1, "unit": "pairs/s", "cores": 1,
+                           "sample": f"first {n1:,} points x all polygons ({len(e1):,} pairs, median of 3 "
+                                     f"runs {t1:.2f} s)"},
         }
 
     out = {
