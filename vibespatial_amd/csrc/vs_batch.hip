@@ -260,15 +260,25 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
 
 // One thread per candidate: exact intersects; survivors appended to the
 // finish stage with one atomic per wave.
+// A left row that is one ring of at most kRingTile coordinates (the
+// footprints) is copied into the lane's column of an LDS tile (x and y, 64
+// lanes wide) and tested from there: its edges are re-read for every right
+// edge near it.  Other rows, and pairs where the right row is the smaller
+// ring, take polygons_intersect_auto; the boolean is the same.
+constexpr int kRingTile = 13;
+constexpr int64_t kRingTileBytesPerWave = 2 * kRingTile * 64 * (int64_t)sizeof(double);
 __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __restrict__ cand_l,
                                                           const int32_t* __restrict__ cand_r,
                                                           const JoinCtl* __restrict__ cctl, int64_t ccap, Polys A,
                                                           Polys B, uint32_t* __restrict__ out_l,
                                                           int32_t* __restrict__ out_r, int64_t capacity,
                                                           JoinCtl* __restrict__ ctl) {
+    extern __shared__ double ring_tile[];
     const unsigned long long nc = cctl->staged;
     if (nc > (unsigned long long)ccap) return;
     const int lane = threadIdx.x & 63;
+    double* tx = ring_tile + (threadIdx.x >> 6) * (2 * kRingTile * 64) + lane;
+    double* ty = tx + kRingTile * 64;
     const uint64_t lt = lanemask_lt();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < (int64_t)nc; i0 += stride) {
@@ -279,7 +289,16 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
         if (i < (int64_t)nc) {
             l = cand_l[i];
             r = cand_r[i];
-            keep = polygons_intersect_auto(A, l, B, r);
+            int32_t ca = 0, na = 0, cb = 0, nb = 0;
+            if (single_ring_of(A, l, ca, na) && na <= kRingTile && !(single_ring_of(B, r, cb, nb) && nb < na)) {
+                for (int32_t k = 0; k < na; ++k) {  // the lane's own column: no other lane reads it
+                    tx[k * 64] = A.x[ca + k];
+                    ty[k * 64] = A.y[ca + k];
+                }
+                keep = ring_intersects_polys(RingRef{tx, ty, 64}, na, B, r);
+            } else {
+                keep = polygons_intersect_auto(A, l, B, r);
+            }
         }
         const uint64_t km = __ballot(keep);
         if (!km) continue;
@@ -435,7 +454,8 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
                                    &W.cctl->staged));
         const unsigned rgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, refine_threads),
                                                                                   (int64_t)cus * 16));
-        hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads), 0, s,
+        hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads),
+                           (unsigned)(refine_threads / 64 * kRingTileBytesPerWave), s,
                            reinterpret_cast<const uint32_t*>(W.cs_l), reinterpret_cast<const int32_t*>(W.cs_r),
                            W.cctl, capacity, A, B, W.stage_l, W.stage_r, capacity, W.ctl);
         VS_LAUNCH_CHECK();

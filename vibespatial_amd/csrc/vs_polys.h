@@ -54,13 +54,14 @@ __device__ inline bool polygons_intersect(const Polys& A, int64_t ra, const Poly
 }
 
 // Same predicate when one side A is a single ring (one part, one ring), with
-// the loops turned around: the other polygon's edges are streamed once, each
-// first checked against A's bounds (exact min/max, so the check only skips
-// edges that cannot touch A) before A's edges are tested against it, and the
-// crossing step of A's first vertex rides along; then the first vertex of
-// each of B's rings is located in A unless it lies outside A's bounds.
-// Segment touch is symmetric and the location checks are those of
-// polygons_intersect, so the answer is the same boolean.
+// the loops turned around: the other polygon's edges are streamed to locate
+// A's first vertex in B (inside or on the boundary: true at once); only if
+// some B edge's box meets A's box are they streamed again, each tested
+// against A's edges; then the first vertex of each of B's rings is located
+// in A unless it lies outside A's bounds.  The bounds checks use exact
+// min/max, so they only skip edges that cannot touch A.  Segment touch is
+// symmetric and the location checks are those of polygons_intersect, so the
+// answer is the same boolean.
 __device__ inline bool single_ring_of(const Polys& P, int64_t r, int32_t& c0, int32_t& n) {
     const int32_t p0 = P.geom[r];
     if (P.geom[r + 1] - p0 != 1) return false;
@@ -71,19 +72,32 @@ __device__ inline bool single_ring_of(const Polys& P, int64_t r, int32_t& c0, in
     return n >= 1;
 }
 
-__device__ inline bool polygons_intersect_ring(const Polys& A, int32_t a0, int32_t na, const Polys& B, int64_t rb) {
+// Ring coordinates k = 0..n-1 at x[k * stride], y[k * stride] (global memory
+// with stride 1, or a lane's column of an LDS tile).
+struct RingRef {
+    const double* x;
+    const double* y;
+    int32_t stride;
+    __device__ __forceinline__ double X(int32_t k) const { return x[k * stride]; }
+    __device__ __forceinline__ double Y(int32_t k) const { return y[k * stride]; }
+};
+
+__device__ inline bool ring_intersects_polys(const RingRef& A, int32_t na, const Polys& B, int64_t rb) {
     const int32_t bp0 = B.geom[rb], bp1 = B.geom[rb + 1];
     if (B.ring[B.part[bp1]] <= B.ring[B.part[bp0]]) return false;
     double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
-    for (int32_t k = a0; k < a0 + na; ++k) {
-        mnx = fmin(mnx, A.x[k]);
-        mny = fmin(mny, A.y[k]);
-        mxx = fmax(mxx, A.x[k]);
-        mxy = fmax(mxy, A.y[k]);
+    for (int32_t k = 0; k < na; ++k) {
+        mnx = fmin(mnx, A.X(k));
+        mny = fmin(mny, A.Y(k));
+        mxx = fmax(mxx, A.X(k));
+        mxy = fmax(mxy, A.Y(k));
     }
-    const double fx = A.x[a0], fy = A.y[a0];
+    // Pass 1: location of A's first vertex in each part of B (inside or on
+    // the boundary decides at once), noting whether any B edge comes near A.
+    const double fx = A.X(0), fy = A.Y(0);
+    bool near = false;
     for (int32_t bp = bp0; bp < bp1; ++bp) {
-        int parity = 0;  // A's first vertex against this part of B
+        int parity = 0;
         for (int32_t br = B.part[bp]; br < B.part[bp + 1]; ++br) {
             const int32_t e0 = B.ring[br], e1 = B.ring[br + 1];
             if (e1 <= e0) continue;
@@ -101,10 +115,7 @@ __device__ inline bool polygons_intersect_ring(const Polys& A, int32_t a0, int32
                 for (int u = 0; u < 4; ++u) {
                     if (c0 + u >= e1) break;
                     const double qx = qxs[u], qy = qys[u];
-                    if (fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy) {
-                        for (int32_t k = a0 + 1; k < a0 + na; ++k)
-                            if (segments_touch(A.x[k - 1], A.y[k - 1], A.x[k], A.y[k], px, py, qx, qy)) return true;
-                    }
+                    near |= fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy;
                     if (edge_step_f64(fx, fy, px, py, qx, qy, parity)) return true;  // on B's boundary
                     px = qx;
                     py = qy;
@@ -113,17 +124,49 @@ __device__ inline bool polygons_intersect_ring(const Polys& A, int32_t a0, int32
         }
         if (parity) return true;  // inside this part of B
     }
+    // Pass 2 (only when some B edge's box meets A's box): boundary contacts.
+    if (near) {
+        for (int32_t br = B.part[bp0]; br < B.part[bp1]; ++br) {
+            const int32_t e0 = B.ring[br], e1 = B.ring[br + 1];
+            if (e1 <= e0) continue;
+            double px = B.x[e0], py = B.y[e0];
+            for (int32_t c0 = e0 + 1; c0 < e1; c0 += 4) {
+                double qxs[4], qys[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int32_t c = c0 + u < e1 ? c0 + u : e1 - 1;
+                    qxs[u] = B.x[c];
+                    qys[u] = B.y[c];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (c0 + u >= e1) break;
+                    const double qx = qxs[u], qy = qys[u];
+                    if (fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy) {
+                        for (int32_t k = 1; k < na; ++k)
+                            if (segments_touch(A.X(k - 1), A.Y(k - 1), A.X(k), A.Y(k), px, py, qx, qy)) return true;
+                    }
+                    px = qx;
+                    py = qy;
+                }
+            }
+        }
+    }
     for (int32_t br = B.part[bp0]; br < B.part[bp1]; ++br) {
         const int32_t e0 = B.ring[br];
         if (B.ring[br + 1] <= e0) continue;
         const double px = B.x[e0], py = B.y[e0];
         if (px < mnx || px > mxx || py < mny || py > mxy) continue;  // outside A's bounds: location 0
         int parity = 0;
-        for (int32_t k = a0 + 1; k < a0 + na; ++k)
-            if (edge_step_f64(px, py, A.x[k - 1], A.y[k - 1], A.x[k], A.y[k], parity)) return true;
+        for (int32_t k = 1; k < na; ++k)
+            if (edge_step_f64(px, py, A.X(k - 1), A.Y(k - 1), A.X(k), A.Y(k), parity)) return true;
         if (parity) return true;
     }
     return false;
+}
+
+__device__ inline bool polygons_intersect_ring(const Polys& A, int32_t a0, int32_t na, const Polys& B, int64_t rb) {
+    return ring_intersects_polys(RingRef{A.x + a0, A.y + a0, 1}, na, B, rb);
 }
 
 // polygons_intersect, streaming the larger side when the smaller is one ring.
