@@ -1404,15 +1404,16 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
         hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(vs::kTrainThreads), 0, t, B.keys, n, iters, lr, B.mlp,
                            B.losses, B.meta);
         VS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(leaf_err_kernel<K>, dim3(stride_grid(S, 4)), dim3(256), 0, t, B.keys, n, S, B.leaf_key,
+        hipLaunchKernelGGL(leaf_err_kernel<K>, dim3(stride_grid(S, 4, 64)), dim3(256), 0, t, B.keys, n, S, B.leaf_key,
                            B.mlp, B.leaf_err, B.meta);
         VS_LAUNCH_CHECK();
     } else {
         hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, t, B.meta, 4);
         VS_LAUNCH_CHECK();
     }
-    // four gather blocks per CU leave room for the side stream's kernels
-    VS_TRY(gather_points(px, py, B.perm, n, B.sx, B.sy, s, 4));
+    // two gather blocks per CU (enough to keep the random reads in flight)
+    // leave room for the side stream's kernels
+    VS_TRY(gather_points(px, py, B.perm, n, B.sx, B.sy, s, 2));
     VS_CHECK_HIP(hipEventRecord(side->join, t));
     VS_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
     return VS_OK;
