@@ -322,6 +322,39 @@ int vs_batched_intersects(const double* ax, const double* ay, const int32_t* a_g
                           void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
+ * WKB ingest (GeoParquet / GeoArrow WKB columns -> SoA on the device)
+ * --------------------------------------------------------------------- */
+
+int64_t vs_wkb_workspace_bytes(int64_t n_rows);
+
+/* Pass 1 over n_rows WKB records: record r is payload[offsets[r],
+ * offsets[r+1]) (device int64 offsets, n_rows + 1); validity (device uint8
+ * per row, or null = all valid): 0 or an empty record = null row.  Supports
+ * Point, Polygon and MultiPolygon in 2D, either byte order, ISO or EWKB type
+ * words (an SRID is skipped).  d_result (device int64[8]): [0] point rows,
+ * [1] polygonal rows, [2] parts, [3] rings, [4] polygon coordinates, [5] first
+ * bad record or -1, [6] its error (1 truncated, 2 unsupported type or
+ * dimension, 3 bad byte order, 4 multipolygon member not a polygon, 5 count
+ * over 2^31, 6 bytes after the geometry), [7] 1 when parts, rings or
+ * coordinates of the column exceed INT32_MAX (vs_wkb_fill then writes
+ * nothing).  The workspace carries the counts to vs_wkb_fill.  Replaces the
+ * reference's device WKB decode (kernels/core/wkb_decode.py:849, used by
+ * io/wkb.py:5049 and io/geoparquet.py:3430). */
+int vs_wkb_scan(const uint8_t* payload, const int64_t* offsets, const uint8_t* validity, int64_t n_rows,
+                int64_t* d_result, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Pass 2 (same records and workspace, after vs_wkb_scan): point rows ->
+ * point_x/point_y (result[0] entries) with their record numbers in point_rows;
+ * polygonal rows -> x/y (result[4]), ring_offsets (result[3] + 1),
+ * part_offsets (result[2] + 1), geom_offsets (result[1] + 1) with their record
+ * numbers in polygon_rows; a Polygon is one part (none when it has no rings),
+ * a MultiPolygon one part per member. */
+int vs_wkb_fill(const uint8_t* payload, const int64_t* offsets, const uint8_t* validity, int64_t n_rows,
+                const void* workspace, int64_t workspace_bytes, double* point_x, double* point_y,
+                int64_t* point_rows, double* x, double* y, int32_t* geom_offsets, int32_t* part_offsets,
+                int32_t* ring_offsets, int64_t* polygon_rows, void* stream);
+
+/* ---------------------------------------------------------------------
  * Primitives
  * --------------------------------------------------------------------- */
 

@@ -26,4 +26,8 @@ def __getattr__(name):
         from . import api
 
         return getattr(api, name)
+    if name in ("read_parquet", "read_file", "decode_wkb", "encode_wkb"):
+        from . import io
+
+        return getattr(io, name)
     raise AttributeError(name)

@@ -101,6 +101,9 @@ SIGNATURES = {
     "vs_batched_intersects_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
     "vs_batched_intersects": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _p, _p, _i64, _i32,
                                              _i64, _p, _p, _p, _p, _i64, _p]),
+    "vs_wkb_workspace_bytes": (_i64, [_i64]),
+    "vs_wkb_scan": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p]),
+    "vs_wkb_fill": (ctypes.c_int, [_p, _p, _p, _i64, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "vs_scan_workspace_bytes": (_i64, [_i64]),
     "vs_exclusive_scan_i32": (ctypes.c_int, [_p, _p, _i64, _p, _p, _i64, _p]),
     "vs_sort_workspace_bytes": (_i64, [_i64, _i32]),

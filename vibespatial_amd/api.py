@@ -394,6 +394,12 @@ class GeoDataFrame:
             return self._geometry
         return self._df[col]
 
+    def to_parquet(self, path, little_endian: bool = True) -> None:
+        """GeoParquet 1.0 with a WKB geometry column."""
+        from .io import to_parquet
+
+        to_parquet(self, path, little_endian)
+
     def to_pandas(self) -> pd.DataFrame:
         out = self._df.copy()
         out.insert(self._geom_pos(), self._geom_name, list(self._geometry))

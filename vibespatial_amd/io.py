@@ -1,4 +1,13 @@
-"""Minimal ESRI shapefile reader (points and polygons) into owned SoA arrays.
+"""Readers into owned SoA arrays: GeoParquet (WKB columns decoded on the GPU,
+GeoArrow native point / polygon / multipolygon columns by buffer views) and a
+minimal ESRI shapefile reader (points and polygons).
+
+GeoParquet follows the reference's read path (io/geoparquet.py:3430
+read_geoparquet: pyarrow scan, the geometry column decoded to owned buffers
+without per-row host objects; io/wkb.py:5049 decode_wkb_owned); the writer
+emits WKB with GeoParquet 1.0 "geo" metadata.
+
+Shapefiles: minimal ESRI shapefile reader (points and polygons) into owned SoA arrays.
 
 Enough of the format to load the reference's upstream test datasets
 (tests/upstream/geopandas/tests/data/: nybb_16a.zip, naturalearth_*), which
@@ -20,7 +29,7 @@ import pandas as pd
 
 from .geometry import GeometryArray, PointArray, PolygonArray
 
-__all__ = ["read_file", "read_shapefile_bytes"]
+__all__ = ["read_file", "read_shapefile_bytes", "read_parquet", "to_parquet", "decode_wkb", "encode_wkb"]
 
 
 def _signed_area2(x: np.ndarray, y: np.ndarray) -> float:
@@ -143,3 +152,239 @@ def read_file(path: str, crs=None):
 
 
 _ = _io
+
+
+# ---------------------------------------------------------------------------
+# WKB
+# ---------------------------------------------------------------------------
+_WKB_ERRORS = {1: "truncated record", 2: "unsupported geometry type or dimension (Point, Polygon, MultiPolygon "
+               "in 2D are read)", 3: "bad byte-order byte", 4: "MultiPolygon member is not a Polygon",
+               5: "count over 2^31", 6: "trailing bytes after the geometry"}
+
+
+def encode_wkb(ga: GeometryArray, little_endian: bool = True):
+    """(payload uint8, offsets int64 (n + 1), validity uint8) of ISO WKB records:
+    Point, Polygon (one part; no rings when empty) or MultiPolygon (several
+    parts); null rows are empty records with validity 0.  Point records are
+    written in bulk; polygon records one row at a time."""
+    bo = "<" if little_endian else ">"
+    flag = b"\x01" if little_endian else b"\x00"
+    n = len(ga)
+    lens = np.zeros(n, dtype=np.int64)
+    valid = np.zeros(n, dtype=np.uint8)
+    P = ga.polygons
+    recs = {}
+    if P is not None:
+        def poly_body(rings):
+            out = [struct.pack(bo + "I", len(rings))]
+            for b0, b1 in rings:
+                out.append(struct.pack(bo + "I", b1 - b0))
+                xy = np.empty(2 * (b1 - b0), dtype=bo + "f8")
+                xy[0::2], xy[1::2] = P.x[b0:b1], P.y[b0:b1]
+                out.append(xy.tobytes())
+            return b"".join(out)
+
+        ro, po, go = P.ring_offsets, P.part_offsets, P.geom_offsets
+        for g, row in enumerate(ga.polygon_rows.tolist()):
+            parts = [[(int(ro[r]), int(ro[r + 1])) for r in range(po[p], po[p + 1])]
+                     for p in range(go[g], go[g + 1])]
+            if len(parts) <= 1:
+                rec = flag + struct.pack(bo + "I", 3) + poly_body(parts[0] if parts else [])
+            else:
+                rec = flag + struct.pack(bo + "II", 6, len(parts)) + b"".join(
+                    flag + struct.pack(bo + "I", 3) + poly_body(r) for r in parts)
+            recs[row] = rec
+            lens[row] = len(rec)
+            valid[row] = 1
+    if ga.points is not None:
+        lens[ga.point_rows] = 21
+        valid[ga.point_rows] = 1
+    offsets = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=offsets[1:])
+    payload = np.empty(int(offsets[-1]), dtype=np.uint8)
+    if ga.points is not None and len(ga.point_rows):
+        rec = np.zeros(len(ga.point_rows), dtype=np.dtype([("f", "u1"), ("t", bo + "u4"), ("x", bo + "f8"),
+                                                           ("y", bo + "f8")]))
+        rec["f"] = 1 if little_endian else 0
+        rec["t"] = 1
+        rec["x"], rec["y"] = ga.points.x, ga.points.y
+        payload[offsets[ga.point_rows][:, None] + np.arange(21)] = rec.view(np.uint8).reshape(-1, 21)
+    for row, rec in recs.items():
+        payload[offsets[row]: offsets[row + 1]] = np.frombuffer(rec, dtype=np.uint8)
+    return payload, offsets, valid
+
+
+def decode_wkb(payload, offsets, validity=None) -> GeometryArray:
+    """GeometryArray from WKB records decoded on the GPU (vs_wkb_scan /
+    vs_wkb_fill): record r is payload[offsets[r]:offsets[r + 1]]; validity 0
+    or an empty record is a null row.  The decoded buffers stay on the device
+    (cached on the arrays) and are also copied to the host arrays."""
+    import torch
+
+    from ._lib import check, lib
+    from .device import DevicePoints, DevicePolygons, ptr, require_gpu, stream
+
+    dev = require_gpu()
+    L = lib()
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n = len(off) - 1
+    if n < 0:
+        raise ValueError("offsets must hold n + 1 entries")
+    data = np.frombuffer(payload, dtype=np.uint8) if isinstance(payload, (bytes, bytearray)) else \
+        np.ascontiguousarray(payload, dtype=np.uint8)
+    if n and (off[0] < 0 or off[-1] > len(data) or np.any(np.diff(off) < 0)):
+        raise ValueError("offsets out of the payload or not non-decreasing")
+    d_data = torch.from_numpy(data.copy() if len(data) else np.zeros(1, np.uint8)).to(dev)
+    d_off = torch.from_numpy(off).to(dev)
+    d_valid = None if validity is None else torch.from_numpy(np.ascontiguousarray(validity, dtype=np.uint8)).to(dev)
+    ws = torch.empty(max(int(L.vs_wkb_workspace_bytes(n)), 1), dtype=torch.uint8, device=dev)
+    res = torch.zeros(8, dtype=torch.int64, device=dev)
+    check(L.vs_wkb_scan(ptr(d_data), ptr(d_off), ptr(d_valid), n, ptr(res), ptr(ws), ws.numel(), stream()),
+          "vs_wkb_scan")
+    r = res.cpu().tolist()
+    if r[5] >= 0:
+        raise ValueError(f"WKB record {r[5]}: {_WKB_ERRORS.get(int(r[6]), 'invalid record')}")
+    if r[7]:
+        raise OverflowError("WKB column holds more than 2^31-1 parts, rings or coordinates")
+    n_pts, n_poly, n_parts, n_rings, n_coords = (int(v) for v in r[:5])
+    f64 = dict(dtype=torch.float64, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    i64 = dict(dtype=torch.int64, device=dev)
+    px, py, prow = torch.empty(max(n_pts, 1), **f64), torch.empty(max(n_pts, 1), **f64), torch.empty(max(n_pts, 1),
+                                                                                                     **i64)
+    x, y = torch.empty(max(n_coords, 1), **f64), torch.empty(max(n_coords, 1), **f64)
+    geom = torch.empty(n_poly + 1, **i32)
+    part = torch.empty(n_parts + 1, **i32)
+    ring = torch.empty(n_rings + 1, **i32)
+    grow = torch.empty(max(n_poly, 1), **i64)
+    check(L.vs_wkb_fill(ptr(d_data), ptr(d_off), ptr(d_valid), n, ptr(ws), ws.numel(), ptr(px), ptr(py), ptr(prow),
+                        ptr(x), ptr(y), ptr(geom), ptr(part), ptr(ring), ptr(grow), stream()), "vs_wkb_fill")
+    tags = np.full(n, -1, dtype=np.int8)
+    points = polygons = prows = grows = None
+    if n_pts:
+        px, py = px[:n_pts], py[:n_pts]
+        points = PointArray(px.cpu().numpy(), py.cpu().numpy())
+        points._device = DevicePoints(px, py)
+        prows = prow[:n_pts].cpu().numpy()
+        tags[prows] = 0
+    if n_poly:
+        x, y = x[:n_coords], y[:n_coords]
+        polygons = PolygonArray(x.cpu().numpy(), y.cpu().numpy(), ring.cpu().numpy(), part.cpu().numpy(),
+                                geom.cpu().numpy())
+        polygons._device = DevicePolygons(x, y, ring, part, geom)
+        grows = grow[:n_poly].cpu().numpy()
+        tags[grows] = 1
+    return GeometryArray(tags, points, prows, polygons, grows)
+
+
+# ---------------------------------------------------------------------------
+# GeoParquet
+# ---------------------------------------------------------------------------
+def _lists(arr):
+    """(offsets rebased to 0, child values) of an Arrow list array."""
+    off = np.asarray(arr.offsets, dtype=np.int64)
+    return off - off[0], arr.values.slice(int(off[0]), int(off[-1] - off[0]))
+
+
+def _xy(arr):
+    """x, y float64 of GeoArrow coordinates (separated struct or interleaved)."""
+    import pyarrow as pa
+
+    if pa.types.is_struct(arr.type):
+        return (np.asarray(arr.field("x").fill_null(np.nan), dtype=np.float64),
+                np.asarray(arr.field("y").fill_null(np.nan), dtype=np.float64))
+    flat = np.asarray(arr.flatten(), dtype=np.float64).reshape(-1, arr.type.list_size)
+    return np.ascontiguousarray(flat[:, 0]), np.ascontiguousarray(flat[:, 1])
+
+
+def _geoarrow_native(col, encoding: str) -> GeometryArray:
+    """GeoArrow native point, polygon or multipolygon column -> GeometryArray
+    (host buffer views).  Null rows hold empty lists and map to no family."""
+    n = len(col)
+    rows = np.flatnonzero(np.asarray(col.is_valid()))
+    tags = np.full(n, -1, np.int8)
+    if encoding == "point":
+        xs, ys = _xy(col)
+        tags[rows] = 0
+        return GeometryArray(tags, PointArray(xs[rows], ys[rows]), rows, None, None)
+    if encoding == "polygon":
+        geom_rings, rings = _lists(col)
+        ring_coords, coords = _lists(rings)
+        nr = np.diff(geom_rings)
+        if np.any(nr[np.asarray(col.is_null())] != 0):
+            raise ValueError("null polygon rows with rings")
+        has = nr[rows] > 0  # an empty polygon has no parts
+        geom = np.concatenate([[0], np.cumsum(has)])
+        part = np.concatenate([geom_rings[rows][has], geom_rings[-1:]])
+    else:
+        geom_parts, parts = _lists(col)
+        if np.any(np.diff(geom_parts)[np.asarray(col.is_null())] != 0):
+            raise ValueError("null multipolygon rows with parts")
+        part, rings = _lists(parts)
+        ring_coords, coords = _lists(rings)
+        geom = np.concatenate([geom_parts[rows], geom_parts[-1:]])
+    xs, ys = _xy(coords)
+    tags[rows] = 1
+    return GeometryArray(tags, None, None, PolygonArray(xs, ys, ring_coords, part, geom), rows)
+
+
+def read_parquet(path, columns=None, crs=None):
+    """GeoDataFrame from a GeoParquet file: WKB geometry decoded on the GPU,
+    GeoArrow native point / polygon / multipolygon columns by buffer views;
+    other columns through pandas (io/geoparquet.py:3430 read_geoparquet)."""
+    import json
+
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+
+    from .api import GeoDataFrame, GeoSeries
+
+    table = pq.read_table(path, columns=columns)
+    raw = (table.schema.metadata or {}).get(b"geo")
+    if raw is None:
+        raise ValueError("no GeoParquet 'geo' metadata in the file")
+    meta = json.loads(raw)
+    name = meta["primary_column"]
+    cmeta = meta["columns"][name]
+    encoding = str(cmeta.get("encoding", "WKB"))
+    col = table.column(name).combine_chunks()
+    if encoding.upper() == "WKB":
+        if isinstance(col, pa.ChunkedArray):
+            col = col.chunk(0) if col.num_chunks == 1 else pa.concat_arrays(col.chunks)
+        col = col.cast(pa.large_binary())
+        _, off_buf, data_buf = col.buffers()
+        off = np.frombuffer(off_buf, dtype=np.int64)[col.offset: col.offset + len(col) + 1]
+        data = np.frombuffer(data_buf, dtype=np.uint8) if data_buf is not None else np.zeros(0, np.uint8)
+        ga = decode_wkb(data, off, np.asarray(col.is_valid(), dtype=np.uint8))
+    elif encoding in ("point", "polygon", "multipolygon"):
+        ga = _geoarrow_native(col, encoding)
+    else:
+        raise NotImplementedError(f"GeoParquet encoding {encoding!r}")
+    rest = table.drop([name])
+    attrs = rest.to_pandas() if rest.num_columns else pd.DataFrame(index=pd.RangeIndex(len(ga)))
+    crs = crs if crs is not None else cmeta.get("crs")
+    gdf = GeoDataFrame(attrs, geometry=GeoSeries(ga, crs=crs), crs=crs)
+    gdf._geom_name = name
+    return gdf
+
+
+def to_parquet(gdf, path, little_endian: bool = True):
+    """Write a GeoDataFrame as GeoParquet 1.0 with a WKB geometry column."""
+    import json
+
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+
+    ga = gdf.geometry.values
+    payload, offsets, valid = encode_wkb(ga, little_endian)
+    geom = pa.LargeBinaryArray.from_buffers(pa.large_binary(), len(ga), [
+        pa.py_buffer(np.packbits(valid.astype(bool), bitorder="little").tobytes()),
+        pa.py_buffer(offsets.tobytes()), pa.py_buffer(payload.tobytes())], null_count=int((valid == 0).sum()))
+    attrs = pa.Table.from_pandas(gdf._df.reset_index(drop=True), preserve_index=False)
+    name = gdf._geom_name
+    table = attrs.append_column(name, geom)
+    types = sorted({{0: "Point", 1: "Polygon"}[int(t)] for t in np.unique(ga.tags) if t >= 0})
+    meta = {"version": "1.0.0", "primary_column": name,
+            "columns": {name: {"encoding": "WKB", "geometry_types": types}}}
+    table = table.replace_schema_metadata({**(table.schema.metadata or {}), b"geo": json.dumps(meta).encode()})
+    pq.write_table(table, path)
