@@ -25,6 +25,10 @@ oracle's join (packed tree + exact location, OpenMP) on a bounded sample of
 the same workload on this host: median of 3 runs on all usable cores (up to
 16) after a warm-up, plus `one_thread`, the same join on one thread over a
 tenth of the sample.
+
+Other workloads (`--workload`): c4 (learned index vs tree, 50M points x 1M
+boxes), c5 (4096 batched polygon intersects jobs), dwithin (the c2 data with
+predicate='dwithin'), wkb (GPU WKB ingest of the c2 data).
 """
 
 from __future__ import annotations
@@ -57,7 +61,7 @@ def main(argv=None):
     ap.add_argument("--n-polys", type=int, default=100_000)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="points in the CPU-baseline sample (0 disables it)")
-    ap.add_argument("--workload", choices=("c2", "c4", "c5", "dwithin"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c4", "c5", "dwithin", "wkb"), default="c2",
                     help="c2: the sjoin metric (default); c4: point-tree sindex.query, learned index vs tree; "
                          "c5: 4096 batched polygon x polygon intersects jobs; dwithin: exact point x polygon "
                          "dwithin join on the c2 data")
@@ -90,6 +94,8 @@ def main(argv=None):
         return _with_copy_probe(args, run_c5)
     if args.workload == "dwithin":
         return _with_copy_probe(args, run_dwithin)
+    if args.workload == "wkb":
+        return _with_copy_probe(args, run_wkb)
     _log("generating workload")
     w = S.c2(n_points=args.n_points, n_polys=args.n_polys)
     Q = DevicePoints.from_host(w.points)
@@ -507,6 +513,106 @@ def run_dwithin(args):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "alg_bytes_per_step_lower_bound": alg},
         "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_wkb(args):
+    """GPU WKB ingest of the c2 data: 10M point records followed by 100k
+    star-polygon records (ISO WKB, little endian), payload and offsets resident
+    in HBM.  A step = vs_wkb_scan -> result read by the host -> output buffers
+    -> vs_wkb_fill.  Algorithmic bytes per step (lower bound): the payload and
+    offsets read once, and the decoded buffers written once (16 per point + 8
+    record number; 16 per coordinate + 4 per ring / part / polygon offset + 8
+    record number)."""
+    import numpy as np
+    import torch
+
+    from oracle import wkb_ref
+    from vibespatial_amd import _lib
+    from vibespatial_amd import synthetic as S
+    from vibespatial_amd.geometry import GeometryArray
+    from vibespatial_amd.io import decode_wkb_device, encode_wkb
+
+    _log("wkb: generating c2 workload and encoding it")
+    w = S.c2(n_points=args.n_points, n_polys=args.n_polys)
+    n, m = len(w.points), len(w.polygons)
+    tags = np.concatenate([np.zeros(n, np.int8), np.ones(m, np.int8)])
+    ga = GeometryArray(tags, w.points, np.arange(n), w.polygons, np.arange(n, n + m))
+    payload, offsets, valid = encode_wkb(ga)
+    dev = torch.device("cuda", 0)
+    d_pay = torch.from_numpy(payload).to(dev)
+    d_off = torch.from_numpy(offsets).to(dev)
+    d_val = torch.from_numpy(valid).to(dev)
+    col = decode_wkb_device(d_pay, d_off, d_val)
+    ws = torch.empty(int(_lib.lib().vs_wkb_workspace_bytes(n + m)), dtype=torch.uint8, device=dev)
+    # parity of the measured decode with the source buffers (bit-exact)
+    P = w.polygons
+    assert torch.equal(col.points.x.cpu().view(torch.int64), torch.from_numpy(w.points.x).view(torch.int64))
+    assert torch.equal(col.points.y.cpu().view(torch.int64), torch.from_numpy(w.points.y).view(torch.int64))
+    assert np.array_equal(col.polygons.x.cpu().numpy(), P.x) and np.array_equal(col.polygons.y.cpu().numpy(), P.y)
+    assert np.array_equal(col.polygons.ring_offsets.cpu().numpy(), P.ring_offsets)
+    assert np.array_equal(col.polygons.part_offsets.cpu().numpy(), P.part_offsets)
+    assert np.array_equal(col.polygons.geom_offsets.cpu().numpy(), P.geom_offsets)
+    del col
+    for _ in range(max(args.warmup, 1)):
+        decode_wkb_device(d_pay, d_off, d_val, workspace=ws)
+    torch.cuda.synchronize()
+    timing = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        col = decode_wkb_device(d_pay, d_off, d_val, workspace=ws, timing=timing)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    scan_ms = statistics.mean(a.elapsed_time(b) for (a, b), _ in timing)
+    fill_ms = statistics.mean(c.elapsed_time(d) for _, (c, d) in timing)
+    V = len(P.x)
+    n_rings, n_parts = len(P.ring_offsets) - 1, len(P.part_offsets) - 1
+    out_bytes = 24 * n + 16 * V + 4 * (n_rings + n_parts + m + 3) + 8 * m
+    alg = len(payload) + 8 * (n + m + 1) + (n + m) + out_bytes
+    device_ms = scan_ms + fill_ms
+
+    cpu = None
+    if args.cpu_sample > 0:
+        # the oracle (plain Python over struct) on the first 3M point records
+        # and the first 30k polygon records
+        ks, kp = min(3_000_000, n), min(30_000, m)
+        pt_pay = payload[: offsets[ks]]
+        pg_pay = payload[offsets[n]: offsets[n + kp]]
+        pg_off = offsets[n: n + kp + 1] - offsets[n]
+        _log(f"wkb: cpu baseline on {ks} point + {kp} polygon records")
+        tc0 = time.perf_counter()
+        wkb_ref.decode_column(pt_pay, offsets[: ks + 1], valid[:ks])
+        wkb_ref.decode_column(pg_pay, pg_off, valid[n: n + kp])
+        tc = time.perf_counter() - tc0
+        cpu = {"value": (len(pt_pay) + len(pg_pay)) / tc / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"first {ks:,} point records + first {kp:,} polygon records "
+                         f"({(len(pt_pay) + len(pg_pay)) / 1e6:.1f} MB) decoded by the Python WKB restatement in "
+                         f"{tc:.1f} s"}
+
+    out = {
+        "metric": "WKB ingest GB/s of payload decoded to device SoA buffers, 10M point + 100k polygon records",
+        "value": len(payload) / (ms * 1e-3) / 1e9,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8->f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n:,} point records + {m:,} star-polygon records (0-3 holes) as ISO WKB, "
+                               f"{len(payload):,} bytes",
+                   "records_per_s": (n + m) / (ms * 1e-3), "coordinates": int(V),
+                   "scan_ms": scan_ms, "fill_ms": fill_ms},
+        "roofline": {"bound": "hbm", "kernel": "wkb decode on the device (scan + prefix sums + fill)",
+                     "achieved": alg / (device_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (device_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_ms": device_ms, "alg_bytes_per_step_lower_bound": alg},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0

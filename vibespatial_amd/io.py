@@ -29,7 +29,8 @@ import pandas as pd
 
 from .geometry import GeometryArray, PointArray, PolygonArray
 
-__all__ = ["read_file", "read_shapefile_bytes", "read_parquet", "to_parquet", "decode_wkb", "encode_wkb"]
+__all__ = ["read_file", "read_shapefile_bytes", "read_parquet", "to_parquet", "decode_wkb", "decode_wkb_device",
+           "encode_wkb"]
 
 
 def _signed_area2(x: np.ndarray, y: np.ndarray) -> float:
@@ -214,18 +215,90 @@ def encode_wkb(ga: GeometryArray, little_endian: bool = True):
     return payload, offsets, valid
 
 
-def decode_wkb(payload, offsets, validity=None) -> GeometryArray:
-    """GeometryArray from WKB records decoded on the GPU (vs_wkb_scan /
-    vs_wkb_fill): record r is payload[offsets[r]:offsets[r + 1]]; validity 0
-    or an empty record is a null row.  The decoded buffers stay on the device
-    (cached on the arrays) and are also copied to the host arrays."""
+class DeviceWkbColumn:
+    """Decoded WKB column on the device: point x / y with their record
+    numbers, polygon SoA buffers with theirs (None / empty when absent)."""
+
+    def __init__(self, n_rows, points, point_rows, polygons, polygon_rows):
+        self.n_rows = n_rows
+        self.points = points
+        self.point_rows = point_rows
+        self.polygons = polygons
+        self.polygon_rows = polygon_rows
+
+
+def decode_wkb_device(d_payload, d_offsets, d_validity=None, workspace=None, timing=None) -> DeviceWkbColumn:
+    """Decode WKB records already in device memory: d_payload uint8,
+    d_offsets int64 (n + 1), d_validity uint8 (n) or None.  Two passes
+    (vs_wkb_scan, then vs_wkb_fill into buffers sized from its totals); the
+    host reads the eight-word result in between.  ``workspace`` may pass a
+    reusable uint8 device tensor of at least vs_wkb_workspace_bytes(n); a
+    ``timing`` list receives the (start, end) device-event pairs of the scan
+    and of the fill, recorded on the launch stream."""
     import torch
 
     from ._lib import check, lib
-    from .device import DevicePoints, DevicePolygons, ptr, require_gpu, stream
+    from .device import DevicePoints, DevicePolygons, ptr, stream
+
+    L = lib()
+    n = int(d_offsets.numel()) - 1
+    if n < 0:
+        raise ValueError("offsets must hold n + 1 entries")
+    if d_payload.dtype != torch.uint8 or d_offsets.dtype != torch.int64:
+        raise TypeError("payload must be uint8 and offsets int64")
+    if d_validity is not None and (d_validity.dtype != torch.uint8 or d_validity.numel() != n):
+        raise ValueError("validity must be uint8 with one entry per record")
+    dev = d_payload.device
+    need = max(int(L.vs_wkb_workspace_bytes(n)), 1)
+    ws = workspace if workspace is not None and workspace.numel() >= need else \
+        torch.empty(need, dtype=torch.uint8, device=dev)
+    res = torch.empty(8, dtype=torch.int64, device=dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if timing is not None else None
+    if ev:
+        ev[0].record()
+    check(L.vs_wkb_scan(ptr(d_payload), ptr(d_offsets), ptr(d_validity), n, ptr(res), ptr(ws), ws.numel(),
+                        stream()), "vs_wkb_scan")
+    if ev:
+        ev[1].record()
+    r = res.cpu().tolist()
+    if r[5] >= 0:
+        raise ValueError(f"WKB record {r[5]}: {_WKB_ERRORS.get(int(r[6]), 'invalid record')}")
+    if r[7]:
+        raise OverflowError("WKB column holds more than 2^31-1 parts, rings or coordinates")
+    n_pts, n_poly, n_parts, n_rings, n_coords = (int(v) for v in r[:5])
+    f64 = dict(dtype=torch.float64, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    i64 = dict(dtype=torch.int64, device=dev)
+    px, py = torch.empty(max(n_pts, 1), **f64), torch.empty(max(n_pts, 1), **f64)
+    prow = torch.empty(max(n_pts, 1), **i64)
+    x, y = torch.empty(max(n_coords, 1), **f64), torch.empty(max(n_coords, 1), **f64)
+    geom = torch.empty(n_poly + 1, **i32)
+    part = torch.empty(n_parts + 1, **i32)
+    ring = torch.empty(n_rings + 1, **i32)
+    grow = torch.empty(max(n_poly, 1), **i64)
+    if ev:
+        ev[2].record()
+    check(L.vs_wkb_fill(ptr(d_payload), ptr(d_offsets), ptr(d_validity), n, ptr(ws), ws.numel(), ptr(px),
+                        ptr(py), ptr(prow), ptr(x), ptr(y), ptr(geom), ptr(part), ptr(ring), ptr(grow), stream()),
+          "vs_wkb_fill")
+    if ev:
+        ev[3].record()
+        timing.append(((ev[0], ev[1]), (ev[2], ev[3])))
+    points = DevicePoints(px[:n_pts], py[:n_pts]) if n_pts else None
+    polygons = DevicePolygons(x[:n_coords], y[:n_coords], ring, part, geom) if n_poly else None
+    return DeviceWkbColumn(n, points, prow[:n_pts], polygons, grow[:n_poly])
+
+
+def decode_wkb(payload, offsets, validity=None) -> GeometryArray:
+    """GeometryArray from WKB records decoded on the GPU: record r is
+    payload[offsets[r]:offsets[r + 1]]; validity 0 or an empty record is a null
+    row.  The decoded buffers stay on the device (cached on the arrays) and are
+    also copied to the host arrays (io/wkb.py:5049 decode_wkb_owned)."""
+    import torch
+
+    from .device import require_gpu
 
     dev = require_gpu()
-    L = lib()
     off = np.ascontiguousarray(offsets, dtype=np.int64)
     n = len(off) - 1
     if n < 0:
@@ -237,42 +310,21 @@ def decode_wkb(payload, offsets, validity=None) -> GeometryArray:
     d_data = torch.from_numpy(data.copy() if len(data) else np.zeros(1, np.uint8)).to(dev)
     d_off = torch.from_numpy(off).to(dev)
     d_valid = None if validity is None else torch.from_numpy(np.ascontiguousarray(validity, dtype=np.uint8)).to(dev)
-    ws = torch.empty(max(int(L.vs_wkb_workspace_bytes(n)), 1), dtype=torch.uint8, device=dev)
-    res = torch.zeros(8, dtype=torch.int64, device=dev)
-    check(L.vs_wkb_scan(ptr(d_data), ptr(d_off), ptr(d_valid), n, ptr(res), ptr(ws), ws.numel(), stream()),
-          "vs_wkb_scan")
-    r = res.cpu().tolist()
-    if r[5] >= 0:
-        raise ValueError(f"WKB record {r[5]}: {_WKB_ERRORS.get(int(r[6]), 'invalid record')}")
-    if r[7]:
-        raise OverflowError("WKB column holds more than 2^31-1 parts, rings or coordinates")
-    n_pts, n_poly, n_parts, n_rings, n_coords = (int(v) for v in r[:5])
-    f64 = dict(dtype=torch.float64, device=dev)
-    i32 = dict(dtype=torch.int32, device=dev)
-    i64 = dict(dtype=torch.int64, device=dev)
-    px, py, prow = torch.empty(max(n_pts, 1), **f64), torch.empty(max(n_pts, 1), **f64), torch.empty(max(n_pts, 1),
-                                                                                                     **i64)
-    x, y = torch.empty(max(n_coords, 1), **f64), torch.empty(max(n_coords, 1), **f64)
-    geom = torch.empty(n_poly + 1, **i32)
-    part = torch.empty(n_parts + 1, **i32)
-    ring = torch.empty(n_rings + 1, **i32)
-    grow = torch.empty(max(n_poly, 1), **i64)
-    check(L.vs_wkb_fill(ptr(d_data), ptr(d_off), ptr(d_valid), n, ptr(ws), ws.numel(), ptr(px), ptr(py), ptr(prow),
-                        ptr(x), ptr(y), ptr(geom), ptr(part), ptr(ring), ptr(grow), stream()), "vs_wkb_fill")
+    col = decode_wkb_device(d_data, d_off, d_valid)
     tags = np.full(n, -1, dtype=np.int8)
     points = polygons = prows = grows = None
-    if n_pts:
-        px, py = px[:n_pts], py[:n_pts]
-        points = PointArray(px.cpu().numpy(), py.cpu().numpy())
-        points._device = DevicePoints(px, py)
-        prows = prow[:n_pts].cpu().numpy()
+    if col.points is not None:
+        Q = col.points
+        points = PointArray(Q.x.cpu().numpy(), Q.y.cpu().numpy())
+        points._device = Q
+        prows = col.point_rows.cpu().numpy()
         tags[prows] = 0
-    if n_poly:
-        x, y = x[:n_coords], y[:n_coords]
-        polygons = PolygonArray(x.cpu().numpy(), y.cpu().numpy(), ring.cpu().numpy(), part.cpu().numpy(),
-                                geom.cpu().numpy())
-        polygons._device = DevicePolygons(x, y, ring, part, geom)
-        grows = grow[:n_poly].cpu().numpy()
+    if col.polygons is not None:
+        D = col.polygons
+        polygons = PolygonArray(D.x.cpu().numpy(), D.y.cpu().numpy(), D.ring_offsets.cpu().numpy(),
+                                D.part_offsets.cpu().numpy(), D.geom_offsets.cpu().numpy())
+        polygons._device = D
+        grows = col.polygon_rows.cpu().numpy()
         tags[grows] = 1
     return GeometryArray(tags, points, prows, polygons, grows)
 
