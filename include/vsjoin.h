@@ -355,6 +355,47 @@ int vs_wkb_fill(const uint8_t* payload, const int64_t* offsets, const uint8_t* v
                 int32_t* ring_offsets, int64_t* polygon_rows, void* stream);
 
 /* ---------------------------------------------------------------------
+ * Nearest join (sindex.nearest / sjoin_nearest)
+ * --------------------------------------------------------------------- */
+
+int64_t vs_pair_distances_workspace_bytes(int64_t n_pairs);
+
+/* fp64 distance of each candidate pair (kinds 0 points, 1 polygons; geometry
+ * arguments as in vs_refine_dwithin; n_right = rows of the right side):
+ * point-point sqrt(dx^2 + dy^2), point-segment GEOS pointToSegment, 0 when a
+ * point is not outside a polygon or two polygons intersect (exact tests),
+ * else the least vertex-segment distance; +inf for empty / NaN geometries
+ * and, when `exclusive`, for point pairs with equal coordinates.  Replaces
+ * _compute_pair_distances_gpu / _compute_mixed_distances_gpu
+ * (spatial/nearest.py:2228, 2482). */
+int vs_pair_distances(int32_t left_kind, int32_t right_kind, const int64_t* left, const int64_t* right,
+                      int64_t n_pairs, int64_t n_right, int32_t exclusive, const double* lx, const double* ly,
+                      const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring, const double* rx,
+                      const double* ry, const int32_t* r_geom, const int32_t* r_part, const int32_t* r_ring,
+                      double* out_distance, void* workspace, int64_t workspace_bytes, void* stream);
+
+int64_t vs_nearest_select_workspace_bytes(int64_t n_pairs, int64_t n_query);
+
+/* One round of the expanding-radius nearest search.  Pairs (left = query
+ * row < n_query) with their distances; radius[q] (device fp64) is the
+ * query's radius this round (NaN = closed).  A query whose least finite
+ * candidate distance m is <= radius[q] keeps its pairs at distance m (all of
+ * them, or with return_all = 0 the one with the least right row) and closes;
+ * kept pairs are compacted in input order to out_left / out_right /
+ * out_distance (may be null).  radius is updated in place for the next round:
+ * NaN when closed or when radius[q] >= final_radius, else radius * growth
+ * (final_radius once that reaches switch_radius); search[q] = the next radius
+ * * (1 + 2^-40), the amount to grow the query box by.  d_result (device
+ * int64[2]): [0] kept pairs, [1] queries still open.  Replaces
+ * _nearest_from_distances (spatial/nearest.py:3237) with ties as equal fp64
+ * distances. */
+int vs_nearest_select(const int64_t* left, const int64_t* right, const double* distance, int64_t n_pairs,
+                      int64_t n_query, double* radius, double* search, double growth, double switch_radius,
+                      double final_radius, int32_t return_all, int64_t* out_left, int64_t* out_right,
+                      double* out_distance, int64_t* d_result, void* workspace, int64_t workspace_bytes,
+                      void* stream);
+
+/* ---------------------------------------------------------------------
  * Primitives
  * --------------------------------------------------------------------- */
 

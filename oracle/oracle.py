@@ -30,6 +30,9 @@ __all__ = [
     "box_query_points",
     "dwithin_pair",
     "dwithin_pairs",
+    "segment_dist_f64",
+    "distance_f64",
+    "nearest_pairs",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -281,3 +284,107 @@ def dwithin_pairs(left_kind: int, right_kind: int, L, R, distance, left_bounds, 
     keep = np.array([dwithin_pair(left_kind, right_kind, L, R, int(a), int(b), float(d[a]))
                      for a, b in zip(lq, lt)], dtype=bool)
     return lq[keep], lt[keep]
+
+
+# ---------------------------------------------------------------------------
+# nearest (fp64 distances as GEOS computes them; brute force)
+# ---------------------------------------------------------------------------
+def segment_dist_f64(px, py, ax, ay, bx, by):
+    """Distance from (px, py) to segments a-b, elementwise over arrays, by
+    GEOS Distance::pointToSegment (the reference's CPU path through shapely;
+    its GPU kernels, spatial/nearest.py:2228-2482, use the same projection):
+    r = ((p - a).(b - a)) / |b - a|^2; r <= 0 -> |p - a|, r >= 1 -> |p - b|,
+    else |((a.y - p.y)(b.x - a.x) - (a.x - p.x)(b.y - a.y)) / |b - a|^2| *
+    sqrt(|b - a|^2); a degenerate segment is its end point."""
+    px, py = np.float64(px), np.float64(py)
+    ax, ay, bx, by = (np.asarray(v, dtype=np.float64) for v in (ax, ay, bx, by))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        da = np.sqrt((px - ax) * (px - ax) + (py - ay) * (py - ay))
+        db = np.sqrt((px - bx) * (px - bx) + (py - by) * (py - by))
+        ux, uy = bx - ax, by - ay
+        len2 = ux * ux + uy * uy
+        r = ((px - ax) * ux + (py - ay) * uy) / len2
+        s = ((ay - py) * ux - (ax - px) * uy) / len2
+        perp = np.abs(s) * np.sqrt(len2)
+    out = np.where(r <= 0.0, da, np.where(r >= 1.0, db, perp))
+    return np.where((ax == bx) & (ay == by), da, out)
+
+
+def _boundary_dist(px, py, P, row):
+    xs, ys = [], []
+    for b, e in _rings_of(P, row):
+        if e - b >= 2:
+            xs.append((P.x[b:e - 1], P.x[b + 1:e]))
+            ys.append((P.y[b:e - 1], P.y[b + 1:e]))
+    if not xs:
+        return np.inf
+    ax = np.concatenate([a for a, _ in xs])
+    bx = np.concatenate([b for _, b in xs])
+    ay = np.concatenate([a for a, _ in ys])
+    by = np.concatenate([b for _, b in ys])
+    d = segment_dist_f64(px, py, ax, ay, bx, by)
+    d = d[~np.isnan(d)]
+    return float(d.min()) if len(d) else np.inf
+
+
+def _poly_empty(P, row):
+    return not any(e > b for b, e in _rings_of(P, row))
+
+
+def distance_f64(left_kind, right_kind, L, R, li, ri, exclusive=False) -> float:
+    """fp64 distance of one pair (kinds: 0 points, 1 polygons), +inf for
+    empty / NaN geometries and, with `exclusive`, equal points; 0 when a point
+    is not outside a polygon (C restatement of the exact location) or two
+    polygons intersect (C restatement), else the least vertex - segment
+    distance over both directions."""
+    if left_kind == 1 and right_kind == 0:
+        return distance_f64(0, 1, R, L, ri, li, exclusive)
+    if left_kind == 0:
+        px, py = float(L.x[li]), float(L.y[li])
+        if px != px or py != py:
+            return np.inf
+        if right_kind == 0:
+            qx, qy = float(R.x[ri]), float(R.y[ri])
+            if qx != qx or qy != qy or (exclusive and px == qx and py == qy):
+                return np.inf
+            dx, dy = np.float64(px) - np.float64(qx), np.float64(py) - np.float64(qy)
+            return float(np.sqrt(dx * dx + dy * dy))
+        if _poly_empty(R, ri):
+            return np.inf
+        if point_location(px, py, R, ri) != 0:
+            return 0.0
+        return _boundary_dist(px, py, R, ri)
+    if _poly_empty(L, li) or _poly_empty(R, ri):
+        return np.inf
+    if polygon_intersects(L, li, R, ri):
+        return 0.0
+    best = np.inf
+    for P, rp, Q, rq in ((L, li, R, ri), (R, ri, L, li)):
+        for b, e in _rings_of(P, rp):
+            for v in range(b, e):
+                best = min(best, _boundary_dist(float(P.x[v]), float(P.y[v]), Q, rq))
+    return best
+
+
+def nearest_pairs(left_kind, right_kind, L, R, max_distance=None, return_all=True, exclusive=False):
+    """Brute-force nearest: for every left row, the right rows at the least
+    finite fp64 distance (<= max_distance when given); ties all kept, or the
+    least right row when not return_all.  (left, right, distance) lexsorted."""
+    nl = len(L.x) if left_kind == 0 else len(L)
+    nr = len(R.x) if right_kind == 0 else len(R)
+    out_l, out_r, out_d = [], [], []
+    for i in range(nl):
+        d = np.array([distance_f64(left_kind, right_kind, L, R, i, j, exclusive) for j in range(nr)])
+        fin = np.isfinite(d)
+        if not fin.any():
+            continue
+        m = d[fin].min()
+        if max_distance is not None and not (m <= max_distance):
+            continue
+        js = np.flatnonzero(d == m)
+        if not return_all:
+            js = js[:1]
+        out_l += [i] * len(js)
+        out_r += js.tolist()
+        out_d += [m] * len(js)
+    return (np.array(out_l, dtype=np.int64), np.array(out_r, dtype=np.int64), np.array(out_d, dtype=np.float64))

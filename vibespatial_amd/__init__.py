@@ -22,7 +22,7 @@ __version__ = "0.1.0"
 
 def __getattr__(name):
     # the GPU-facing modules import torch and the library lazily
-    if name in ("sjoin", "GeoSeries", "GeoDataFrame", "SpatialIndex", "PREDICATES"):
+    if name in ("sjoin", "sjoin_nearest", "GeoSeries", "GeoDataFrame", "SpatialIndex", "PREDICATES"):
         from . import api
 
         return getattr(api, name)

@@ -285,6 +285,80 @@ class SpatialIndex:
 
         return sparse.coo_array(dense)
 
+    def _nearest_family(self, qf, qa, tf, ta, max_distance, return_all, exclusive):
+        """(query family idx, tree family idx, distance) device tensors."""
+        qb = J.point_boxes(device_points(qa)) if qf == "point" else J.polygon_bounds(device_polygons(qa))
+        tb_h, qb_h = ta.bounds(), qa.bounds()
+        fin = np.isfinite(tb_h).all(axis=1)
+        if not fin.any():
+            return None
+        both = np.concatenate([tb_h[fin], qb_h[np.isfinite(qb_h).all(axis=1)]])
+        x0, y0 = both[:, 0].min(), both[:, 1].min()
+        x1, y1 = both[:, 2].max(), both[:, 3].max()
+        # a radius at which every grown query box covers every tree box
+        cover = float(np.hypot(x1 - x0, y1 - y0)) * 1.01 + 1e-300
+        if not np.isfinite(cover):
+            cover = float(np.finfo(np.float64).max)
+        t0, t1 = tb_h[fin].min(axis=0), tb_h[fin].max(axis=0)
+        w, h = t1[2] - t0[0], t1[3] - t0[1]
+        n_t = int(fin.sum())
+        # the mean spacing of the tree rows as the first radius
+        start = float(np.sqrt(w * h / n_t)) if w > 0 and h > 0 else max(w, h) / n_t
+        if not (start > 0) or not np.isfinite(start):
+            start = cover / 1024
+        return J.nearest_pairs(qb, _dev_geom(qf, qa), self._index.tree(tf), _dev_geom(tf, ta),
+                               max_distance=max_distance, return_all=return_all, exclusive=exclusive,
+                               start_radius=start, cover_radius=cover)
+
+    def nearest(self, geometry, return_all=True, max_distance=None, return_distance=False, exclusive=False):
+        """Nearest tree rows for each input geometry (api/sindex.py:2018-2277):
+        (2, n) indices lexsorted by input row, then tree row, and the fp64
+        distances when `return_distance`.  Ties (equal distances) are all
+        returned unless not `return_all` (then the least tree row).  Columns
+        holding both points and polygons are searched per family pair and the
+        per-row minimum taken over the families."""
+        q = as_geometry_array(geometry)
+        if max_distance is not None and not (float(max_distance) >= 0):
+            raise ValueError("max_distance must be a non-negative number")
+        fams_q = [(f, a, rows) for f, a, rows in (("point", q.points, q.point_rows),
+                                                   ("polygon", q.polygons, q.polygon_rows)) if a is not None]
+        fams_t = [(f, a, rows) for f, a, rows in (("point", self._ga.points, self._ga.point_rows),
+                                                   ("polygon", self._ga.polygons, self._ga.polygon_rows))
+                  if a is not None]
+        if exclusive and any(qf == "polygon" for qf, _, _ in fams_q) and any(tf == "polygon" for tf, _, _ in fams_t):
+            raise NotImplementedError("exclusive nearest between polygons (geometric equality)")
+        parts = []
+        for qf, qa, qrows in fams_q:
+            for tf, ta, trows in fams_t:
+                # per family pair all ties are kept; the selection across
+                # families below applies return_all
+                res = self._nearest_family(qf, qa, tf, ta, max_distance, return_all or len(fams_t) > 1, exclusive)
+                if res is None:
+                    continue
+                l, r, d = (t.cpu().numpy() for t in res)
+                parts.append((qrows[l], trows[r], d))
+        if parts:
+            lh = np.concatenate([p[0] for p in parts]).astype(np.int64)
+            rh = np.concatenate([p[1] for p in parts]).astype(np.int64)
+            dh = np.concatenate([p[2] for p in parts])
+        else:
+            lh = rh = np.empty(0, dtype=np.int64)
+            dh = np.empty(0, dtype=np.float64)
+        if len(fams_t) > 1 and len(lh):
+            o = np.lexsort((rh, dh, lh))
+            lh, rh, dh = lh[o], rh[o], dh[o]
+            first = np.r_[True, lh[1:] != lh[:-1]]
+            m = np.maximum.accumulate(np.where(first, np.arange(len(lh)), 0))  # index of each row's first pair
+            keep = dh == dh[m]
+            if not return_all:
+                keep &= first
+            lh, rh, dh = lh[keep], rh[keep], dh[keep]
+        o = np.lexsort((rh, lh))
+        idx = np.vstack([lh[o], rh[o]])
+        if return_distance:
+            return idx, dh[o]
+        return idx
+
     def intersection(self, coordinates):
         """Tree rows whose bbox intersects a bounds tuple or a point (x, y)."""
         if not isinstance(coordinates, tuple) or len(coordinates) not in (2, 4):
@@ -394,6 +468,11 @@ class GeoDataFrame:
             return self._geometry
         return self._df[col]
 
+    def sjoin_nearest(self, right, how="inner", max_distance=None, lsuffix="left", rsuffix="right",
+                      distance_col=None, exclusive=False):
+        return sjoin_nearest(self, right, how=how, max_distance=max_distance, lsuffix=lsuffix, rsuffix=rsuffix,
+                             distance_col=distance_col, exclusive=exclusive)
+
     def to_parquet(self, path, little_endian: bool = True) -> None:
         """GeoParquet 1.0 with a WKB geometry column."""
         from .io import to_parquet
@@ -487,7 +566,28 @@ def sjoin(left_df, right_df, how="inner", predicate="intersects", lsuffix="left"
     return _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribute)
 
 
-def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribute):
+def sjoin_nearest(left_df, right_df, how="inner", max_distance=None, lsuffix="left", rsuffix="right",
+                  distance_col=None, exclusive=False):
+    """Join on the nearest geometry (api/tools/sjoin.py:1398-1560): every
+    equidistant nearest right row; how='right' queries the right geometries
+    against the left index (api/tools/sjoin.py:1173-1212)."""
+    _basic_checks(left_df, right_df, how, lsuffix, rsuffix)
+    if how == "right":
+        res = left_df.sindex.nearest(right_df.geometry.values, return_all=True, max_distance=max_distance,
+                                     return_distance=True, exclusive=exclusive)
+        (input_idx, tree_idx), dist = res
+        l_idx, r_idx = tree_idx, input_idx
+        o = np.argsort(l_idx, kind="stable")
+        l_idx, r_idx, dist = l_idx[o], r_idx[o], dist[o]
+    else:
+        (l_idx, r_idx), dist = right_df.sindex.nearest(left_df.geometry.values, return_all=True,
+                                                        max_distance=max_distance, return_distance=True,
+                                                        exclusive=exclusive)
+    extra = {distance_col: dist} if distance_col is not None else None
+    return _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, None, extra)
+
+
+def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribute, extra=None):
     """GeoPandas-shaped frame for the join (api/_native_results.py:301-472)."""
     ldf, rdf = left_df._df, right_df._df
     on_attribute = on_attribute or []
@@ -499,6 +599,7 @@ def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribu
     lren = {c: f"{c}_{lsuffix}" for c in overlap if c in lcols}
     rren = {c: f"{c}_{rsuffix}" for c in overlap}
     if how == "right":
+        l_idx0, r_idx0 = l_idx, r_idx
         l_idx, r_idx = l_idx.copy(), r_idx.copy()
         o = np.lexsort((l_idx, r_idx))
         l_idx, r_idx = l_idx[o], r_idx[o]
@@ -518,10 +619,15 @@ def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribu
             out[lren.get(c, c)] = left_part[lren.get(c, c)].to_numpy()
         for c in rcols:
             out[rren.get(c, c)] = rdf[c].to_numpy()[r_all]
+        for name, vals in (extra or {}).items():
+            ov = np.full(len(r_all), np.nan)
+            ov[: len(l_idx)] = np.asarray(vals, dtype=np.float64)[np.lexsort((l_idx0, r_idx0))]
+            out[name] = ov[o]
         geoms = right_df.geometry.take(r_all)
         g = GeoDataFrame(out, geometry=geoms, crs=right_df.crs)
         g._geom_name = right_df._geom_name
         return g
+    extra_cols = {}
     if how in ("left", "outer"):
         matched = np.zeros(len(ldf), dtype=bool)
         matched[l_idx] = True
@@ -530,8 +636,12 @@ def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribu
         r_all = np.concatenate([r_idx, np.full(len(miss), -1)])
         o = np.argsort(l_all, kind="stable")
         l_all, r_all = l_all[o], r_all[o]
+        for name, vals in (extra or {}).items():
+            extra_cols[name] = np.concatenate([np.asarray(vals, dtype=np.float64), np.full(len(miss), np.nan)])[o]
     else:
         l_all, r_all = l_idx, r_idx
+        for name, vals in (extra or {}).items():
+            extra_cols[name] = np.asarray(vals, dtype=np.float64)
     out = pd.DataFrame(index=ldf.index[l_all] if len(ldf.index) else None)
     for c in lcols:
         out[lren.get(c, c)] = ldf[c].to_numpy()[l_all]
@@ -546,6 +656,8 @@ def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribu
             vals = vals.astype(object) if vals.dtype.kind not in "fc" else vals.astype(float)
             vals[~valid] = np.nan
         out[rren.get(c, c)] = vals
+    for name, vals in extra_cols.items():
+        out[name] = vals
     geoms = left_df.geometry.take(l_all)
     g = GeoDataFrame(out, geometry=geoms, crs=left_df.crs)
     g._geom_name = left_df._geom_name

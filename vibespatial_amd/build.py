@@ -22,7 +22,7 @@ ARCH = os.environ.get("VSJOIN_ARCH", "gfx950")
 
 SOURCES = [
     "vs_api.hip", "vs_sort.hip", "vs_index.hip", "vs_join.hip", "vs_finish.hip", "vs_refine.hip", "vs_learned.hip",
-    "vs_batch.hip", "vs_wkb.hip",
+    "vs_batch.hip", "vs_wkb.hip", "vs_nearest.hip",
 ]
 
 FLAGS = [

@@ -121,13 +121,8 @@ def refine_dwithin(left: torch.Tensor, right: torch.Tensor, distance: torch.Tens
         e = torch.empty(0, dtype=torch.int64, device=dev)
         return e, e.clone()
 
-    def kind_args(g):
-        if isinstance(g, DevicePoints):
-            return 0, [ptr(g.x), ptr(g.y), ptr(None), ptr(None), ptr(None)]
-        return 1, g.args()
-
-    lk, la = kind_args(lgeom)
-    rk, ra = kind_args(rgeom)
+    lk, la = _kind_args(lgeom)
+    rk, ra = _kind_args(rgeom)
     d = distance.to(device=dev, dtype=torch.float64).contiguous()
     out_l = torch.empty(n, dtype=torch.int64, device=dev)
     out_r = torch.empty(n, dtype=torch.int64, device=dev)
@@ -138,6 +133,86 @@ def refine_dwithin(left: torch.Tensor, right: torch.Tensor, distance: torch.Tens
           "vs_refine_dwithin")
     k = int(res[0].item())
     return out_l[:k], out_r[:k]
+
+
+def lexsort_pairs(left: torch.Tensor, right: torch.Tensor):
+    """(left, right) int64 pairs sorted by left, then right (vs_lexsort_pairs,
+    in place on copies)."""
+    dev = require_gpu()
+    L = lib()
+    l, r = left.clone(), right.clone()
+    n = int(l.numel())
+    if n > 1:
+        ws = _ws(L.vs_lexsort_pairs_workspace_bytes(n), dev)
+        check(L.vs_lexsort_pairs(ptr(l), ptr(r), n, ptr(ws), ws.numel(), stream()), "vs_lexsort_pairs")
+    return l, r
+
+
+def _kind_args(g):
+    if isinstance(g, DevicePoints):
+        return 0, [ptr(g.x), ptr(g.y), ptr(None), ptr(None), ptr(None)]
+    return 1, g.args()
+
+
+def pair_distances(left: torch.Tensor, right: torch.Tensor, lgeom, rgeom, exclusive: bool = False):
+    """fp64 distance of each (left, right) pair (vs_pair_distances); +inf for
+    empty / NaN geometries and, with `exclusive`, for equal point pairs."""
+    dev = require_gpu()
+    L = lib()
+    n = int(left.numel())
+    out = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+    if n:
+        lk, la = _kind_args(lgeom)
+        rk, ra = _kind_args(rgeom)
+        ws = _ws(L.vs_pair_distances_workspace_bytes(n), dev)
+        check(L.vs_pair_distances(lk, rk, ptr(left.contiguous()), ptr(right.contiguous()), n, len(rgeom),
+                                  int(bool(exclusive)), *la, *ra, ptr(out), ptr(ws), ws.numel(), stream()),
+              "vs_pair_distances")
+    return out[:n]
+
+
+def nearest_pairs(qboxes: torch.Tensor, qgeom, tree: DeviceTree, tgeom, *, max_distance=None, return_all=True,
+                  exclusive=False, start_radius: float, cover_radius: float, growth: float = 4.0):
+    """Lexsorted (left, right, distance) of each query's nearest tree rows
+    (ties kept unless not `return_all`), by an expanding-radius search: every
+    open query's box grown by its radius -> box query -> pair distances ->
+    vs_nearest_select.  `cover_radius` is a radius at which a grown query box
+    covers every tree box; without `max_distance` the last round uses it."""
+    dev = require_gpu()
+    L = lib()
+    nq = int(qboxes.shape[0])
+    final = float(max_distance) if max_distance is not None else float(cover_radius)
+    r0 = min(float(start_radius), final)
+    if not (r0 > 0):
+        r0 = final
+    radius = torch.full((nq,), r0, dtype=torch.float64, device=dev)
+    search = torch.full((nq,), r0 * (1.0 + 2.0 ** -40), dtype=torch.float64, device=dev)
+    no_box = ~torch.isfinite(qboxes).all(dim=1)  # empty / NaN query rows are closed from the start
+    radius[no_box] = float("nan")
+    search[no_box] = float("nan")
+    outs_l, outs_r = [], []
+    res = torch.zeros(2, dtype=torch.int64, device=dev)
+    for _ in range(200):
+        l, r, _ = query_boxes(expand_boxes(qboxes, search), tree)
+        d = pair_distances(l, r, qgeom, tgeom, exclusive)
+        n = int(l.numel())
+        ol = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+        orr = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+        ws = _ws(L.vs_nearest_select_workspace_bytes(n, nq), dev)
+        check(L.vs_nearest_select(ptr(l), ptr(r), ptr(d), n, nq, ptr(radius), ptr(search), float(growth), final, final,
+                                  int(bool(return_all)), ptr(ol), ptr(orr), ptr(None), ptr(res), ptr(ws), ws.numel(),
+                                  stream()), "vs_nearest_select")
+        k, still_open = (int(v) for v in res.cpu().tolist())
+        outs_l.append(ol[:k])
+        outs_r.append(orr[:k])
+        if still_open == 0:
+            break
+    else:
+        raise RuntimeError("nearest search did not close every query")
+    lo = torch.cat(outs_l) if outs_l else torch.empty(0, dtype=torch.int64, device=dev)
+    ro = torch.cat(outs_r) if outs_r else torch.empty(0, dtype=torch.int64, device=dev)
+    lo, ro = lexsort_pairs(lo.contiguous(), ro.contiguous())
+    return lo, ro, pair_distances(lo, ro, qgeom, tgeom, exclusive)
 
 
 def _read_result(res: torch.Tensor):
