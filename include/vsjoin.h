@@ -384,16 +384,60 @@ int64_t vs_nearest_select_workspace_bytes(int64_t n_pairs, int64_t n_query);
  * kept pairs are compacted in input order to out_left / out_right /
  * out_distance (may be null).  radius is updated in place for the next round:
  * NaN when closed or when radius[q] >= final_radius, else radius * growth
- * (final_radius once that reaches switch_radius); search[q] = the next radius
- * * (1 + 2^-40), the amount to grow the query box by.  d_result (device
+ * (start_radius after a radius-0 round; final_radius once that reaches
+ * switch_radius); search[q] = the next radius times (1 + 2^-40), the
+ * amount to grow the query box by.  d_result (device
  * int64[2]): [0] kept pairs, [1] queries still open.  Replaces
  * _nearest_from_distances (spatial/nearest.py:3237) with ties as equal fp64
  * distances. */
 int vs_nearest_select(const int64_t* left, const int64_t* right, const double* distance, int64_t n_pairs,
-                      int64_t n_query, double* radius, double* search, double growth, double switch_radius,
-                      double final_radius, int32_t return_all, int64_t* out_left, int64_t* out_right,
-                      double* out_distance, int64_t* d_result, void* workspace, int64_t workspace_bytes,
-                      void* stream);
+                      int64_t n_query, double* radius, double* search, double growth, double start_radius,
+                      double switch_radius, double final_radius, int32_t return_all, int64_t* out_left,
+                      int64_t* out_right, double* out_distance, int64_t* d_result, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+
+int64_t vs_nearest_open_workspace_bytes(int64_t n_query);
+
+/* The open queries of the next round (search[q] not NaN), in row order:
+ * open_rows (int64) and their boxes (n x 4 fp64) grown by search[q] with
+ * every bound rounded one ulp outward; d_count (device int64[1]) = how
+ * many. */
+int vs_nearest_open(const double* boxes, const double* search, int64_t n_query, int64_t* open_rows,
+                    double* open_boxes, int64_t* d_count, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* For each listed query row q (rows, n_rows; query_boxes n x 4 over all
+ * rows): an upper bound of its nearest distance to the tree's geometries --
+ * the least, over the items, of the farthest-point distance from the query
+ * box to the item box or to the item's rep point, by a nearest-first
+ * depth-first walk of the tree pruned by the bound so far -- sets
+ * radius[q] = min(bound, cap) and search[q] = radius times (1 + 2^-40);
+ * rows without a finite bound keep their radius.  rep: vs_nearest_rep_points
+ * of the tree's items (tightens the bound).  row_boxes (n_rows x 4, e.g. the
+ * open boxes of vs_nearest_open) only order the rows (Hilbert order of their
+ * centres, so a wave's lanes walk nearby parts of the tree). */
+int64_t vs_nearest_bound_workspace_bytes(int64_t n_rows);
+int vs_nearest_bound(const double* query_boxes, const int64_t* rows, int64_t n_rows, const double* row_boxes,
+                     const vs_tree_layout* layout, const double* node_boxes, const double* rep, double cap,
+                     double* radius, double* search, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* rep (n_items x 2 fp64, per item slot of a tree): a point of item
+ * item_ids[slot] -- the point (kind 0: x, y per row) or the first coordinate
+ * of the polygonal row (kind 1; NaN when empty). */
+int vs_nearest_rep_points(int32_t kind, const int32_t* item_ids, int64_t n_items, const double* x, const double* y,
+                          const int32_t* geom, const int32_t* part, const int32_t* ring, double* rep, void* stream);
+
+int64_t vs_nearest_finish_workspace_bytes(int64_t n_pairs);
+
+/* The rounds' kept pairs (each query's pairs from one round, sorted by right
+ * row there) -> out_* sorted by left row (stable radix sort, left < n_query),
+ * distances carried along: the lexsorted result. */
+int vs_nearest_finish(const int64_t* left, const int64_t* right, const double* distance, int64_t n_pairs,
+                      int64_t n_query, int64_t* out_left, int64_t* out_right, double* out_distance, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+
+/* left[i] = rows[left[i]] (pairs of an open-query subset back to query rows;
+ * rows ascending keeps the order). */
+int vs_map_left(const int64_t* rows, int64_t* left, int64_t n_pairs, void* stream);
 
 /* ---------------------------------------------------------------------
  * Primitives

@@ -107,7 +107,16 @@ SIGNATURES = {
     "vs_pair_distances_workspace_bytes": (_i64, [_i64]),
     "vs_pair_distances": (_i32, [_i32, _i32, _p, _p, _i64, _i64, _i32] + [_p] * 10 + [_p, _p, _i64, _p]),
     "vs_nearest_select_workspace_bytes": (_i64, [_i64, _i64]),
-    "vs_nearest_select": (_i32, [_p, _p, _p, _i64, _i64, _p, _p, _d, _d, _d, _i32, _p, _p, _p, _p, _p, _i64, _p]),
+    "vs_nearest_select": (_i32, [_p, _p, _p, _i64, _i64, _p, _p, _d, _d, _d, _d, _i32, _p, _p, _p, _p, _p, _i64,
+                                 _p]),
+    "vs_nearest_open_workspace_bytes": (_i64, [_i64]),
+    "vs_nearest_open": (_i32, [_p, _p, _i64, _p, _p, _p, _p, _i64, _p]),
+    "vs_map_left": (_i32, [_p, _p, _i64, _p]),
+    "vs_nearest_finish_workspace_bytes": (_i64, [_i64]),
+    "vs_nearest_finish": (_i32, [_p, _p, _p, _i64, _i64, _p, _p, _p, _p, _i64, _p]),
+    "vs_nearest_bound_workspace_bytes": (_i64, [_i64]),
+    "vs_nearest_bound": (_i32, [_p, _p, _i64, _p, _LP, _p, _p, _d, _p, _p, _p, _i64, _p]),
+    "vs_nearest_rep_points": (_i32, [_i32, _p, _i64, _p, _p, _p, _p, _p, _p, _p]),
     "vs_scan_workspace_bytes": (_i64, [_i64]),
     "vs_exclusive_scan_i32": (ctypes.c_int, [_p, _p, _i64, _p, _p, _i64, _p]),
     "vs_sort_workspace_bytes": (_i64, [_i64, _i32]),

@@ -174,10 +174,14 @@ def pair_distances(left: torch.Tensor, right: torch.Tensor, lgeom, rgeom, exclus
 def nearest_pairs(qboxes: torch.Tensor, qgeom, tree: DeviceTree, tgeom, *, max_distance=None, return_all=True,
                   exclusive=False, start_radius: float, cover_radius: float, growth: float = 4.0):
     """Lexsorted (left, right, distance) of each query's nearest tree rows
-    (ties kept unless not `return_all`), by an expanding-radius search: every
-    open query's box grown by its radius -> box query -> pair distances ->
-    vs_nearest_select.  `cover_radius` is a radius at which a grown query box
-    covers every tree box; without `max_distance` the last round uses it."""
+    (ties kept unless not `return_all`), by an expanding-radius search.
+    Points against polygons first take a radius-0 round: the fused point join
+    with predicate intersects gives every pair at distance 0.  Then each round
+    takes the open queries (vs_nearest_open: rows and boxes grown by their
+    search radius) -> box query -> rows mapped back (vs_map_left) -> pair
+    distances -> vs_nearest_select (closes queries, grows the others' radius).
+    `cover_radius` is a radius at which a grown query box covers every tree
+    box; without `max_distance` the last round uses it."""
     dev = require_gpu()
     L = lib()
     nq = int(qboxes.shape[0])
@@ -185,34 +189,80 @@ def nearest_pairs(qboxes: torch.Tensor, qgeom, tree: DeviceTree, tgeom, *, max_d
     r0 = min(float(start_radius), final)
     if not (r0 > 0):
         r0 = final
-    radius = torch.full((nq,), r0, dtype=torch.float64, device=dev)
-    search = torch.full((nq,), r0 * (1.0 + 2.0 ** -40), dtype=torch.float64, device=dev)
     no_box = ~torch.isfinite(qboxes).all(dim=1)  # empty / NaN query rows are closed from the start
+    pip = isinstance(qgeom, DevicePoints) and isinstance(tgeom, DevicePolygons)
+    first = 0.0 if pip else r0
+    radius = torch.full((nq,), first, dtype=torch.float64, device=dev)
+    search = torch.full((nq,), first * (1.0 + 2.0 ** -40), dtype=torch.float64, device=dev)
     radius[no_box] = float("nan")
     search[no_box] = float("nan")
+    qboxes = qboxes.contiguous()
     outs_l, outs_r = [], []
     res = torch.zeros(2, dtype=torch.int64, device=dev)
-    for _ in range(200):
-        l, r, _ = query_boxes(expand_boxes(qboxes, search), tree)
-        d = pair_distances(l, r, qgeom, tgeom, exclusive)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    open_rows = torch.empty(max(nq, 1), dtype=torch.int64, device=dev)
+    open_boxes = torch.empty((max(nq, 1), 4), dtype=torch.float64, device=dev)
+    open_ws = _ws(L.vs_nearest_open_workspace_bytes(nq), dev)
+
+    outs_d = []
+
+    def select(l, r, d):
         n = int(l.numel())
         ol = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
         orr = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+        od = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
         ws = _ws(L.vs_nearest_select_workspace_bytes(n, nq), dev)
-        check(L.vs_nearest_select(ptr(l), ptr(r), ptr(d), n, nq, ptr(radius), ptr(search), float(growth), final, final,
-                                  int(bool(return_all)), ptr(ol), ptr(orr), ptr(None), ptr(res), ptr(ws), ws.numel(),
-                                  stream()), "vs_nearest_select")
+        check(L.vs_nearest_select(ptr(l), ptr(r), ptr(d), n, nq, ptr(radius), ptr(search), float(growth), r0, final,
+                                  final, int(bool(return_all)), ptr(ol), ptr(orr), ptr(od), ptr(res), ptr(ws),
+                                  ws.numel(), stream()), "vs_nearest_select")
         k, still_open = (int(v) for v in res.cpu().tolist())
         outs_l.append(ol[:k])
         outs_r.append(orr[:k])
+        outs_d.append(od[:k])
+        return still_open
+
+    def open_queries():
+        check(L.vs_nearest_open(ptr(qboxes), ptr(search), nq, ptr(open_rows), ptr(open_boxes), ptr(cnt),
+                                ptr(open_ws), open_ws.numel(), stream()), "vs_nearest_open")
+        return int(cnt.item())
+
+    still_open = nq
+    if pip:
+        # radius 0: the pairs with the point not outside the polygon
+        l, r, _ = sjoin_points_polygons(qgeom, tgeom, tree, "intersects")
+        still_open = select(l, r, torch.zeros(max(int(l.numel()), 1), dtype=torch.float64, device=dev))
+    if still_open:
+        # each open query's radius from a descent of the tree (an upper bound
+        # of its nearest distance), so the next round closes nearly all
+        m = open_queries()
+        rep = torch.empty((max(tree.n_items, 1), 2), dtype=torch.float64, device=dev)
+        tk, ta = _kind_args(tgeom)
+        check(L.vs_nearest_rep_points(tk, ptr(tree.item_ids), tree.n_items, *ta, ptr(rep), stream()),
+              "vs_nearest_rep_points")
+        bws = _ws(L.vs_nearest_bound_workspace_bytes(m), dev)
+        check(L.vs_nearest_bound(ptr(qboxes), ptr(open_rows), m, ptr(open_boxes), ctypes.byref(tree.layout),
+                                 ptr(tree.node_boxes), ptr(rep), final, ptr(radius), ptr(search), ptr(bws), bws.numel(),
+                                 stream()), "vs_nearest_bound")
+    for _ in range(200):
         if still_open == 0:
             break
+        m = open_queries()
+        l, r, _ = query_boxes(open_boxes[:m], tree)
+        check(L.vs_map_left(ptr(open_rows), ptr(l), int(l.numel()), stream()), "vs_map_left")
+        still_open = select(l, r, pair_distances(l, r, qgeom, tgeom, exclusive))
     else:
         raise RuntimeError("nearest search did not close every query")
     lo = torch.cat(outs_l) if outs_l else torch.empty(0, dtype=torch.int64, device=dev)
     ro = torch.cat(outs_r) if outs_r else torch.empty(0, dtype=torch.int64, device=dev)
-    lo, ro = lexsort_pairs(lo.contiguous(), ro.contiguous())
-    return lo, ro, pair_distances(lo, ro, qgeom, tgeom, exclusive)
+    do = torch.cat(outs_d) if outs_d else torch.empty(0, dtype=torch.float64, device=dev)
+    n = int(lo.numel())
+    fl = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    fr = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    fd = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+    ws = _ws(L.vs_nearest_finish_workspace_bytes(n), dev)
+    check(L.vs_nearest_finish(ptr(lo), ptr(ro), ptr(do), n, nq, ptr(fl), ptr(fr), ptr(fd), ptr(ws), ws.numel(),
+                              stream()), "vs_nearest_finish")
+    return fl[:n], fr[:n], fd[:n]
 
 
 def _read_result(res: torch.Tensor):
