@@ -396,6 +396,11 @@ int vs_nearest_select(const int64_t* left, const int64_t* right, const double* d
                       int64_t* out_right, double* out_distance, int64_t* d_result, void* workspace,
                       int64_t workspace_bytes, void* stream);
 
+/* radius[q] = first_radius and search[q] = first_radius times (1 + 2^-40)
+ * for every query with a finite box (n_query x 4), NaN (closed) otherwise. */
+int vs_nearest_init(const double* boxes, int64_t n_query, double first_radius, double* radius, double* search,
+                    void* stream);
+
 int64_t vs_nearest_open_workspace_bytes(int64_t n_query);
 
 /* The open queries of the next round (search[q] not NaN), in row order:

@@ -109,6 +109,7 @@ SIGNATURES = {
     "vs_nearest_select_workspace_bytes": (_i64, [_i64, _i64]),
     "vs_nearest_select": (_i32, [_p, _p, _p, _i64, _i64, _p, _p, _d, _d, _d, _d, _i32, _p, _p, _p, _p, _p, _i64,
                                  _p]),
+    "vs_nearest_init": (_i32, [_p, _i64, _d, _p, _p, _p]),
     "vs_nearest_open_workspace_bytes": (_i64, [_i64]),
     "vs_nearest_open": (_i32, [_p, _p, _i64, _p, _p, _p, _p, _i64, _p]),
     "vs_map_left": (_i32, [_p, _p, _i64, _p]),

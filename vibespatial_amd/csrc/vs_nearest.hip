@@ -271,6 +271,19 @@ __global__ void nearest_result_kernel(const long long* __restrict__ total, const
     }
 }
 
+// First radius of every query: r0, or NaN (closed, never matches) for a
+// query whose box is not finite (empty geometry, NaN point).
+__global__ __launch_bounds__(256) void nearest_init_radius_kernel(const double4* __restrict__ boxes, int64_t n,
+                                                                 double r0, double* __restrict__ radius,
+                                                                 double* __restrict__ search) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const double4 b = boxes[q];
+    const bool ok = isfinite(b.x) && isfinite(b.y) && isfinite(b.z) && isfinite(b.w);
+    radius[q] = ok ? r0 : NAN;
+    search[q] = ok ? r0 * kSearchSlack : NAN;
+}
+
 // Open queries (search radius not NaN): their rows, and their boxes grown by
 // the search radius with every bound rounded one ulp outward.
 __global__ __launch_bounds__(256) void open_flags_kernel(const double* __restrict__ search, int64_t n,
@@ -528,6 +541,19 @@ int vs_pair_distances(int32_t left_kind, int32_t right_kind, const int64_t* left
     const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
     hipLaunchKernelGGL(vs::pair_distances_kernel, dim3(grid), dim3(256), 0, s, left_kind, right_kind, left, right,
                        n_pairs, ord, exclusive ? 1 : 0, L, R, out_distance);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int vs_nearest_init(const double* boxes, int64_t n_query, double first_radius, double* radius, double* search,
+                    void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n_query >= 0, "vs_nearest_init: n_query < 0");
+    if (n_query == 0) return VS_OK;
+    VS_REQUIRE(boxes && radius && search, "vs_nearest_init: null pointer");
+    hipLaunchKernelGGL(vs::nearest_init_radius_kernel, dim3((unsigned)vs::ceil_div(n_query, 256)), dim3(256), 0,
+                       vs::stream_of(stream), reinterpret_cast<const double4*>(boxes), n_query, first_radius, radius,
+                       search);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }

@@ -189,14 +189,13 @@ def nearest_pairs(qboxes: torch.Tensor, qgeom, tree: DeviceTree, tgeom, *, max_d
     r0 = min(float(start_radius), final)
     if not (r0 > 0):
         r0 = final
-    no_box = ~torch.isfinite(qboxes).all(dim=1)  # empty / NaN query rows are closed from the start
     pip = isinstance(qgeom, DevicePoints) and isinstance(tgeom, DevicePolygons)
-    first = 0.0 if pip else r0
-    radius = torch.full((nq,), first, dtype=torch.float64, device=dev)
-    search = torch.full((nq,), first * (1.0 + 2.0 ** -40), dtype=torch.float64, device=dev)
-    radius[no_box] = float("nan")
-    search[no_box] = float("nan")
     qboxes = qboxes.contiguous()
+    radius = torch.empty(max(nq, 1), dtype=torch.float64, device=dev)
+    search = torch.empty(max(nq, 1), dtype=torch.float64, device=dev)
+    # empty / NaN query rows are closed from the start
+    check(L.vs_nearest_init(ptr(qboxes), nq, 0.0 if pip else r0, ptr(radius), ptr(search), stream()),
+          "vs_nearest_init")
     outs_l, outs_r = [], []
     res = torch.zeros(2, dtype=torch.int64, device=dev)
     cnt = torch.zeros(1, dtype=torch.int64, device=dev)
