@@ -58,6 +58,10 @@ extern "C" {
 #define VS_PRED_TOUCHES 3    /* location == 1 */
 #define VS_PRED_CONTAINS 4   /* polygon (left) contains point (right): location == 2 */
 #define VS_PRED_COVERS 5     /* polygon (left) covers point (right): location != 0 */
+/* polygon x polygon only (vs_refine_pairs mode 1): */
+#define VS_PRED_OVERLAPS 6
+#define VS_PRED_CROSSES 7 /* never true for two areal rows */
+#define VS_PRED_CONTAINS_PROPERLY 8
 
 /* Library version (major * 10000 + minor * 100 + patch). */
 int vs_version(void);
@@ -166,7 +170,12 @@ int64_t vs_refine_pairs_workspace_bytes(int64_t n_pairs);
  *           keep = predicate(location of the point in the polygon), with
  *           VS_PRED_WITHIN / VS_PRED_CONTAINS: interior, VS_PRED_TOUCHES:
  *           boundary, otherwise: not outside.
- *   mode 1: left rows polygons A, right rows polygons B; keep = intersects.
+ *   mode 1: left rows polygons A, right rows polygons B; keep =
+ *           predicate(A, B): intersects, or the exact relations of two areal
+ *           rows -- within / covered_by (A inside B, closed), contains /
+ *           covers, contains_properly (no boundary contact), touches
+ *           (boundaries meet, interiors do not), overlaps (interiors meet,
+ *           neither inside the other), crosses (never).
  *   mode 2: left rows polygons B, right rows points (px, py); keep as mode 0
  *           for the right point located in the left polygon (contains,
  *           covers, contains_properly, touches, intersects of polygons over

@@ -177,10 +177,7 @@ def test_golden_sindex_table(golden):
     for row in table["rows"]:
         q = [box(*b) for b in row["query"]]
         arg = q[0] if row["scalar"] else q
-        if row["requires"] == "polygon_de9im":
-            with pytest.raises(NotImplementedError):
-                tree.sindex.query(arg, predicate=row["predicate"])
-            continue
+        # rows marked polygon_de9im need the polygon x polygon relations
         res = tree.sindex.query(arg, predicate=row["predicate"])
         assert res.tolist() == row["expected"], (row["source"], row["predicate"], row["query"])
 

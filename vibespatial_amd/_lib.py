@@ -27,6 +27,9 @@ PRED = {
     "touches": 3,
     "contains": 4,
     "covers": 5,
+    "overlaps": 6,
+    "crosses": 7,
+    "contains_properly": 8,
 }
 
 

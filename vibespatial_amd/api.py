@@ -61,6 +61,7 @@ _EMPTY_PAIRS = {
     ("point", "polygon"): {"contains", "contains_properly", "covers", "crosses", "overlaps"},
     ("polygon", "point"): {"within", "covered_by", "crosses", "overlaps"},
     ("point", "point"): {"touches", "crosses", "overlaps"},
+    ("polygon", "polygon"): {"crosses"},  # two areal rows never cross
 }
 
 
@@ -166,8 +167,6 @@ def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, pred
     if q_fam == "polygon" and t_fam == "polygon":
         if l.numel() == 0:
             return l, r  # no bbox overlap: every predicate but disjoint is false
-        if predicate != "intersects":
-            raise NotImplementedError(f"polygon x polygon predicate {predicate!r} (full DE-9IM) is not implemented")
         return _refine(1, predicate, l, r, None, qarr, tarr)
     raise NotImplementedError(f"{q_fam} x {t_fam} predicate {predicate!r}")
 

@@ -4,7 +4,8 @@
 //   * polygon x polygon intersects after a bbox query, one thread per pair:
 //     any edge pair touches or crosses (exact orientation + collinear
 //     containment), else a vertex of one located in the other
-//     (src/vibespatial/predicates/polygon_kernels.py:131-228);
+//     (src/vibespatial/predicates/polygon_kernels.py:131-228); the other
+//     polygon x polygon predicates through the exact relation of vs_relate.h;
 //   * point x polygon location for swapped predicates (contains / covers of a
 //     polygon over tree points), predicate mapping of
 //     src/vibespatial/predicates/point_relations.py:100-129;
@@ -16,6 +17,7 @@
 #include "vs_exact.h"
 #include "vs_internal.h"
 #include "vs_polys.h"
+#include "vs_relate.h"
 
 namespace vs {
 
@@ -35,7 +37,8 @@ __device__ __forceinline__ int loc_keep(int pred, int loc) {
 }
 
 // mode 0: left = points (px, py), right = polygons B; keep = pred(location).
-// mode 1: left = polygons A, right = polygons B; keep = intersects.
+// mode 1: left = polygons A, right = polygons B; keep = pred(A, B), exact
+//         (intersects, or the relations of vs_relate.h).
 // mode 2: left = polygons B, right = points (px, py); keep = pred(location of
 //         the right point in the left polygon) -- contains / covers / touches /
 //         intersects of a query polygon over tree points.
@@ -53,7 +56,7 @@ __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, c
             const int loc = (x == x && y == y) ? point_location_f64(x, y, B.x, B.y, B.geom, B.part, B.ring, gi) : 0;
             k = loc_keep(pred, loc);
         } else {
-            k = polygons_intersect_auto(A, l, B, r) ? 1 : 0;
+            k = polygon_predicate(pred, A, l, B, r) ? 1 : 0;
         }
         keep[i] = k;
     }
