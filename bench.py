@@ -29,7 +29,8 @@ tenth of the sample.
 Other workloads (`--workload`): c4 (learned index vs tree, 50M points x 1M
 boxes), c5 (4096 batched polygon intersects jobs), dwithin (the c2 data with
 predicate='dwithin'), wkb (GPU WKB ingest of the c2 data), nearest
-(sjoin_nearest of the c2 points to the c2 polygons).
+(sjoin_nearest of the c2 points to the c2 polygons), relate (polygon x polygon
+predicates, --predicate).
 """
 
 from __future__ import annotations
@@ -62,11 +63,12 @@ def main(argv=None):
     ap.add_argument("--n-polys", type=int, default=100_000)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="points in the CPU-baseline sample (0 disables it)")
-    ap.add_argument("--workload", choices=("c2", "c4", "c5", "dwithin", "wkb", "nearest"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c4", "c5", "dwithin", "wkb", "nearest", "relate"), default="c2",
                     help="c2: the sjoin metric (default); c4: point-tree sindex.query, learned index vs tree; "
                          "c5: 4096 batched polygon x polygon intersects jobs; dwithin: exact point x polygon "
                          "dwithin join on the c2 data")
     ap.add_argument("--distance", type=float, default=25.0, help="dwithin distance (workload dwithin)")
+    ap.add_argument("--predicate", default="overlaps", help="polygon x polygon predicate (workload relate)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once (a kernel of known traffic for "
                          "scaling hardware byte counters)")
@@ -99,6 +101,8 @@ def main(argv=None):
         return _with_copy_probe(args, run_wkb)
     if args.workload == "nearest":
         return _with_copy_probe(args, run_nearest)
+    if args.workload == "relate":
+        return _with_copy_probe(args, run_relate)
     _log("generating workload")
     w = S.c2(n_points=args.n_points, n_polys=args.n_polys)
     Q = DevicePoints.from_host(w.points)
@@ -584,6 +588,77 @@ def run_nearest(args):
         "config": {"workload": f"{n:,} GMM points x {len(w.polygons):,} star polygons (0-3 holes), nearest "
                                "polygon of every point, no max_distance",
                    "pairs": k, "pairs_at_distance_0": zero, "start_radius": start, "cover_radius": cover},
+        "roofline": {"bound": "hbm", "kernel": "whole step", "achieved": alg / (ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes_per_step_lower_bound": alg},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_relate(args):
+    """sindex.query(polygons, predicate=P) between two star-polygon columns:
+    the 100k c2 polygons (0-3 holes) query a tree over 100k more star polygons
+    of the same kind (another seed).  A step = box query over the tree (built
+    once) -> exact polygon x polygon refine (vs_refine_pairs mode 1) ->
+    compaction.  Algorithmic bytes per step (lower bound): 32 per query box,
+    16 per candidate pair, and per candidate both rows' coordinates
+    (16 (V_a + V_b)), + 16 per kept pair."""
+    import numpy as np
+    import torch
+
+    from vibespatial_amd import api
+    from vibespatial_amd import join as J
+    from vibespatial_amd import synthetic as S
+    from vibespatial_amd.device import device_polygons
+
+    _log(f"relate: generating polygons ({args.predicate})")
+    A = S.c2(n_points=1, n_polys=args.n_polys).polygons
+    rg = np.random.default_rng(11)
+    m = args.n_polys
+    holes = rg.integers(0, 4, m)
+    nh = int(holes.sum())
+    B = S.star_polygons(rg, rg.uniform(0, 1e5, m), rg.uniform(0, 1e5, m), rg.uniform(50, 300, m),
+                        np.clip(np.round(rg.lognormal(np.log(24.0), 1.0, m)), 4, 10_000).astype(np.int64),
+                        holes=holes, hole_vertices=rg.integers(4, 17, nh), hole_frac=rg.uniform(0.1, 0.4, nh))
+    DA, DB = device_polygons(A), device_polygons(B)
+    tree = J.build_tree(J.polygon_bounds(DB))
+    qb = J.polygon_bounds(DA)
+    torch.cuda.synchronize()
+
+    def step():
+        l, r, _ = J.query_boxes(qb, tree)
+        kl, _ = api._refine(1, args.predicate, l, r, None, A, B)
+        return int(l.numel()), int(kl.numel())
+
+    for _ in range(max(args.warmup, 1)):
+        cands, pairs = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        c2, p2 = step()
+        assert (c2, p2) == (cands, pairs)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    alg = 32 * len(A) + 16 * cands + 16 * int(len(A.x) / max(len(A), 1) + len(B.x) / max(len(B), 1)) * cands \
+        + 16 * pairs
+    out = {
+        "metric": f"sindex.query(predicate={args.predicate!r}) candidate pairs/sec, 100k x 100k star polygons, "
+                  "exact relations",
+        "value": cands / (ms * 1e-3),
+        "unit": "candidate pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(A):,} x {len(B):,} star polygons (0-3 holes, lognormal vertices), "
+                               f"predicate {args.predicate}", "bbox_candidates": cands, "pairs": pairs},
         "roofline": {"bound": "hbm", "kernel": "whole step", "achieved": alg / (ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "alg_bytes_per_step_lower_bound": alg},

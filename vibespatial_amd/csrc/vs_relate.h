@@ -252,10 +252,53 @@ VS_HD Relation relate_polygons(const Polys& A, int64_t ra, const Polys& B, int64
     return R;
 }
 
+VS_HD double4 row_box(const Polys& P, int64_t row) {
+    double4 b = make_double4(INFINITY, INFINITY, -INFINITY, -INFINITY);
+    for (int32_t c = P.ring[P.part[P.geom[row]]]; c < P.ring[P.part[P.geom[row + 1]]]; ++c) {
+        b.x = fmin(b.x, P.x[c]);
+        b.y = fmin(b.y, P.y[c]);
+        b.z = fmax(b.z, P.x[c]);
+        b.w = fmax(b.w, P.y[c]);
+    }
+    return b;
+}
+
+// Some edge of A crosses some edge of B properly (interior points of both).
+VS_HD bool any_proper_crossing(const Polys& A, int64_t ra, const Polys& B, int64_t rb) {
+    for (int32_t r = A.part[A.geom[ra]]; r < A.part[A.geom[ra + 1]]; ++r)
+        for (int32_t c = A.ring[r] + 1; c < A.ring[r + 1]; ++c) {
+            const double ax = A.x[c - 1], ay = A.y[c - 1], bx = A.x[c], by = A.y[c];
+            for (int32_t q = B.part[B.geom[rb]]; q < B.part[B.geom[rb + 1]]; ++q)
+                for (int32_t d = B.ring[q] + 1; d < B.ring[q + 1]; ++d) {
+                    const double cx = B.x[d - 1], cy = B.y[d - 1], dx = B.x[d], dy = B.y[d];
+                    if (fmax(ax, bx) < fmin(cx, dx) || fmax(cx, dx) < fmin(ax, bx) || fmax(ay, by) < fmin(cy, dy) ||
+                        fmax(cy, dy) < fmin(ay, by))
+                        continue;
+                    if (orient2d(ax, ay, bx, by, cx, cy) * orient2d(ax, ay, bx, by, dx, dy) >= 0) continue;
+                    if (orient2d(cx, cy, dx, dy, ax, ay) * orient2d(cx, cy, dx, dy, bx, by) < 0) return true;
+                }
+        }
+    return false;
+}
+
 // predicate(A row, B row) for two polygonal rows (codes of vsjoin.h).
+// Cheap necessary conditions first (box containment for the containment
+// predicates; intersects, then a proper crossing -- which settles every
+// relation -- for touches / overlaps), the full relation only when needed.
 VS_HD bool polygon_predicate(int pred, const Polys& A, int64_t ra, const Polys& B, int64_t rb) {
     if (pred == VS_PRED_INTERSECTS) return polygons_intersect_auto(A, ra, B, rb);
     if (pred == VS_PRED_CROSSES) return false;
+    if (pred == VS_PRED_WITHIN || pred == VS_PRED_COVERED_BY || pred == VS_PRED_CONTAINS ||
+        pred == VS_PRED_COVERS || pred == VS_PRED_CONTAINS_PROPERLY) {
+        const bool a_inner = pred == VS_PRED_WITHIN || pred == VS_PRED_COVERED_BY;
+        const double4 in = a_inner ? row_box(A, ra) : row_box(B, rb);
+        const double4 out = a_inner ? row_box(B, rb) : row_box(A, ra);
+        if (!(out.x <= in.x && in.z <= out.z && out.y <= in.y && in.w <= out.w)) return false;
+        if (any_proper_crossing(A, ra, B, rb)) return false;
+    } else {  // touches, overlaps
+        if (!polygons_intersect_auto(A, ra, B, rb)) return false;
+        if (any_proper_crossing(A, ra, B, rb)) return pred == VS_PRED_OVERLAPS;
+    }
     const Relation R = relate_polygons(A, ra, B, rb);
     switch (pred) {
         case VS_PRED_WITHIN:
