@@ -42,12 +42,16 @@ __device__ __forceinline__ int loc_keep(int pred, int loc) {
 // mode 2: left = polygons B, right = points (px, py); keep = pred(location of
 //         the right point in the left polygon) -- contains / covers / touches /
 //         intersects of a query polygon over tree points.
+// Pairs are visited in order[t] when given (polygon pairs grouped by right
+// row, so the lanes of a wave mostly share one right polygon).
 __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, const int64_t* __restrict__ left,
                                                           const int64_t* __restrict__ right, int64_t n,
+                                                          const int32_t* __restrict__ order,
                                                           const double* __restrict__ px,
                                                           const double* __restrict__ py, Polys A, Polys B,
                                                           int32_t* __restrict__ keep) {
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = order ? (int64_t)order[t] : t;
         const int64_t l = left[i], r = right[i];
         int k;
         if (mode == 0 || mode == 2) {
@@ -182,7 +186,16 @@ int64_t lexsort_workspace_bytes(int64_t n) {
 
 extern "C" {
 
-int64_t vs_refine_pairs_workspace_bytes(int64_t n_pairs) { return vs::refine_workspace_bytes(n_pairs); }
+int64_t vs_refine_pairs_workspace_bytes(int64_t n_pairs) {
+    vs::Carver c(nullptr);
+    c.take<char>(vs::refine_workspace_bytes(n_pairs));
+    c.take<uint32_t>(n_pairs);
+    c.take<int32_t>(n_pairs);
+    c.take<uint32_t>(n_pairs);
+    c.take<int32_t>(n_pairs);
+    c.take<char>(vs::sort_workspace_bytes_u32(n_pairs));
+    return c.bytes;
+}
 
 int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const int64_t* right, int64_t n_pairs,
                     const double* px, const double* py, const double* ax, const double* ay,
@@ -193,10 +206,17 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
     vs::clear_error();
     VS_REQUIRE(mode >= 0 && mode <= 2, "vs_refine_pairs: mode %d not in {0, 1, 2}", mode);
     VS_REQUIRE(n_pairs >= 0 && n_pairs < (int64_t)INT32_MAX, "vs_refine_pairs: n_pairs out of range");
-    VS_REQUIRE(workspace_bytes >= vs::refine_workspace_bytes(n_pairs), "vs_refine_pairs: workspace too small");
+    VS_REQUIRE(workspace_bytes >= vs_refine_pairs_workspace_bytes(n_pairs), "vs_refine_pairs: workspace too small");
     VS_REQUIRE(d_result != nullptr, "vs_refine_pairs: null result");
     hipStream_t s = vs::stream_of(stream);
-    vs::Carver c(workspace);
+    vs::Carver c0(workspace);
+    void* refine_ws = c0.take<char>(vs::refine_workspace_bytes(n_pairs));
+    uint32_t* key_in = c0.take<uint32_t>(n_pairs);
+    int32_t* idx_in = c0.take<int32_t>(n_pairs);
+    uint32_t* key_out = c0.take<uint32_t>(n_pairs);
+    int32_t* order = c0.take<int32_t>(n_pairs);
+    void* sort_ws = c0.take<char>(vs::sort_workspace_bytes_u32(n_pairs));
+    vs::Carver c(refine_ws);
     int32_t* keep = c.take<int32_t>(n_pairs);
     int32_t* pos = c.take<int32_t>(n_pairs);
     long long* total = c.take<long long>(1);
@@ -204,9 +224,18 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
     const vs::Polys A = {ax, ay, a_geom, a_part, a_ring};
     const vs::Polys B = {bx, by, b_geom, b_part, b_ring};
     if (n_pairs > 0) {
+        const int32_t* ord = nullptr;
+        if (mode == 1) {  // polygon pairs: group by right row
+            hipLaunchKernelGGL(vs::right_keys_kernel, dim3((unsigned)vs::ceil_div(n_pairs, 256)), dim3(256), 0, s,
+                               right, n_pairs, key_in, idx_in);
+            VS_LAUNCH_CHECK();
+            VS_TRY(vs::sort_pairs_into_u32(key_in, idx_in, key_out, order, n_pairs, 0, 32, sort_ws,
+                                           vs::sort_workspace_bytes_u32(n_pairs), s));
+            ord = order;
+        }
         const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
         hipLaunchKernelGGL(vs::refine_pairs_kernel, dim3(grid), dim3(256), 0, s, mode, predicate, left, right,
-                           n_pairs, px, py, A, B, keep);
+                           n_pairs, ord, px, py, A, B, keep);
         VS_LAUNCH_CHECK();
     }
     VS_TRY(vs::exclusive_scan_i32(keep, pos, n_pairs, total, scan_ws, s));
