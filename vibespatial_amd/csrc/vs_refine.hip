@@ -43,13 +43,23 @@ __device__ __forceinline__ int loc_keep(int pred, int loc) {
 //         the right point in the left polygon) -- contains / covers / touches /
 //         intersects of a query polygon over tree points.
 // Pairs are visited in order[t] when given (polygon pairs grouped by right
-// row, so the lanes of a wave mostly share one right polygon).
+// row, so the lanes of a wave mostly share one right polygon).  Polygon pairs
+// whose coordinate counts multiply to more than kHeavyPair go to `heavy`
+// (count in *n_heavy) for refine_heavy_kernel, one wave per pair; pairs the
+// cheap tests leave undecided go to `undecided` (count in *n_undecided) for
+// refine_relate_kernel, so this kernel carries none of the full relation's
+// registers.
+constexpr int64_t kHeavyPair = 1 << 15;
+
 __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, const int64_t* __restrict__ left,
                                                           const int64_t* __restrict__ right, int64_t n,
                                                           const int32_t* __restrict__ order,
                                                           const double* __restrict__ px,
                                                           const double* __restrict__ py, Polys A, Polys B,
-                                                          int32_t* __restrict__ keep) {
+                                                          int32_t* __restrict__ keep, int32_t* __restrict__ heavy,
+                                                          unsigned long long* __restrict__ n_heavy,
+                                                          int32_t* __restrict__ undecided,
+                                                          unsigned long long* __restrict__ n_undecided) {
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
         const int64_t i = order ? (int64_t)order[t] : t;
         const int64_t l = left[i], r = right[i];
@@ -59,11 +69,58 @@ __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, c
             const double x = px[pi], y = py[pi];
             const int loc = (x == x && y == y) ? point_location_f64(x, y, B.x, B.y, B.geom, B.part, B.ring, gi) : 0;
             k = loc_keep(pred, loc);
+        } else if ((int64_t)row_coords(A, l) * row_coords(B, r) > kHeavyPair) {
+            heavy[atomicAdd(n_heavy, 1ull)] = (int32_t)i;
+            k = 0;  // decided by refine_heavy_kernel
         } else {
-            k = polygon_predicate(pred, A, l, B, r) ? 1 : 0;
+            const int f = polygon_predicate_fast(pred, A, l, B, r);
+            if (f < 0) undecided[atomicAdd(n_undecided, 1ull)] = (int32_t)i;
+            k = f > 0 ? 1 : 0;
         }
         keep[i] = k;
     }
+}
+
+// One wave per heavy polygon pair (grid-stride over the device count).
+__global__ __launch_bounds__(256) void refine_heavy_kernel(int pred, const int64_t* __restrict__ left,
+                                                          const int64_t* __restrict__ right,
+                                                          const int32_t* __restrict__ heavy,
+                                                          const unsigned long long* __restrict__ n_heavy, Polys A,
+                                                          Polys B, int32_t* __restrict__ keep,
+                                                          int32_t* __restrict__ undecided,
+                                                          unsigned long long* __restrict__ n_undecided) {
+    const int lane = threadIdx.x & 63;
+    const int64_t m = (int64_t)*n_heavy;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < m; w += waves) {
+        const int64_t i = heavy[w];
+        const int f = wave_polygon_predicate_fast(pred, A, left[i], B, right[i], lane);
+        if (lane == 0) {
+            keep[i] = f > 0 ? 1 : 0;
+            if (f < 0) undecided[atomicAdd(n_undecided, 1ull)] = (int32_t)i;
+        }
+    }
+}
+
+// The full relation for the pairs the cheap tests left undecided, one wave
+// per pair (grid-stride over the device count; lanes split the edges).
+__global__ __launch_bounds__(256) void refine_relate_kernel(int pred, const int64_t* __restrict__ left,
+                                                           const int64_t* __restrict__ right,
+                                                           const int32_t* __restrict__ undecided,
+                                                           const unsigned long long* __restrict__ n_undecided,
+                                                           Polys A, Polys B, int32_t* __restrict__ keep) {
+    const int lane = threadIdx.x & 63;
+    const int64_t m = (int64_t)*n_undecided;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < m; w += waves) {
+        const int64_t i = undecided[w];
+        const bool k = relation_decides(pred, wave_relate_polygons(A, left[i], B, right[i], lane));
+        if (lane == 0) keep[i] = k ? 1 : 0;
+    }
+}
+
+__global__ void zero_count_kernel(unsigned long long* c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0;
 }
 
 __global__ __launch_bounds__(256) void compact_pairs_kernel(const int64_t* __restrict__ left,
@@ -194,6 +251,9 @@ int64_t vs_refine_pairs_workspace_bytes(int64_t n_pairs) {
     c.take<uint32_t>(n_pairs);
     c.take<int32_t>(n_pairs);
     c.take<char>(vs::sort_workspace_bytes_u32(n_pairs));
+    c.take<int32_t>(n_pairs);
+    c.take<int32_t>(n_pairs);
+    c.take<unsigned long long>(2);
     return c.bytes;
 }
 
@@ -216,6 +276,11 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
     uint32_t* key_out = c0.take<uint32_t>(n_pairs);
     int32_t* order = c0.take<int32_t>(n_pairs);
     void* sort_ws = c0.take<char>(vs::sort_workspace_bytes_u32(n_pairs));
+    int32_t* heavy = c0.take<int32_t>(n_pairs);
+    int32_t* undecided = c0.take<int32_t>(n_pairs);
+    unsigned long long* counts = c0.take<unsigned long long>(2);  // heavy, undecided
+    unsigned long long* n_heavy = counts;
+    unsigned long long* n_undecided = counts + 1;
     vs::Carver c(refine_ws);
     int32_t* keep = c.take<int32_t>(n_pairs);
     int32_t* pos = c.take<int32_t>(n_pairs);
@@ -233,10 +298,26 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
                                            vs::sort_workspace_bytes_u32(n_pairs), s));
             ord = order;
         }
+        if (mode == 1) {
+            hipLaunchKernelGGL(vs::zero_count_kernel, dim3(1), dim3(64), 0, s, counts);
+            VS_LAUNCH_CHECK();
+        }
         const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
         hipLaunchKernelGGL(vs::refine_pairs_kernel, dim3(grid), dim3(256), 0, s, mode, predicate, left, right,
-                           n_pairs, ord, px, py, A, B, keep);
+                           n_pairs, ord, px, py, A, B, keep, heavy, n_heavy, undecided, n_undecided);
         VS_LAUNCH_CHECK();
+        if (mode == 1) {
+            const int hgrid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 4), 2048);
+            hipLaunchKernelGGL(vs::refine_heavy_kernel, dim3(hgrid), dim3(256), 0, s, predicate, left, right, heavy,
+                               n_heavy, A, B, keep, undecided, n_undecided);
+            VS_LAUNCH_CHECK();
+            if (predicate != VS_PRED_INTERSECTS && predicate != VS_PRED_CROSSES) {
+                const int rgrid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 4), 2048);
+                hipLaunchKernelGGL(vs::refine_relate_kernel, dim3(rgrid), dim3(256), 0, s, predicate, left, right,
+                                   undecided, n_undecided, A, B, keep);
+                VS_LAUNCH_CHECK();
+            }
+        }
     }
     VS_TRY(vs::exclusive_scan_i32(keep, pos, n_pairs, total, scan_ws, s));
     if (n_pairs > 0) {
