@@ -512,12 +512,13 @@ def _maybe_make_list(obj):
     return obj
 
 
-def _basic_checks(left_df, right_df, how, lsuffix, rsuffix, on_attribute=None):
+def _basic_checks(left_df, right_df, how, lsuffix, rsuffix, on_attribute=None, allowed_hows=None):
+    """api/tools/sjoin.py:268-321: sjoin allows outer, sjoin_nearest does not."""
     if not isinstance(left_df, GeoDataFrame):
         raise ValueError(f"'left_df' should be GeoDataFrame, got {type(left_df)}")
     if not isinstance(right_df, GeoDataFrame):
         raise ValueError(f"'right_df' should be GeoDataFrame, got {type(right_df)}")
-    allowed = ["left", "right", "inner", "outer"]
+    allowed = allowed_hows if allowed_hows is not None else ["left", "right", "inner"]
     if how not in allowed:
         raise ValueError(f'`how` was "{how}" but is expected to be in {allowed}')
     if left_df.crs != right_df.crs:
@@ -552,7 +553,8 @@ def sjoin(left_df, right_df, how="inner", predicate="intersects", lsuffix="left"
         first = next(iter(kwargs.keys()))
         raise TypeError(f"sjoin() got an unexpected keyword argument '{first}'")
     on_attribute = _maybe_make_list(on_attribute)
-    _basic_checks(left_df, right_df, how, lsuffix, rsuffix, on_attribute)
+    _basic_checks(left_df, right_df, how, lsuffix, rsuffix, on_attribute,
+                  allowed_hows=["left", "right", "inner", "outer"])
     tree = right_df.sindex
     tree._check(predicate, distance)
     res = tree.query(left_df.geometry.values, predicate=predicate, sort=True, distance=distance)

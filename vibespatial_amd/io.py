@@ -298,7 +298,6 @@ def decode_wkb(payload, offsets, validity=None) -> GeometryArray:
 
     from .device import require_gpu
 
-    dev = require_gpu()
     off = np.ascontiguousarray(offsets, dtype=np.int64)
     n = len(off) - 1
     if n < 0:
@@ -307,6 +306,9 @@ def decode_wkb(payload, offsets, validity=None) -> GeometryArray:
         np.ascontiguousarray(payload, dtype=np.uint8)
     if n and (off[0] < 0 or off[-1] > len(data) or np.any(np.diff(off) < 0)):
         raise ValueError("offsets out of the payload or not non-decreasing")
+    if validity is not None and len(validity) != n:
+        raise ValueError("validity must hold one entry per record")
+    dev = require_gpu()
     d_data = torch.from_numpy(data.copy() if len(data) else np.zeros(1, np.uint8)).to(dev)
     d_off = torch.from_numpy(off).to(dev)
     d_valid = None if validity is None else torch.from_numpy(np.ascontiguousarray(validity, dtype=np.uint8)).to(dev)
