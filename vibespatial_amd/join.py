@@ -175,8 +175,8 @@ def nearest_pairs(qboxes: torch.Tensor, qgeom, tree: DeviceTree, tgeom, *, max_d
                   exclusive=False, start_radius: float, cover_radius: float, growth: float = 4.0):
     """Lexsorted (left, right, distance) of each query's nearest tree rows
     (ties kept unless not `return_all`), by an expanding-radius search.
-    Points against polygons first take a radius-0 round: the fused point join
-    with predicate intersects gives every pair at distance 0.  Then each round
+    Points against polygons first take a radius-0 round: the point join with
+    predicate intersects (learned form) gives every pair at distance 0.  Then each round
     takes the open queries (vs_nearest_open: rows and boxes grown by their
     search radius) -> box query -> rows mapped back (vs_map_left) -> pair
     distances -> vs_nearest_select (closes queries, grows the others' radius).
@@ -227,8 +227,9 @@ def nearest_pairs(qboxes: torch.Tensor, qgeom, tree: DeviceTree, tgeom, *, max_d
 
     still_open = nq
     if pip:
-        # radius 0: the pairs with the point not outside the polygon
-        l, r, _ = sjoin_points_polygons(qgeom, tgeom, tree, "intersects")
+        # radius 0: the pairs with the point not outside the polygon (the
+        # learned form of the point join: the same pairs, bit for bit)
+        l, r, _ = sjoin_points_polygons_learned(qgeom, tgeom, polygon_bounds(tgeom), "intersects")
         still_open = select(l, r, torch.zeros(max(int(l.numel()), 1), dtype=torch.float64, device=dev))
     if still_open:
         # each open query's radius from a descent of the tree (an upper bound
