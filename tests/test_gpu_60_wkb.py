@@ -202,3 +202,18 @@ def test_geoparquet_round_trip_feeds_sjoin(tmp_path):
             j2 = sjoin(l2, r2, predicate=pred)
             assert j1.index.tolist() == j2.index.tolist() and j1["index_right"].tolist() == j2["index_right"].tolist()
             assert j2["b"].tolist() == j1["b"].tolist()
+
+
+def test_read_parquet_builds_the_index(tmp_path):
+    # build_index=True: the trees exist on return, built from the decoded
+    # device buffers, and answer queries like a lazily built index
+    rng = np.random.default_rng(31)
+    P = S.star_polygons(rng, rng.uniform(0, 100, 200), rng.uniform(0, 100, 200), rng.uniform(1, 5, 200),
+                        rng.integers(4, 20, 200))
+    polys = [GeometryArray.from_polygons(P)[i] for i in range(len(P))] + [Point(5.0, 5.0), None]
+    GeoDataFrame({"v": np.arange(len(polys)), "geometry": polys}).to_parquet(tmp_path / "m.parquet")
+    g1 = read_parquet(tmp_path / "m.parquet", build_index=True)
+    assert set(g1.sindex._index._trees) == {"point", "polygon"}
+    g2 = read_parquet(tmp_path / "m.parquet")
+    q = [Point(*rng.uniform(0, 100, 2)) for _ in range(500)]
+    assert g1.sindex.query(q, predicate="intersects").tolist() == g2.sindex.query(q, predicate="intersects").tolist()

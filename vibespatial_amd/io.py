@@ -129,8 +129,20 @@ def read_shapefile_bytes(shp: bytes, dbf: bytes | None = None):
     return ga, attrs
 
 
-def read_file(path: str, crs=None):
-    """GeoDataFrame from a .shp path, a directory holding one, or a .zip of one."""
+def build_read_index(gdf) -> None:
+    """Build the frame's spatial index now, on the device, from the decoded
+    buffers (io/file.py:1786-1853 _attach_gpu_spatial_index, the reference's
+    build_index=True): one packed Hilbert tree per geometry family present."""
+    idx = gdf.sindex
+    ga = gdf.geometry.values
+    for fam, arr in (("point", ga.points), ("polygon", ga.polygons)):
+        if arr is not None:
+            idx._index.tree(fam)
+
+
+def read_file(path: str, crs=None, build_index: bool = False):
+    """GeoDataFrame from a .shp path, a directory holding one, or a .zip of one;
+    build_index=True also builds its spatial index on the device."""
     from .api import GeoDataFrame, GeoSeries
 
     if path.endswith(".zip"):
@@ -149,7 +161,10 @@ def read_file(path: str, crs=None):
         dbf_path = path[:-4] + ".dbf"
         dbf = open(dbf_path, "rb").read() if os.path.exists(dbf_path) else None
     ga, attrs = read_shapefile_bytes(shp, dbf)
-    return GeoDataFrame(attrs, geometry=GeoSeries(ga, crs=crs), crs=crs)
+    gdf = GeoDataFrame(attrs, geometry=GeoSeries(ga, crs=crs), crs=crs)
+    if build_index:
+        build_read_index(gdf)
+    return gdf
 
 
 _ = _io
@@ -382,10 +397,12 @@ def _geoarrow_native(col, encoding: str) -> GeometryArray:
     return GeometryArray(tags, None, None, PolygonArray(xs, ys, ring_coords, part, geom), rows)
 
 
-def read_parquet(path, columns=None, crs=None):
+def read_parquet(path, columns=None, crs=None, build_index: bool = False):
     """GeoDataFrame from a GeoParquet file: WKB geometry decoded on the GPU,
     GeoArrow native point / polygon / multipolygon columns by buffer views;
-    other columns through pandas (io/geoparquet.py:3430 read_geoparquet)."""
+    other columns through pandas (io/geoparquet.py:3430 read_geoparquet).
+    build_index=True also builds the spatial index on the device from the
+    decoded buffers before returning."""
     import json
 
     import pyarrow as pa
@@ -419,6 +436,8 @@ def read_parquet(path, columns=None, crs=None):
     crs = crs if crs is not None else cmeta.get("crs")
     gdf = GeoDataFrame(attrs, geometry=GeoSeries(ga, crs=crs), crs=crs)
     gdf._geom_name = name
+    if build_index:
+        build_read_index(gdf)
     return gdf
 
 
