@@ -1,8 +1,10 @@
 """vibespatial_amd -- MI355X-native spatial-join hot path of vibeSpatial.
 
 ``sjoin(left, right, predicate=...)`` and ``GeoSeries.sindex.query`` over a
-Hilbert-packed spatial index, with exact point-in-polygon and polygon
-``intersects`` refinement, computed by hand-written gfx950 HIP kernels in
+Hilbert-packed spatial index (or a learned rank index over Hilbert-sorted
+points), with exact point-in-polygon, polygon relations and distance
+decisions; ``sjoin_nearest`` / ``sindex.nearest``; GeoParquet / WKB ingest
+decoded on the GPU.  Computed by hand-written gfx950 HIP kernels in
 ``libvsjoin.so`` (C-ABI in ``include/vsjoin.h``, bound with ctypes).
 """
 
