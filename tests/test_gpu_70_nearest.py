@@ -14,7 +14,7 @@ import pytest
 import oracle
 from vibespatial_amd import GeoDataFrame, GeoSeries, sjoin_nearest
 from vibespatial_amd import synthetic as S
-from vibespatial_amd.geometry import Point, PointArray, PolygonArray, box
+from vibespatial_amd.geometry import Point, PointArray, Polygon, PolygonArray, box
 
 pytestmark = pytest.mark.gpu
 
@@ -129,3 +129,23 @@ def test_null_rows_and_sjoin_nearest():
     assert jr.index.tolist() == [0, 1, 2, 3]
     assert np.isnan(jr["index_left"].tolist()[0]) and jr["index_left"].tolist()[1:] == [2, 4, 4]
     assert jr["dist"].tolist()[1:] == [0.0, math.sqrt(2), math.sqrt(2)]
+
+
+def test_mixed_tree_ties_and_edge_options():
+    print("nearest mixed tree", flush=True)
+    # a tree of points and polygons: ties across the two families, the
+    # least tree row kept when return_all is off, max_distance 0 keeps exact
+    # contacts only, empty / NaN rows never match
+    tree = GeoSeries([Point(3, 0), box(-1, 2, 1, 4), None, Point(), box(10, 10, 12, 12), Point(0, -2),
+                      Polygon([])])
+    q = GeoSeries([Point(0, 0), Point(11, 11), Point(1, 4), Point(50, 50)])
+    (l, r), d = tree.sindex.nearest(q.values, return_distance=True)
+    got = list(zip(l.tolist(), r.tolist(), d.tolist()))
+    # (0, 0): box(-1, 2, 1, 4) at 2 and (0, -2) at 2 tie; (3, 0) at 3
+    assert got[:2] == [(0, 1, 2.0), (0, 5, 2.0)]
+    assert (1, 4, 0.0) in got and (2, 1, 0.0) in got
+    assert [g for g in got if g[0] == 3] == [(3, 4, math.sqrt(38.0 * 38.0 + 38.0 * 38.0))]
+    (l, r), d = tree.sindex.nearest(q.values, return_all=False, return_distance=True)
+    assert list(zip(l.tolist(), r.tolist())) == [(0, 1), (1, 4), (2, 1), (3, 4)]
+    res = tree.sindex.nearest(q.values, max_distance=0.0)
+    assert res.tolist() == [[1, 2], [4, 1]]

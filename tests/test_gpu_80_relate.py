@@ -123,3 +123,21 @@ def test_snapped_stars():
 def test_reference_query_vectors(pred, query, expected):
     tree = GeoSeries([Point(i, i) for i in range(5)] + [box(10, 10, 20, 20)])
     assert tree.sindex.query(query, predicate=pred).tolist() == expected
+
+
+def test_shared_chains_and_multipolygon_parts():
+    print("relate shared chains", flush=True)
+    # an L-shaped polygon and the square completing it share a two-edge chain;
+    # a multipolygon whose two parts touch at a corner; identical multipolygons
+    L = Polygon([(0, 0), (4, 0), (4, 2), (2, 2), (2, 4), (0, 4), (0, 0)])
+    sq = box(2, 2, 4, 4)
+    full = box(0, 0, 4, 4)
+    M = MultiPolygon([box(0, 0, 2, 2), box(2, 2, 4, 4)])
+    geoms = [L, sq, full, M, MultiPolygon([box(0, 0, 2, 2), box(2, 2, 4, 4)]), box(1, 1, 3, 3)]
+    P = PolygonArray.from_geometries(geoms)
+    for pred in PREDS:
+        assert _pairs(P, P, pred) == _expect(P, P, pred), pred
+    # spot checks of the expected relations themselves
+    assert (0, 1) in _expect(P, P, "touches") and (0, 2) in _expect(P, P, "within")
+    assert (3, 4) in _expect(P, P, "within") and (3, 4) in _expect(P, P, "contains")
+    assert (5, 3) in _expect(P, P, "overlaps")
