@@ -74,8 +74,8 @@ def main(argv=None):
                          "scaling hardware byte counters)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the join shards by point ranges across GPUs by design, but only the 1-GPU kernel is built)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the join shards by point ranges across GPUs by design; only the 1-GPU path is implemented)",
               file=sys.stderr)
         return 2
 

@@ -13,9 +13,9 @@ Follows the reference's host WKB rules:
     each member a full Polygon record with its own header.
 The reference decodes little-endian ISO records natively and sends big-endian
 and SRID-tagged ones through a shapely bridge; both give the same geometry, so
-here every record is decoded by the same rules.  Only the families this build
-holds (Point, Polygon, MultiPolygon) are accepted; the rest are errors with
-the codes of vsjoin.h's vs_wkb_scan.
+here every record is decoded by the same rules.  Only the families of the
+point / polygon geometry model (Point, Polygon, MultiPolygon) are accepted;
+the rest are errors with the codes of vsjoin.h's vs_wkb_scan.
 """
 
 from __future__ import annotations
