@@ -132,3 +132,18 @@ def test_null_rows_keep_their_row_numbers():
     assert tree.sindex.query(Point(5.0, 5.0), predicate="dwithin", distance=0.0).tolist() == [3]
     res = tree.sindex.query([None, Point(5.0, 4.0)], predicate="dwithin", distance=1.0)
     assert res.tolist() == [[1], [3]]
+
+
+def test_polygons_with_holes():
+    print("dwithin holed polygons", flush=True)
+    rng = np.random.default_rng(23)
+    holes = rng.integers(0, 3, 80)
+    nh = int(holes.sum())
+    P = S.star_polygons(rng, rng.uniform(0, 150, 80), rng.uniform(0, 150, 80), rng.uniform(4, 10, 80),
+                        rng.integers(6, 20, 80), holes=holes, hole_vertices=rng.integers(4, 9, nh),
+                        hole_frac=rng.uniform(0.2, 0.5, nh))
+    q = _points(rng, 2000, -5, 155)
+    for d in (0.0, 0.5, 2.0):
+        l, r = _query(P, q, d)
+        el, er = _expect(0, 1, q, P, d)
+        assert np.array_equal(l, el) and np.array_equal(r, er), d

@@ -149,3 +149,24 @@ def test_mixed_tree_ties_and_edge_options():
     assert list(zip(l.tolist(), r.tolist())) == [(0, 1), (1, 4), (2, 1), (3, 4)]
     res = tree.sindex.nearest(q.values, max_distance=0.0)
     assert res.tolist() == [[1, 2], [4, 1]]
+
+
+def _holed(rng, n, lo, hi, extent):
+    holes = rng.integers(0, 3, n)
+    nh = int(holes.sum())
+    return S.star_polygons(rng, rng.uniform(0, extent, n), rng.uniform(0, extent, n), rng.uniform(lo, hi, n),
+                           rng.integers(6, 20, n), holes=holes, hole_vertices=rng.integers(4, 9, nh),
+                           hole_frac=rng.uniform(0.2, 0.5, nh))
+
+
+def test_polygons_with_holes():
+    print("nearest holed polygons", flush=True)
+    rng = np.random.default_rng(17)
+    P = _holed(rng, 60, 4, 10, 150)
+    # points at hole centres (inside a hole: distance to the hole ring) and random ones
+    q = PointArray(np.concatenate([P.x[P.ring_offsets[1:40]], rng.uniform(0, 150, 200)]),
+                   np.concatenate([P.y[P.ring_offsets[1:40]], rng.uniform(0, 150, 200)]))
+    _check(0, 1, q, P)
+    _check(0, 1, q, P, max_distance=2.0, return_all=False)
+    A = _holed(rng, 40, 2, 6, 150)
+    _check(1, 1, A, P)
