@@ -373,15 +373,22 @@ int64_t vs_pair_distances_workspace_bytes(int64_t n_pairs);
  * arguments as in vs_refine_dwithin; n_right = rows of the right side):
  * point-point sqrt(dx^2 + dy^2), point-segment GEOS pointToSegment, 0 when a
  * point is not outside a polygon or two polygons intersect (exact tests),
- * else the least vertex-segment distance; +inf for empty / NaN geometries
- * and, when `exclusive`, for point pairs with equal coordinates.  Replaces
- * _compute_pair_distances_gpu / _compute_mixed_distances_gpu
+ * else the least vertex-segment distance; +inf for empty / NaN geometries.
+ * flags: VS_DIST_EXCLUSIVE, point pairs with equal coordinates at +inf;
+ * VS_DIST_OUTSIDE, the caller knows every left point lies outside every right
+ * polygon it is paired with (the location steps are skipped; a wrong claim
+ * gives wrong distances).  cap (device fp64 per left row, or NULL; NaN = no
+ * cap): a pair farther than cap[left] gets +inf instead of its distance.
+ * Replaces _compute_pair_distances_gpu / _compute_mixed_distances_gpu
  * (spatial/nearest.py:2228, 2482). */
+#define VS_DIST_EXCLUSIVE 1
+#define VS_DIST_OUTSIDE 2
 int vs_pair_distances(int32_t left_kind, int32_t right_kind, const int64_t* left, const int64_t* right,
-                      int64_t n_pairs, int64_t n_right, int32_t exclusive, const double* lx, const double* ly,
-                      const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring, const double* rx,
-                      const double* ry, const int32_t* r_geom, const int32_t* r_part, const int32_t* r_ring,
-                      double* out_distance, void* workspace, int64_t workspace_bytes, void* stream);
+                      int64_t n_pairs, int64_t n_right, int32_t flags, const double* cap, const double* lx,
+                      const double* ly, const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring,
+                      const double* rx, const double* ry, const int32_t* r_geom, const int32_t* r_part,
+                      const int32_t* r_ring, double* out_distance, void* workspace, int64_t workspace_bytes,
+                      void* stream);
 
 int64_t vs_nearest_select_workspace_bytes(int64_t n_pairs, int64_t n_query);
 

@@ -170,3 +170,33 @@ def test_polygons_with_holes():
     _check(0, 1, q, P, max_distance=2.0, return_all=False)
     A = _holed(rng, 40, 2, 6, 150)
     _check(1, 1, A, P)
+
+
+def test_pair_distances_cap_and_outside_flag():
+    print("pair distances cap / outside", flush=True)
+    import torch
+
+    from vibespatial_amd import join as J
+    from vibespatial_amd.device import DevicePoints, DevicePolygons
+
+    rng = np.random.default_rng(23)
+    P = _holed(rng, 30, 3, 8, 100)
+    q = PointArray(rng.uniform(-10, 110, 400), rng.uniform(-10, 110, 400))
+    left = np.repeat(np.arange(400), 30)
+    right = np.tile(np.arange(30), 400)
+    expect = np.array([oracle.distance_f64(0, 1, q, P, int(a), int(b)) for a, b in zip(left, right)])
+    Q, DP = DevicePoints.from_host(q), DevicePolygons.from_host(P)
+    dl, dr = (torch.as_tensor(a, device="cuda") for a in (left, right))
+    full = J.pair_distances(dl, dr, Q, DP).cpu().numpy()
+    assert np.array_equal(full.view(np.int64), expect.view(np.int64))
+    cap = rng.uniform(0, 30, 400)
+    cap[::7] = np.nan  # no cap
+    capped = J.pair_distances(dl, dr, Q, DP, cap=torch.as_tensor(cap, device="cuda")).cpu().numpy()
+    c = cap[left]
+    want = np.where(expect > c, np.inf, expect)
+    assert np.array_equal(capped.view(np.int64), want.view(np.int64))
+    # points outside every polygon: skipping the location gives the same distances
+    out = expect > 0
+    o = torch.as_tensor(np.flatnonzero(out), device="cuda")
+    got = J.pair_distances(dl[o], dr[o], Q, DP, outside=True).cpu().numpy()
+    assert np.array_equal(got.view(np.int64), expect[out].view(np.int64))

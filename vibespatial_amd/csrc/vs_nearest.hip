@@ -81,25 +81,34 @@ __device__ double boundary_dist_f64(double px, double py, const Polys& P, int64_
 }
 
 // One pass over the edges: 0 when p is on the boundary or inside a part (the
-// exact location's even-odd steps), else the least segment distance.
-__device__ double point_polygon_dist_f64(double px, double py, const Polys& P, int64_t row) {
+// exact location's even-odd steps), else the least segment distance.  With
+// `outside` the point is known to lie outside the row and the location steps
+// are skipped.  Distances above `cap` are not needed: segments farther than
+// cap are skipped and, when no segment comes within cap, the result is +inf.
+__device__ double point_polygon_dist_f64(double px, double py, const Polys& P, int64_t row, bool outside,
+                                         double cap) {
     if (px != px || py != py || poly_empty(P, row)) return INFINITY;
-    double best = INFINITY;
+    double best = cap;
+    bool found = false;
     for (int32_t pp = P.geom[row]; pp < P.geom[row + 1]; ++pp) {
         int parity = 0;
         for (int32_t r = P.part[pp]; r < P.part[pp + 1]; ++r)
             for (int32_t c = P.ring[r] + 1; c < P.ring[r + 1]; ++c) {
                 const double ax = P.x[c - 1], ay = P.y[c - 1], bx = P.x[c], by = P.y[c];
-                if (edge_step_f64(px, py, ax, ay, bx, by, parity)) return 0.0;  // on the boundary
+                if (!outside && edge_step_f64(px, py, ax, ay, bx, by, parity)) return 0.0;  // on the boundary
                 const double lim = best * (1.0 + 0x1p-30);
                 if (px - fmax(ax, bx) > lim || fmin(ax, bx) - px > lim || py - fmax(ay, by) > lim ||
                     fmin(ay, by) - py > lim)
                     continue;
-                best = fmin(best, segment_dist_f64(px, py, ax, ay, bx, by));
+                const double d = segment_dist_f64(px, py, ax, ay, bx, by);
+                if (d <= best) {
+                    best = d;
+                    found = true;
+                }
             }
         if (parity) return 0.0;  // inside this part
     }
-    return best;
+    return found ? best : INFINITY;
 }
 
 __device__ double polygons_dist_f64(const Polys& A, int64_t ra, const Polys& B, int64_t rb) {
@@ -113,31 +122,38 @@ __device__ double polygons_dist_f64(const Polys& A, int64_t ra, const Polys& B, 
     return best;
 }
 
-// Kinds: 0 point rows (x[row], y[row]), 1 polygonal rows.  exclusive: a
-// point pair with equal coordinates is at +inf (equal geometries excluded).
-// Pairs are visited in right-row order (order[t]) when given.
+// Kinds: 0 point rows (x[row], y[row]), 1 polygonal rows.  Flags:
+// kDistExclusive, a point pair with equal coordinates is at +inf (equal
+// geometries excluded); kDistOutside, left points lie outside every right
+// polygon (no location steps).  cap (per left row, may be null): a pair
+// farther than cap[left] gets +inf.  Pairs are visited in right-row order
+// (order[t]) when given.
+constexpr int kDistExclusive = 1, kDistOutside = 2;
+
 __global__ __launch_bounds__(256) void pair_distances_kernel(int left_kind, int right_kind,
                                                             const int64_t* __restrict__ left,
                                                             const int64_t* __restrict__ right, int64_t n,
-                                                            const int32_t* __restrict__ order, int exclusive,
-                                                            Polys L, Polys R, double* __restrict__ out) {
+                                                            const int32_t* __restrict__ order, int flags,
+                                                            const double* __restrict__ cap, Polys L, Polys R,
+                                                            double* __restrict__ out) {
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
         const int64_t i = order ? (int64_t)order[t] : t;
         const int64_t l = left[i], r = right[i];
+        const double c = cap && cap[l] == cap[l] ? cap[l] : INFINITY;  // NaN: no cap
         double d;
         if (left_kind == 0 && right_kind == 0) {
             const double px = L.x[l], py = L.y[l], qx = R.x[r], qy = R.y[r];
             if (px != px || py != py || qx != qx || qy != qy) d = INFINITY;
-            else if (exclusive && px == qx && py == qy) d = INFINITY;
+            else if ((flags & kDistExclusive) && px == qx && py == qy) d = INFINITY;
             else d = point_dist_f64(px, py, qx, qy);
         } else if (left_kind == 0) {
-            d = point_polygon_dist_f64(L.x[l], L.y[l], R, r);
+            d = point_polygon_dist_f64(L.x[l], L.y[l], R, r, (flags & kDistOutside) != 0, c);
         } else if (right_kind == 0) {
-            d = point_polygon_dist_f64(R.x[r], R.y[r], L, l);
+            d = point_polygon_dist_f64(R.x[r], R.y[r], L, l, false, c);
         } else {
             d = polygons_dist_f64(L, l, R, r);
         }
-        out[i] = d;
+        out[i] = d > c ? INFINITY : d;
     }
 }
 
@@ -505,10 +521,11 @@ extern "C" {
 int64_t vs_pair_distances_workspace_bytes(int64_t n_pairs) { return vs::pair_distances_workspace_bytes(n_pairs); }
 
 int vs_pair_distances(int32_t left_kind, int32_t right_kind, const int64_t* left, const int64_t* right,
-                      int64_t n_pairs, int64_t n_right, int32_t exclusive, const double* lx, const double* ly,
-                      const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring, const double* rx,
-                      const double* ry, const int32_t* r_geom, const int32_t* r_part, const int32_t* r_ring,
-                      double* out_distance, void* workspace, int64_t workspace_bytes, void* stream) {
+                      int64_t n_pairs, int64_t n_right, int32_t flags, const double* cap, const double* lx,
+                      const double* ly, const int32_t* l_geom, const int32_t* l_part, const int32_t* l_ring,
+                      const double* rx, const double* ry, const int32_t* r_geom, const int32_t* r_part,
+                      const int32_t* r_ring, double* out_distance, void* workspace, int64_t workspace_bytes,
+                      void* stream) {
     vs::clear_error();
     VS_REQUIRE((left_kind == 0 || left_kind == 1) && (right_kind == 0 || right_kind == 1),
                "vs_pair_distances: kinds must be 0 (points) or 1 (polygons)");
@@ -540,7 +557,7 @@ int vs_pair_distances(int32_t left_kind, int32_t right_kind, const int64_t* left
     }
     const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
     hipLaunchKernelGGL(vs::pair_distances_kernel, dim3(grid), dim3(256), 0, s, left_kind, right_kind, left, right,
-                       n_pairs, ord, exclusive ? 1 : 0, L, R, out_distance);
+                       n_pairs, ord, flags & (vs::kDistExclusive | vs::kDistOutside), cap, L, R, out_distance);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }

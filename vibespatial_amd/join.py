@@ -154,9 +154,12 @@ def _kind_args(g):
     return 1, g.args()
 
 
-def pair_distances(left: torch.Tensor, right: torch.Tensor, lgeom, rgeom, exclusive: bool = False):
+def pair_distances(left: torch.Tensor, right: torch.Tensor, lgeom, rgeom, exclusive: bool = False,
+                   outside: bool = False, cap: torch.Tensor | None = None):
     """fp64 distance of each (left, right) pair (vs_pair_distances); +inf for
-    empty / NaN geometries and, with `exclusive`, for equal point pairs."""
+    empty / NaN geometries and, with `exclusive`, for equal point pairs.
+    `outside`: every left point is known to lie outside its right polygons.
+    `cap` (fp64 per left row): pairs farther than cap[left] get +inf."""
     dev = require_gpu()
     L = lib()
     n = int(left.numel())
@@ -164,9 +167,10 @@ def pair_distances(left: torch.Tensor, right: torch.Tensor, lgeom, rgeom, exclus
     if n:
         lk, la = _kind_args(lgeom)
         rk, ra = _kind_args(rgeom)
+        flags = (1 if exclusive else 0) | (2 if outside else 0)
         ws = _ws(L.vs_pair_distances_workspace_bytes(n), dev)
-        check(L.vs_pair_distances(lk, rk, ptr(left.contiguous()), ptr(right.contiguous()), n, len(rgeom),
-                                  int(bool(exclusive)), *la, *ra, ptr(out), ptr(ws), ws.numel(), stream()),
+        check(L.vs_pair_distances(lk, rk, ptr(left.contiguous()), ptr(right.contiguous()), n, len(rgeom), flags,
+                                  ptr(cap), *la, *ra, ptr(out), ptr(ws), ws.numel(), stream()),
               "vs_pair_distances")
     return out[:n]
 
@@ -243,13 +247,20 @@ def nearest_pairs(qboxes: torch.Tensor, qgeom, tree: DeviceTree, tgeom, *, max_d
         check(L.vs_nearest_bound(ptr(qboxes), ptr(open_rows), m, ptr(open_boxes), ctypes.byref(tree.layout),
                                  ptr(tree.node_boxes), ptr(rep), final, ptr(radius), ptr(search), ptr(bws), bws.numel(),
                                  stream()), "vs_nearest_bound")
+    # candidate pairs per open query: a box query whose output overflows its
+    # capacity traverses the tree again, so the capacity starts generous and
+    # follows what the previous round found
+    per_query = 16.0
     for _ in range(200):
         if still_open == 0:
             break
         m = open_queries()
-        l, r, _ = query_boxes(open_boxes[:m], tree)
+        l, r, qs = query_boxes(open_boxes[:m], tree, capacity=max(1024, int(per_query * m)))
+        per_query = max(4.0, 1.25 * qs.staged / max(m, 1))
         check(L.vs_map_left(ptr(open_rows), ptr(l), int(l.numel()), stream()), "vs_map_left")
-        still_open = select(l, r, pair_distances(l, r, qgeom, tgeom, exclusive))
+        # only distances up to the query's radius decide this round; after
+        # the radius-0 round the open points lie outside every polygon
+        still_open = select(l, r, pair_distances(l, r, qgeom, tgeom, exclusive, outside=pip, cap=radius))
     else:
         raise RuntimeError("nearest search did not close every query")
     lo = torch.cat(outs_l) if outs_l else torch.empty(0, dtype=torch.int64, device=dev)
