@@ -429,8 +429,10 @@ int vs_nearest_open(const double* boxes, const double* search, int64_t n_query, 
 /* For each listed query row q (rows, n_rows; query_boxes n x 4 over all
  * rows): an upper bound of its nearest distance to the tree's geometries --
  * the least, over the items, of the farthest-point distance from the query
- * box to the item box or to the item's rep point, by a nearest-first
- * depth-first walk of the tree pruned by the bound so far -- sets
+ * box to the item box or to the item's rep point, and for a point query
+ * against a polygonal tree (tree_kind 1: item_ids and the tree's polygon
+ * arrays as in vs_pair_distances) of the item's fp64 distance, by a
+ * nearest-first depth-first walk of the tree pruned by the bound so far -- sets
  * radius[q] = min(bound, cap) and search[q] = radius times (1 + 2^-40);
  * rows without a finite bound keep their radius.  rep: vs_nearest_rep_points
  * of the tree's items (tightens the bound).  row_boxes (n_rows x 4, e.g. the
@@ -438,8 +440,10 @@ int vs_nearest_open(const double* boxes, const double* search, int64_t n_query, 
  * centres, so a wave's lanes walk nearby parts of the tree). */
 int64_t vs_nearest_bound_workspace_bytes(int64_t n_rows);
 int vs_nearest_bound(const double* query_boxes, const int64_t* rows, int64_t n_rows, const double* row_boxes,
-                     const vs_tree_layout* layout, const double* node_boxes, const double* rep, double cap,
-                     double* radius, double* search, void* workspace, int64_t workspace_bytes, void* stream);
+                     const vs_tree_layout* layout, const double* node_boxes, const double* rep, int32_t tree_kind,
+                     const int32_t* item_ids, const double* tx, const double* ty, const int32_t* t_geom,
+                     const int32_t* t_part, const int32_t* t_ring, double cap, double* radius, double* search,
+                     void* workspace, int64_t workspace_bytes, void* stream);
 
 /* rep (n_items x 2 fp64, per item slot of a tree): a point of item
  * item_ids[slot] -- the point (kind 0: x, y per row) or the first coordinate

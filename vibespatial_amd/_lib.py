@@ -119,7 +119,7 @@ SIGNATURES = {
     "vs_nearest_finish_workspace_bytes": (_i64, [_i64]),
     "vs_nearest_finish": (_i32, [_p, _p, _p, _i64, _i64, _p, _p, _p, _p, _i64, _p]),
     "vs_nearest_bound_workspace_bytes": (_i64, [_i64]),
-    "vs_nearest_bound": (_i32, [_p, _p, _i64, _p, _LP, _p, _p, _d, _p, _p, _p, _i64, _p]),
+    "vs_nearest_bound": (_i32, [_p, _p, _i64, _p, _LP, _p, _p, _i32, _p] + [_p] * 5 + [_d, _p, _p, _p, _i64, _p]),
     "vs_nearest_rep_points": (_i32, [_i32, _p, _i64, _p, _p, _p, _p, _p, _p, _p]),
     "vs_scan_workspace_bytes": (_i64, [_i64]),
     "vs_exclusive_scan_i32": (ctypes.c_int, [_p, _p, _i64, _p, _p, _i64, _p]),

@@ -346,6 +346,9 @@ __global__ void count_to_result_kernel(const long long* __restrict__ total, int6
 struct NearestTree {
     const double4* box;
     const double2* rep;  // per item slot: a point of the item's geometry (NaN when empty)
+    Polys P;                 // polygonal items (exact distances from point queries) when polys
+    const int32_t* item_ids;  // item slot -> row of P
+    int32_t polys;
     int64_t level_start[VS_MAX_LEVELS];
     int64_t level_count[VS_MAX_LEVELS];
     int32_t n_levels;
@@ -355,7 +358,11 @@ struct NearestTree {
 // Upper bound of a query's nearest distance: the least, over the tree's
 // items, of the farthest-point distance from the query box to the item box or
 // to a point of the item (every geometry lies in its box and contains that
-// point, so its distance is at most either).
+// point, so its distance is at most either).  A point query against
+// polygonal items also takes the own fp64 distance (the value the pair
+// distances compute) of the first kBoundExact items the walk reaches (the
+// nearest boxes first): the bound is then usually the nearest distance
+// itself, at a bounded cost per query.
 // Depth-first over the packed tree, nearest child first, pruning nodes whose
 // box is farther (least axis-gap distance) than the bound so far, for at most
 // kBoundPops nodes.  A full stack or the visit limit only drops nodes, which
@@ -363,6 +370,7 @@ struct NearestTree {
 // finite box is met; scaled up by 2^-40 for the rounding of the fp64 values.
 constexpr int kBoundStack = 256;
 constexpr int kBoundPops = 4096;  // visit limit per query (bounds the time of a pathological query)
+constexpr int kBoundExact = 2;    // exact item distances per query
 
 __device__ __forceinline__ double box_gap2(const double4 q, const double4 b) {
     const double gx = fmax(0.0, fmax(b.x - q.z, q.x - b.z));
@@ -376,6 +384,7 @@ __device__ double nearest_upper_bound(const double4 q, const NearestTree& T) {
     const int last = T.n_levels - 1;
     for (int64_t c = 0; c < T.level_count[0] && sp < kBoundStack; ++c) stack[sp++] = c;  // level 0
     double ub2 = INFINITY;
+    int exact = (T.polys && q.x == q.z && q.y == q.w) ? kBoundExact : 0;
     for (int pops = 0; sp > 0 && pops < kBoundPops; ++pops) {
         const int64_t e = stack[--sp];
         const int l = (int)(e >> 48);
@@ -391,6 +400,13 @@ __device__ double nearest_upper_bound(const double4 q, const NearestTree& T) {
             if (p.x == p.x) {
                 const double ex = fmax(fabs(p.x - q.x), fabs(p.x - q.z)), ey = fmax(fabs(p.y - q.y), fabs(p.y - q.w));
                 ub2 = fmin(ub2, ex * ex + ey * ey);
+            }
+            if (exact > 0) {
+                --exact;
+                // at most the box bound just taken up to rounding; +inf
+                // (rounding, empty row) leaves the bound as it is
+                const double d = point_polygon_dist_f64(q.x, q.y, T.P, T.item_ids[c], false, sqrt(ub2));
+                if (d < INFINITY) ub2 = fmin(ub2, d * d);
             }
             continue;
         }
@@ -627,8 +643,10 @@ int64_t vs_nearest_bound_workspace_bytes(int64_t n_rows) {
 }
 
 int vs_nearest_bound(const double* query_boxes, const int64_t* rows, int64_t n_rows, const double* row_boxes,
-                     const vs_tree_layout* layout, const double* node_boxes, const double* rep, double cap,
-                     double* radius, double* search, void* workspace, int64_t workspace_bytes, void* stream) {
+                     const vs_tree_layout* layout, const double* node_boxes, const double* rep, int32_t tree_kind,
+                     const int32_t* item_ids, const double* tx, const double* ty, const int32_t* t_geom,
+                     const int32_t* t_part, const int32_t* t_ring, double cap, double* radius, double* search,
+                     void* workspace, int64_t workspace_bytes, void* stream) {
     vs::clear_error();
     VS_REQUIRE(n_rows >= 0 && n_rows < (int64_t)INT32_MAX, "vs_nearest_bound: n_rows out of range");
     VS_REQUIRE(workspace_bytes >= vs_nearest_bound_workspace_bytes(n_rows), "vs_nearest_bound: workspace too small");
@@ -637,6 +655,9 @@ int vs_nearest_bound(const double* query_boxes, const int64_t* rows, int64_t n_r
     if (n_rows == 0 || layout->n_items == 0) return VS_OK;
     VS_REQUIRE(query_boxes && rows && row_boxes && node_boxes && rep && radius && search,
                "vs_nearest_bound: null pointer");
+    VS_REQUIRE(tree_kind == 0 || tree_kind == 1, "vs_nearest_bound: tree_kind must be 0 (points) or 1 (polygons)");
+    VS_REQUIRE(tree_kind == 0 || (item_ids && tx && ty && t_geom && t_part && t_ring),
+               "vs_nearest_bound: polygonal tree needs item_ids and its arrays");
     hipStream_t s = vs::stream_of(stream);
     vs::Carver cw(workspace);
     int32_t* perm = cw.take<int32_t>(n_rows);
@@ -646,6 +667,11 @@ int vs_nearest_bound(const double* query_boxes, const int64_t* rows, int64_t n_r
     memset(&T, 0, sizeof(T));
     T.box = reinterpret_cast<const double4*>(node_boxes);
     T.rep = reinterpret_cast<const double2*>(rep);
+    if (tree_kind == 1) {
+        T.P = vs::Polys{tx, ty, t_geom, t_part, t_ring};
+        T.item_ids = item_ids;
+        T.polys = 1;
+    }
     T.n_levels = layout->n_levels;
     T.fanout = layout->fanout;
     for (int l = 0; l < layout->n_levels; ++l) {

@@ -245,8 +245,8 @@ def nearest_pairs(qboxes: torch.Tensor, qgeom, tree: DeviceTree, tgeom, *, max_d
               "vs_nearest_rep_points")
         bws = _ws(L.vs_nearest_bound_workspace_bytes(m), dev)
         check(L.vs_nearest_bound(ptr(qboxes), ptr(open_rows), m, ptr(open_boxes), ctypes.byref(tree.layout),
-                                 ptr(tree.node_boxes), ptr(rep), final, ptr(radius), ptr(search), ptr(bws), bws.numel(),
-                                 stream()), "vs_nearest_bound")
+                                 ptr(tree.node_boxes), ptr(rep), tk, ptr(tree.item_ids), *ta, final, ptr(radius),
+                                 ptr(search), ptr(bws), bws.numel(), stream()), "vs_nearest_bound")
     # candidate pairs per open query: a box query whose output overflows its
     # capacity traverses the tree again, so the capacity starts generous and
     # follows what the previous round found
