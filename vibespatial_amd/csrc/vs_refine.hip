@@ -48,8 +48,14 @@ __device__ __forceinline__ int loc_keep(int pred, int loc) {
 // (count in *n_heavy) for refine_heavy_kernel, one wave per pair; pairs the
 // cheap tests leave undecided go to `undecided` (count in *n_undecided) for
 // refine_relate_kernel, so this kernel carries none of the full relation's
-// registers.
-constexpr int64_t kHeavyPair = 1 << 15;
+// registers.  At 2^8 small pairs (box x box: 25) stay one per lane; one lane
+// walking the edge pairs of two many-vertex polygons leaves the rest of its
+// wave idle, so those go one wave per pair (100k x 100k star polygons,
+// overlaps: 49 ms per step at 2^15, 24 at 2^12, 15.7 at 2^8, flat below).
+#ifndef VS_HEAVY_PAIR_LOG2
+#define VS_HEAVY_PAIR_LOG2 8
+#endif
+constexpr int64_t kHeavyPair = (int64_t)1 << VS_HEAVY_PAIR_LOG2;
 
 __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, const int64_t* __restrict__ left,
                                                           const int64_t* __restrict__ right, int64_t n,
