@@ -480,12 +480,11 @@ def run_dwithin(args):
     n = len(w.points)
     d = torch.full((n,), float(args.distance), dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
-    state = {}
 
     def step():
         qb = J.expand_boxes(J.point_boxes(Q), d)
-        l, r, st = J.query_boxes(qb, tree, capacity=state.get("cap"))
-        state["cap"] = st.pairs + 1024
+        # output sized by the library (the tree's last hits per query box)
+        l, r, st = J.query_boxes(qb, tree)
         kl, kr = J.refine_dwithin(l, r, d, Q, P)
         return int(l.numel()), int(kl.numel())
 

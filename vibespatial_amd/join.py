@@ -54,6 +54,9 @@ class DeviceTree:
         self.item_ids = item_ids
         self.item_keys = item_keys
         self.item_bounds = item_bounds
+        # staged pairs per query box of the last box query over this tree:
+        # sizes the next query's output (an undersized one traverses twice)
+        self.hits_per_query = 0.0
 
     @property
     def n_items(self) -> int:
@@ -351,7 +354,8 @@ def query_boxes(qboxes: torch.Tensor, tree: DeviceTree, capacity: int | None = N
 
     def run(qb):
         n = int(qb.shape[0])
-        cap = int(capacity) if capacity is not None else max(1024, 2 * n)
+        per_query = max(2.0, 1.25 * tree.hits_per_query)
+        cap = int(capacity) if capacity is not None else max(1024, int(per_query * n))
         cap = min(cap, PAIR_LIMIT)
         retries = 0
         while True:
@@ -366,6 +370,8 @@ def query_boxes(qboxes: torch.Tensor, tree: DeviceTree, capacity: int | None = N
                 "vs_query_boxes",
             )
             k, staged, cand, cverts = _read_result(res)
+            if n:
+                tree.hits_per_query = staged / n
             if staged <= cap:
                 return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts)
             if staged > PAIR_LIMIT:
