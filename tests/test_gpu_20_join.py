@@ -244,3 +244,24 @@ def test_upstream_world_cities_shape():
     countries = GeoDataFrame({"country": world["name"].to_numpy(), "geometry": world.geometry})
     res = sjoin(cities, countries, how="inner", predicate="intersects")
     assert res.shape == (213, 4)
+
+
+def test_box_query_output_sized_from_previous_queries():
+    print("box query sizing", flush=True)
+    # one tree, queries whose hits per box go up and down: the output size
+    # follows the last query (too small -> a second pass, too large -> slack)
+    rng = np.random.default_rng(31)
+    c = rng.uniform(0, 1000, (5000, 2))
+    s = rng.uniform(1, 20, (5000, 2))
+    tb = np.column_stack([c - s, c + s])
+    tree = J.build_tree(torch.as_tensor(tb, device="cuda"))
+    expect_retries = []
+    for half in (30.0, 1.0, 200.0, 0.5, 30.0):
+        q = rng.uniform(0, 1000, (3000, 2))
+        qb = np.column_stack([q - half, q + half])
+        l, r, st = J.query_boxes(torch.as_tensor(qb, device="cuda"), tree)
+        el, er = oracle.bbox_pairs(qb, tb)
+        order = np.lexsort((er, el))
+        assert np.array_equal(l.cpu().numpy(), el[order]) and np.array_equal(r.cpu().numpy(), er[order]), half
+        expect_retries.append(st.retries)
+    assert expect_retries[2] == 1 and expect_retries[3] == 0  # grown past the hint, then slack
