@@ -350,33 +350,38 @@ __device__ inline int32_t row_coords(const Polys& P, int64_t row) {
 // properly.
 __device__ inline bool wave_any_edge_pair(int mode, const Polys& A, int64_t ra, const Polys& B, int64_t rb,
                                           int lane) {
-    for (int32_t r = A.part[A.geom[ra]]; r < A.part[A.geom[ra + 1]]; ++r) {
-        const int32_t e0 = A.ring[r] + 1, e1 = A.ring[r + 1];
-        for (int32_t base = e0; base < e1; base += 64) {
-            const int32_t c = base + lane;
-            bool hit = false;
-            if (c < e1) {
-                const double ax = A.x[c - 1], ay = A.y[c - 1], bx = A.x[c], by = A.y[c];
-                for (int32_t q = B.part[B.geom[rb]]; q < B.part[B.geom[rb + 1]] && !hit; ++q)
-                    for (int32_t d = B.ring[q] + 1; d < B.ring[q + 1]; ++d) {
-                        const double cx = B.x[d - 1], cy = B.y[d - 1], dx = B.x[d], dy = B.y[d];
-                        if (fmax(ax, bx) < fmin(cx, dx) || fmax(cx, dx) < fmin(ax, bx) ||
-                            fmax(ay, by) < fmin(cy, dy) || fmax(cy, dy) < fmin(ay, by))
-                            continue;
-                        if (mode == 0) {
-                            if (segments_touch(ax, ay, bx, by, cx, cy, dx, dy)) {
-                                hit = true;
-                                break;
-                            }
-                        } else if (orient2d(ax, ay, bx, by, cx, cy) * orient2d(ax, ay, bx, by, dx, dy) < 0 &&
-                                   orient2d(cx, cy, dx, dy, ax, ay) * orient2d(cx, cy, dx, dy, bx, by) < 0) {
+    // the row's rings are contiguous coordinates: lanes take its edges
+    // (c - 1, c) across every ring at once, c not the start of a ring
+    const int32_t r0 = A.part[A.geom[ra]], r1 = A.part[A.geom[ra + 1]];
+    if (r1 <= r0) return false;
+    const int32_t e0 = A.ring[r0] + 1, e1 = A.ring[r1];
+    for (int32_t base = e0; base < e1; base += 64) {
+        const int32_t c = base + lane;
+        bool hit = false;
+        bool edge = c < e1;
+        for (int32_t r = r0 + 1; edge && r < r1; ++r)
+            if (A.ring[r] == c) edge = false;
+        if (edge) {
+            const double ax = A.x[c - 1], ay = A.y[c - 1], bx = A.x[c], by = A.y[c];
+            for (int32_t q = B.part[B.geom[rb]]; q < B.part[B.geom[rb + 1]] && !hit; ++q)
+                for (int32_t d = B.ring[q] + 1; d < B.ring[q + 1]; ++d) {
+                    const double cx = B.x[d - 1], cy = B.y[d - 1], dx = B.x[d], dy = B.y[d];
+                    if (fmax(ax, bx) < fmin(cx, dx) || fmax(cx, dx) < fmin(ax, bx) ||
+                        fmax(ay, by) < fmin(cy, dy) || fmax(cy, dy) < fmin(ay, by))
+                        continue;
+                    if (mode == 0) {
+                        if (segments_touch(ax, ay, bx, by, cx, cy, dx, dy)) {
                             hit = true;
                             break;
                         }
+                    } else if (orient2d(ax, ay, bx, by, cx, cy) * orient2d(ax, ay, bx, by, dx, dy) < 0 &&
+                               orient2d(cx, cy, dx, dy, ax, ay) * orient2d(cx, cy, dx, dy, bx, by) < 0) {
+                        hit = true;
+                        break;
                     }
-            }
-            if (__ballot(hit) != 0) return true;
+                }
         }
+        if (__ballot(hit) != 0) return true;
     }
     return false;
 }
