@@ -421,6 +421,46 @@ __device__ inline double4 wave_row_box(const Polys& P, int64_t row, int lane) {
     return b;
 }
 
+// ring_interior_left by one wave (every lane returns the answer): lanes scan
+// strided vertices for the lowest (then leftmost, then first) one and a
+// shuffle reduction picks it -- the vertex the serial scan finds.
+__device__ inline bool wave_ring_interior_left(const Polys& P, int32_t r, bool shell, int lane) {
+    const int32_t c0 = P.ring[r], c1 = P.ring[r + 1] - 1;  // last coordinate closes the ring
+    const int32_t m = c1 - c0;
+    if (m < 3) return shell;
+    int32_t k = c1;  // c1: no vertex yet
+    double ky = INFINITY, kx = INFINITY;
+    for (int32_t c = c0 + lane; c < c1; c += 64) {
+        const double y = P.y[c], x = P.x[c];
+        if (k == c1 || y < ky || (y == ky && x < kx)) {
+            k = c;
+            ky = y;
+            kx = x;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t k2 = __shfl_xor(k, o, 64);
+        const double y2 = __shfl_xor(ky, o, 64), x2 = __shfl_xor(kx, o, 64);
+        if (k2 != c1 && (k == c1 || y2 < ky || (y2 == ky && (x2 < kx || (x2 == kx && k2 < k))))) {
+            k = k2;
+            ky = y2;
+            kx = x2;
+        }
+    }
+    k = __shfl(k, 0, 64);  // one vertex for every lane (NaN coordinates compare unordered)
+    int32_t a = k, b = k;  // neighbours distinct from the vertex (repeated points skipped)
+    for (int32_t s = 1; s < m; ++s) {
+        a = c0 + ((k - c0 - s) % m + m) % m;
+        if (P.x[a] != P.x[k] || P.y[a] != P.y[k]) break;
+    }
+    for (int32_t s = 1; s < m; ++s) {
+        b = c0 + (k - c0 + s) % m;
+        if (P.x[b] != P.x[k] || P.y[b] != P.y[k]) break;
+    }
+    const bool ccw = orient2d(P.x[a], P.y[a], P.x[k], P.y[k], P.x[b], P.y[b]) > 0;
+    return shell ? ccw : !ccw;
+}
+
 // relate_side by one wave: lanes split each ring's edges, votes merge.
 __device__ inline RelSide wave_relate_side(const Polys& X, int64_t rx, const Polys& Y, int64_t ry, bool first,
                                            int lane) {
@@ -429,7 +469,7 @@ __device__ inline RelSide wave_relate_side(const Polys& X, int64_t rx, const Pol
     for (int32_t pp = X.geom[rx]; pp < X.geom[rx + 1]; ++pp) {
         bool part_in = false;
         for (int32_t r = X.part[pp]; r < X.part[pp + 1]; ++r) {
-            const bool left_x = ring_interior_left(X, r, r == X.part[pp]);
+            const bool left_x = wave_ring_interior_left(X, r, r == X.part[pp], lane);
             for (int32_t c = X.ring[r] + 1 + lane; c < X.ring[r + 1]; c += 64)
                 relate_edge(X, c, left_x, Y, ry, first, S, part_in);
         }
