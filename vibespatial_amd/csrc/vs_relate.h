@@ -351,12 +351,16 @@ __device__ inline int32_t row_coords(const Polys& P, int64_t row) {
 __device__ inline bool wave_any_edge_pair(int mode, const Polys& A, int64_t ra, const Polys& B, int64_t rb,
                                           int lane) {
     // the row's rings are contiguous coordinates: lanes take its edges
-    // (c - 1, c) across every ring at once, c not the start of a ring
+    // (c - 1, c) across every ring at once, c not the start of a ring; a row
+    // of at most 32 (16) positions gives each edge 2 (4) lanes, which split
+    // the other row's edges by stride
     const int32_t r0 = A.part[A.geom[ra]], r1 = A.part[A.geom[ra + 1]];
     if (r1 <= r0) return false;
     const int32_t e0 = A.ring[r0] + 1, e1 = A.ring[r1];
-    for (int32_t base = e0; base < e1; base += 64) {
-        const int32_t c = base + lane;
+    const int split = e1 - e0 <= 16 ? 4 : (e1 - e0 <= 32 ? 2 : 1);
+    const int sub = lane % split;
+    for (int32_t base = e0; base < e1; base += 64 / split) {
+        const int32_t c = base + lane / split;
         bool hit = false;
         bool edge = c < e1;
         for (int32_t r = r0 + 1; edge && r < r1; ++r)
@@ -364,7 +368,7 @@ __device__ inline bool wave_any_edge_pair(int mode, const Polys& A, int64_t ra, 
         if (edge) {
             const double ax = A.x[c - 1], ay = A.y[c - 1], bx = A.x[c], by = A.y[c];
             for (int32_t q = B.part[B.geom[rb]]; q < B.part[B.geom[rb + 1]] && !hit; ++q)
-                for (int32_t d = B.ring[q] + 1; d < B.ring[q + 1]; ++d) {
+                for (int32_t d = B.ring[q] + 1 + sub; d < B.ring[q + 1]; d += split) {
                     const double cx = B.x[d - 1], cy = B.y[d - 1], dx = B.x[d], dy = B.y[d];
                     if (fmax(ax, bx) < fmin(cx, dx) || fmax(cx, dx) < fmin(ax, bx) ||
                         fmax(ay, by) < fmin(cy, dy) || fmax(cy, dy) < fmin(ay, by))
