@@ -252,9 +252,9 @@ __global__ __launch_bounds__(256) void nearest_radius_kernel(int64_t nq, double*
                                                             double switch_radius, double final_radius,
                                                             unsigned long long* __restrict__ open) {
     __shared__ int cnt[4];
-    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int o = 0;
-    if (q < nq) {
+    // grid-stride: one count atomic per block, few blocks
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < nq; q += (int64_t)gridDim.x * 256) {
         const double r = radius[q];
         if (r == r) {
             const double m = __longlong_as_double((long long)mind[q]);
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void nearest_radius_kernel(int64_t nq, double*
             } else {
                 nr = r > 0.0 ? r * growth : start_radius;
                 if (!(nr < switch_radius)) nr = final_radius;
-                o = 1;
+                ++o;
             }
             radius[q] = nr;
             search[q] = nr * kSearchSlack;
@@ -767,7 +767,8 @@ int vs_nearest_select(const int64_t* left, const int64_t* right, const double* d
         VS_LAUNCH_CHECK();
     }
     if (n_query > 0) {
-        hipLaunchKernelGGL(vs::nearest_radius_kernel, dim3((unsigned)vs::ceil_div(n_query, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL(vs::nearest_radius_kernel, dim3((unsigned)std::min<int64_t>(vs::ceil_div(n_query, 256), 2048)),
+                           dim3(256), 0, s,
                            n_query, radius, search, W.mind, growth, start_radius, switch_radius, final_radius,
                            W.open);
         VS_LAUNCH_CHECK();
