@@ -282,6 +282,26 @@ int vs_learned_query_boxes(const void* index, int64_t n_points, int32_t key_byte
                            int32_t sort, int64_t* out_left, int64_t* out_right, int64_t* d_result,
                            void* workspace, int64_t workspace_bytes, void* stream);
 
+int64_t vs_learned_query_polygons_workspace_bytes(int64_t n_polys, int64_t n_coords, int64_t capacity);
+
+/* sindex.query(polygons, predicate) on a point tree through its learned
+ * index: (polygon row, point) pairs with predicate(polygon, point) --
+ * VS_PRED_CONTAINS / VS_PRED_CONTAINS_PROPERLY: the point is interior,
+ * VS_PRED_TOUCHES: on the boundary, VS_PRED_COVERS / VS_PRED_INTERSECTS: not
+ * outside -- decided exactly (the FP32 refine with escalation of the learned
+ * join); `sort` as for vs_query_boxes; d_result int64[16] as for the learned
+ * join.  poly_bounds from vs_polygon_bounds.  Replaces the polygon-over-point
+ * route of query_spatial_index on point trees (spatial/query_box.py:760-863
+ * for the candidate boxes, predicates/point_relations.py:130-146 for the
+ * swapped predicate mapping). */
+int vs_learned_query_polygons(const void* index, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
+                              const double* x, const double* y, const int32_t* geom_offsets,
+                              const int32_t* part_offsets, const int32_t* ring_offsets, int64_t n_polys,
+                              int64_t n_parts, int64_t n_rings, int64_t n_coords, const double* poly_bounds,
+                              int32_t predicate, int32_t lookup_mode, int64_t capacity, int32_t sort,
+                              int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                              int64_t workspace_bytes, void* stream);
+
 int64_t vs_sjoin_points_polygons_learned_workspace_bytes(int64_t n_points, int64_t n_polys, int64_t n_coords,
                                                          int64_t capacity);
 

@@ -95,6 +95,12 @@ SIGNATURES = {
     "vs_learned_query_boxes_workspace_bytes": (_i64, [_i64, _i64]),
     "vs_learned_query_boxes": (ctypes.c_int,
                                [_p, _i64, _i32, _i32, _p, _i64, _i32, _i64, _i32, _p, _p, _p, _p, _i64, _p]),
+    "vs_learned_query_polygons_workspace_bytes": (_i64, [_i64, _i64, _i64]),
+    "vs_learned_query_polygons": (
+        ctypes.c_int,
+        [_p, _i64, _i32, _i32, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _p, _i32, _i32, _i64, _i32, _p, _p, _p,
+         _p, _i64, _p],
+    ),
     "vs_sjoin_points_polygons_learned_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
     "vs_sjoin_points_polygons_learned": (
         ctypes.c_int,

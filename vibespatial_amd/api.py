@@ -10,20 +10,31 @@ sorts only on request, ``query_utils.py:202-206``; a sorted result satisfies
 both contracts).
 
 Routing (all compute on the GPU through libvsjoin; host numpy only maps rows of
-mixed point/polygon columns and assembles frames):
-  point  x polygon  within / intersects / covered_by / touches -> fused join
-  polygon x point   contains / contains_properly / covers / intersects / touches
-                    -> bbox query + exact location refine (mode 2)
-  polygon x polygon intersects -> bbox query + exact edge/vertex refine (mode 1)
+mixed point/polygon columns and assembles frames), chosen per family pair by
+``_strategy`` (the reference's choice is spatial/spatial_index_device.py:228-390):
+  point  x polygon  within / intersects / covered_by / touches -> learned join:
+                    rank index over the query points, polygon-major FP32
+                    refine with exact escalation (vs_sjoin_points_polygons_learned)
+  polygon x point   (point tree) contains / contains_properly / covers /
+                    intersects / touches -> the tree's learned index and the
+                    same FP32 refine (vs_learned_query_polygons); axis-aligned
+                    rectangles with intersects / covers, and predicate None,
+                    -> the learned box query alone (vs_learned_query_boxes)
   point  x point    intersects / within / contains / covers / covered_by /
-                    contains_properly -> bbox query (boxes are the points)
-  predicate None    -> bbox query
-  dwithin (every family pair) -> query with boxes grown by each query row's
-                    distance, exact distance refine (vs_refine_dwithin)
+                    contains_properly / None -> learned box query (equality)
+  polygon x polygon every predicate -> packed-tree bbox query + exact refine
+                    (vs_refine_pairs mode 1: intersects, and the areal
+                    relations within / contains / covers / covered_by /
+                    contains_properly / touches / overlaps of vs_relate.h,
+                    with heavy pairs one wave per pair and undecided pairs
+                    through the wave-parallel relation)
+  point  x polygon  predicate None -> packed-tree bbox query
+  dwithin (every family pair) -> packed-tree query with boxes grown by each
+                    query row's distance, exact distance refine
+                    (vs_refine_dwithin)
 Combinations whose answer is empty by definition (a point cannot contain a
-polygon, points cannot cross, ...) return no pairs, matching the reference's
-mapping (predicates/point_relations.py:100-146).  Polygon x polygon predicates
-other than intersects and dwithin (full DE-9IM) raise NotImplementedError.
+polygon, points cannot cross, two areal rows never cross, ...) return no pairs,
+matching the reference's mapping (predicates/point_relations.py:100-146).
 """
 
 from __future__ import annotations
@@ -116,36 +127,135 @@ def _family_bounds_device(ga: GeometryArray):
 
 
 class _FamilyIndex:
-    """Packed trees over the tree column, one per family (built lazily)."""
+    """Indexes over the tree column, one set per family, built lazily and kept
+    with the column: the packed Hilbert tree of each family, the polygon
+    family's device bounds, and a learned rank index over the point family
+    (64-bit Hilbert keys)."""
 
     def __init__(self, ga: GeometryArray):
         self.ga = ga
         self._trees = {}
+        self._bounds = {}
+        self._learned = None
 
-    def tree(self, family: str) -> J.DeviceTree:
-        t = self._trees.get(family)
-        if t is None:
+    def bounds(self, family: str) -> torch.Tensor:
+        b = self._bounds.get(family)
+        if b is None:
             if family == "point":
                 b = J.point_boxes(device_points(self.ga.points))
             elif family == "polygon":
                 b = J.polygon_bounds(device_polygons(self.ga.polygons))
             else:
                 b = _family_bounds_device(self.ga)
-            t = J.build_tree(b)
+            self._bounds[family] = b
+        return b
+
+    def tree(self, family: str) -> J.DeviceTree:
+        t = self._trees.get(family)
+        if t is None:
+            t = J.build_tree(self.bounds(family))
             self._trees[family] = t
         return t
+
+    def learned(self) -> J.LearnedPointIndex:
+        if self._learned is None:
+            self._learned = J.LearnedPointIndex(device_points(self.ga.points), key_bytes=8)
+        return self._learned
 
 
 def _dev_geom(fam: str, arr):
     return device_points(arr) if fam == "point" else device_polygons(arr)
 
 
-def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, predicate, dist=None):
+# Point x polygon predicates the fused learned join decides (location codes).
+_POINT_POLY_PREDS = ("within", "intersects", "covered_by", "touches")
+# Polygon-over-point predicates of the learned polygon query.
+_POLY_POINT_PREDS = ("contains", "contains_properly", "covers", "intersects", "touches")
+
+
+def all_rectangles(pa: PolygonArray) -> bool:
+    """Every non-empty row is one axis-aligned rectangle with positive width
+    and height (one part, one closed ring of five coordinates, no holes).  For
+    such rows `intersects` / `covers` of a point equals the inclusive point-in-
+    box test.  Cached on the array."""
+    cached = getattr(pa, "_all_rect", None)
+    if cached is not None:
+        return cached
+    g, p, r = pa.geom_offsets, pa.part_offsets, pa.ring_offsets
+    nparts = np.diff(g)
+    ok = bool(np.all(nparts <= 1))
+    if ok:
+        rows = np.flatnonzero(nparts == 1)
+        parts = g[rows]
+        ok = bool(np.all(p[parts + 1] - p[parts] == 1))
+        if ok and len(rows):
+            rings = p[parts]
+            ok = bool(np.all(r[rings + 1] - r[rings] == 5))
+            if ok:
+                c = r[rings][:, None] + np.arange(5)
+                x, y = pa.x[c], pa.y[c]
+                closed = (x[:, 0] == x[:, 4]) & (y[:, 0] == y[:, 4])
+                # edges alternate vertical / horizontal, starting either way
+                a = (x[:, 0] == x[:, 1]) & (y[:, 1] == y[:, 2]) & (x[:, 2] == x[:, 3]) & (y[:, 3] == y[:, 4])
+                b = (y[:, 0] == y[:, 1]) & (x[:, 1] == x[:, 2]) & (y[:, 2] == y[:, 3]) & (x[:, 3] == x[:, 4])
+                area = (x[:, 0] != x[:, 2]) & (y[:, 0] != y[:, 2])
+                ok = bool(np.all(closed & (a | b) & area))
+    pa._all_rect = ok
+    return ok
+
+
+def _strategy(q_fam: str, t_fam: str, predicate, qarr) -> str:
+    """The device route of one family pair (the reference chooses among
+    point-grid, Morton-range and brute-force candidates in
+    spatial/spatial_index_device.py:228-390; here):
+      learned_join     point x polygon predicates: learned rank index over the
+                       query points built per call, polygon-major FP32 refine
+                       with exact escalation (vs_sjoin_points_polygons_learned);
+      learned_boxes    point trees queried by boxes whose test is exact:
+                       predicate None (bbox), points (equality), or
+                       axis-aligned rectangles with intersects / covers
+                       (spatial/query_box.py:760-863: the pure box path);
+      learned_polygons other polygon-over-point predicates on point trees:
+                       the FP32 point-in-polygon refine over the tree's
+                       learned index (vs_learned_query_polygons);
+      tree             polygon trees queried by polygons, bbox-only point
+                       queries of polygon trees, and dwithin: packed Hilbert
+                       tree + exact refine."""
+    if predicate == "dwithin":
+        return "tree"
+    if t_fam == "point":
+        if q_fam == "point" or predicate is None:
+            return "learned_boxes"
+        if predicate in ("intersects", "covers") and all_rectangles(qarr):
+            return "learned_boxes"
+        return "learned_polygons"
+    if q_fam == "point" and predicate in _POINT_POLY_PREDS:
+        return "learned_join"
+    return "tree"
+
+
+def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, predicate, dist=None, log=None):
     """Pairs (query family idx, tree family idx), lexsorted, device int64.
-    dist: per query-family row distances (fp64) for dwithin."""
+    dist: per query-family row distances (fp64) for dwithin.  log: a list the
+    chosen strategy is appended to."""
     dev = require_gpu()
     if predicate in _EMPTY_PAIRS.get((q_fam, t_fam), ()):
         return _empty_pairs(dev)
+    how = _strategy(q_fam, t_fam, predicate, qarr)
+    if log is not None:
+        log.append((q_fam, t_fam, how))
+    if how == "learned_join":
+        l, r, _ = J.sjoin_points_polygons_learned(device_points(qarr), device_polygons(tarr),
+                                                  tindex.bounds("polygon"), predicate)
+        return l, r
+    if how == "learned_boxes":
+        qb = J.point_boxes(device_points(qarr)) if q_fam == "point" else J.polygon_bounds(device_polygons(qarr))
+        l, r, _ = tindex.learned().query_boxes(qb)
+        return l, r
+    if how == "learned_polygons":
+        P = device_polygons(qarr)
+        l, r, _ = tindex.learned().query_polygons(P, J.polygon_bounds(P), predicate)
+        return l, r
     tree = tindex.tree(t_fam)
     if predicate == "dwithin":
         # boxes grown by each row's distance (outward-rounded), then the exact
@@ -154,16 +264,10 @@ def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, pred
         qb = J.point_boxes(device_points(qarr)) if q_fam == "point" else J.polygon_bounds(device_polygons(qarr))
         l, r, _ = J.query_boxes(J.expand_boxes(qb, d), tree)
         return J.refine_dwithin(l, r, d, _dev_geom(q_fam, qarr), _dev_geom(t_fam, tarr))
-    if q_fam == "point" and t_fam == "polygon" and predicate is not None:
-        l, r, _ = J.sjoin_points_polygons(device_points(qarr), device_polygons(tarr), tree, predicate)
-        return l, r
     qb = J.point_boxes(device_points(qarr)) if q_fam == "point" else J.polygon_bounds(device_polygons(qarr))
     l, r, _ = J.query_boxes(qb, tree)
-    if predicate is None or (q_fam == "point" and t_fam == "point"):
+    if predicate is None:
         return l, r
-    if q_fam == "polygon" and t_fam == "point":
-        pred = {"contains_properly": "contains"}.get(predicate, predicate)
-        return _refine(2, pred, l, r, tarr, None, qarr)
     if q_fam == "polygon" and t_fam == "polygon":
         if l.numel() == 0:
             return l, r  # no bbox overlap: every predicate but disjoint is false
@@ -178,9 +282,11 @@ def _map_rows(idx: torch.Tensor, rows: np.ndarray) -> torch.Tensor:
     return torch.as_tensor(rows, device=idx.device)[idx]
 
 
-def query_pairs(query: GeometryArray, tree_ga: GeometryArray, tindex: _FamilyIndex, predicate, distance=None):
+def query_pairs(query: GeometryArray, tree_ga: GeometryArray, tindex: _FamilyIndex, predicate, distance=None,
+                log=None):
     """Lexsorted (query_row, tree_row) pairs as device int64 tensors.
-    distance: per query row fp64 array (dwithin only)."""
+    distance: per query row fp64 array (dwithin only); log: a list that
+    receives (query family, tree family, strategy) per family pair."""
     dev = require_gpu()
     fams_q = [(f, a, rows) for f, a, rows in (("point", query.points, query.point_rows),
                                                ("polygon", query.polygons, query.polygon_rows)) if a is not None]
@@ -189,13 +295,13 @@ def query_pairs(query: GeometryArray, tree_ga: GeometryArray, tindex: _FamilyInd
     if len(fams_q) == 1 and len(fams_t) == 1:
         (qf, qa, qrows), (tf, ta, trows) = fams_q[0], fams_t[0]
         dist = None if distance is None else distance[qrows]
-        l, r = _pairs_family(qf, qa, tf, ta, tindex, predicate, dist)
+        l, r = _pairs_family(qf, qa, tf, ta, tindex, predicate, dist, log)
         return _map_rows(l, qrows), _map_rows(r, trows)
     parts_l, parts_r = [], []
     for qf, qa, qrows in fams_q:
         for tf, ta, trows in fams_t:
             dist = None if distance is None else distance[qrows]
-            l, r = _pairs_family(qf, qa, tf, ta, tindex, predicate, dist)
+            l, r = _pairs_family(qf, qa, tf, ta, tindex, predicate, dist, log)
             parts_l.append(qrows[l.cpu().numpy()])
             parts_r.append(trows[r.cpu().numpy()])
     if not parts_l:
@@ -215,6 +321,9 @@ class SpatialIndex:
     def __init__(self, geometry):
         self._ga = as_geometry_array(geometry)
         self._index = _FamilyIndex(self._ga)
+        # (query family, tree family, strategy) of each family pair of the
+        # last query (see _strategy)
+        self.last_strategy = []
 
     @property
     def valid_query_predicates(self):
@@ -265,7 +374,8 @@ class SpatialIndex:
                     dist = np.ascontiguousarray(np.broadcast_to(np.asarray(distance, dtype=np.float64), (len(q),)))
                 except ValueError as exc:
                     raise ValueError("distance array must be broadcastable to the geometry input") from exc
-            l, r = query_pairs(q, self._ga, self._index, predicate, dist)
+            self.last_strategy = []
+            l, r = query_pairs(q, self._ga, self._index, predicate, dist, self.last_strategy)
         if return_device:
             return r if scalar else torch.stack([l, r])
         lh, rh = l.cpu().numpy(), r.cpu().numpy()
@@ -404,8 +514,15 @@ class GeoSeries:
         return pd.Series([names[int(t)] for t in self.values.tags], index=self.index)
 
     def take(self, idx) -> "GeoSeries":
-        idx = np.asarray(idx, dtype=np.int64)
-        return GeoSeries([self.values[int(i)] if i >= 0 else None for i in idx], crs=self.crs, name=self.name)
+        """Rows idx (positions; -1 gives a null row), buffers gathered without
+        per-row objects."""
+        return GeoSeries(self.values.take(np.asarray(idx, dtype=np.int64)), crs=self.crs, name=self.name)
+
+    def isna(self) -> pd.Series:
+        return pd.Series(self.values.tags < 0, index=self.index)
+
+    def notna(self) -> pd.Series:
+        return pd.Series(self.values.tags >= 0, index=self.index)
 
 
 class GeoDataFrame:
@@ -628,8 +745,11 @@ def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribu
         g = GeoDataFrame(out, geometry=geoms, crs=right_df.crs)
         g._geom_name = right_df._geom_name
         return g
+    if how == "outer":
+        return _assemble_outer(left_df, right_df, l_idx, r_idx, index_left, index_right, lcols, rcols, lren, rren,
+                               extra)
     extra_cols = {}
-    if how in ("left", "outer"):
+    if how == "left":
         matched = np.zeros(len(ldf), dtype=bool)
         matched[l_idx] = True
         miss = np.flatnonzero(~matched)
@@ -651,16 +771,79 @@ def _assemble(left_df, right_df, l_idx, r_idx, how, lsuffix, rsuffix, on_attribu
     out[index_right] = np.where(valid, ridx[np.maximum(r_all, 0)] if len(ridx) else 0, np.nan) \
         if how != "inner" else (ridx[r_all] if len(r_all) else np.empty(0, dtype=ridx.dtype))
     for c in rcols:
-        col = rdf[c].to_numpy()
-        vals = col[np.maximum(r_all, 0)] if len(col) else np.empty(len(r_all))
-        if how != "inner" and (~valid).any():
-            vals = vals.astype(object) if vals.dtype.kind not in "fc" else vals.astype(float)
-            vals[~valid] = np.nan
-        out[rren.get(c, c)] = vals
+        out[rren.get(c, c)] = _column_take(rdf[c].to_numpy(), r_all)
     for name, vals in extra_cols.items():
         out[name] = vals
     geoms = left_df.geometry.take(l_all)
     g = GeoDataFrame(out, geometry=geoms, crs=left_df.crs)
     g._geom_name = left_df._geom_name
     g._geom_at = len(lcols)
+    return g
+
+
+def outer_indexers(l_idx, r_idx, n_left: int, n_right: int):
+    """Row indexers of an outer join (api/_native_results.py:402-448): every
+    left row in order, repeated once per match (once, with right -1, when it
+    has none), then the right rows no left row matched (left -1).  Pairs are
+    taken in lexsorted order.  Returns (left, right, position of each input
+    pair in the output)."""
+    l_idx = np.asarray(l_idx, dtype=np.int64)
+    r_idx = np.asarray(r_idx, dtype=np.int64)
+    order = np.lexsort((r_idx, l_idx))
+    l_s, r_s = l_idx[order], r_idx[order]
+    counts = np.bincount(l_s, minlength=n_left) if len(l_s) else np.zeros(n_left, np.int64)
+    repeats = np.maximum(counts, 1)
+    out_l = np.repeat(np.arange(n_left, dtype=np.int64), repeats)
+    out_r = np.full(len(out_l), -1, dtype=np.int64)
+    target = np.empty(0, dtype=np.int64)
+    if len(l_s):
+        first_match = np.cumsum(counts) - counts
+        rank = np.arange(len(l_s)) - np.repeat(first_match, counts)
+        row_start = np.cumsum(repeats) - repeats
+        target = row_start[l_s] + rank
+        out_r[target] = r_s
+    matched_right = np.zeros(n_right, dtype=bool)
+    matched_right[r_idx] = True
+    miss = np.flatnonzero(~matched_right)
+    pos = np.empty(len(l_idx), dtype=np.int64)
+    pos[order] = target
+    return (np.concatenate([out_l, np.full(len(miss), -1, np.int64)]), np.concatenate([out_r, miss]), pos)
+
+
+def _column_take(values: np.ndarray, idx: np.ndarray) -> np.ndarray:
+    """values[idx] with NaN where idx < 0 (object or float columns then)."""
+    valid = idx >= 0
+    out = values[np.maximum(idx, 0)] if len(values) else np.empty(len(idx))
+    if (~valid).any():
+        # as pandas reindexing fills: integers and floats become float64,
+        # everything else object
+        out = out.astype(float) if out.dtype.kind in "iufc" else out.astype(object)
+        out[~valid] = np.nan
+    return out
+
+
+def _assemble_outer(left_df, right_df, l_idx, r_idx, index_left, index_right, lcols, rcols, lren, rren, extra):
+    """Outer join frame: a fresh RangeIndex, both frames' index labels as the
+    index_<suffix> columns (NaN where the side is missing), left columns,
+    right columns, and the left geometry filled from the right geometry where
+    the left row is missing (api/_native_results.py:476-500)."""
+    ldf, rdf = left_df._df, right_df._df
+    lo, ro, pos = outer_indexers(l_idx, r_idx, len(ldf), len(rdf))
+    out = pd.DataFrame(index=pd.RangeIndex(len(lo)))
+    out[index_left] = _column_take(np.asarray(ldf.index), lo)
+    for c in lcols:
+        out[lren.get(c, c)] = _column_take(ldf[c].to_numpy(), lo)
+    out[index_right] = _column_take(np.asarray(rdf.index), ro)
+    for c in rcols:
+        out[rren.get(c, c)] = _column_take(rdf[c].to_numpy(), ro)
+    for name, vals in (extra or {}).items():
+        col = np.full(len(lo), np.nan)
+        col[pos] = np.asarray(vals, dtype=np.float64)
+        out[name] = col
+    # left rows come first (every lo >= 0), the unmatched right rows last
+    n_first = int(np.count_nonzero(lo >= 0))
+    geoms = GeometryArray.concat([left_df.geometry.values.take(lo[:n_first]),
+                                  right_df.geometry.values.take(ro[n_first:])])
+    g = GeoDataFrame(out, geometry=GeoSeries(geoms, crs=left_df.crs), crs=left_df.crs)
+    g._geom_name = left_df._geom_name
     return g

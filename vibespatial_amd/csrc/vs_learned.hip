@@ -679,7 +679,11 @@ struct RefineView {
     const uint8_t* flags;
 };
 
-enum { kLModePip = 0, kLModeBox = 1 };
+// kLModePip: query q is polygon row q, pairs (point, q), the point index is
+// built over the left column; kLModePipQ: the same refine with pairs (q,
+// point), the polygons being the query column of a prebuilt point index;
+// kLModeBox: query q is a box, pairs (q, point).
+enum { kLModePip = 0, kLModeBox = 1, kLModePipQ = 2 };
 
 template <int PRED>
 __device__ __forceinline__ bool loc_pred(int loc) {
@@ -1025,7 +1029,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
         float E = 0.0f;
         int ne_fixed = -1;  // edges staged once for the whole item (small single-part polygons)
         Slabs fixed_slabs = {1, 0.0f, 0.0f};
-        if (MODE == kLModePip) {
+        if (MODE != kLModeBox) {
             info = poly_info[q];
             cx = 0.5 * (box.x + box.z);
             cy = 0.5 * (box.y + box.w);
@@ -1039,7 +1043,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
             // kPosPerThread positions per round and refines its own candidates
             // at once against the item's slab lists (no candidate compaction).
             Slabs S = {1, 0.0f, 0.0f};
-            if (MODE == kLModePip) {
+            if (MODE != kLModeBox) {
                 const int ne = stage_all_edges(info.x, info.y, cx, cy, R, Et, Ec, s_wsum);
                 S = build_slabs(Et, ne, (float)(box.y - cy), (float)(box.w - cy), s_bstart, s_bcur, s_blist);
             }
@@ -1069,7 +1073,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
 #pragma unroll
                 for (int j = 0; j < kPosPerThread; ++j) {
                     bool keep = px[j] >= box.x && px[j] <= box.z && py[j] >= box.y && py[j] <= box.w;
-                    if (MODE == kLModePip && keep) {
+                    if (MODE != kLModeBox && keep) {
                         ++st_cand;
                         st_cverts += (unsigned long long)(info.y - info.x);
                         bool esc = false;
@@ -1088,7 +1092,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                         if (keep) {
                             const int32_t orig = perm[pos[j]];
                             outb[base + __popcll(m & lanemask_lt())] =
-                                MODE == kLModeBox ? make_int2(q, orig) : make_int2(orig, q);
+                                MODE == kLModePip ? make_int2(orig, q) : make_int2(q, orig);
                         }
                     }
                 }
@@ -1251,7 +1255,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                 if (loc_pred<PRED>(loc[j])) {
                     const int32_t orig = perm[cand[k]];
                     const int slot = atomicAdd(&s_nout, 1);
-                    outb[slot] = make_int2(orig, q);
+                    outb[slot] = MODE == kLModePip ? make_int2(orig, q) : make_int2(q, orig);
                 }
             }
             if (tid == 0) {
@@ -1496,7 +1500,7 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
     VS_LAUNCH_CHECK();
     if (nq > 0) {
         const unsigned gq = (unsigned)ceil_div(nq, 256);
-        int4* pinfo = mode == kLModePip ? Q.poly_info : nullptr;
+        int4* pinfo = mode != kLModeBox ? Q.poly_info : nullptr;
         hipLaunchKernelGGL(cover_cells_kernel<K>, dim3(gq), dim3(256), 0, s, qbox, nq, B.ext, Q.keyr, Q.nranges,
                            R.geom, R.part, R.ring, pinfo);
         VS_LAUNCH_CHECK();
@@ -1527,7 +1531,13 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
                        Q.total_items, Q.item_q, Q.item_cap, Q.order, Q.poly_info, Q.ranges, Q.nranges, B.sx, B.sy,  \
                        B.perm, qbox, R, Q.stage_l, Q.stage_r, capacity, Q.ctl)
         if (mode == kLModeBox) VS_LJ(kLModeBox, VS_PRED_INTERSECTS);
-        else if (pred == VS_PRED_WITHIN) VS_LJ(kLModePip, VS_PRED_WITHIN);
+        else if (mode == kLModePipQ) {
+            // polygon (left) over point (right): interior for contains and
+            // contains_properly, boundary for touches, else not outside
+            if (pred == VS_PRED_CONTAINS || pred == VS_PRED_CONTAINS_PROPERLY) VS_LJ(kLModePipQ, VS_PRED_WITHIN);
+            else if (pred == VS_PRED_TOUCHES) VS_LJ(kLModePipQ, VS_PRED_TOUCHES);
+            else VS_LJ(kLModePipQ, VS_PRED_INTERSECTS);
+        } else if (pred == VS_PRED_WITHIN) VS_LJ(kLModePip, VS_PRED_WITHIN);
         else if (pred == VS_PRED_TOUCHES) VS_LJ(kLModePip, VS_PRED_TOUCHES);
         else VS_LJ(kLModePip, VS_PRED_INTERSECTS);
 #undef VS_LJ
@@ -1667,6 +1677,52 @@ int vs_learned_query_boxes(const void* index, int64_t n_points, int32_t key_byte
     const auto B = vs::carve_index<uint64_t>(ci, n_points, n_leaves);
     return vs::learned_query_t<uint64_t>(vs::kLModeBox, VS_PRED_INTERSECTS, qb, n_boxes,
                                          vs::view_of<uint64_t>(B, n_points, n_leaves, lookup_mode), B, R, n_boxes,
+                                         capacity, sort != 0, Q, out_left, out_right, d_result, s);
+}
+
+int64_t vs_learned_query_polygons_workspace_bytes(int64_t n_polys, int64_t n_coords, int64_t capacity) {
+    vs::Carver c(nullptr);
+    c.take<uint8_t>(n_coords);
+    vs::carve_query(c, n_polys, n_polys, capacity);
+    return c.bytes;
+}
+
+int vs_learned_query_polygons(const void* index, int64_t n_points, int32_t key_bytes, int32_t n_leaves,
+                              const double* x, const double* y, const int32_t* geom_offsets,
+                              const int32_t* part_offsets, const int32_t* ring_offsets, int64_t n_polys,
+                              int64_t n_parts, int64_t n_rings, int64_t n_coords, const double* poly_bounds,
+                              int32_t predicate, int32_t lookup_mode, int64_t capacity, int32_t sort,
+                              int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
+                              int64_t workspace_bytes, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(key_bytes == 4 || key_bytes == 8, "vs_learned_query_polygons: key_bytes must be 4 or 8");
+    VS_REQUIRE(n_polys >= 0 && n_polys < (int64_t)INT32_MAX, "vs_learned_query_polygons: n_polys out of range");
+    VS_REQUIRE(n_coords >= 0 && n_coords < (int64_t)INT32_MAX, "vs_learned_query_polygons: n_coords out of range");
+    VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_learned_query_polygons: capacity out of range");
+    VS_REQUIRE(predicate == VS_PRED_CONTAINS || predicate == VS_PRED_CONTAINS_PROPERLY || predicate == VS_PRED_COVERS ||
+                   predicate == VS_PRED_INTERSECTS || predicate == VS_PRED_TOUCHES,
+               "vs_learned_query_polygons: unsupported predicate %d", predicate);
+    VS_REQUIRE(lookup_mode >= 0 && lookup_mode <= 2, "vs_learned_query_polygons: lookup_mode in {0, 1, 2}");
+    VS_REQUIRE(d_result != nullptr, "vs_learned_query_polygons: null result");
+    VS_REQUIRE(workspace_bytes >= vs_learned_query_polygons_workspace_bytes(n_polys, n_coords, capacity),
+               "vs_learned_query_polygons: workspace too small");
+    vs::Carver ci(const_cast<void*>(index));
+    vs::Carver cw(workspace);
+    uint8_t* flags = cw.take<uint8_t>(n_coords);
+    const vs::QueryBufs Q = vs::carve_query(cw, n_polys, n_polys, capacity);
+    hipStream_t s = vs::stream_of(stream);
+    VS_TRY(vs::build_refine_flags(part_offsets, ring_offsets, n_parts, n_rings, n_coords, flags, s));
+    const vs::RefineView R = {x, y, geom_offsets, part_offsets, ring_offsets, flags};
+    const double4* qb = reinterpret_cast<const double4*>(poly_bounds);
+    if (key_bytes == 4) {
+        const auto B = vs::carve_index<uint32_t>(ci, n_points, n_leaves);
+        return vs::learned_query_t<uint32_t>(vs::kLModePipQ, predicate, qb, n_polys,
+                                             vs::view_of<uint32_t>(B, n_points, n_leaves, lookup_mode), B, R, n_polys,
+                                             capacity, sort != 0, Q, out_left, out_right, d_result, s);
+    }
+    const auto B = vs::carve_index<uint64_t>(ci, n_points, n_leaves);
+    return vs::learned_query_t<uint64_t>(vs::kLModePipQ, predicate, qb, n_polys,
+                                         vs::view_of<uint64_t>(B, n_points, n_leaves, lookup_mode), B, R, n_polys,
                                          capacity, sort != 0, Q, out_left, out_right, d_result, s);
 }
 

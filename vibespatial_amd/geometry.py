@@ -33,6 +33,7 @@ __all__ = [
     "PolygonArray",
     "GeometryArray",
     "points_from_xy",
+    "box_array",
     "as_geometry_array",
 ]
 
@@ -207,6 +208,54 @@ class PointArray:
         return self.x.nbytes + self.y.nbytes
 
 
+def _ranges(starts: np.ndarray, counts: np.ndarray) -> np.ndarray:
+    """Concatenation of arange(starts[i], starts[i] + counts[i]) over i."""
+    counts = np.asarray(counts, dtype=np.int64)
+    total = int(counts.sum())
+    if total == 0:
+        return np.empty(0, dtype=np.int64)
+    first = np.cumsum(counts) - counts
+    return np.repeat(np.asarray(starts, dtype=np.int64) - first, counts) + np.arange(total, dtype=np.int64)
+
+
+def _offsets(counts: np.ndarray) -> np.ndarray:
+    return np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+
+
+def concat_polygons(arrays: Sequence["PolygonArray"]) -> "PolygonArray":
+    """Rows of the arrays one after another."""
+    xs, ys, ro, po, go = [], [], [np.zeros(1, np.int64)], [np.zeros(1, np.int64)], [np.zeros(1, np.int64)]
+    nc = nr = npart = 0
+    for a in arrays:
+        xs.append(a.x)
+        ys.append(a.y)
+        ro.append(a.ring_offsets[1:].astype(np.int64) + nc)
+        po.append(a.part_offsets[1:].astype(np.int64) + nr)
+        go.append(a.geom_offsets[1:].astype(np.int64) + npart)
+        nc += len(a.x)
+        nr += len(a.ring_offsets) - 1
+        npart += len(a.part_offsets) - 1
+    x = np.concatenate(xs) if xs else np.empty(0)
+    y = np.concatenate(ys) if ys else np.empty(0)
+    return PolygonArray(x, y, np.concatenate(ro), np.concatenate(po), np.concatenate(go))
+
+
+def box_array(bounds: np.ndarray, ccw: bool = True) -> "PolygonArray":
+    """Rectangles from an (n, 4) [minx, miny, maxx, maxy] array, one closed
+    ring of five coordinates each (the vectorised form of ``box``)."""
+    b = np.asarray(bounds, dtype=np.float64).reshape(-1, 4)
+    n = len(b)
+    x0, y0, x1, y1 = b[:, 0], b[:, 1], b[:, 2], b[:, 3]
+    if ccw:
+        xs = np.stack([x1, x1, x0, x0, x1], axis=1)
+        ys = np.stack([y0, y1, y1, y0, y0], axis=1)
+    else:
+        xs = np.stack([x0, x0, x1, x1, x0], axis=1)
+        ys = np.stack([y0, y1, y1, y0, y0], axis=1)
+    k = np.arange(n + 1, dtype=np.int64)
+    return PolygonArray(xs.ravel(), ys.ravel(), 5 * k, k, k)
+
+
 def points_from_xy(x, y) -> PointArray:
     return PointArray(np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64))
 
@@ -277,8 +326,18 @@ class PolygonArray:
         return len(self.x)
 
     def take(self, idx) -> "PolygonArray":
+        """Rows idx (in that order), by offset arithmetic on the buffers."""
         idx = np.asarray(idx, dtype=np.int64)
-        return PolygonArray.from_geometries([self.scalar(int(i)) for i in idx])
+        g, p, r = self.geom_offsets.astype(np.int64), self.part_offsets.astype(np.int64), \
+            self.ring_offsets.astype(np.int64)
+        n_parts = g[idx + 1] - g[idx]
+        parts = _ranges(g[idx], n_parts)
+        n_rings = p[parts + 1] - p[parts]
+        rings = _ranges(p[parts], n_rings)
+        n_coords = r[rings + 1] - r[rings]
+        coords = _ranges(r[rings], n_coords)
+        return PolygonArray(self.x[coords], self.y[coords], _offsets(n_coords), _offsets(n_rings),
+                            _offsets(n_parts))
 
     def scalar(self, i: int):
         parts = []
@@ -379,6 +438,53 @@ class GeometryArray:
         if self.polygons is not None:
             out[self.polygon_rows] = self.polygons.bounds()
         return out
+
+    def _family_index(self) -> np.ndarray:
+        """Row -> index of the row inside its family array (-1 for null rows)."""
+        fi = getattr(self, "_fam_idx", None)
+        if fi is None:
+            fi = np.full(len(self), -1, dtype=np.int64)
+            if self.points is not None:
+                fi[self.point_rows] = np.arange(len(self.point_rows))
+            if self.polygons is not None:
+                fi[self.polygon_rows] = np.arange(len(self.polygon_rows))
+            self._fam_idx = fi
+        return fi
+
+    def take(self, idx) -> "GeometryArray":
+        """Rows idx in that order; idx < 0 gives a null row."""
+        idx = np.asarray(idx, dtype=np.int64)
+        valid = idx >= 0
+        src = np.maximum(idx, 0)
+        tags = np.where(valid, self.tags[src] if len(self.tags) else TAG_NULL, TAG_NULL).astype(np.int8)
+        fi = self._family_index()
+        pts = prow = polys = qrow = None
+        m = tags == TAG_POINT
+        if m.any():
+            pts = self.points.take(fi[idx[m]])
+            prow = np.flatnonzero(m)
+        m = tags == TAG_POLYGON
+        if m.any():
+            polys = self.polygons.take(fi[idx[m]])
+            qrow = np.flatnonzero(m)
+        return GeometryArray(tags, pts, prow, polys, qrow)
+
+    @staticmethod
+    def concat(arrays: Sequence["GeometryArray"]) -> "GeometryArray":
+        """Rows of the columns one after another."""
+        tags = np.concatenate([a.tags for a in arrays]) if arrays else np.empty(0, np.int8)
+        base, prow, qrow, pts, polys = 0, [], [], [], []
+        for a in arrays:
+            if a.points is not None:
+                pts.append(a.points)
+                prow.append(a.point_rows + base)
+            if a.polygons is not None:
+                polys.append(a.polygons)
+                qrow.append(a.polygon_rows + base)
+            base += len(a)
+        P = PointArray(np.concatenate([p.x for p in pts]), np.concatenate([p.y for p in pts])) if pts else None
+        Q = concat_polygons(polys) if polys else None
+        return GeometryArray(tags, P, np.concatenate(prow) if pts else None, Q, np.concatenate(qrow) if polys else None)
 
     def __getitem__(self, i: int):
         t = self.tags[i]

@@ -535,6 +535,38 @@ class LearnedPointIndex:
 
         return _split_boxes(run, boxes)
 
+    def query_polygons(self, P: DevicePolygons, bounds: torch.Tensor, predicate: str, lookup: str = "learned",
+                       capacity: int | None = None, sort: bool = True):
+        """(polygon row, point) pairs with predicate(polygon, point) exact
+        (contains / contains_properly: interior, touches: boundary, covers /
+        intersects: not outside), through the index and the FP32 refine with
+        exact escalation (vs_learned_query_polygons); lexsorted when `sort`."""
+        dev = require_gpu()
+        L = lib()
+        if predicate not in ("contains", "contains_properly", "covers", "intersects", "touches"):
+            raise NotImplementedError(f"polygon x point predicate {predicate!r}")
+        nq = len(P)
+        cap = int(capacity) if capacity is not None else max(1024, 4 * nq)
+        cap = min(cap, PAIR_LIMIT)
+        retries = 0
+        while True:
+            ws = _ws(L.vs_learned_query_polygons_workspace_bytes(nq, P.n_coords, cap), dev)
+            out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+            out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+            res = torch.zeros(16, dtype=torch.int64, device=dev)
+            check(L.vs_learned_query_polygons(ptr(self.buf), self.n, self.key_bytes, self.n_leaves, *P.args(), nq,
+                                              P.n_parts, P.n_rings, P.n_coords, ptr(bounds), PRED[predicate],
+                                              LOOKUP_MODES[lookup], cap, int(bool(sort)), ptr(out_l), ptr(out_r),
+                                              ptr(res), ptr(ws), ws.numel(), stream()), "vs_learned_query_polygons")
+            h = res.cpu().tolist()
+            if h[1] <= cap:
+                return out_l[: h[0]], out_r[: h[0]], _learned_stats(h, cap, retries)
+            if h[1] > PAIR_LIMIT:
+                raise OverflowError(f"{h[1]} pairs exceed the int32 pair limit of one call")
+            del ws, out_l, out_r
+            cap = int(h[1])
+            retries += 1
+
 
 def mlp_loss_grad(params: torch.Tensor, u: torch.Tensor, t: torch.Tensor):
     """(loss, grad[97]) of the rank MLP on (u, t), from the device kernels."""
