@@ -514,9 +514,13 @@ int vs_point_order(const double* px, const double* py, int64_t n_points, int32_t
                    int64_t workspace_bytes, void* stream);
 
 /* Benchmark instrumentation (no reference counterpart): when enabled, the
- * traversal kernel of the next join/query is bracketed by two HIP events
- * recorded on the caller's stream; vs_last_kernel_ms waits for the second
- * event and returns the elapsed milliseconds.  Process-wide, not reentrant. */
+ * dominant kernel (traversal / learned join / batched refine) of each
+ * join or query the calling host thread makes is bracketed by two HIP events
+ * recorded on the call's stream; vs_last_kernel_ms waits for the second
+ * event and returns the elapsed milliseconds of the last one.  The state is
+ * per host thread (threads time their own calls); vs_set_kernel_timing(1)
+ * creates the thread's events on the device current then, and
+ * vs_set_kernel_timing(0) destroys them. */
 int vs_set_kernel_timing(int32_t enable);
 int vs_last_kernel_ms(float* ms);
 

@@ -416,25 +416,28 @@ int grid_for(int64_t n_waves) {
     return (int)((g + 7) / 8 * 8);
 }
 
-// Optional timing of the traversal kernel with HIP events recorded on the
-// launch stream (instrumentation for benchmarks; process-wide, not reentrant).
-static bool g_timing = false;
-static hipEvent_t g_ev[2];
-static bool g_ev_ready = false;
+// Optional timing of the dominant kernel of the calling host thread's next
+// join or query, with two HIP events recorded on the launch stream
+// (instrumentation for benchmarks).  The state is per host thread, so
+// threads time their own calls; the events are created by
+// vs_set_kernel_timing(1) on the device current there, never inside a join.
+struct KernelTiming {
+    bool on = false;
+    bool recorded = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+static thread_local KernelTiming t_timing;
 
 int timing_begin(hipStream_t s) {
-    if (!g_timing) return VS_OK;
-    if (!g_ev_ready) {
-        VS_CHECK_HIP(hipEventCreate(&g_ev[0]));
-        VS_CHECK_HIP(hipEventCreate(&g_ev[1]));
-        g_ev_ready = true;
-    }
-    VS_CHECK_HIP(hipEventRecord(g_ev[0], s));
+    if (!t_timing.on) return VS_OK;
+    VS_CHECK_HIP(hipEventRecord(t_timing.ev[0], s));
     return VS_OK;
 }
 
 int timing_end(hipStream_t s) {
-    if (g_timing && g_ev_ready) VS_CHECK_HIP(hipEventRecord(g_ev[1], s));
+    if (!t_timing.on) return VS_OK;
+    VS_CHECK_HIP(hipEventRecord(t_timing.ev[1], s));
+    t_timing.recorded = true;
     return VS_OK;
 }
 
@@ -499,16 +502,27 @@ extern "C" {
 
 int vs_set_kernel_timing(int32_t enable) {
     vs::clear_error();
-    vs::g_timing = enable != 0;
+    vs::KernelTiming& T = vs::t_timing;
+    if (enable && !T.on) {
+        VS_CHECK_HIP(hipEventCreate(&T.ev[0]));
+        VS_CHECK_HIP(hipEventCreate(&T.ev[1]));
+        T.on = true;
+        T.recorded = false;
+    } else if (!enable && T.on) {
+        VS_CHECK_HIP(hipEventDestroy(T.ev[0]));
+        VS_CHECK_HIP(hipEventDestroy(T.ev[1]));
+        T = vs::KernelTiming();
+    }
     return VS_OK;
 }
 
 int vs_last_kernel_ms(float* ms) {
     vs::clear_error();
     VS_REQUIRE(ms != nullptr, "vs_last_kernel_ms: null output");
-    VS_REQUIRE(vs::g_timing && vs::g_ev_ready, "vs_last_kernel_ms: timing not enabled or nothing timed yet");
-    VS_CHECK_HIP(hipEventSynchronize(vs::g_ev[1]));
-    VS_CHECK_HIP(hipEventElapsedTime(ms, vs::g_ev[0], vs::g_ev[1]));
+    const vs::KernelTiming& T = vs::t_timing;
+    VS_REQUIRE(T.on && T.recorded, "vs_last_kernel_ms: timing not enabled on this thread or nothing timed yet");
+    VS_CHECK_HIP(hipEventSynchronize(T.ev[1]));
+    VS_CHECK_HIP(hipEventElapsedTime(ms, T.ev[0], T.ev[1]));
     return VS_OK;
 }
 

@@ -1359,21 +1359,21 @@ __global__ void zero_i32_kernel(int32_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
-// A second stream per device (default priority) for work that overlaps the
-// caller's stream inside one call, with a fork / join event pair.  Created on
-// first use and kept for the process.
+// A second stream (default priority) for work that overlaps the caller's
+// stream inside one call, with a fork / join event pair: one per host thread
+// and device, created on the thread's first such call on the device and kept
+// until the process ends.  Threads never share one, so concurrent calls from
+// different host threads do not interfere.
 struct SideStream {
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
 };
 
 static int side_stream(SideStream** out) {
-    static std::mutex mu;
-    static SideStream per_dev[64];
+    static thread_local SideStream per_dev[64];
     int dev = 0;
     VS_CHECK_HIP(hipGetDevice(&dev));
     VS_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
-    std::lock_guard<std::mutex> lock(mu);
     SideStream& S = per_dev[dev];
     if (!S.stream) {
         VS_CHECK_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
@@ -1386,13 +1386,12 @@ static int side_stream(SideStream** out) {
 
 // Keys and their sort on the caller's stream; then the network's training,
 // the leaf table and the error pass (which need only the sorted keys) run on
-// the side stream while the caller's stream gathers the sorted coordinates.
-// The call joins both before returning.  Calls that share a device serialise
-// on the side stream's events, so one call must finish enqueueing before the
-// next starts (the library's entry points are not reentrant on a device).
+// the side stream while the caller's stream gathers the sorted coordinates
+// (fork), or after them on the caller's stream (no fork: nothing is created).
+// The call joins both before returning.
 template <class K>
 int index_build_t(const double* px, const double* py, int64_t n, int S, int iters, float lr, const IndexBufs<K>& B,
-                  hipStream_t s) {
+                  hipStream_t s, bool fork = true) {
     if (sizeof(K) == 4)
         VS_TRY(point_index_build_u32(px, py, n, reinterpret_cast<uint32_t*>(B.keys), B.perm, nullptr, nullptr, B.ext,
                                      B.build_ws, s));
@@ -1400,10 +1399,13 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
         VS_TRY(point_index_build_u64(px, py, n, reinterpret_cast<uint64_t*>(B.keys), B.perm, nullptr, nullptr, B.ext,
                                      B.build_ws, s));
     SideStream* side = nullptr;
-    VS_TRY(side_stream(&side));
-    VS_CHECK_HIP(hipEventRecord(side->fork, s));
-    VS_CHECK_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
-    const hipStream_t t = side->stream;
+    hipStream_t t = s;
+    if (fork) {
+        VS_TRY(side_stream(&side));
+        VS_CHECK_HIP(hipEventRecord(side->fork, s));
+        VS_CHECK_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+        t = side->stream;
+    }
     if (n >= 2) {
         hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(vs::kTrainThreads), 0, t, B.keys, n, iters, lr, B.mlp,
                            B.losses, B.meta);
@@ -1417,9 +1419,11 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
     }
     // two gather blocks per CU (enough to keep the random reads in flight)
     // leave room for the side stream's kernels
-    VS_TRY(gather_points(px, py, B.perm, n, B.sx, B.sy, s, 2));
-    VS_CHECK_HIP(hipEventRecord(side->join, t));
-    VS_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
+    VS_TRY(gather_points(px, py, B.perm, n, B.sx, B.sy, s, fork ? 2 : 8));
+    if (fork) {
+        VS_CHECK_HIP(hipEventRecord(side->join, t));
+        VS_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
+    }
     return VS_OK;
 }
 
@@ -1548,6 +1552,37 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
     hipLaunchKernelGGL(publish_learned_kernel, dim3(1), dim3(64), 0, s, Q.ctl, Q.cstat, B.meta, d_result);
     VS_LAUNCH_CHECK();
     return VS_OK;
+}
+
+// The learned join on a carved workspace (fork: the index build's side
+// stream; the allocation-free entry point runs everything on `s`).
+int learned_join_impl(const double* px, const double* py, int64_t n_points, const double* x, const double* y,
+                      const int32_t* geom_offsets, const int32_t* part_offsets, const int32_t* ring_offsets,
+                      int64_t n_polys, int64_t n_parts, int64_t n_rings, int64_t n_coords, const double* poly_bounds,
+                      int32_t predicate, int32_t train_iters, int32_t lookup_mode, int64_t capacity,
+                      int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace, hipStream_t s,
+                      bool fork) {
+    VS_REQUIRE(n_points >= 0 && n_points < (int64_t)INT32_MAX, "learned join: n_points out of range");
+    VS_REQUIRE(n_polys >= 0 && n_polys < (int64_t)INT32_MAX, "learned join: n_polys out of range");
+    VS_REQUIRE(n_coords >= 0 && n_coords < (int64_t)INT32_MAX, "learned join: n_coords out of range");
+    VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "learned join: capacity out of range");
+    VS_REQUIRE(predicate == VS_PRED_WITHIN || predicate == VS_PRED_INTERSECTS || predicate == VS_PRED_COVERED_BY ||
+                   predicate == VS_PRED_TOUCHES,
+               "learned join: unsupported predicate %d", predicate);
+    VS_REQUIRE(lookup_mode >= 0 && lookup_mode <= 2, "learned join: lookup_mode in {0, 1, 2}");
+    VS_REQUIRE(train_iters >= 0 && train_iters <= 100000, "learned join: bad train_iters");
+    VS_REQUIRE(d_result != nullptr, "learned join: null result");
+    const int S = leaves_for(n_points, 512);
+    Carver c(workspace);
+    const auto B = carve_index<uint32_t>(c, n_points, S);
+    uint8_t* flags = c.take<uint8_t>(n_coords);
+    const QueryBufs Q = carve_query(c, n_polys, n_points, capacity);
+    VS_TRY(index_build_t<uint32_t>(px, py, n_points, S, train_iters, kJoinTrainLr, B, s, fork));
+    VS_TRY(build_refine_flags(part_offsets, ring_offsets, n_parts, n_rings, n_coords, flags, s));
+    const RefineView R = {x, y, geom_offsets, part_offsets, ring_offsets, flags};
+    return learned_query_t<uint32_t>(kLModePip, predicate, reinterpret_cast<const double4*>(poly_bounds), n_polys,
+                                     view_of<uint32_t>(B, n_points, S, lookup_mode), B, R, n_points, capacity, true,
+                                     Q, out_left, out_right, d_result, s);
 }
 
 }  // namespace vs
@@ -1744,30 +1779,11 @@ int vs_sjoin_points_polygons_learned(const double* px, const double* py, int64_t
                                      int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
                                      void* stream) {
     vs::clear_error();
-    VS_REQUIRE(n_points >= 0 && n_points < (int64_t)INT32_MAX, "learned join: n_points out of range");
-    VS_REQUIRE(n_polys >= 0 && n_polys < (int64_t)INT32_MAX, "learned join: n_polys out of range");
-    VS_REQUIRE(n_coords >= 0 && n_coords < (int64_t)INT32_MAX, "learned join: n_coords out of range");
-    VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "learned join: capacity out of range");
-    VS_REQUIRE(predicate == VS_PRED_WITHIN || predicate == VS_PRED_INTERSECTS || predicate == VS_PRED_COVERED_BY ||
-                   predicate == VS_PRED_TOUCHES,
-               "learned join: unsupported predicate %d", predicate);
-    VS_REQUIRE(lookup_mode >= 0 && lookup_mode <= 2, "learned join: lookup_mode in {0, 1, 2}");
-    VS_REQUIRE(train_iters >= 0 && train_iters <= 100000, "learned join: bad train_iters");
-    VS_REQUIRE(d_result != nullptr, "learned join: null result");
     VS_REQUIRE(workspace_bytes >= vs_sjoin_points_polygons_learned_workspace_bytes(n_points, n_polys, n_coords, capacity),
                "learned join: workspace too small");
-    const int S = vs::leaves_for(n_points, 512);
-    vs::Carver c(workspace);
-    const auto B = vs::carve_index<uint32_t>(c, n_points, S);
-    uint8_t* flags = c.take<uint8_t>(n_coords);
-    const vs::QueryBufs Q = vs::carve_query(c, n_polys, n_points, capacity);
-    hipStream_t s = vs::stream_of(stream);
-    VS_TRY(vs::index_build_t<uint32_t>(px, py, n_points, S, train_iters, vs::kJoinTrainLr, B, s));
-    VS_TRY(vs::build_refine_flags(part_offsets, ring_offsets, n_parts, n_rings, n_coords, flags, s));
-    const vs::RefineView R = {x, y, geom_offsets, part_offsets, ring_offsets, flags};
-    return vs::learned_query_t<uint32_t>(vs::kLModePip, predicate, reinterpret_cast<const double4*>(poly_bounds),
-                                         n_polys, vs::view_of<uint32_t>(B, n_points, S, lookup_mode), B, R, n_points,
-                                         capacity, true, Q, out_left, out_right, d_result, s);
+    return vs::learned_join_impl(px, py, n_points, x, y, geom_offsets, part_offsets, ring_offsets, n_polys, n_parts,
+                                 n_rings, n_coords, poly_bounds, predicate, train_iters, lookup_mode, capacity, out_left,
+                                 out_right, d_result, workspace, vs::stream_of(stream), true);
 }
 
 }  // extern "C"

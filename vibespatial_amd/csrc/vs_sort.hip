@@ -141,6 +141,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_fused_kernel(const int
     if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = prefix + tot;
 }
 
+__global__ void zero_total_kernel(long long* __restrict__ p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
 int64_t scan_workspace_bytes(int64_t n) {
     Carver c(nullptr);
     c.take<long long>(ceil_div(n, kScanTile) + 1);
@@ -156,7 +160,11 @@ int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, long long* d_
     Carver c(ws);
     long long* part = c.take<long long>(nb + 1);
     if (n == 0) {
-        if (d_total) VS_CHECK_HIP(hipMemsetAsync(d_total, 0, sizeof(long long), s));
+        // a kernel, not a memset: callers may capture this in a HIP graph
+        if (d_total) {
+            hipLaunchKernelGGL(zero_total_kernel, dim3(1), dim3(64), 0, s, d_total);
+            VS_LAUNCH_CHECK();
+        }
         return VS_OK;
     }
     hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part);
