@@ -357,8 +357,11 @@ int vs_batched_intersects(const double* ax, const double* ay, const int32_t* a_g
 int64_t vs_wkb_workspace_bytes(int64_t n_rows);
 
 /* Pass 1 over n_rows WKB records: record r is payload[offsets[r],
- * offsets[r+1]) (device int64 offsets, n_rows + 1); validity (device uint8
- * per row, or null = all valid): 0 or an empty record = null row.  Supports
+ * offsets[r+1]) (device int64 offsets, n_rows + 1) of the payload_bytes
+ * bytes at payload; a record not wholly inside [0, payload_bytes) or whose
+ * end offset is below its start is reported as truncated, so no read leaves
+ * the payload whatever the offsets hold.  validity (device uint8 per row, or
+ * null = all valid): 0 or an empty record = null row.  Supports
  * Point, Polygon and MultiPolygon in 2D, either byte order, ISO or EWKB type
  * words (an SRID is skipped).  d_result (device int64[8]): [0] point rows,
  * [1] polygonal rows, [2] parts, [3] rings, [4] polygon coordinates, [5] first
@@ -369,8 +372,8 @@ int64_t vs_wkb_workspace_bytes(int64_t n_rows);
  * nothing).  The workspace carries the counts to vs_wkb_fill.  Replaces the
  * reference's device WKB decode (kernels/core/wkb_decode.py:849, used by
  * io/wkb.py:5049 and io/geoparquet.py:3430). */
-int vs_wkb_scan(const uint8_t* payload, const int64_t* offsets, const uint8_t* validity, int64_t n_rows,
-                int64_t* d_result, void* workspace, int64_t workspace_bytes, void* stream);
+int vs_wkb_scan(const uint8_t* payload, int64_t payload_bytes, const int64_t* offsets, const uint8_t* validity,
+                int64_t n_rows, int64_t* d_result, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* Pass 2 (same records and workspace, after vs_wkb_scan): point rows ->
  * point_x/point_y (result[0] entries) with their record numbers in point_rows;
@@ -378,8 +381,8 @@ int vs_wkb_scan(const uint8_t* payload, const int64_t* offsets, const uint8_t* v
  * part_offsets (result[2] + 1), geom_offsets (result[1] + 1) with their record
  * numbers in polygon_rows; a Polygon is one part (none when it has no rings),
  * a MultiPolygon one part per member. */
-int vs_wkb_fill(const uint8_t* payload, const int64_t* offsets, const uint8_t* validity, int64_t n_rows,
-                const void* workspace, int64_t workspace_bytes, double* point_x, double* point_y,
+int vs_wkb_fill(const uint8_t* payload, int64_t payload_bytes, const int64_t* offsets, const uint8_t* validity,
+                int64_t n_rows, const void* workspace, int64_t workspace_bytes, double* point_x, double* point_y,
                 int64_t* point_rows, double* x, double* y, int32_t* geom_offsets, int32_t* part_offsets,
                 int32_t* ring_offsets, int64_t* polygon_rows, void* stream);
 

@@ -265,3 +265,32 @@ def test_box_query_output_sized_from_previous_queries():
         assert np.array_equal(l.cpu().numpy(), el[order]) and np.array_equal(r.cpu().numpy(), er[order]), half
         expect_retries.append(st.retries)
     assert expect_retries[2] == 1 and expect_retries[3] == 0  # grown past the hint, then slack
+
+
+def test_box_query_hint_does_not_oversize_a_large_query():
+    """A query with about 5,000 hits per box followed by 400k point boxes:
+    the second query's output is sized from at most HINT_SLOTS_PER_QUERY
+    slots per box, not from the first query's hits per box; queries given a
+    capacity (nearest rounds) and dwithin leave the plain hint alone."""
+    print("box query hint clamp", flush=True)
+    rng = np.random.default_rng(32)
+    c = rng.uniform(0, 1000, (5000, 2))
+    tb = np.column_stack([c - 2, c + 2])
+    tree = J.build_tree(torch.as_tensor(tb, device="cuda"))
+    big = np.array([[-10.0, -10.0, 1010.0, 1010.0]] * 64)
+    l, r, st = J.query_boxes(torch.as_tensor(big, device="cuda"), tree)
+    assert st.pairs == 64 * 5000
+    assert tree.hits_per_query["bbox"] == 5000
+    q = rng.uniform(0, 1000, (400_000, 2))
+    qb = np.column_stack([q, q])
+    l, r, st = J.query_boxes(torch.as_tensor(qb, device="cuda"), tree)
+    assert st.capacity <= J.HINT_SLOTS_PER_QUERY * 400_000
+    el, er = oracle.bbox_pairs(qb[:20_000], tb)
+    o = np.lexsort((er, el))
+    ln, rn = l.cpu().numpy(), r.cpu().numpy()
+    m = ln < 20_000
+    assert np.array_equal(ln[m], el[o]) and np.array_equal(rn[m], er[o])
+    # an explicit capacity (as the nearest search passes) does not move the hint
+    before = dict(tree.hits_per_query)
+    J.query_boxes(torch.as_tensor(big, device="cuda"), tree, capacity=64 * 5000 + 10)
+    assert tree.hits_per_query == before

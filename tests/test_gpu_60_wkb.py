@@ -217,3 +217,28 @@ def test_read_parquet_builds_the_index(tmp_path):
     g2 = read_parquet(tmp_path / "m.parquet")
     q = [Point(*rng.uniform(0, 100, 2)) for _ in range(500)]
     assert g1.sindex.query(q, predicate="intersects").tolist() == g2.sindex.query(q, predicate="intersects").tolist()
+
+
+def test_device_offsets_outside_the_payload_are_reported():
+    """decode_wkb_device takes offsets as they are: each record is bounded by
+    the payload's size in the kernels (reported as truncated), and the
+    column's first / last offsets are checked against it."""
+    import torch
+
+    from vibespatial_amd.io import decode_wkb_device
+
+    ga = GeometryArray.from_scalars([Point(1, 2), box(0, 0, 1, 1), Point(3, 4)])
+    payload, offsets, _ = encode_wkb(ga)
+    d_pay = torch.from_numpy(payload).cuda()
+    ok = decode_wkb_device(d_pay, torch.from_numpy(offsets).cuda())
+    assert ok.points is not None and len(ok.points) == 2
+    n = len(payload)
+    cases = [
+        (np.array([0, 21, n + 4096, n], np.int64), "record 1"),  # a middle record reaching past the payload
+        (np.array([0, 21, 10, n], np.int64), "record 1"),  # an end before its start
+        (np.array([0, 21, 114, n + 8], np.int64), "offsets out of the payload"),
+        (np.array([-16, 21, 114, n], np.int64), "offsets out of the payload"),
+    ]
+    for off, msg in cases:
+        with pytest.raises(ValueError, match=msg):
+            decode_wkb_device(d_pay, torch.from_numpy(off).cuda())

@@ -111,8 +111,8 @@ SIGNATURES = {
     "vs_batched_intersects": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _p, _p, _i64, _i32,
                                              _i64, _p, _p, _p, _p, _i64, _p]),
     "vs_wkb_workspace_bytes": (_i64, [_i64]),
-    "vs_wkb_scan": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i64, _p]),
-    "vs_wkb_fill": (ctypes.c_int, [_p, _p, _p, _i64, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "vs_wkb_scan": (ctypes.c_int, [_p, _i64, _p, _p, _i64, _p, _p, _i64, _p]),
+    "vs_wkb_fill": (ctypes.c_int, [_p, _i64, _p, _p, _i64, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "vs_pair_distances_workspace_bytes": (_i64, [_i64]),
     "vs_pair_distances": (_i32, [_i32, _i32, _p, _p, _i64, _i64, _i32, _p] + [_p] * 10 + [_p, _p, _i64, _p]),
     "vs_nearest_select_workspace_bytes": (_i64, [_i64, _i64]),

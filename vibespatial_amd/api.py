@@ -262,7 +262,7 @@ def _pairs_family(q_fam: str, qarr, t_fam: str, tarr, tindex: _FamilyIndex, pred
         # distance refine (spatial/query.py:663-826 route)
         d = torch.as_tensor(np.ascontiguousarray(dist, dtype=np.float64), device=dev)
         qb = J.point_boxes(device_points(qarr)) if q_fam == "point" else J.polygon_bounds(device_polygons(qarr))
-        l, r, _ = J.query_boxes(J.expand_boxes(qb, d), tree)
+        l, r, _ = J.query_boxes(J.expand_boxes(qb, d), tree, kind="dwithin")
         return J.refine_dwithin(l, r, d, _dev_geom(q_fam, qarr), _dev_geom(t_fam, tarr))
     qb = J.point_boxes(device_points(qarr)) if q_fam == "point" else J.polygon_bounds(device_polygons(qarr))
     l, r, _ = J.query_boxes(qb, tree)

@@ -118,11 +118,15 @@ struct WkbRow {
 
 // Kind and counts of one record; an invalid record counts as a null row and
 // carries its error code.
-__device__ WkbRow wkb_row(const uint8_t* __restrict__ data, const int64_t* __restrict__ off,
+__device__ WkbRow wkb_row(const uint8_t* __restrict__ data, int64_t data_bytes, const int64_t* __restrict__ off,
                           const uint8_t* __restrict__ valid, int64_t row) {
     WkbRow R = {kKindNull, kWkbOk, 0, 0, 0, 0, true};
     const int64_t b0 = off[row], b1 = off[row + 1];
-    if ((valid && !valid[row]) || b1 <= b0) return R;
+    if ((valid && !valid[row]) || b1 == b0) return R;
+    if (b0 < 0 || b1 < b0 || b1 > data_bytes) {  // the record's bytes are not all inside the payload
+        R.err = kWkbTruncated;
+        return R;
+    }
     WkbCursor c = {data, b0, b1, true, true};
     const int t = c.header();
     int err = kWkbOk;
@@ -244,6 +248,7 @@ __global__ void wkb_init_kernel(unsigned long long* status, long long* totals) {
 }
 
 __global__ __launch_bounds__(kWkbThreads) void wkb_count_kernel(const uint8_t* __restrict__ data,
+                                                               int64_t data_bytes,
                                                                const int64_t* __restrict__ off,
                                                                const uint8_t* __restrict__ valid, int64_t n,
                                                                WkbWs W) {
@@ -255,7 +260,7 @@ __global__ __launch_bounds__(kWkbThreads) void wkb_count_kernel(const uint8_t* _
     for (int j = 0; j < kWkbRowsPerThread; ++j) {
         const int64_t row = base + (int64_t)j * kWkbThreads;
         if (row >= n) break;
-        const WkbRow R = wkb_row(data, off, valid, row);
+        const WkbRow R = wkb_row(data, data_bytes, off, valid, row);
         if (R.err != kWkbOk && bad < 0) {
             bad = row;
             bad_code = R.err;
@@ -324,7 +329,7 @@ __global__ __launch_bounds__(1024) void wkb_tiles_kernel(WkbWs W, int64_t* __res
 }
 
 __global__ __launch_bounds__(kWkbThreads) void wkb_fill_rows_kernel(
-    const uint8_t* __restrict__ data, const int64_t* __restrict__ off, const uint8_t* __restrict__ valid, int64_t n,
+    const uint8_t* __restrict__ data, int64_t data_bytes, const int64_t* __restrict__ off, const uint8_t* __restrict__ valid, int64_t n,
     WkbWs W, double* __restrict__ px, double* __restrict__ py, int64_t* __restrict__ point_rows,
     int32_t* __restrict__ geom_offsets, int32_t* __restrict__ part_offsets, int32_t* __restrict__ ring_offsets,
     int64_t* __restrict__ polygon_rows) {
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(kWkbThreads) void wkb_fill_rows_kernel(
         if (tile0 + (int64_t)j * kWkbThreads >= n) break;  // the same in every thread of the block
         const int64_t row = tile0 + (int64_t)j * kWkbThreads + threadIdx.x;
         WkbRow R = {kKindNull, kWkbOk, 0, 0, 0, 0, true};
-        if (row < n) R = wkb_row(data, off, valid, row);
+        if (row < n) R = wkb_row(data, data_bytes, off, valid, row);
         long long v[kWkbCols], tot[kWkbCols];
         row_columns(R, v);
         block_scan_cols(v, lds, tot);
@@ -427,9 +432,10 @@ extern "C" {
 
 int64_t vs_wkb_workspace_bytes(int64_t n_rows) { return vs::wkb_workspace_bytes(n_rows); }
 
-int vs_wkb_scan(const uint8_t* payload, const int64_t* offsets, const uint8_t* validity, int64_t n_rows,
-                int64_t* d_result, void* workspace, int64_t workspace_bytes, void* stream) {
+int vs_wkb_scan(const uint8_t* payload, int64_t payload_bytes, const int64_t* offsets, const uint8_t* validity,
+                int64_t n_rows, int64_t* d_result, void* workspace, int64_t workspace_bytes, void* stream) {
     vs::clear_error();
+    VS_REQUIRE(payload_bytes >= 0, "vs_wkb_scan: negative payload size");
     VS_REQUIRE(n_rows >= 0 && n_rows < (int64_t)INT32_MAX, "vs_wkb_scan: n_rows out of range");
     VS_REQUIRE(workspace_bytes >= vs::wkb_workspace_bytes(n_rows), "vs_wkb_scan: workspace too small");
     VS_REQUIRE(d_result != nullptr && (n_rows == 0 || offsets != nullptr), "vs_wkb_scan: null pointer");
@@ -440,7 +446,7 @@ int vs_wkb_scan(const uint8_t* payload, const int64_t* offsets, const uint8_t* v
     VS_LAUNCH_CHECK();
     if (n_rows > 0) {
         hipLaunchKernelGGL(vs::wkb_count_kernel, dim3((unsigned)W.n_tiles), dim3(vs::kWkbThreads), 0, s, payload,
-                           offsets, validity, n_rows, W);
+                           payload_bytes, offsets, validity, n_rows, W);
         VS_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(vs::wkb_tiles_kernel, dim3(1), dim3(1024), 0, s, W, d_result);
@@ -448,11 +454,12 @@ int vs_wkb_scan(const uint8_t* payload, const int64_t* offsets, const uint8_t* v
     return VS_OK;
 }
 
-int vs_wkb_fill(const uint8_t* payload, const int64_t* offsets, const uint8_t* validity, int64_t n_rows,
-                const void* workspace, int64_t workspace_bytes, double* point_x, double* point_y,
+int vs_wkb_fill(const uint8_t* payload, int64_t payload_bytes, const int64_t* offsets, const uint8_t* validity,
+                int64_t n_rows, const void* workspace, int64_t workspace_bytes, double* point_x, double* point_y,
                 int64_t* point_rows, double* x, double* y, int32_t* geom_offsets, int32_t* part_offsets,
                 int32_t* ring_offsets, int64_t* polygon_rows, void* stream) {
     vs::clear_error();
+    VS_REQUIRE(payload_bytes >= 0, "vs_wkb_fill: negative payload size");
     VS_REQUIRE(n_rows >= 0 && n_rows < (int64_t)INT32_MAX, "vs_wkb_fill: n_rows out of range");
     VS_REQUIRE(workspace_bytes >= vs::wkb_workspace_bytes(n_rows), "vs_wkb_fill: workspace too small");
     VS_REQUIRE(geom_offsets && part_offsets && ring_offsets, "vs_wkb_fill: null offsets");
@@ -461,7 +468,7 @@ int vs_wkb_fill(const uint8_t* payload, const int64_t* offsets, const uint8_t* v
     const vs::WkbWs W = vs::carve_wkb(c, n_rows);
     if (n_rows > 0) {
         hipLaunchKernelGGL(vs::wkb_fill_rows_kernel, dim3((unsigned)W.n_tiles), dim3(vs::kWkbThreads), 0, s,
-                           payload, offsets, validity, n_rows, W, point_x, point_y, point_rows, geom_offsets,
+                           payload, payload_bytes, offsets, validity, n_rows, W, point_x, point_y, point_rows, geom_offsets,
                            part_offsets, ring_offsets, polygon_rows);
         VS_LAUNCH_CHECK();
         const int64_t waves = n_rows < vs::kWkbPolyWaves ? n_rows : vs::kWkbPolyWaves;

@@ -261,6 +261,8 @@ def decode_wkb_device(d_payload, d_offsets, d_validity=None, workspace=None, tim
         raise ValueError("offsets must hold n + 1 entries")
     if d_payload.dtype != torch.uint8 or d_offsets.dtype != torch.int64:
         raise TypeError("payload must be uint8 and offsets int64")
+    if not (d_payload.is_cuda and d_offsets.is_cuda and d_payload.is_contiguous() and d_offsets.is_contiguous()):
+        raise ValueError("payload and offsets must be contiguous device tensors")
     if d_validity is not None and (d_validity.dtype != torch.uint8 or d_validity.numel() != n):
         raise ValueError("validity must be uint8 with one entry per record")
     dev = d_payload.device
@@ -271,11 +273,17 @@ def decode_wkb_device(d_payload, d_offsets, d_validity=None, workspace=None, tim
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if timing is not None else None
     if ev:
         ev[0].record()
-    check(L.vs_wkb_scan(ptr(d_payload), ptr(d_offsets), ptr(d_validity), n, ptr(res), ptr(ws), ws.numel(),
+    # the kernels bound every record by the payload's size (a record outside
+    # it is reported as truncated), so device offsets need no host check
+    nbytes = int(d_payload.numel())
+    check(L.vs_wkb_scan(ptr(d_payload), nbytes, ptr(d_offsets), ptr(d_validity), n, ptr(res), ptr(ws), ws.numel(),
                         stream()), "vs_wkb_scan")
     if ev:
         ev[1].record()
-    r = res.cpu().tolist()
+    # the result words and the column's first and last offsets in one read
+    r = torch.cat([res, d_offsets[:1], d_offsets[n:]]).cpu().tolist() if n else res.cpu().tolist() + [0, 0]
+    if r[8] < 0 or r[9] > nbytes:
+        raise ValueError(f"offsets out of the payload: first {r[8]}, last {r[9]}, payload {nbytes} bytes")
     if r[5] >= 0:
         raise ValueError(f"WKB record {r[5]}: {_WKB_ERRORS.get(int(r[6]), 'invalid record')}")
     if r[7]:
@@ -293,7 +301,7 @@ def decode_wkb_device(d_payload, d_offsets, d_validity=None, workspace=None, tim
     grow = torch.empty(max(n_poly, 1), **i64)
     if ev:
         ev[2].record()
-    check(L.vs_wkb_fill(ptr(d_payload), ptr(d_offsets), ptr(d_validity), n, ptr(ws), ws.numel(), ptr(px),
+    check(L.vs_wkb_fill(ptr(d_payload), nbytes, ptr(d_offsets), ptr(d_validity), n, ptr(ws), ws.numel(), ptr(px),
                         ptr(py), ptr(prow), ptr(x), ptr(y), ptr(geom), ptr(part), ptr(ring), ptr(grow), stream()),
           "vs_wkb_fill")
     if ev:

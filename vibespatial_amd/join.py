@@ -54,9 +54,11 @@ class DeviceTree:
         self.item_ids = item_ids
         self.item_keys = item_keys
         self.item_bounds = item_bounds
-        # staged pairs per query box of the last box query over this tree:
-        # sizes the next query's output (an undersized one traverses twice)
-        self.hits_per_query = 0.0
+        # staged pairs per query box of the last box query of each kind
+        # ("bbox", "dwithin", ...) over this tree that had no caller
+        # capacity: sizes the next such query's output (an undersized one
+        # traverses twice)
+        self.hits_per_query = {}
 
     @property
     def n_items(self) -> int:
@@ -345,17 +347,29 @@ def _split_boxes(run, boxes: torch.Tensor):
         return torch.cat([l0, l1 + h]), torch.cat([r0, r1]), s0
 
 
-def query_boxes(qboxes: torch.Tensor, tree: DeviceTree, capacity: int | None = None, sort: bool = True):
+# A box query sized from the tree's hint never starts above this many slots
+# per query box (at least 1024 in all): a hint from queries with many hits per
+# box must not size a large query of few hits (the retry covers the rest).
+HINT_SLOTS_PER_QUERY = 32
+
+
+def query_boxes(qboxes: torch.Tensor, tree: DeviceTree, capacity: int | None = None, sort: bool = True,
+                kind: str = "bbox"):
     """(left, right, stats): (query, item) pairs whose boxes overlap; lexsorted
     when `sort`, else grouped by query with items in unspecified order.  Query
-    sets whose hits exceed one call's int32 slot limit are split."""
+    sets whose hits exceed one call's int32 slot limit are split.  Without a
+    `capacity` the output is sized from the hits per box of the tree's last
+    query of the same `kind`, clamped to HINT_SLOTS_PER_QUERY per box."""
     dev = require_gpu()
     L = lib()
 
     def run(qb):
         n = int(qb.shape[0])
-        per_query = max(2.0, 1.25 * tree.hits_per_query)
-        cap = int(capacity) if capacity is not None else max(1024, int(per_query * n))
+        if capacity is not None:
+            cap = int(capacity)
+        else:
+            per_query = min(max(2.0, 1.25 * tree.hits_per_query.get(kind, 0.0)), float(HINT_SLOTS_PER_QUERY))
+            cap = max(1024, int(per_query * n))
         cap = min(cap, PAIR_LIMIT)
         retries = 0
         while True:
@@ -370,8 +384,8 @@ def query_boxes(qboxes: torch.Tensor, tree: DeviceTree, capacity: int | None = N
                 "vs_query_boxes",
             )
             k, staged, cand, cverts = _read_result(res)
-            if n:
-                tree.hits_per_query = staged / n
+            if n and capacity is None:
+                tree.hits_per_query[kind] = staged / n
             if staged <= cap:
                 return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts)
             if staged > PAIR_LIMIT:
