@@ -176,6 +176,9 @@ int64_t vs_refine_pairs_workspace_bytes(int64_t n_pairs);
  *           covers, contains_properly (no boundary contact), touches
  *           (boundaries meet, interiors do not), overlaps (interiors meet,
  *           neither inside the other), crosses (never).
+ *   bits 8-15 of mode (mode 1): log2 of the coordinate-count product above
+ *           which a pair is refined one wave per pair (0: the default 2^8;
+ *           63: never); the results are the same whatever the threshold.
  *   mode 2: left rows polygons B, right rows points (px, py); keep as mode 0
  *           for the right point located in the left polygon (contains,
  *           covers, contains_properly, touches, intersects of polygons over

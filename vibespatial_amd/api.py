@@ -85,7 +85,9 @@ def _empty_pairs(dev):
 
 
 def _refine(mode: int, predicate: str, left, right, points: PointArray | None, A: PolygonArray | None,
-            B: PolygonArray | None):
+            B: PolygonArray | None, heavy_log2: int = 0):
+    """vs_refine_pairs; heavy_log2 moves the one-wave-per-pair threshold of
+    polygon pairs (0: the library default, 63: never)."""
     dev = require_gpu()
     L = lib()
     n = int(left.numel())
@@ -102,7 +104,7 @@ def _refine(mode: int, predicate: str, left, right, points: PointArray | None, A
     a_args = DA.args() if DA is not None else none
     b_args = DB.args() if DB is not None else none
     check(
-        L.vs_refine_pairs(mode, PRED.get(predicate, 1), ptr(left), ptr(right), n,
+        L.vs_refine_pairs(mode | (int(heavy_log2) << 8), PRED.get(predicate, 1), ptr(left), ptr(right), n,
                           ptr(P.x if P else None), ptr(P.y if P else None), *a_args, *b_args,
                           ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
         "vs_refine_pairs",

@@ -44,7 +44,7 @@ __device__ __forceinline__ int loc_keep(int pred, int loc) {
 //         intersects of a query polygon over tree points.
 // Pairs are visited in order[t] when given (polygon pairs grouped by right
 // row, so the lanes of a wave mostly share one right polygon).  Polygon pairs
-// whose coordinate counts multiply to more than kHeavyPair go to `heavy`
+// whose coordinate counts multiply to more than the heavy-pair threshold go to `heavy`
 // (count in *n_heavy) for refine_heavy_kernel, one wave per pair; pairs the
 // cheap tests leave undecided go to `undecided` (count in *n_undecided) for
 // refine_relate_kernel, so this kernel carries none of the full relation's
@@ -52,10 +52,9 @@ __device__ __forceinline__ int loc_keep(int pred, int loc) {
 // walking the edge pairs of two many-vertex polygons leaves the rest of its
 // wave idle, so those go one wave per pair (100k x 100k star polygons,
 // overlaps: 49 ms per step at 2^15, 24 at 2^12, 15.7 at 2^8, flat below).
-#ifndef VS_HEAVY_PAIR_LOG2
-#define VS_HEAVY_PAIR_LOG2 8
-#endif
-constexpr int64_t kHeavyPair = (int64_t)1 << VS_HEAVY_PAIR_LOG2;
+// The caller can move the threshold (bits 8-15 of vs_refine_pairs' mode), so
+// tests run the same pairs through both routes.
+constexpr int kHeavyPairLog2 = 8;
 
 __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, const int64_t* __restrict__ left,
                                                           const int64_t* __restrict__ right, int64_t n,
@@ -65,7 +64,8 @@ __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, c
                                                           int32_t* __restrict__ keep, int32_t* __restrict__ heavy,
                                                           unsigned long long* __restrict__ n_heavy,
                                                           int32_t* __restrict__ undecided,
-                                                          unsigned long long* __restrict__ n_undecided) {
+                                                          unsigned long long* __restrict__ n_undecided,
+                                                          int64_t heavy_pair) {
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
         const int64_t i = order ? (int64_t)order[t] : t;
         const int64_t l = left[i], r = right[i];
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void refine_pairs_kernel(int mode, int pred, c
             const double x = px[pi], y = py[pi];
             const int loc = (x == x && y == y) ? point_location_f64(x, y, B.x, B.y, B.geom, B.part, B.ring, gi) : 0;
             k = loc_keep(pred, loc);
-        } else if ((int64_t)row_coords(A, l) * row_coords(B, r) > kHeavyPair) {
+        } else if ((int64_t)row_coords(A, l) * row_coords(B, r) > heavy_pair) {
             heavy[atomicAdd(n_heavy, 1ull)] = (int32_t)i;
             k = 0;  // decided by refine_heavy_kernel
         } else {
@@ -270,7 +270,12 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
                     int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
                     int64_t workspace_bytes, void* stream) {
     vs::clear_error();
+    const int heavy_log2 = (mode >> 8) & 0xFF;
+    mode &= 0xFF;
     VS_REQUIRE(mode >= 0 && mode <= 2, "vs_refine_pairs: mode %d not in {0, 1, 2}", mode);
+    VS_REQUIRE(heavy_log2 <= 63, "vs_refine_pairs: heavy-pair threshold 2^%d out of range", heavy_log2);
+    const int64_t heavy_pair = heavy_log2 == 0 ? ((int64_t)1 << vs::kHeavyPairLog2)
+                               : heavy_log2 >= 63 ? INT64_MAX : ((int64_t)1 << heavy_log2);
     VS_REQUIRE(n_pairs >= 0 && n_pairs < (int64_t)INT32_MAX, "vs_refine_pairs: n_pairs out of range");
     VS_REQUIRE(workspace_bytes >= vs_refine_pairs_workspace_bytes(n_pairs), "vs_refine_pairs: workspace too small");
     VS_REQUIRE(d_result != nullptr, "vs_refine_pairs: null result");
@@ -310,7 +315,7 @@ int vs_refine_pairs(int32_t mode, int32_t predicate, const int64_t* left, const 
         }
         const int grid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 256), 16384);
         hipLaunchKernelGGL(vs::refine_pairs_kernel, dim3(grid), dim3(256), 0, s, mode, predicate, left, right,
-                           n_pairs, ord, px, py, A, B, keep, heavy, n_heavy, undecided, n_undecided);
+                           n_pairs, ord, px, py, A, B, keep, heavy, n_heavy, undecided, n_undecided, heavy_pair);
         VS_LAUNCH_CHECK();
         if (mode == 1) {
             const int hgrid = (int)std::min<int64_t>(vs::ceil_div(n_pairs, 4), 2048);
