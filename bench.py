@@ -69,6 +69,9 @@ def main(argv=None):
                          "dwithin join on the c2 data")
     ap.add_argument("--distance", type=float, default=25.0, help="dwithin distance (workload dwithin)")
     ap.add_argument("--predicate", default="overlaps", help="polygon x polygon predicate (workload relate)")
+    ap.add_argument("--extras", type=int, default=1, choices=(0, 1),
+                    help="c2: also time the lookup ablation and the public API (0: only the two forms, for "
+                         "profiler runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once (a kernel of known traffic for "
                          "scaling hardware byte counters)")
@@ -146,6 +149,41 @@ def main(argv=None):
     st_l, ms_l, kms_l = measure(step_learned, "learned+fp32")
     assert st_p.pairs == st_l.pairs and st_p.candidates == st_l.candidates, "forms disagree"
     K = st_l.pairs
+
+    # the same join with the network left out of the lookups (leaf table
+    # alone; plain binary search over the keys) and with other training
+    # lengths: the network's share of the learned form's time
+    ablation = {}
+    for name, kw in (("leaves", {"lookup": "leaves"}), ("binary", {"lookup": "binary"}),
+                     ("untrained", {"train_iters": 0}), ("train_64", {"train_iters": 64})) if args.extras else ():
+        st_a, ms_a, kms_a = measure(lambda cap, kw=kw: J.sjoin_points_polygons_learned(
+            Q, P, J.polygon_bounds(P), "within", capacity=cap, **kw), f"learned lookup={name}")
+        assert st_a.pairs == K
+        ablation[name] = {"ms_per_step": ms_a, "kernel_ms": kms_a, "lookup_probes": st_a.lookup_probes,
+                          "mlp_leaf_error": st_a.mlp_leaf_error}
+
+    # the public API on the same data: GeoSeries(polygons).sindex.query(points,
+    # predicate='within', return_device=True), the route sjoin takes (device
+    # pairs; frame assembly on the host excluded)
+    ms_pub, strategy = None, []
+    if args.extras:
+        from vibespatial_amd.api import GeoSeries
+
+        gs = GeoSeries(w.polygons)
+        w.points._device = Q  # the column's device copy is resident, as after the first query
+        w.polygons._device = P
+        res = gs.sindex.query(w.points, predicate="within", return_device=True)
+        assert torch.equal(res[0], step_learned(None)[0]) and int(res[0].numel()) == K
+        del res
+        torch.cuda.synchronize()
+        _log(f"public api: timing {args.steps} steps")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = gs.sindex.query(w.points, predicate="within", return_device=True)
+            del res
+        torch.cuda.synchronize()
+        ms_pub = (time.perf_counter() - t0) * 1e3 / args.steps
+        strategy = gs.sindex.last_strategy
     # algorithmic bytes per launch of each form's dominant kernel
     alg_p = 16 * n_pts + poly_bytes + 16 * st_p.candidates + 16 * st_p.candidate_vertices + 8 * K
     alg_l = 16 * n_pts + poly_bytes + 8 * st_l.candidates + 8 * st_l.candidate_vertices + 32 * st_l.escalated_edges \
@@ -155,41 +193,21 @@ def main(argv=None):
     value = K / (ms_l * 1e-3)
     value_p = K / (ms_p * 1e-3)
     ms_per_step = ms_l
+    traffic = _measured_traffic(n_pts, len(P_h))
 
     cpu = None
     if args.cpu_sample > 0:
         import oracle
 
-        cores = min(16, len(os.sched_getaffinity(0)))
         ns = min(args.cpu_sample, n_pts)
-        _log(f"cpu baseline on {ns} points, {cores} threads")
         px, py = w.points.x[:ns], w.points.y[:ns]
-        oracle.sjoin_points_polygons(px[: ns // 10], py[: ns // 10], P_h, "within", nthreads=cores)  # warm-up
-        runs = []
-        for _ in range(3):  # median of 3 after the warm-up
-            tc0 = time.perf_counter()
-            el, _ = oracle.sjoin_points_polygons(px, py, P_h, "within", nthreads=cores)
-            runs.append(time.perf_counter() - tc0)
-        tc = statistics.median(runs)
-        # one thread on a tenth of the sample, median of 3
-        n1 = max(ns // 10, 1)
-        runs1 = []
-        for _ in range(3):
-            tc0 = time.perf_counter()
-            e1, _ = oracle.sjoin_points_polygons(px[:n1], py[:n1], P_h, "within", nthreads=1)
-            runs1.append(time.perf_counter() - tc0)
-        t1 = statistics.median(runs1)
-        cpu = {
-            "value": len(el) / tc,
-            "unit": "pairs/s",
-            "cores": cores,
-            "kind": "port",
-            "sample": f"first {ns:,} of the {n_pts:,} points x all {len(P_h):,} polygons, within "
-                      f"({len(el):,} pairs, median of 3 runs {tc:.2f} s; tree build included)",
-            "one_thread": {"value": len(e1) / t1, "unit": "pairs/s", "cores": 1,
-                           "sample": f"first {n1:,} points x all polygons ({len(e1):,} pairs, median of 3 "
-                                     f"runs {t1:.2f} s)"},
-        }
+        cpu = _cpu_timing(lambda nt: len(oracle.sjoin_points_polygons(px, py, P_h, "within", nthreads=nt)[0]),
+                          lambda: len(oracle.sjoin_points_polygons(px[: ns // 10], py[: ns // 10], P_h, "within",
+                                                                   nthreads=1)[0]), "pairs/s")
+        cpu["sample"] = (f"first {ns:,} of the {n_pts:,} points x all {len(P_h):,} polygons, within "
+                         f"({cpu.pop('count'):,} pairs; {cpu.pop('runs_s')}; tree build included)")
+        cpu["one_thread"]["sample"] = (f"first {ns // 10:,} points x all polygons ({cpu['one_thread'].pop('count'):,} "
+                                       f"pairs; {cpu['one_thread'].pop('runs_s')})")
 
     out = {
         "metric": METRIC,
@@ -221,6 +239,10 @@ def main(argv=None):
             "plain_fp64": {"value": value_p, "ms_per_step": ms_p, "kernel_ms": kms_p,
                            "roofline_frac": ach_p / HBM_PEAK_GBS},
             "learned_vs_plain": value / value_p,
+            "public_api": {"call": "GeoSeries(polygons).sindex.query(points, predicate='within', "
+                                   "return_device=True)", "strategy": [list(t) for t in strategy],
+                           "ms_per_step": ms_pub, "value": K / (ms_pub * 1e-3) if ms_pub else None},
+            "lookup_ablation": ablation,
         },
         "roofline": {
             "bound": "hbm",
@@ -229,9 +251,15 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach_l / HBM_PEAK_GBS,
-            "traffic": _measured_traffic(n_pts, len(P_h)),
+            "traffic": traffic,
             "kernel_ms": kms_l,
             "alg_bytes_per_launch": alg_l,
+            # the same algorithmic bytes over the whole step (index build,
+            # covers, scan + refine, finish), and the measured DRAM bytes of
+            # the kernel over its duration
+            "step_frac": alg_l / (ms_l * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "dram_frac": None if traffic is None else traffic / (kms_l * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic_source": None if traffic is None else _TRAFFIC_SOURCE,
         },
         "cpu_baseline": cpu,
     }
@@ -250,8 +278,58 @@ def main(argv=None):
 _TRAFFIC_BYTES = {(10_000_000, 100_000): 1_152_207_514}
 
 
+_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of learned_join_kernel on this workload, scaled by a "
+                   "1 GiB copy (profiles/learned_join_kernel_10Mpts_100kpolys_pmc_traffic.md); not measured in "
+                   "this run")
+
+
 def _measured_traffic(n_points, n_polys):
     return _TRAFFIC_BYTES.get((n_points, n_polys))
+
+
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    cores = min(visible, int(share)) if share and share.isdigit() and int(share) > 0 else visible
+    return model, visible, cores
+
+
+def _cpu_timing(run_all, run_one, unit, reps=5):
+    """CPU baseline protocol: one warm-up, then the median of `reps` runs, on
+    every usable core (the affinity set, within OMP_NUM_THREADS when the host
+    caps this process's share), and on one thread over a tenth of the
+    sample.  run_all(nthreads) and run_one() return the result count."""
+    model, visible, cores = _cpu_info()
+    _log(f"cpu baseline: {cores} threads ({visible} visible, {model})")
+    run_all(cores)  # warm-up
+    runs = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        count = run_all(cores)
+        runs.append(time.perf_counter() - t0)
+    tc = statistics.median(runs)
+    run_one()
+    runs1 = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        c1 = run_one()
+        runs1.append(time.perf_counter() - t0)
+    t1 = statistics.median(runs1)
+    return {
+        "value": count / tc, "unit": unit, "cores": cores, "kind": "port", "cpu_model": model,
+        "cores_visible": visible, "count": count, "runs_s": f"median of {reps} runs after a warm-up {tc:.3f} s",
+        "one_thread": {"value": c1 / t1, "unit": unit, "cores": 1, "count": c1,
+                       "runs_s": f"median of {reps} runs {t1:.3f} s"},
+    }
 
 
 def _copy_probe(args):
@@ -343,6 +421,19 @@ def run_c4(args):
     ht, ms_t, _ = measure(tree_step, "tree")
     assert hl == ht, "forms disagree"
     alg = 16 * n_pts + 32 * n_box + 24 * hl
+    cpu = None
+    if args.cpu_sample > 0:
+        import oracle
+
+        ns, nq = min(n_pts, 5_000_000), min(n_box, 100_000)
+        px, py, bx = w.points.x[:ns], w.points.y[:ns], w.meta["boxes"][:nq]
+        cpu = _cpu_timing(lambda nt: len(oracle.box_query_points(px, py, bx, nthreads=nt)[0]),
+                          lambda: len(oracle.box_query_points(px[: ns // 10], py[: ns // 10], bx, nthreads=1)[0]),
+                          "hits/s", reps=3)
+        cpu["sample"] = (f"first {ns:,} of the {n_pts:,} points (tree build included) x first {nq:,} boxes "
+                         f"({cpu.pop('count'):,} hits; {cpu.pop('runs_s')})")
+        cpu["one_thread"]["sample"] = (f"first {ns // 10:,} points x the same boxes "
+                                       f"({cpu['one_thread'].pop('count'):,} hits; {cpu['one_thread'].pop('runs_s')})")
     out = {
         "metric": "sindex.query hits/sec @50M pts x 1M boxes (point tree, intersects); learned vs tree",
         "value": hl / (ms_l * 1e-3),
@@ -368,7 +459,7 @@ def run_c4(args):
         "roofline": {"bound": "hbm", "kernel": "whole step (build + query)", "achieved": alg / (ms_l * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms_l * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "alg_bytes_per_step": alg},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -425,7 +516,29 @@ def run_c5(args):
     assert stg.pairs == st.pairs
     VA = np.diff(left.ring_offsets[left.part_offsets[left.geom_offsets]])
     VB = np.diff(right.ring_offsets[right.part_offsets[right.geom_offsets]])
-    alg = 16 * int(VA.sum() + VB.sum()) + 8 * st.pairs  # + sum_C 16 (V_a + V_b): candidate list not kept
+    # sum_q 16 V_q + sum_C 16 (V_a + V_b) + 8 K (the candidate term counted on the device)
+    alg = 16 * int(VA.sum() + VB.sum()) + 16 * st.candidate_vertices + 8 * st.pairs
+    cpu = None
+    if args.cpu_sample > 0:
+        import oracle
+
+        nj = min(512, len(lj) - 1)  # a bounded sample: the first jobs
+        Ls, Rs = left.take(np.arange(lj[nj])), right.take(np.arange(rj[nj]))
+        jl = np.repeat(np.arange(nj), np.diff(lj[: nj + 1]))
+        jr = np.repeat(np.arange(nj), np.diff(rj[: nj + 1]))
+
+        def per_job(nt):  # every job's join: the oracle over the rows, pairs inside one job
+            a, b = oracle.sjoin_polygons_polygons(Ls, Rs, nthreads=nt)
+            return int((jl[a] == jr[b]).sum())
+
+        nj1 = max(nj // 10, 1)
+        L1, R1 = left.take(np.arange(lj[nj1])), right.take(np.arange(rj[nj1]))
+        cpu = _cpu_timing(per_job, lambda: len(oracle.sjoin_polygons_polygons(L1, R1, nthreads=1)[0]), "pairs/s",
+                          reps=3)
+        cpu["sample"] = (f"first {nj} of the {len(lj) - 1} jobs ({lj[nj]:,} x {rj[nj]:,} rows; "
+                         f"{cpu.pop('count'):,} pairs; {cpu.pop('runs_s')})")
+        cpu["one_thread"]["sample"] = (f"first {nj1} jobs ({cpu['one_thread'].pop('count'):,} pairs; "
+                                       f"{cpu['one_thread'].pop('runs_s')})")
     out = {
         "metric": "batched sjoin(intersects) refined pairs/sec, 4096 polygon x polygon jobs",
         "value": st.pairs / (ms_step * 1e-3),
@@ -451,8 +564,9 @@ def run_c5(args):
         },
         "roofline": {"bound": "hbm", "kernel": "whole step", "achieved": alg / (ms_step * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "alg_bytes_per_step_lower_bound": alg},
-        "cpu_baseline": None,
+                     "traffic": None, "alg_bytes_per_step": alg,
+                     "candidate_vertices": st.candidate_vertices},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0

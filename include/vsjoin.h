@@ -248,8 +248,11 @@ int64_t vs_learned_index_bytes(int64_t n_points, int32_t key_bytes, int32_t n_le
 /* Byte offsets inside the index buffer of: keys, perm, sorted x, sorted y,
  * extent (4 doubles), MLP params (97 floats: w1[32], b1[32], w2[32], b2),
  * losses (2 floats: before / after training), leaf keys (n_leaves + 1),
- * leaf errors (int32 n_leaves), meta (int32[4]: [0] max MLP leaf error). */
-int vs_learned_index_layout(int64_t n_points, int32_t key_bytes, int32_t n_leaves, int64_t* offsets10);
+ * leaf errors (int32 n_leaves), meta (int32[4]: [0] max MLP leaf error),
+ * routing windows (int32 pairs, n_leaves: for each leaf the network predicts,
+ * the first and last true leaf of the keys it routes there; first > last
+ * when it routes none). */
+int vs_learned_index_layout(int64_t n_points, int32_t key_bytes, int32_t n_leaves, int64_t* offsets11);
 
 /* Builds the index over points: Hilbert keys over the point extent, radix-
  * sorted (key_bytes 8: a 2^32 x 2^32 grid; key_bytes 4: a 2^12 x 2^12 grid,
@@ -260,7 +263,8 @@ int vs_learned_index_build(const double* px, const double* py, int64_t n_points,
                            int32_t train_iters, float lr, void* index, int64_t index_bytes, void* stream);
 
 /* Batch lower_bound of query keys in the index's sorted keys.  lookup_mode:
- * 0 learned (MLP leaf window + leaf error window), 1 leaf table only (binary
+ * 0 learned (the network's routing window of leaves, its max leaf error when
+ * the window is empty, then the leaf error window), 1 leaf table only (binary
  * search over all leaves, then the leaf error window), 2 plain binary search.
  * d_stat3 (device uint64[3], accumulated): probes, MLP-window fallbacks,
  * error-window fallbacks.  Results are exact in every mode. */
@@ -331,27 +335,30 @@ int vs_sjoin_points_polygons_learned(const double* px, const double* py, int64_t
  * Batched polygon x polygon intersects
  * --------------------------------------------------------------------- */
 
-int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t capacity);
+int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t n_right_coords,
+                                              int64_t capacity);
 
 /* Many independent sjoin(left_j, right_j, predicate='intersects') jobs in one
  * fixed launch sequence (a forest of packed trees, one persistent traversal,
  * one exact refine, one finish).  Job j owns left rows [left_job_offsets[j],
  * left_job_offsets[j+1]) of A and right rows [right_job_offsets[j],
  * right_job_offsets[j+1]) of B (device int64 arrays of n_jobs + 1 entries,
- * non-decreasing); only pairs inside one job are tested.  Output: global row
- * pairs lexsorted, so each job's pairs are contiguous and lexsorted.
+ * non-decreasing); only pairs inside one job are tested.  n_right_coords:
+ * the length of bx / by (sizes the y-slab index of the right rows).  Output:
+ * global row pairs lexsorted, so each job's pairs are contiguous and lexsorted.
  * tile: bits 0-7 traversal waves per block (1-4, 0 = 4), bits 8-23 refine
  * block size (multiple of 64 up to 256, 0 = 256).  d_result (device int64[4]):
  * [0] pairs, [1] bbox candidates staged (> capacity: retry with that
- * capacity), [2] bbox candidates.  Replaces, per job, the reference's
+ * capacity), [2] bbox candidates, [3] sum over the candidates of both rows'
+ * coordinate counts.  Replaces, per job, the reference's
  * polygon x polygon sjoin path (spatial/spatial_index_device.py:228-390,
  * predicates/polygon.py:849-996, spatial/query.py:143-153). */
 int vs_batched_intersects(const double* ax, const double* ay, const int32_t* a_geom, const int32_t* a_part,
                           const int32_t* a_ring, int64_t n_left, const double* bx, const double* by,
                           const int32_t* b_geom, const int32_t* b_part, const int32_t* b_ring, int64_t n_right,
-                          const int64_t* left_job_offsets, const int64_t* right_job_offsets, int64_t n_jobs,
-                          int32_t tile, int64_t capacity, int64_t* out_left, int64_t* out_right, int64_t* d_result,
-                          void* workspace, int64_t workspace_bytes, void* stream);
+                          int64_t n_right_coords, const int64_t* left_job_offsets, const int64_t* right_job_offsets,
+                          int64_t n_jobs, int32_t tile, int64_t capacity, int64_t* out_left, int64_t* out_right,
+                          int64_t* d_result, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
  * WKB ingest (GeoParquet / GeoArrow WKB columns -> SoA on the device)
@@ -557,6 +564,13 @@ int vs_host_point_segment_within(double px, double py, double ax, double ay, dou
 
 /* 64-bit Hilbert index of 32-bit grid coordinates. */
 uint64_t vs_host_hilbert64(uint32_t x, uint32_t y);
+
+/* The learned index's cover of the box [x0, x1] x [y0, y1] over a point
+ * extent (extent4: x0, y0, x1, y1 of the points) for key_bytes 4 or 8: at
+ * most 16 inclusive key ranges (lo, hi) written to out32 (uint64[32]);
+ * returns their count.  Every point of the box has its key in one of them. */
+int vs_host_cover_ranges(double x0, double y0, double x1, double y1, const double* extent4, int32_t key_bytes,
+                         uint64_t* out32);
 
 #ifdef __cplusplus
 }

@@ -333,7 +333,8 @@ def _poly_empty(P, row):
 
 def distance_f64(left_kind, right_kind, L, R, li, ri, exclusive=False) -> float:
     """fp64 distance of one pair (kinds: 0 points, 1 polygons), +inf for
-    empty / NaN geometries and, with `exclusive`, equal points; 0 when a point
+    empty / NaN geometries and, with `exclusive`, equal geometries (equal
+    points; polygons that cover each other, relate_ref); 0 when a point
     is not outside a polygon (C restatement of the exact location) or two
     polygons intersect (C restatement), else the least vertex - segment
     distance over both directions."""
@@ -357,6 +358,12 @@ def distance_f64(left_kind, right_kind, L, R, li, ri, exclusive=False) -> float:
     if _poly_empty(L, li) or _poly_empty(R, ri):
         return np.inf
     if polygon_intersects(L, li, R, ri):
+        if exclusive:
+            from . import relate_ref
+
+            # topological equality (GEOS equals): each covers the other
+            if relate_ref.predicate("covers", L, li, R, ri) and relate_ref.predicate("covered_by", L, li, R, ri):
+                return np.inf
         return 0.0
     best = np.inf
     for P, rp, Q, rq in ((L, li, R, ri), (R, ri, L, li)):

@@ -130,3 +130,45 @@ def test_workspace_queries_are_positive_and_monotone():
     assert 0 < L.vs_sjoin_points_polygons_workspace_bytes(1000, 10, 100, 10) < \
         L.vs_sjoin_points_polygons_workspace_bytes(1000, 10, 100, 10**6)
     assert L.vs_sort_workspace_bytes(10**6, 4) < L.vs_sort_workspace_bytes(10**6, 8)
+
+
+@pytest.mark.parametrize("key_bytes", [4, 8])
+def test_host_cover_ranges_hold_every_box_point(key_bytes):
+    """The learned index's box cover: at most 16 sorted, disjoint key ranges
+    that hold the key of every point inside the box (keys computed with the
+    host Hilbert encoder on the grid the index uses), and that hold fewer
+    points outside the box than the unrefined cell cover would."""
+    L = _lib.lib()
+    rng = np.random.default_rng(30 + key_bytes)
+    ext = np.array([0.0, 0.0, 1e5, 1e5])
+    px, py = rng.uniform(0, 1e5, 200_000), rng.uniform(0, 1e5, 200_000)
+    bits = 4 * key_bytes
+    cells = 4294967295.0 if bits == 32 else float((1 << bits) - 1)
+    qx = np.minimum(np.maximum((px - ext[0]) * (cells / (ext[2] - ext[0])), 0), cells).astype(np.uint64)
+    qy = np.minimum(np.maximum((py - ext[1]) * (cells / (ext[3] - ext[1])), 0), cells).astype(np.uint64)
+    sh = 0 if key_bytes == 8 else 16
+    keys = np.array([L.vs_host_hilbert64(int(a) << sh, int(b) << sh) for a, b in zip(qx, qy)], dtype=np.uint64)
+    if key_bytes == 4:  # 16-bit grid key, low 8 bits cleared (the 2^12 index grid)
+        keys = (keys >> np.uint64(32)) & np.uint64(0xFFFFFF00)
+    order = np.argsort(keys)
+    ks = keys[order]
+    out = np.zeros(32, dtype=np.uint64)
+    scanned = inside = 0
+    for _ in range(300):
+        c = rng.uniform(0, 1e5, 2)
+        w, h = rng.lognormal(np.log(300), 1.0, 2)
+        x0, y0, x1, y1 = c[0] - w / 2, c[1] - h / 2, c[0] + w / 2, c[1] + h / 2
+        m = L.vs_host_cover_ranges(x0, y0, x1, y1, ext.ctypes.data_as(ctypes.c_void_p), key_bytes,
+                                   out.ctypes.data_as(ctypes.c_void_p))
+        assert 1 <= m <= 16
+        lo, hi = out[0:2 * m:2], out[1:2 * m:2]
+        assert np.all(lo <= hi) and np.all(hi[:-1] < lo[1:])  # sorted, disjoint
+        inbox = (px >= x0) & (px <= x1) & (py >= y0) & (py <= y1)
+        k = keys[inbox]
+        j = np.searchsorted(lo, k, side="right") - 1
+        assert np.all(j >= 0) and np.all(k <= hi[j])
+        a, b = np.searchsorted(ks, lo, "left"), np.searchsorted(ks, hi, "right")
+        scanned += int((b - a).sum())
+        inside += int(inbox.sum())
+    print(f"cover: {scanned} positions scanned for {inside} points in the boxes ({scanned / max(inside, 1):.3f})")
+    assert scanned >= inside

@@ -90,21 +90,59 @@ def test_learned_lower_bound_is_exact_in_every_mode(key_bytes):
     assert probes["learned"] < probes["leaves"] and probes["learned"] < probes["binary"]
 
 
+def _route_stats(idx, keys_np):
+    """Mean log2 width of the routing window and max |predicted - true leaf|
+    over the index's own keys, computed on the host from the device tables."""
+    win = idx.leaf_win.cpu().numpy()
+    width = (win[:, 1] - win[:, 0] + 1).astype(np.float64)
+    used = width > 0
+    return float(np.mean(np.log2(width[used]))), int(used.sum())
+
+
 def test_training_improves_the_rank_model():
+    """Training (full-batch Adam on the rank loss, by the device kernels)
+    against the same network at its initialisation (train_iters=0): the loss,
+    the routing windows and the lookup probes all fall strictly."""
     rng = np.random.default_rng(5)
     px, py = _zipf_points(400_000, rng)
-    trained = J.LearnedPointIndex(_dp(px, py), key_bytes=8, train_iters=128)
+    trained = J.LearnedPointIndex(_dp(px, py), key_bytes=8, train_iters=256, lr=3e-4)
     init = J.LearnedPointIndex(_dp(px, py), key_bytes=8, train_iters=0)
     l0, l1 = trained.losses.cpu().numpy()
-    assert l1 <= l0
-    assert trained.mlp_leaf_error <= init.mlp_leaf_error
+    i0, i1 = init.losses.cpu().numpy()
+    assert i0 == i1 == l0  # same data, same initialisation
+    assert l1 < l0
+    keys_np = trained.keys.cpu().numpy()
+    w_t, used_t = _route_stats(trained, keys_np)
+    w_i, used_i = _route_stats(init, keys_np)
     keys = trained.keys
-    q = keys[torch.randint(0, keys.numel(), (50_000,), device=keys.device)]
+    q = keys[torch.randint(0, keys.numel(), (50_000,), device=keys.device, generator=torch.Generator(
+        device=keys.device).manual_seed(3))]
+    q = q + 1  # between keys as well as on them
     _, st_t = trained.lower_bound(q, "learned")
     _, st_i = init.lower_bound(q, "learned")
+    _, st_l = init.lower_bound(q, "leaves")
     print(f"loss {l0:.3e} -> {l1:.3e}; MLP leaf error {init.mlp_leaf_error} -> {trained.mlp_leaf_error}; "
-          f"probes {st_i['probes'] / 5e4:.2f} -> {st_t['probes'] / 5e4:.2f}")
-    assert st_t["probes"] <= st_i["probes"]
+          f"mean log2 routing window {w_i:.3f} -> {w_t:.3f}; probes {st_i['probes'] / 5e4:.3f} -> "
+          f"{st_t['probes'] / 5e4:.3f} (leaf table alone {st_l['probes'] / 5e4:.3f}); fallbacks "
+          f"{st_i['mlp_fallbacks']} -> {st_t['mlp_fallbacks']}")
+    assert w_t < w_i
+    assert st_t["probes"] < st_i["probes"] < st_l["probes"]
+
+
+def test_routing_windows_cover_every_key():
+    """Every index key's true leaf lies in the routing window of its predicted
+    leaf, so lookups of index keys never fall back."""
+    rng = np.random.default_rng(6)
+    px, py = _zipf_points(300_000, rng)
+    idx = J.LearnedPointIndex(_dp(px, py), key_bytes=4, train_iters=64)
+    keys = idx.keys
+    _, st = idx.lower_bound(keys[1:], "learned")
+    _, st_b = idx.lower_bound(keys[1:], "leaves")
+    n = keys.numel() - 1
+    print(f"probes per lookup: learned {st['probes'] / n:.2f}, leaf table {st_b['probes'] / n:.2f}; "
+          f"window fallbacks {st['mlp_fallbacks']}")
+    assert st["mlp_fallbacks"] == 0
+    assert st["probes"] < 0.75 * st_b["probes"]
 
 
 # ---------------------------------------------------------------------------

@@ -109,6 +109,30 @@ def test_polygons_polygons():
     _check(1, 1, A, B, max_distance=4.0)
 
 
+def test_polygons_polygons_exclusive_equal_rows():
+    """exclusive between polygon columns: tree rows topologically equal to
+    the query row (same ring with another start vertex or orientation) are
+    left out, and the next nearest rows returned, as the oracle decides."""
+    print("nearest polygons x polygons exclusive", flush=True)
+    rng = np.random.default_rng(8)
+    B = S.star_polygons(rng, rng.uniform(0, 150, 50), rng.uniform(0, 150, 50), rng.uniform(2, 7, 50),
+                        rng.integers(4, 20, 50))
+    geoms = []
+    for i in range(40):
+        g = B.scalar(i)
+        ring = np.asarray(g.parts[0][0])[:-1]
+        k = i % 3
+        if k == 1:
+            ring = np.roll(ring, 2, axis=0)
+        elif k == 2:
+            ring = ring[::-1]
+        geoms.append(Polygon([tuple(p) for p in ring]))
+    A = PolygonArray.from_geometries(geoms + [B.scalar(45), box(0, 0, 3, 3)])
+    _check(1, 1, A, B, exclusive=True)
+    _check(1, 1, A, B, exclusive=True, return_all=False)
+    _check(1, 1, A, B, exclusive=True, max_distance=2.0)
+
+
 def test_null_rows_and_sjoin_nearest():
     print("sjoin_nearest", flush=True)
     left = GeoDataFrame({"a": np.arange(6), "geometry": [Point(0, 0), None, Point(5, 5), Point(), Point(9, 1),

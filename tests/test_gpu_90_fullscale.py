@@ -1,0 +1,194 @@
+"""Parity at the full sizes of the workloads: C3 (10M points x 100k polygons
+at UTM offsets with on-edge points and shared edges, within and intersects),
+C5 (every one of the 4096 batched polygon x polygon jobs), C4 (50M points x
+1M boxes, learned index against the packed tree, per-box checksums, sampled
+brute force), and the device Hilbert keys against the oracle's encoder."""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from vibespatial_amd import _lib
+from vibespatial_amd import join as J
+from vibespatial_amd import synthetic as S
+from vibespatial_amd.device import DevicePoints, DevicePolygons
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, len(os.sched_getaffinity(0)))
+
+
+def _q(v, lo, hi, cells):
+    """The device's grid quantisation (vs_exact.h quantize / key_scale) in
+    fp64 numpy: clamp((v - lo) * (cells / (hi - lo)), 0, cells), truncated."""
+    scale = cells / (hi - lo) if hi > lo else 0.0
+    q = (v - lo) * scale
+    return np.where(q > 0.0, np.minimum(q, cells), 0.0).astype(np.uint64)
+
+
+# ---------------------------------------------------------------------------
+# device Hilbert keys == the oracle's encoder
+# ---------------------------------------------------------------------------
+def test_device_hilbert_keys_match_oracle():
+    L = _lib.lib()
+    w = S.c2(n_points=1_000_000, n_polys=100_000)
+    P = DevicePolygons.from_host(w.polygons)
+    bounds = J.polygon_bounds(P)
+    tree = J.build_tree(bounds)
+    b = bounds.cpu().numpy()
+    cx, cy = 0.5 * (b[:, 0] + b[:, 2]), 0.5 * (b[:, 1] + b[:, 3])
+    qx, qy = _q(cx, cx.min(), cx.max(), 4294967295.0), _q(cy, cy.min(), cy.max(), 4294967295.0)
+    ids = tree.item_ids.cpu().numpy()
+    keys = tree.item_keys.cpu().numpy().view(np.uint64)
+    # packed tree over polygon boxes: 64-bit keys of the box centres, sorted
+    assert np.all(keys[:-1] <= keys[1:])
+    top = oracle.hilbert16((qx[ids] >> np.uint64(16)).astype(np.uint32), (qy[ids] >> np.uint64(16)).astype(np.uint32))
+    assert np.array_equal((keys >> np.uint64(32)).astype(np.uint32), top)
+    sample = np.random.default_rng(0).choice(len(ids), 2000, replace=False)
+    full = np.array([L.vs_host_hilbert64(int(qx[ids[i]]), int(qy[ids[i]])) for i in sample], dtype=np.uint64)
+    assert np.array_equal(keys[sample], full)
+
+    # learned index, 64-bit keys (2^32 grid over the point extent)
+    px, py = w.points.x, w.points.y
+    Q = DevicePoints.from_host(w.points)
+    idx = J.LearnedPointIndex(Q, key_bytes=8, train_iters=0)
+    perm = idx.perm.cpu().numpy()
+    k8 = idx.keys.cpu().numpy().view(np.uint64)
+    assert np.all(k8[:-1] <= k8[1:])
+    qx, qy = _q(px, px.min(), px.max(), 4294967295.0), _q(py, py.min(), py.max(), 4294967295.0)
+    top = oracle.hilbert16((qx[perm] >> np.uint64(16)).astype(np.uint32),
+                           (qy[perm] >> np.uint64(16)).astype(np.uint32))
+    assert np.array_equal((k8 >> np.uint64(32)).astype(np.uint32), top)
+
+    # learned index, 32-bit keys: 16-bit grid, low 8 bits cleared (the 2^12 grid)
+    idx4 = J.LearnedPointIndex(Q, key_bytes=4, train_iters=0)
+    perm4 = idx4.perm.cpu().numpy()
+    k4 = idx4.keys.cpu().numpy().view(np.uint32)
+    qx, qy = _q(px, px.min(), px.max(), 65535.0), _q(py, py.min(), py.max(), 65535.0)
+    h = oracle.hilbert16(qx[perm4].astype(np.uint32), qy[perm4].astype(np.uint32))
+    assert np.array_equal(k4, h & np.uint32(0xFFFFFF00))
+
+
+# ---------------------------------------------------------------------------
+# C3 at full size
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def c3_full():
+    def verify(px, py, P, rows):
+        return oracle.point_location_pairs(px, py, np.arange(len(px)), rows, P, nthreads=NTHREADS)
+
+    return S.c3(n_points=10_000_000, n_polys=100_000, verify=verify)
+
+
+@pytest.mark.parametrize("predicate", ["within", "intersects"])
+def test_c3_full_size_bit_exact(c3_full, predicate):
+    w = c3_full
+    print(f"c3 10M x 100k {predicate}", flush=True)
+    Q = DevicePoints.from_host(w.points)
+    P = DevicePolygons.from_host(w.polygons)
+    l, r, st = J.sjoin_points_polygons_learned(Q, P, J.polygon_bounds(P), predicate)
+    el, er = oracle.sjoin_points_polygons(w.points.x, w.points.y, w.polygons, predicate, nthreads=NTHREADS)
+    assert np.array_equal(l.cpu().numpy(), el) and np.array_equal(r.cpu().numpy(), er)
+    print(f"c3 {predicate}: {len(el)} pairs; escalation {st.escalated_edges} of "
+          f"{st.orient_tests + st.escalated_edges} orientation decisions ({st.escalation_fraction:.3e}); "
+          f"pairs with an escalation {st.escalated_pairs} of {st.candidates} candidates")
+    assert st.escalated_edges > 0
+
+
+# ---------------------------------------------------------------------------
+# C5: all 4096 jobs
+# ---------------------------------------------------------------------------
+def test_c5_all_jobs_bit_exact():
+    print("c5 4096 jobs", flush=True)
+    left, right, lj, rj = S.c5()
+    A, B = DevicePolygons.from_host(left), DevicePolygons.from_host(right)
+    l, r, st = J.batched_intersects(A, B, torch.from_numpy(lj).cuda(), torch.from_numpy(rj).cuda())
+    l, r = l.cpu().numpy(), r.cpu().numpy()
+    # the oracle over all rows at once, restricted to pairs inside one job,
+    # is every job's own join (a pair's rows belong to exactly one job each)
+    el, er = oracle.sjoin_polygons_polygons(left, right, nthreads=NTHREADS)
+    job_l = np.repeat(np.arange(len(lj) - 1), np.diff(lj))
+    job_r = np.repeat(np.arange(len(rj) - 1), np.diff(rj))
+    same = job_l[el] == job_r[er]
+    el, er = el[same], er[same]
+    assert np.array_equal(l, el) and np.array_equal(r, er)
+    # per job: its pairs are a contiguous, lexsorted run of its own rows
+    assert np.all(job_l[l] == job_r[r])
+    VA = np.diff(left.ring_offsets[left.part_offsets[left.geom_offsets]])
+    VB = np.diff(right.ring_offsets[right.part_offsets[right.geom_offsets]])
+    assert st.candidates >= st.pairs == len(el)
+    print(f"c5: {st.pairs} pairs of {st.candidates} candidates; sum over candidates of V_a + V_b "
+          f"{st.candidate_vertices}")
+    assert st.candidate_vertices > st.candidates * (VA.min() + VB.min())
+
+
+# ---------------------------------------------------------------------------
+# C4 at full size: 50M points x 1M boxes
+# ---------------------------------------------------------------------------
+def _box_checksums(l, r, n_boxes):
+    """Per box: hit count, sum of point ids, sum of (id * 2654435761 mod 2^31)."""
+    cnt = torch.bincount(l, minlength=n_boxes)
+    s1 = torch.zeros(n_boxes, dtype=torch.int64, device=l.device).index_add_(0, l, r)
+    s2 = torch.zeros(n_boxes, dtype=torch.int64, device=l.device).index_add_(0, l, (r * 2654435761) & 0x7FFFFFFF)
+    return cnt, s1, s2
+
+
+def test_c4_full_size_learned_equals_tree():
+    print("c4 50M x 1M", flush=True)
+    w = S.c4()
+    px, py = w.points.x, w.points.y
+    boxes = w.meta["boxes"]
+    nb = len(boxes)
+    Q = DevicePoints.from_host(w.points)
+    bt = torch.from_numpy(boxes).cuda()
+    edges = np.linspace(0, nb, 5).astype(np.int64)
+    idx = J.LearnedPointIndex(Q, key_bytes=8)
+    acc_l = [torch.zeros(nb, dtype=torch.int64, device="cuda") for _ in range(3)]
+    hits_l = 0
+    sample = np.sort(np.random.default_rng(2).choice(nb, 200, replace=False))
+    sample_t = torch.from_numpy(sample).cuda()
+    sampled = {}
+    for i in range(4):
+        b0, b1 = int(edges[i]), int(edges[i + 1])
+        l, r, st = idx.query_boxes(bt[b0:b1], "learned", sort=True)
+        l += b0
+        hits_l += st.pairs
+        for a, v in zip(acc_l, _box_checksums(l, r, nb)):
+            a += v
+        m = torch.isin(l, sample_t)
+        ln, rn = l[m].cpu().numpy(), r[m].cpu().numpy()
+        for bx in np.unique(ln):
+            sampled[int(bx)] = rn[ln == bx]
+        del l, r, m
+    del idx
+    torch.cuda.empty_cache()
+    pts = torch.stack([Q.x, Q.y, Q.x, Q.y], dim=1).contiguous()
+    tree = J.build_tree(pts)
+    del pts
+    acc_t = [torch.zeros(nb, dtype=torch.int64, device="cuda") for _ in range(3)]
+    hits_t = 0
+    for i in range(4):
+        b0, b1 = int(edges[i]), int(edges[i + 1])
+        l, r, st = J.query_boxes(bt[b0:b1], tree, sort=False)
+        l += b0
+        hits_t += st.pairs
+        for a, v in zip(acc_t, _box_checksums(l, r, nb)):
+            a += v
+        del l, r
+    print(f"c4: {hits_l} hits (learned), {hits_t} (tree)")
+    assert hits_l == hits_t
+    for a, b in zip(acc_l, acc_t):
+        assert torch.equal(a, b)
+    # sampled brute force: points sorted by x, each sampled box's x-range sliced
+    o = np.argsort(px, kind="stable")
+    sx, sy = px[o], py[o]
+    for bx in sample:
+        x0, y0, x1, y1 = boxes[bx]
+        a, b = np.searchsorted(sx, x0, "left"), np.searchsorted(sx, x1, "right")
+        sel = o[a:b][(sy[a:b] >= y0) & (sy[a:b] <= y1)]
+        assert np.array_equal(np.sort(sel), sampled.get(int(bx), np.empty(0, np.int64))), bx

@@ -55,6 +55,33 @@ def test_point_polygon_and_polygon_distances():
     assert oracle.distance_f64(0, 0, pts, pts, 0, 0, exclusive=True) == math.inf
 
 
+def test_oracle_exclusive_polygons_is_topological_equality():
+    """exclusive drops equal polygons whatever their vertex order (start vertex,
+    orientation, a collinear extra vertex), and only those."""
+    from vibespatial_amd.geometry import Polygon
+
+    sq = [(0, 0), (10, 0), (10, 10), (0, 10)]
+    P = PolygonArray.from_geometries([
+        Polygon(sq),
+        Polygon(sq[2:] + sq[:2]),            # other start vertex
+        Polygon(sq[::-1]),                   # clockwise
+        Polygon([(0, 0), (5, 0), (10, 0), (10, 10), (0, 10)]),  # collinear extra vertex
+        box(0, 0, 10, 10.5),                 # not equal: larger
+        box(0, 0, 9, 10),                    # not equal: smaller, inside
+        box(20, 0, 30, 10),                  # apart
+    ])
+    for j in range(4):
+        assert oracle.distance_f64(1, 1, P, P, 0, j, exclusive=True) == math.inf
+        assert oracle.distance_f64(1, 1, P, P, 0, j) == 0.0
+    assert oracle.distance_f64(1, 1, P, P, 0, 4, exclusive=True) == 0.0
+    assert oracle.distance_f64(1, 1, P, P, 0, 5, exclusive=True) == 0.0
+    assert oracle.distance_f64(1, 1, P, P, 0, 6, exclusive=True) == 10.0
+    l, r, d = oracle.nearest_pairs(1, 1, P, P, exclusive=True)
+    assert list(zip(l.tolist(), r.tolist())) == [(0, 4), (0, 5), (1, 4), (1, 5), (2, 4), (2, 5), (3, 4), (3, 5),
+                                                 (4, 0), (4, 1), (4, 2), (4, 3), (4, 5), (5, 0), (5, 1), (5, 2),
+                                                 (5, 3), (5, 4), (6, 0), (6, 1), (6, 2), (6, 3), (6, 4)]
+
+
 _GRID = PointArray(np.arange(10.0), np.arange(10.0))
 
 

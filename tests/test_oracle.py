@@ -108,10 +108,6 @@ def test_polygon_intersects_is_symmetric_and_matches_rationals():
 def test_point_location_matches_rationals_on_stars_with_holes():
     rng = np.random.default_rng(4)
     m = 25
-    P = S.star_polygons(rng, rng.uniform(0, 100, m), rng.uniform(0, 100, m), rng.uniform(5, 20, m),
-                        rng.integers(4, 30, m), holes=rng.integers(0, 4, m),
-                        hole_vertices=rng.integers(4, 9, 200)[: 0] if False else None, hole_frac=None) \
-        if False else None
     holes = rng.integers(0, 4, m)
     nh = int(holes.sum())
     P = S.star_polygons(rng, rng.uniform(0, 100, m), rng.uniform(0, 100, m), rng.uniform(5, 20, m),

@@ -107,9 +107,9 @@ SIGNATURES = {
         [_p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _p, _i32, _i32, _i32, _i64, _p, _p, _p, _p, _i64,
          _p],
     ),
-    "vs_batched_intersects_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
-    "vs_batched_intersects": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _p, _p, _i64, _i32,
-                                             _i64, _p, _p, _p, _p, _i64, _p]),
+    "vs_batched_intersects_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64, _i64]),
+    "vs_batched_intersects": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p, _p, _i64,
+                                             _i32, _i64, _p, _p, _p, _p, _i64, _p]),
     "vs_wkb_workspace_bytes": (_i64, [_i64]),
     "vs_wkb_scan": (ctypes.c_int, [_p, _i64, _p, _p, _i64, _p, _p, _i64, _p]),
     "vs_wkb_fill": (ctypes.c_int, [_p, _i64, _p, _p, _i64, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
@@ -142,6 +142,7 @@ SIGNATURES = {
     "vs_host_point_point_within": (ctypes.c_int, [_d] * 5),
     "vs_host_point_segment_within": (ctypes.c_int, [_d] * 7),
     "vs_host_hilbert64": (_u64, [_u32, _u32]),
+    "vs_host_cover_ranges": (ctypes.c_int, [_d, _d, _d, _d, _p, _i32, _p]),
 }
 
 _lock = threading.Lock()

@@ -427,7 +427,9 @@ class SpatialIndex:
         distances when `return_distance`.  Ties (equal distances) are all
         returned unless not `return_all` (then the least tree row).  Columns
         holding both points and polygons are searched per family pair and the
-        per-row minimum taken over the families."""
+        per-row minimum taken over the families.  `exclusive` leaves out tree
+        rows equal to the input geometry (topological equality, as GEOS
+        equals: equal coordinates for points, mutual covers for polygons)."""
         q = as_geometry_array(geometry)
         if max_distance is not None and not (float(max_distance) >= 0):
             raise ValueError("max_distance must be a non-negative number")
@@ -436,8 +438,6 @@ class SpatialIndex:
         fams_t = [(f, a, rows) for f, a, rows in (("point", self._ga.points, self._ga.point_rows),
                                                    ("polygon", self._ga.polygons, self._ga.polygon_rows))
                   if a is not None]
-        if exclusive and any(qf == "polygon" for qf, _, _ in fams_q) and any(tf == "polygon" for tf, _, _ in fams_t):
-            raise NotImplementedError("exclusive nearest between polygons (geometric equality)")
         parts = []
         for qf, qa, qrows in fams_q:
             for tf, ta, trows in fams_t:

@@ -17,8 +17,11 @@
 //      round-robin and each wave walks its own job's tree with a shared LDS
 //      stack, staging bbox candidates;
 //   4. one thread per candidate runs the exact intersects refine (edges of
-//      the pair, then one vertex per ring); survivors go to the finish pass
-//      (radix sort by left row, sort of right rows per left row).
+//      the pair, then one vertex per ring); a right row of one ring with many
+//      edges is read through its y-slab index (built once per call), so a
+//      candidate visits only the edges in its own y-range; survivors go to
+//      the finish pass (radix sort by left row, sort of right rows per left
+//      row).
 // Tile shape (`tile`): waves per block of the traversal and the refine block
 // size, chosen by the host autotuner.
 #include "vs_join_common.h"
@@ -258,8 +261,160 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
     flush();
 }
 
+// ---------------------------------------------------------------------------
+// Y-slab index of the right rows.  A right row of one ring with at least
+// kRowSlabMinEdges edges (and fewer than 2^16 coordinates) gets its y-range
+// [ylo, yhi] cut into B equal slabs (B a power of two, about one per four
+// edges, at most kRowSlabMax); edge e = (c0 + e - 1, c0 + e) is listed in
+// every slab from slab(min y) to slab(max y).  slab() is monotone in y, so a
+// point finds in slab(py) every edge whose closed y-range holds py -- every
+// edge that can toggle its crossing parity or put it on the boundary -- and a
+// box [y0, y1] finds in slabs slab(y0)..slab(y1) every edge whose y-range
+// meets it.  Row r's list starts at 17 r + 2 (first coordinate of r): B + 1
+// slab starts, then at most 2 (n - 1) edge numbers (a row that would need
+// more takes one slab holding every edge).
+// ---------------------------------------------------------------------------
+constexpr int kRowSlabMinEdges = 16;
+constexpr int kRowSlabMax = 16;
+
+struct RowSlabs {
+    const double2* geo;    // (ylo, slabs per unit of y)
+    const int32_t* nslab;  // B, 0: the row has no index
+    const uint16_t* list;
+};
+
+__device__ __forceinline__ int row_slab(double y, double ylo, double inv, int nb) {
+    double f = (y - ylo) * inv;
+    f = fmin(fmax(f, 0.0), (double)(nb - 1));
+    return (int)f;
+}
+
+__device__ __forceinline__ int64_t row_slab_base(int64_t r, int32_t c0) { return 17 * r + 2 * (int64_t)c0; }
+
+// One 64-thread block per row (grid-stride): counts per slab, starts, entries.
+__global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_rows, const double4* __restrict__ rbox,
+                                                            double2* __restrict__ geo, int32_t* __restrict__ nslab,
+                                                            uint16_t* __restrict__ list, int64_t list_cap) {
+    __shared__ int cnt[kRowSlabMax + 1];
+    __shared__ int cur[kRowSlabMax];
+    const int lane = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
+        int32_t c0 = 0, n = 0;
+        // at most 2 (n - 1) + 17 entries from 17 r + 2 c0 on, every value below 2^16
+        const bool ok = single_ring_of(B, r, c0, n) && n - 1 >= kRowSlabMinEdges && n <= 32768 && c0 >= 0 &&
+                        row_slab_base(r, c0) + 17 + 2 * (int64_t)n <= list_cap;
+        if (!ok) {
+            if (lane == 0) nslab[r] = 0;
+            continue;  // block-uniform
+        }
+        const int ne = n - 1;
+        const double4 bb = rbox[r];
+        int nb = 1;
+        while (nb < kRowSlabMax && nb * 4 < ne) nb <<= 1;
+        double inv = (bb.w > bb.y) ? (double)nb / (bb.w - bb.y) : 0.0;
+        if (!(inv > 0.0) || !(inv < INFINITY)) {
+            nb = 1;
+            inv = 0.0;
+        }
+        uint16_t* L = list + row_slab_base(r, c0);
+        if (lane <= kRowSlabMax) cnt[lane] = 0;
+        __syncthreads();
+        int total = 0;
+        for (int e = 1 + lane; e <= ne; e += 64) {
+            const double ay = B.y[c0 + e - 1], by = B.y[c0 + e];
+            const int b0 = row_slab(fmin(ay, by), bb.y, inv, nb), b1 = row_slab(fmax(ay, by), bb.y, inv, nb);
+            total += b1 - b0 + 1;
+            for (int b = b0; b <= b1; ++b) atomicAdd(&cnt[b + 1], 1);
+        }
+        total = wave_sum_i32(total);
+        if (total > 2 * ne) {  // long edges: one slab with every edge
+            nb = 1;
+            inv = 0.0;
+            if (lane == 0) {
+                L[0] = 0;
+                L[1] = (uint16_t)ne;
+            }
+            for (int e = 1 + lane; e <= ne; e += 64) L[2 + e - 1] = (uint16_t)e;
+        } else {
+            int v = lane < nb ? cnt[lane + 1] : 0;
+            v = wave_inclusive_scan(v);
+            __syncthreads();
+            if (lane < nb) {
+                cur[lane] = v - cnt[lane + 1];  // exclusive start of slab lane
+                L[lane + 1] = (uint16_t)v;
+            }
+            if (lane == 0) L[0] = 0;
+            __syncthreads();
+            uint16_t* body = L + nb + 1;
+            for (int e = 1 + lane; e <= ne; e += 64) {
+                const double ay = B.y[c0 + e - 1], by = B.y[c0 + e];
+                const int b0 = row_slab(fmin(ay, by), bb.y, inv, nb), b1 = row_slab(fmax(ay, by), bb.y, inv, nb);
+                for (int b = b0; b <= b1; ++b) body[atomicAdd(&cur[b], 1)] = (uint16_t)e;
+            }
+        }
+        if (lane == 0) {
+            geo[r] = make_double2(bb.y, inv);
+            nslab[r] = nb;
+        }
+        __syncthreads();  // cnt / cur are reused by the next row
+    }
+}
+
+// The exact intersects of ring A (na coordinates, a lane's LDS column) and
+// right row B, a single ring c0b.. of nb coordinates read through its slab
+// index: the same three steps as ring_intersects_polys (A's first vertex
+// located in B -- inside or on the boundary decides; B's edges near A tested
+// against A's edges; B's first vertex located in A), with each step visiting
+// only the slab edges that can change its outcome, so the same boolean.
+__device__ inline bool ring_intersects_slabbed(const RingRef& A, int32_t na, const double* __restrict__ bx,
+                                               const double* __restrict__ by, int32_t c0b,
+                                               const uint16_t* __restrict__ L, int nb, double2 g) {
+    double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+    for (int32_t k = 0; k < na; ++k) {
+        mnx = fmin(mnx, A.X(k));
+        mny = fmin(mny, A.Y(k));
+        mxx = fmax(mxx, A.X(k));
+        mxy = fmax(mxy, A.Y(k));
+    }
+    const uint16_t* body = L + nb + 1;
+    {
+        const double fx = A.X(0), fy = A.Y(0);
+        const int b = row_slab(fy, g.x, g.y, nb);
+        int parity = 0;
+        const int k1 = L[b + 1];
+        for (int k = L[b]; k < k1; ++k) {
+            const int32_t c = c0b + body[k];
+            if (edge_step_f64(fx, fy, bx[c - 1], by[c - 1], bx[c], by[c], parity)) return true;
+        }
+        if (parity) return true;
+    }
+    {
+        const int k0 = L[row_slab(mny, g.x, g.y, nb)], k1 = L[row_slab(mxy, g.x, g.y, nb) + 1];
+        for (int k = k0; k < k1; ++k) {
+            const int32_t c = c0b + body[k];
+            const double px = bx[c - 1], py = by[c - 1], qx = bx[c], qy = by[c];
+            if (fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy) {
+                for (int32_t j = 1; j < na; ++j)
+                    if (segments_touch(A.X(j - 1), A.Y(j - 1), A.X(j), A.Y(j), px, py, qx, qy)) return true;
+            }
+        }
+    }
+    const double px = bx[c0b], py = by[c0b];
+    if (px < mnx || px > mxx || py < mny || py > mxy) return false;  // outside A's bounds: location 0
+    int parity = 0;
+    for (int32_t j = 1; j < na; ++j)
+        if (edge_step_f64(px, py, A.X(j - 1), A.Y(j - 1), A.X(j), A.Y(j), parity)) return true;
+    return parity != 0;
+}
+
+// Coordinates of row r over all its rings.
+__device__ __forceinline__ int32_t row_coords(const Polys& P, int64_t r) {
+    return P.ring[P.part[P.geom[r + 1]]] - P.ring[P.part[P.geom[r]]];
+}
+
 // One thread per candidate: exact intersects; survivors appended to the
-// finish stage with one atomic per wave.
+// finish stage with one atomic per wave.  ctl->stat[1] accumulates, over the
+// candidates, the coordinates of both rows.
 // A left row that is one ring of at most kRingTile coordinates (the
 // footprints) is copied into the lane's column of an LDS tile (x and y, 64
 // lanes wide) and tested from there: its edges are re-read for every right
@@ -270,7 +425,7 @@ constexpr int64_t kRingTileBytesPerWave = 2 * kRingTile * 64 * (int64_t)sizeof(d
 __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __restrict__ cand_l,
                                                           const int32_t* __restrict__ cand_r,
                                                           const JoinCtl* __restrict__ cctl, int64_t ccap, Polys A,
-                                                          Polys B, uint32_t* __restrict__ out_l,
+                                                          Polys B, RowSlabs RS, uint32_t* __restrict__ out_l,
                                                           int32_t* __restrict__ out_r, int64_t capacity,
                                                           JoinCtl* __restrict__ ctl) {
     extern __shared__ double ring_tile[];
@@ -281,6 +436,7 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
     double* ty = tx + kRingTile * 64;
     const uint64_t lt = lanemask_lt();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long cverts = 0;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < (int64_t)nc; i0 += stride) {
         const int64_t i = i0 + lane;
         bool keep = false;
@@ -289,13 +445,22 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
         if (i < (int64_t)nc) {
             l = cand_l[i];
             r = cand_r[i];
+            cverts += (unsigned long long)(row_coords(A, l) + row_coords(B, r));
             int32_t ca = 0, na = 0, cb = 0, nb = 0;
-            if (single_ring_of(A, l, ca, na) && na <= kRingTile && !(single_ring_of(B, r, cb, nb) && nb < na)) {
+            const int nslab = RS.nslab[r];
+            const bool tile_a = single_ring_of(A, l, ca, na) && na <= kRingTile;
+            if (tile_a && (nslab > 0 || !(single_ring_of(B, r, cb, nb) && nb < na))) {
                 for (int32_t k = 0; k < na; ++k) {  // the lane's own column: no other lane reads it
                     tx[k * 64] = A.x[ca + k];
                     ty[k * 64] = A.y[ca + k];
                 }
-                keep = ring_intersects_polys(RingRef{tx, ty, 64}, na, B, r);
+                if (nslab > 0) {  // one ring of more than kRingTile coordinates, indexed
+                    single_ring_of(B, r, cb, nb);
+                    keep = ring_intersects_slabbed(RingRef{tx, ty, 64}, na, B.x, B.y, cb,
+                                                   RS.list + row_slab_base(r, cb), nslab, RS.geo[r]);
+                } else {
+                    keep = ring_intersects_polys(RingRef{tx, ty, 64}, na, B, r);
+                }
             } else {
                 keep = polygons_intersect_auto(A, l, B, r);
             }
@@ -313,15 +478,19 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
             }
         }
     }
+    for (int o = 32; o > 0; o >>= 1) cverts += __shfl_xor(cverts, o, 64);
+    if (lane == 0 && cverts) atomicAdd(&ctl->stat[1], cverts);
 }
 
 // d_result: [0] pairs K, [1] bbox candidates staged (the stage that bounds
-// `capacity`: survivors are a subset), [2] bbox candidates, [3] 0.
-__global__ void publish_batch_kernel(const JoinCtl* __restrict__ cctl, int64_t* __restrict__ result) {
+// `capacity`: survivors are a subset), [2] bbox candidates, [3] sum over the
+// candidates of both rows' coordinates (0 when the candidates overflowed).
+__global__ void publish_batch_kernel(const JoinCtl* __restrict__ cctl, const JoinCtl* __restrict__ ctl,
+                                     int64_t* __restrict__ result) {
     if (threadIdx.x == 0) {
         result[1] = (int64_t)cctl->staged;
         result[2] = (int64_t)cctl->staged;
-        result[3] = 0;
+        result[3] = (int64_t)ctl->stat[1];
     }
 }
 
@@ -353,6 +522,9 @@ struct BatchWs {
     void* fin_ws;
     uint32_t* stage_l;
     int32_t* stage_r;
+    double2* slab_geo;
+    int32_t* slab_n;
+    uint16_t* slab_list;
 };
 
 // Forest nodes are bounded by the items and their parents: < n_right * 16/15 + levels * jobs.
@@ -360,8 +532,8 @@ static int64_t forest_node_bound(int64_t n_right, int64_t n_jobs) {
     return n_right + n_right / (kFF - 1) + 1 + (int64_t)kFMaxLevels * n_jobs;
 }
 
-static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t capacity,
-                           int64_t* bytes) {
+static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t n_right_coords,
+                           int64_t capacity, int64_t* bytes) {
     Carver c(base);
     BatchWs W;
     W.lbox = c.take<double4>(n_left);
@@ -393,6 +565,9 @@ static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n
     W.stage_l = nullptr;
     W.stage_r = nullptr;
     if (base) finish_stage(W.fin_ws, capacity, &W.stage_l, &W.stage_r);
+    W.slab_geo = c.take<double2>(n_right);
+    W.slab_n = c.take<int32_t>(n_right);
+    W.slab_list = c.take<uint16_t>(17 * n_right + 2 * n_right_coords);
     if (bytes) *bytes = c.bytes;
     return W;
 }
@@ -403,7 +578,8 @@ static int bits_for_jobs(int64_t n) {
     return b;
 }
 
-int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n_right, const int64_t* left_job,
+int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n_right, int64_t n_right_coords,
+                       const int64_t* left_job,
                        const int64_t* right_job, int64_t n_jobs, int tile, int64_t capacity, int64_t* out_l,
                        int64_t* out_r, int64_t* d_result, const BatchWs& W, hipStream_t s) {
     ctl_reset(W.cctl, s);
@@ -432,6 +608,10 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         VS_LAUNCH_CHECK();
         VS_TRY(exclusive_scan_i32(W.nodes, W.tree_off, n_jobs, W.total_nodes, W.scan_ws, s));
         VS_TRY(exclusive_scan_i32(W.waves, W.wave_off, n_jobs, W.total_waves, W.scan_ws, s));
+        hipLaunchKernelGGL(row_slab_build_kernel, dim3((unsigned)std::min<int64_t>(n_right, (int64_t)cus * 32)),
+                           dim3(64), 0, s, B, n_right, W.rbox, W.slab_geo, W.slab_n, W.slab_list,
+                           17 * n_right + 2 * n_right_coords);
+        VS_LAUNCH_CHECK();
         hipLaunchKernelGGL(forest_build_kernel, dim3((unsigned)std::min<int64_t>(n_jobs, 65535)), dim3(256), 0, s,
                            W.rbox, W.rperm, right_job, W.tree_off, n_jobs, W.fbox);
         VS_LAUNCH_CHECK();
@@ -457,12 +637,13 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads),
                            (unsigned)(refine_threads / 64 * kRingTileBytesPerWave), s,
                            reinterpret_cast<const uint32_t*>(W.cs_l), reinterpret_cast<const int32_t*>(W.cs_r),
-                           W.cctl, capacity, A, B, W.stage_l, W.stage_r, capacity, W.ctl);
+                           W.cctl, capacity, A, B, RowSlabs{W.slab_geo, W.slab_n, W.slab_list}, W.stage_l,
+                           W.stage_r, capacity, W.ctl);
         VS_LAUNCH_CHECK();
         VS_TRY(timing_end_public(s));
     }
     VS_TRY(finish_pairs(W.fin_ws, W.ctl, n_left, capacity, true, out_l, out_r, d_result, s));
-    hipLaunchKernelGGL(publish_batch_kernel, dim3(1), dim3(64), 0, s, W.cctl, d_result);
+    hipLaunchKernelGGL(publish_batch_kernel, dim3(1), dim3(64), 0, s, W.cctl, W.ctl, d_result);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }
@@ -471,18 +652,19 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
 
 extern "C" {
 
-int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t capacity) {
+int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t n_right_coords,
+                                              int64_t capacity) {
     int64_t b = 0;
-    vs::carve_batch(nullptr, n_jobs, n_left, n_right, capacity, &b);
+    vs::carve_batch(nullptr, n_jobs, n_left, n_right, n_right_coords, capacity, &b);
     return b;
 }
 
 int vs_batched_intersects(const double* ax, const double* ay, const int32_t* a_geom, const int32_t* a_part,
                           const int32_t* a_ring, int64_t n_left, const double* bx, const double* by,
                           const int32_t* b_geom, const int32_t* b_part, const int32_t* b_ring, int64_t n_right,
-                          const int64_t* left_job_offsets, const int64_t* right_job_offsets, int64_t n_jobs,
-                          int32_t tile, int64_t capacity, int64_t* out_left, int64_t* out_right, int64_t* d_result,
-                          void* workspace, int64_t workspace_bytes, void* stream) {
+                          int64_t n_right_coords, const int64_t* left_job_offsets, const int64_t* right_job_offsets,
+                          int64_t n_jobs, int32_t tile, int64_t capacity, int64_t* out_left, int64_t* out_right,
+                          int64_t* d_result, void* workspace, int64_t workspace_bytes, void* stream) {
     vs::clear_error();
     VS_REQUIRE(n_jobs >= 0 && n_jobs < (int64_t)INT32_MAX, "vs_batched_intersects: n_jobs out of range");
     VS_REQUIRE(n_left >= 0 && n_left < (int64_t)INT32_MAX && n_right >= 0 && n_right < (int64_t)INT32_MAX,
@@ -490,13 +672,16 @@ int vs_batched_intersects(const double* ax, const double* ay, const int32_t* a_g
     VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_batched_intersects: capacity out of range");
     VS_REQUIRE(d_result != nullptr, "vs_batched_intersects: null result");
     VS_REQUIRE(n_jobs == 0 || (left_job_offsets && right_job_offsets), "vs_batched_intersects: null job offsets");
+    VS_REQUIRE(n_right_coords >= 0 && n_right_coords < (int64_t)INT32_MAX,
+               "vs_batched_intersects: n_right_coords out of range");
     int64_t need = 0;
-    const vs::BatchWs W = vs::carve_batch(workspace, n_jobs, n_left, n_right, capacity, &need);
+    const vs::BatchWs W = vs::carve_batch(workspace, n_jobs, n_left, n_right, n_right_coords, capacity, &need);
     VS_REQUIRE(workspace_bytes >= need, "vs_batched_intersects: workspace %lld < %lld bytes",
                (long long)workspace_bytes, (long long)need);
     const vs::Polys A = {ax, ay, a_geom, a_part, a_ring};
     const vs::Polys B = {bx, by, b_geom, b_part, b_ring};
-    return vs::batched_intersects(A, n_left, B, n_right, left_job_offsets, right_job_offsets, n_jobs, tile, capacity,
+    return vs::batched_intersects(A, n_left, B, n_right, n_right_coords, left_job_offsets, right_job_offsets, n_jobs,
+                                  tile, capacity,
                                   out_left, out_right, d_result, W, vs::stream_of(stream));
 }
 

@@ -139,6 +139,23 @@ __device__ __forceinline__ int readlane_i32(int v, int lane) {
     return __builtin_amdgcn_readlane(v, lane);
 }
 
+// Values every lane of the wave holds alike (block-uniform LDS reads, kernel
+// arguments indexed by them): the first active lane's copy, kept in scalar
+// registers by the compiler.
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float uniform_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ double uniform_f64(double v) {
+    union { double d; int i[2]; } u;
+    u.d = v;
+    u.i[0] = __builtin_amdgcn_readfirstlane(u.i[0]);
+    u.i[1] = __builtin_amdgcn_readfirstlane(u.i[1]);
+    return u.d;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_min(T v) {
     for (int o = 32; o > 0; o >>= 1) {

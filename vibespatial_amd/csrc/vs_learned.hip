@@ -293,6 +293,7 @@ struct LearnedView {
     const float* mlp;
     const K* leaf_key;  // S + 1
     const int32_t* leaf_err;
+    const int2* leaf_win;  // per predicted leaf: [first, last] true leaf of the keys it routes
     const int32_t* meta;  // [0] E (MLP leaf error)
     int32_t S;
     int32_t mode;  // 0 learned, 1 leaf table only (binary search over leaves), 2 plain binary search
@@ -328,7 +329,8 @@ __device__ __forceinline__ double plm_pred(K q, K k0, K k1, int64_t p0, int64_t 
 template <class K>
 __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ keys, int64_t n, int S,
                                                       K* __restrict__ leaf_key, const float* __restrict__ mlp,
-                                                      int32_t* __restrict__ leaf_err, int32_t* __restrict__ meta) {
+                                                      int32_t* __restrict__ leaf_err, int2* __restrict__ leaf_win,
+                                                      int32_t* __restrict__ meta) {
     __shared__ float P[kNParams];
     for (int k = threadIdx.x; k < kNParams; k += blockDim.x) P[k] = mlp[k];
     __syncthreads();
@@ -343,6 +345,7 @@ __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ key
             if (s == S - 1) leaf_key[S] = k1;
         }
         double emax = 0.0;
+        int sp_lo = INT_MAX, sp_hi = -1;  // predicted leaves of this leaf's first occurrences
         // Positions [p0, p1) in passes of 64 kB keys (one pass for a 512-key
         // leaf), then p1 itself from registers.  The network's weights are read
         // once per hidden unit for kB keys (same operation order as mlp_eval,
@@ -379,6 +382,8 @@ __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ key
                 const int sp = leaf_of_pred(y[b], S);
                 const int d = sp > (int)s ? sp - (int)s : (int)s - sp;
                 mlp_e = d > mlp_e ? d : mlp_e;
+                sp_lo = min(sp_lo, sp);
+                sp_hi = max(sp_hi, sp);
             }
         }
         // position p1 (the same in every lane)
@@ -387,9 +392,29 @@ __global__ __launch_bounds__(256) void leaf_err_kernel(const K* __restrict__ key
             const int sp = leaf_of_pred(mlp_eval(P, key_unit<K>(k1)), S);
             const int d = sp > (int)s ? sp - (int)s : (int)s - sp;
             mlp_e = d > mlp_e ? d : mlp_e;
+            sp_lo = min(sp_lo, sp);
+            sp_hi = max(sp_hi, sp);
         }
         emax = wave_max(emax);
         if (lane == 0) leaf_err[s] = (int32_t)ceil(emax);
+        // Routing windows.  A query q with keys[0] < q <= keys[n-1] resolves
+        // to the true leaf of the first occurrence k_b of the least key >= q;
+        // the distinct key before k_b, k_a, is keys[p0] or a first occurrence
+        // of this leaf.  For a monotone network the predicted leaf of q lies
+        // between those of k_a and k_b, so leaf s is recorded for every
+        // predicted leaf from the least to the greatest over keys[p0] and this
+        // leaf's first occurrences.  Lookups verify the window (a non-monotone
+        // network only costs a fallback).
+        sp_lo = -wave_max(-sp_lo);
+        sp_hi = wave_max(sp_hi);
+        if (sp_hi >= 0) {
+            const int spa = leaf_of_pred(mlp_eval(P, key_unit<K>(k0)), S);
+            const int a = min(sp_lo, spa), b = max(sp_hi, spa);
+            for (int t = a + lane; t <= b; t += 64) {
+                atomicMin(&leaf_win[t].x, (int)s);
+                atomicMax(&leaf_win[t].y, (int)s);
+            }
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         const int v = __shfl_xor(mlp_e, o, 64);
@@ -424,9 +449,17 @@ __device__ int64_t learned_lower_bound(K q, const LearnedView<K>& V, LookupStats
         const int S = V.S;
         int wl = 0, wh = S - 1;
         if (V.mode == 0) {
-            const int E = V.meta[0];
             const int sp = mlp_leaf<K>(V.mlp, q, S);
-            int l2 = sp - E < 0 ? 0 : sp - E, h2 = sp + E > S - 1 ? S - 1 : sp + E;
+            const int2 w = V.leaf_win[sp];
+            int l2, h2;
+            if (w.x <= w.y) {  // the leaves this prediction routes to
+                l2 = w.x;
+                h2 = w.y;
+            } else {  // nothing routed here: the global error bound
+                const int E = V.meta[0];
+                l2 = sp - E < 0 ? 0 : sp - E;
+                h2 = sp + E > S - 1 ? S - 1 : sp + E;
+            }
             st.probes += 2;
             if (V.leaf_key[l2] < q && (h2 == S - 1 || V.leaf_key[h2 + 1] >= q)) {
                 wl = l2;
@@ -468,11 +501,15 @@ __device__ int64_t learned_lower_bound(K q, const LearnedView<K>& V, LookupStats
 // Cover of a box on the key grid: the finest level, no finer than min_l (the
 // index grid), whose cells covering the box number at most kMaxRanges (long
 // thin boxes get thin covers; by level bits-1 at most 2 x 2 remain), each cell
-// a key range [code 4^l, (code+1) 4^l);
-// the codes are sorted (a bitonic network on registers) and consecutive ones
-// merged.  Writes (lo, hi) pairs to out and returns their count.
-__device__ int cover_ranges(double x0, double y0, double x1, double y1, const KeyScale& ks, int bits, int min_l,
-                            unsigned long long* __restrict__ out) {
+// a key range [code 4^l, (code+1) 4^l); the codes are sorted (a bitonic
+// network on registers, each code carrying its cell).  Then each cell is split
+// into its four children one level finer (never finer than the index grid;
+// a child's code is its parent's followed by two digits) and the children that
+// miss the box are dropped, when the kept children still merge into at most
+// kMaxRanges ranges (else the cells themselves are used).  Consecutive codes
+// merge.  Writes (lo, hi) pairs to out and returns their count.
+__host__ __device__ int cover_ranges(double x0, double y0, double x1, double y1, const KeyScale& ks, int bits,
+                                     int min_l, unsigned long long* __restrict__ out) {
     if (!(x0 <= x1 && y0 <= y1)) return 0;
     const uint32_t qx0 = quantize(x0, ks.x0, ks.sx, bits), qx1 = quantize(x1, ks.x0, ks.sx, bits);
     const uint32_t qy0 = quantize(y0, ks.y0, ks.sy, bits), qy1 = quantize(y1, ks.y0, ks.sy, bits);
@@ -482,10 +519,12 @@ __device__ int cover_ranges(double x0, double y0, double x1, double y1, const Ke
     const uint32_t ax = qx0 >> l, ay = qy0 >> l;
     const uint32_t nx = (qx1 >> l) - ax + 1, ncell = nx * ((qy1 >> l) - ay + 1);
     uint64_t code[kMaxRanges];
+    uint32_t tag[kMaxRanges];
 #pragma unroll
     for (int t = 0; t < kMaxRanges; ++t) {
         const uint32_t a = ax + (uint32_t)t % nx, c = ay + (uint32_t)t / nx;
         code[t] = (uint32_t)t < ncell ? hilbert64(a << (32 - lev), c << (32 - lev)) >> (64 - 2 * lev) : ~0ull;
+        tag[t] = (uint32_t)t;
     }
 #pragma unroll
     for (int k = 2; k <= kMaxRanges; k <<= 1)
@@ -500,9 +539,73 @@ __device__ int cover_ranges(double x0, double y0, double x1, double y1, const Ke
                     if ((a > b) == asc) {
                         code[i] = b;
                         code[o] = a;
+                        const uint32_t ta = tag[i];
+                        tag[i] = tag[o];
+                        tag[o] = ta;
                     }
                 }
             }
+    if (l > min_l) {
+        // children one level finer: kept ones as a 4-bit mask per cell,
+        // indexed by the child's last Hilbert digit
+        const int lc = l - 1, levc = bits - lc;
+        const uint32_t cx0 = qx0 >> lc, cx1 = qx1 >> lc, cy0 = qy0 >> lc, cy1 = qy1 >> lc;
+        uint64_t masks = 0;
+#pragma unroll
+        for (int t = 0; t < kMaxRanges; ++t) {
+            if ((uint32_t)t >= ncell) continue;
+            const uint32_t px = 2 * (ax + tag[t] % nx), py = 2 * (ay + tag[t] / nx);
+            uint32_t m = 0;
+#pragma unroll
+            for (int ch = 0; ch < 4; ++ch) {
+                const uint32_t cx = px + (ch & 1), cy = py + (ch >> 1);
+                if (cx < cx0 || cx > cx1 || cy < cy0 || cy > cy1) continue;
+                const uint64_t cc = hilbert64(cx << (32 - levc), cy << (32 - levc)) >> (64 - 2 * levc);
+                m |= 1u << (uint32_t)(cc & 3u);
+            }
+            masks |= (uint64_t)m << (4 * t);
+        }
+        // ranges of the kept children (count first; emit when they fit);
+        // loops unrolled so the code array stays in registers
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+            int m = 0;
+            uint64_t rlo = 0, rhi = 0;
+            bool open = false;
+#pragma unroll
+            for (int t = 0; t < kMaxRanges; ++t) {
+                if ((uint32_t)t >= ncell) continue;
+                const uint32_t cm = (uint32_t)(masks >> (4 * t)) & 15u;
+#pragma unroll
+                for (uint32_t h = 0; h < 4; ++h) {
+                    if (!(cm & (1u << h))) continue;
+                    const uint64_t c = (code[t] << 2) | h;
+                    if (open && c == rhi + 1) {
+                        rhi = c;
+                        continue;
+                    }
+                    if (open) {
+                        if (pass == 1) {
+                            out[2 * m] = rlo << (2 * lc);
+                            out[2 * m + 1] = (rhi << (2 * lc)) | ((1ull << (2 * lc)) - 1ull);
+                        }
+                        ++m;
+                    }
+                    rlo = rhi = c;
+                    open = true;
+                }
+            }
+            if (open) {
+                if (pass == 1) {
+                    out[2 * m] = rlo << (2 * lc);
+                    out[2 * m + 1] = (rhi << (2 * lc)) | ((1ull << (2 * lc)) - 1ull);
+                }
+                ++m;
+            }
+            if (pass == 1) return m;
+            if (m > kMaxRanges || m == 0) break;  // too many: the cells themselves
+        }
+    }
     const uint64_t span = (1ull << (2 * l)) - 1ull;
     int m = 0;
     uint64_t rlo = code[0], rhi = code[0];
@@ -1006,8 +1109,9 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
             }
         }
         __syncthreads();
-        if ((long long)s_item >= T) break;
-        const int q = s_q;
+        if ((long long)uniform_i32(s_item) >= T) break;
+        // item values are block-uniform: kept in scalar registers
+        const int q = uniform_i32(s_q);
         if (tid < s_nr) s_rng[tid] = ranges[(int64_t)q * kMaxRanges + tid];
         __syncthreads();
         if (tid == 0) {
@@ -1020,9 +1124,9 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
         }
         __syncthreads();
         const double4 box = qbox[q];
-        const int nr = s_nr;
-        const int total = s_pre[nr];
-        const int first = s_chunk * kItemPos;
+        const int nr = uniform_i32(s_nr);
+        const int total = uniform_i32(s_pre[nr]);
+        const int first = uniform_i32(s_chunk) * kItemPos;
         const int last = (first + kItemPos < total) ? first + kItemPos : total;
         int4 info = make_int4(0, 0, 0, 0);
         double cx = 0, cy = 0;
@@ -1035,7 +1139,9 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
             cy = 0.5 * (box.y + box.w);
             double M = fmax(fmax(box.z - cx, cx - box.x), fmax(box.w - cy, cy - box.y));
             M *= 1.0 + 4.440892098500626e-16;
-            E = fp32_det_bound(M);
+            E = uniform_f32(fp32_det_bound(M));
+            cx = uniform_f64(cx);
+            cy = uniform_f64(cy);
         }
         const bool direct = MODE == kLModeBox || (info.z == 1 && info.y - info.x <= kEdgeCap + 1);
         if (direct) {
@@ -1072,15 +1178,20 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                 }
 #pragma unroll
                 for (int j = 0; j < kPosPerThread; ++j) {
-                    bool keep = px[j] >= box.x && px[j] <= box.z && py[j] >= box.y && py[j] <= box.w;
-                    if (MODE != kLModeBox && keep) {
-                        ++st_cand;
-                        st_cverts += (unsigned long long)(info.y - info.x);
+                    const bool inbox = px[j] >= box.x && px[j] <= box.z && py[j] >= box.y && py[j] <= box.w;
+                    bool keep = inbox;
+                    if (MODE != kLModeBox) {
                         bool esc = false;
-                        const int loc = refine_one(Et, Ec, s_bstart, s_blist, S, R, (float)(px[j] - cx),
-                                                   (float)(py[j] - cy), px[j], py[j], E, esc, n_orient, n_esc);
-                        st_escpairs += esc ? 1 : 0;
-                        keep = loc_pred<PRED>(loc);
+                        if (inbox) {
+                            const int loc = refine_one(Et, Ec, s_bstart, s_blist, S, R, (float)(px[j] - cx),
+                                                       (float)(py[j] - cy), px[j], py[j], E, esc, n_orient, n_esc);
+                            keep = loc_pred<PRED>(loc);
+                        }
+                        // wave-uniform counters (scalar registers)
+                        const unsigned nb = (unsigned)__popcll(__ballot(inbox));
+                        st_cand += nb;
+                        st_cverts += (unsigned long long)nb * (unsigned long long)(info.y - info.x);
+                        st_escpairs += (unsigned)__popcll(__ballot(esc));
                     }
                     // wave-aggregated append to the block buffer
                     const uint64_t m = __ballot(keep);
@@ -1249,16 +1360,16 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
 #pragma unroll
             for (int j = 0; j < kCandPerThread; ++j) {
                 const int k = tid + j * kLJThreads;
+                st_escpairs += (unsigned)__popcll(__ballot(k < ncand && esc[j]));
                 if (k >= ncand) continue;
                 if (!done[j] && par[j]) loc[j] = 2;
-                st_escpairs += esc[j] ? 1 : 0;
                 if (loc_pred<PRED>(loc[j])) {
                     const int32_t orig = perm[cand[k]];
                     const int slot = atomicAdd(&s_nout, 1);
                     outb[slot] = MODE == kLModePip ? make_int2(orig, q) : make_int2(q, orig);
                 }
             }
-            if (tid == 0) {
+            if (wid == 0) {  // block counts, taken once per block
                 st_cand += (unsigned long long)ncand;
                 st_cverts += (unsigned long long)ncand * (unsigned long long)(c1 - c0);
             }
@@ -1269,16 +1380,21 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
         __syncthreads();
     }
     flush();
-    // counters: [0] candidates, [1] candidate vertices, [2] fp32 orientation
-    // evaluations, [3] escalated edges; candidate pairs with an escalation
-    unsigned long long v[5] = {st_cand, st_cverts, st_orient, st_esc, st_escpairs};
-    for (int k = 0; k < 5; ++k) {
+    // counters: [0] candidates, [1] candidate vertices (wave-uniform), [2] fp32
+    // orientation evaluations, [3] escalated edges (per lane); candidate pairs
+    // with an escalation (wave-uniform)
+    unsigned long long v[2] = {st_orient, st_esc};
+    for (int k = 0; k < 2; ++k) {
         unsigned long long x = v[k];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if (lane == 0 && x) {
-            if (k < 4) atomicAdd(&ctl->stat[k], x);
-            else atomicAdd(&ctl->escalated_pairs, x);
-        }
+        v[k] = x;
+    }
+    if (lane == 0) {
+        if (st_cand) atomicAdd(&ctl->stat[0], st_cand);
+        if (st_cverts) atomicAdd(&ctl->stat[1], st_cverts);
+        if (v[0]) atomicAdd(&ctl->stat[2], v[0]);
+        if (v[1]) atomicAdd(&ctl->stat[3], v[1]);
+        if (st_escpairs) atomicAdd(&ctl->escalated_pairs, st_escpairs);
     }
 }
 
@@ -1329,6 +1445,7 @@ struct IndexBufs {
     float* losses;
     K* leaf_key;
     int32_t* leaf_err;
+    int2* leaf_win;
     int32_t* meta;
     void* build_ws;
 };
@@ -1345,6 +1462,7 @@ IndexBufs<K> carve_index(Carver& c, int64_t n, int S) {
     B.losses = c.take<float>(2);
     B.leaf_key = c.take<K>(S + 1);
     B.leaf_err = c.take<int32_t>(S);
+    B.leaf_win = c.take<int2>(S);
     B.meta = c.take<int32_t>(4);
     B.build_ws = c.take<char>(sizeof(K) == 4 ? point_index_workspace_bytes_u32(n) : point_index_workspace_bytes_u64(n));
     return B;
@@ -1357,6 +1475,11 @@ inline int leaves_for(int64_t n, int leaf_size) {
 
 __global__ void zero_i32_kernel(int32_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// Empty routing windows (first > last) before the error pass records them.
+__global__ __launch_bounds__(256) void clear_win_kernel(int2* __restrict__ w, int S) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < S; i += gridDim.x * 256) w[i] = make_int2(INT_MAX, -1);
 }
 
 // A second stream (default priority) for work that overlaps the caller's
@@ -1407,11 +1530,13 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
         t = side->stream;
     }
     if (n >= 2) {
+        hipLaunchKernelGGL(clear_win_kernel, dim3(stride_grid(S, 256, 64)), dim3(256), 0, t, B.leaf_win, S);
+        VS_LAUNCH_CHECK();
         hipLaunchKernelGGL(mlp_train_kernel<K>, dim3(1), dim3(vs::kTrainThreads), 0, t, B.keys, n, iters, lr, B.mlp,
                            B.losses, B.meta);
         VS_LAUNCH_CHECK();
         hipLaunchKernelGGL(leaf_err_kernel<K>, dim3(stride_grid(S, 4, 64)), dim3(256), 0, t, B.keys, n, S, B.leaf_key,
-                           B.mlp, B.leaf_err, B.meta);
+                           B.mlp, B.leaf_err, B.leaf_win, B.meta);
         VS_LAUNCH_CHECK();
     } else {
         hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, t, B.meta, 4);
@@ -1435,6 +1560,7 @@ LearnedView<K> view_of(const IndexBufs<K>& B, int64_t n, int S, int mode) {
     V.mlp = B.mlp;
     V.leaf_key = B.leaf_key;
     V.leaf_err = B.leaf_err;
+    V.leaf_win = B.leaf_win;
     V.meta = B.meta;
     V.S = S;
     V.mode = n < 2 ? 2 : mode;
@@ -1624,21 +1750,23 @@ int vs_learned_index_build(const double* px, const double* py, int64_t n_points,
 
 // Offsets (bytes from the index base) of the index's arrays, for callers that
 // read them (tests, persistence): keys, perm, sorted x, sorted y, extent, mlp
-// params, losses, leaf keys, leaf errors, meta.
-int vs_learned_index_layout(int64_t n_points, int32_t key_bytes, int32_t n_leaves, int64_t* offsets10) {
+// params, losses, leaf keys, leaf errors, meta, routing windows.
+int vs_learned_index_layout(int64_t n_points, int32_t key_bytes, int32_t n_leaves, int64_t* offsets11) {
     vs::clear_error();
-    VS_REQUIRE(offsets10 != nullptr, "vs_learned_index_layout: null output");
+    VS_REQUIRE(offsets11 != nullptr, "vs_learned_index_layout: null output");
     VS_REQUIRE(key_bytes == 4 || key_bytes == 8, "vs_learned_index_layout: key_bytes must be 4 or 8");
     char* base = reinterpret_cast<char*>(4096);  // any non-null base: offsets are relative
     vs::Carver c(base);
     if (key_bytes == 4) {
         const auto B = vs::carve_index<uint32_t>(c, n_points, n_leaves);
-        const void* p[10] = {B.keys, B.perm, B.sx, B.sy, B.ext, B.mlp, B.losses, B.leaf_key, B.leaf_err, B.meta};
-        for (int k = 0; k < 10; ++k) offsets10[k] = (const char*)p[k] - base;
+        const void* p[11] = {B.keys, B.perm, B.sx, B.sy, B.ext, B.mlp, B.losses, B.leaf_key, B.leaf_err, B.meta,
+                             B.leaf_win};
+        for (int k = 0; k < 11; ++k) offsets11[k] = (const char*)p[k] - base;
     } else {
         const auto B = vs::carve_index<uint64_t>(c, n_points, n_leaves);
-        const void* p[10] = {B.keys, B.perm, B.sx, B.sy, B.ext, B.mlp, B.losses, B.leaf_key, B.leaf_err, B.meta};
-        for (int k = 0; k < 10; ++k) offsets10[k] = (const char*)p[k] - base;
+        const void* p[11] = {B.keys, B.perm, B.sx, B.sy, B.ext, B.mlp, B.losses, B.leaf_key, B.leaf_err, B.meta,
+                             B.leaf_win};
+        for (int k = 0; k < 11; ++k) offsets11[k] = (const char*)p[k] - base;
     }
     return VS_OK;
 }
@@ -1784,6 +1912,23 @@ int vs_sjoin_points_polygons_learned(const double* px, const double* py, int64_t
     return vs::learned_join_impl(px, py, n_points, x, y, geom_offsets, part_offsets, ring_offsets, n_polys, n_parts,
                                  n_rings, n_coords, poly_bounds, predicate, train_iters, lookup_mode, capacity, out_left,
                                  out_right, d_result, workspace, vs::stream_of(stream), true);
+}
+
+// The cover of a box as the learned index computes it, on the host.
+int vs_host_cover_ranges(double x0, double y0, double x1, double y1, const double* extent4, int32_t key_bytes,
+                         uint64_t* out32) {
+    vs::clear_error();
+    VS_REQUIRE(key_bytes == 4 || key_bytes == 8, "vs_host_cover_ranges: key_bytes must be 4 or 8");
+    VS_REQUIRE(extent4 != nullptr && out32 != nullptr, "vs_host_cover_ranges: null argument");
+    const int bits = 4 * key_bytes;
+    const double cells = bits >= 32 ? 4294967295.0 : (double)((1ull << bits) - 1);
+    vs::KeyScale ks;
+    ks.x0 = extent4[0];
+    ks.y0 = extent4[1];
+    ks.sx = extent4[2] > extent4[0] ? cells / (extent4[2] - extent4[0]) : 0.0;
+    ks.sy = extent4[3] > extent4[1] ? cells / (extent4[3] - extent4[1]) : 0.0;
+    const int min_l = bits - (key_bytes == 4 ? vs::point_index_grid_bits<uint32_t>() : vs::point_index_grid_bits<uint64_t>());
+    return vs::cover_ranges(x0, y0, x1, y1, ks, bits, min_l, reinterpret_cast<unsigned long long*>(out32));
 }
 
 }  // extern "C"

@@ -159,10 +159,33 @@ def _kind_args(g):
     return 1, g.args()
 
 
+def refine_polygon_pairs(predicate: str, left: torch.Tensor, right: torch.Tensor, A: DevicePolygons,
+                         B: DevicePolygons):
+    """The (left, right) pairs with predicate(A row, B row), exact, in input
+    order (vs_refine_pairs mode 1)."""
+    dev = require_gpu()
+    L = lib()
+    n = int(left.numel())
+    if n == 0:
+        return left[:0], right[:0]
+    out_l = torch.empty(n, dtype=torch.int64, device=dev)
+    out_r = torch.empty(n, dtype=torch.int64, device=dev)
+    res = torch.zeros(4, dtype=torch.int64, device=dev)
+    ws = _ws(L.vs_refine_pairs_workspace_bytes(n), dev)
+    none = ctypes.c_void_p(0)
+    check(L.vs_refine_pairs(1, PRED[predicate], ptr(left.contiguous()), ptr(right.contiguous()), n, none, none,
+                            *A.args(), *B.args(), ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
+          "vs_refine_pairs")
+    k = int(res[0].item())
+    return out_l[:k], out_r[:k]
+
+
 def pair_distances(left: torch.Tensor, right: torch.Tensor, lgeom, rgeom, exclusive: bool = False,
                    outside: bool = False, cap: torch.Tensor | None = None):
     """fp64 distance of each (left, right) pair (vs_pair_distances); +inf for
-    empty / NaN geometries and, with `exclusive`, for equal point pairs.
+    empty / NaN geometries and, with `exclusive`, for equal geometries: point
+    pairs with equal coordinates, and polygon pairs that are topologically
+    equal (each covers the other, exact; a pair at distance 0 is tested).
     `outside`: every left point is known to lie outside its right polygons.
     `cap` (fp64 per left row): pairs farther than cap[left] get +inf."""
     dev = require_gpu()
@@ -177,6 +200,16 @@ def pair_distances(left: torch.Tensor, right: torch.Tensor, lgeom, rgeom, exclus
         check(L.vs_pair_distances(lk, rk, ptr(left.contiguous()), ptr(right.contiguous()), n, len(rgeom), flags,
                                   ptr(cap), *la, *ra, ptr(out), ptr(ws), ws.numel(), stream()),
               "vs_pair_distances")
+        if exclusive and isinstance(lgeom, DevicePolygons) and isinstance(rgeom, DevicePolygons):
+            z = torch.nonzero(out[:n] == 0).flatten()
+            if z.numel():
+                lz, rz = left[z], right[z]
+                l1, r1 = refine_polygon_pairs("covers", lz, rz, lgeom, rgeom)
+                l2, r2 = refine_polygon_pairs("covered_by", l1, r1, lgeom, rgeom)
+                if l2.numel():
+                    nr = max(len(rgeom), 1)
+                    eq = torch.isin(lz * nr + rz, l2 * nr + r2)
+                    out[z[eq]] = float("inf")
     return out[:n]
 
 
@@ -462,10 +495,10 @@ def sjoin_points_polygons_learned(Q: DevicePoints, P: DevicePolygons, bounds: to
 class LearnedPointIndex:
     """Learned rank index over Hilbert-sorted points (device buffers)."""
 
-    NAMES = ("keys", "perm", "sx", "sy", "extent", "mlp", "losses", "leaf_keys", "leaf_err", "meta")
+    NAMES = ("keys", "perm", "sx", "sy", "extent", "mlp", "losses", "leaf_keys", "leaf_err", "meta", "leaf_win")
 
-    def __init__(self, Q: DevicePoints, key_bytes: int = 8, leaf_size: int = 512, train_iters: int = 64,
-                 lr: float = 0.01):
+    def __init__(self, Q: DevicePoints, key_bytes: int = 8, leaf_size: int = 512, train_iters: int = 256,
+                 lr: float = 3e-4):
         dev = require_gpu()
         L = lib()
         self.n = len(Q)
@@ -475,7 +508,7 @@ class LearnedPointIndex:
         self.buf = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
         check(L.vs_learned_index_build(ptr(Q.x), ptr(Q.y), self.n, key_bytes, self.n_leaves, train_iters, lr,
                                        ptr(self.buf), self.buf.numel(), stream()), "vs_learned_index_build")
-        off = (ctypes.c_int64 * 10)()
+        off = (ctypes.c_int64 * 11)()
         check(L.vs_learned_index_layout(self.n, key_bytes, self.n_leaves, off), "vs_learned_index_layout")
         self.offsets = dict(zip(self.NAMES, list(off)))
 
@@ -507,6 +540,12 @@ class LearnedPointIndex:
     @property
     def leaf_err(self) -> torch.Tensor:
         return self._view("leaf_err", torch.int32, self.n_leaves)
+
+    @property
+    def leaf_win(self) -> torch.Tensor:
+        """(n_leaves, 2) int32: per predicted leaf, the first and last true
+        leaf of the keys the network routes there (first > last: none)."""
+        return self._view("leaf_win", torch.int32, 2 * self.n_leaves).view(self.n_leaves, 2)
 
     def lower_bound(self, qkeys: torch.Tensor, lookup: str = "learned"):
         dev = require_gpu()
@@ -629,18 +668,19 @@ def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Ten
     cap = min(cap, PAIR_LIMIT)
     retries = 0
     while True:
-        ws = _ws(L.vs_batched_intersects_workspace_bytes(n_jobs, len(A), len(B), cap), dev)
+        ws = _ws(L.vs_batched_intersects_workspace_bytes(n_jobs, len(A), len(B), B.n_coords, cap), dev)
         out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
         out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
         res = torch.zeros(4, dtype=torch.int64, device=dev)
         check(L.vs_batched_intersects(ptr(A.x), ptr(A.y), ptr(A.geom_offsets), ptr(A.part_offsets),
                                       ptr(A.ring_offsets), len(A), ptr(B.x), ptr(B.y), ptr(B.geom_offsets),
-                                      ptr(B.part_offsets), ptr(B.ring_offsets), len(B), ptr(lj), ptr(rj), n_jobs,
-                                      code, cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(), stream()),
+                                      ptr(B.part_offsets), ptr(B.ring_offsets), len(B), B.n_coords, ptr(lj), ptr(rj),
+                                      n_jobs, code, cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(),
+                                      stream()),
               "vs_batched_intersects")
-        k, staged, cand, _ = _read_result(res)
+        k, staged, cand, cverts = _read_result(res)
         if staged <= cap:
-            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, 0)
+            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts)
         if staged > PAIR_LIMIT:
             raise OverflowError(f"{staged} bbox candidates exceed the int32 slot limit of one call")
         del ws, out_l, out_r
@@ -690,7 +730,8 @@ class BatchedIntersectsGraph:
         if tile is None:
             tile = _TILE_CACHE.get(_tile_key(self.n_jobs, len(A), len(B)), (4, 256))
         self.code = _encode_tile(*tile)
-        self.ws = _ws(L.vs_batched_intersects_workspace_bytes(self.n_jobs, len(A), len(B), self.cap), dev)
+        self.ws = _ws(L.vs_batched_intersects_workspace_bytes(self.n_jobs, len(A), len(B), B.n_coords, self.cap),
+                      dev)
         self.out_l = torch.empty(max(self.cap, 1), dtype=torch.int64, device=dev)
         self.out_r = torch.empty(max(self.cap, 1), dtype=torch.int64, device=dev)
         self.res = torch.zeros(4, dtype=torch.int64, device=dev)
@@ -709,15 +750,15 @@ class BatchedIntersectsGraph:
         A, B, L = self.A, self.B, lib()
         check(L.vs_batched_intersects(ptr(A.x), ptr(A.y), ptr(A.geom_offsets), ptr(A.part_offsets),
                                       ptr(A.ring_offsets), len(A), ptr(B.x), ptr(B.y), ptr(B.geom_offsets),
-                                      ptr(B.part_offsets), ptr(B.ring_offsets), len(B), ptr(self.lj), ptr(self.rj),
-                                      self.n_jobs, self.code, self.cap, ptr(self.out_l), ptr(self.out_r),
+                                      ptr(B.part_offsets), ptr(B.ring_offsets), len(B), B.n_coords, ptr(self.lj),
+                                      ptr(self.rj), self.n_jobs, self.code, self.cap, ptr(self.out_l), ptr(self.out_r),
                                       ptr(self.res), ptr(self.ws), self.ws.numel(), stream()),
               "vs_batched_intersects")
 
     def replay(self):
         """(left, right, stats) views of the graph's output buffers."""
         self.graph.replay()
-        k, staged, cand, _ = _read_result(self.res)
+        k, staged, cand, cverts = _read_result(self.res)
         if staged > self.cap:
             raise OverflowError(f"{staged} bbox candidates exceed the captured capacity {self.cap}")
-        return self.out_l[:k], self.out_r[:k], JoinStats(k, staged, cand, self.cap, 0, 0)
+        return self.out_l[:k], self.out_r[:k], JoinStats(k, staged, cand, self.cap, 0, cverts)

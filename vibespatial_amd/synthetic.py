@@ -25,7 +25,8 @@ Workloads:
       about centres U[0,1e5]^2 (seed 4); 1M axis-aligned boxes (seed 5)
       centred on fresh draws from the same mixture, width and height
       independent LogNormal(ln 50, 1.0).
-  c5: 4096 polygon x polygon intersects jobs (seed 6): per job
+  c5: 4096 polygon x polygon intersects jobs (job sizes seed 6, rows of
+      each run of 64 jobs seed (6, run)): per job
       L = clip(LogNormal(ln 2000, 1.0), 1e3, 2e5) left footprints (stars with
       4-12 vertices, radius U(2.5, 20) m) and R = clip(LogNormal(ln 300, 1.0),
       10, 5000) right parcels (stars, radius U(40, 70) m, vertices
@@ -83,34 +84,30 @@ def _sorted_angles_spacing(rng, counts: np.ndarray) -> np.ndarray:
     """Same distribution as _sorted_angles without a sort: the order statistics
     of n uniforms are the normalised partial sums of n + 1 exponential
     spacings; the ring's largest angular gap is its largest spacing, the
-    wrap-around gap being E_1 + E_{n+1}.  Rings with a gap >= pi are redrawn."""
+    wrap-around gap being E_1 + E_{n+1}.  Rings with a gap >= pi are redrawn.
+    Rings are drawn in groups of equal vertex count (one (rings, n + 1) array
+    of spacings per group and redraw)."""
     counts = np.asarray(counts, dtype=np.int64)
     off = np.concatenate([[0], np.cumsum(counts)[:-1]])
     ang = np.empty(int(counts.sum()))
-    todo = np.arange(len(counts))
-    while len(todo):
-        c = counts[todo]
-        e = rng.exponential(1.0, int(c.sum() + len(c)))
-        o1 = np.concatenate([[0], np.cumsum(c + 1)[:-1]])  # first spacing of each ring
-        tot = np.add.reduceat(e, o1)
-        cs = np.cumsum(e)
-        seg = np.repeat(np.arange(len(todo)), c + 1)
-        within = np.arange(len(e)) - o1[seg]
-        base = cs[o1] - e[o1]
-        u = (cs - base[seg]) / tot[seg]  # partial sums E_1..E_k over the ring
-        gap = e / tot[seg]
-        lastk = o1 + c  # the (n+1)-th spacing
-        wrap = (e[o1] + e[lastk]) / tot
-        inner = gap.copy()
-        inner[o1] = 0.0
-        inner[lastk] = 0.0
-        ok = np.maximum(np.maximum.reduceat(inner, o1), wrap) < 0.5
-        take = within < c[seg]  # the first n partial sums of each ring
-        dst = np.repeat(off[todo], c)
-        dst = dst + (np.arange(int(c.sum())) - np.repeat(np.concatenate([[0], np.cumsum(c)[:-1]]), c))
-        keep = np.repeat(ok, c)
-        ang[dst[keep]] = 2 * np.pi * u[take][keep]
-        todo = todo[~ok]
+    order = np.argsort(counts, kind="stable")
+    sc = counts[order]
+    starts = np.flatnonzero(np.concatenate([[True], sc[1:] != sc[:-1]])) if len(sc) else np.empty(0, np.int64)
+    ends = np.concatenate([starts[1:], [len(sc)]])
+    for g0, g1 in zip(starts, ends):
+        n = int(sc[g0])
+        todo = order[g0:g1]
+        while len(todo):
+            e = rng.exponential(1.0, (len(todo), n + 1))
+            cs = np.cumsum(e, axis=1)
+            tot = cs[:, n]
+            gap = e[:, 0] + e[:, n]
+            if n > 1:
+                gap = np.maximum(gap, e[:, 1:n].max(axis=1))
+            ok = gap < 0.5 * tot
+            rows = todo[ok]
+            ang[off[rows][:, None] + np.arange(n)] = (2 * np.pi) * (cs[ok, :n] / tot[ok, None])
+            todo = todo[~ok]
     return ang
 
 
@@ -264,30 +261,64 @@ def c4(n_points: int = 50_000_000, n_boxes: int = 1_000_000):
     return Workload("c4", PointArray(x, y), None, "intersects", boxes=boxes)
 
 
+def _c5_chunk(k, jobs, nl, nr, side, ox, oy):
+    """Right parcels and left footprints of jobs[0] .. jobs[-1], drawn from
+    the chunk's own generator (seed (6, k)), so the result does not depend on
+    how chunks are spread over threads."""
+    rng = np.random.default_rng([6, k])
+    jl = np.repeat(jobs, nl[jobs])
+    jr = np.repeat(jobs, nr[jobs])
+    rcx = ox[jr] + rng.uniform(0, 1, len(jr)) * side[jr]
+    rcy = oy[jr] + rng.uniform(0, 1, len(jr)) * side[jr]
+    rR = rng.uniform(40.0, 70.0, len(jr))
+    rnv = np.clip(np.round(rng.lognormal(np.log(40.0), 0.5, len(jr))), 4, 400).astype(np.int64)
+    right = star_polygons(rng, rcx, rcy, rR, rnv, spacing=True)
+    lcx = ox[jl] + rng.uniform(0, 1, len(jl)) * side[jl]
+    lcy = oy[jl] + rng.uniform(0, 1, len(jl)) * side[jl]
+    lR = rng.uniform(2.5, 20.0, len(jl))
+    lnv = rng.integers(4, 13, len(jl))
+    left = star_polygons(rng, lcx, lcy, lR, lnv, spacing=True)
+    return left, right
+
+
+def _concat_polygons(parts) -> PolygonArray:
+    """Row-wise concatenation of PolygonArrays (offsets rebased)."""
+    xs, ys, rings, partso, geoms = [], [], [np.zeros(1, np.int64)], [np.zeros(1, np.int64)], [np.zeros(1, np.int64)]
+    nc = nr_ = npt = 0
+    for P in parts:
+        xs.append(P.x)
+        ys.append(P.y)
+        rings.append(np.asarray(P.ring_offsets[1:], np.int64) + nc)
+        partso.append(np.asarray(P.part_offsets[1:], np.int64) + nr_)
+        geoms.append(np.asarray(P.geom_offsets[1:], np.int64) + npt)
+        nc += len(P.x)
+        nr_ += len(P.ring_offsets) - 1
+        npt += len(P.part_offsets) - 1
+    return PolygonArray(np.concatenate(xs), np.concatenate(ys), np.concatenate(rings), np.concatenate(partso),
+                        np.concatenate(geoms))
+
+
 def c5(n_jobs: int = 4096, left_median: float = 2000.0, right_median: float = 300.0, overlap: bool = False,
-       left_clip=(1_000, 200_000), right_clip=(10, 5_000)):
+       left_clip=(1_000, 200_000), right_clip=(10, 5_000), threads: int | None = None):
     """(left, right, left_job, right_job): PolygonArrays of every job's rows
-    and int64 row offsets per job (n_jobs + 1)."""
+    and int64 row offsets per job (n_jobs + 1).  Job sizes come from seed 6;
+    the rows of each run of 64 jobs from the generator seeded (6, run), built
+    on `threads` threads (default: the usable cores, at most 16)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
     rng = np.random.default_rng(6)
     nl = np.clip(np.round(rng.lognormal(np.log(left_median), 1.0, n_jobs)), *left_clip).astype(np.int64)
     nr = np.clip(np.round(rng.lognormal(np.log(right_median), 1.0, n_jobs)), *right_clip).astype(np.int64)
     side = 120.0 * np.sqrt(nr)
     ox = np.zeros(n_jobs) if overlap else (np.arange(n_jobs) % 64) * 2.0e4
     oy = np.zeros(n_jobs) if overlap else (np.arange(n_jobs) // 64) * 2.0e4
-    jl = np.repeat(np.arange(n_jobs), nl)
-    jr = np.repeat(np.arange(n_jobs), nr)
-    # right parcels
-    rcx = ox[jr] + rng.uniform(0, 1, len(jr)) * side[jr]
-    rcy = oy[jr] + rng.uniform(0, 1, len(jr)) * side[jr]
-    rR = rng.uniform(40.0, 70.0, len(jr))
-    rnv = np.clip(np.round(rng.lognormal(np.log(40.0), 0.5, len(jr))), 4, 400).astype(np.int64)
-    right = star_polygons(rng, rcx, rcy, rR, rnv, spacing=True)
-    # left footprints
-    lcx = ox[jl] + rng.uniform(0, 1, len(jl)) * side[jl]
-    lcy = oy[jl] + rng.uniform(0, 1, len(jl)) * side[jl]
-    lR = rng.uniform(2.5, 20.0, len(jl))
-    lnv = rng.integers(4, 13, len(jl))
-    left = star_polygons(rng, lcx, lcy, lR, lnv, spacing=True)
+    chunks = [np.arange(j, min(j + 64, n_jobs)) for j in range(0, n_jobs, 64)]
+    threads = threads or min(16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        out = list(ex.map(lambda kj: _c5_chunk(kj[0], kj[1], nl, nr, side, ox, oy), enumerate(chunks)))
+    left = _concat_polygons([o[0] for o in out])
+    right = _concat_polygons([o[1] for o in out])
     lj = np.concatenate([[0], np.cumsum(nl)]).astype(np.int64)
     rj = np.concatenate([[0], np.cumsum(nr)]).astype(np.int64)
     return left, right, lj, rj
