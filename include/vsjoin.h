@@ -32,7 +32,8 @@
  *     If result[1] > capacity the outputs are not valid: call again with
  *     capacity >= result[1].
  *   - Thread safety: calls are reentrant; concurrent calls must use distinct
- *     workspaces.
+ *     workspaces (vs_sjoin_points_polygons_noalloc shares one array per GPU:
+ *     its calls must not overlap).
  */
 #ifndef VSJOIN_H
 #define VSJOIN_H
@@ -146,6 +147,30 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
                              const int32_t* item_ids, int32_t predicate, int64_t capacity, int64_t* out_left,
                              int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
                              void* stream);
+
+/* Bytes of the fixed device scratch array behind vs_sjoin_points_polygons_noalloc
+ * (one copy per GPU, held until the process ends). */
+#define VS_NOALLOC_SCRATCH_BYTES (64ll << 20)
+
+/* vs_sjoin_points_polygons without a workspace argument and without any
+ * device allocation: its work memory is a fixed VS_NOALLOC_SCRATCH_BYTES
+ * array in device memory (one per GPU, its address looked up on the first
+ * call on each device).  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * vs_sjoin_points_polygons; in addition VS_ERR_INVALID when
+ * vs_sjoin_points_polygons_workspace_bytes(n_points, n_polys, n_coords,
+ * capacity) exceeds VS_NOALLOC_SCRATCH_BYTES (about 1M points x 10k
+ * polygons with a capacity of 1M pairs fit).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller.  The caller still
+ * provides the tree (vs_index_build), outputs and d_result. */
+int vs_sjoin_points_polygons_noalloc(const double* px, const double* py, int64_t n_points, const double* x,
+                                     const double* y, const int32_t* geom_offsets, const int32_t* part_offsets,
+                                     const int32_t* ring_offsets, int64_t n_polys, int64_t n_parts, int64_t n_rings,
+                                     int64_t n_coords, const vs_tree_layout* layout, const double* node_boxes,
+                                     const int32_t* item_ids, int32_t predicate, int64_t capacity,
+                                     int64_t* out_left, int64_t* out_right, int64_t* d_result, void* stream);
 
 int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity);
 

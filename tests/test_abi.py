@@ -172,3 +172,24 @@ def test_host_cover_ranges_hold_every_box_point(key_bytes):
         inside += int(inbox.sum())
     print(f"cover: {scanned} positions scanned for {inside} points in the boxes ({scanned / max(inside, 1):.3f})")
     assert scanned >= inside
+
+
+def test_noalloc_scratch_limit_and_rejection_without_device_work():
+    text = open(os.path.join(ROOT, "include", "vsjoin.h")).read()
+    m = re.search(r"#define VS_NOALLOC_SCRATCH_BYTES \((\d+)ll << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == _lib.NOALLOC_SCRATCH_BYTES
+    L = _lib.lib()
+    # 10M points x 100k polygons needs ~535 MiB of work memory: refused by the
+    # size check, which runs before any device call (null pointers are never read)
+    n, m_, v, cap = 10_000_000, 100_000, 5_000_000, 10_000_000
+    assert L.vs_sjoin_points_polygons_workspace_bytes(n, m_, v, cap) > _lib.NOALLOC_SCRATCH_BYTES
+    lay = _lib.TreeLayout()
+    _lib.check(L.vs_tree_plan(m_, 16, ctypes.byref(lay)), "vs_tree_plan")
+    res = L.vs_sjoin_points_polygons_noalloc(None, None, n, None, None, None, None, None, m_, m_, m_, v,
+                                             ctypes.byref(lay), None, None, _lib.PRED["within"], cap, None, None,
+                                             None, None)
+    assert res == -1
+    msg = _lib.last_error()
+    assert "scratch" in msg and str(_lib.NOALLOC_SCRATCH_BYTES) in msg
+    assert L.vs_sjoin_points_polygons_noalloc(None, None, -1, None, None, None, None, None, 0, 0, 0, 0,
+                                              ctypes.byref(lay), None, None, 0, 0, None, None, None, None) == -1

@@ -19,6 +19,7 @@ __all__ = ["lib", "check", "VsjoinError", "TreeLayout", "LIB_PATH", "PRED"]
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libvsjoin.so")
 
 MAX_LEVELS = 16
+NOALLOC_SCRATCH_BYTES = 64 << 20  # VS_NOALLOC_SCRATCH_BYTES
 
 PRED = {
     "within": 0,
@@ -70,6 +71,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _LP, _p, _p, _i32, _i64, _p, _p, _p, _p, _i64,
          _p],
+    ),
+    "vs_sjoin_points_polygons_noalloc": (
+        ctypes.c_int,
+        [_p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _LP, _p, _p, _i32, _i64, _p, _p, _p, _p],
     ),
     "vs_query_boxes_workspace_bytes": (_i64, [_i64, _i64]),
     "vs_query_boxes": (ctypes.c_int, [_p, _i64, _LP, _p, _p, _i64, _i32, _p, _p, _p, _p, _i64, _p]),

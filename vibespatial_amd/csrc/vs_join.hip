@@ -31,6 +31,8 @@
 // Waves are handed out from 8 queues, one per XCD, each over a contiguous
 // range of the Hilbert order, so an XCD's L2 sees a spatially compact slice;
 // an idle XCD steals from the others.  Every wave exits when all 8 are empty.
+#include <mutex>
+
 #include "vs_exact.h"
 #include "vs_internal.h"
 #include "vs_join_common.h"
@@ -498,6 +500,38 @@ int timing_end_public(hipStream_t s);
 
 }  // namespace vs
 
+// Work memory of vs_sjoin_points_polygons_noalloc: a fixed array in device
+// memory, one copy per GPU, held from module load to process end.  Host code
+// never passes `scratch` itself to a kernel (on the host it names a stand-in,
+// not the device address): scratch_ptr() looks its address up once per
+// device, with that device current, and every later use goes through that
+// pointer.
+static __device__ __align__(256) unsigned char scratch[VS_NOALLOC_SCRATCH_BYTES];
+
+namespace vs {
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kScratchDevices] = {};
+
+int scratch_ptr(void** out) {
+    int dev = 0;
+    VS_CHECK_HIP(hipGetDevice(&dev));
+    VS_REQUIRE(dev >= 0 && dev < kScratchDevices, "scratch: device %d beyond the %d supported", dev,
+               kScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        VS_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        VS_REQUIRE(p != nullptr, "scratch: no device address on device %d", dev);
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return VS_OK;
+}
+
+}  // namespace vs
+
 extern "C" {
 
 int vs_set_kernel_timing(int32_t enable) {
@@ -572,6 +606,28 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
         VS_TRY(vs::launch_traverse<vs::kModePip>(predicate, J.sx, J.sy, nullptr, J.perm, n_points, T, P, J,
                                                  capacity, s));
     return vs::finish_pairs(J.fin_ws, J.ctl, n_points, capacity, true, out_left, out_right, d_result, s);
+}
+
+int vs_sjoin_points_polygons_noalloc(const double* px, const double* py, int64_t n_points, const double* x,
+                                     const double* y, const int32_t* geom_offsets, const int32_t* part_offsets,
+                                     const int32_t* ring_offsets, int64_t n_polys, int64_t n_parts, int64_t n_rings,
+                                     int64_t n_coords, const vs_tree_layout* layout, const double* node_boxes,
+                                     const int32_t* item_ids, int32_t predicate, int64_t capacity,
+                                     int64_t* out_left, int64_t* out_right, int64_t* d_result, void* stream) {
+    vs::clear_error();
+    VS_REQUIRE(n_points >= 0 && n_polys >= 0 && n_coords >= 0 && capacity >= 0,
+               "vs_sjoin_points_polygons_noalloc: negative size");
+    const int64_t need = vs_sjoin_points_polygons_workspace_bytes(n_points, n_polys, n_coords, capacity);
+    VS_REQUIRE(need <= VS_NOALLOC_SCRATCH_BYTES,
+               "vs_sjoin_points_polygons_noalloc: %lld points x %lld polygons (%lld coordinates, capacity %lld) "
+               "need %lld bytes of scratch, above the fixed %lld",
+               (long long)n_points, (long long)n_polys, (long long)n_coords, (long long)capacity, (long long)need,
+               (long long)VS_NOALLOC_SCRATCH_BYTES);
+    void* ws = nullptr;
+    VS_TRY(vs::scratch_ptr(&ws));
+    return vs_sjoin_points_polygons(px, py, n_points, x, y, geom_offsets, part_offsets, ring_offsets, n_polys,
+                                    n_parts, n_rings, n_coords, layout, node_boxes, item_ids, predicate, capacity,
+                                    out_left, out_right, d_result, ws, VS_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity) {
