@@ -260,6 +260,10 @@ def main(argv=None):
             "step_frac": alg_l / (ms_l * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "dram_frac": None if traffic is None else traffic / (kms_l * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic_source": None if traffic is None else _TRAFFIC_SOURCE,
+            "note": "frac: the byte model's bytes over the kernel's time; the model charges every bbox candidate "
+                    "for streaming its polygon's edges (8 + 8 V_q), which the kernel reads once per polygon "
+                    "into LDS, so frac can pass 1; step_frac: the same bytes over the whole step; dram_frac: "
+                    "the measured DRAM bytes over the kernel's time",
         },
         "cpu_baseline": cpu,
     }
@@ -271,11 +275,11 @@ def main(argv=None):
 # HBM bytes per learned_join_kernel launch on the default workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch, mean of the
 # warm-up and the timed dispatch) scaled by what the same counters read for a
-# 1 GiB device-to-device copy in the same run (FETCH_SIZE 524,318 KiB -> x2.0,
-# WRITE_SIZE 1,048,576 KiB -> x1.0):
-#   (2.0 * 415,112.5 + 1.0 * 294,977.65) KiB = 1,152,207,514 bytes.
+# 1 GiB device-to-device copy in the same run (FETCH_SIZE 524,319.7 KiB ->
+# x1.99988, WRITE_SIZE 1,048,576 KiB -> x1.0):
+#   (1.99988 * 329,035.8 + 1.0 * 81,614.2) KiB = 757,397,518 bytes.
 # The summary with both commands is profiles/learned_join_kernel_10Mpts_100kpolys_pmc_traffic.md.
-_TRAFFIC_BYTES = {(10_000_000, 100_000): 1_152_207_514}
+_TRAFFIC_BYTES = {(10_000_000, 100_000): 757_397_518}
 
 
 _TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of learned_join_kernel on this workload, scaled by a "
@@ -558,7 +562,7 @@ def run_c5(args):
             "pairs": st.pairs,
             "bbox_candidates": st.candidates,
             "tile": list(best),
-            "autotune_ms": {f"{k[0]}x{k[1]}": v for k, v in times.items()},
+            "autotune_ms": {f"{k[0]}x{k[1]}/order{k[2]}": v for k, v in times.items()},
             "traverse_refine_kernel_ms": statistics.mean(kms),
             "graph_replay_ms_per_step": ms_graph,
         },

@@ -34,6 +34,14 @@
  *   - Thread safety: calls are reentrant; concurrent calls must use distinct
  *     workspaces (vs_sjoin_points_polygons_noalloc shares one array per GPU:
  *     its calls must not overlap).
+ *   - Side stream: vs_learned_index_build and vs_sjoin_points_polygons_learned
+ *     overlap the network's training and the error pass with the coordinate
+ *     gather on a second stream (non-blocking, default priority) with a
+ *     fork / join event pair.  There is one such stream per host thread and
+ *     device, created on that thread's first such call on the device and
+ *     kept until the process ends; before returning, the call makes the
+ *     caller's stream wait for it, so the work stays ordered on the caller's
+ *     stream.  Other entry points use only the caller's stream.
  */
 #ifndef VSJOIN_H
 #define VSJOIN_H
@@ -372,7 +380,9 @@ int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, in
  * the length of bx / by (sizes the y-slab index of the right rows).  Output:
  * global row pairs lexsorted, so each job's pairs are contiguous and lexsorted.
  * tile: bits 0-7 traversal waves per block (1-4, 0 = 4), bits 8-23 refine
- * block size (multiple of 64 up to 256, 0 = 256).  d_result (device int64[4]):
+ * block size (multiple of 64 up to 256, 0 = 256), bit 24 candidate order
+ * of the refine (0: by right row; 1: by right rows' coordinate-count bins,
+ * largest first, then right row).  Every tile gives the same pairs.  d_result (device int64[4]):
  * [0] pairs, [1] bbox candidates staged (> capacity: retry with that
  * capacity), [2] bbox candidates, [3] sum over the candidates of both rows'
  * coordinate counts.  Replaces, per job, the reference's

@@ -132,6 +132,27 @@ def test_workspace_queries_are_positive_and_monotone():
     assert L.vs_sort_workspace_bytes(10**6, 4) < L.vs_sort_workspace_bytes(10**6, 8)
 
 
+def test_noalloc_scratch_limit_and_rejection_without_device_work():
+    text = open(os.path.join(ROOT, "include", "vsjoin.h")).read()
+    m = re.search(r"#define VS_NOALLOC_SCRATCH_BYTES \((\d+)ll << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == _lib.NOALLOC_SCRATCH_BYTES
+    L = _lib.lib()
+    # 10M points x 100k polygons needs ~535 MiB of work memory: refused by the
+    # size check, which runs before any device call (null pointers are never read)
+    n, m_, v, cap = 10_000_000, 100_000, 5_000_000, 10_000_000
+    assert L.vs_sjoin_points_polygons_workspace_bytes(n, m_, v, cap) > _lib.NOALLOC_SCRATCH_BYTES
+    lay = _lib.TreeLayout()
+    _lib.check(L.vs_tree_plan(m_, 16, ctypes.byref(lay)), "vs_tree_plan")
+    res = L.vs_sjoin_points_polygons_noalloc(None, None, n, None, None, None, None, None, m_, m_, m_, v,
+                                             ctypes.byref(lay), None, None, _lib.PRED["within"], cap, None, None,
+                                             None, None)
+    assert res == -1
+    msg = _lib.last_error()
+    assert "scratch" in msg and str(_lib.NOALLOC_SCRATCH_BYTES) in msg
+    assert L.vs_sjoin_points_polygons_noalloc(None, None, -1, None, None, None, None, None, 0, 0, 0, 0,
+                                              ctypes.byref(lay), None, None, 0, 0, None, None, None, None) == -1
+
+
 @pytest.mark.parametrize("key_bytes", [4, 8])
 def test_host_cover_ranges_hold_every_box_point(key_bytes):
     """The learned index's box cover: at most 16 sorted, disjoint key ranges
@@ -172,24 +193,43 @@ def test_host_cover_ranges_hold_every_box_point(key_bytes):
         inside += int(inbox.sum())
     print(f"cover: {scanned} positions scanned for {inside} points in the boxes ({scanned / max(inside, 1):.3f})")
     assert scanned >= inside
+    # degenerate boxes: a point box on a data point, zero width / height, the
+    # whole extent, beyond the extent, edges on grid-cell boundaries, inverted
+    cell = (ext[2] - ext[0]) / (2 ** (12 if key_bytes == 4 else 32))
+    special = [(px[7], py[7], px[7], py[7]), (px[9], 0.0, px[9], 1e5), (0.0, py[11], 1e5, py[11]),
+               (-1.0, -1.0, 1e5 + 1, 1e5 + 1), (2e5, 2e5, 3e5, 3e5), (-5e4, -5e4, -1.0, -1.0),
+               (64 * cell, 64 * cell, 128 * cell, 96 * cell), (0.0, 0.0, 0.0, 0.0), (1e5, 1e5, 1e5, 1e5)]
+    for x0, y0, x1, y1 in special:
+        m = L.vs_host_cover_ranges(x0, y0, x1, y1, ext.ctypes.data_as(ctypes.c_void_p), key_bytes,
+                                   out.ctypes.data_as(ctypes.c_void_p))
+        assert 1 <= m <= 16
+        lo, hi = out[0:2 * m:2], out[1:2 * m:2]
+        assert np.all(lo <= hi) and np.all(hi[:-1] < lo[1:])
+        inbox = (px >= x0) & (px <= x1) & (py >= y0) & (py <= y1)
+        k = keys[inbox]
+        j = np.searchsorted(lo, k, side="right") - 1
+        assert np.all(j >= 0) and np.all(k <= hi[j]), (x0, y0, x1, y1)
+    assert L.vs_host_cover_ranges(5.0, 5.0, 4.0, 6.0, ext.ctypes.data_as(ctypes.c_void_p), key_bytes,
+                                  out.ctypes.data_as(ctypes.c_void_p)) == 0  # inverted: empty
+    assert L.vs_host_cover_ranges(np.nan, 5.0, 4.0, 6.0, ext.ctypes.data_as(ctypes.c_void_p), key_bytes,
+                                  out.ctypes.data_as(ctypes.c_void_p)) == 0  # NaN: empty
 
 
-def test_noalloc_scratch_limit_and_rejection_without_device_work():
-    text = open(os.path.join(ROOT, "include", "vsjoin.h")).read()
-    m = re.search(r"#define VS_NOALLOC_SCRATCH_BYTES \((\d+)ll << (\d+)\)", text)
-    assert m and int(m.group(1)) << int(m.group(2)) == _lib.NOALLOC_SCRATCH_BYTES
+def test_hilbert_children_digits_diagonal_rule():
+    """The four children of a Hilbert cell are visited around the 2 x 2
+    square: a child's diagonal partner has its last digit xor 2 (the rule the
+    box covers use to get all four digits from two encodings)."""
     L = _lib.lib()
-    # 10M points x 100k polygons needs ~535 MiB of work memory: refused by the
-    # size check, which runs before any device call (null pointers are never read)
-    n, m_, v, cap = 10_000_000, 100_000, 5_000_000, 10_000_000
-    assert L.vs_sjoin_points_polygons_workspace_bytes(n, m_, v, cap) > _lib.NOALLOC_SCRATCH_BYTES
-    lay = _lib.TreeLayout()
-    _lib.check(L.vs_tree_plan(m_, 16, ctypes.byref(lay)), "vs_tree_plan")
-    res = L.vs_sjoin_points_polygons_noalloc(None, None, n, None, None, None, None, None, m_, m_, m_, v,
-                                             ctypes.byref(lay), None, None, _lib.PRED["within"], cap, None, None,
-                                             None, None)
-    assert res == -1
-    msg = _lib.last_error()
-    assert "scratch" in msg and str(_lib.NOALLOC_SCRATCH_BYTES) in msg
-    assert L.vs_sjoin_points_polygons_noalloc(None, None, -1, None, None, None, None, None, 0, 0, 0, 0,
-                                              ctypes.byref(lay), None, None, 0, 0, None, None, None, None) == -1
+    rng = np.random.default_rng(40)
+    for lev in (2, 5, 12, 20, 31):
+        for _ in range(300):
+            px = int(rng.integers(0, 2 ** (lev - 1))) * 2
+            py = int(rng.integers(0, 2 ** (lev - 1))) * 2
+            sh = 32 - lev
+
+            def digit(x, y):
+                return (L.vs_host_hilbert64(x << sh, y << sh) >> (64 - 2 * lev)) & 3
+
+            d = {(i, j): digit(px + i, py + j) for i in (0, 1) for j in (0, 1)}
+            assert sorted(d.values()) == [0, 1, 2, 3]
+            assert d[(1, 1)] == d[(0, 0)] ^ 2 and d[(0, 1)] == d[(1, 0)] ^ 2

@@ -45,13 +45,39 @@ def _small_batch(n_jobs=48):
                 right_clip=(1, 400))
 
 
-def test_batched_matches_oracle_per_job_with_overlapping_jobs():
-    print("batched 48 overlapping jobs", flush=True)
+@pytest.mark.parametrize("tile", [(4, 256, 0), (4, 256, 1)])
+def test_batched_matches_oracle_per_job_with_overlapping_jobs(tile):
+    print(f"batched 48 overlapping jobs, tile {tile}", flush=True)
     left, right, lj, rj = _small_batch()
-    l, r, st = _run(left, right, lj, rj)
+    l, r, st = _run(left, right, lj, rj, tile=tile)
     el, er = _expected(left, right, lj, rj)
     assert np.array_equal(l, el) and np.array_equal(r, er)
     assert st.candidates >= st.pairs > 0
+
+
+def test_batched_refine_on_holes_multiparts_and_long_rings():
+    """General rows in the batched refine: left rows with holes and several
+    parts, right rings longer than 64 edges (their y-slab index), left rings
+    longer than the LDS tile, against the oracle in both candidate orders."""
+    from vibespatial_amd.geometry import MultiPolygon, PolygonArray
+
+    rng = np.random.default_rng(12)
+    m = 60
+    holes = rng.integers(0, 2, m)
+    nh = int(holes.sum())
+    A = S.star_polygons(rng, rng.uniform(0, 200, m), rng.uniform(0, 200, m), rng.uniform(3, 15, m),
+                        rng.integers(4, 90, m), holes=holes, hole_vertices=rng.integers(4, 8, nh),
+                        hole_frac=rng.uniform(0.1, 0.3, nh))
+    B = S.star_polygons(rng, rng.uniform(0, 200, 40), rng.uniform(0, 200, 40), rng.uniform(10, 40, 40),
+                        rng.integers(4, 300, 40))
+    extra = PolygonArray.from_geometries([MultiPolygon([A.scalar(0), A.scalar(1)]), A.scalar(2)])
+    A2 = PolygonArray.from_geometries([A.scalar(i) for i in range(m)] + [extra.scalar(0), extra.scalar(1)])
+    lj = np.array([0, 30, len(A2)], dtype=np.int64)
+    rj = np.array([0, 25, 40], dtype=np.int64)
+    el, er = _expected(A2, B, lj, rj)
+    for tile in [(4, 256, 0), (2, 128, 1)]:
+        l, r, _ = _run(A2, B, lj, rj, tile=tile)
+        assert np.array_equal(l, el) and np.array_equal(r, er), tile
 
 
 def test_batched_edge_cases_empty_and_single_row_jobs():

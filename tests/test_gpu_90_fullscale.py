@@ -107,7 +107,10 @@ def test_c5_all_jobs_bit_exact():
     print("c5 4096 jobs", flush=True)
     left, right, lj, rj = S.c5()
     A, B = DevicePolygons.from_host(left), DevicePolygons.from_host(right)
-    l, r, st = J.batched_intersects(A, B, torch.from_numpy(lj).cuda(), torch.from_numpy(rj).cuda())
+    tl, tr = torch.from_numpy(lj).cuda(), torch.from_numpy(rj).cuda()
+    l, r, st = J.batched_intersects(A, B, tl, tr, tile=(4, 256, 1))
+    l0, r0, _ = J.batched_intersects(A, B, tl, tr, tile=(4, 256, 0))  # candidates by right row only
+    assert torch.equal(l0, l) and torch.equal(r0, r)
     l, r = l.cpu().numpy(), r.cpu().numpy()
     # the oracle over all rows at once, restricted to pairs inside one job,
     # is every job's own join (a pair's rows belong to exactly one job each)
@@ -151,7 +154,8 @@ def test_c4_full_size_learned_equals_tree():
     acc_l = [torch.zeros(nb, dtype=torch.int64, device="cuda") for _ in range(3)]
     hits_l = 0
     sample = np.sort(np.random.default_rng(2).choice(nb, 200, replace=False))
-    sample_t = torch.from_numpy(sample).cuda()
+    mark = torch.zeros(nb, dtype=torch.bool, device="cuda")
+    mark[torch.from_numpy(sample).cuda()] = True
     sampled = {}
     for i in range(4):
         b0, b1 = int(edges[i]), int(edges[i + 1])
@@ -160,7 +164,7 @@ def test_c4_full_size_learned_equals_tree():
         hits_l += st.pairs
         for a, v in zip(acc_l, _box_checksums(l, r, nb)):
             a += v
-        m = torch.isin(l, sample_t)
+        m = mark[l]
         ln, rn = l[m].cpu().numpy(), r[m].cpu().numpy()
         for bx in np.unique(ln):
             sampled[int(bx)] = rn[ln == bx]

@@ -263,19 +263,20 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
 
 // ---------------------------------------------------------------------------
 // Y-slab index of the right rows.  A right row of one ring with at least
-// kRowSlabMinEdges edges (and fewer than 2^16 coordinates) gets its y-range
+// kRowSlabMinEdges edges (and at most 2^15 coordinates) gets its y-range
 // [ylo, yhi] cut into B equal slabs (B a power of two, about one per four
 // edges, at most kRowSlabMax); edge e = (c0 + e - 1, c0 + e) is listed in
 // every slab from slab(min y) to slab(max y).  slab() is monotone in y, so a
 // point finds in slab(py) every edge whose closed y-range holds py -- every
 // edge that can toggle its crossing parity or put it on the boundary -- and a
 // box [y0, y1] finds in slabs slab(y0)..slab(y1) every edge whose y-range
-// meets it.  Row r's list starts at 17 r + 2 (first coordinate of r): B + 1
-// slab starts, then at most 2 (n - 1) edge numbers (a row that would need
-// more takes one slab holding every edge).
+// meets it.  Row r's list is 3 n entries from 3 c0 (its first coordinate c0,
+// n coordinates; single-ring rows own disjoint coordinate ranges, so disjoint
+// lists): B + 1 slab starts, then at most 2 (n - 1) edge numbers (a row that
+// would need more takes one slab holding every edge).
 // ---------------------------------------------------------------------------
-constexpr int kRowSlabMinEdges = 16;
-constexpr int kRowSlabMax = 16;
+constexpr int kRowSlabMinEdges = 64;  // smaller rows are streamed whole (four edges in flight)
+constexpr int kRowSlabMax = 64;
 
 struct RowSlabs {
     const double2* geo;    // (ylo, slabs per unit of y)
@@ -289,7 +290,7 @@ __device__ __forceinline__ int row_slab(double y, double ylo, double inv, int nb
     return (int)f;
 }
 
-__device__ __forceinline__ int64_t row_slab_base(int64_t r, int32_t c0) { return 17 * r + 2 * (int64_t)c0; }
+__device__ __forceinline__ int64_t row_slab_base(int32_t c0) { return 3 * (int64_t)c0; }
 
 // One 64-thread block per row (grid-stride): counts per slab, starts, entries.
 __global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_rows, const double4* __restrict__ rbox,
@@ -300,9 +301,9 @@ __global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_r
     const int lane = threadIdx.x;
     for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
         int32_t c0 = 0, n = 0;
-        // at most 2 (n - 1) + 17 entries from 17 r + 2 c0 on, every value below 2^16
+        // B + 1 + 2 (n - 1) <= 3 n entries from 3 c0 on, every value below 2^16
         const bool ok = single_ring_of(B, r, c0, n) && n - 1 >= kRowSlabMinEdges && n <= 32768 && c0 >= 0 &&
-                        row_slab_base(r, c0) + 17 + 2 * (int64_t)n <= list_cap;
+                        row_slab_base(c0) + 3 * (int64_t)n <= list_cap;
         if (!ok) {
             if (lane == 0) nslab[r] = 0;
             continue;  // block-uniform
@@ -316,8 +317,9 @@ __global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_r
             nb = 1;
             inv = 0.0;
         }
-        uint16_t* L = list + row_slab_base(r, c0);
-        if (lane <= kRowSlabMax) cnt[lane] = 0;
+        uint16_t* L = list + row_slab_base(c0);
+        if (lane < kRowSlabMax) cnt[lane + 1] = 0;
+        if (lane == 0) cnt[0] = 0;
         __syncthreads();
         int total = 0;
         for (int e = 1 + lane; e <= ne; e += 64) {
@@ -377,25 +379,56 @@ __device__ inline bool ring_intersects_slabbed(const RingRef& A, int32_t na, con
         mxy = fmax(mxy, A.Y(k));
     }
     const uint16_t* body = L + nb + 1;
+    // Slab entries four at a time: the four edge numbers, then the four
+    // edges' coordinates, are loaded together (memory-level parallelism).
+    constexpr int kU = 4;
     {
         const double fx = A.X(0), fy = A.Y(0);
         const int b = row_slab(fy, g.x, g.y, nb);
         int parity = 0;
-        const int k1 = L[b + 1];
-        for (int k = L[b]; k < k1; ++k) {
-            const int32_t c = c0b + body[k];
-            if (edge_step_f64(fx, fy, bx[c - 1], by[c - 1], bx[c], by[c], parity)) return true;
+        const int k0 = L[b], k1 = L[b + 1];
+        for (int k = k0; k < k1; k += kU) {
+            int32_t c[kU];
+            double ax[kU], ay[kU], qx[kU], qy[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) c[u] = c0b + body[k + u < k1 ? k + u : k1 - 1];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                ax[u] = bx[c[u] - 1];
+                ay[u] = by[c[u] - 1];
+                qx[u] = bx[c[u]];
+                qy[u] = by[c[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (k + u >= k1) break;
+                if (edge_step_f64(fx, fy, ax[u], ay[u], qx[u], qy[u], parity)) return true;
+            }
         }
         if (parity) return true;
     }
     {
         const int k0 = L[row_slab(mny, g.x, g.y, nb)], k1 = L[row_slab(mxy, g.x, g.y, nb) + 1];
-        for (int k = k0; k < k1; ++k) {
-            const int32_t c = c0b + body[k];
-            const double px = bx[c - 1], py = by[c - 1], qx = bx[c], qy = by[c];
-            if (fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy) {
-                for (int32_t j = 1; j < na; ++j)
-                    if (segments_touch(A.X(j - 1), A.Y(j - 1), A.X(j), A.Y(j), px, py, qx, qy)) return true;
+        for (int k = k0; k < k1; k += kU) {
+            int32_t c[kU];
+            double ax[kU], ay[kU], qx[kU], qy[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) c[u] = c0b + body[k + u < k1 ? k + u : k1 - 1];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                ax[u] = bx[c[u] - 1];
+                ay[u] = by[c[u] - 1];
+                qx[u] = bx[c[u]];
+                qy[u] = by[c[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (k + u >= k1) break;
+                const double px = ax[u], py = ay[u], ex = qx[u], ey = qy[u];
+                if (fmax(px, ex) >= mnx && fmin(px, ex) <= mxx && fmax(py, ey) >= mny && fmin(py, ey) <= mxy) {
+                    for (int32_t j = 1; j < na; ++j)
+                        if (segments_touch(A.X(j - 1), A.Y(j - 1), A.X(j), A.Y(j), px, py, ex, ey)) return true;
+                }
             }
         }
     }
@@ -412,6 +445,23 @@ __device__ __forceinline__ int32_t row_coords(const Polys& P, int64_t r) {
     return P.ring[P.part[P.geom[r + 1]]] - P.ring[P.part[P.geom[r]]];
 }
 
+// Candidate order by work: key = (15 - min(15, floor(log2 coords of the
+// right row))) << rbits | right row, so a sort on it puts the candidates of
+// the largest right rows first and keeps each right row's candidates
+// together; the 64 lanes of a wave then refine rows of similar edge counts.
+__global__ __launch_bounds__(256) void cand_cost_keys_kernel(const int32_t* __restrict__ cand_r,
+                                                            const JoinCtl* __restrict__ cctl, int64_t ccap, Polys B,
+                                                            int rbits, uint32_t* __restrict__ key) {
+    const unsigned long long staged = cctl->staged;
+    const int64_t n = staged < (unsigned long long)ccap ? (int64_t)staged : ccap;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int32_t r = cand_r[i];
+        const int32_t v = row_coords(B, r);
+        const int lg = v > 1 ? 31 - __clz(v) : 0;
+        key[i] = ((uint32_t)(15 - (lg < 15 ? lg : 15)) << rbits) | (uint32_t)r;
+    }
+}
+
 // One thread per candidate: exact intersects; survivors appended to the
 // finish stage with one atomic per wave.  ctl->stat[1] accumulates, over the
 // candidates, the coordinates of both rows.
@@ -423,7 +473,7 @@ __device__ __forceinline__ int32_t row_coords(const Polys& P, int64_t r) {
 constexpr int kRingTile = 13;
 constexpr int64_t kRingTileBytesPerWave = 2 * kRingTile * 64 * (int64_t)sizeof(double);
 __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __restrict__ cand_l,
-                                                          const int32_t* __restrict__ cand_r,
+                                                          const int32_t* __restrict__ cand_r, uint32_t rmask,
                                                           const JoinCtl* __restrict__ cctl, int64_t ccap, Polys A,
                                                           Polys B, RowSlabs RS, uint32_t* __restrict__ out_l,
                                                           int32_t* __restrict__ out_r, int64_t capacity,
@@ -444,7 +494,7 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
         int32_t r = 0;
         if (i < (int64_t)nc) {
             l = cand_l[i];
-            r = cand_r[i];
+            r = (int32_t)((uint32_t)cand_r[i] & rmask);
             cverts += (unsigned long long)(row_coords(A, l) + row_coords(B, r));
             int32_t ca = 0, na = 0, cb = 0, nb = 0;
             const int nslab = RS.nslab[r];
@@ -457,7 +507,7 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
                 if (nslab > 0) {  // one ring of more than kRingTile coordinates, indexed
                     single_ring_of(B, r, cb, nb);
                     keep = ring_intersects_slabbed(RingRef{tx, ty, 64}, na, B.x, B.y, cb,
-                                                   RS.list + row_slab_base(r, cb), nslab, RS.geo[r]);
+                                                   RS.list + row_slab_base(cb), nslab, RS.geo[r]);
                 } else {
                     keep = ring_intersects_polys(RingRef{tx, ty, 64}, na, B, r);
                 }
@@ -514,8 +564,9 @@ struct BatchWs {
     double4* fbox;
     uint32_t* cand_l;
     int32_t* cand_r;
-    uint32_t* cs_r;  // candidates sorted by right row
+    uint32_t* cs_r;  // candidates sorted by right row (or by work bin, then right row)
     int32_t* cs_l;
+    uint32_t* ckey;  // work-bin keys
     void* csort_ws;
     JoinCtl* cctl;
     JoinCtl* ctl;
@@ -558,6 +609,7 @@ static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n
     W.cand_r = c.take<int32_t>(capacity);
     W.cs_r = c.take<uint32_t>(capacity);
     W.cs_l = c.take<int32_t>(capacity);
+    W.ckey = c.take<uint32_t>(capacity);
     W.csort_ws = c.take<char>(sort_workspace_bytes_u32(capacity));
     W.cctl = c.take<JoinCtl>(1);
     W.ctl = c.take<JoinCtl>(1);
@@ -567,7 +619,7 @@ static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n
     if (base) finish_stage(W.fin_ws, capacity, &W.stage_l, &W.stage_r);
     W.slab_geo = c.take<double2>(n_right);
     W.slab_n = c.take<int32_t>(n_right);
-    W.slab_list = c.take<uint16_t>(17 * n_right + 2 * n_right_coords);
+    W.slab_list = c.take<uint16_t>(3 * n_right_coords);
     if (bytes) *bytes = c.bytes;
     return W;
 }
@@ -586,6 +638,8 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
     ctl_reset(W.ctl, s);
     const int waves_per_block = (tile & 0xFF) ? (tile & 0xFF) : 4;
     const int refine_threads = ((tile >> 8) & 0xFFFF) ? ((tile >> 8) & 0xFFFF) : 256;
+    const int rbits = bits_for_jobs(n_right);
+    const bool by_work = ((tile >> 24) & 1) != 0 && rbits <= 28;
     VS_REQUIRE(waves_per_block >= 1 && waves_per_block <= 4, "batched join: tile waves per block in [1, 4]");
     VS_REQUIRE(refine_threads % 64 == 0 && refine_threads <= 256, "batched join: refine block in {64,..,256}");
     int dev = 0, cus = 256;
@@ -610,7 +664,7 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         VS_TRY(exclusive_scan_i32(W.waves, W.wave_off, n_jobs, W.total_waves, W.scan_ws, s));
         hipLaunchKernelGGL(row_slab_build_kernel, dim3((unsigned)std::min<int64_t>(n_right, (int64_t)cus * 32)),
                            dim3(64), 0, s, B, n_right, W.rbox, W.slab_geo, W.slab_n, W.slab_list,
-                           17 * n_right + 2 * n_right_coords);
+                           3 * n_right_coords);
         VS_LAUNCH_CHECK();
         hipLaunchKernelGGL(forest_build_kernel, dim3((unsigned)std::min<int64_t>(n_jobs, 65535)), dim3(256), 0, s,
                            W.rbox, W.rperm, right_job, W.tree_off, n_jobs, W.fbox);
@@ -627,17 +681,27 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
                            W.rperm, W.cand_l, W.cand_r, capacity, W.cctl);
         VS_LAUNCH_CHECK();
         // candidates grouped by right row: the lanes of a wave then share
-        // parcels (same coordinates, same edge count)
-        VS_TRY(sort_pairs_into_u32(reinterpret_cast<const uint32_t*>(W.cand_r),
-                                   reinterpret_cast<const int32_t*>(W.cand_l), W.cs_r, W.cs_l, capacity, 0,
-                                   bits_for_jobs(n_right), W.csort_ws, sort_workspace_bytes_u32(capacity), s,
-                                   &W.cctl->staged));
+        // parcels (same coordinates, same edge count); with the work order,
+        // the largest parcels' candidates first
+        if (by_work) {
+            hipLaunchKernelGGL(cand_cost_keys_kernel, dim3(stride_grid(capacity, 256, 8)), dim3(256), 0, s, W.cand_r,
+                               W.cctl, capacity, B, rbits, W.ckey);
+            VS_LAUNCH_CHECK();
+            VS_TRY(sort_pairs_into_u32(W.ckey, reinterpret_cast<const int32_t*>(W.cand_l), W.cs_r, W.cs_l, capacity,
+                                       0, rbits + 4, W.csort_ws, sort_workspace_bytes_u32(capacity), s,
+                                       &W.cctl->staged));
+        } else {
+            VS_TRY(sort_pairs_into_u32(reinterpret_cast<const uint32_t*>(W.cand_r),
+                                       reinterpret_cast<const int32_t*>(W.cand_l), W.cs_r, W.cs_l, capacity, 0, rbits,
+                                       W.csort_ws, sort_workspace_bytes_u32(capacity), s, &W.cctl->staged));
+        }
+        const uint32_t rmask = by_work ? (uint32_t)((1ull << rbits) - 1ull) : 0xFFFFFFFFu;
         const unsigned rgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, refine_threads),
                                                                                   (int64_t)cus * 16));
         hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads),
                            (unsigned)(refine_threads / 64 * kRingTileBytesPerWave), s,
                            reinterpret_cast<const uint32_t*>(W.cs_l), reinterpret_cast<const int32_t*>(W.cs_r),
-                           W.cctl, capacity, A, B, RowSlabs{W.slab_geo, W.slab_n, W.slab_list}, W.stage_l,
+                           rmask, W.cctl, capacity, A, B, RowSlabs{W.slab_geo, W.slab_n, W.slab_list}, W.stage_l,
                            W.stage_r, capacity, W.ctl);
         VS_LAUNCH_CHECK();
         VS_TRY(timing_end_public(s));

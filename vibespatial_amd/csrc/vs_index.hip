@@ -248,11 +248,13 @@ __global__ __launch_bounds__(256) void hilbert_keys_points_kernel(const double* 
                                                                  int64_t n,
                                                                  const double4* __restrict__ ext,
                                                                  int bits, int zero_bits, K* __restrict__ keys,
-                                                                 int32_t* __restrict__ ids) {
+                                                                 int32_t* __restrict__ ids,
+                                                                 double2* __restrict__ xy) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const KeyScale ks = key_scale(ext, bits);
     const double a = px[i], c = py[i];
+    if (xy) xy[i] = make_double2(a, c);  // interleaved copy for the gather (one line per point)
     uint64_t key = ~0ull;
     if (a == a && c == c) {
         const uint32_t qx = quantize(a, ks.x0, ks.sx, bits) << (32 - bits);
@@ -439,7 +441,45 @@ int64_t point_order_workspace_bytes(int64_t n) {
     c.take<double4>(1);
     c.take<char>(extent_workspace_bytes());
     c.take<char>(sort_workspace_bytes_u32(n));
+    c.take<double2>(n);
     return c.bytes;
+}
+
+// sx[i], sy[i] = xy[perm[i]]: the points interleaved by the key pass, so each
+// random read is one 16-byte load within one cache line (two separate x / y
+// arrays cost two lines per point).
+__global__ __launch_bounds__(256) void gather_xy_kernel(const double2* __restrict__ xy,
+                                                       const int32_t* __restrict__ perm, int64_t n,
+                                                       double* __restrict__ sx, double* __restrict__ sy) {
+    for (int64_t t0 = (int64_t)blockIdx.x * kGatherTile; t0 < n; t0 += (int64_t)gridDim.x * kGatherTile) {
+        const int64_t base = t0 + threadIdx.x;
+        int32_t j[kGatherItems];
+        double2 v[kGatherItems];
+#pragma unroll
+        for (int b = 0; b < kGatherItems; ++b) {
+            const int64_t i = base + 256 * b;
+            j[b] = i < n ? perm[i] : -1;
+        }
+#pragma unroll
+        for (int b = 0; b < kGatherItems; ++b) v[b] = j[b] >= 0 ? xy[j[b]] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int b = 0; b < kGatherItems; ++b) {
+            const int64_t i = base + 256 * b;
+            if (i < n) {
+                sx[i] = v[b].x;
+                sy[i] = v[b].y;
+            }
+        }
+    }
+}
+
+int gather_points_xy(const double2* xy, const int32_t* perm, int64_t n, double* sx, double* sy, hipStream_t s,
+                     int blocks_per_cu) {
+    if (n == 0) return VS_OK;
+    hipLaunchKernelGGL(gather_xy_kernel, dim3(stride_grid(n, kGatherTile, blocks_per_cu)), dim3(256), 0, s, xy, perm,
+                       n, sx, sy);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
 }
 
 // Point index for the learned form: keys of `bits` per axis (K holds 2*bits),
@@ -452,7 +492,19 @@ int64_t point_index_workspace_bytes_t(int64_t n) {
     c.take<char>(sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n));
     c.take<K>(n);  // unsorted keys and ids
     c.take<int32_t>(n);
+    c.take<double2>(n);  // interleaved points for the gather
     return c.bytes;
+}
+
+// The interleaved points point_index_build_t leaves in its workspace.
+template <class K>
+const double2* point_index_xy_t(void* ws, int64_t n) {
+    Carver c(ws);
+    c.take<char>(extent_workspace_bytes());
+    c.take<char>(sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n));
+    c.take<K>(n);
+    c.take<int32_t>(n);
+    return c.take<double2>(n);
 }
 
 int gather_points(const double* px, const double* py, const int32_t* perm, int64_t n, double* sx, double* sy,
@@ -474,12 +526,13 @@ int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, 
     void* sort_ws = c.take<char>(sb);
     K* keys_in = c.take<K>(n);
     int32_t* ids_in = c.take<int32_t>(n);
+    double2* xy = c.take<double2>(n);
     VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
     if (n == 0) return VS_OK;
     const int bits = 4 * (int)sizeof(K);
     const int zero_bits = 2 * (bits - point_index_grid_bits<K>());
     hipLaunchKernelGGL(hilbert_keys_points_kernel<K>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py, n,
-                       ext, bits, zero_bits, keys_in, ids_in);
+                       ext, bits, zero_bits, keys_in, ids_in, xy);
     VS_LAUNCH_CHECK();
     if (sizeof(K) == 4)
         VS_TRY(sort_pairs_into_u32(reinterpret_cast<const uint32_t*>(keys_in), ids_in,
@@ -487,11 +540,13 @@ int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, 
     else
         VS_TRY(sort_pairs_into_u64(reinterpret_cast<const uint64_t*>(keys_in), ids_in,
                                    reinterpret_cast<uint64_t*>(keys), perm, n, zero_bits, 64, sort_ws, sb, s));
-    if (sx) VS_TRY(gather_points(px, py, perm, n, sx, sy, s));
+    if (sx) VS_TRY(gather_points_xy(xy, perm, n, sx, sy, s, 8));
     return VS_OK;
 }
 
 int64_t point_index_workspace_bytes_u32(int64_t n) { return point_index_workspace_bytes_t<uint32_t>(n); }
+const double2* point_index_xy_u32(void* ws, int64_t n) { return point_index_xy_t<uint32_t>(ws, n); }
+const double2* point_index_xy_u64(void* ws, int64_t n) { return point_index_xy_t<uint64_t>(ws, n); }
 int64_t point_index_workspace_bytes_u64(int64_t n) { return point_index_workspace_bytes_t<uint64_t>(n); }
 int point_index_build_u32(const double* px, const double* py, int64_t n, uint32_t* keys, int32_t* perm, double* sx,
                           double* sy, double4* ext, void* ws, hipStream_t s) {
@@ -512,14 +567,15 @@ int point_order(const double* px, const double* py, int64_t n, int bits, int32_t
     double4* ext = c.take<double4>(1);
     void* ext_ws = c.take<char>(extent_workspace_bytes());
     void* sort_ws = c.take<char>(sort_workspace_bytes_u32(n));
+    double2* xy = c.take<double2>(n);
     VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
     hipLaunchKernelGGL(hilbert_keys_points_kernel<uint32_t>, dim3((unsigned)ceil_div(n, 256)), dim3(256),
-                       0, s, px, py, n, ext, 16, 0, keys_in, ids_in);
+                       0, s, px, py, n, ext, 16, 0, keys_in, ids_in, xy);
     VS_LAUNCH_CHECK();
     VS_TRY(sort_pairs_into_u32(keys_in, ids_in, keys, perm, n, 32 - 2 * bits, 32, sort_ws,
                                sort_workspace_bytes_u32(n), s));
-    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)ceil_div(n, kGatherTile)), dim3(256), 0, s, px, py,
-                       perm, n, sx, sy);
+    hipLaunchKernelGGL(gather_xy_kernel, dim3((unsigned)ceil_div(n, kGatherTile)), dim3(256), 0, s, xy, perm, n,
+                       sx, sy);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }

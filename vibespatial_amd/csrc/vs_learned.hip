@@ -47,6 +47,10 @@ namespace vs {
 
 int64_t point_index_workspace_bytes_u32(int64_t n);
 int64_t point_index_workspace_bytes_u64(int64_t n);
+const double2* point_index_xy_u32(void* ws, int64_t n);
+const double2* point_index_xy_u64(void* ws, int64_t n);
+int gather_points_xy(const double2* xy, const int32_t* perm, int64_t n, double* sx, double* sy, hipStream_t s,
+                     int blocks_per_cu);
 int point_index_build_u32(const double* px, const double* py, int64_t n, uint32_t* keys, int32_t* perm, double* sx,
                           double* sy, double4* ext, void* ws, hipStream_t s);
 int point_index_build_u64(const double* px, const double* py, int64_t n, uint64_t* keys, int32_t* perm, double* sx,
@@ -513,11 +517,18 @@ __host__ __device__ int cover_ranges(double x0, double y0, double x1, double y1,
     if (!(x0 <= x1 && y0 <= y1)) return 0;
     const uint32_t qx0 = quantize(x0, ks.x0, ks.sx, bits), qx1 = quantize(x1, ks.x0, ks.sx, bits);
     const uint32_t qy0 = quantize(y0, ks.y0, ks.sy, bits), qy1 = quantize(y1, ks.y0, ks.sy, bits);
+    // cells per axis in 64 bits (a box over a whole 2^32 axis spans 2^32
+    // cells), their product only once both are small
+    auto too_many = [&](int lv) {
+        const uint64_t cx = (uint64_t)(qx1 >> lv) - (uint64_t)(qx0 >> lv) + 1;
+        const uint64_t cy = (uint64_t)(qy1 >> lv) - (uint64_t)(qy0 >> lv) + 1;
+        return cx > (uint64_t)kMaxRanges || cy > (uint64_t)kMaxRanges || cx * cy > (uint64_t)kMaxRanges;
+    };
     int l = min_l;  // cells no finer than the index grid, so every range is a union of grid cells
-    while (l < bits - 1 && (uint64_t)((qx1 >> l) - (qx0 >> l) + 1) * ((qy1 >> l) - (qy0 >> l) + 1) > kMaxRanges) ++l;
+    while (l < bits - 1 && too_many(l)) ++l;
     const int lev = bits - l;
     const uint32_t ax = qx0 >> l, ay = qy0 >> l;
-    const uint32_t nx = (qx1 >> l) - ax + 1, ncell = nx * ((qy1 >> l) - ay + 1);
+    const uint32_t nx = (qx1 >> l) - ax + 1, ncell = nx * ((qy1 >> l) - ay + 1);  // <= 16 here
     uint64_t code[kMaxRanges];
     uint32_t tag[kMaxRanges];
 #pragma unroll
@@ -555,13 +566,19 @@ __host__ __device__ int cover_ranges(double x0, double y0, double x1, double y1,
         for (int t = 0; t < kMaxRanges; ++t) {
             if ((uint32_t)t >= ncell) continue;
             const uint32_t px = 2 * (ax + tag[t] % nx), py = 2 * (ay + tag[t] / nx);
+            // last digits of the children (px, py) and (px + 1, py); the curve
+            // visits the four children around the 2 x 2 square, so a child's
+            // diagonal partner has its digit xor 2
+            const uint32_t d00 = (uint32_t)(hilbert64(px << (32 - levc), py << (32 - levc)) >> (64 - 2 * levc)) & 3u;
+            const uint32_t d10 =
+                (uint32_t)(hilbert64((px + 1) << (32 - levc), py << (32 - levc)) >> (64 - 2 * levc)) & 3u;
+            const uint32_t dig[4] = {d00, d10, d10 ^ 2u, d00 ^ 2u};  // (0,0) (1,0) (0,1) (1,1)
             uint32_t m = 0;
 #pragma unroll
             for (int ch = 0; ch < 4; ++ch) {
                 const uint32_t cx = px + (ch & 1), cy = py + (ch >> 1);
                 if (cx < cx0 || cx > cx1 || cy < cy0 || cy > cy1) continue;
-                const uint64_t cc = hilbert64(cx << (32 - levc), cy << (32 - levc)) >> (64 - 2 * levc);
-                m |= 1u << (uint32_t)(cc & 3u);
+                m |= 1u << dig[ch];
             }
             masks |= (uint64_t)m << (4 * t);
         }
@@ -1112,15 +1129,13 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
         if ((long long)uniform_i32(s_item) >= T) break;
         // item values are block-uniform: kept in scalar registers
         const int q = uniform_i32(s_q);
-        if (tid < s_nr) s_rng[tid] = ranges[(int64_t)q * kMaxRanges + tid];
-        __syncthreads();
-        if (tid == 0) {
-            int acc = 0;
-            for (int r = 0; r < s_nr; ++r) {
-                s_pre[r] = acc;
-                acc += s_rng[r].y - s_rng[r].x;
-            }
-            s_pre[s_nr] = acc;
+        if (tid < 64) {  // wave 0: the ranges and their exclusive prefix in one pass
+            const int nr0 = uniform_i32(s_nr);
+            const int2 rg = tid < nr0 ? ranges[(int64_t)q * kMaxRanges + tid] : make_int2(0, 0);
+            const int len = rg.y - rg.x;
+            const int inc = wave_inclusive_scan(len);
+            if (tid < nr0) s_rng[tid] = rg;
+            if (tid <= nr0) s_pre[tid] = inc - len;
         }
         __syncthreads();
         const double4 box = qbox[q];
@@ -1544,7 +1559,8 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
     }
     // two gather blocks per CU (enough to keep the random reads in flight)
     // leave room for the side stream's kernels
-    VS_TRY(gather_points(px, py, B.perm, n, B.sx, B.sy, s, fork ? 2 : 8));
+    const double2* xy = sizeof(K) == 4 ? point_index_xy_u32(B.build_ws, n) : point_index_xy_u64(B.build_ws, n);
+    VS_TRY(gather_points_xy(xy, B.perm, n, B.sx, B.sy, s, fork ? 2 : 8));
     if (fork) {
         VS_CHECK_HIP(hipEventRecord(side->join, t));
         VS_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));

@@ -503,10 +503,14 @@ def as_geometry_array(obj) -> GeometryArray:
         return obj
     if hasattr(obj, "values") and isinstance(getattr(obj, "values"), GeometryArray):
         return obj.values
-    if isinstance(obj, PointArray):
-        return GeometryArray.from_points(obj)
-    if isinstance(obj, PolygonArray):
-        return GeometryArray.from_polygons(obj)
+    if isinstance(obj, (PointArray, PolygonArray)):
+        # one wrapper per array (its row tables are O(n) to build), like the
+        # column of a GeoSeries
+        ga = getattr(obj, "_geometry_array", None)
+        if ga is None:
+            ga = GeometryArray.from_points(obj) if isinstance(obj, PointArray) else GeometryArray.from_polygons(obj)
+            obj._geometry_array = ga
+        return ga
     if isinstance(obj, _Geom) or obj is None:
         return GeometryArray.from_scalars([obj])
     if isinstance(obj, (list, tuple, np.ndarray)):

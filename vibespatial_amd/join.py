@@ -634,18 +634,95 @@ def mlp_loss_grad(params: torch.Tensor, u: torch.Tensor, t: torch.Tensor):
 # ---------------------------------------------------------------------------
 # batched polygon x polygon intersects
 # ---------------------------------------------------------------------------
-def _encode_tile(waves_per_block: int, refine_threads: int) -> int:
-    return int(waves_per_block) | (int(refine_threads) << 8)
+def _encode_tile(waves_per_block: int, refine_threads: int, order: int = 0) -> int:
+    return int(waves_per_block) | (int(refine_threads) << 8) | (int(order) << 24)
 
 
-TILE_CANDIDATES = [(w, t) for w in (1, 2, 4) for t in (64, 128, 256)]
-_TILE_CACHE: dict = {}
+# (traversal waves per block, refine block size, candidate order: 0 by right
+# row, 1 by the right rows' coordinate-count bins, largest first)
+TILE_CANDIDATES = [(w, t, o) for o in (0, 1) for w in (1, 2, 4) for t in (64, 128, 256)]
 
 
 def _tile_key(n_jobs, n_left, n_right):
     # shape class: powers of two of the sizes (an autotune result carries over
     # to batches of similar shape)
     return (int(n_jobs).bit_length(), int(n_left).bit_length(), int(n_right).bit_length())
+
+
+class TileCache:
+    """Autotuned batched-join tiles per (device name, shape class), kept in
+    memory and in a JSON file that later processes load (like the
+    reference's on-disk kernel cache, cuda/_runtime.py:457-550).  The file is
+    $VIBESPATIAL_AMD_CACHE_DIR/batched_tiles.json (default
+    ~/.cache/vibespatial_amd); it is rewritten whole through a temporary file
+    and an atomic rename, and an unreadable file counts as empty."""
+
+    FILE = "batched_tiles.json"
+
+    def __init__(self):
+        self._mem: dict = {}
+        self._loaded_from = None
+
+    @staticmethod
+    def path() -> str:
+        import os
+
+        d = os.environ.get("VIBESPATIAL_AMD_CACHE_DIR") or os.path.join(os.path.expanduser("~"), ".cache",
+                                                                         "vibespatial_amd")
+        return os.path.join(d, TileCache.FILE)
+
+    def _load(self):
+        import json
+
+        p = self.path()
+        if self._loaded_from == p:
+            return
+        self._loaded_from = p
+        try:
+            with open(p) as f:
+                data = json.load(f)
+        except (OSError, ValueError):
+            return
+        for k, v in data.items() if isinstance(data, dict) else ():
+            try:
+                dev, a, b, c = k.split("|")
+                tile = tuple(int(t) for t in v)
+            except (ValueError, TypeError, IndexError):
+                continue
+            if tile in TILE_CANDIDATES:
+                self._mem.setdefault((dev, int(a), int(b), int(c)), tile)
+
+    def get(self, device: str, key, default=None):
+        self._load()
+        return self._mem.get((device, *key), default)
+
+    def put(self, device: str, key, tile) -> None:
+        import json
+        import os
+        import tempfile
+
+        self._load()
+        self._mem[(device, *key)] = tuple(int(t) for t in tile)
+        p = self.path()
+        try:
+            os.makedirs(os.path.dirname(p), exist_ok=True)
+            data = {"|".join(str(x) for x in k): list(v) for k, v in self._mem.items()}
+            fd, tmp = tempfile.mkstemp(dir=os.path.dirname(p), prefix=".tiles.")
+            with os.fdopen(fd, "w") as f:
+                json.dump(data, f, indent=1, sort_keys=True)
+            os.replace(tmp, p)
+        except OSError:
+            pass  # the in-memory entry still serves this process
+
+
+_TILE_CACHE = TileCache()
+
+
+def _device_name() -> str:
+    try:
+        return torch.cuda.get_device_name(torch.cuda.current_device())
+    except Exception:  # pragma: no cover
+        return "unknown"
 
 
 def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Tensor, right_job: torch.Tensor,
@@ -662,7 +739,7 @@ def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Ten
     lj = left_job.to(device=dev, dtype=torch.int64).contiguous()
     rj = right_job.to(device=dev, dtype=torch.int64).contiguous()
     if tile is None:
-        tile = _TILE_CACHE.get(_tile_key(n_jobs, len(A), len(B)), (4, 256))
+        tile = _TILE_CACHE.get(_device_name(), _tile_key(n_jobs, len(A), len(B)), (4, 256, 1))
     code = _encode_tile(*tile)
     cap = int(capacity) if capacity is not None else max(1024, 4 * len(A))
     cap = min(cap, PAIR_LIMIT)
@@ -691,8 +768,9 @@ def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Ten
 def autotune_batched(A: DevicePolygons, B: DevicePolygons, left_job: torch.Tensor, right_job: torch.Tensor,
                      reps: int = 2, candidates=None):
     """Times every tile shape on this batch (HIP events on the current stream)
-    and caches the fastest for batches of the same shape class.  Returns
-    (best tile, {tile: ms})."""
+    and caches the fastest for batches of the same shape class on this device
+    (TileCache: in memory and on disk, so later processes start from it).
+    Returns (best tile, {tile: ms})."""
     require_gpu()
     cands = list(candidates or TILE_CANDIDATES)
     _, _, st = batched_intersects(A, B, left_job, right_job, tile=cands[0])
@@ -708,7 +786,7 @@ def autotune_batched(A: DevicePolygons, B: DevicePolygons, left_job: torch.Tenso
         e1.synchronize()
         times[t] = e0.elapsed_time(e1) / reps
     best = min(times, key=times.get)
-    _TILE_CACHE[_tile_key(int(left_job.numel()) - 1, len(A), len(B))] = best
+    _TILE_CACHE.put(_device_name(), _tile_key(int(left_job.numel()) - 1, len(A), len(B)), best)
     return best, times
 
 
@@ -728,7 +806,7 @@ class BatchedIntersectsGraph:
         self.A, self.B = A, B
         self.cap = min(int(capacity), PAIR_LIMIT)
         if tile is None:
-            tile = _TILE_CACHE.get(_tile_key(self.n_jobs, len(A), len(B)), (4, 256))
+            tile = _TILE_CACHE.get(_device_name(), _tile_key(self.n_jobs, len(A), len(B)), (4, 256, 1))
         self.code = _encode_tile(*tile)
         self.ws = _ws(L.vs_batched_intersects_workspace_bytes(self.n_jobs, len(A), len(B), B.n_coords, self.cap),
                       dev)
