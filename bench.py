@@ -472,7 +472,8 @@ def run_c4(args):
 def run_c5(args):
     """4,096 independent polygon x polygon sjoin(intersects) jobs
     (vibespatial_amd.synthetic.c5) in one batched launch sequence: forest
-    build, persistent traversal, exact refine, finish.  A step = one batched
+    build, persistent traversal, cell-grid decisions, exact refine of the
+    rest, finish.  A step = one batched
     call over every job, inputs resident; the tile shape is autotuned first.
     Algorithmic bytes per step: sum_q 16 V_q + sum_C 16 (V_a + V_b) + 8 K."""
     import numpy as np
@@ -518,6 +519,21 @@ def run_c5(args):
     torch.cuda.synchronize()
     ms_graph = (time.perf_counter() - t0) * 1e3 / args.steps
     assert stg.pairs == st.pairs
+    # ablation: every bbox candidate refined exactly (no cell-grid decisions)
+    J.batched_intersects(A, B, tl, tr, capacity=cap, tile=best, cell_grid=False)
+    _lib.check(L.vs_set_kernel_timing(1), "vs_set_kernel_timing")
+    kms0 = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, _, st0 = J.batched_intersects(A, B, tl, tr, capacity=cap, tile=best, cell_grid=False)
+        ms = _lib.ctypes.c_float()
+        _lib.check(L.vs_last_kernel_ms(_lib.ctypes.byref(ms)), "vs_last_kernel_ms")
+        kms0.append(ms.value)
+    torch.cuda.synchronize()
+    ms_nogrid = (time.perf_counter() - t0) * 1e3 / args.steps
+    _lib.check(L.vs_set_kernel_timing(0), "vs_set_kernel_timing")
+    assert st0.pairs == st.pairs and st0.refined == st0.candidates
     VA = np.diff(left.ring_offsets[left.part_offsets[left.geom_offsets]])
     VB = np.diff(right.ring_offsets[right.part_offsets[right.geom_offsets]])
     # sum_q 16 V_q + sum_C 16 (V_a + V_b) + 8 K (the candidate term counted on the device)
@@ -561,10 +577,12 @@ def run_c5(args):
                         "(vibespatial_amd.synthetic.c5)",
             "pairs": st.pairs,
             "bbox_candidates": st.candidates,
+            "refined_exactly": st.refined,
             "tile": list(best),
             "autotune_ms": {f"{k[0]}x{k[1]}/order{k[2]}": v for k, v in times.items()},
             "traverse_refine_kernel_ms": statistics.mean(kms),
             "graph_replay_ms_per_step": ms_graph,
+            "without_cell_grids": {"ms_per_step": ms_nogrid, "traverse_refine_kernel_ms": statistics.mean(kms0)},
         },
         "roofline": {"bound": "hbm", "kernel": "whole step", "achieved": alg / (ms_step * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -373,7 +373,8 @@ int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, in
 
 /* Many independent sjoin(left_j, right_j, predicate='intersects') jobs in one
  * fixed launch sequence (a forest of packed trees, one persistent traversal,
- * one exact refine, one finish).  Job j owns left rows [left_job_offsets[j],
+ * decisions from 32 x 32 cell grids of the right rows where they settle a
+ * pair, one exact refine of the rest, one finish).  Job j owns left rows [left_job_offsets[j],
  * left_job_offsets[j+1]) of A and right rows [right_job_offsets[j],
  * right_job_offsets[j+1]) of B (device int64 arrays of n_jobs + 1 entries,
  * non-decreasing); only pairs inside one job are tested.  n_right_coords:
@@ -382,10 +383,13 @@ int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, in
  * tile: bits 0-7 traversal waves per block (1-4, 0 = 4), bits 8-23 refine
  * block size (multiple of 64 up to 256, 0 = 256), bit 24 candidate order
  * of the refine (0: by right row; 1: by right rows' coordinate-count bins,
- * largest first, then right row).  Every tile gives the same pairs.  d_result (device int64[4]):
- * [0] pairs, [1] bbox candidates staged (> capacity: retry with that
- * capacity), [2] bbox candidates, [3] sum over the candidates of both rows'
- * coordinate counts.  Replaces, per job, the reference's
+ * largest first, then right row), bit 25 set: no cell grids (every bbox
+ * candidate refined exactly), bits 26-27: right rings read through a y-slab
+ * index from 128 (0), 16 (1) or 64 (2) edges on, or never (3).  Every tile gives the same pairs.  d_result
+ * (device int64[5]): [0] pairs, [1] bbox candidates staged (> capacity:
+ * retry with that capacity), [2] bbox candidates, [3] sum over the
+ * candidates of both rows' coordinate counts, [4] candidates left to the
+ * exact refine.  Replaces, per job, the reference's
  * polygon x polygon sjoin path (spatial/spatial_index_device.py:228-390,
  * predicates/polygon.py:849-996, spatial/query.py:143-153). */
 int vs_batched_intersects(const double* ax, const double* ay, const int32_t* a_geom, const int32_t* a_part,
@@ -606,6 +610,26 @@ uint64_t vs_host_hilbert64(uint32_t x, uint32_t y);
  * returns their count.  Every point of the box has its key in one of them. */
 int vs_host_cover_ranges(double x0, double y0, double x1, double y1, const double* extent4, int32_t key_bytes,
                          uint64_t* out32);
+
+/* The 32 x 32 cell grid that vs_batched_intersects builds for a right row of
+ * one closed ring (x, y: its n coordinates; box4: its bounds): geo6 =
+ * (x0, y0, wx, wy, 1/wx, 1/wy), cells64[2 j] = boundary bits and
+ * cells64[2 j + 1] = inside bits of grid row j.  Returns 1, or 0 when the
+ * ring gets no grid (fewer than 4 coordinates, not closed, a non-finite
+ * coordinate, an empty side). */
+int vs_host_cell_grid(const double* x, const double* y, int64_t n, const double* box4, double* geo6,
+                      uint32_t* cells64);
+
+/* The batched join's decision for a left ring (x, y: its n >= 1 finite-first
+ * coordinates; box4: its bounds) against such a grid: 1 intersects, 0 does
+ * not, 2 undecided with the exact test's first step (the left first vertex
+ * located in the right ring) skippable, 3 undecided. */
+int vs_host_cell_decide(const double* geo6, const uint32_t* cells64, const double* box4, const double* x,
+                        const double* y, int32_t n);
+
+/* The lowest and highest grid cell (0..31) whose closed range holds v along
+ * axis 0 (x) or 1 (y) of such a grid, into lo_hi[2]. */
+int vs_host_cell_span(const double* geo6, int32_t axis, double v, int32_t* lo_hi);
 
 #ifdef __cplusplus
 }

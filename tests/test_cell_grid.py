@@ -1,0 +1,264 @@
+"""The batched join's cell grids on the host (vs_host_cell_grid /
+vs_host_cell_decide, the device code compiled for the host) against the
+oracle's exact point location and polygon intersects: every point of a cell
+left unmarked has the cell's location, and every decision the grid makes is
+the exact predicate's answer."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from vibespatial_amd import _lib
+from vibespatial_amd import synthetic as S
+from vibespatial_amd.geometry import PolygonArray
+
+G = 32
+
+
+def _ptr(a):
+    return a.ctypes.data
+
+
+def _ring_array(x, y) -> PolygonArray:
+    x, y = np.asarray(x, np.float64), np.asarray(y, np.float64)
+    return PolygonArray(x, y, np.array([0, len(x)]), np.array([0, 1]), np.array([0, 1]))
+
+
+def _grid(x, y):
+    L = _lib.lib()
+    x, y = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64)
+    box = np.array([np.nanmin(x), np.nanmin(y), np.nanmax(x), np.nanmax(y)]) if len(x) else np.zeros(4)
+    geo = np.zeros(6)
+    cells = np.zeros(2 * G, np.uint32)
+    ok = L.vs_host_cell_grid(_ptr(x), _ptr(y), len(x), _ptr(box), _ptr(geo), _ptr(cells))
+    return ok, geo, cells
+
+
+def _decide(geo, cells, box, x, y):
+    box = np.ascontiguousarray(box, np.float64)
+    x, y = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64)
+    return _lib.lib().vs_host_cell_decide(_ptr(geo), _ptr(cells), _ptr(box), _ptr(x), _ptr(y), len(x))
+
+
+def _boundaries_touch(ax, ay, bx, by) -> bool:
+    """Some edge of ring a touches some edge of ring b (oracle segments_touch
+    on the pairs whose boxes meet)."""
+    A = np.stack([ax[:-1], ay[:-1], ax[1:], ay[1:]], 1)
+    B = np.stack([bx[:-1], by[:-1], bx[1:], by[1:]], 1)
+
+    def box(E):
+        return (np.minimum(E[:, 0], E[:, 2]), np.minimum(E[:, 1], E[:, 3]), np.maximum(E[:, 0], E[:, 2]),
+                np.maximum(E[:, 1], E[:, 3]))
+
+    a0, a1, a2, a3 = box(A)
+    b0, b1, b2, b3 = box(B)
+    ov = (a0[:, None] <= b2[None]) & (b0[None] <= a2[:, None]) & (a1[:, None] <= b3[None]) & (b1[None] <= a3[:, None])
+    return any(oracle.segments_touch(A[i, :2], A[i, 2:], B[j, :2], B[j, 2:]) for i, j in zip(*np.nonzero(ov)))
+
+
+def _cell_edges(base, w):
+    # E(i) = base + i * w, rounded product then rounded sum (no fused multiply-add)
+    return base + np.arange(G + 1, dtype=np.float64) * w
+
+
+def _rings():
+    rng = np.random.default_rng(31)
+    out = []
+    for nv in (4, 5, 9, 40, 150, 400):
+        P = S.star_polygons(rng, np.array([1000.0]), np.array([-500.0]), np.array([55.0]), np.array([nv]),
+                            spacing=True)
+        out.append((P.x, P.y))
+    # the same star far from the origin (UTM-like offsets)
+    P = S.star_polygons(rng, np.array([5.0e5]), np.array([3.0e6]), np.array([60.0]), np.array([37]))
+    out.append((P.x, P.y))
+    # staircase on the integer grid of a 32 x 32 box: vertices and horizontal
+    # edges exactly on cell lines (half-open crossing rule)
+    sx = [0, 32, 32, 24, 24, 16, 16, 8, 8, 0, 0]
+    sy = [0, 0, 32, 32, 24, 24, 16, 16, 8, 8, 0]
+    out.append((np.array(sx, float), np.array(sy, float)))
+    # comb (deep concavities) and a self-crossing bow tie
+    cx = [0, 10, 10, 9, 9, 7, 7, 5, 5, 3, 3, 1, 1, 0, 0]
+    cy = [0, 0, 10, 10, 1, 1, 10, 10, 1, 1, 10, 10, 1, 1, 0]
+    out.append((np.array(cx, float), np.array(cy, float)))
+    # (a bow tie's diagonals span its whole box: every cell is marked)
+    out.append((np.array([0.0, 4.0, 4.0, 0.0, 0.0]), np.array([0.0, 3.0, 0.0, 3.0, 0.0])))
+    # a square with a small self-crossing loop on its left side
+    lx = [0, 10, 10, 0, 0, 3, 3, 0, 0]
+    ly = [0, 0, 10, 10, 2, 4, 2, 4, 0]
+    out.append((np.array(lx, float), np.array(ly, float)))
+    # clockwise star at an offset of 2^39 cells (the widest offset a grid allows)
+    P = S.star_polygons(rng, np.array([3.3e13 + 0.1]), np.array([-7.1e12]), np.array([1000.3]), np.array([23]))
+    out.append((P.x[::-1].copy(), P.y[::-1].copy()))
+    return out
+
+
+BOW_TIE = 9
+
+
+def _cell_points(geo, rng, i, j, k=6):
+    """Points of cell (i, j)'s closed range: its corners, its side midpoints
+    and k random interior points (infinite sides replaced by a point 3 cells out)."""
+    X, Y = _cell_edges(geo[0], geo[2]), _cell_edges(geo[1], geo[3])
+    x0 = X[i] if i > 0 else X[0] - 3 * geo[2]
+    x1 = X[i + 1] if i < G - 1 else X[G] + 3 * geo[2]
+    y0 = Y[j] if j > 0 else Y[0] - 3 * geo[3]
+    y1 = Y[j + 1] if j < G - 1 else Y[G] + 3 * geo[3]
+    xs = [X[i], X[i + 1] if i < G - 1 else x1, x0, x1, 0.5 * (x0 + x1)]
+    ys = [Y[j], Y[j + 1] if j < G - 1 else y1, y0, y1, 0.5 * (y0 + y1)]
+    px = np.array([a for a in xs for _ in ys] + list(rng.uniform(x0, x1, k)))
+    py = np.array([b for _ in xs for b in ys] + list(rng.uniform(y0, y1, k)))
+    keep = (px >= (X[i] if i > 0 else -np.inf)) & (px <= (X[i + 1] if i < G - 1 else np.inf))
+    keep &= (py >= (Y[j] if j > 0 else -np.inf)) & (py <= (Y[j + 1] if j < G - 1 else np.inf))
+    return px[keep], py[keep]
+
+
+@pytest.mark.parametrize("case", range(len(_rings())))
+def test_unmarked_cells_have_one_location(case):
+    x, y = _rings()[case]
+    ok, geo, cells = _grid(x, y)
+    assert ok
+    P = _ring_array(x, y)
+    bnd, inn = cells[0::2], cells[1::2]
+    rng = np.random.default_rng(case)
+    pts_x, pts_y, want = [], [], []
+    n_free = 0
+    for j in range(G):
+        for i in range(G):
+            if (bnd[j] >> np.uint32(i)) & np.uint32(1):
+                continue
+            n_free += 1
+            px, py = _cell_points(geo, rng, i, j)
+            pts_x.append(px)
+            pts_y.append(py)
+            want.append(np.full(len(px), 2 if (inn[j] >> np.uint32(i)) & np.uint32(1) else 0, np.uint8))
+    if case == BOW_TIE:
+        assert n_free == 0
+        return
+    assert n_free > 0
+    px, py, want = np.concatenate(pts_x), np.concatenate(pts_y), np.concatenate(want)
+    got = oracle.point_location_pairs(px, py, np.arange(len(px)), np.zeros(len(px), np.int64), P)
+    assert np.array_equal(got, want)
+    # every vertex lies in marked cells only (all cells holding it)
+    X, Y = _cell_edges(geo[0], geo[2]), _cell_edges(geo[1], geo[3])
+    for vx, vy in zip(x, y):
+        i0 = max(0, min(G - 1, int(np.searchsorted(X[1:G], vx, "left"))))
+        j0 = max(0, min(G - 1, int(np.searchsorted(Y[1:G], vy, "left"))))
+        assert (bnd[j0] >> np.uint32(i0)) & np.uint32(1)
+
+
+def test_rings_without_a_grid():
+    sq = (np.array([0.0, 1.0, 1.0, 0.0, 0.0]), np.array([0.0, 0.0, 1.0, 1.0, 0.0]))
+    assert _grid(*sq)[0] == 1
+    assert _grid(sq[0][:-1], sq[1][:-1])[0] == 0                     # not closed
+    assert _grid(np.array([0.0, 1.0, 0.0]), np.array([0.0, 1.0, 0.0]))[0] == 0  # fewer than 4 coordinates
+    assert _grid(np.array([0.0, 1.0, 2.0, 0.0]), np.array([5.0, 5.0, 5.0, 5.0]))[0] == 0  # zero height
+    bad_y = sq[1].copy()
+    bad_y[2] = np.nan
+    assert _grid(sq[0], bad_y)[0] == 0                              # a NaN coordinate
+    inf_x = sq[0].copy()
+    inf_x[1] = np.inf
+    assert _grid(inf_x, sq[1])[0] == 0
+    # cells narrower than 2^-40 of the offset (the first index guess could be off by more than one)
+    assert _grid(1.0e16 + 64.0 * sq[0], sq[1])[0] == 0
+    assert _grid(1.0e12 + 64.0 * sq[0], sq[1])[0] == 1
+
+
+def _footprints(rng, x, y, n):
+    """Small rings around ring (x, y): random stars over its box, plus rings
+    sharing one of its vertices or lying along one of its edges."""
+    x0, y0, x1, y1 = x.min(), y.min(), x.max(), y.max()
+    w, h = x1 - x0, y1 - y0
+    R = rng.uniform(0.02, 0.2, n) * max(w, h)
+    cx = rng.uniform(x0 - 0.2 * w, x1 + 0.2 * w, n)
+    cy = rng.uniform(y0 - 0.2 * h, y1 + 0.2 * h, n)
+    F = S.star_polygons(rng, cx, cy, R, rng.integers(4, 13, n))
+    rings = [(F.x[F.ring_offsets[k]:F.ring_offsets[k + 1]], F.y[F.ring_offsets[k]:F.ring_offsets[k + 1]])
+             for k in range(n)]
+    for k in rng.choice(len(x) - 1, min(12, len(x) - 1), replace=False):
+        vx, vy = x[k], y[k]
+        d = 0.05 * max(w, h)
+        for sgn in (1.0, -1.0):  # a triangle with a vertex on B's vertex, pointing in or out
+            rings.append((np.array([vx, vx + sgn * d, vx + sgn * d, vx]), np.array([vy, vy - d, vy + d, vy])))
+        ax, ay, bx, by = x[k], y[k], x[k + 1], y[k + 1]
+        rings.append((np.array([ax, bx, 0.5 * (ax + bx) + (by - ay), ax]),
+                      np.array([ay, by, 0.5 * (ay + by) - (bx - ax), ay])))  # shares edge k
+    return rings
+
+
+@pytest.mark.parametrize("case", range(len(_rings())))
+def test_grid_decisions_are_the_exact_predicate(case):
+    x, y = _rings()[case]
+    ok, geo, cells = _grid(x, y)
+    B = _ring_array(x, y)
+    rng = np.random.default_rng(100 + case)
+    seen = {0: 0, 1: 0, 2: 0, 3: 0}
+    for fx, fy in _footprints(rng, x, y, 300):
+        A = _ring_array(fx, fy)
+        box = np.array([fx.min(), fy.min(), fx.max(), fy.max()])
+        d = _decide(geo, cells, box, fx, fy)
+        seen[d] += 1
+        exact = oracle.polygon_intersects(A, 0, B, 0)
+        if d in (0, 1):
+            assert exact == bool(d)
+        elif d == 2:
+            # the exact test without its first step: boundary contact, or B's first vertex in A
+            touch = _boundaries_touch(np.asarray(fx, float), np.asarray(fy, float), x, y)
+            assert touch or oracle.point_location(fx[0], fy[0], B, 0) == 0
+            assert exact == (touch or oracle.point_location(x[0], y[0], A, 0) != 0)
+    print(f"ring {case}: decisions {seen}")
+    if case == BOW_TIE:
+        assert seen[0] == seen[1] == seen[2] == 0
+    else:
+        assert seen[0] + seen[1] > 0 and seen[3] > 0
+
+
+def test_a_planted_grid_error_is_caught():
+    """The location check above fails on a grid whose inside bits are
+    shifted by one column (a wrong prefix position)."""
+    x, y = _rings()[3]
+    ok, geo, cells = _grid(x, y)
+    bnd, inn = cells[0::2], cells[1::2] << np.uint32(1)
+    P = _ring_array(x, y)
+    rng = np.random.default_rng(0)
+    bad = 0
+    for j in range(G):
+        for i in range(G):
+            if (bnd[j] >> np.uint32(i)) & np.uint32(1):
+                continue
+            px, py = _cell_points(geo, rng, i, j, k=1)
+            want = 2 if (inn[j] >> np.uint32(i)) & np.uint32(1) else 0
+            got = oracle.point_location_pairs(px, py, np.arange(len(px)), np.zeros(len(px), np.int64), P)
+            bad += int(np.any(got != want))
+    assert bad > 0
+
+
+def _span_brute(E, v):
+    """Cells whose closed range holds v: cell 0 is (-inf, E1], cell 31 [E31, inf)."""
+    cells = [i for i in range(G)
+             if (i == 0 or E[i] <= v) and (i == G - 1 or v <= E[i + 1])]
+    return cells[0], cells[-1]
+
+
+@pytest.mark.parametrize("offset_cells", [0.0, 1.0e3, 2.0 ** 20, 2.0 ** 39.9])
+def test_cell_index_search_is_exact(offset_cells):
+    """cell_lo / cell_hi (first guess corrected by one comparison on each
+    side) equal a scan of the cell edges, for values on, next to and between
+    the edges, outside the box, and for boxes offset by up to 2^40 cells."""
+    L = _lib.lib()
+    rng = np.random.default_rng(int(offset_cells) % 1000 + 5)
+    for trial in range(20):
+        w = rng.uniform(0.1, 10.0) * 32
+        x0 = offset_cells * (w / 32) * rng.choice([-1.0, 1.0]) + rng.uniform(-1, 1)
+        x = np.array([x0, x0 + w, x0 + w, x0, x0])
+        y = np.array([0.0, 0.0, w, w, 0.0])
+        ok, geo, _ = _grid(x, y)
+        assert ok
+        E = _cell_edges(geo[0], geo[2])
+        vals = list(E) + [np.nextafter(e, np.inf) for e in E] + [np.nextafter(e, -np.inf) for e in E]
+        vals += list(rng.uniform(E[0] - w, E[-1] + w, 200)) + [E[0] - 1e9, E[-1] + 1e9]
+        out = np.zeros(2, np.int32)
+        for v in vals:
+            L.vs_host_cell_span(_ptr(geo), 0, float(v), _ptr(out))
+            assert tuple(out) == _span_brute(E, v), (trial, v)

@@ -46,13 +46,80 @@ def _small_batch(n_jobs=48):
 
 
 @pytest.mark.parametrize("tile", [(4, 256, 0), (4, 256, 1)])
-def test_batched_matches_oracle_per_job_with_overlapping_jobs(tile):
-    print(f"batched 48 overlapping jobs, tile {tile}", flush=True)
+@pytest.mark.parametrize("cell_grid", [True, False])
+def test_batched_matches_oracle_per_job_with_overlapping_jobs(tile, cell_grid):
+    print(f"batched 48 overlapping jobs, tile {tile}, cell grid {cell_grid}", flush=True)
     left, right, lj, rj = _small_batch()
-    l, r, st = _run(left, right, lj, rj, tile=tile)
+    l, r, st = _run(left, right, lj, rj, tile=tile, cell_grid=cell_grid)
     el, er = _expected(left, right, lj, rj)
     assert np.array_equal(l, el) and np.array_equal(r, er)
     assert st.candidates >= st.pairs > 0
+    if cell_grid:
+        assert 0 < st.refined < st.candidates
+    else:
+        assert st.refined == st.candidates
+
+
+def _rings_array(rings):
+    from vibespatial_amd.geometry import PolygonArray
+
+    xs = [np.asarray(x, np.float64) for x, _ in rings]
+    ys = [np.asarray(y, np.float64) for _, y in rings]
+    ro = np.concatenate([[0], np.cumsum([len(x) for x in xs])]).astype(np.int64)
+    k = np.arange(len(rings) + 1, dtype=np.int64)
+    return PolygonArray(np.concatenate(xs), np.concatenate(ys), ro, k, k)
+
+
+def _touching_batch():
+    """Job 0: parcels (stars of 40-200 vertices) and footprints placed on
+    them at random, on their vertices (triangles pointing in and out), along
+    their edges (triangles sharing an edge), inside and outside; job 1:
+    parcels that get no cell grid (not closed, zero height, three
+    coordinates) with footprints over them."""
+    rng = np.random.default_rng(77)
+    n_par = 12
+    P = S.star_polygons(rng, rng.uniform(0, 400, n_par), rng.uniform(0, 400, n_par), rng.uniform(40, 70, n_par),
+                        rng.integers(40, 200, n_par), spacing=True)
+    parcels = [(P.x[P.ring_offsets[k]:P.ring_offsets[k + 1]], P.y[P.ring_offsets[k]:P.ring_offsets[k + 1]])
+               for k in range(n_par)]
+    feet = []
+    F = S.star_polygons(rng, rng.uniform(-20, 420, 600), rng.uniform(-20, 420, 600), rng.uniform(2.5, 20, 600),
+                        rng.integers(4, 13, 600))
+    feet += [(F.x[F.ring_offsets[k]:F.ring_offsets[k + 1]], F.y[F.ring_offsets[k]:F.ring_offsets[k + 1]])
+             for k in range(600)]
+    for x, y in parcels:
+        for k in rng.choice(len(x) - 1, 8, replace=False):
+            vx, vy, d = x[k], y[k], 3.0
+            for sgn in (1.0, -1.0):
+                feet.append(([vx, vx + sgn * d, vx + sgn * d, vx], [vy, vy - d, vy + d, vy]))
+                feet.append(([vx + sgn * d, vx, vx + sgn * d, vx + sgn * d], [vy - d, vy, vy + d, vy - d]))
+            ax, ay, bx, by = x[k], y[k], x[k + 1], y[k + 1]
+            for sgn in (1.0, -1.0):
+                feet.append(([ax, bx, 0.5 * (ax + bx) + sgn * (by - ay), ax],
+                             [ay, by, 0.5 * (ay + by) - sgn * (bx - ax), ay]))
+    # job 1: rows without a grid
+    odd = [([0, 50, 50, 0], [0, 0, 50, 50]),                     # not closed
+           ([100, 150, 170, 100], [10, 10, 10, 10]),             # zero height
+           ([300, 340, 300], [0, 20, 0])]                        # three coordinates
+    odd_feet = [([x0 + 5, x0 + 15, x0 + 10, x0 + 5], [5, 5, 12, 5]) for x0 in (0, 100, 140, 300, 310)]
+    odd_feet += [([x0 + 30, x0 + 60, x0 + 45, x0 + 30], [-5, -5, 30, -5]) for x0 in (0, 100, 300)]
+    left = _rings_array(feet + odd_feet)
+    right = _rings_array(parcels + odd)
+    lj = np.array([0, len(feet), len(feet) + len(odd_feet)], np.int64)
+    rj = np.array([0, len(parcels), len(parcels) + len(odd)], np.int64)
+    return left, right, lj, rj
+
+
+@pytest.mark.parametrize("cell_grid", [True, False])
+def test_batched_footprints_touching_parcels_and_rows_without_grid(cell_grid):
+    print(f"batched touching footprints, cell grid {cell_grid}", flush=True)
+    left, right, lj, rj = _touching_batch()
+    el, er = _expected(left, right, lj, rj)
+    for tile in [(4, 256, 0), (4, 256, 1)]:
+        l, r, st = _run(left, right, lj, rj, tile=tile, cell_grid=cell_grid)
+        assert np.array_equal(l, el) and np.array_equal(r, er), tile
+    if cell_grid:
+        assert st.refined < st.candidates
 
 
 def test_batched_refine_on_holes_multiparts_and_long_rings():

@@ -111,6 +111,10 @@ def test_c5_all_jobs_bit_exact():
     l, r, st = J.batched_intersects(A, B, tl, tr, tile=(4, 256, 1))
     l0, r0, _ = J.batched_intersects(A, B, tl, tr, tile=(4, 256, 0))  # candidates by right row only
     assert torch.equal(l0, l) and torch.equal(r0, r)
+    l0, r0, st0 = J.batched_intersects(A, B, tl, tr, tile=(4, 256, 1), cell_grid=False)  # every candidate refined
+    assert torch.equal(l0, l) and torch.equal(r0, r) and st0.refined == st0.candidates
+    print(f"c5: {st.refined} of {st.candidates} candidates refined exactly with the cell grids")
+    assert st.refined < st.candidates
     l, r = l.cpu().numpy(), r.cpu().numpy()
     # the oracle over all rows at once, restricted to pairs inside one job,
     # is every job's own join (a pair's rows belong to exactly one job each)

@@ -16,14 +16,19 @@
 //      consecutive sorted left rows; a persistent kernel deals the waves out
 //      round-robin and each wave walks its own job's tree with a shared LDS
 //      stack, staging bbox candidates;
-//   4. one thread per candidate runs the exact intersects refine (edges of
-//      the pair, then one vertex per ring); a right row of one ring with many
-//      edges is read through its y-slab index (built once per call), so a
-//      candidate visits only the edges in its own y-range; survivors go to
-//      the finish pass (radix sort by left row, sort of right rows per left
-//      row).
+//   4. every candidate whose left row is one small ring is first decided
+//      from the right row's cell grid (vs_grid.h) where the grid allows:
+//      pairs found to intersect go straight to the finish stage, pairs found
+//      apart are dropped, the rest are kept for step 5;
+//   5. one thread per remaining candidate runs the exact intersects refine
+//      (edges of the pair, then one vertex per ring); a right row of one ring
+//      with many edges is read through its y-slab index (built once per
+//      call), so a candidate visits only the edges in its own y-range;
+//      survivors go to the finish pass (radix sort by left row, sort of
+//      right rows per left row).
 // Tile shape (`tile`): waves per block of the traversal and the refine block
 // size, chosen by the host autotuner.
+#include "vs_grid.h"
 #include "vs_join_common.h"
 #include "vs_polys.h"
 
@@ -263,7 +268,8 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
 
 // ---------------------------------------------------------------------------
 // Y-slab index of the right rows.  A right row of one ring with at least
-// kRowSlabMinEdges edges (and at most 2^15 coordinates) gets its y-range
+// min_edges edges (128, or 16 / 64 / never by the tile; and at most 2^15
+// coordinates) gets its y-range
 // [ylo, yhi] cut into B equal slabs (B a power of two, about one per four
 // edges, at most kRowSlabMax); edge e = (c0 + e - 1, c0 + e) is listed in
 // every slab from slab(min y) to slab(max y).  slab() is monotone in y, so a
@@ -275,7 +281,7 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
 // lists): B + 1 slab starts, then at most 2 (n - 1) edge numbers (a row that
 // would need more takes one slab holding every edge).
 // ---------------------------------------------------------------------------
-constexpr int kRowSlabMinEdges = 64;  // smaller rows are streamed whole (four edges in flight)
+constexpr int kRowSlabMinEdges[4] = {128, 16, 64, 1 << 30};  // by tile bits 26-27 (3: none); smaller rows are streamed whole
 constexpr int kRowSlabMax = 64;
 
 struct RowSlabs {
@@ -295,14 +301,15 @@ __device__ __forceinline__ int64_t row_slab_base(int32_t c0) { return 3 * (int64
 // One 64-thread block per row (grid-stride): counts per slab, starts, entries.
 __global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_rows, const double4* __restrict__ rbox,
                                                             double2* __restrict__ geo, int32_t* __restrict__ nslab,
-                                                            uint16_t* __restrict__ list, int64_t list_cap) {
+                                                            uint16_t* __restrict__ list, int64_t list_cap,
+                                                            int min_edges) {
     __shared__ int cnt[kRowSlabMax + 1];
     __shared__ int cur[kRowSlabMax];
     const int lane = threadIdx.x;
     for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
         int32_t c0 = 0, n = 0;
         // B + 1 + 2 (n - 1) <= 3 n entries from 3 c0 on, every value below 2^16
-        const bool ok = single_ring_of(B, r, c0, n) && n - 1 >= kRowSlabMinEdges && n <= 32768 && c0 >= 0 &&
+        const bool ok = single_ring_of(B, r, c0, n) && n - 1 >= min_edges && n <= 32768 && c0 >= 0 &&
                         row_slab_base(c0) + 3 * (int64_t)n <= list_cap;
         if (!ok) {
             if (lane == 0) nslab[r] = 0;
@@ -362,15 +369,110 @@ __global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_r
     }
 }
 
+// Per row: info = (first coordinate c0 when the row is one ring of at least
+// one coordinate, else -1; coordinates of the row over all rings) and, when
+// first != nullptr, the ring's first vertex.  Later kernels read these two
+// independent words instead of walking geom -> part -> ring per pair.
+__global__ __launch_bounds__(256) void row_info_kernel(Polys P, int64_t n_rows, int2* __restrict__ info,
+                                                      double2* __restrict__ first) {
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n_rows; r += (int64_t)gridDim.x * 256) {
+        const int32_t p0 = P.geom[r], p1 = P.geom[r + 1];
+        const int32_t q0 = P.part[p0], q1 = P.part[p1];
+        const int32_t c0 = P.ring[q0], c1 = P.ring[q1];
+        const bool single = p1 - p0 == 1 && q1 - q0 == 1 && c1 > c0;
+        info[r] = make_int2(single ? c0 : -1, c1 - c0);
+        if (first) first[r] = single ? make_double2(P.x[c0], P.y[c0]) : make_double2(NAN, NAN);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell grids of the right rows (vs_grid.h): 16 lanes per row, 16 rows per
+// 256-thread block (grid-stride, block-uniform trip count).  A row gets no
+// grid (geo.z = 0) unless it is one closed ring of at least 4 finite
+// coordinates with a box of non-empty sides.
+// ---------------------------------------------------------------------------
+struct RowGrids {
+    const double4* geo;  // (x0, y0, wx, wy); wx = 0: no grid
+    const double2* inv;  // (1 / wx, 1 / wy)
+    const uint2* cells;  // kGrid rows per right row: (boundary bits, inside bits)
+};
+
+constexpr int kGridRowsPerBlock = 16;
+
+__global__ __launch_bounds__(256) void row_grid_build_kernel(Polys B, int64_t n_rows, const int2* __restrict__ rinfo,
+                                                            const double4* __restrict__ rbox,
+                                                            double4* __restrict__ geo, double2* __restrict__ inv,
+                                                            uint2* __restrict__ cells) {
+    __shared__ uint32_t bnd_s[kGridRowsPerBlock][kGrid];
+    __shared__ uint32_t tog_s[kGridRowsPerBlock][kGrid];
+    const int grp = threadIdx.x >> 4, sub = threadIdx.x & 15, lane = threadIdx.x & 63;
+    uint32_t* bnd = bnd_s[grp];
+    uint32_t* tog = tog_s[grp];
+    for (int64_t r0 = (int64_t)blockIdx.x * kGridRowsPerBlock; r0 < n_rows;
+         r0 += (int64_t)gridDim.x * kGridRowsPerBlock) {
+        const int64_t r = r0 + grp;
+        int32_t c0 = 0, n = 0;
+        GridAxes g{};
+        bool ok = false;
+        if (r < n_rows) {
+            const int2 ri = rinfo[r];
+            c0 = ri.x;
+            n = ri.y;
+            ok = c0 >= 0 && n >= 4;
+        }
+        if (ok) {
+            const double4 bb = rbox[r];
+            ok = grid_setup(bb.x, bb.y, bb.z, bb.w, g) && B.x[c0] == B.x[c0 + n - 1] && B.y[c0] == B.y[c0 + n - 1];
+        }
+        bnd[sub] = 0u;
+        bnd[sub + 16] = 0u;
+        tog[sub] = 0u;
+        tog[sub + 16] = 0u;
+        __syncthreads();
+        bool bad = false;
+        if (ok) {
+            for (int32_t e = 1 + sub; e < n; e += 16) {
+                const double ax = B.x[c0 + e - 1], ay = B.y[c0 + e - 1], bx = B.x[c0 + e], by = B.y[c0 + e];
+                if (!(fabs(ax) < INFINITY && fabs(ay) < INFINITY && fabs(bx) < INFINITY && fabs(by) < INFINITY)) {
+                    bad = true;
+                    continue;
+                }
+                const GridEdge ge = grid_edge(g, ax, ay, bx, by);
+                const uint32_t m = span_mask(ge.i0, ge.i1);
+                for (int j = ge.j0; j <= ge.j1; ++j) {
+                    atomicOr(&bnd[j], m);
+                    if (ge.k < kGrid && grid_crosses(g, j, ay, by)) atomicXor(&tog[j], 1u << ge.k);
+                }
+            }
+        }
+        const bool any_bad = ((__ballot(bad) >> (lane & 48)) & 0xFFFFull) != 0;  // this row's 16 lanes
+        __syncthreads();
+        if (ok) {
+            cells[r * kGrid + sub] = make_uint2(bnd[sub], prefix_xor32(tog[sub]));
+            cells[r * kGrid + sub + 16] = make_uint2(bnd[sub + 16], prefix_xor32(tog[sub + 16]));
+            if (sub == 0) {
+                geo[r] = any_bad ? make_double4(0.0, 0.0, 0.0, 0.0) : make_double4(g.x0, g.y0, g.wx, g.wy);
+                inv[r] = make_double2(g.ix, g.iy);
+            }
+        } else if (r < n_rows && sub == 0) {
+            geo[r] = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+        __syncthreads();  // bnd / tog are cleared for the next rows
+    }
+}
+
 // The exact intersects of ring A (na coordinates, a lane's LDS column) and
 // right row B, a single ring c0b.. of nb coordinates read through its slab
 // index: the same three steps as ring_intersects_polys (A's first vertex
 // located in B -- inside or on the boundary decides; B's edges near A tested
 // against A's edges; B's first vertex located in A), with each step visiting
 // only the slab edges that can change its outcome, so the same boolean.
+// skip_first: A's first vertex is known to lie outside B, off its boundary
+// (the first step's outcome), so that step is not run.
 __device__ inline bool ring_intersects_slabbed(const RingRef& A, int32_t na, const double* __restrict__ bx,
                                                const double* __restrict__ by, int32_t c0b,
-                                               const uint16_t* __restrict__ L, int nb, double2 g) {
+                                               const uint16_t* __restrict__ L, int nb, double2 g,
+                                               bool skip_first) {
     double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
     for (int32_t k = 0; k < na; ++k) {
         mnx = fmin(mnx, A.X(k));
@@ -382,7 +484,7 @@ __device__ inline bool ring_intersects_slabbed(const RingRef& A, int32_t na, con
     // Slab entries four at a time: the four edge numbers, then the four
     // edges' coordinates, are loaded together (memory-level parallelism).
     constexpr int kU = 4;
-    {
+    if (!skip_first) {
         const double fx = A.X(0), fy = A.Y(0);
         const int b = row_slab(fy, g.x, g.y, nb);
         int parity = 0;
@@ -440,76 +542,237 @@ __device__ inline bool ring_intersects_slabbed(const RingRef& A, int32_t na, con
     return parity != 0;
 }
 
-// Coordinates of row r over all its rings.
-__device__ __forceinline__ int32_t row_coords(const Polys& P, int64_t r) {
-    return P.ring[P.part[P.geom[r + 1]]] - P.ring[P.part[P.geom[r]]];
-}
+constexpr int kRingTile = 13;  // left rings of at most this many coordinates are refined from LDS
 
-// Candidate order by work: key = (15 - min(15, floor(log2 coords of the
-// right row))) << rbits | right row, so a sort on it puts the candidates of
-// the largest right rows first and keeps each right row's candidates
-// together; the 64 lanes of a wave then refine rows of similar edge counts.
-__global__ __launch_bounds__(256) void cand_cost_keys_kernel(const int32_t* __restrict__ cand_r,
-                                                            const JoinCtl* __restrict__ cctl, int64_t ccap, Polys B,
-                                                            int rbits, uint32_t* __restrict__ key) {
-    const unsigned long long staged = cctl->staged;
-    const int64_t n = staged < (unsigned long long)ccap ? (int64_t)staged : ccap;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int32_t r = cand_r[i];
-        const int32_t v = row_coords(B, r);
-        const int lg = v > 1 ? 31 - __clz(v) : 0;
-        key[i] = ((uint32_t)(15 - (lg < 15 ? lg : 15)) << rbits) | (uint32_t)r;
+// One thread per bbox candidate, in the traversal's order.  A pair whose
+// left row is one ring of at most kRingTile coordinates (with a finite first
+// vertex and no more coordinates than the right row) and whose right row has
+// a cell grid is decided from the grid where it can be (grid_decide: the left
+// box first, then the left vertices): pairs that intersect are appended to
+// the finish stage, pairs that do not are dropped.  Every other pair is appended to (key, left id) for the sort that
+// orders the refine: key = the right row, or with by_work
+// (15 - min(15, floor(log2 coords of the right row))) << (rbits + 1) |
+// (first step still to run) << rbits | right row, so the candidates of the
+// largest right rows come first, each right row's together (in at most two
+// groups), and a wave's lanes refine rows of similar edge counts through the
+// same steps.  Bit 31
+// of the left id marks a pair whose exact test may skip its first step
+// (a left vertex lies in a free outside cell of the right row).  Both lists are gathered per
+// block in LDS and appended with one global atomic per kClsBuf - 256 or more
+// entries (one atomic per wave on the two counters serialised the kernel).
+// ctl->stat[1] accumulates, over all candidates, the coordinates of both rows.
+// grid_decide with its loads issued in groups: the first eight grid rows of
+// the left box at once (further rows, rare, one by one), then the left
+// vertices eight at a time; each vertex's row is one of the box's rows (the
+// vertex lies in the box), read again through the cache.  The same decisions
+// as grid_decide, which the host tests check.
+__device__ inline int grid_decide_wide(const GridAxes& g, const uint2* __restrict__ cells, double4 ab,
+                                       const double* __restrict__ x, const double* __restrict__ y, int n) {
+    const int i0 = cell_lo(ab.x, g.x0, g.wx, g.ix), i1 = cell_hi(ab.z, g.x0, g.wx, g.ix);
+    const int j0 = cell_lo(ab.y, g.y0, g.wy, g.iy), j1 = cell_hi(ab.w, g.y0, g.wy, g.iy);
+    const uint32_t m = span_mask(i0, i1);
+    uint2 rows[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rows[u] = (j0 + u <= j1) ? cells[j0 + u] : make_uint2(0u, 0xFFFFFFFFu);
+    // rows past j1 read as free and inside: neutral for both tests below
+    uint32_t marks = 0u, all_in = m, any_in = 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const uint2 c = (j0 + u <= j1) ? rows[u] : make_uint2(0u, m);
+        marks |= c.x & m;
+        all_in &= c.y;
+        any_in |= (j0 + u <= j1) ? (c.y & m) : 0u;
     }
+    for (int j = j0 + 8; j <= j1; ++j) {
+        const uint2 c = cells[j];
+        marks |= c.x & m;
+        all_in &= c.y;
+        any_in |= c.y & m;
+    }
+    if (!marks) {
+        if (all_in == m) return 1;
+        if (any_in == 0u) return 0;
+    }
+    bool finite = true, in = false, out = false;
+    for (int k0 = 0; k0 < n; k0 += 8) {
+        double vx[8], vy[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = k0 + u < n ? k0 + u : n - 1;
+            vx[u] = x[k];
+            vy[u] = y[k];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            finite = finite && fabs(vx[u]) < INFINITY && fabs(vy[u]) < INFINITY;
+            if (!finite) break;
+            const int ci = cell_lo(vx[u], g.x0, g.wx, g.ix), cj = cell_lo(vy[u], g.y0, g.wy, g.iy);
+            const uint2 c = cells[cj];
+            if (!((c.x >> ci) & 1u)) {
+                if ((c.y >> ci) & 1u) in = true;
+                else out = true;
+            }
+        }
+        if (!finite) break;
+    }
+    if (!finite) return 3;
+    return in ? 1 : (out ? 2 : 3);
 }
 
-// One thread per candidate: exact intersects; survivors appended to the
-// finish stage with one atomic per wave.  ctl->stat[1] accumulates, over the
-// candidates, the coordinates of both rows.
-// A left row that is one ring of at most kRingTile coordinates (the
-// footprints) is copied into the lane's column of an LDS tile (x and y, 64
-// lanes wide) and tested from there: its edges are re-read for every right
-// edge near it.  Other rows, and pairs where the right row is the smaller
-// ring, take polygons_intersect_auto; the boolean is the same.
-constexpr int kRingTile = 13;
+constexpr int kClsBuf = 1024;
+
+__global__ __launch_bounds__(256) void cand_classify_kernel(const uint32_t* __restrict__ cand_l,
+                                                           const int32_t* __restrict__ cand_r,
+                                                           JoinCtl* __restrict__ cctl, int64_t ccap,
+                                                           const int2* __restrict__ linfo,
+                                                           const double2* __restrict__ lfirst,
+                                                           const int2* __restrict__ rinfo,
+                                                           const double4* __restrict__ lbox,
+                                                           const double* __restrict__ ax,
+                                                           const double* __restrict__ ay, RowGrids G,
+                                                           int use_grid, int by_work, int rbits,
+                                                           uint32_t* __restrict__ ukey, int32_t* __restrict__ uval,
+                                                           uint32_t* __restrict__ out_l, int32_t* __restrict__ out_r,
+                                                           int64_t capacity, JoinCtl* __restrict__ ctl) {
+    __shared__ uint2 tbuf[kClsBuf];  // decided true: (left, right)
+    __shared__ uint2 ubuf[kClsBuf];  // undecided: (key, left | flag)
+    __shared__ unsigned int tn, un;
+    __shared__ unsigned long long gbase;
+    const unsigned long long nc = cctl->staged;
+    if (nc > (unsigned long long)ccap) return;  // grid-uniform
+    if (threadIdx.x == 0) {
+        tn = 0u;
+        un = 0u;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt();
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    unsigned long long cverts = 0;
+    // flush the cnt entries of buf to (a, b) at a slot block from `counter`, bounded by cap
+    auto flush = [&](uint2* buf, unsigned int cnt, unsigned int* n, unsigned long long* counter, uint32_t* a,
+                     int32_t* b, int64_t cap) {
+        if (threadIdx.x == 0) gbase = atomicAdd(counter, (unsigned long long)cnt);
+        __syncthreads();
+        const unsigned long long base = gbase;
+        for (unsigned int k = threadIdx.x; k < cnt; k += 256) {
+            const unsigned long long slot = base + k;
+            if (slot < (unsigned long long)cap) {
+                const uint2 v = buf[k];
+                a[slot] = v.x;
+                b[slot] = (int32_t)v.y;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) *n = 0u;
+        __syncthreads();
+    };
+    for (int64_t b0 = (int64_t)blockIdx.x * 256; b0 < (int64_t)nc; b0 += stride) {  // block-uniform
+        const int64_t i = b0 + threadIdx.x;
+        int dec = -1;  // -1 no candidate, 0 apart, 1 intersects, 2 / 3 undecided
+        uint32_t l = 0;
+        int32_t r = 0, vb = 0;
+        if (i < (int64_t)nc) {
+            l = cand_l[i];
+            r = cand_r[i];
+            const int2 li = linfo[l], ri = rinfo[r];
+            vb = ri.y;
+            cverts += (unsigned long long)(li.y + vb);
+            dec = 3;
+            if (use_grid) {
+                // every load that depends only on (l, r) issued together
+                const double4 gg = G.geo[r];
+                const double2 gi = G.inv[r];
+                const double2 f = lfirst[l];
+                const double4 ab = lbox[l];
+                if (li.x >= 0 && li.y <= kRingTile && li.y <= vb && gg.z > 0.0 && fabs(f.x) < INFINITY &&
+                    fabs(f.y) < INFINITY)
+                    dec = grid_decide_wide(GridAxes{gg.x, gg.y, gg.z, gg.w, gi.x, gi.y}, G.cells + (int64_t)r * kGrid,
+                                           ab, ax + li.x, ay + li.x, li.y);
+            }
+        }
+        const uint64_t tm = __ballot(dec == 1), um = __ballot(dec >= 2);
+        unsigned int tb = 0u, ub = 0u;
+        if (lane == 0) {
+            if (tm) tb = atomicAdd(&tn, (unsigned int)__popcll(tm));
+            if (um) ub = atomicAdd(&un, (unsigned int)__popcll(um));
+        }
+        tb = __shfl(tb, 0, 64);
+        ub = __shfl(ub, 0, 64);
+        if (dec == 1) tbuf[tb + (unsigned int)__popcll(tm & lt)] = make_uint2(l, (uint32_t)r);
+        if (dec >= 2) {
+            uint32_t key = (uint32_t)r;
+            if (by_work) {
+                const int lg = vb > 1 ? 31 - __clz(vb) : 0;
+                key |= ((uint32_t)(15 - (lg < 15 ? lg : 15)) << (rbits + 1)) | ((dec == 3 ? 1u : 0u) << rbits);
+            }
+            ubuf[ub + (unsigned int)__popcll(um & lt)] = make_uint2(key, l | (dec == 2 ? 0x80000000u : 0u));
+        }
+        __syncthreads();
+        const unsigned int tcount = tn, ucount = un;  // block-uniform: read between two barriers
+        __syncthreads();
+        const bool last = b0 + stride >= (int64_t)nc;  // each step adds at most 256 entries per list
+        if (tcount > (unsigned int)(kClsBuf - 256) || (last && tcount > 0u))
+            flush(tbuf, tcount, &tn, &ctl->staged, out_l, out_r, capacity);
+        if (ucount > (unsigned int)(kClsBuf - 256) || (last && ucount > 0u))
+            flush(ubuf, ucount, &un, &cctl->stat[0], ukey, uval, ccap);
+    }
+    for (int o = 32; o > 0; o >>= 1) cverts += __shfl_xor(cverts, o, 64);
+    if (lane == 0 && cverts) atomicAdd(&ctl->stat[1], cverts);
+}
+
+// One thread per candidate left by cand_classify_kernel (count in *count,
+// at most ccap): exact intersects; survivors appended to the finish stage
+// with one atomic per wave.  A left row that is one ring of at most
+// kRingTile coordinates (the footprints) is copied into the lane's column of
+// an LDS tile (x and y, 64 lanes wide) and tested from there: its edges are
+// re-read for every right edge near it.  Other rows, and pairs where the
+// right row is the smaller ring, take polygons_intersect_auto; the boolean is
+// the same.  Bit 31 of a left id: the first step of the tile path (locating
+// the left ring's first vertex in the right row) is skipped (grid_decide's
+// outcome 2: its answer is "outside, off the boundary" unless the rings'
+// boundaries touch, which the second step finds).
 constexpr int64_t kRingTileBytesPerWave = 2 * kRingTile * 64 * (int64_t)sizeof(double);
 __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __restrict__ cand_l,
                                                           const int32_t* __restrict__ cand_r, uint32_t rmask,
-                                                          const JoinCtl* __restrict__ cctl, int64_t ccap, Polys A,
-                                                          Polys B, RowSlabs RS, uint32_t* __restrict__ out_l,
-                                                          int32_t* __restrict__ out_r, int64_t capacity,
-                                                          JoinCtl* __restrict__ ctl) {
+                                                          const unsigned long long* __restrict__ count,
+                                                          int64_t ccap, Polys A, Polys B,
+                                                          const int2* __restrict__ linfo,
+                                                          const int2* __restrict__ rinfo, RowSlabs RS,
+                                                          uint32_t* __restrict__ out_l, int32_t* __restrict__ out_r,
+                                                          int64_t capacity, JoinCtl* __restrict__ ctl) {
     extern __shared__ double ring_tile[];
-    const unsigned long long nc = cctl->staged;
+    const unsigned long long nc = *count;
     if (nc > (unsigned long long)ccap) return;
     const int lane = threadIdx.x & 63;
     double* tx = ring_tile + (threadIdx.x >> 6) * (2 * kRingTile * 64) + lane;
     double* ty = tx + kRingTile * 64;
     const uint64_t lt = lanemask_lt();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    unsigned long long cverts = 0;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < (int64_t)nc; i0 += stride) {
         const int64_t i = i0 + lane;
         bool keep = false;
         uint32_t l = 0;
         int32_t r = 0;
         if (i < (int64_t)nc) {
-            l = cand_l[i];
+            const uint32_t lv = cand_l[i];
+            const bool skip_first = (lv >> 31) != 0u;
+            l = lv & 0x7FFFFFFFu;
             r = (int32_t)((uint32_t)cand_r[i] & rmask);
-            cverts += (unsigned long long)(row_coords(A, l) + row_coords(B, r));
-            int32_t ca = 0, na = 0, cb = 0, nb = 0;
+            const int2 li = linfo[l], ri = rinfo[r];
+            const int32_t ca = li.x, na = li.y, cb = ri.x, nb = ri.y;
             const int nslab = RS.nslab[r];
-            const bool tile_a = single_ring_of(A, l, ca, na) && na <= kRingTile;
-            if (tile_a && (nslab > 0 || !(single_ring_of(B, r, cb, nb) && nb < na))) {
+            const bool tile_a = ca >= 0 && na <= kRingTile;
+            if (tile_a && (nslab > 0 || !(cb >= 0 && nb < na))) {
                 for (int32_t k = 0; k < na; ++k) {  // the lane's own column: no other lane reads it
                     tx[k * 64] = A.x[ca + k];
                     ty[k * 64] = A.y[ca + k];
                 }
                 if (nslab > 0) {  // one ring of more than kRingTile coordinates, indexed
-                    single_ring_of(B, r, cb, nb);
                     keep = ring_intersects_slabbed(RingRef{tx, ty, 64}, na, B.x, B.y, cb,
-                                                   RS.list + row_slab_base(cb), nslab, RS.geo[r]);
+                                                   RS.list + row_slab_base(cb), nslab, RS.geo[r], skip_first);
                 } else {
-                    keep = ring_intersects_polys(RingRef{tx, ty, 64}, na, B, r);
+                    keep = ring_intersects_polys(RingRef{tx, ty, 64}, na, B, r, skip_first);
                 }
             } else {
                 keep = polygons_intersect_auto(A, l, B, r);
@@ -528,19 +791,19 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) cverts += __shfl_xor(cverts, o, 64);
-    if (lane == 0 && cverts) atomicAdd(&ctl->stat[1], cverts);
 }
 
 // d_result: [0] pairs K, [1] bbox candidates staged (the stage that bounds
 // `capacity`: survivors are a subset), [2] bbox candidates, [3] sum over the
-// candidates of both rows' coordinates (0 when the candidates overflowed).
+// candidates of both rows' coordinates, [4] candidates left to the exact
+// refine ([3] and [4] are 0 when the candidates overflowed).
 __global__ void publish_batch_kernel(const JoinCtl* __restrict__ cctl, const JoinCtl* __restrict__ ctl,
                                      int64_t* __restrict__ result) {
     if (threadIdx.x == 0) {
         result[1] = (int64_t)cctl->staged;
         result[2] = (int64_t)cctl->staged;
         result[3] = (int64_t)ctl->stat[1];
+        result[4] = (int64_t)cctl->stat[0];
     }
 }
 
@@ -564,9 +827,9 @@ struct BatchWs {
     double4* fbox;
     uint32_t* cand_l;
     int32_t* cand_r;
-    uint32_t* cs_r;  // candidates sorted by right row (or by work bin, then right row)
-    int32_t* cs_l;
-    uint32_t* ckey;  // work-bin keys
+    uint32_t* cs_r;  // undecided candidates' sort keys, sorted
+    int32_t* cs_l;   // undecided candidates' left ids (bit 31: first step answered)
+    uint32_t* ckey;  // undecided candidates' sort keys, in classify order
     void* csort_ws;
     JoinCtl* cctl;
     JoinCtl* ctl;
@@ -576,6 +839,12 @@ struct BatchWs {
     double2* slab_geo;
     int32_t* slab_n;
     uint16_t* slab_list;
+    double4* grid_geo;
+    double2* grid_inv;
+    uint2* grid_cells;
+    int2* linfo;
+    double2* lfirst;
+    int2* rinfo;
 };
 
 // Forest nodes are bounded by the items and their parents: < n_right * 16/15 + levels * jobs.
@@ -620,6 +889,12 @@ static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n
     W.slab_geo = c.take<double2>(n_right);
     W.slab_n = c.take<int32_t>(n_right);
     W.slab_list = c.take<uint16_t>(3 * n_right_coords);
+    W.grid_geo = c.take<double4>(n_right);
+    W.grid_inv = c.take<double2>(n_right);
+    W.grid_cells = c.take<uint2>(n_right * kGrid);
+    W.linfo = c.take<int2>(n_left);
+    W.lfirst = c.take<double2>(n_left);
+    W.rinfo = c.take<int2>(n_right);
     if (bytes) *bytes = c.bytes;
     return W;
 }
@@ -639,7 +914,8 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
     const int waves_per_block = (tile & 0xFF) ? (tile & 0xFF) : 4;
     const int refine_threads = ((tile >> 8) & 0xFFFF) ? ((tile >> 8) & 0xFFFF) : 256;
     const int rbits = bits_for_jobs(n_right);
-    const bool by_work = ((tile >> 24) & 1) != 0 && rbits <= 28;
+    const bool by_work = ((tile >> 24) & 1) != 0 && rbits <= 27;
+    const bool use_grid = ((tile >> 25) & 1) == 0;
     VS_REQUIRE(waves_per_block >= 1 && waves_per_block <= 4, "batched join: tile waves per block in [1, 4]");
     VS_REQUIRE(refine_threads % 64 == 0 && refine_threads <= 256, "batched join: refine block in {64,..,256}");
     int dev = 0, cus = 256;
@@ -650,6 +926,12 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
     if (n_jobs > 0 && n_left > 0 && n_right > 0) {
         VS_TRY(polygon_bounds(A.x, A.y, A.geom, A.part, A.ring, n_left, reinterpret_cast<double*>(W.lbox), s));
         VS_TRY(polygon_bounds(B.x, B.y, B.geom, B.part, B.ring, n_right, reinterpret_cast<double*>(W.rbox), s));
+        hipLaunchKernelGGL(row_info_kernel, dim3(stride_grid(n_left, 256, 8)), dim3(256), 0, s, A, n_left, W.linfo,
+                           W.lfirst);
+        VS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(row_info_kernel, dim3(stride_grid(n_right, 256, 8)), dim3(256), 0, s, B, n_right, W.rinfo,
+                           static_cast<double2*>(nullptr));
+        VS_LAUNCH_CHECK();
         const int jb = bits_for_jobs(n_jobs);
         // right rows: (job, Hilbert) order, forest
         VS_TRY(centre_extent(W.rbox, n_right, W.ext, W.ext_ws, s));
@@ -664,8 +946,14 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         VS_TRY(exclusive_scan_i32(W.waves, W.wave_off, n_jobs, W.total_waves, W.scan_ws, s));
         hipLaunchKernelGGL(row_slab_build_kernel, dim3((unsigned)std::min<int64_t>(n_right, (int64_t)cus * 32)),
                            dim3(64), 0, s, B, n_right, W.rbox, W.slab_geo, W.slab_n, W.slab_list,
-                           3 * n_right_coords);
+                           3 * n_right_coords, kRowSlabMinEdges[(tile >> 26) & 3]);
         VS_LAUNCH_CHECK();
+        if (use_grid) {
+            hipLaunchKernelGGL(row_grid_build_kernel,
+                               dim3((unsigned)std::min<int64_t>(ceil_div(n_right, kGridRowsPerBlock), (int64_t)cus * 8)),
+                               dim3(256), 0, s, B, n_right, W.rinfo, W.rbox, W.grid_geo, W.grid_inv, W.grid_cells);
+            VS_LAUNCH_CHECK();
+        }
         hipLaunchKernelGGL(forest_build_kernel, dim3((unsigned)std::min<int64_t>(n_jobs, 65535)), dim3(256), 0, s,
                            W.rbox, W.rperm, right_job, W.tree_off, n_jobs, W.fbox);
         VS_LAUNCH_CHECK();
@@ -680,29 +968,27 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
                            W.lperm, left_job, right_job, n_jobs, W.wave_off, W.total_waves, W.fbox, W.tree_off,
                            W.rperm, W.cand_l, W.cand_r, capacity, W.cctl);
         VS_LAUNCH_CHECK();
-        // candidates grouped by right row: the lanes of a wave then share
-        // parcels (same coordinates, same edge count); with the work order,
-        // the largest parcels' candidates first
-        if (by_work) {
-            hipLaunchKernelGGL(cand_cost_keys_kernel, dim3(stride_grid(capacity, 256, 8)), dim3(256), 0, s, W.cand_r,
-                               W.cctl, capacity, B, rbits, W.ckey);
-            VS_LAUNCH_CHECK();
-            VS_TRY(sort_pairs_into_u32(W.ckey, reinterpret_cast<const int32_t*>(W.cand_l), W.cs_r, W.cs_l, capacity,
-                                       0, rbits + 4, W.csort_ws, sort_workspace_bytes_u32(capacity), s,
-                                       &W.cctl->staged));
-        } else {
-            VS_TRY(sort_pairs_into_u32(reinterpret_cast<const uint32_t*>(W.cand_r),
-                                       reinterpret_cast<const int32_t*>(W.cand_l), W.cs_r, W.cs_l, capacity, 0, rbits,
-                                       W.csort_ws, sort_workspace_bytes_u32(capacity), s, &W.cctl->staged));
-        }
+        // candidates decided from the cell grids where possible; the rest
+        // grouped by right row (the lanes of a wave then share parcels: same
+        // coordinates, same edge count), with the work order the largest
+        // parcels' candidates first
+        hipLaunchKernelGGL(cand_classify_kernel, dim3(stride_grid(capacity, 256, 8)), dim3(256), 0, s,
+                           reinterpret_cast<const uint32_t*>(W.cand_l), W.cand_r, W.cctl, capacity, W.linfo,
+                           W.lfirst, W.rinfo, W.lbox, A.x, A.y,
+                           RowGrids{W.grid_geo, W.grid_inv, W.grid_cells}, use_grid ? 1 : 0, by_work ? 1 : 0, rbits,
+                           W.ckey, W.cs_l, W.stage_l, W.stage_r, capacity, W.ctl);
+        VS_LAUNCH_CHECK();
+        VS_TRY(sort_pairs_into_u32(W.ckey, W.cs_l, W.cs_r, reinterpret_cast<int32_t*>(W.cand_l), capacity, 0,
+                                   by_work ? rbits + 5 : rbits, W.csort_ws, sort_workspace_bytes_u32(capacity), s,
+                                   &W.cctl->stat[0]));
         const uint32_t rmask = by_work ? (uint32_t)((1ull << rbits) - 1ull) : 0xFFFFFFFFu;
         const unsigned rgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, refine_threads),
                                                                                   (int64_t)cus * 16));
         hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads),
-                           (unsigned)(refine_threads / 64 * kRingTileBytesPerWave), s,
-                           reinterpret_cast<const uint32_t*>(W.cs_l), reinterpret_cast<const int32_t*>(W.cs_r),
-                           rmask, W.cctl, capacity, A, B, RowSlabs{W.slab_geo, W.slab_n, W.slab_list}, W.stage_l,
-                           W.stage_r, capacity, W.ctl);
+                           (unsigned)(refine_threads / 64 * kRingTileBytesPerWave), s, W.cand_l,
+                           reinterpret_cast<const int32_t*>(W.cs_r), rmask, &W.cctl->stat[0], capacity, A, B,
+                           W.linfo, W.rinfo, RowSlabs{W.slab_geo, W.slab_n, W.slab_list}, W.stage_l, W.stage_r,
+                           capacity, W.ctl);
         VS_LAUNCH_CHECK();
         VS_TRY(timing_end_public(s));
     }
@@ -715,6 +1001,46 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
 }  // namespace vs
 
 extern "C" {
+
+// The cell grid of one ring on the host, built as the device builds it.
+int vs_host_cell_grid(const double* x, const double* y, int64_t n, const double* box4, double* geo6,
+                      uint32_t* cells64) {
+    vs::clear_error();
+    VS_REQUIRE(x && y && box4 && geo6 && cells64, "vs_host_cell_grid: null argument");
+    VS_REQUIRE(n >= 0, "vs_host_cell_grid: negative size");
+    vs::GridAxes g;
+    uint2 cells[vs::kGrid];
+    if (!vs::grid_build_serial(x, y, n, box4[0], box4[1], box4[2], box4[3], g, cells)) return 0;
+    const double v[6] = {g.x0, g.y0, g.wx, g.wy, g.ix, g.iy};
+    for (int i = 0; i < 6; ++i) geo6[i] = v[i];
+    for (int j = 0; j < vs::kGrid; ++j) {
+        cells64[2 * j] = cells[j].x;
+        cells64[2 * j + 1] = cells[j].y;
+    }
+    return 1;
+}
+
+// grid_decide on the host for ring (x, y, n) with bounds box4: 1 intersects,
+// 0 apart, 2 undecided with the exact test's first step skippable, 3 undecided.
+int vs_host_cell_decide(const double* geo6, const uint32_t* cells64, const double* box4, const double* x,
+                        const double* y, int32_t n) {
+    vs::clear_error();
+    VS_REQUIRE(geo6 && cells64 && box4 && x && y && n >= 1, "vs_host_cell_decide: bad argument");
+    const vs::GridAxes g{geo6[0], geo6[1], geo6[2], geo6[3], geo6[4], geo6[5]};
+    uint2 cells[vs::kGrid];
+    for (int j = 0; j < vs::kGrid; ++j) cells[j] = make_uint2(cells64[2 * j], cells64[2 * j + 1]);
+    return vs::grid_decide(g, cells, box4[0], box4[1], box4[2], box4[3], x[0], y[0], x, y, n);
+}
+
+// cell_lo / cell_hi of v along axis 0 (x) or 1 (y) of a grid (geo6 as above).
+int vs_host_cell_span(const double* geo6, int32_t axis, double v, int32_t* lo_hi) {
+    vs::clear_error();
+    VS_REQUIRE(geo6 && lo_hi && (axis == 0 || axis == 1), "vs_host_cell_span: bad argument");
+    const double base = geo6[axis], w = geo6[2 + axis], inv = geo6[4 + axis];
+    lo_hi[0] = vs::cell_lo(v, base, w, inv);
+    lo_hi[1] = vs::cell_hi(v, base, w, inv);
+    return VS_OK;
+}
 
 int64_t vs_batched_intersects_workspace_bytes(int64_t n_jobs, int64_t n_left, int64_t n_right, int64_t n_right_coords,
                                               int64_t capacity) {

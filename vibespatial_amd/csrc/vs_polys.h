@@ -82,7 +82,11 @@ struct RingRef {
     __device__ __forceinline__ double Y(int32_t k) const { return y[k * stride]; }
 };
 
-__device__ inline bool ring_intersects_polys(const RingRef& A, int32_t na, const Polys& B, int64_t rb) {
+// skip_first: A's first vertex is known to lie outside every part of B and
+// off its boundary (pass 1's outcome apart from `near`), so pass 1 is not
+// run and pass 2 tests every edge's box.
+__device__ inline bool ring_intersects_polys(const RingRef& A, int32_t na, const Polys& B, int64_t rb,
+                                             bool skip_first = false) {
     const int32_t bp0 = B.geom[rb], bp1 = B.geom[rb + 1];
     if (B.ring[B.part[bp1]] <= B.ring[B.part[bp0]]) return false;
     double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
@@ -95,8 +99,8 @@ __device__ inline bool ring_intersects_polys(const RingRef& A, int32_t na, const
     // Pass 1: location of A's first vertex in each part of B (inside or on
     // the boundary decides at once), noting whether any B edge comes near A.
     const double fx = A.X(0), fy = A.Y(0);
-    bool near = false;
-    for (int32_t bp = bp0; bp < bp1; ++bp) {
+    bool near = skip_first;
+    for (int32_t bp = skip_first ? bp1 : bp0; bp < bp1; ++bp) {
         int parity = 0;
         for (int32_t br = B.part[bp]; br < B.part[bp + 1]; ++br) {
             const int32_t e0 = B.ring[br], e1 = B.ring[br + 1];

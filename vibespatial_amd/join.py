@@ -42,6 +42,7 @@ class JoinStats:
     capacity: int
     retries: int
     candidate_vertices: int = 0
+    refined: int = -1  # batched join: candidates left to the exact refine after the cell-grid decisions
 
 
 class DeviceTree:
@@ -345,9 +346,9 @@ def sjoin_points_polygons(Q: DevicePoints, P: DevicePolygons, tree: DeviceTree, 
             ),
             "vs_sjoin_points_polygons",
         )
-        k, staged, cand, cverts = _read_result(res)
+        k, staged, cand, cverts, refined = res.cpu().tolist()
         if staged <= cap:
-            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts)
+            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts, refined)
         if staged >= 2**31 - 1:
             raise OverflowError(f"{staged} pairs exceed the int32 pair limit")
         cap = staged
@@ -634,8 +635,9 @@ def mlp_loss_grad(params: torch.Tensor, u: torch.Tensor, t: torch.Tensor):
 # ---------------------------------------------------------------------------
 # batched polygon x polygon intersects
 # ---------------------------------------------------------------------------
-def _encode_tile(waves_per_block: int, refine_threads: int, order: int = 0) -> int:
-    return int(waves_per_block) | (int(refine_threads) << 8) | (int(order) << 24)
+def _encode_tile(waves_per_block: int, refine_threads: int, order: int = 0, slab: int = 0) -> int:
+    # slab: y-slab index for right rings of 128 (0), 16 (1) or 64 (2) edges and more, or none (3)
+    return int(waves_per_block) | (int(refine_threads) << 8) | (int(order) << 24) | (int(slab) << 26)
 
 
 # (traversal waves per block, refine block size, candidate order: 0 by right
@@ -726,11 +728,12 @@ def _device_name() -> str:
 
 
 def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Tensor, right_job: torch.Tensor,
-                       capacity: int | None = None, tile: tuple[int, int] | None = None):
+                       capacity: int | None = None, tile: tuple[int, int] | None = None, cell_grid: bool = True):
     """Many independent polygon x polygon sjoin(intersects) jobs in one launch
     sequence.  left_job / right_job: device int64 row offsets (n_jobs + 1).
     Returns (left, right, stats) with global row ids, lexsorted; job j's pairs
-    are the contiguous run with left in [left_job[j], left_job[j+1])."""
+    are the contiguous run with left in [left_job[j], left_job[j+1]).
+    cell_grid=False refines every bbox candidate exactly (same pairs)."""
     dev = require_gpu()
     L = lib()
     n_jobs = int(left_job.numel()) - 1
@@ -740,7 +743,7 @@ def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Ten
     rj = right_job.to(device=dev, dtype=torch.int64).contiguous()
     if tile is None:
         tile = _TILE_CACHE.get(_device_name(), _tile_key(n_jobs, len(A), len(B)), (4, 256, 1))
-    code = _encode_tile(*tile)
+    code = _encode_tile(*tile) | (0 if cell_grid else 1 << 25)
     cap = int(capacity) if capacity is not None else max(1024, 4 * len(A))
     cap = min(cap, PAIR_LIMIT)
     retries = 0
@@ -748,16 +751,16 @@ def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Ten
         ws = _ws(L.vs_batched_intersects_workspace_bytes(n_jobs, len(A), len(B), B.n_coords, cap), dev)
         out_l = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
         out_r = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
-        res = torch.zeros(4, dtype=torch.int64, device=dev)
+        res = torch.zeros(5, dtype=torch.int64, device=dev)
         check(L.vs_batched_intersects(ptr(A.x), ptr(A.y), ptr(A.geom_offsets), ptr(A.part_offsets),
                                       ptr(A.ring_offsets), len(A), ptr(B.x), ptr(B.y), ptr(B.geom_offsets),
                                       ptr(B.part_offsets), ptr(B.ring_offsets), len(B), B.n_coords, ptr(lj), ptr(rj),
                                       n_jobs, code, cap, ptr(out_l), ptr(out_r), ptr(res), ptr(ws), ws.numel(),
                                       stream()),
               "vs_batched_intersects")
-        k, staged, cand, cverts = _read_result(res)
+        k, staged, cand, cverts, refined = res.cpu().tolist()
         if staged <= cap:
-            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts)
+            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts, refined)
         if staged > PAIR_LIMIT:
             raise OverflowError(f"{staged} bbox candidates exceed the int32 slot limit of one call")
         del ws, out_l, out_r
@@ -797,7 +800,7 @@ class BatchedIntersectsGraph:
     that overflows raises (capture again with a larger capacity)."""
 
     def __init__(self, A: DevicePolygons, B: DevicePolygons, left_job: torch.Tensor, right_job: torch.Tensor,
-                 capacity: int, tile: tuple[int, int] | None = None):
+                 capacity: int, tile: tuple[int, int] | None = None, cell_grid: bool = True):
         dev = require_gpu()
         L = lib()
         self.n_jobs = int(left_job.numel()) - 1
@@ -807,12 +810,12 @@ class BatchedIntersectsGraph:
         self.cap = min(int(capacity), PAIR_LIMIT)
         if tile is None:
             tile = _TILE_CACHE.get(_device_name(), _tile_key(self.n_jobs, len(A), len(B)), (4, 256, 1))
-        self.code = _encode_tile(*tile)
+        self.code = _encode_tile(*tile) | (0 if cell_grid else 1 << 25)
         self.ws = _ws(L.vs_batched_intersects_workspace_bytes(self.n_jobs, len(A), len(B), B.n_coords, self.cap),
                       dev)
         self.out_l = torch.empty(max(self.cap, 1), dtype=torch.int64, device=dev)
         self.out_r = torch.empty(max(self.cap, 1), dtype=torch.int64, device=dev)
-        self.res = torch.zeros(4, dtype=torch.int64, device=dev)
+        self.res = torch.zeros(5, dtype=torch.int64, device=dev)
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
@@ -836,7 +839,7 @@ class BatchedIntersectsGraph:
     def replay(self):
         """(left, right, stats) views of the graph's output buffers."""
         self.graph.replay()
-        k, staged, cand, cverts = _read_result(self.res)
+        k, staged, cand, cverts, refined = self.res.cpu().tolist()
         if staged > self.cap:
             raise OverflowError(f"{staged} bbox candidates exceed the captured capacity {self.cap}")
-        return self.out_l[:k], self.out_r[:k], JoinStats(k, staged, cand, self.cap, 0, cverts)
+        return self.out_l[:k], self.out_r[:k], JoinStats(k, staged, cand, self.cap, 0, cverts, refined)
