@@ -620,12 +620,11 @@ int vs_host_cover_ranges(double x0, double y0, double x1, double y1, const doubl
 int vs_host_cell_grid(const double* x, const double* y, int64_t n, const double* box4, double* geo6,
                       uint32_t* cells64);
 
-/* The batched join's decision for a left ring (x, y: its n >= 1 finite-first
- * coordinates; box4: its bounds) against such a grid: 1 intersects, 0 does
- * not, 2 undecided with the exact test's first step (the left first vertex
- * located in the right ring) skippable, 3 undecided. */
-int vs_host_cell_decide(const double* geo6, const uint32_t* cells64, const double* box4, const double* x,
-                        const double* y, int32_t n);
+/* The batched join's decision for a left ring (x, y: its n >= 1
+ * coordinates) against such a grid: 1 intersects, 0 does not, 2 undecided with the exact
+ * test's first step (the left first vertex located in the right ring)
+ * skippable, 3 undecided (also for a non-finite coordinate). */
+int vs_host_cell_decide(const double* geo6, const uint32_t* cells64, const double* x, const double* y, int32_t n);
 
 /* The lowest and highest grid cell (0..31) whose closed range holds v along
  * axis 0 (x) or 1 (y) of such a grid, into lo_hi[2]. */

@@ -36,10 +36,9 @@ def _grid(x, y):
     return ok, geo, cells
 
 
-def _decide(geo, cells, box, x, y):
-    box = np.ascontiguousarray(box, np.float64)
+def _decide(geo, cells, x, y):
     x, y = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64)
-    return _lib.lib().vs_host_cell_decide(_ptr(geo), _ptr(cells), _ptr(box), _ptr(x), _ptr(y), len(x))
+    return _lib.lib().vs_host_cell_decide(_ptr(geo), _ptr(cells), _ptr(x), _ptr(y), len(x))
 
 
 def _boundaries_touch(ax, ay, bx, by) -> bool:
@@ -196,8 +195,7 @@ def test_grid_decisions_are_the_exact_predicate(case):
     seen = {0: 0, 1: 0, 2: 0, 3: 0}
     for fx, fy in _footprints(rng, x, y, 300):
         A = _ring_array(fx, fy)
-        box = np.array([fx.min(), fy.min(), fx.max(), fy.max()])
-        d = _decide(geo, cells, box, fx, fy)
+        d = _decide(geo, cells, fx, fy)
         seen[d] += 1
         exact = oracle.polygon_intersects(A, 0, B, 0)
         if d in (0, 1):

@@ -149,7 +149,7 @@ SIGNATURES = {
     "vs_host_hilbert64": (_u64, [_u32, _u32]),
     "vs_host_cover_ranges": (ctypes.c_int, [_d, _d, _d, _d, _p, _i32, _p]),
     "vs_host_cell_grid": (ctypes.c_int, [_p, _p, _i64, _p, _p, _p]),
-    "vs_host_cell_decide": (ctypes.c_int, [_p, _p, _p, _p, _p, _i32]),
+    "vs_host_cell_decide": (ctypes.c_int, [_p, _p, _p, _p, _i32]),
     "vs_host_cell_span": (ctypes.c_int, [_p, _i32, _d, _p]),
 }
 

@@ -34,8 +34,8 @@
 
 namespace vs {
 
-int polygon_bounds(const double* x, const double* y, const int32_t* geom, const int32_t* part, const int32_t* ring,
-                   int64_t n, double* out, hipStream_t s);
+int polygon_bounds_info(const double* x, const double* y, const int32_t* geom, const int32_t* part,
+                        const int32_t* ring, int64_t n, double* bounds, int2* info, double2* first, hipStream_t s);
 int centre_extent(const double4* b, int64_t n, double4* ext, void* ws, hipStream_t s);
 int64_t extent_workspace_bytes();
 int sort_pairs_u64(uint64_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
@@ -369,22 +369,6 @@ __global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_r
     }
 }
 
-// Per row: info = (first coordinate c0 when the row is one ring of at least
-// one coordinate, else -1; coordinates of the row over all rings) and, when
-// first != nullptr, the ring's first vertex.  Later kernels read these two
-// independent words instead of walking geom -> part -> ring per pair.
-__global__ __launch_bounds__(256) void row_info_kernel(Polys P, int64_t n_rows, int2* __restrict__ info,
-                                                      double2* __restrict__ first) {
-    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n_rows; r += (int64_t)gridDim.x * 256) {
-        const int32_t p0 = P.geom[r], p1 = P.geom[r + 1];
-        const int32_t q0 = P.part[p0], q1 = P.part[p1];
-        const int32_t c0 = P.ring[q0], c1 = P.ring[q1];
-        const bool single = p1 - p0 == 1 && q1 - q0 == 1 && c1 > c0;
-        info[r] = make_int2(single ? c0 : -1, c1 - c0);
-        if (first) first[r] = single ? make_double2(P.x[c0], P.y[c0]) : make_double2(NAN, NAN);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Cell grids of the right rows (vs_grid.h): 16 lanes per row, 16 rows per
 // 256-thread block (grid-stride, block-uniform trip count).  A row gets no
@@ -416,13 +400,10 @@ __global__ __launch_bounds__(256) void row_grid_build_kernel(Polys B, int64_t n_
         bool ok = false;
         if (r < n_rows) {
             const int2 ri = rinfo[r];
+            const double4 bb = rbox[r];
             c0 = ri.x;
             n = ri.y;
-            ok = c0 >= 0 && n >= 4;
-        }
-        if (ok) {
-            const double4 bb = rbox[r];
-            ok = grid_setup(bb.x, bb.y, bb.z, bb.w, g) && B.x[c0] == B.x[c0 + n - 1] && B.y[c0] == B.y[c0 + n - 1];
+            ok = c0 >= 0 && n >= 4 && grid_setup(bb.x, bb.y, bb.z, bb.w, g);
         }
         bnd[sub] = 0u;
         bnd[sub + 16] = 0u;
@@ -431,8 +412,10 @@ __global__ __launch_bounds__(256) void row_grid_build_kernel(Polys B, int64_t n_
         __syncthreads();
         bool bad = false;
         if (ok) {
+            // lane 0 also checks that the ring is closed (first == last), with its first edge's loads
             for (int32_t e = 1 + sub; e < n; e += 16) {
                 const double ax = B.x[c0 + e - 1], ay = B.y[c0 + e - 1], bx = B.x[c0 + e], by = B.y[c0 + e];
+                if (e == 1) bad = bad || !(ax == B.x[c0 + n - 1] && ay == B.y[c0 + n - 1]);
                 if (!(fabs(ax) < INFINITY && fabs(ay) < INFINITY && fabs(bx) < INFINITY && fabs(by) < INFINITY)) {
                     bad = true;
                     continue;
@@ -545,11 +528,11 @@ __device__ inline bool ring_intersects_slabbed(const RingRef& A, int32_t na, con
 constexpr int kRingTile = 13;  // left rings of at most this many coordinates are refined from LDS
 
 // One thread per bbox candidate, in the traversal's order.  A pair whose
-// left row is one ring of at most kRingTile coordinates (with a finite first
-// vertex and no more coordinates than the right row) and whose right row has
-// a cell grid is decided from the grid where it can be (grid_decide: the left
-// box first, then the left vertices): pairs that intersect are appended to
-// the finish stage, pairs that do not are dropped.  Every other pair is appended to (key, left id) for the sort that
+// left row is one ring of at most kRingTile coordinates (no more than the
+// right row's) and whose right row has a cell grid is decided from the grid
+// where it can be (grid_decide: the left ring's box first, then its
+// vertices): pairs that intersect are appended to the finish stage, pairs
+// that do not are dropped.  Every other pair is appended to (key, left id) for the sort that
 // orders the refine: key = the right row, or with by_work
 // (15 - min(15, floor(log2 coords of the right row))) << (rbits + 1) |
 // (first step still to run) << rbits | right row, so the candidates of the
@@ -561,64 +544,6 @@ constexpr int kRingTile = 13;  // left rings of at most this many coordinates ar
 // block in LDS and appended with one global atomic per kClsBuf - 256 or more
 // entries (one atomic per wave on the two counters serialised the kernel).
 // ctl->stat[1] accumulates, over all candidates, the coordinates of both rows.
-// grid_decide with its loads issued in groups: the first eight grid rows of
-// the left box at once (further rows, rare, one by one), then the left
-// vertices eight at a time; each vertex's row is one of the box's rows (the
-// vertex lies in the box), read again through the cache.  The same decisions
-// as grid_decide, which the host tests check.
-__device__ inline int grid_decide_wide(const GridAxes& g, const uint2* __restrict__ cells, double4 ab,
-                                       const double* __restrict__ x, const double* __restrict__ y, int n) {
-    const int i0 = cell_lo(ab.x, g.x0, g.wx, g.ix), i1 = cell_hi(ab.z, g.x0, g.wx, g.ix);
-    const int j0 = cell_lo(ab.y, g.y0, g.wy, g.iy), j1 = cell_hi(ab.w, g.y0, g.wy, g.iy);
-    const uint32_t m = span_mask(i0, i1);
-    uint2 rows[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) rows[u] = (j0 + u <= j1) ? cells[j0 + u] : make_uint2(0u, 0xFFFFFFFFu);
-    // rows past j1 read as free and inside: neutral for both tests below
-    uint32_t marks = 0u, all_in = m, any_in = 0u;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const uint2 c = (j0 + u <= j1) ? rows[u] : make_uint2(0u, m);
-        marks |= c.x & m;
-        all_in &= c.y;
-        any_in |= (j0 + u <= j1) ? (c.y & m) : 0u;
-    }
-    for (int j = j0 + 8; j <= j1; ++j) {
-        const uint2 c = cells[j];
-        marks |= c.x & m;
-        all_in &= c.y;
-        any_in |= c.y & m;
-    }
-    if (!marks) {
-        if (all_in == m) return 1;
-        if (any_in == 0u) return 0;
-    }
-    bool finite = true, in = false, out = false;
-    for (int k0 = 0; k0 < n; k0 += 8) {
-        double vx[8], vy[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int k = k0 + u < n ? k0 + u : n - 1;
-            vx[u] = x[k];
-            vy[u] = y[k];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            finite = finite && fabs(vx[u]) < INFINITY && fabs(vy[u]) < INFINITY;
-            if (!finite) break;
-            const int ci = cell_lo(vx[u], g.x0, g.wx, g.ix), cj = cell_lo(vy[u], g.y0, g.wy, g.iy);
-            const uint2 c = cells[cj];
-            if (!((c.x >> ci) & 1u)) {
-                if ((c.y >> ci) & 1u) in = true;
-                else out = true;
-            }
-        }
-        if (!finite) break;
-    }
-    if (!finite) return 3;
-    return in ? 1 : (out ? 2 : 3);
-}
-
 constexpr int kClsBuf = 1024;
 
 __global__ __launch_bounds__(256) void cand_classify_kernel(const uint32_t* __restrict__ cand_l,
@@ -685,10 +610,9 @@ __global__ __launch_bounds__(256) void cand_classify_kernel(const uint32_t* __re
                 const double2 gi = G.inv[r];
                 const double2 f = lfirst[l];
                 const double4 ab = lbox[l];
-                if (li.x >= 0 && li.y <= kRingTile && li.y <= vb && gg.z > 0.0 && fabs(f.x) < INFINITY &&
-                    fabs(f.y) < INFINITY)
-                    dec = grid_decide_wide(GridAxes{gg.x, gg.y, gg.z, gg.w, gi.x, gi.y}, G.cells + (int64_t)r * kGrid,
-                                           ab, ax + li.x, ay + li.x, li.y);
+                if (li.x >= 0 && li.y <= kRingTile && li.y <= vb && gg.z > 0.0)
+                    dec = grid_decide(GridAxes{gg.x, gg.y, gg.z, gg.w, gi.x, gi.y}, G.cells + (int64_t)r * kGrid,
+                                      ab.x, ab.y, ab.z, ab.w, f.x, f.y, ax + li.x, ay + li.x, li.y);
             }
         }
         const uint64_t tm = __ballot(dec == 1), um = __ballot(dec >= 2);
@@ -924,14 +848,11 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
     }
     if (n_jobs > 0 && n_left > 0 && n_right > 0) {
-        VS_TRY(polygon_bounds(A.x, A.y, A.geom, A.part, A.ring, n_left, reinterpret_cast<double*>(W.lbox), s));
-        VS_TRY(polygon_bounds(B.x, B.y, B.geom, B.part, B.ring, n_right, reinterpret_cast<double*>(W.rbox), s));
-        hipLaunchKernelGGL(row_info_kernel, dim3(stride_grid(n_left, 256, 8)), dim3(256), 0, s, A, n_left, W.linfo,
-                           W.lfirst);
-        VS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(row_info_kernel, dim3(stride_grid(n_right, 256, 8)), dim3(256), 0, s, B, n_right, W.rinfo,
-                           static_cast<double2*>(nullptr));
-        VS_LAUNCH_CHECK();
+        // bounds, ring info (first coordinate of a single ring, coordinates of the row), left first vertices
+        VS_TRY(polygon_bounds_info(A.x, A.y, A.geom, A.part, A.ring, n_left, reinterpret_cast<double*>(W.lbox),
+                                   W.linfo, W.lfirst, s));
+        VS_TRY(polygon_bounds_info(B.x, B.y, B.geom, B.part, B.ring, n_right, reinterpret_cast<double*>(W.rbox),
+                                   W.rinfo, static_cast<double2*>(nullptr), s));
         const int jb = bits_for_jobs(n_jobs);
         // right rows: (job, Hilbert) order, forest
         VS_TRY(centre_extent(W.rbox, n_right, W.ext, W.ext_ws, s));
@@ -1020,16 +941,24 @@ int vs_host_cell_grid(const double* x, const double* y, int64_t n, const double*
     return 1;
 }
 
-// grid_decide on the host for ring (x, y, n) with bounds box4: 1 intersects,
-// 0 apart, 2 undecided with the exact test's first step skippable, 3 undecided.
-int vs_host_cell_decide(const double* geo6, const uint32_t* cells64, const double* box4, const double* x,
-                        const double* y, int32_t n) {
+// grid_decide on the host for ring (x, y, n), n >= 1, with its box
+// computed as the device's bounds pass does (min / max, NaN ignored):
+// 1 intersects, 0 apart, 2 undecided with the exact test's first step
+// skippable, 3 undecided.
+int vs_host_cell_decide(const double* geo6, const uint32_t* cells64, const double* x, const double* y, int32_t n) {
     vs::clear_error();
-    VS_REQUIRE(geo6 && cells64 && box4 && x && y && n >= 1, "vs_host_cell_decide: bad argument");
+    VS_REQUIRE(geo6 && cells64 && x && y && n >= 1, "vs_host_cell_decide: bad argument");
     const vs::GridAxes g{geo6[0], geo6[1], geo6[2], geo6[3], geo6[4], geo6[5]};
     uint2 cells[vs::kGrid];
     for (int j = 0; j < vs::kGrid; ++j) cells[j] = make_uint2(cells64[2 * j], cells64[2 * j + 1]);
-    return vs::grid_decide(g, cells, box4[0], box4[1], box4[2], box4[3], x[0], y[0], x, y, n);
+    double b[4] = {x[0], y[0], x[0], y[0]};
+    for (int k = 1; k < n; ++k) {
+        b[0] = fmin(b[0], x[k]);
+        b[1] = fmin(b[1], y[k]);
+        b[2] = fmax(b[2], x[k]);
+        b[3] = fmax(b[3], y[k]);
+    }
+    return vs::grid_decide(g, cells, b[0], b[1], b[2], b[3], x[0], y[0], x, y, n);
 }
 
 // cell_lo / cell_hi of v along axis 0 (x) or 1 (y) of a grid (geo6 as above).

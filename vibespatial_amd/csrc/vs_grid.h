@@ -147,42 +147,62 @@ VS_HD bool grid_crosses(const GridAxes& g, int j, double ay, double by) {
     return (ay > f) != (by > f);
 }
 
-// Candidate decision for ring A (bounding box (ax0, ay0, ax1, ay1), first
-// vertex (fx, fy) finite, coordinates x[0..n), y[0..n)) against the grid of
-// ring B: 1 intersects, 0 does not, 2 undecided but the first step of the
-// exact test may be skipped, 3 undecided.  cells[j] = (boundary bits, inside
-// bits) of row j.  The coordinates are read only when the box does not
-// decide.
+// Candidate decision for ring A against the grid of ring B: (ax0, ay0, ax1,
+// ay1) = A's bounding box, (fx, fy) = A's first vertex, x[0..n), y[0..n) =
+// A's coordinates (1 <= n), read only when the box does not decide.
+// Returns 1 intersects, 0 does not, 2 undecided but the first step of the
+// exact test may be skipped, 3 undecided (also when the first vertex, or for
+// the vertex test any coordinate, is not finite).  cells[j] = (boundary bits, inside bits) of grid row j.  Loads
+// are issued in groups: the first eight grid rows of the box at once (rows
+// past the box repeat its last row, which changes none of the folds), then
+// the vertices eight at a time (indices past n - 1 repeat the last vertex,
+// which changes neither test), then each vertex's row.
 VS_HD int grid_decide(const GridAxes& g, const uint2* cells, double ax0, double ay0, double ax1, double ay1,
                       double fx, double fy, const double* x, const double* y, int n) {
+    if (!(fabs(fx) < INFINITY && fabs(fy) < INFINITY)) return 3;
     const int i0 = cell_lo(ax0, g.x0, g.wx, g.ix), i1 = cell_hi(ax1, g.x0, g.wx, g.ix);
     const int j0 = cell_lo(ay0, g.y0, g.wy, g.iy), j1 = cell_hi(ay1, g.y0, g.wy, g.iy);
     const uint32_t m = span_mask(i0, i1);
-    uint32_t all_in = m, any_in = 0;
-    bool marked = false;
-    for (int j = j0; j <= j1; ++j) {
+    uint2 rows[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rows[u] = cells[j0 + u <= j1 ? j0 + u : j1];
+    uint32_t marks = 0u, all_in = m, any_in = 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        marks |= rows[u].x & m;
+        all_in &= rows[u].y;
+        any_in |= rows[u].y & m;
+    }
+    for (int j = j0 + 8; j <= j1; ++j) {
         const uint2 c = cells[j];
-        if (c.x & m) {
-            marked = true;
-            break;
-        }
+        marks |= c.x & m;
         all_in &= c.y;
         any_in |= c.y & m;
     }
-    if (!marked) {
+    if (!marks) {
         if (all_in == m) return 1;
-        if (any_in == 0) return 0;
+        if (any_in == 0u) return 0;
     }
     bool finite = true, in = false, out = false;
-    for (int k = 0; k < n; ++k) {
-        const double vx = x[k], vy = y[k];
-        finite = finite && fabs(vx) < INFINITY && fabs(vy) < INFINITY;
+    for (int k0 = 0; k0 < n; k0 += 8) {
+        double vx[8], vy[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = k0 + u < n ? k0 + u : n - 1;
+            vx[u] = x[k];
+            vy[u] = y[k];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) finite = finite && fabs(vx[u]) < INFINITY && fabs(vy[u]) < INFINITY;
         if (!finite) break;
-        const int ci = cell_lo(vx, g.x0, g.wx, g.ix), cj = cell_lo(vy, g.y0, g.wy, g.iy);
-        const uint2 c = cells[cj];
-        if (!((c.x >> ci) & 1u)) {
-            if ((c.y >> ci) & 1u) in = true;
-            else out = true;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int ci = cell_lo(vx[u], g.x0, g.wx, g.ix), cj = cell_lo(vy[u], g.y0, g.wy, g.iy);
+            const uint2 c = cells[cj];
+            if (!((c.x >> ci) & 1u)) {
+                if ((c.y >> ci) & 1u) in = true;
+                else out = true;
+            }
         }
     }
     if (!finite) return 3;

@@ -33,17 +33,28 @@ int64_t sort_workspace_bytes_u32(int64_t n);
 //     pass (four loads in flight per lane), reduced with 16-wide shuffles.
 // Every load of a pass is issued before the first use.
 constexpr int kBoundsRows = 16;
+// With info != nullptr also, per row, info = (c0 when the row is one ring of
+// at least one coordinate, else -1; coordinates of the row) and, with
+// first != nullptr, that ring's first vertex (NaN otherwise): the offsets
+// are already in hand.
 __global__ __launch_bounds__(256) void polygon_bounds_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const int32_t* __restrict__ geom,
     const int32_t* __restrict__ part, const int32_t* __restrict__ ring, int64_t n_rows,
-    double4* __restrict__ out) {
+    double4* __restrict__ out, int2* __restrict__ info, double2* __restrict__ first) {
     const int lane = threadIdx.x & 63;
     const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kBoundsRows;
     if (r0 >= n_rows) return;  // wave-uniform
     int32_t c0 = 0, c1 = 0;
     if (lane < kBoundsRows && r0 + lane < n_rows) {
-        c0 = ring[part[geom[r0 + lane]]];
-        c1 = ring[part[geom[r0 + lane + 1]]];
+        const int32_t p0 = geom[r0 + lane], p1 = geom[r0 + lane + 1];
+        const int32_t q0 = part[p0], q1 = part[p1];
+        c0 = ring[q0];
+        c1 = ring[q1];
+        if (info) {
+            const bool single = p1 - p0 == 1 && q1 - q0 == 1 && c1 > c0;
+            info[r0 + lane] = make_int2(single ? c0 : -1, c1 - c0);
+            if (first) first[r0 + lane] = single ? make_double2(x[c0], y[c0]) : make_double2(NAN, NAN);
+        }
     }
     int longest = c1 - c0;
     for (int o = 32; o > 0; o >>= 1) longest = max(longest, __shfl_xor(longest, o, 64));
@@ -309,7 +320,17 @@ int polygon_bounds(const double* x, const double* y, const int32_t* geom, const 
                    const int32_t* ring, int64_t n, double* bounds, hipStream_t s) {
     if (n == 0) return VS_OK;
     hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 4 * kBoundsRows)), dim3(256), 0, s, x, y,
-                       geom, part, ring, n, reinterpret_cast<double4*>(bounds));
+                       geom, part, ring, n, reinterpret_cast<double4*>(bounds), static_cast<int2*>(nullptr),
+                       static_cast<double2*>(nullptr));
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+int polygon_bounds_info(const double* x, const double* y, const int32_t* geom, const int32_t* part,
+                        const int32_t* ring, int64_t n, double* bounds, int2* info, double2* first, hipStream_t s) {
+    if (n == 0) return VS_OK;
+    hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 4 * kBoundsRows)), dim3(256), 0, s, x, y,
+                       geom, part, ring, n, reinterpret_cast<double4*>(bounds), info, first);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }
