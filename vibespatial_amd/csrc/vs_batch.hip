@@ -35,7 +35,7 @@
 namespace vs {
 
 int polygon_bounds_info(const double* x, const double* y, const int32_t* geom, const int32_t* part,
-                        const int32_t* ring, int64_t n, double* bounds, int2* info, double2* first, hipStream_t s);
+                        const int32_t* ring, int64_t n, double* bounds, int2* info, RowRec* rec, hipStream_t s);
 int centre_extent(const double4* b, int64_t n, double4* ext, void* ws, hipStream_t s);
 int64_t extent_workspace_bytes();
 int sort_pairs_u64(uint64_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
@@ -549,10 +549,8 @@ constexpr int kClsBuf = 1024;
 __global__ __launch_bounds__(256) void cand_classify_kernel(const uint32_t* __restrict__ cand_l,
                                                            const int32_t* __restrict__ cand_r,
                                                            JoinCtl* __restrict__ cctl, int64_t ccap,
-                                                           const int2* __restrict__ linfo,
-                                                           const double2* __restrict__ lfirst,
+                                                           const RowRec* __restrict__ lrec,
                                                            const int2* __restrict__ rinfo,
-                                                           const double4* __restrict__ lbox,
                                                            const double* __restrict__ ax,
                                                            const double* __restrict__ ay, RowGrids G,
                                                            int use_grid, int by_work, int rbits,
@@ -600,20 +598,19 @@ __global__ __launch_bounds__(256) void cand_classify_kernel(const uint32_t* __re
         if (i < (int64_t)nc) {
             l = cand_l[i];
             r = cand_r[i];
-            const int2 li = linfo[l], ri = rinfo[r];
+            // every load that depends only on (l, r) issued together
+            const RowRec L = lrec[l];
+            const int2 ri = rinfo[r];
+            const double4 gg = use_grid ? G.geo[r] : make_double4(0.0, 0.0, 0.0, 0.0);
+            const double2 gi = use_grid ? G.inv[r] : make_double2(0.0, 0.0);
+            const int2 li = L.info;
             vb = ri.y;
             cverts += (unsigned long long)(li.y + vb);
             dec = 3;
-            if (use_grid) {
-                // every load that depends only on (l, r) issued together
-                const double4 gg = G.geo[r];
-                const double2 gi = G.inv[r];
-                const double2 f = lfirst[l];
-                const double4 ab = lbox[l];
-                if (li.x >= 0 && li.y <= kRingTile && li.y <= vb && gg.z > 0.0)
-                    dec = grid_decide(GridAxes{gg.x, gg.y, gg.z, gg.w, gi.x, gi.y}, G.cells + (int64_t)r * kGrid,
-                                      ab.x, ab.y, ab.z, ab.w, f.x, f.y, ax + li.x, ay + li.x, li.y);
-            }
+            if (use_grid && li.x >= 0 && li.y <= kRingTile && li.y <= vb && gg.z > 0.0)
+                dec = grid_decide(GridAxes{gg.x, gg.y, gg.z, gg.w, gi.x, gi.y}, G.cells + (int64_t)r * kGrid,
+                                  L.box.x, L.box.y, L.box.z, L.box.w, L.first.x, L.first.y, ax + li.x, ay + li.x,
+                                  li.y);
         }
         const uint64_t tm = __ballot(dec == 1), um = __ballot(dec >= 2);
         unsigned int tb = 0u, ub = 0u;
@@ -661,7 +658,7 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
                                                           const int32_t* __restrict__ cand_r, uint32_t rmask,
                                                           const unsigned long long* __restrict__ count,
                                                           int64_t ccap, Polys A, Polys B,
-                                                          const int2* __restrict__ linfo,
+                                                          const RowRec* __restrict__ lrec,
                                                           const int2* __restrict__ rinfo, RowSlabs RS,
                                                           uint32_t* __restrict__ out_l, int32_t* __restrict__ out_r,
                                                           int64_t capacity, JoinCtl* __restrict__ ctl) {
@@ -683,7 +680,7 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
             const bool skip_first = (lv >> 31) != 0u;
             l = lv & 0x7FFFFFFFu;
             r = (int32_t)((uint32_t)cand_r[i] & rmask);
-            const int2 li = linfo[l], ri = rinfo[r];
+            const int2 li = lrec[l].info, ri = rinfo[r];
             const int32_t ca = li.x, na = li.y, cb = ri.x, nb = ri.y;
             const int nslab = RS.nslab[r];
             const bool tile_a = ca >= 0 && na <= kRingTile;
@@ -766,8 +763,7 @@ struct BatchWs {
     double4* grid_geo;
     double2* grid_inv;
     uint2* grid_cells;
-    int2* linfo;
-    double2* lfirst;
+    RowRec* lrec;
     int2* rinfo;
 };
 
@@ -816,8 +812,7 @@ static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n
     W.grid_geo = c.take<double4>(n_right);
     W.grid_inv = c.take<double2>(n_right);
     W.grid_cells = c.take<uint2>(n_right * kGrid);
-    W.linfo = c.take<int2>(n_left);
-    W.lfirst = c.take<double2>(n_left);
+    W.lrec = c.take<RowRec>(n_left);
     W.rinfo = c.take<int2>(n_right);
     if (bytes) *bytes = c.bytes;
     return W;
@@ -848,11 +843,11 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
     }
     if (n_jobs > 0 && n_left > 0 && n_right > 0) {
-        // bounds, ring info (first coordinate of a single ring, coordinates of the row), left first vertices
+        // bounds; left rows' records (bounds, first vertex, ring info); right rows' ring info
         VS_TRY(polygon_bounds_info(A.x, A.y, A.geom, A.part, A.ring, n_left, reinterpret_cast<double*>(W.lbox),
-                                   W.linfo, W.lfirst, s));
+                                   static_cast<int2*>(nullptr), W.lrec, s));
         VS_TRY(polygon_bounds_info(B.x, B.y, B.geom, B.part, B.ring, n_right, reinterpret_cast<double*>(W.rbox),
-                                   W.rinfo, static_cast<double2*>(nullptr), s));
+                                   W.rinfo, static_cast<RowRec*>(nullptr), s));
         const int jb = bits_for_jobs(n_jobs);
         // right rows: (job, Hilbert) order, forest
         VS_TRY(centre_extent(W.rbox, n_right, W.ext, W.ext_ws, s));
@@ -894,8 +889,8 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         // coordinates, same edge count), with the work order the largest
         // parcels' candidates first
         hipLaunchKernelGGL(cand_classify_kernel, dim3(stride_grid(capacity, 256, 8)), dim3(256), 0, s,
-                           reinterpret_cast<const uint32_t*>(W.cand_l), W.cand_r, W.cctl, capacity, W.linfo,
-                           W.lfirst, W.rinfo, W.lbox, A.x, A.y,
+                           reinterpret_cast<const uint32_t*>(W.cand_l), W.cand_r, W.cctl, capacity, W.lrec,
+                           W.rinfo, A.x, A.y,
                            RowGrids{W.grid_geo, W.grid_inv, W.grid_cells}, use_grid ? 1 : 0, by_work ? 1 : 0, rbits,
                            W.ckey, W.cs_l, W.stage_l, W.stage_r, capacity, W.ctl);
         VS_LAUNCH_CHECK();
@@ -908,7 +903,7 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         hipLaunchKernelGGL(batch_refine_kernel, dim3(rgrid), dim3(refine_threads),
                            (unsigned)(refine_threads / 64 * kRingTileBytesPerWave), s, W.cand_l,
                            reinterpret_cast<const int32_t*>(W.cs_r), rmask, &W.cctl->stat[0], capacity, A, B,
-                           W.linfo, W.rinfo, RowSlabs{W.slab_geo, W.slab_n, W.slab_list}, W.stage_l, W.stage_r,
+                           W.lrec, W.rinfo, RowSlabs{W.slab_geo, W.slab_n, W.slab_list}, W.stage_l, W.stage_r,
                            capacity, W.ctl);
         VS_LAUNCH_CHECK();
         VS_TRY(timing_end_public(s));

@@ -34,33 +34,41 @@ int64_t sort_workspace_bytes_u32(int64_t n);
 // Every load of a pass is issued before the first use.
 constexpr int kBoundsRows = 16;
 // With info != nullptr also, per row, info = (c0 when the row is one ring of
-// at least one coordinate, else -1; coordinates of the row) and, with
-// first != nullptr, that ring's first vertex (NaN otherwise): the offsets
-// are already in hand.
+// at least one coordinate, else -1; coordinates of the row), and with
+// rec != nullptr the row's RowRec (bounds, that ring's first vertex, info):
+// the offsets are already in hand.
 __global__ __launch_bounds__(256) void polygon_bounds_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const int32_t* __restrict__ geom,
     const int32_t* __restrict__ part, const int32_t* __restrict__ ring, int64_t n_rows,
-    double4* __restrict__ out, int2* __restrict__ info, double2* __restrict__ first) {
+    double4* __restrict__ out, int2* __restrict__ info, RowRec* __restrict__ rec) {
     const int lane = threadIdx.x & 63;
     const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kBoundsRows;
     if (r0 >= n_rows) return;  // wave-uniform
-    int32_t c0 = 0, c1 = 0;
+    int32_t c0 = 0, c1 = 0, ci = -1;  // ci: c0 for a single ring, else -1
     if (lane < kBoundsRows && r0 + lane < n_rows) {
         const int32_t p0 = geom[r0 + lane], p1 = geom[r0 + lane + 1];
         const int32_t q0 = part[p0], q1 = part[p1];
         c0 = ring[q0];
         c1 = ring[q1];
-        if (info) {
-            const bool single = p1 - p0 == 1 && q1 - q0 == 1 && c1 > c0;
-            info[r0 + lane] = make_int2(single ? c0 : -1, c1 - c0);
-            if (first) first[r0 + lane] = single ? make_double2(x[c0], y[c0]) : make_double2(NAN, NAN);
-        }
+        ci = (p1 - p0 == 1 && q1 - q0 == 1 && c1 > c0) ? c0 : -1;
+        if (info) info[r0 + lane] = make_int2(ci, c1 - c0);
     }
+    // the whole record of a row is written by the lane that holds its first
+    // coordinate after the reduction (one 64-byte store per row)
+    auto write_rec = [&](int64_t row, double4 box, int32_t a0, int32_t a1, int32_t single_c0, double fx,
+                         double fy) {
+        RowRec v;
+        v.box = box;
+        v.first = single_c0 >= 0 ? make_double2(fx, fy) : make_double2(NAN, NAN);
+        v.info = make_int2(single_c0, a1 - a0);
+        v.pad = make_int2(0, 0);
+        rec[row] = v;
+    };
     int longest = c1 - c0;
     for (int o = 32; o > 0; o >>= 1) longest = max(longest, __shfl_xor(longest, o, 64));
     if (longest <= 16) {
         const int j = lane >> 2, sub = lane & 3;
-        const int32_t a0 = __shfl(c0, j, 64), a1 = __shfl(c1, j, 64);
+        const int32_t a0 = __shfl(c0, j, 64), a1 = __shfl(c1, j, 64), aj = __shfl(ci, j, 64);
         double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
         double xv[4], yv[4];
 #pragma unroll
@@ -87,6 +95,7 @@ __global__ __launch_bounds__(256) void polygon_bounds_kernel(
         if (sub == 0 && row < n_rows) {
             if (a1 <= a0 || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
             out[row] = make_double4(x0, y0, x1, y1);
+            if (rec) write_rec(row, make_double4(x0, y0, x1, y1), a0, a1, aj, xv[0], yv[0]);  // xv[0] = x[a0]
         }
         return;
     }
@@ -95,8 +104,8 @@ __global__ __launch_bounds__(256) void polygon_bounds_kernel(
     for (int j0 = 0; j0 < kBoundsRows; j0 += 4) {
         const int j = j0 + g;
         const int64_t row = r0 + j;
-        const int32_t a0 = __shfl(c0, j, 64), a1 = __shfl(c1, j, 64);
-        double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+        const int32_t a0 = __shfl(c0, j, 64), a1 = __shfl(c1, j, 64), aj = __shfl(ci, j, 64);
+        double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, fx = NAN, fy = NAN;
         for (int32_t cb = a0; cb < a1; cb += 64) {
             double xv[4], yv[4];
 #pragma unroll
@@ -105,6 +114,10 @@ __global__ __launch_bounds__(256) void polygon_bounds_kernel(
                 const bool ok = c < a1;
                 xv[u] = ok ? x[c] : INFINITY;
                 yv[u] = ok ? y[c] : INFINITY;
+            }
+            if (cb == a0) {  // sub 0 holds x[a0] in its first pass
+                fx = xv[0];
+                fy = yv[0];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -123,6 +136,7 @@ __global__ __launch_bounds__(256) void polygon_bounds_kernel(
         if (sub == 0 && row < n_rows) {
             if (a1 <= a0 || !(x0 <= x1) || !(y0 <= y1)) x0 = y0 = x1 = y1 = NAN;
             out[row] = make_double4(x0, y0, x1, y1);
+            if (rec) write_rec(row, make_double4(x0, y0, x1, y1), a0, a1, aj, fx, fy);
         }
     }
 }
@@ -321,16 +335,16 @@ int polygon_bounds(const double* x, const double* y, const int32_t* geom, const 
     if (n == 0) return VS_OK;
     hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 4 * kBoundsRows)), dim3(256), 0, s, x, y,
                        geom, part, ring, n, reinterpret_cast<double4*>(bounds), static_cast<int2*>(nullptr),
-                       static_cast<double2*>(nullptr));
+                       static_cast<RowRec*>(nullptr));
     VS_LAUNCH_CHECK();
     return VS_OK;
 }
 
 int polygon_bounds_info(const double* x, const double* y, const int32_t* geom, const int32_t* part,
-                        const int32_t* ring, int64_t n, double* bounds, int2* info, double2* first, hipStream_t s) {
+                        const int32_t* ring, int64_t n, double* bounds, int2* info, RowRec* rec, hipStream_t s) {
     if (n == 0) return VS_OK;
     hipLaunchKernelGGL(polygon_bounds_kernel, dim3((unsigned)ceil_div(n, 4 * kBoundsRows)), dim3(256), 0, s, x, y,
-                       geom, part, ring, n, reinterpret_cast<double4*>(bounds), info, first);
+                       geom, part, ring, n, reinterpret_cast<double4*>(bounds), info, rec);
     VS_LAUNCH_CHECK();
     return VS_OK;
 }

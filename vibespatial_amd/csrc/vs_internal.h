@@ -74,6 +74,18 @@ struct Carver {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// One row's bounds, ring info and first vertex in one 64-byte record (one
+// cache line): box = bounds (NaN when empty); first = first vertex of a
+// single ring (NaN otherwise); info = (first coordinate of a single ring of
+// at least one coordinate, else -1; coordinates of the row).
+struct alignas(64) RowRec {
+    double4 box;
+    double2 first;
+    int2 info;
+    int2 pad;
+};
+
+
 // Grid of the learned point index: keys are Hilbert indices of a 2^g x 2^g
 // grid over the point extent, left-aligned in K (low bits zero).  32-bit keys
 // use g = 12 (three 8-bit radix passes); 64-bit keys g = 32.
