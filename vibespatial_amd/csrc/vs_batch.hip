@@ -642,6 +642,74 @@ __global__ __launch_bounds__(256) void cand_classify_kernel(const uint32_t* __re
     if (lane == 0 && cverts) atomicAdd(&ctl->stat[1], cverts);
 }
 
+// The exact intersects of ring A (na coordinates, a lane's LDS column) and
+// right ring B (coordinates c0b .. c0b + nb - 1, nb <= 65536): the three
+// terms of ring_intersects_polys -- A's first vertex on B's boundary or
+// inside B, some edge of B touching some edge of A, B's first vertex in A --
+// OR-ed, so the order of evaluation does not change the boolean.  One stream
+// over B's edges (four loaded together) runs the first term's crossing steps
+// (unless skip_first: then that term is known to be false unless the second
+// is true) and lists, in the lane's LDS column `near` (kNearList uint16
+// entries, stride 64), the edges whose box meets A's box; the segment tests
+// run over that list when it fills and at the end.  Each lane thus tests its
+// own near edges: a wave runs the segment loop max(list length) times rather
+// than once for every edge index that is near for some lane.
+constexpr int kNearList = 16;
+
+__device__ inline bool ring_intersects_listed(const RingRef& A, int32_t na, const double* __restrict__ bx,
+                                              const double* __restrict__ by, int32_t c0b, int32_t nb,
+                                              bool skip_first, uint16_t* near) {
+    double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+    for (int32_t k = 0; k < na; ++k) {
+        mnx = fmin(mnx, A.X(k));
+        mny = fmin(mny, A.Y(k));
+        mxx = fmax(mxx, A.X(k));
+        mxy = fmax(mxy, A.Y(k));
+    }
+    const double fx = A.X(0), fy = A.Y(0);
+    int cnt = 0, parity = 0;
+    auto touches = [&]() -> bool {  // segment tests over the listed edges; empties the list
+        for (int t = 0; t < cnt; ++t) {
+            const int32_t e = c0b + (int32_t)near[t * 64];
+            const double px = bx[e - 1], py = by[e - 1], ex = bx[e], ey = by[e];
+            for (int32_t k = 1; k < na; ++k)
+                if (segments_touch(A.X(k - 1), A.Y(k - 1), A.X(k), A.Y(k), px, py, ex, ey)) return true;
+        }
+        cnt = 0;
+        return false;
+    };
+    double px = bx[c0b], py = by[c0b];
+    for (int32_t c0 = 1; c0 < nb; c0 += 4) {
+        double qxs[4], qys[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int32_t c = c0 + u < nb ? c0 + u : nb - 1;
+            qxs[u] = bx[c0b + c];
+            qys[u] = by[c0b + c];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (c0 + u >= nb) break;
+            const double qx = qxs[u], qy = qys[u];
+            if (!skip_first && edge_step_f64(fx, fy, px, py, qx, qy, parity)) return true;  // on B's boundary
+            if (fmax(px, qx) >= mnx && fmin(px, qx) <= mxx && fmax(py, qy) >= mny && fmin(py, qy) <= mxy) {
+                near[cnt * 64] = (uint16_t)(c0 + u);
+                if (++cnt == kNearList && touches()) return true;
+            }
+            px = qx;
+            py = qy;
+        }
+    }
+    if (parity) return true;  // A's first vertex inside B
+    if (touches()) return true;
+    const double qx = bx[c0b], qy = by[c0b];
+    if (qx < mnx || qx > mxx || qy < mny || qy > mxy) return false;  // B's first vertex outside A's box
+    int par = 0;
+    for (int32_t k = 1; k < na; ++k)
+        if (edge_step_f64(qx, qy, A.X(k - 1), A.Y(k - 1), A.X(k), A.Y(k), par)) return true;
+    return par != 0;
+}
+
 // One thread per candidate left by cand_classify_kernel (count in *count,
 // at most ccap): exact intersects; survivors appended to the finish stage
 // with one atomic per wave.  A left row that is one ring of at most
@@ -653,7 +721,8 @@ __global__ __launch_bounds__(256) void cand_classify_kernel(const uint32_t* __re
 // the left ring's first vertex in the right row) is skipped (grid_decide's
 // outcome 2: its answer is "outside, off the boundary" unless the rings'
 // boundaries touch, which the second step finds).
-constexpr int64_t kRingTileBytesPerWave = 2 * kRingTile * 64 * (int64_t)sizeof(double);
+constexpr int64_t kRingTileBytesPerWave =
+    2 * kRingTile * 64 * (int64_t)sizeof(double) + kNearList * 64 * (int64_t)sizeof(uint16_t);
 __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __restrict__ cand_l,
                                                           const int32_t* __restrict__ cand_r, uint32_t rmask,
                                                           const unsigned long long* __restrict__ count,
@@ -666,8 +735,11 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
     const unsigned long long nc = *count;
     if (nc > (unsigned long long)ccap) return;
     const int lane = threadIdx.x & 63;
-    double* tx = ring_tile + (threadIdx.x >> 6) * (2 * kRingTile * 64) + lane;
+    // per wave: x and y columns of the ring tile, then the near-edge lists
+    char* wbase = reinterpret_cast<char*>(ring_tile) + (threadIdx.x >> 6) * kRingTileBytesPerWave;
+    double* tx = reinterpret_cast<double*>(wbase) + lane;
     double* ty = tx + kRingTile * 64;
+    uint16_t* near = reinterpret_cast<uint16_t*>(wbase + 2 * kRingTile * 64 * sizeof(double)) + lane;
     const uint64_t lt = lanemask_lt();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < (int64_t)nc; i0 += stride) {
@@ -693,7 +765,9 @@ __global__ __launch_bounds__(256) void batch_refine_kernel(const uint32_t* __res
                     keep = ring_intersects_slabbed(RingRef{tx, ty, 64}, na, B.x, B.y, cb,
                                                    RS.list + row_slab_base(cb), nslab, RS.geo[r], skip_first);
                 } else {
-                    keep = ring_intersects_polys(RingRef{tx, ty, 64}, na, B, r, skip_first);
+                    keep = (cb >= 0 && nb <= 65536)
+                               ? ring_intersects_listed(RingRef{tx, ty, 64}, na, B.x, B.y, cb, nb, skip_first, near)
+                               : ring_intersects_polys(RingRef{tx, ty, 64}, na, B, r, skip_first);
                 }
             } else {
                 keep = polygons_intersect_auto(A, l, B, r);
