@@ -144,15 +144,48 @@ __global__ __launch_bounds__(256) void forest_build_kernel(const double4* __rest
     }
 }
 
+// One traversal task (64 consecutive sorted left rows of one job) with every
+// per-job word the traversal needs, so a task starts with one 32-byte load
+// instead of a binary search over the job offsets.
+struct TaskRec {
+    int32_t s0;    // first sorted left row
+    int32_t cnt;   // rows (1..64)
+    int32_t r0;    // first right row of the job
+    int32_t nr;    // right rows of the job
+    int32_t toff;  // the job's tree in the forest
+    int32_t pad[3];
+};
+
+// One wave per job (grid-stride): its tasks' records.
+__global__ __launch_bounds__(256) void task_recs_kernel(const int64_t* __restrict__ left_job,
+                                                       const int64_t* __restrict__ right_job, int64_t n_jobs,
+                                                       const int32_t* __restrict__ wave_off,
+                                                       const int32_t* __restrict__ tree_off,
+                                                       TaskRec* __restrict__ tasks) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); j < n_jobs; j += (int64_t)gridDim.x * 4) {
+        const int64_t l0 = left_job[j], nl = left_job[j + 1] - l0;
+        const int32_t w0 = wave_off[j], r0 = (int32_t)right_job[j], nr = (int32_t)(right_job[j + 1] - right_job[j]);
+        const int32_t toff = tree_off[j];
+        for (int64_t k = lane; 64 * k < nl; k += 64) {
+            TaskRec t;
+            t.s0 = (int32_t)(l0 + 64 * k);
+            t.cnt = (int32_t)(nl - 64 * k < 64 ? nl - 64 * k : 64);
+            t.r0 = r0;
+            t.nr = nr;
+            t.toff = toff;
+            t.pad[0] = t.pad[1] = t.pad[2] = 0;
+            tasks[w0 + k] = t;
+        }
+    }
+}
+
 // Persistent: waves take (job, 64 sorted left rows) tasks round-robin.
 __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __restrict__ lbox,
                                                           const int32_t* __restrict__ lperm,
-                                                          const int64_t* __restrict__ left_job,
-                                                          const int64_t* __restrict__ right_job, int64_t n_jobs,
-                                                          const int32_t* __restrict__ wave_off,
+                                                          const TaskRec* __restrict__ tasks,
                                                           const long long* __restrict__ total_waves,
                                                           const double4* __restrict__ fbox,
-                                                          const int32_t* __restrict__ tree_off,
                                                           const int32_t* __restrict__ rperm,
                                                           uint32_t* __restrict__ cand_l, int32_t* __restrict__ cand_r,
                                                           int64_t capacity, JoinCtl* __restrict__ ctl) {
@@ -185,31 +218,21 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
     // serialise every task start in L2)
     const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
     for (long long tt = (long long)blockIdx.x * (blockDim.x >> 6) + w; tt < W; tt += nw) {
-        const unsigned int t = (unsigned int)tt;
-        // job of wave t: last j with wave_off[j] <= t
-        int64_t lo = 0, hi = n_jobs - 1;
-        while (lo < hi) {
-            const int64_t m = (lo + hi + 1) >> 1;
-            if (wave_off[m] <= (int32_t)t) lo = m;
-            else hi = m - 1;
-        }
-        const int64_t j = lo;
-        const int64_t s0 = left_job[j] + 64 * (int64_t)(t - (unsigned)wave_off[j]);
-        const int64_t s1 = (s0 + 64 < left_job[j + 1]) ? s0 + 64 : left_job[j + 1];
-        const int64_t idx = s0 + lane;
-        const bool inr = idx < s1;
+        const TaskRec task = tasks[tt];
+        const int64_t idx = (int64_t)task.s0 + lane;
+        const bool inr = lane < task.cnt;
         const int32_t orig = inr ? lperm[idx] : 0;
         const double4 b = inr ? lbox[orig] : make_double4(NAN, NAN, NAN, NAN);
         const bool valid = inr && b.x == b.x && b.y == b.y && b.z == b.z && b.w == b.w;
         const double ux0 = wave_min(valid ? b.x : INFINITY), uy0 = wave_min(valid ? b.y : INFINITY);
         const double ux1 = wave_max(valid ? b.z : -INFINITY), uy1 = wave_max(valid ? b.w : -INFINITY);
-        const int64_t r0 = right_job[j];
+        const int64_t r0 = task.r0;
         int64_t size[kFMaxLevels];
-        const int d = forest_levels(right_job[j + 1] - r0, size);
+        const int d = forest_levels(task.nr, size);
         if (d == 0 || __ballot(valid) == 0) continue;
         int64_t start[kFMaxLevels];
         {
-            int64_t acc = tree_off[j];
+            int64_t acc = task.toff;
             for (int l = 0; l < d; ++l) {
                 start[l] = acc;
                 acc += size[l];
@@ -233,8 +256,10 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
             const int ng = (int)((cl - first) < kFF ? (cl - first) : kFF);
             double4 nb = make_double4(NAN, NAN, NAN, NAN);
             bool ov = false;
+            int32_t rid = 0;  // leaf level: the group's right row ids, loaded with the boxes
             if (lane < ng) {
                 nb = fbox[sl + first + lane];
+                if (l == d - 1) rid = rperm[r0 + first + lane];
                 ov = box_overlap(ux0, uy0, ux1, uy1, nb.x, nb.y, nb.z, nb.w);
             }
             uint64_t m = __ballot(ov);
@@ -249,7 +274,8 @@ __global__ __launch_bounds__(256) void forest_query_kernel(const double4* __rest
                     if (!hm) continue;
                     const int k = __popcll(hm);
                     if (nbuf + k > kFOut) flush();
-                    if (hit) outbuf[nbuf + __popcll(hm & lt)] = make_int2(orig, rperm[r0 + first + s]);
+                    const int32_t rr = __builtin_amdgcn_readlane(rid, s);
+                    if (hit) outbuf[nbuf + __popcll(hm & lt)] = make_int2(orig, rr);
                     nbuf += k;
                 }
             } else {
@@ -299,7 +325,8 @@ __device__ __forceinline__ int row_slab(double y, double ylo, double inv, int nb
 __device__ __forceinline__ int64_t row_slab_base(int32_t c0) { return 3 * (int64_t)c0; }
 
 // One 64-thread block per row (grid-stride): counts per slab, starts, entries.
-__global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_rows, const double4* __restrict__ rbox,
+__global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_rows, const int2* __restrict__ rinfo,
+                                                            const double4* __restrict__ rbox,
                                                             double2* __restrict__ geo, int32_t* __restrict__ nslab,
                                                             uint16_t* __restrict__ list, int64_t list_cap,
                                                             int min_edges) {
@@ -307,10 +334,10 @@ __global__ __launch_bounds__(64) void row_slab_build_kernel(Polys B, int64_t n_r
     __shared__ int cur[kRowSlabMax];
     const int lane = threadIdx.x;
     for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
-        int32_t c0 = 0, n = 0;
+        const int2 ri = rinfo[r];
+        const int32_t c0 = ri.x, n = ri.y;
         // B + 1 + 2 (n - 1) <= 3 n entries from 3 c0 on, every value below 2^16
-        const bool ok = single_ring_of(B, r, c0, n) && n - 1 >= min_edges && n <= 32768 && c0 >= 0 &&
-                        row_slab_base(c0) + 3 * (int64_t)n <= list_cap;
+        const bool ok = c0 >= 0 && n - 1 >= min_edges && n <= 32768 && row_slab_base(c0) + 3 * (int64_t)n <= list_cap;
         if (!ok) {
             if (lane == 0) nslab[r] = 0;
             continue;  // block-uniform
@@ -816,6 +843,7 @@ struct BatchWs {
     int32_t* tree_off;
     int32_t* waves;
     int32_t* wave_off;
+    TaskRec* tasks;
     long long* total_nodes;
     long long* total_waves;
     void* scan_ws;
@@ -864,6 +892,7 @@ static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n
     W.tree_off = c.take<int32_t>(n_jobs);
     W.waves = c.take<int32_t>(n_jobs);
     W.wave_off = c.take<int32_t>(n_jobs);
+    W.tasks = c.take<TaskRec>(n_left / 64 + n_jobs + 1);  // sum over jobs of ceil(rows / 64)
     W.total_nodes = c.take<long long>(1);
     W.total_waves = c.take<long long>(1);
     W.scan_ws = c.take<char>(scan_workspace_bytes(n_jobs));
@@ -935,7 +964,7 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         VS_TRY(exclusive_scan_i32(W.nodes, W.tree_off, n_jobs, W.total_nodes, W.scan_ws, s));
         VS_TRY(exclusive_scan_i32(W.waves, W.wave_off, n_jobs, W.total_waves, W.scan_ws, s));
         hipLaunchKernelGGL(row_slab_build_kernel, dim3((unsigned)std::min<int64_t>(n_right, (int64_t)cus * 32)),
-                           dim3(64), 0, s, B, n_right, W.rbox, W.slab_geo, W.slab_n, W.slab_list,
+                           dim3(64), 0, s, B, n_right, W.rinfo, W.rbox, W.slab_geo, W.slab_n, W.slab_list,
                            3 * n_right_coords, kRowSlabMinEdges[(tile >> 26) & 3]);
         VS_LAUNCH_CHECK();
         if (use_grid) {
@@ -954,9 +983,11 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
         VS_LAUNCH_CHECK();
         VS_TRY(sort_pairs_u64(W.lkey, W.lperm, n_left, 0, 32 + jb, W.sort_ws, sort_workspace_bytes_u64(n_left), s));
         VS_TRY(timing_begin_public(s));
+        hipLaunchKernelGGL(task_recs_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n_jobs, 4), (int64_t)cus * 8)),
+                           dim3(256), 0, s, left_job, right_job, n_jobs, W.wave_off, W.tree_off, W.tasks);
+        VS_LAUNCH_CHECK();
         hipLaunchKernelGGL(forest_query_kernel, dim3((unsigned)(cus * 8)), dim3(64 * waves_per_block), 0, s, W.lbox,
-                           W.lperm, left_job, right_job, n_jobs, W.wave_off, W.total_waves, W.fbox, W.tree_off,
-                           W.rperm, W.cand_l, W.cand_r, capacity, W.cctl);
+                           W.lperm, W.tasks, W.total_waves, W.fbox, W.rperm, W.cand_l, W.cand_r, capacity, W.cctl);
         VS_LAUNCH_CHECK();
         // candidates decided from the cell grids where possible; the rest
         // grouped by right row (the lanes of a wave then share parcels: same
