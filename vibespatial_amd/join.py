@@ -42,7 +42,11 @@ class JoinStats:
     capacity: int
     retries: int
     candidate_vertices: int = 0
-    refined: int = -1  # batched join: candidates left to the exact refine after the cell-grid decisions
+
+
+@dataclass
+class BatchStats(JoinStats):
+    refined: int = 0  # candidates left to the exact refine after the cell-grid decisions
 
 
 class DeviceTree:
@@ -346,9 +350,9 @@ def sjoin_points_polygons(Q: DevicePoints, P: DevicePolygons, tree: DeviceTree, 
             ),
             "vs_sjoin_points_polygons",
         )
-        k, staged, cand, cverts, refined = res.cpu().tolist()
+        k, staged, cand, cverts = _read_result(res)
         if staged <= cap:
-            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts, refined)
+            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts)
         if staged >= 2**31 - 1:
             raise OverflowError(f"{staged} pairs exceed the int32 pair limit")
         cap = staged
@@ -454,8 +458,11 @@ class LearnedStats(JoinStats):
 
 
 def _learned_stats(h, cap, retries) -> LearnedStats:
-    return LearnedStats(int(h[0]), int(h[1]), int(h[2]), cap, retries, int(h[3]), int(h[4]), int(h[5]), int(h[6]),
-                        int(h[7]), int(h[8]), int(h[9]), int(h[10]), int(h[11]))
+    """LearnedStats from the learned join's d_result words (vsjoin.h)."""
+    return LearnedStats(pairs=int(h[0]), staged=int(h[1]), candidates=int(h[2]), capacity=cap, retries=retries,
+                        candidate_vertices=int(h[3]), orient_tests=int(h[4]), escalated_edges=int(h[5]),
+                        escalated_pairs=int(h[6]), lookup_probes=int(h[7]), mlp_fallbacks=int(h[8]),
+                        window_fallbacks=int(h[9]), scanned=int(h[10]), mlp_leaf_error=int(h[11]))
 
 
 LOOKUP_MODES = {"learned": 0, "leaves": 1, "binary": 2}
@@ -760,7 +767,7 @@ def batched_intersects(A: DevicePolygons, B: DevicePolygons, left_job: torch.Ten
               "vs_batched_intersects")
         k, staged, cand, cverts, refined = res.cpu().tolist()
         if staged <= cap:
-            return out_l[:k], out_r[:k], JoinStats(k, staged, cand, cap, retries, cverts, refined)
+            return out_l[:k], out_r[:k], BatchStats(k, staged, cand, cap, retries, cverts, refined)
         if staged > PAIR_LIMIT:
             raise OverflowError(f"{staged} bbox candidates exceed the int32 slot limit of one call")
         del ws, out_l, out_r
@@ -842,4 +849,4 @@ class BatchedIntersectsGraph:
         k, staged, cand, cverts, refined = self.res.cpu().tolist()
         if staged > self.cap:
             raise OverflowError(f"{staged} bbox candidates exceed the captured capacity {self.cap}")
-        return self.out_l[:k], self.out_r[:k], JoinStats(k, staged, cand, self.cap, 0, cverts, refined)
+        return self.out_l[:k], self.out_r[:k], BatchStats(k, staged, cand, self.cap, 0, cverts, refined)
