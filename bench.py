@@ -72,6 +72,9 @@ def main(argv=None):
     ap.add_argument("--extras", type=int, default=1, choices=(0, 1),
                     help="c2: also time the lookup ablation and the public API (0: only the two forms, for "
                          "profiler runs)")
+    ap.add_argument("--tile", default="", metavar="W,T,O",
+                    help="c5: use this tile (traversal waves per block, refine block, candidate order) instead "
+                         "of autotuning, and skip the ablation (for profiler runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once (a kernel of known traffic for "
                          "scaling hardware byte counters)")
@@ -491,18 +494,21 @@ def run_c5(args):
     torch.cuda.synchronize()
     L = _lib.lib()
     _log("c5: autotune")
-    best, times = J.autotune_batched(A, B, tl, tr, reps=1)
-    _, _, st = J.batched_intersects(A, B, tl, tr)
+    if args.tile:
+        best, times = tuple(int(v) for v in args.tile.split(",")), {}
+    else:
+        best, times = J.autotune_batched(A, B, tl, tr, reps=1)
+    _, _, st = J.batched_intersects(A, B, tl, tr, tile=best)
     cap = st.staged + 1024
     for _ in range(max(args.warmup, 1)):
-        J.batched_intersects(A, B, tl, tr, capacity=cap)
+        J.batched_intersects(A, B, tl, tr, capacity=cap, tile=best)
     _lib.check(L.vs_set_kernel_timing(1), "vs_set_kernel_timing")
     kms = []
     torch.cuda.synchronize()
     _log(f"c5: timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        _, _, st = J.batched_intersects(A, B, tl, tr, capacity=cap)
+        _, _, st = J.batched_intersects(A, B, tl, tr, capacity=cap, tile=best)
         ms = _lib.ctypes.c_float()
         _lib.check(L.vs_last_kernel_ms(_lib.ctypes.byref(ms)), "vs_last_kernel_ms")
         kms.append(ms.value)
@@ -520,20 +526,23 @@ def run_c5(args):
     ms_graph = (time.perf_counter() - t0) * 1e3 / args.steps
     assert stg.pairs == st.pairs
     # ablation: every bbox candidate refined exactly (no cell-grid decisions)
-    J.batched_intersects(A, B, tl, tr, capacity=cap, tile=best, cell_grid=False)
-    _lib.check(L.vs_set_kernel_timing(1), "vs_set_kernel_timing")
-    kms0 = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _, _, st0 = J.batched_intersects(A, B, tl, tr, capacity=cap, tile=best, cell_grid=False)
-        ms = _lib.ctypes.c_float()
-        _lib.check(L.vs_last_kernel_ms(_lib.ctypes.byref(ms)), "vs_last_kernel_ms")
-        kms0.append(ms.value)
-    torch.cuda.synchronize()
-    ms_nogrid = (time.perf_counter() - t0) * 1e3 / args.steps
-    _lib.check(L.vs_set_kernel_timing(0), "vs_set_kernel_timing")
-    assert st0.pairs == st.pairs and st0.refined == st0.candidates
+    ablation = None
+    if not args.tile:
+        J.batched_intersects(A, B, tl, tr, capacity=cap, tile=best, cell_grid=False)
+        _lib.check(L.vs_set_kernel_timing(1), "vs_set_kernel_timing")
+        kms0 = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, _, st0 = J.batched_intersects(A, B, tl, tr, capacity=cap, tile=best, cell_grid=False)
+            ms = _lib.ctypes.c_float()
+            _lib.check(L.vs_last_kernel_ms(_lib.ctypes.byref(ms)), "vs_last_kernel_ms")
+            kms0.append(ms.value)
+        torch.cuda.synchronize()
+        ms_nogrid = (time.perf_counter() - t0) * 1e3 / args.steps
+        _lib.check(L.vs_set_kernel_timing(0), "vs_set_kernel_timing")
+        assert st0.pairs == st.pairs and st0.refined == st0.candidates
+        ablation = {"ms_per_step": ms_nogrid, "traverse_refine_kernel_ms": statistics.mean(kms0)}
     VA = np.diff(left.ring_offsets[left.part_offsets[left.geom_offsets]])
     VB = np.diff(right.ring_offsets[right.part_offsets[right.geom_offsets]])
     # sum_q 16 V_q + sum_C 16 (V_a + V_b) + 8 K (the candidate term counted on the device)
@@ -582,7 +591,7 @@ def run_c5(args):
             "autotune_ms": {f"{k[0]}x{k[1]}/order{k[2]}": v for k, v in times.items()},
             "traverse_refine_kernel_ms": statistics.mean(kms),
             "graph_replay_ms_per_step": ms_graph,
-            "without_cell_grids": {"ms_per_step": ms_nogrid, "traverse_refine_kernel_ms": statistics.mean(kms0)},
+            "without_cell_grids": ablation,
         },
         "roofline": {"bound": "hbm", "kernel": "whole step", "achieved": alg / (ms_step * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -9,9 +9,10 @@
 // left_job[j+1]) with right rows [right_job[j], right_job[j+1]) only.
 //
 //   1. bounds of every left and right row;
-//   2. right rows sorted by (job, Hilbert key of the bbox centre over the
-//      extent of all right centres); a packed tree per job (fan-out 16, root
-//      first, items last) is built by one block per job into one forest array;
+//   2. right rows sorted by (job, 8-level Hilbert key of the bbox centre over
+//      the extent of the job's right centres); a packed tree per job (fan-out
+//      16, root first, items last) is built by one block per job into one
+//      forest array;
 //   3. left rows sorted the same way; every job is cut into waves of 64
 //      consecutive sorted left rows; a persistent kernel deals the waves out
 //      round-robin and each wave walks its own job's tree with a shared LDS
@@ -36,8 +37,6 @@ namespace vs {
 
 int polygon_bounds_info(const double* x, const double* y, const int32_t* geom, const int32_t* part,
                         const int32_t* ring, int64_t n, double* bounds, int2* info, RowRec* rec, hipStream_t s);
-int centre_extent(const double4* b, int64_t n, double4* ext, void* ws, hipStream_t s);
-int64_t extent_workspace_bytes();
 int sort_pairs_u64(uint64_t*, int32_t*, int64_t, int, int, void*, int64_t, hipStream_t);
 int64_t sort_workspace_bytes_u64(int64_t n);
 int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, long long* d_total, void* ws, hipStream_t s);
@@ -63,33 +62,123 @@ __host__ __device__ inline int forest_levels(int64_t n, int64_t* size) {
     return d;
 }
 
-__device__ __forceinline__ int64_t job_of(const int64_t* __restrict__ off, int64_t n_jobs, int64_t row) {
-    // last job j with off[j] <= row (empty jobs share their offset with the next)
-    int64_t lo = 0, hi = n_jobs - 1;
-    while (lo < hi) {
-        const int64_t m = (lo + hi + 1) >> 1;
-        if (off[m] <= row) lo = m;
-        else hi = m - 1;
+// Job of every row without a search per row: mark[k] counts the jobs j >= 1
+// starting at row k (a job starts at left_job[j]; empty jobs share their
+// start with the next), the exclusive scan of the marks plus the row's own
+// mark is #{j >= 1 : off[j] <= row} = the last job j with off[j] <= row.
+__global__ __launch_bounds__(256) void job_marks_kernel(const int64_t* __restrict__ job_off, int64_t n_jobs,
+                                                       int64_t n_rows, int32_t* __restrict__ mark) {
+    for (int64_t j = 1 + (int64_t)blockIdx.x * 256 + threadIdx.x; j < n_jobs; j += (int64_t)gridDim.x * 256) {
+        const int64_t k = job_off[j];
+        if (k < n_rows) atomicAdd(&mark[k], 1);
     }
-    return lo;
 }
 
-// key = job << 32 | 16-level Hilbert code of the bbox centre (NaN -> last).
+__global__ __launch_bounds__(256) void add_marks_kernel(const int32_t* __restrict__ mark, int64_t n,
+                                                       int32_t* __restrict__ row_job) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        row_job[i] += mark[i];
+}
+
+// Extent of the box centres of each job's rows, as order-preserving 64-bit
+// integers (ordered_f64: the integer order of the codes is the order of the
+// doubles) so that atomicMin / atomicMax fold it: jext[j] = (min x, min y,
+// max x, max y).  NaN boxes are skipped.  jext starts at (~0, ~0, 0, 0)
+// (job_extent_init_kernel), which decodes to NaN for a job with no rows.
+__device__ __forceinline__ unsigned long long ordered_f64(double d) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double from_ordered_f64(unsigned long long u) {
+    return __longlong_as_double((long long)((u >> 63) ? (u & 0x7FFFFFFFFFFFFFFFull) : ~u));
+}
+
+__global__ __launch_bounds__(256) void job_extent_init_kernel(int64_t n_jobs, ulonglong4* __restrict__ jext) {
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n_jobs; j += (int64_t)gridDim.x * 256)
+        jext[j] = make_ulonglong4(~0ull, ~0ull, 0ull, 0ull);
+}
+
+__global__ __launch_bounds__(256) void job_extent_kernel(const double4* __restrict__ b, int64_t n,
+                                                        const int32_t* __restrict__ row_job,
+                                                        ulonglong4* __restrict__ jext) {
+    // each wave folds a contiguous chunk of rows, carrying the running extent
+    // of its current job in registers and folding it into jext when the job
+    // changes: a few atomics per wave instead of four per row
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * 4, wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t chunk = (n + waves - 1) / waves;
+    const int64_t c0 = wv * chunk, c1 = c0 + chunk < n ? c0 + chunk : n;
+    int64_t cj = -1;
+    double rx0 = INFINITY, ry0 = INFINITY, rx1 = -INFINITY, ry1 = -INFINITY;
+    auto fold = [&]() {
+        if (cj >= 0 && lane == 0 && rx0 <= rx1) {
+            atomicMin(&jext[cj].x, ordered_f64(rx0));
+            atomicMin(&jext[cj].y, ordered_f64(ry0));
+            atomicMax(&jext[cj].z, ordered_f64(rx1));
+            atomicMax(&jext[cj].w, ordered_f64(ry1));
+        }
+    };
+    for (int64_t i0 = c0; i0 < c1; i0 += 64) {
+        const int64_t i = i0 + lane;
+        const bool in = i < c1;
+        int64_t j = -1;
+        double cx = NAN, cy = NAN;
+        if (in) {
+            j = row_job[i];
+            const double4 v = b[i];
+            cx = 0.5 * (v.x + v.z);
+            cy = 0.5 * (v.y + v.w);
+        }
+        const bool ok = in && cx == cx && cy == cy;
+        uint64_t rem = __ballot(in);
+        while (rem) {  // one pass per distinct job among the 64 rows (usually one)
+            const int sl = __ffsll((unsigned long long)rem) - 1;
+            const int64_t js = __shfl(j, sl, 64);
+            const bool mine = in && j == js;
+            rem &= ~__ballot(mine);
+            const double x0 = wave_min(mine && ok ? cx : INFINITY), y0 = wave_min(mine && ok ? cy : INFINITY);
+            const double x1 = wave_max(mine && ok ? cx : -INFINITY), y1 = wave_max(mine && ok ? cy : -INFINITY);
+            if (js != cj) {
+                fold();
+                cj = js;
+                rx0 = x0;
+                ry0 = y0;
+                rx1 = x1;
+                ry1 = y1;
+            } else {
+                rx0 = fmin(rx0, x0);
+                ry0 = fmin(ry0, y0);
+                rx1 = fmax(rx1, x1);
+                ry1 = fmax(ry1, y1);
+            }
+        }
+    }
+    fold();
+}
+
+// key = job << 16 | 8-level Hilbert code of the bbox centre over its job's
+// centre extent (NaN -> last): 16 + log2(jobs) bits to sort.
+constexpr int kJobKeyBits = 16;
 __global__ __launch_bounds__(256) void job_keys_kernel(const double4* __restrict__ b, int64_t n,
-                                                      const int64_t* __restrict__ job_off, int64_t n_jobs,
-                                                      const double4* __restrict__ ext, uint64_t* __restrict__ keys,
-                                                      int32_t* __restrict__ ids) {
+                                                      const int32_t* __restrict__ row_job,
+                                                      const ulonglong4* __restrict__ jext,
+                                                      uint64_t* __restrict__ keys, int32_t* __restrict__ ids) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const KeyScale ks = key_scale(ext, 16);
+    const int64_t j = row_job[i];
     const double4 v = b[i];
-    uint64_t h = 0xFFFFFFFFull;
+    uint64_t h = (1ull << kJobKeyBits) - 1;
     if (v.x == v.x && v.y == v.y && v.z == v.z && v.w == v.w) {
-        const uint32_t qx = quantize(0.5 * (v.x + v.z), ks.x0, ks.sx, 16) << 16;
-        const uint32_t qy = quantize(0.5 * (v.y + v.w), ks.y0, ks.sy, 16) << 16;
-        h = hilbert64(qx, qy) >> 32;
+        const ulonglong4 e = jext[j];
+        const double4 ext = make_double4(from_ordered_f64(e.x), from_ordered_f64(e.y), from_ordered_f64(e.z),
+                                         from_ordered_f64(e.w));
+        const KeyScale ks = key_scale(&ext, kJobKeyBits / 2);
+        const uint32_t qx = quantize(0.5 * (v.x + v.z), ks.x0, ks.sx, kJobKeyBits / 2) << (32 - kJobKeyBits / 2);
+        const uint32_t qy = quantize(0.5 * (v.y + v.w), ks.y0, ks.sy, kJobKeyBits / 2) << (32 - kJobKeyBits / 2);
+        h = hilbert64(qx, qy) >> (64 - kJobKeyBits);
     }
-    keys[i] = ((uint64_t)job_of(job_off, n_jobs, i) << 32) | h;
+    keys[i] = ((uint64_t)j << kJobKeyBits) | h;
     ids[i] = (int32_t)i;
 }
 
@@ -837,8 +926,11 @@ struct BatchWs {
     uint64_t* rkey;
     int32_t* rperm;
     void* sort_ws;
-    double4* ext;
-    void* ext_ws;
+    ulonglong4* jext;
+    int32_t* row_mark;
+    int32_t* row_job;
+    long long* total_rows;
+    void* row_scan_ws;
     int32_t* nodes;
     int32_t* tree_off;
     int32_t* waves;
@@ -886,8 +978,11 @@ static BatchWs carve_batch(void* base, int64_t n_jobs, int64_t n_left, int64_t n
     W.rperm = c.take<int32_t>(n_right);
     const int64_t nmax = n_left > n_right ? n_left : n_right;
     W.sort_ws = c.take<char>(sort_workspace_bytes_u64(nmax));
-    W.ext = c.take<double4>(1);
-    W.ext_ws = c.take<char>(extent_workspace_bytes());
+    W.jext = c.take<ulonglong4>(n_jobs);
+    W.row_mark = c.take<int32_t>(nmax);
+    W.row_job = c.take<int32_t>(nmax);
+    W.total_rows = c.take<long long>(1);
+    W.row_scan_ws = c.take<char>(scan_workspace_bytes(nmax));
     W.nodes = c.take<int32_t>(n_jobs);
     W.tree_off = c.take<int32_t>(n_jobs);
     W.waves = c.take<int32_t>(n_jobs);
@@ -953,11 +1048,29 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
                                    W.rinfo, static_cast<RowRec*>(nullptr), s));
         const int jb = bits_for_jobs(n_jobs);
         // right rows: (job, Hilbert) order, forest
-        VS_TRY(centre_extent(W.rbox, n_right, W.ext, W.ext_ws, s));
-        hipLaunchKernelGGL(job_keys_kernel, dim3((unsigned)ceil_div(n_right, 256)), dim3(256), 0, s, W.rbox, n_right,
-                           right_job, n_jobs, W.ext, W.rkey, W.rperm);
+        const unsigned jgrid = (unsigned)std::min<int64_t>(ceil_div(n_jobs, 256), (int64_t)cus * 4);
+        // job of every row (marks, scan, own mark), extents of the job centres, keys
+        auto row_jobs = [&](const int64_t* off, int64_t n) -> int {
+            fill_i32(W.row_mark, n, 0, s);
+            hipLaunchKernelGGL(job_marks_kernel, dim3(jgrid), dim3(256), 0, s, off, n_jobs, n, W.row_mark);
+            VS_LAUNCH_CHECK();
+            VS_TRY(exclusive_scan_i32(W.row_mark, W.row_job, n, W.total_rows, W.row_scan_ws, s));
+            hipLaunchKernelGGL(add_marks_kernel, dim3(stride_grid(n, 256, 8)), dim3(256), 0, s, W.row_mark, n,
+                               W.row_job);
+            VS_LAUNCH_CHECK();
+            hipLaunchKernelGGL(job_extent_init_kernel, dim3(jgrid), dim3(256), 0, s, n_jobs, W.jext);
+            VS_LAUNCH_CHECK();
+            return VS_OK;
+        };
+        VS_TRY(row_jobs(right_job, n_right));
+        hipLaunchKernelGGL(job_extent_kernel, dim3(stride_grid(n_right, 256, 4)), dim3(256), 0, s, W.rbox, n_right,
+                           W.row_job, W.jext);
         VS_LAUNCH_CHECK();
-        VS_TRY(sort_pairs_u64(W.rkey, W.rperm, n_right, 0, 32 + jb, W.sort_ws, sort_workspace_bytes_u64(n_right), s));
+        hipLaunchKernelGGL(job_keys_kernel, dim3((unsigned)ceil_div(n_right, 256)), dim3(256), 0, s, W.rbox, n_right,
+                           W.row_job, W.jext, W.rkey, W.rperm);
+        VS_LAUNCH_CHECK();
+        VS_TRY(sort_pairs_u64(W.rkey, W.rperm, n_right, 0, kJobKeyBits + jb, W.sort_ws,
+                              sort_workspace_bytes_u64(n_right), s));
         hipLaunchKernelGGL(job_sizes_kernel, dim3((unsigned)ceil_div(n_jobs, 256)), dim3(256), 0, s, left_job,
                            right_job, n_jobs, W.nodes, W.waves);
         VS_LAUNCH_CHECK();
@@ -977,11 +1090,15 @@ int batched_intersects(const Polys& A, int64_t n_left, const Polys& B, int64_t n
                            W.rbox, W.rperm, right_job, W.tree_off, n_jobs, W.fbox);
         VS_LAUNCH_CHECK();
         // left rows: (job, Hilbert) order, then the traversal
-        VS_TRY(centre_extent(W.lbox, n_left, W.ext, W.ext_ws, s));
-        hipLaunchKernelGGL(job_keys_kernel, dim3((unsigned)ceil_div(n_left, 256)), dim3(256), 0, s, W.lbox, n_left,
-                           left_job, n_jobs, W.ext, W.lkey, W.lperm);
+        VS_TRY(row_jobs(left_job, n_left));
+        hipLaunchKernelGGL(job_extent_kernel, dim3(stride_grid(n_left, 256, 4)), dim3(256), 0, s, W.lbox, n_left,
+                           W.row_job, W.jext);
         VS_LAUNCH_CHECK();
-        VS_TRY(sort_pairs_u64(W.lkey, W.lperm, n_left, 0, 32 + jb, W.sort_ws, sort_workspace_bytes_u64(n_left), s));
+        hipLaunchKernelGGL(job_keys_kernel, dim3((unsigned)ceil_div(n_left, 256)), dim3(256), 0, s, W.lbox, n_left,
+                           W.row_job, W.jext, W.lkey, W.lperm);
+        VS_LAUNCH_CHECK();
+        VS_TRY(sort_pairs_u64(W.lkey, W.lperm, n_left, 0, kJobKeyBits + jb, W.sort_ws,
+                              sort_workspace_bytes_u64(n_left), s));
         VS_TRY(timing_begin_public(s));
         hipLaunchKernelGGL(task_recs_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n_jobs, 4), (int64_t)cus * 8)),
                            dim3(256), 0, s, left_job, right_job, n_jobs, W.wave_off, W.tree_off, W.tasks);
