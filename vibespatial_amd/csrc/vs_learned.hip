@@ -955,14 +955,18 @@ __device__ int stage_all_edges(int32_t lo, int32_t hi, double cx, double cy, con
     int ne = 0;
     for (int32_t base = lo; base < hi; base += kLJThreads) {
         const int32_t c = base + tid;
-        const bool rel = c > lo && c < hi && !(R.flags[c] & 1);
+        const bool inr = c > lo && c < hi;
+        // flags and coordinates in one round of loads
         double ax = 0, ay = 0, bx = 0, by = 0;
-        if (rel) {
+        int f = 1;
+        if (inr) {
+            f = R.flags[c];
             ax = R.x[c - 1];
             ay = R.y[c - 1];
             bx = R.x[c];
             by = R.y[c];
         }
+        const bool rel = inr && !(f & 1);
         const int inc = wave_inclusive_scan(rel ? 1 : 0);
         if (lane == 63) s_wsum[wid] = inc;
         __syncthreads();
@@ -1053,12 +1057,13 @@ template <int MODE, int PRED>
 // block's item start-up is hidden by the others on the CU).
 __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
     int64_t nq, const int32_t* __restrict__ item_off, const long long* __restrict__ total_items,
-    const int32_t* __restrict__ item_q, int64_t item_cap, const int32_t* __restrict__ order,
+    const int4* __restrict__ item_rec, int64_t item_cap, const int32_t* __restrict__ order,
     const int4* __restrict__ poly_info, const int2* __restrict__ ranges, const int32_t* __restrict__ nranges,
     const double* __restrict__ sx, const double* __restrict__ sy, const int32_t* __restrict__ perm,
     const double4* __restrict__ qbox, RefineView R, uint32_t* __restrict__ stage_l, int32_t* __restrict__ stage_r,
     int64_t capacity, JoinCtl* __restrict__ ctl) {
-    __shared__ int s_item, s_q, s_chunk, s_ncand, s_nout, s_nr;
+    __shared__ int s_ncand, s_nout;
+    __shared__ int4 s_claim[kClaim];  // records (query, chunk, range count) of the block's claimed items
     __shared__ unsigned long long s_base;
     __shared__ int2 s_rng[kMaxRanges];
     __shared__ int s_pre[kMaxRanges + 1];  // exclusive prefix of the range lengths
@@ -1093,55 +1098,58 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
         if (tid == 0) s_nout = 0;
         __syncthreads();
     };
-    // Thread 0 claims kClaim consecutive items per queue atomic (the items are
-    // in descending cost order, so late claims are small): one same-address
-    // atomic per item would serialise in L2.
-    int claim_next = 0, claim_left = 0;
+    // Wave 0 claims kClaim consecutive items per queue atomic (the items are
+    // in descending cost order, so late claims are small: one same-address
+    // atomic per item would serialise in L2) and loads their records at once.
+    int claim_k = kClaim;  // the next claimed item to run (block-uniform)
     for (;;) {
-        if (tid == 0) {
-            if (claim_left == 0) {
-                claim_next = (int)atomicAdd(&ctl->queue[0], (unsigned)kClaim);
-                claim_left = kClaim;
-            }
-            const int item = claim_next++;
-            --claim_left;
-            s_item = item;
-            if ((long long)item < T) {
-                int64_t lo;
-                if (item < item_cap) {
-                    lo = item_q[item];
-                } else {  // beyond the map: last sorted index with item_off <= item
-                    lo = 0;
-                    int64_t hi = nq - 1;
-                    while (lo < hi) {
-                        const int64_t mid = (lo + hi + 1) >> 1;
-                        if (item_off[mid] <= item) lo = mid;
-                        else hi = mid - 1;
+        if (claim_k == kClaim) {
+            if (tid < 64) {
+                int cbase = 0;
+                if (lane == 0) cbase = (int)atomicAdd(&ctl->queue[0], (unsigned)kClaim);
+                cbase = __shfl(cbase, 0, 64);
+                if (lane < kClaim) {
+                    const int item = cbase + lane;
+                    int4 rec = make_int4(-1, 0, 0, 0);  // q < 0: past the last item
+                    if ((long long)item < T) {
+                        if (item < item_cap) {
+                            rec = item_rec[item];
+                        } else {  // beyond the map: last sorted index with item_off <= item
+                            int64_t lo = 0, hi = nq - 1;
+                            while (lo < hi) {
+                                const int64_t mid = (lo + hi + 1) >> 1;
+                                if (item_off[mid] <= item) lo = mid;
+                                else hi = mid - 1;
+                            }
+                            const int qq = order[lo];
+                            rec = make_int4(qq, item - item_off[lo], nranges[qq], 0);
+                        }
                     }
+                    s_claim[lane] = rec;
                 }
-                const int q = order[lo];
-                s_q = q;
-                s_chunk = item - item_off[lo];
-                s_nr = nranges[q];
             }
+            claim_k = 0;
+            __syncthreads();
         }
-        __syncthreads();
-        if ((long long)uniform_i32(s_item) >= T) break;
         // item values are block-uniform: kept in scalar registers
-        const int q = uniform_i32(s_q);
+        const int q = uniform_i32(s_claim[claim_k].x);
+        const int chunk = uniform_i32(s_claim[claim_k].y);
+        const int nr = uniform_i32(s_claim[claim_k].z);
+        ++claim_k;
+        if (q < 0) break;
+        // the query's box and polygon info load beside its ranges
+        const double4 box = qbox[q];
+        const int4 qinfo = MODE != kLModeBox ? poly_info[q] : make_int4(0, 0, 0, 0);
         if (tid < 64) {  // wave 0: the ranges and their exclusive prefix in one pass
-            const int nr0 = uniform_i32(s_nr);
-            const int2 rg = tid < nr0 ? ranges[(int64_t)q * kMaxRanges + tid] : make_int2(0, 0);
+            const int2 rg = tid < nr ? ranges[(int64_t)q * kMaxRanges + tid] : make_int2(0, 0);
             const int len = rg.y - rg.x;
             const int inc = wave_inclusive_scan(len);
-            if (tid < nr0) s_rng[tid] = rg;
-            if (tid <= nr0) s_pre[tid] = inc - len;
+            if (tid < nr) s_rng[tid] = rg;
+            if (tid <= nr) s_pre[tid] = inc - len;
         }
         __syncthreads();
-        const double4 box = qbox[q];
-        const int nr = uniform_i32(s_nr);
         const int total = uniform_i32(s_pre[nr]);
-        const int first = uniform_i32(s_chunk) * kItemPos;
+        const int first = chunk * kItemPos;
         const int last = (first + kItemPos < total) ? first + kItemPos : total;
         int4 info = make_int4(0, 0, 0, 0);
         double cx = 0, cy = 0;
@@ -1149,7 +1157,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
         int ne_fixed = -1;  // edges staged once for the whole item (small single-part polygons)
         Slabs fixed_slabs = {1, 0.0f, 0.0f};
         if (MODE != kLModeBox) {
-            info = poly_info[q];
+            info = qinfo;
             cx = 0.5 * (box.x + box.z);
             cy = 0.5 * (box.y + box.w);
             double M = fmax(fmax(box.z - cx, cx - box.x), fmax(box.w - cy, cy - box.y));
@@ -1173,7 +1181,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
             for (int r0 = first; r0 < last; r0 += kRoundPos) {
                 __syncthreads();  // s_nout is stable
                 if (s_nout > kOutCap - kRoundPos) flush();
-                int32_t pos[kPosPerThread];
+                int32_t pos[kPosPerThread], orig[kPosPerThread];
                 double px[kPosPerThread], py[kPosPerThread];
 #pragma unroll
                 for (int j = 0; j < kPosPerThread; ++j) {
@@ -1186,10 +1194,14 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                         pos[j] = s_rng[r].x + (off - s_pre[r]);
                     }
                 }
+                // coordinates and original ids in one round of loads (the ids of
+                // the scanned ranges are contiguous, so the extra ones cost
+                // little next to a dependent load after the refine)
 #pragma unroll
                 for (int j = 0; j < kPosPerThread; ++j) {
                     px[j] = pos[j] >= 0 ? sx[pos[j]] : NAN;
                     py[j] = pos[j] >= 0 ? sy[pos[j]] : NAN;
+                    orig[j] = pos[j] >= 0 ? perm[pos[j]] : 0;
                 }
 #pragma unroll
                 for (int j = 0; j < kPosPerThread; ++j) {
@@ -1215,11 +1227,9 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                         int base = 0;
                         if (lane == leader) base = atomicAdd(&s_nout, __popcll(m));
                         base = __shfl(base, leader, 64);
-                        if (keep) {
-                            const int32_t orig = perm[pos[j]];
+                        if (keep)
                             outb[base + __popcll(m & lanemask_lt())] =
-                                MODE == kLModePip ? make_int2(orig, q) : make_int2(q, orig);
-                        }
+                                MODE == kLModePip ? make_int2(orig[j], q) : make_int2(q, orig[j]);
                     }
                 }
             }
@@ -1421,15 +1431,22 @@ __global__ __launch_bounds__(256) void gather_counts_kernel(const int32_t* __res
     if (i < nq) out[i] = n_items[order[i]];
 }
 
-// item_q[k] = query owning item k, for the first `cap` items (items beyond
-// are located by binary search over item_off).
+// item_rec[k] = (query, chunk within the query, the query's range count, 0)
+// of item k, for the first `cap` items (items beyond are located by binary
+// search over item_off).  q runs over the queries in LPT order.
 __global__ __launch_bounds__(256) void item_map_kernel(const int32_t* __restrict__ item_off,
-                                                      const int32_t* __restrict__ n_items, int64_t nq,
-                                                      int32_t* __restrict__ item_q, int64_t cap) {
+                                                      const int32_t* __restrict__ n_items,
+                                                      const int32_t* __restrict__ order,
+                                                      const int32_t* __restrict__ nranges, int64_t nq,
+                                                      int4* __restrict__ item_rec, int64_t cap) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (q >= nq) return;
     const int64_t o = item_off[q];
-    for (int64_t k = 0; k < n_items[q] && o + k < cap; ++k) item_q[o + k] = (int32_t)q;
+    const int32_t n = n_items[q];
+    if (n == 0 || o >= cap) return;
+    const int32_t qq = order[q];
+    const int32_t nr = nranges[qq];
+    for (int32_t k = 0; k < n && o + k < cap; ++k) item_rec[o + k] = make_int4(qq, k, nr, 0);
 }
 
 __global__ void publish_learned_kernel(const JoinCtl* __restrict__ ctl, const unsigned long long* __restrict__ cst,
@@ -1584,7 +1601,7 @@ LearnedView<K> view_of(const IndexBufs<K>& B, int64_t n, int S, int mode) {
 }
 
 struct QueryBufs {
-    int32_t* item_q;
+    int4* item_rec;
     int64_t item_cap;
     int4* poly_info;
     uint32_t* cost_in;  // unsorted cost keys and query ids
@@ -1614,7 +1631,7 @@ inline int64_t item_cap_for(int64_t nq, int64_t n_left) { return 2 * nq + (8 * n
 QueryBufs carve_query(Carver& c, int64_t nq, int64_t n_left, int64_t capacity) {
     QueryBufs Q;
     Q.item_cap = item_cap_for(nq, n_left);
-    Q.item_q = c.take<int32_t>(Q.item_cap);
+    Q.item_rec = c.take<int4>(Q.item_cap);
     Q.poly_info = c.take<int4>(nq);
     Q.cost_in = c.take<uint32_t>(nq);
     Q.order_in = c.take<int32_t>(nq);
@@ -1662,8 +1679,8 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
         hipLaunchKernelGGL(gather_counts_kernel, dim3(gq), dim3(256), 0, s, Q.n_items, Q.order, nq, Q.n_items_sorted);
         VS_LAUNCH_CHECK();
         VS_TRY(exclusive_scan_i32(Q.n_items_sorted, Q.item_off, nq, Q.total_items, Q.scan_ws, s));
-        hipLaunchKernelGGL(item_map_kernel, dim3(gq), dim3(256), 0, s, Q.item_off, Q.n_items_sorted, nq, Q.item_q,
-                           Q.item_cap);
+        hipLaunchKernelGGL(item_map_kernel, dim3(gq), dim3(256), 0, s, Q.item_off, Q.n_items_sorted, Q.order,
+                           Q.nranges, nq, Q.item_rec, Q.item_cap);
         VS_LAUNCH_CHECK();
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {
@@ -1674,7 +1691,7 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
         VS_TRY(timing_begin_public(s));
 #define VS_LJ(M, P)                                                                                          \
     hipLaunchKernelGGL((learned_join_kernel<M, P>), dim3(grid), dim3(kLJThreads), 0, s, nq, Q.item_off,       \
-                       Q.total_items, Q.item_q, Q.item_cap, Q.order, Q.poly_info, Q.ranges, Q.nranges, B.sx, B.sy,  \
+                       Q.total_items, Q.item_rec, Q.item_cap, Q.order, Q.poly_info, Q.ranges, Q.nranges, B.sx, B.sy,  \
                        B.perm, qbox, R, Q.stage_l, Q.stage_r, capacity, Q.ctl)
         if (mode == kLModeBox) VS_LJ(kLModeBox, VS_PRED_INTERSECTS);
         else if (mode == kLModePipQ) {
