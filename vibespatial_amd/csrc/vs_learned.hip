@@ -986,6 +986,85 @@ __device__ int stage_all_edges(int32_t lo, int32_t hi, double cx, double cy, con
     return ne;
 }
 
+// stage_all_edges + build_slabs for a polygon whose edges fit one pass of the
+// block (hi - lo <= kLJThreads): each thread stages its own edge and counts
+// and lists it in its slabs from registers, so the pair takes four barriers
+// instead of eight.  The slabs and lists are those build_slabs makes (list
+// order within a slab aside, which no decision depends on).  Block-uniform.
+__device__ Slabs stage_and_bin(int32_t lo, int32_t hi, double cx, double cy, float ylo, float yhi,
+                               const RefineView& R, float4* Et, int32_t* Ec, int* s_wsum, int* start, int* cur,
+                               uint16_t* list) {
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    const int32_t c = lo + tid;
+    const bool inr = c > lo && c < hi;
+    double ax = 0, ay = 0, bx = 0, by = 0;
+    int f = 1;
+    if (inr) {
+        f = R.flags[c];
+        ax = R.x[c - 1];
+        ay = R.y[c - 1];
+        bx = R.x[c];
+        by = R.y[c];
+    }
+    const bool rel = inr && !(f & 1);
+    if (tid <= kMaxBins) start[tid] = 0;
+    const int inc = wave_inclusive_scan(rel ? 1 : 0);
+    if (lane == 63) s_wsum[wid] = inc;
+    __syncthreads();
+    int woff = 0, ne = 0;
+    for (int w2 = 0; w2 < kLJThreads / 64; ++w2) {
+        woff += w2 < wid ? s_wsum[w2] : 0;
+        ne += s_wsum[w2];
+    }
+    Slabs S;
+    S.B = 1;
+    while (S.B < kMaxBins && S.B * 4 < ne) S.B <<= 1;
+    S.ylo = ylo;
+    S.inv = (yhi > ylo) ? (float)S.B / (yhi - ylo) : 0.0f;
+    if (!(S.inv > 0.0f) || !(S.inv < INFINITY)) {
+        S.B = 1;
+        S.inv = 0.0f;
+    }
+    const int slot = woff + inc - 1;
+    int b0 = 0, b1 = -1;
+    if (rel) {
+        const float4 ed = make_float4((float)(ax - cx), (float)(ay - cy), (float)(bx - cx), (float)(by - cy));
+        Et[slot] = ed;
+        Ec[slot] = c;
+        if (S.B > 1) {
+            b0 = slab_of(fminf(ed.y, ed.w), S);
+            b1 = slab_of(fmaxf(ed.y, ed.w), S);
+            for (int b = b0; b <= b1; ++b) atomicAdd(&start[b + 1], 1);
+        }
+    }
+    __syncthreads();
+    if (S.B > 1) {
+        if (tid < 64) {  // counts -> slab starts, and the fill cursors beside them
+            const int v = tid < S.B ? start[tid + 1] : 0;
+            const int incs = wave_inclusive_scan(v);
+            if (tid < S.B) {
+                start[tid + 1] = incs;
+                cur[tid] = incs - v;
+            }
+        }
+        __syncthreads();
+        if (start[S.B] <= kBinCap) {
+            for (int b = b0; b <= b1; ++b) list[atomicAdd(&cur[b], 1)] = (uint16_t)slot;
+            __syncthreads();
+            return S;
+        }
+        S.B = 1;  // too many long edges for the slab lists: one slab (start[0..1] below, read of start[S.B >= 2] above)
+        S.inv = 0.0f;
+    }
+    if (tid == 0) {
+        start[0] = 0;
+        start[1] = ne;
+    }
+    for (int e = tid; e < ne; e += kLJThreads) list[e] = (uint16_t)e;
+    __syncthreads();
+    return S;
+}
+
 // LDS buffers of the staged edges and their slab lists.
 struct EdgeLds {
     float4* Et;
@@ -1147,6 +1226,9 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
             if (tid < nr) s_rng[tid] = rg;
             if (tid <= nr) s_pre[tid] = inc - len;
         }
+        // the block's buffered pair count, read by every thread before the
+        // barrier that precedes the item's first append
+        const int nout_item = uniform_i32(s_nout);
         __syncthreads();
         const int total = uniform_i32(s_pre[nr]);
         const int first = chunk * kItemPos;
@@ -1171,18 +1253,13 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
             // Box queries and small single-part polygons: every thread scans
             // kPosPerThread positions per round and refines its own candidates
             // at once against the item's slab lists (no candidate compaction).
-            Slabs S = {1, 0.0f, 0.0f};
-            if (MODE != kLModeBox) {
-                const int ne = stage_all_edges(info.x, info.y, cx, cy, R, Et, Ec, s_wsum);
-                S = build_slabs(Et, ne, (float)(box.y - cy), (float)(box.w - cy), s_bstart, s_bcur, s_blist);
-            }
-            unsigned n_orient = 0;
-            unsigned long long n_esc = 0;
-            for (int r0 = first; r0 < last; r0 += kRoundPos) {
-                __syncthreads();  // s_nout is stable
-                if (s_nout > kOutCap - kRoundPos) flush();
-                int32_t pos[kPosPerThread], orig[kPosPerThread];
-                double px[kPosPerThread], py[kPosPerThread];
+            int32_t pos[kPosPerThread], orig[kPosPerThread];
+            double px[kPosPerThread], py[kPosPerThread];
+            // a round's positions, their coordinates and original ids in one
+            // round of loads (the ids of the scanned ranges are contiguous, so
+            // the extra ones cost little next to a dependent load after the
+            // refine)
+            auto load_round = [&](int r0) {
 #pragma unroll
                 for (int j = 0; j < kPosPerThread; ++j) {
                     const int off = r0 + tid + j * kLJThreads;
@@ -1194,15 +1271,41 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                         pos[j] = s_rng[r].x + (off - s_pre[r]);
                     }
                 }
-                // coordinates and original ids in one round of loads (the ids of
-                // the scanned ranges are contiguous, so the extra ones cost
-                // little next to a dependent load after the refine)
 #pragma unroll
                 for (int j = 0; j < kPosPerThread; ++j) {
                     px[j] = pos[j] >= 0 ? sx[pos[j]] : NAN;
                     py[j] = pos[j] >= 0 ? sy[pos[j]] : NAN;
                     orig[j] = pos[j] >= 0 ? perm[pos[j]] : 0;
                 }
+            };
+            // the first round's loads go out before the edge staging's, so the
+            // two round trips overlap
+            load_round(first);
+            Slabs S = {1, 0.0f, 0.0f};
+            if (MODE != kLModeBox) {
+                const float ylo = (float)(box.y - cy), yhi = (float)(box.w - cy);
+                if (info.y - info.x <= kLJThreads) {
+                    S = stage_and_bin(info.x, info.y, cx, cy, ylo, yhi, R, Et, Ec, s_wsum, s_bstart, s_bcur, s_blist);
+                } else {
+                    const int ne = stage_all_edges(info.x, info.y, cx, cy, R, Et, Ec, s_wsum);
+                    S = build_slabs(Et, ne, ylo, yhi, s_bstart, s_bcur, s_blist);
+                }
+            }
+            unsigned n_orient = 0;
+            unsigned long long n_esc = 0;
+            for (int r0 = first; r0 < last; r0 += kRoundPos) {
+                // the flush test must see one value in every wave: a count read
+                // between the previous round's appends and a barrier that this
+                // round's appends follow (for the first round, the count read
+                // before the ranges barrier)
+                int nout_now = nout_item;
+                if (r0 != first) {
+                    __syncthreads();
+                    nout_now = uniform_i32(s_nout);
+                    load_round(r0);
+                    __syncthreads();
+                }
+                if (nout_now > kOutCap - kRoundPos) flush();
 #pragma unroll
                 for (int j = 0; j < kPosPerThread; ++j) {
                     const bool inbox = px[j] >= box.x && px[j] <= box.z && py[j] >= box.y && py[j] <= box.w;
