@@ -68,7 +68,10 @@ constexpr int kChunk = 512;                // positions scanned per round
 constexpr int kItemRounds = 8;             // rounds per work item (items span kChunk * kItemRounds positions)
 constexpr int kItemPos = kChunk * kItemRounds;
 constexpr int kLJThreads = 256;
-constexpr int kClaim = 4;                  // work items taken per queue atomic
+constexpr int kClaim = 4;                  // work items taken per queue atomic (one per wave in wave mode)
+constexpr int kWavePos = 256;              // wave mode: at most this many positions per item (4 per lane)
+constexpr int kWaveEdges = 128;            // wave mode: at most this many edges per polygon (2 per lane)
+static_assert(kLJThreads / 64 == kClaim, "wave mode runs one claimed item per wave");
 constexpr int kCandPerThread = kChunk / kLJThreads;
 
 // ---------------------------------------------------------------------------
@@ -696,8 +699,9 @@ __global__ __launch_bounds__(256) void cover_lookup_kernel(int64_t nq, const uns
 
 // Per query: total scan length and its number of work items (kItemPos
 // positions each); stat[3] += positions scanned.  Also the query's cost key
-// for the longest-first order: 255 - 8 log2(1 + items x vertices) (8 bits,
-// ascending = most expensive first) with idx[q] = q.
+// for the longest-first order, 8 bits, ascending = most expensive first:
+// (127 or, for a small item, 255) - min(127, 4 log2(1 + items x vertices)),
+// with idx[q] = q.
 __global__ __launch_bounds__(256) void cover_total_kernel(int64_t nq, const int2* __restrict__ ranges,
                                                          const int32_t* __restrict__ nranges,
                                                          const int4* __restrict__ poly_info,
@@ -715,10 +719,16 @@ __global__ __launch_bounds__(256) void cover_total_kernel(int64_t nq, const int2
         const int32_t items = (int32_t)((total + kItemPos - 1) / kItemPos);
         n_items[q] = items;
         scan[0] += (unsigned long long)total;
-        const double v = poly_info ? (double)(poly_info[q].y - poly_info[q].x) : 1.0;
+        const int4 info = poly_info ? poly_info[q] : make_int4(0, 0, 1, 0);
+        const double v = poly_info ? (double)(info.y - info.x) : 1.0;
         const double cost = (double)items * (v > 1.0 ? v : 1.0);
-        const int lg = (int)(8.0 * log2(1.0 + cost));
-        key[q] = (uint32_t)(255 - (lg > 255 ? 255 : lg));
+        int lg = (int)(4.0 * log2(1.0 + cost));
+        lg = lg > 127 ? 127 : lg;
+        // small items (one wave's positions, a single part of at most
+        // kWaveEdges edges) sort after all others: a block that claims four of
+        // them runs one per wave
+        const bool small = total <= kWavePos && info.z == 1 && info.y - info.x <= kWaveEdges + 1;
+        key[q] = (uint32_t)((small ? 255 : 127) - lg);
         idx[q] = (int32_t)q;
     }
     block_add_counters<1>(scan, red, stat + 3);
@@ -986,6 +996,14 @@ __device__ int stage_all_edges(int32_t lo, int32_t hi, double cx, double cy, con
     return ne;
 }
 
+// Orders one wave's LDS accesses around a point where its lanes exchange data
+// through LDS (a compiler barrier: a wave's LDS instructions run in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // stage_all_edges + build_slabs for a polygon whose edges fit one pass of the
 // block (hi - lo <= kLJThreads): each thread stages its own edge and counts
 // and lists it in its slabs from registers, so the pair takes four barriers
@@ -1189,7 +1207,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                 cbase = __shfl(cbase, 0, 64);
                 if (lane < kClaim) {
                     const int item = cbase + lane;
-                    int4 rec = make_int4(-1, 0, 0, 0);  // q < 0: past the last item
+                    int4 rec = make_int4(-1, 0, 0, 1);  // q < 0: past the last item (no work in wave mode)
                     if ((long long)item < T) {
                         if (item < item_cap) {
                             rec = item_rec[item];
@@ -1207,8 +1225,189 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                     s_claim[lane] = rec;
                 }
             }
+            // the buffered pair count, read by every thread before the barrier
+            // that precedes any append of the claimed items
+            const int nout_claim = uniform_i32(s_nout);
             claim_k = 0;
             __syncthreads();
+            const int4 c0 = s_claim[0], c1 = s_claim[1], c2 = s_claim[2], c3 = s_claim[3];
+            if (uniform_i32(c0.x) >= 0 && uniform_i32(c0.w & c1.w & c2.w & c3.w)) {
+                // Wave mode: four small items, one per wave, each synchronised
+                // within its wave only (no block barrier per item).  Each
+                // appends at most kWavePos pairs.
+                if (nout_claim > kOutCap - kClaim * kWavePos) flush();
+                const int4 rec = s_claim[wid];
+                const int wq = uniform_i32(rec.x), wnr = uniform_i32(rec.z);
+                if (wq >= 0) {
+                    // ranges in lanes: lane k < wnr holds range k and the
+                    // exclusive prefix of the range lengths
+                    const int2 rg = lane < wnr ? ranges[(int64_t)wq * kMaxRanges + lane] : make_int2(0, 0);
+                    const double4 wbox = qbox[wq];
+                    const int4 winfo = MODE != kLModeBox ? poly_info[wq] : make_int4(0, 0, 0, 0);
+                    const int len = rg.y - rg.x;
+                    const int winc = wave_inclusive_scan(len);
+                    const int wpre = winc - len;
+                    const int wtotal = wnr > 0 ? readlane_i32(winc, wnr - 1) : 0;
+                    int32_t wpos[kPosPerThread], worig[kPosPerThread];
+                    double wpx[kPosPerThread], wpy[kPosPerThread];
+#pragma unroll
+                    for (int j = 0; j < kPosPerThread; ++j) {
+                        const int off = lane + 64 * j;
+                        int r = 0;  // last range with prefix <= off (every lane takes part in the shuffles)
+                        for (int step = kMaxRanges / 2; step > 0; step >>= 1) {
+                            const int cand_r = r + step;
+                            const int pv = __shfl(wpre, cand_r, 64);
+                            if (cand_r < wnr && pv <= off) r = cand_r;
+                        }
+                        const int rx = __shfl(rg.x, r, 64), rp = __shfl(wpre, r, 64);
+                        wpos[j] = off < wtotal ? rx + (off - rp) : -1;
+                    }
+#pragma unroll
+                    for (int j = 0; j < kPosPerThread; ++j) {
+                        wpx[j] = wpos[j] >= 0 ? sx[wpos[j]] : NAN;
+                        wpy[j] = wpos[j] >= 0 ? sy[wpos[j]] : NAN;
+                        worig[j] = wpos[j] >= 0 ? perm[wpos[j]] : 0;
+                    }
+                    double wcx = 0, wcy = 0;
+                    float wE = 0.0f;
+                    Slabs WS = {1, 0.0f, 0.0f};
+                    // this wave's quarter of the edge, slab and list buffers
+                    float4* wEt = Et + wid * (kEdgeCap / kClaim);
+                    int32_t* wEc = Ec + wid * (kEdgeCap / kClaim);
+                    uint16_t* wlist = s_blist + wid * (kBinCap / kClaim);
+                    int* wstart = cand + wid * (kChunk / kClaim);  // cand is free in wave mode
+                    int* wcur = wstart + kMaxBins / 2 + 1;
+                    if (MODE != kLModeBox) {
+                        wcx = 0.5 * (wbox.x + wbox.z);
+                        wcy = 0.5 * (wbox.y + wbox.w);
+                        double M = fmax(fmax(wbox.z - wcx, wcx - wbox.x), fmax(wbox.w - wcy, wcy - wbox.y));
+                        M *= 1.0 + 4.440892098500626e-16;
+                        wE = uniform_f32(fp32_det_bound(M));
+                        wcx = uniform_f64(wcx);
+                        wcy = uniform_f64(wcy);
+                        // edges (c-1, c), c = lo + 1 + lane + 64 k: two per lane
+                        const int32_t lo = winfo.x, hi = winfo.y;
+                        float4 ed[2];
+                        int eslot[2];
+                        bool erel[2];
+                        int ne = 0;
+#pragma unroll
+                        for (int k = 0; k < 2; ++k) {
+                            const int32_t c = lo + 1 + lane + 64 * k;
+                            double ax = 0, ay = 0, bx = 0, by = 0;
+                            int f = 1;
+                            if (c < hi) {
+                                f = R.flags[c];
+                                ax = R.x[c - 1];
+                                ay = R.y[c - 1];
+                                bx = R.x[c];
+                                by = R.y[c];
+                            }
+                            erel[k] = c < hi && !(f & 1);
+                            ed[k] = make_float4((float)(ax - wcx), (float)(ay - wcy), (float)(bx - wcx),
+                                                (float)(by - wcy));
+                            const int einc = wave_inclusive_scan(erel[k] ? 1 : 0);
+                            eslot[k] = ne + einc - 1;
+                            ne += readlane_i32(einc, 63);
+                            if (erel[k]) {
+                                wEt[eslot[k]] = ed[k];
+                                wEc[eslot[k]] = c;
+                            }
+                        }
+                        // y-slabs as build_slabs makes them, within the wave
+                        WS.B = 1;
+                        while (WS.B < kMaxBins / 2 && WS.B * 4 < ne) WS.B <<= 1;
+                        const float ylo = (float)(wbox.y - wcy), yhi = (float)(wbox.w - wcy);
+                        WS.ylo = ylo;
+                        WS.inv = (yhi > ylo) ? (float)WS.B / (yhi - ylo) : 0.0f;
+                        if (!(WS.inv > 0.0f) || !(WS.inv < INFINITY)) {
+                            WS.B = 1;
+                            WS.inv = 0.0f;
+                        }
+                        bool listed = false;
+                        if (WS.B > 1) {
+                            if (lane <= WS.B) wstart[lane] = 0;
+                            wave_lds_sync();
+                            int eb0[2], eb1[2];
+#pragma unroll
+                            for (int k = 0; k < 2; ++k) {
+                                eb0[k] = 0;
+                                eb1[k] = -1;
+                                if (erel[k]) {
+                                    eb0[k] = slab_of(fminf(ed[k].y, ed[k].w), WS);
+                                    eb1[k] = slab_of(fmaxf(ed[k].y, ed[k].w), WS);
+                                    for (int b = eb0[k]; b <= eb1[k]; ++b) atomicAdd(&wstart[b + 1], 1);
+                                }
+                            }
+                            wave_lds_sync();
+                            const int v = lane < WS.B ? wstart[lane + 1] : 0;
+                            const int incs = wave_inclusive_scan(v);
+                            if (lane < WS.B) {
+                                wstart[lane + 1] = incs;
+                                wcur[lane] = incs - v;
+                            }
+                            const int wtot = readlane_i32(incs, 63);
+                            wave_lds_sync();
+                            if (wtot <= kBinCap / kClaim) {
+#pragma unroll
+                                for (int k = 0; k < 2; ++k)
+                                    for (int b = eb0[k]; b <= eb1[k]; ++b)
+                                        wlist[atomicAdd(&wcur[b], 1)] = (uint16_t)eslot[k];
+                                listed = true;
+                            } else {  // too many long edges: one slab
+                                WS.B = 1;
+                                WS.inv = 0.0f;
+                            }
+                        }
+                        if (!listed) {
+                            if (lane == 0) {
+                                wstart[0] = 0;
+                                wstart[1] = ne;
+                            }
+#pragma unroll
+                            for (int k = 0; k < 2; ++k)
+                                if (erel[k]) wlist[eslot[k]] = (uint16_t)eslot[k];
+                        }
+                        wave_lds_sync();
+                    }
+                    unsigned n_orient = 0;
+                    unsigned long long n_esc = 0;
+#pragma unroll
+                    for (int j = 0; j < kPosPerThread; ++j) {
+                        const bool inbox =
+                            wpx[j] >= wbox.x && wpx[j] <= wbox.z && wpy[j] >= wbox.y && wpy[j] <= wbox.w;
+                        bool keep = inbox;
+                        if (MODE != kLModeBox) {
+                            bool esc = false;
+                            if (inbox) {
+                                const int loc = refine_one(wEt, wEc, wstart, wlist, WS, R, (float)(wpx[j] - wcx),
+                                                           (float)(wpy[j] - wcy), wpx[j], wpy[j], wE, esc, n_orient,
+                                                           n_esc);
+                                keep = loc_pred<PRED>(loc);
+                            }
+                            const unsigned nb = (unsigned)__popcll(__ballot(inbox));
+                            st_cand += nb;
+                            st_cverts += (unsigned long long)nb * (unsigned long long)(winfo.y - winfo.x);
+                            st_escpairs += (unsigned)__popcll(__ballot(esc));
+                        }
+                        const uint64_t m = __ballot(keep);
+                        if (m) {
+                            const int leader = __ffsll((unsigned long long)m) - 1;
+                            int base = 0;
+                            if (lane == leader) base = atomicAdd(&s_nout, __popcll(m));
+                            base = __shfl(base, leader, 64);
+                            if (keep)
+                                outb[base + __popcll(m & lanemask_lt())] =
+                                    MODE == kLModePip ? make_int2(worig[j], wq) : make_int2(wq, worig[j]);
+                        }
+                    }
+                    st_orient += n_orient;
+                    st_esc += n_esc;
+                }
+                __syncthreads();
+                claim_k = kClaim;
+                continue;
+            }
         }
         // item values are block-uniform: kept in scalar registers
         const int q = uniform_i32(s_claim[claim_k].x);
@@ -1534,12 +1733,13 @@ __global__ __launch_bounds__(256) void gather_counts_kernel(const int32_t* __res
     if (i < nq) out[i] = n_items[order[i]];
 }
 
-// item_rec[k] = (query, chunk within the query, the query's range count, 0)
-// of item k, for the first `cap` items (items beyond are located by binary
+// item_rec[k] = (query, chunk within the query, the query's range count,
+// small-item flag) of item k, for the first `cap` items (items beyond are located by binary
 // search over item_off).  q runs over the queries in LPT order.
 __global__ __launch_bounds__(256) void item_map_kernel(const int32_t* __restrict__ item_off,
                                                       const int32_t* __restrict__ n_items,
                                                       const int32_t* __restrict__ order,
+                                                      const uint32_t* __restrict__ cost_key,
                                                       const int32_t* __restrict__ nranges, int64_t nq,
                                                       int4* __restrict__ item_rec, int64_t cap) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1549,7 +1749,8 @@ __global__ __launch_bounds__(256) void item_map_kernel(const int32_t* __restrict
     if (n == 0 || o >= cap) return;
     const int32_t qq = order[q];
     const int32_t nr = nranges[qq];
-    for (int32_t k = 0; k < n && o + k < cap; ++k) item_rec[o + k] = make_int4(qq, k, nr, 0);
+    const int32_t small = cost_key[q] >= 128u;  // see cover_total_kernel
+    for (int32_t k = 0; k < n && o + k < cap; ++k) item_rec[o + k] = make_int4(qq, k, nr, small);
 }
 
 __global__ void publish_learned_kernel(const JoinCtl* __restrict__ ctl, const unsigned long long* __restrict__ cst,
@@ -1783,7 +1984,7 @@ int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const L
         VS_LAUNCH_CHECK();
         VS_TRY(exclusive_scan_i32(Q.n_items_sorted, Q.item_off, nq, Q.total_items, Q.scan_ws, s));
         hipLaunchKernelGGL(item_map_kernel, dim3(gq), dim3(256), 0, s, Q.item_off, Q.n_items_sorted, Q.order,
-                           Q.nranges, nq, Q.item_rec, Q.item_cap);
+                           Q.cost_key, Q.nranges, nq, Q.item_rec, Q.item_cap);
         VS_LAUNCH_CHECK();
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {
