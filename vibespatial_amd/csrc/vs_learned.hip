@@ -859,7 +859,7 @@ __device__ Slabs build_slabs(const float4* __restrict__ Et, int ne, float ylo, f
     const int tid = threadIdx.x;
     Slabs S;
     S.B = 1;
-    while (S.B < kMaxBins && S.B * 4 < ne) S.B <<= 1;
+    while (S.B < kMaxBins && S.B * 2 < ne) S.B <<= 1;
     S.ylo = ylo;
     S.inv = (yhi > ylo) ? (float)S.B / (yhi - ylo) : 0.0f;
     if (!(S.inv > 0.0f) || !(S.inv < INFINITY)) {
@@ -943,7 +943,25 @@ __device__ __forceinline__ int refine_one(const float4* __restrict__ Et, const i
     int par = 0;
     const int b = slab_of(pv, S);
     const int k1 = start[b + 1];
-    for (int k = start[b]; k < k1; ++k) {
+    int k = start[b];
+    // two edges per step: both list entries, then both edges, load together
+    for (; k + 1 < k1; k += 2) {
+        const int e0 = list[k], e1 = list[k + 1];
+        const float4 d0 = Et[e0], d1 = Et[e1];
+        if (edge_step_f32(pu, pv, d0.x, d0.y, d0.z, d0.w, E, par, n_orient)) {
+            ++n_esc;
+            esc = true;
+            const int32_t c = Ec[e0];
+            if (edge_step_f64(pxd, pyd, R.x[c - 1], R.y[c - 1], R.x[c], R.y[c], par)) return 1;
+        }
+        if (edge_step_f32(pu, pv, d1.x, d1.y, d1.z, d1.w, E, par, n_orient)) {
+            ++n_esc;
+            esc = true;
+            const int32_t c = Ec[e1];
+            if (edge_step_f64(pxd, pyd, R.x[c - 1], R.y[c - 1], R.x[c], R.y[c], par)) return 1;
+        }
+    }
+    if (k < k1) {
         const int e = list[k];
         const float4 ed = Et[e];
         if (edge_step_f32(pu, pv, ed.x, ed.y, ed.z, ed.w, E, par, n_orient)) {
@@ -1036,7 +1054,7 @@ __device__ Slabs stage_and_bin(int32_t lo, int32_t hi, double cx, double cy, flo
     }
     Slabs S;
     S.B = 1;
-    while (S.B < kMaxBins && S.B * 4 < ne) S.B <<= 1;
+    while (S.B < kMaxBins && S.B * 2 < ne) S.B <<= 1;
     S.ylo = ylo;
     S.inv = (yhi > ylo) ? (float)S.B / (yhi - ylo) : 0.0f;
     if (!(S.inv > 0.0f) || !(S.inv < INFINITY)) {
@@ -1316,7 +1334,7 @@ __global__ __launch_bounds__(kLJThreads, 4) void learned_join_kernel(
                         }
                         // y-slabs as build_slabs makes them, within the wave
                         WS.B = 1;
-                        while (WS.B < kMaxBins / 2 && WS.B * 4 < ne) WS.B <<= 1;
+                        while (WS.B < kMaxBins / 2 && WS.B * 2 < ne) WS.B <<= 1;
                         const float ylo = (float)(wbox.y - wcy), yhi = (float)(wbox.w - wcy);
                         WS.ylo = ylo;
                         WS.inv = (yhi > ylo) ? (float)WS.B / (yhi - ylo) : 0.0f;
