@@ -278,11 +278,11 @@ def main(argv=None):
 # HBM bytes per learned_join_kernel launch on the default workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch, mean of the
 # warm-up and the timed dispatch) scaled by what the same counters read for a
-# 1 GiB device-to-device copy in the same run (FETCH_SIZE 524,319.7 KiB ->
-# x1.99988, WRITE_SIZE 1,048,576 KiB -> x1.0):
-#   (1.99988 * 329,035.8 + 1.0 * 81,614.2) KiB = 757,397,518 bytes.
+# 1 GiB device-to-device copy in the same run (FETCH_SIZE 524,316.5 KiB ->
+# x1.99989, WRITE_SIZE 1,048,576 KiB -> x1.0):
+#   (1.99989 * 353,285.6 + 1.0 * 97,870.5) KiB = 823,708,818 bytes.
 # The summary with both commands is profiles/learned_join_kernel_10Mpts_100kpolys_pmc_traffic.md.
-_TRAFFIC_BYTES = {(10_000_000, 100_000): 757_397_518}
+_TRAFFIC_BYTES = {(10_000_000, 100_000): 823_708_818}
 
 
 _TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of learned_join_kernel on this workload, scaled by a "
