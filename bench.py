@@ -75,6 +75,9 @@ def main(argv=None):
     ap.add_argument("--tile", default="", metavar="W,T,O",
                     help="c5: use this tile (traversal waves per block, refine block, candidate order) instead "
                          "of autotuning, and skip the ablation (for profiler runs)")
+    ap.add_argument("--tree-fanout", type=int, default=64,
+                    help="c4: fanout of the point tree of the tree form (one query box per wave tests a node's "
+                         "children on its lanes, so 64 fills the wave)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once (a kernel of known traffic for "
                          "scaling hardware byte counters)")
@@ -399,7 +402,7 @@ def run_c4(args):
 
     def tree_step(caps):
         pts = torch.stack([Q.x, Q.y, Q.x, Q.y], dim=1).contiguous()
-        tree = J.build_tree(pts)
+        tree = J.build_tree(pts, fanout=args.tree_fanout)
         del pts
         hits, out = 0, []
         for i, b in enumerate(batches):
@@ -460,7 +463,7 @@ def run_c4(args):
             "hits": hl,
             "positions_scanned": ex_l.get("scanned"),
             "mlp_leaf_error": ex_l.get("mlp_leaf_error"),
-            "tree": {"value": ht / (ms_t * 1e-3), "ms_per_step": ms_t},
+            "tree": {"value": ht / (ms_t * 1e-3), "ms_per_step": ms_t, "fanout": args.tree_fanout},
             "learned_vs_tree": ms_t / ms_l,
         },
         "roofline": {"bound": "hbm", "kernel": "whole step (build + query)", "achieved": alg / (ms_l * 1e-3) / 1e9,

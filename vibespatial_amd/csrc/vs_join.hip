@@ -50,7 +50,8 @@ int point_order(const double* px, const double* py, int64_t n, int bits, int32_t
 
 constexpr int kJoinThreads = 256;
 constexpr int kJoinWaves = kJoinThreads / 64;
-constexpr int kStackMax = 256;
+constexpr int kStackMax = 256;     // per-wave traversal stack (point queries)
+constexpr int kStackMaxBox = 512;  // box queries: one box per wave, so a 64-wide tree fits
 constexpr int kTopMax = 640;  // 20 KB of fp64 node boxes staged per block
 
 struct TreeView {
@@ -77,6 +78,7 @@ struct PolyView {
 };
 
 constexpr int kOutBuf = 256;  // per-wave LDS buffer of refined pairs
+constexpr int kBoxClaim = 4;  // box mode: query boxes per queue atomic
 
 
 enum { kModePip = 0, kModeBox = 1 };
@@ -166,15 +168,17 @@ __device__ bool refine_pip_wave(double px, double py, bool active, int32_t c0, i
 }
 
 // Persistent traversal kernel.  MODE kModePip: queries are points (qa, qb =
-// sorted x, y) with fused refine.  MODE kModeBox: queries are boxes given as
-// (qbox, perm) and every bbox pair is emitted.
+// sorted x, y), 64 per wave, with fused refine.  MODE kModeBox: queries are
+// boxes given as (qbox, perm), one per wave (claimed kBoxClaim at a time):
+// the lanes test a node's children, and a leaf group's hits are appended with
+// one ballot; every bbox pair is emitted.
 template <int MODE, int PRED>
 __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
     const double* __restrict__ qx, const double* __restrict__ qy, const double4* __restrict__ qbox,
     const int32_t* __restrict__ perm, int64_t n, TreeView T, PolyView P, uint32_t* __restrict__ stage_l,
     int32_t* __restrict__ stage_r, int64_t capacity, JoinCtl* __restrict__ ctl) {
     __shared__ double4 top[kTopMax];
-    __shared__ long long stack_s[kJoinWaves][kStackMax];
+    __shared__ long long stack_s[kJoinWaves][MODE == kModeBox ? kStackMaxBox : kStackMax];
     __shared__ int2 outbuf_s[kJoinWaves][kOutBuf];
     for (int i = threadIdx.x; i < T.n_top; i += kJoinThreads) top[i] = T.box[i];
     __syncthreads();
@@ -202,7 +206,8 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         nbuf = 0;
     };
-    const int64_t W = (n + 63) >> 6;
+    // work units: 64 points (kModePip) or kBoxClaim boxes (kModeBox)
+    const int64_t W = MODE == kModeBox ? (n + kBoxClaim - 1) / kBoxClaim : (n + 63) >> 6;
     const int home = blockIdx.x & 7;
     const int last = T.n_levels - 1;
     const int F = T.fanout;
@@ -216,6 +221,52 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
         t = __shfl(t, 0, 64);
         if ((int64_t)t >= r1 - r0) {
             ++xq;
+            continue;
+        }
+        if (MODE == kModeBox) {
+            for (int bq = 0; bq < kBoxClaim; ++bq) {
+                const int64_t qi = (r0 + t) * kBoxClaim + bq;  // wave-uniform
+                if (qi >= n) break;
+                const int32_t orig = perm ? perm[qi] : (int32_t)qi;
+                const double4 qb = qbox[orig];
+                if (!(qb.x == qb.x && qb.y == qb.y && qb.z == qb.z && qb.w == qb.w) || T.level_count[0] <= 0)
+                    continue;
+                int sp = 1;
+                if (lane == 0) stack[0] = 0;  // group 0 of level 0 = the root
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                while (sp > 0) {
+                    --sp;
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const long long e = stack[sp];
+                    const int l = (int)(e & 15);
+                    const int64_t first = (e >> 4) * F;
+                    const int64_t cl = T.level_count[l];
+                    const int ng = (int)((cl - first) < F ? (cl - first) : F);
+                    const int64_t g = T.level_start[l] + first + lane;
+                    bool ov = false;
+                    if (lane < ng) {
+                        const double4 b = (g < T.n_top) ? top[g] : T.box[g];
+                        ov = box_overlap(qb.x, qb.y, qb.z, qb.w, b.x, b.y, b.z, b.w);
+                    }
+                    uint64_t m = __ballot(ov);
+                    if (l == last) {
+                        if (m) {  // this leaf group's hits, one lane each
+                            const int k = __popcll(m);
+                            if (nbuf + k > kOutBuf) flush();
+                            if (ov) outbuf[nbuf + __popcll(m & lt)] = make_int2(orig, T.item_ids[first + lane]);
+                            nbuf += k;
+                        }
+                    } else {
+                        while (m) {
+                            const int s2 = 63 - __clzll((long long)m);
+                            m &= ~(1ull << s2);
+                            if (lane == 0) stack[sp] = ((first + s2) << 4) | (long long)(l + 1);
+                            ++sp;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    }
+                }
+            }
             continue;
         }
         const int64_t idx = (r0 + t) * 64 + lane;
@@ -359,12 +410,13 @@ TreeView make_tree_view(const vs_tree_layout* L, const double* node_boxes, const
     return T;
 }
 
-int check_layout(const vs_tree_layout* L) {
+int check_layout(const vs_tree_layout* L, int stack_max) {
     VS_REQUIRE(L != nullptr, "null tree layout");
     VS_REQUIRE(L->n_levels >= 1 && L->n_levels <= VS_MAX_LEVELS, "tree layout: bad level count %d",
                L->n_levels);
     VS_REQUIRE(L->fanout >= 2 && L->fanout <= 64, "tree layout: bad fanout %d", L->fanout);
-    VS_REQUIRE((int64_t)L->n_levels * L->fanout <= kStackMax, "tree layout: too deep for the stack");
+    // a depth-first walk keeps at most fanout - 1 siblings per level
+    VS_REQUIRE((int64_t)L->n_levels * L->fanout <= stack_max, "tree layout: too deep for the stack");
     VS_REQUIRE(L->level_count[L->n_levels - 1] == L->n_items, "tree layout: item level mismatch");
     return VS_OK;
 }
@@ -450,7 +502,7 @@ template <int MODE>
 int launch_traverse(int pred, const double* qx, const double* qy, const double4* qbox,
                     const int32_t* perm, int64_t n, const TreeView& T, const PolyView& P,
                     const JoinWs& J, int64_t capacity, hipStream_t s) {
-    const int grid = grid_for(ceil_div(n, 64));
+    const int grid = grid_for(MODE == kModeBox ? ceil_div(n, kBoxClaim) : ceil_div(n, 64));
     VS_TRY(timing_begin(s));
 #define VS_LAUNCH_T(PR)                                                                          \
     hipLaunchKernelGGL((traverse_kernel<MODE, PR>), dim3(grid), dim3(kJoinThreads), 0, s, qx, qy, \
@@ -575,7 +627,7 @@ int vs_sjoin_points_polygons(const double* px, const double* py, int64_t n_point
                              int64_t* out_right, int64_t* d_result, void* workspace, int64_t workspace_bytes,
                              void* stream) {
     vs::clear_error();
-    VS_TRY(vs::check_layout(layout));
+    VS_TRY(vs::check_layout(layout, vs::kStackMax));
     VS_REQUIRE(layout->n_items == n_polys, "vs_sjoin_points_polygons: layout has %lld items, %lld polygons",
                (long long)layout->n_items, (long long)n_polys);
     VS_REQUIRE(n_points >= 0 && n_points < (int64_t)INT32_MAX, "vs_sjoin_points_polygons: n_points out of range");
@@ -641,7 +693,7 @@ int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_l
                    int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,
                    int64_t workspace_bytes, void* stream) {
     vs::clear_error();
-    VS_TRY(vs::check_layout(layout));
+    VS_TRY(vs::check_layout(layout, vs::kStackMaxBox));
     VS_REQUIRE(n_queries >= 0 && n_queries < (int64_t)INT32_MAX, "vs_query_boxes: n_queries out of range");
     VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "vs_query_boxes: capacity out of range");
     VS_REQUIRE(d_result != nullptr, "vs_query_boxes: null result buffer");
