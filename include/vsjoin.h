@@ -186,9 +186,13 @@ int64_t vs_query_boxes_workspace_bytes(int64_t n_queries, int64_t capacity);
  * (inclusive; NaN boxes never match), int64.  query_boxes: 4 doubles per
  * query.  sort != 0: lexsorted (query, item); sort == 0: grouped by query in
  * ascending order, items within a query in unspecified order (the reference's
- * sort=False, spatial/query.py:143-153).  Replaces spatial_index_device_query's
- * candidate output (spatial/spatial_index_device.py:228-390) and the
- * bbox_overlap kernels (kernels/core/spatial_query_source.py:415-555). */
+ * sort=False, spatial/query.py:143-153).  capacity also sets the traversal's
+ * shape (the pairs are the same either way): at least 32 slots per query box,
+ * one box per wave with the lanes over a node's children (boxes with many
+ * hits); fewer, 64 boxes per wave over the union of their boxes.  Replaces
+ * spatial_index_device_query's candidate output
+ * (spatial/spatial_index_device.py:228-390) and the bbox_overlap kernels
+ * (kernels/core/spatial_query_source.py:415-555). */
 int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_layout* layout,
                    const double* node_boxes, const int32_t* item_ids, int64_t capacity, int32_t sort,
                    int64_t* out_left, int64_t* out_right, int64_t* d_result, void* workspace,

@@ -267,6 +267,32 @@ def test_box_query_output_sized_from_previous_queries():
     assert expect_retries[2] == 1 and expect_retries[3] == 0  # grown past the hint, then slack
 
 
+def test_box_query_both_traversal_shapes_agree():
+    """Box queries run 64 boxes per wave when the caller expects few hits per
+    box (capacity below 32 per box) and one box per wave otherwise; the same
+    queries through both shapes, on 16- and 64-wide point trees, give the
+    oracle's pairs (NaN and empty-hit boxes included)."""
+    print("box query traversal shapes", flush=True)
+    rng = np.random.default_rng(33)
+    p = rng.uniform(0, 1000, (20_000, 2))
+    tb = np.column_stack([p, p])
+    q = rng.uniform(0, 1000, (2_000, 2))
+    half = rng.uniform(1, 15, (2_000, 1))
+    qb = np.column_stack([q - half, q + half])
+    qb[7] = np.nan
+    qb[11] = [2000.0, 2000.0, 2001.0, 2001.0]
+    el, er = oracle.bbox_pairs(qb, tb)
+    o = np.lexsort((er, el))
+    n_hits = len(el)
+    assert n_hits < 16 * len(qb)  # so capacity 16 per box is enough for the 64-box shape
+    for fanout in (16, 64):
+        tree = J.build_tree(torch.as_tensor(tb, device="cuda"), fanout=fanout)
+        for per_box in (16, 48):  # below / above the one-box threshold
+            l, r, st = J.query_boxes(torch.as_tensor(qb, device="cuda"), tree, capacity=per_box * len(qb))
+            assert st.retries == 0 and st.pairs == n_hits, (fanout, per_box)
+            assert np.array_equal(l.cpu().numpy(), el[o]) and np.array_equal(r.cpu().numpy(), er[o]), (fanout, per_box)
+
+
 def test_box_query_hint_does_not_oversize_a_large_query():
     """A query with about 5,000 hits per box followed by 400k point boxes:
     the second query's output is sized from at most HINT_SLOTS_PER_QUERY

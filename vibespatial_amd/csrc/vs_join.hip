@@ -78,7 +78,8 @@ struct PolyView {
 };
 
 constexpr int kOutBuf = 256;  // per-wave LDS buffer of refined pairs
-constexpr int kBoxClaim = 4;  // box mode: query boxes per queue atomic
+constexpr int kBoxClaim = 4;    // one-box mode: query boxes per queue atomic
+constexpr int kOneBoxHits = 32;  // expected hits per box from which box queries run one box per wave
 
 
 enum { kModePip = 0, kModeBox = 1 };
@@ -169,14 +170,16 @@ __device__ bool refine_pip_wave(double px, double py, bool active, int32_t c0, i
 
 // Persistent traversal kernel.  MODE kModePip: queries are points (qa, qb =
 // sorted x, y), 64 per wave, with fused refine.  MODE kModeBox: queries are
-// boxes given as (qbox, perm), one per wave (claimed kBoxClaim at a time):
-// the lanes test a node's children, and a leaf group's hits are appended with
-// one ballot; every bbox pair is emitted.
+// boxes given as (qbox, perm) and every bbox pair is emitted; with one_box,
+// one box per wave (claimed kBoxClaim at a time): the lanes test a node's
+// children and a leaf group's hits are appended with one ballot (boxes with
+// many hits); else 64 boxes per wave walk the union of their boxes (many
+// boxes with few hits).
 template <int MODE, int PRED>
 __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
     const double* __restrict__ qx, const double* __restrict__ qy, const double4* __restrict__ qbox,
     const int32_t* __restrict__ perm, int64_t n, TreeView T, PolyView P, uint32_t* __restrict__ stage_l,
-    int32_t* __restrict__ stage_r, int64_t capacity, JoinCtl* __restrict__ ctl) {
+    int32_t* __restrict__ stage_r, int64_t capacity, JoinCtl* __restrict__ ctl, int one_box) {
     __shared__ double4 top[kTopMax];
     __shared__ long long stack_s[kJoinWaves][MODE == kModeBox ? kStackMaxBox : kStackMax];
     __shared__ int2 outbuf_s[kJoinWaves][kOutBuf];
@@ -206,8 +209,9 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         nbuf = 0;
     };
-    // work units: 64 points (kModePip) or kBoxClaim boxes (kModeBox)
-    const int64_t W = MODE == kModeBox ? (n + kBoxClaim - 1) / kBoxClaim : (n + 63) >> 6;
+    // work units: kBoxClaim boxes (one-box mode) or 64 queries
+    const bool one = MODE == kModeBox && one_box;
+    const int64_t W = one ? (n + kBoxClaim - 1) / kBoxClaim : (n + 63) >> 6;
     const int home = blockIdx.x & 7;
     const int last = T.n_levels - 1;
     const int F = T.fanout;
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(kJoinThreads) void traverse_kernel(
             ++xq;
             continue;
         }
-        if (MODE == kModeBox) {
+        if (one) {
             for (int bq = 0; bq < kBoxClaim; ++bq) {
                 const int64_t qi = (r0 + t) * kBoxClaim + bq;  // wave-uniform
                 if (qi >= n) break;
@@ -501,12 +505,12 @@ int timing_end_public(hipStream_t s) { return timing_end(s); }
 template <int MODE>
 int launch_traverse(int pred, const double* qx, const double* qy, const double4* qbox,
                     const int32_t* perm, int64_t n, const TreeView& T, const PolyView& P,
-                    const JoinWs& J, int64_t capacity, hipStream_t s) {
-    const int grid = grid_for(MODE == kModeBox ? ceil_div(n, kBoxClaim) : ceil_div(n, 64));
+                    const JoinWs& J, int64_t capacity, hipStream_t s, bool one_box = false) {
+    const int grid = grid_for(one_box ? ceil_div(n, kBoxClaim) : ceil_div(n, 64));
     VS_TRY(timing_begin(s));
 #define VS_LAUNCH_T(PR)                                                                          \
     hipLaunchKernelGGL((traverse_kernel<MODE, PR>), dim3(grid), dim3(kJoinThreads), 0, s, qx, qy, \
-                       qbox, perm, n, T, P, J.stage_l, J.stage_r, capacity, J.ctl)
+                       qbox, perm, n, T, P, J.stage_l, J.stage_r, capacity, J.ctl, (int)one_box)
     switch (pred) {
         case VS_PRED_WITHIN: VS_LAUNCH_T(VS_PRED_WITHIN); break;
         case VS_PRED_INTERSECTS: VS_LAUNCH_T(VS_PRED_INTERSECTS); break;
@@ -709,8 +713,11 @@ int vs_query_boxes(const double* query_boxes, int64_t n_queries, const vs_tree_l
     if (n_queries > 0 && layout->n_items > 0) {
         const double4* qb = reinterpret_cast<const double4*>(query_boxes);
         VS_TRY(vs::box_order(qb, n_queries, J.perm, J.order_ws, s));  // waves of nearby boxes
+        // the traversal's shape follows the hits the caller expects per box
+        // (capacity / n_queries); both shapes give the same pairs
+        const bool one_box = capacity >= (int64_t)vs::kOneBoxHits * n_queries;
         VS_TRY(vs::launch_traverse<vs::kModeBox>(VS_PRED_INTERSECTS, nullptr, nullptr, qb, J.perm, n_queries, T, P,
-                                                 J, capacity, s));
+                                                 J, capacity, s, one_box));
     }
     return vs::finish_pairs(J.fin_ws, J.ctl, n_queries, capacity, sort != 0, out_left, out_right, d_result, s);
 }
