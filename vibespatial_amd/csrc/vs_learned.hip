@@ -1159,6 +1159,13 @@ __device__ int stage_edges(int32_t lo, int32_t hi, double xmn, double ymn, doubl
     return ne;
 }
 
+// Wave mode carves each wave's quarter out of the block's buffers: edges and
+// their coordinate indices (kEdgeCap / kClaim), slab lists (kBinCap / kClaim
+// entries) and slab starts and cursors (kMaxBins / 2 + 1 and kMaxBins / 2
+// ints of the candidate buffer).
+static_assert(kEdgeCap / kClaim >= kWaveEdges, "wave mode: a wave's edge buffer holds kWaveEdges edges");
+static_assert(kChunk / kClaim >= kMaxBins + 1, "wave mode: a wave's slab starts and cursors fit its quarter of cand");
+
 // Persistent polygon-major join.  MODE kLModePip: query q is polygon row q
 // (qbox = its bounds), output pairs (point, q).  MODE kLModeBox: query q is a
 // box, output pairs (q, point).  Items (up to kItemPos scanned positions of one
