@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void hilbert_keys_points_kernel(const double* 
     }
     if (zero_bits > 0) key = (key >> zero_bits) << zero_bits;
     keys[i] = (K)key;
-    ids[i] = (int32_t)i;
+    if (ids) ids[i] = (int32_t)i;  // null: the sort makes the indices itself
 }
 
 // Items level: boxes gathered into Hilbert order.
@@ -472,7 +472,6 @@ int64_t point_order_workspace_bytes(int64_t n) {
     Carver c(nullptr);
     c.take<uint32_t>(n);
     c.take<uint32_t>(n);
-    c.take<int32_t>(n);
     c.take<double4>(1);
     c.take<char>(extent_workspace_bytes());
     c.take<char>(sort_workspace_bytes_u32(n));
@@ -525,8 +524,7 @@ int64_t point_index_workspace_bytes_t(int64_t n) {
     Carver c(nullptr);
     c.take<char>(extent_workspace_bytes());
     c.take<char>(sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n));
-    c.take<K>(n);  // unsorted keys and ids
-    c.take<int32_t>(n);
+    c.take<K>(n);        // unsorted keys (the sort makes the ids)
     c.take<double2>(n);  // interleaved points for the gather
     return c.bytes;
 }
@@ -538,7 +536,6 @@ const double2* point_index_xy_t(void* ws, int64_t n) {
     c.take<char>(extent_workspace_bytes());
     c.take<char>(sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n));
     c.take<K>(n);
-    c.take<int32_t>(n);
     return c.take<double2>(n);
 }
 
@@ -560,20 +557,20 @@ int point_index_build_t(const double* px, const double* py, int64_t n, K* keys, 
     const int64_t sb = sizeof(K) == 4 ? sort_workspace_bytes_u32(n) : sort_workspace_bytes_u64(n);
     void* sort_ws = c.take<char>(sb);
     K* keys_in = c.take<K>(n);
-    int32_t* ids_in = c.take<int32_t>(n);
     double2* xy = c.take<double2>(n);
     VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
     if (n == 0) return VS_OK;
     const int bits = 4 * (int)sizeof(K);
     const int zero_bits = 2 * (bits - point_index_grid_bits<K>());
     hipLaunchKernelGGL(hilbert_keys_points_kernel<K>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, px, py, n,
-                       ext, bits, zero_bits, keys_in, ids_in, xy);
+                       ext, bits, zero_bits, keys_in, nullptr, xy);
     VS_LAUNCH_CHECK();
+    // no id array: the first radix pass takes each key's index as its value
     if (sizeof(K) == 4)
-        VS_TRY(sort_pairs_into_u32(reinterpret_cast<const uint32_t*>(keys_in), ids_in,
+        VS_TRY(sort_pairs_into_u32(reinterpret_cast<const uint32_t*>(keys_in), nullptr,
                                    reinterpret_cast<uint32_t*>(keys), perm, n, zero_bits, 32, sort_ws, sb, s));
     else
-        VS_TRY(sort_pairs_into_u64(reinterpret_cast<const uint64_t*>(keys_in), ids_in,
+        VS_TRY(sort_pairs_into_u64(reinterpret_cast<const uint64_t*>(keys_in), nullptr,
                                    reinterpret_cast<uint64_t*>(keys), perm, n, zero_bits, 64, sort_ws, sb, s));
     if (sx) VS_TRY(gather_points_xy(xy, perm, n, sx, sy, s, 8));
     return VS_OK;
@@ -598,16 +595,15 @@ int point_order(const double* px, const double* py, int64_t n, int bits, int32_t
     Carver c(ws);
     uint32_t* keys = c.take<uint32_t>(n);
     uint32_t* keys_in = c.take<uint32_t>(n);
-    int32_t* ids_in = c.take<int32_t>(n);
     double4* ext = c.take<double4>(1);
     void* ext_ws = c.take<char>(extent_workspace_bytes());
     void* sort_ws = c.take<char>(sort_workspace_bytes_u32(n));
     double2* xy = c.take<double2>(n);
     VS_TRY(point_extent(px, py, n, ext, ext_ws, s));
     hipLaunchKernelGGL(hilbert_keys_points_kernel<uint32_t>, dim3((unsigned)ceil_div(n, 256)), dim3(256),
-                       0, s, px, py, n, ext, 16, 0, keys_in, ids_in, xy);
+                       0, s, px, py, n, ext, 16, 0, keys_in, nullptr, xy);
     VS_LAUNCH_CHECK();
-    VS_TRY(sort_pairs_into_u32(keys_in, ids_in, keys, perm, n, 32 - 2 * bits, 32, sort_ws,
+    VS_TRY(sort_pairs_into_u32(keys_in, nullptr, keys, perm, n, 32 - 2 * bits, 32, sort_ws,
                                sort_workspace_bytes_u32(n), s));
     hipLaunchKernelGGL(gather_xy_kernel, dim3((unsigned)ceil_div(n, kGatherTile)), dim3(256), 0, s, xy, perm, n,
                        sx, sy);

@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
         const int64_t idx = base + (int64_t)i * 64 + lane;
         const bool valid = idx < n;
         k[i] = valid ? keys_in[idx] : (K)0;
-        v[i] = valid ? vals_in[idx] : 0;
+        v[i] = valid ? (vals_in ? vals_in[idx] : (int32_t)idx) : 0;
         const int d = (int)((k[i] >> shift) & dmask);
         uint64_t m = __ballot(valid);
 #pragma unroll
@@ -362,12 +362,18 @@ int sort_pairs_nocopy(K* keys, int32_t* vals, int64_t n, const unsigned long lon
     return VS_OK;
 }
 
+__global__ void iota_i32_kernel(int32_t* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = (int32_t)i;
+}
+
 // Sorts (in_keys, in_vals) into (out_keys, out_vals), stable, by bits
 // [begin_bit, end_bit), with no copy: with an odd pass count the first pass
 // writes the outputs, with an even one the workspace alternates, so the last
 // pass always lands in the outputs.  The inputs are only read; they must not
-// alias the outputs.  n: element capacity; d_n (device, optional): see
-// device_count (an overflowed count sorts nothing).
+// alias the outputs; in_vals == nullptr: each key's value is its index.  n:
+// element capacity; d_n (device, optional): see device_count (an overflowed
+// count sorts nothing).
 template <class K>
 int sort_pairs_into(const K* in_keys, const int32_t* in_vals, K* out_keys, int32_t* out_vals, int64_t n,
                     const unsigned long long* d_n, int begin_bit, int end_bit, void* ws, int64_t ws_bytes,
@@ -381,7 +387,12 @@ int sort_pairs_into(const K* in_keys, const int32_t* in_vals, K* out_keys, int32
     if (n <= 1 || passes == 0) {  // nothing to order: copy all n slots (a device count only limits the valid ones)
         if (n > 0) {
             VS_CHECK_HIP(hipMemcpyAsync(out_keys, in_keys, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
-            VS_CHECK_HIP(hipMemcpyAsync(out_vals, in_vals, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+            if (in_vals) {
+                VS_CHECK_HIP(hipMemcpyAsync(out_vals, in_vals, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+            } else {
+                hipLaunchKernelGGL(iota_i32_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, out_vals, n);
+                VS_LAUNCH_CHECK();
+            }
         }
         return VS_OK;
     }
