@@ -1873,9 +1873,18 @@ static int side_stream(SideStream** out) {
 // the side stream while the caller's stream gathers the sorted coordinates
 // (fork), or after them on the caller's stream (no fork: nothing is created).
 // The call joins both before returning.
+// The side stream of an index build between its fork and its join.
+struct IndexFork {
+    SideStream* side = nullptr;
+    hipStream_t t = nullptr;
+};
+
+// Everything up to the join: keys and sort on `s`, then (fork) training and
+// the error pass on the side stream beside the coordinate gather on `s`.
+// The caller may queue more work on `s` before index_build_join.
 template <class K>
-int index_build_t(const double* px, const double* py, int64_t n, int S, int iters, float lr, const IndexBufs<K>& B,
-                  hipStream_t s, bool fork = true) {
+int index_build_fork(const double* px, const double* py, int64_t n, int S, int iters, float lr,
+                     const IndexBufs<K>& B, hipStream_t s, bool fork, IndexFork* out) {
     if (sizeof(K) == 4)
         VS_TRY(point_index_build_u32(px, py, n, reinterpret_cast<uint32_t*>(B.keys), B.perm, nullptr, nullptr, B.ext,
                                      B.build_ws, s));
@@ -1907,11 +1916,26 @@ int index_build_t(const double* px, const double* py, int64_t n, int S, int iter
     // leave room for the side stream's kernels
     const double2* xy = sizeof(K) == 4 ? point_index_xy_u32(B.build_ws, n) : point_index_xy_u64(B.build_ws, n);
     VS_TRY(gather_points_xy(xy, B.perm, n, B.sx, B.sy, s, fork ? 2 : 8));
-    if (fork) {
-        VS_CHECK_HIP(hipEventRecord(side->join, t));
-        VS_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
+    out->side = side;
+    out->t = t;
+    return VS_OK;
+}
+
+// `s` waits for the side stream's work.
+inline int index_build_join(const IndexFork& f, hipStream_t s) {
+    if (f.side) {
+        VS_CHECK_HIP(hipEventRecord(f.side->join, f.t));
+        VS_CHECK_HIP(hipStreamWaitEvent(s, f.side->join, 0));
     }
     return VS_OK;
+}
+
+template <class K>
+int index_build_t(const double* px, const double* py, int64_t n, int S, int iters, float lr, const IndexBufs<K>& B,
+                  hipStream_t s, bool fork = true) {
+    IndexFork f;
+    VS_TRY(index_build_fork<K>(px, py, n, S, iters, lr, B, s, fork, &f));
+    return index_build_join(f, s);
 }
 
 template <class K>
@@ -1984,18 +2008,31 @@ QueryBufs carve_query(Carver& c, int64_t nq, int64_t n_left, int64_t capacity) {
     return Q;
 }
 
+// Cover cells of every query box (and, for polygon queries, their coordinate
+// ranges and part counts); needs only the point extent of the index.
+template <class K>
+int cover_cells_launch(int mode, const double4* qbox, int64_t nq, const IndexBufs<K>& B, const RefineView& R,
+                       const QueryBufs& Q, hipStream_t s) {
+    if (nq <= 0) return VS_OK;
+    int4* pinfo = mode != kLModeBox ? Q.poly_info : nullptr;
+    hipLaunchKernelGGL(cover_cells_kernel<K>, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, s, qbox, nq, B.ext,
+                       Q.keyr, Q.nranges, R.geom, R.part, R.ring, pinfo);
+    VS_LAUNCH_CHECK();
+    return VS_OK;
+}
+
+// cells_ready: cover_cells_launch already ran on `s` for these queries.
 template <class K>
 int learned_query_t(int mode, int pred, const double4* qbox, int64_t nq, const LearnedView<K>& V,
                     const IndexBufs<K>& B, const RefineView& R, int64_t n_left, int64_t capacity, bool sort,
-                    const QueryBufs& Q, int64_t* out_l, int64_t* out_r, int64_t* d_result, hipStream_t s) {
+                    const QueryBufs& Q, int64_t* out_l, int64_t* out_r, int64_t* d_result, hipStream_t s,
+                    bool cells_ready = false) {
     ctl_reset(Q.ctl, s, Q.cstat, 4);
     VS_LAUNCH_CHECK();
     if (nq > 0) {
         const unsigned gq = (unsigned)ceil_div(nq, 256);
         int4* pinfo = mode != kLModeBox ? Q.poly_info : nullptr;
-        hipLaunchKernelGGL(cover_cells_kernel<K>, dim3(gq), dim3(256), 0, s, qbox, nq, B.ext, Q.keyr, Q.nranges,
-                           R.geom, R.part, R.ring, pinfo);
-        VS_LAUNCH_CHECK();
+        if (!cells_ready) VS_TRY(cover_cells_launch<K>(mode, qbox, nq, B, R, Q, s));
         hipLaunchKernelGGL(cover_lookup_kernel<K>, dim3(stride_grid(nq * kMaxRanges * 2, 256, 8)), dim3(256), 0, s,
                            nq, Q.keyr, Q.nranges, V, reinterpret_cast<int32_t*>(Q.ranges), Q.cstat);
         VS_LAUNCH_CHECK();
@@ -2065,12 +2102,17 @@ int learned_join_impl(const double* px, const double* py, int64_t n_points, cons
     const auto B = carve_index<uint32_t>(c, n_points, S);
     uint8_t* flags = c.take<uint8_t>(n_coords);
     const QueryBufs Q = carve_query(c, n_polys, n_points, capacity);
-    VS_TRY(index_build_t<uint32_t>(px, py, n_points, S, train_iters, kJoinTrainLr, B, s, fork));
+    // the polygon side (refine flags, cover cells) runs on `s` after the
+    // gather, beside the side stream's training
+    IndexFork f;
+    VS_TRY(index_build_fork<uint32_t>(px, py, n_points, S, train_iters, kJoinTrainLr, B, s, fork, &f));
     VS_TRY(build_refine_flags(part_offsets, ring_offsets, n_parts, n_rings, n_coords, flags, s));
     const RefineView R = {x, y, geom_offsets, part_offsets, ring_offsets, flags};
-    return learned_query_t<uint32_t>(kLModePip, predicate, reinterpret_cast<const double4*>(poly_bounds), n_polys,
-                                     view_of<uint32_t>(B, n_points, S, lookup_mode), B, R, n_points, capacity, true,
-                                     Q, out_left, out_right, d_result, s);
+    const double4* qb = reinterpret_cast<const double4*>(poly_bounds);
+    VS_TRY(cover_cells_launch<uint32_t>(kLModePip, qb, n_polys, B, R, Q, s));
+    VS_TRY(index_build_join(f, s));
+    return learned_query_t<uint32_t>(kLModePip, predicate, qb, n_polys, view_of<uint32_t>(B, n_points, S, lookup_mode),
+                                     B, R, n_points, capacity, true, Q, out_left, out_right, d_result, s, true);
 }
 
 }  // namespace vs
