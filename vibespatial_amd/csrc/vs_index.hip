@@ -175,13 +175,25 @@ __global__ __launch_bounds__(256) void point_extent_partial_kernel(const double*
                                                                   double4* __restrict__ part) {
     __shared__ double4 red[4];
     double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const double a = px[i], c = py[i];
+    auto fold = [&](double a, double c) {
         if (a == a && c == c) {
             x0 = fmin(x0, a); y0 = fmin(y0, c);
             x1 = fmax(x1, a); y1 = fmax(y1, c);
         }
+    };
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {  // four points' loads in flight per thread
+        double a[4], c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            a[k] = px[i + k * stride];
+            c[k] = py[i + k * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fold(a[k], c[k]);
     }
+    for (; i < n; i += stride) fold(px[i], py[i]);
     x0 = wave_min(x0); y0 = wave_min(y0); x1 = wave_max(x1); y1 = wave_max(y1);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double4(x0, y0, x1, y1);
     __syncthreads();
@@ -216,7 +228,7 @@ __global__ __launch_bounds__(256) void extent_final_kernel(const double4* __rest
     }
 }
 
-constexpr int kExtentBlocks = 512;
+constexpr int kExtentBlocks = 2048;  // 8 blocks per CU
 
 int64_t extent_workspace_bytes() { return sizeof(double4) * (kExtentBlocks + 1) + 256; }
 
