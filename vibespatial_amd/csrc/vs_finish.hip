@@ -7,11 +7,13 @@
 //
 //   1. LSD radix sort of the staged pairs by left id (u32 keys over
 //      ceil(log2 n_left) bits, stable).  The staged count is read on the device
-//      (no host sync); an overflowed stage sorts nothing.
-//   2. emit: one thread per sorted slot writes out_l; with sort_right (the
-//      default, lexsorted output) the first slot of each
-//      left id's run sorts that run's right ids (insertion sort, runs of at
-//      most kShortRun) and writes out_r.  Longer runs are listed.
+//      (no host sync); an overflowed stage sorts nothing.  Its last pass
+//      writes out_l / out_r as int64 directly.
+//   2. runs: with sort_right (the default, lexsorted output) the first slot
+//      of each left id's run of two or more sorts that run's right ids in
+//      out_r (insertion sort, runs of at most kShortRun).  Longer runs are
+//      listed.  (With no sort pass -- at most one left row -- an emit pass
+//      converts and sorts instead.)
 //   3. long runs: one block per listed run; runs up to kRunTile are bitonic
 //      sorted in LDS, longer ones are sorted in kRunTile chunks and merged in
 //      global memory (merge by co-rank: each element's output slot is its
@@ -97,6 +99,41 @@ __global__ __launch_bounds__(256) void emit_pairs_kernel(const uint32_t* __restr
     }
 }
 
+// After the sort's last pass wrote out_l / out_r: sorts each short run of
+// equal left ids by right id in out_r and lists the long runs.
+__global__ __launch_bounds__(256) void fix_runs_kernel(const int64_t* __restrict__ out_l,
+                                                      int64_t* __restrict__ out_r,
+                                                      const JoinCtl* __restrict__ ctl, int64_t capacity,
+                                                      int32_t* __restrict__ long_runs,
+                                                      unsigned int* __restrict__ n_long) {
+    const unsigned long long staged = ctl->staged;
+    if (staged > (unsigned long long)capacity) return;
+    const int64_t n = (int64_t)staged;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t k = out_l[i];
+        const int64_t kp = i > 0 ? out_l[i - 1] : ~k;
+        const int64_t kn = i + 1 < n ? out_l[i + 1] : ~k;
+        if (kp == k || kn != k) continue;  // not the first slot of a run of two or more
+        int64_t e = i + 2;
+        while (e < n && e - i <= kShortRun && out_l[e] == k) ++e;
+        const int len = (int)(e - i);
+        if (len > kShortRun) {
+            long_runs[atomicAdd(n_long, 1u)] = (int32_t)i;
+            continue;
+        }
+        int64_t* r = out_r + i;
+        for (int a = 1; a < len; ++a) {
+            const int64_t v = r[a];
+            int b = a - 1;
+            while (b >= 0 && r[b] > v) {
+                r[b + 1] = r[b];
+                --b;
+            }
+            r[b + 1] = v;
+        }
+    }
+}
+
 __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo, int64_t hi, int32_t x) {
     while (lo < hi) {
         const int64_t m = (lo + hi) >> 1;
@@ -134,13 +171,19 @@ __device__ void block_bitonic_sort(int32_t* t, int P) {
     }
 }
 
+// WIDE: the sorted pairs are in out_l / out_r (int64; wkeys = out_l) and vals
+// is scratch; else in keys / vals (u32 / int32).
+template <bool WIDE>
 __global__ __launch_bounds__(kRunThreads) void long_runs_kernel(const uint32_t* __restrict__ keys,
+                                                               const int64_t* __restrict__ wkeys,
                                                                int32_t* __restrict__ vals,
                                                                int32_t* __restrict__ tmp,
                                                                const JoinCtl* __restrict__ ctl, int64_t capacity,
                                                                const int32_t* __restrict__ long_runs,
                                                                const unsigned int* __restrict__ n_long,
                                                                int64_t* __restrict__ out_r) {
+    auto key_at = [&](int64_t i) -> int64_t { return WIDE ? wkeys[i] : (int64_t)keys[i]; };
+    auto val_at = [&](int64_t i) -> int32_t { return WIDE ? (int32_t)out_r[i] : vals[i]; };
     __shared__ int32_t t[kRunTile];
     __shared__ long long s_end;
     const unsigned long long staged = ctl->staged;
@@ -151,9 +194,9 @@ __global__ __launch_bounds__(kRunThreads) void long_runs_kernel(const uint32_t* 
         const int64_t start = long_runs[j];
         if (threadIdx.x == 0) {
             // end of the run: gallop from a slot known to hold the key, then bisect
-            const uint32_t k = keys[start];
+            const int64_t k = key_at(start);
             int64_t lo = start + kShortRun, step = 64, hi = lo + step;
-            while (hi < n && keys[hi] == k) {
+            while (hi < n && key_at(hi) == k) {
                 lo = hi;
                 step *= 2;
                 hi = lo + step;
@@ -161,7 +204,7 @@ __global__ __launch_bounds__(kRunThreads) void long_runs_kernel(const uint32_t* 
             if (hi > n) hi = n;
             while (hi - lo > 1) {  // keys[lo] == k; hi == n or keys[hi] != k
                 const int64_t m = (lo + hi) >> 1;
-                if (keys[m] == k) lo = m;
+                if (key_at(m) == k) lo = m;
                 else hi = m;
             }
             s_end = hi;
@@ -172,7 +215,7 @@ __global__ __launch_bounds__(kRunThreads) void long_runs_kernel(const uint32_t* 
         if (L <= kRunTile) {
             int P = 64;
             while (P < L) P <<= 1;
-            for (int i = threadIdx.x; i < P; i += kRunThreads) t[i] = i < L ? vals[start + i] : INT_MAX;
+            for (int i = threadIdx.x; i < P; i += kRunThreads) t[i] = i < L ? val_at(start + i) : INT_MAX;
             __syncthreads();
             block_bitonic_sort(t, P);
             for (int i = threadIdx.x; i < L; i += kRunThreads) out_r[start + i] = t[i];
@@ -182,7 +225,8 @@ __global__ __launch_bounds__(kRunThreads) void long_runs_kernel(const uint32_t* 
         // sorted chunks of kRunTile, in place
         for (int64_t c0 = 0; c0 < L; c0 += kRunTile) {
             const int64_t cl = (L - c0) < kRunTile ? (L - c0) : kRunTile;
-            for (int i = threadIdx.x; i < kRunTile; i += kRunThreads) t[i] = i < cl ? vals[start + c0 + i] : INT_MAX;
+            for (int i = threadIdx.x; i < kRunTile; i += kRunThreads)
+                t[i] = i < cl ? val_at(start + c0 + i) : INT_MAX;
             __syncthreads();
             block_bitonic_sort(t, kRunTile);
             for (int i = threadIdx.x; i < cl; i += kRunThreads) vals[start + c0 + i] = t[i];
@@ -232,8 +276,25 @@ int finish_pairs(void* fin_ws, JoinCtl* ctl, int64_t n_left, int64_t capacity, b
                (long long)capacity);
     Carver c(fin_ws);
     const FinishBufs F = carve_finish(c, capacity);
+    const int nbits = bits_for(n_left);
     SortResult<uint32_t> r;
-    VS_TRY(sort_pairs_u32_dev(F.stage_l, F.stage_r, capacity, &ctl->staged, 0, bits_for(n_left), F.sort_ws,
+    if (capacity > 1 && nbits > 0) {  // the sort's last pass writes the output pairs
+        VS_TRY(sort_pairs_u32_dev_wide(F.stage_l, F.stage_r, capacity, &ctl->staged, 0, nbits, F.sort_ws,
+                                       sort_workspace_bytes_u32(capacity), s, out_l, out_r, &r));
+        if (sort_right) {
+            const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, 256), 65536));
+            hipLaunchKernelGGL(fix_runs_kernel, dim3(grid), dim3(256), 0, s, out_l, out_r, ctl, capacity,
+                               F.long_runs, &ctl->n_long);
+            VS_LAUNCH_CHECK();
+            hipLaunchKernelGGL(long_runs_kernel<true>, dim3(512), dim3(kRunThreads), 0, s, nullptr, out_l, r.vals,
+                               r.alt_vals, ctl, capacity, F.long_runs, &ctl->n_long, out_r);
+            VS_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(publish_result_kernel, dim3(1), dim3(64), 0, s, ctl, d_result);
+        VS_LAUNCH_CHECK();
+        return VS_OK;
+    }
+    VS_TRY(sort_pairs_u32_dev(F.stage_l, F.stage_r, capacity, &ctl->staged, 0, nbits, F.sort_ws,
                               sort_workspace_bytes_u32(capacity), s, &r));
     if (capacity > 0) {
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, 256), 65536));
@@ -241,8 +302,8 @@ int finish_pairs(void* fin_ws, JoinCtl* ctl, int64_t n_left, int64_t capacity, b
                            (int)sort_right, F.long_runs, &ctl->n_long, out_l, out_r);
         VS_LAUNCH_CHECK();
         if (sort_right) {
-            hipLaunchKernelGGL(long_runs_kernel, dim3(512), dim3(kRunThreads), 0, s, r.keys, r.vals, r.alt_vals, ctl,
-                               capacity, F.long_runs, &ctl->n_long, out_r);
+            hipLaunchKernelGGL(long_runs_kernel<false>, dim3(512), dim3(kRunThreads), 0, s, r.keys, nullptr, r.vals,
+                               r.alt_vals, ctl, capacity, F.long_runs, &ctl->n_long, out_r);
             VS_LAUNCH_CHECK();
         }
     }

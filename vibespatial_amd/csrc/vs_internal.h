@@ -122,6 +122,12 @@ int sort_pairs_into_u64(const uint64_t* in_keys, const int32_t* in_vals, uint64_
                         const unsigned long long* d_n = nullptr);
 int sort_pairs_u32_dev(uint32_t* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit,
                        int end_bit, void* ws, int64_t ws_bytes, hipStream_t s, SortResult<uint32_t>* res);
+// The same with the last pass writing int64 keys and values to out_k / out_v
+// (res->keys null; res->vals, res->alt_vals and res->alt_keys free buffers of
+// n elements).  Needs begin_bit < end_bit and n > 1 for the wide pass to run.
+int sort_pairs_u32_dev_wide(uint32_t* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit,
+                            int end_bit, void* ws, int64_t ws_bytes, hipStream_t s, int64_t* out_k, int64_t* out_v,
+                            SortResult<uint32_t>* res);
 
 // ---------------------------------------------------------------------------
 // Device helpers for 64-wide wavefronts.

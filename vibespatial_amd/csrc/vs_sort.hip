@@ -223,10 +223,12 @@ __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const K* __restrict
     hist[(int64_t)d * nblocks + blockIdx.x] = t;
 }
 
-template <class K>
+// WIDE: the pass writes its keys and values as int64 (a sort's last pass
+// writing a join's output pairs directly).
+template <class K, bool WIDE>
 __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
-    const K* __restrict__ keys_in, const int32_t* __restrict__ vals_in, K* __restrict__ keys_out,
-    int32_t* __restrict__ vals_out, int64_t n, const unsigned long long* __restrict__ d_n, int shift,
+    const K* __restrict__ keys_in, const int32_t* __restrict__ vals_in, void* __restrict__ keys_out,
+    void* __restrict__ vals_out, int64_t n, const unsigned long long* __restrict__ d_n, int shift,
     int dmask, const int32_t* __restrict__ hist_scan, int nblocks) {
     n = device_count(n, d_n);
     if ((int64_t)blockIdx.x * kRsTile >= n) return;  // whole tile past the valid count (block-uniform)
@@ -303,8 +305,13 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
         const K kk = skeys[p];
         const int d = (int)((kk >> shift) & dmask);
         const int64_t g = digit_global[d] + (p - digit_block_off[d]);
-        keys_out[g] = kk;
-        vals_out[g] = svals[p];
+        if (WIDE) {
+            static_cast<int64_t*>(keys_out)[g] = (int64_t)kk;
+            static_cast<int64_t*>(vals_out)[g] = (int64_t)svals[p];
+        } else {
+            static_cast<K*>(keys_out)[g] = kk;
+            static_cast<int32_t*>(vals_out)[g] = svals[p];
+        }
     }
 }
 
@@ -323,9 +330,14 @@ int64_t sort_workspace_bytes_t(int64_t n) {
 // returns the buffers that hold the result (the inputs or the workspace's
 // alternates, by the parity of the pass count) and the other pair.
 // n: element capacity; d_n (device, optional): see device_count.
+// wide_l / wide_r (both or neither): the last pass writes the sorted keys and
+// values there as int64 instead of into a K / int32 buffer; then res->keys is
+// null and res->vals, res->alt_vals (int32) and res->alt_keys (K), n elements
+// each, are free for the caller.
 template <class K>
 int sort_pairs_nocopy(K* keys, int32_t* vals, int64_t n, const unsigned long long* d_n, int begin_bit, int end_bit,
-                      void* ws, int64_t ws_bytes, hipStream_t s, SortResult<K>* res) {
+                      void* ws, int64_t ws_bytes, hipStream_t s, SortResult<K>* res, int64_t* wide_l = nullptr,
+                      int64_t* wide_r = nullptr) {
     VS_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX, "sort_pairs: n=%lld out of range", (long long)n);
     VS_REQUIRE(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit,
                "sort_pairs: bad bit range [%d, %d)", begin_bit, end_bit);
@@ -340,6 +352,7 @@ int sort_pairs_nocopy(K* keys, int32_t* vals, int64_t n, const unsigned long lon
     int32_t* vi = vals;
     K* ko = kalt;
     int32_t* vo = valt;
+    const bool wide = wide_l != nullptr && n > 1 && begin_bit < end_bit;
     if (n > 1) {
         for (int shift = begin_bit; shift < end_bit; shift += 8) {
             const int width = end_bit - shift < 8 ? end_bit - shift : 8;
@@ -348,8 +361,18 @@ int sort_pairs_nocopy(K* keys, int32_t* vals, int64_t n, const unsigned long lon
                                nb);
             VS_LAUNCH_CHECK();
             VS_TRY(exclusive_scan_i32(hist, hist, (int64_t)nb * kRsBins, nullptr, scan_ws, s));
-            hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n, d_n,
-                               shift, dmask, hist, nb);
+            if (wide && shift + 8 >= end_bit) {
+                hipLaunchKernelGGL((rs_scatter_kernel<K, true>), dim3(nb), dim3(kRsThreads), 0, s, ki, vi, wide_l,
+                                   wide_r, n, d_n, shift, dmask, hist, nb);
+                VS_LAUNCH_CHECK();
+                res->keys = nullptr;
+                res->vals = vi;  // read by this pass, free once it ran
+                res->alt_keys = ko;
+                res->alt_vals = vo;
+                return VS_OK;
+            }
+            hipLaunchKernelGGL((rs_scatter_kernel<K, false>), dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n,
+                               d_n, shift, dmask, hist, nb);
             VS_LAUNCH_CHECK();
             K* tk = ki; ki = ko; ko = tk;
             int32_t* tv = vi; vi = vo; vo = tv;
@@ -412,8 +435,8 @@ int sort_pairs_into(const K* in_keys, const int32_t* in_vals, K* out_keys, int32
         hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, n, d_n, shift, dmask, hist, nb);
         VS_LAUNCH_CHECK();
         VS_TRY(exclusive_scan_i32(hist, hist, (int64_t)nb * kRsBins, nullptr, scan_ws, s));
-        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n, d_n, shift,
-                           dmask, hist, nb);
+        hipLaunchKernelGGL((rs_scatter_kernel<K, false>), dim3(nb), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n, d_n,
+                           shift, dmask, hist, nb);
         VS_LAUNCH_CHECK();
         ki = ko;
         vi = vo;
@@ -455,6 +478,11 @@ int sort_pairs_into_u64(const uint64_t* ik, const int32_t* iv, uint64_t* ok, int
 int sort_pairs_u32_dev(uint32_t* k, int32_t* v, int64_t n, const unsigned long long* d_n, int b0, int b1, void* ws,
                        int64_t wb, hipStream_t s, SortResult<uint32_t>* res) {
     return sort_pairs_nocopy<uint32_t>(k, v, n, d_n, b0, b1, ws, wb, s, res);
+}
+int sort_pairs_u32_dev_wide(uint32_t* k, int32_t* v, int64_t n, const unsigned long long* d_n, int b0, int b1,
+                            void* ws, int64_t wb, hipStream_t s, int64_t* out_k, int64_t* out_v,
+                            SortResult<uint32_t>* res) {
+    return sort_pairs_nocopy<uint32_t>(k, v, n, d_n, b0, b1, ws, wb, s, res, out_k, out_v);
 }
 
 }  // namespace vs
