@@ -142,6 +142,33 @@ def test_query_boxes_capacity_retry():
     assert np.array_equal(l.cpu().numpy(), el) and np.array_equal(r.cpu().numpy(), er)
 
 
+def test_query_boxes_every_run_length_is_sorted():
+    """Runs of one left id of every length 1..40 (sorted in registers up to 8,
+    in place up to 32, by the long-run pass beyond), the right ids arriving
+    in tree order rather than id order: unit boxes on a line at shuffled
+    positions, each query box covering a stretch of exactly L of them."""
+    print("query_boxes run lengths 1..40", flush=True)
+    rng = np.random.default_rng(7)
+    n_t = 2000
+    pos = rng.permutation(n_t).astype(np.float64)
+    tb = np.stack([pos * 10, np.zeros(n_t), pos * 10 + 1, np.ones(n_t)], axis=1)
+    tree = J.build_tree(torch.from_numpy(tb).to(_dev()))
+    lens = np.repeat(np.arange(1, 41), 3)
+    rng.shuffle(lens)
+    starts = rng.integers(0, n_t - 40, lens.size)
+    qb = np.stack([starts * 10 + 0.5, np.full(lens.size, 0.25), (starts + lens - 1) * 10 + 0.5,
+                   np.full(lens.size, 0.75)], axis=1)
+    l, r, st = J.query_boxes(torch.from_numpy(qb).to(_dev()), tree)
+    l, r = l.cpu().numpy(), r.cpu().numpy()
+    assert np.array_equal(np.bincount(l, minlength=lens.size), lens)
+    order = np.argsort(pos)  # box id at each position
+    for q in range(lens.size):
+        s0 = starts[q]
+        assert np.array_equal(r[l == q], np.sort(order[s0:s0 + lens[q]]))
+    el, er = oracle.bbox_pairs(qb, tb)
+    assert np.array_equal(l, el) and np.array_equal(r, er)
+
+
 @pytest.mark.parametrize("n_t", [30_000, 70_001])
 def test_query_boxes_long_runs_are_sorted(n_t):
     """Queries with many hits: runs of one left id longer than the short-run

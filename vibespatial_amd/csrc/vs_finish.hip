@@ -11,8 +11,9 @@
 //      writes out_l / out_r as int64 directly.
 //   2. runs: with sort_right (the default, lexsorted output) the first slot
 //      of each left id's run of two or more sorts that run's right ids in
-//      out_r (insertion sort, runs of at most kShortRun).  Longer runs are
-//      listed.  (With no sort pass -- at most one left row -- an emit pass
+//      out_r, in registers (from an LDS tile loaded and stored coalesced):
+//      runs of at most kNetRun by a 19-comparator network, runs of at most
+//      kShortRun by a bitonic network.  Longer runs are listed.  (With no sort pass -- at most one left row -- an emit pass
 //      converts and sorts instead.)
 //   3. long runs: one block per listed run; runs up to kRunTile are bitonic
 //      sorted in LDS, longer ones are sorted in kRunTile chunks and merged in
@@ -29,6 +30,9 @@
 namespace vs {
 
 constexpr int kShortRun = 32;
+constexpr int kNetRun = 8;  // runs sorted in registers by fix_runs_kernel
+static_assert(kNetRun < kShortRun && (kShortRun & (kShortRun - 1)) == 0, "bitonic runs: a power of two");
+constexpr int kFixTile = 2048;  // slots per fix_runs_kernel tile
 constexpr int kRunTile = 4096;  // int32 elements sorted in LDS by one block (16 KB)
 constexpr int kRunThreads = 256;
 
@@ -100,37 +104,96 @@ __global__ __launch_bounds__(256) void emit_pairs_kernel(const uint32_t* __restr
 }
 
 // After the sort's last pass wrote out_l / out_r: sorts each short run of
-// equal left ids by right id in out_r and lists the long runs.
+// equal left ids by right id in out_r and lists the long runs.  One block per
+// tile of kFixTile slots: the keys (low words) of slots t0 - 1 .. t1 + kShortRun
+// and the right ids of t0 .. t1 + kShortRun - 1 are loaded coalesced into LDS;
+// each run starting in the tile is sorted in registers by its first slot's
+// thread and written back to LDS (its part past t1 straight to out_r); the
+// tile is then stored coalesced, except its leading slots that continue a run
+// begun before t0 (the previous tile's block owns that run).
 __global__ __launch_bounds__(256) void fix_runs_kernel(const int64_t* __restrict__ out_l,
                                                       int64_t* __restrict__ out_r,
                                                       const JoinCtl* __restrict__ ctl, int64_t capacity,
                                                       int32_t* __restrict__ long_runs,
                                                       unsigned int* __restrict__ n_long) {
+    constexpr uint32_t kNoKey = 0xFFFFFFFFu;  // left ids are below it (checked on the host)
+    __shared__ uint32_t s_key[kFixTile + kShortRun + 2];  // slot t0 - 1 + x
+    __shared__ int32_t s_val[kFixTile + kShortRun];       // slot t0 + x
     const unsigned long long staged = ctl->staged;
     if (staged > (unsigned long long)capacity) return;
     const int64_t n = (int64_t)staged;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int64_t k = out_l[i];
-        const int64_t kp = i > 0 ? out_l[i - 1] : ~k;
-        const int64_t kn = i + 1 < n ? out_l[i + 1] : ~k;
-        if (kp == k || kn != k) continue;  // not the first slot of a run of two or more
-        int64_t e = i + 2;
-        while (e < n && e - i <= kShortRun && out_l[e] == k) ++e;
-        const int len = (int)(e - i);
-        if (len > kShortRun) {
-            long_runs[atomicAdd(n_long, 1u)] = (int32_t)i;
-            continue;
+    for (int64_t t0 = (int64_t)blockIdx.x * kFixTile; t0 < n; t0 += (int64_t)gridDim.x * kFixTile) {
+        for (int x = threadIdx.x; x < kFixTile + kShortRun + 2; x += 256) {
+            const int64_t p = t0 - 1 + x;
+            s_key[x] = p >= 0 && p < n ? (uint32_t)out_l[p] : kNoKey;
         }
-        int64_t* r = out_r + i;
-        for (int a = 1; a < len; ++a) {
-            const int64_t v = r[a];
-            int b = a - 1;
-            while (b >= 0 && r[b] > v) {
-                r[b + 1] = r[b];
-                --b;
+        for (int x = threadIdx.x; x < kFixTile + kShortRun; x += 256) {
+            const int64_t p = t0 + x;
+            s_val[x] = p < n ? (int32_t)out_r[p] : 0;
+        }
+        __syncthreads();
+        for (int x = threadIdx.x; x < kFixTile && t0 + x < n; x += 256) {
+            const uint32_t k = s_key[x + 1];
+            if (s_key[x] == k || s_key[x + 2] != k) continue;  // not the first slot of a run of two or more
+            int32_t w[kShortRun];
+            int len = 2;
+#pragma unroll
+            for (int j = 2; j <= kNetRun; ++j) len += (len == j && s_key[x + 1 + j] == k);
+#pragma unroll
+            for (int j = 0; j < kNetRun; ++j) w[j] = s_val[x + j];
+            if (len <= kNetRun) {
+#pragma unroll
+                for (int j = 2; j < kNetRun; ++j) w[j] = j < len ? w[j] : INT32_MAX;
+                // Optimal 8-input network, depth 6 (Knuth, TAOCP vol. 3, 5.3.4).
+                constexpr int kNet[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
+                                             {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
+                                             {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+                for (int c = 0; c < 19; ++c) {
+                    const int32_t a = w[kNet[c][0]], b = w[kNet[c][1]];
+                    w[kNet[c][0]] = min(a, b);
+                    w[kNet[c][1]] = max(a, b);
+                }
+            } else {
+                // len == kNetRun + 1 so far: slots x .. x + kNetRun hold k.
+#pragma unroll
+                for (int j = kNetRun + 1; j <= kShortRun; ++j) len += (len == j && s_key[x + 1 + j] == k);
+                if (len > kShortRun) {
+                    long_runs[atomicAdd(n_long, 1u)] = (int32_t)(t0 + x);
+                    continue;
+                }
+#pragma unroll
+                for (int j = kNetRun; j < kShortRun; ++j) w[j] = j < len ? s_val[x + j] : INT32_MAX;
+                // Bitonic network over kShortRun values, every index fixed at compile time.
+#pragma unroll
+                for (int size = 2; size <= kShortRun; size <<= 1) {
+#pragma unroll
+                    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+                        for (int u = 0; u < kShortRun; ++u) {
+                            const int v = u ^ stride;
+                            if (v > u) {
+                                const int32_t a = w[u], b = w[v];
+                                const bool up = (u & size) == 0;
+                                w[u] = up ? min(a, b) : max(a, b);
+                                w[v] = up ? max(a, b) : min(a, b);
+                            }
+                        }
+                    }
+                }
             }
-            r[b + 1] = v;
+#pragma unroll
+            for (int j = 0; j < kShortRun; ++j) {
+                if (j < len) {
+                    if (x + j < kFixTile) s_val[x + j] = w[j];
+                    else out_r[t0 + x + j] = w[j];
+                }
+            }
         }
+        __syncthreads();
+        for (int x = threadIdx.x; x < kFixTile && t0 + x < n; x += 256)
+            if (s_key[x + 1] != s_key[0]) out_r[t0 + x] = s_val[x];
+        __syncthreads();
     }
 }
 
@@ -274,6 +337,8 @@ int finish_pairs(void* fin_ws, JoinCtl* ctl, int64_t n_left, int64_t capacity, b
                  int64_t* out_r, int64_t* d_result, hipStream_t s) {
     VS_REQUIRE(capacity >= 0 && capacity < (int64_t)INT32_MAX, "join capacity %lld out of range [0, 2^31-1)",
                (long long)capacity);
+    VS_REQUIRE(n_left >= 0 && n_left < (int64_t)UINT32_MAX, "left row count %lld out of range [0, 2^32-1)",
+               (long long)n_left);
     Carver c(fin_ws);
     const FinishBufs F = carve_finish(c, capacity);
     const int nbits = bits_for(n_left);
@@ -282,7 +347,8 @@ int finish_pairs(void* fin_ws, JoinCtl* ctl, int64_t n_left, int64_t capacity, b
         VS_TRY(sort_pairs_u32_dev_wide(F.stage_l, F.stage_r, capacity, &ctl->staged, 0, nbits, F.sort_ws,
                                        sort_workspace_bytes_u32(capacity), s, out_l, out_r, &r));
         if (sort_right) {
-            const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, 256), 65536));
+            const unsigned grid =
+                (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(capacity, kFixTile), 65536));
             hipLaunchKernelGGL(fix_runs_kernel, dim3(grid), dim3(256), 0, s, out_l, out_r, ctl, capacity,
                                F.long_runs, &ctl->n_long);
             VS_LAUNCH_CHECK();
