@@ -32,7 +32,7 @@ namespace vs {
 constexpr int kShortRun = 32;
 constexpr int kNetRun = 8;  // runs sorted in registers by fix_runs_kernel
 static_assert(kNetRun < kShortRun && (kShortRun & (kShortRun - 1)) == 0, "bitonic runs: a power of two");
-constexpr int kFixTile = 2048;  // slots per fix_runs_kernel tile
+constexpr int kFixTile = 1024;  // slots per fix_runs_kernel tile
 constexpr int kRunTile = 4096;  // int32 elements sorted in LDS by one block (16 KB)
 constexpr int kRunThreads = 256;
 
