@@ -123,13 +123,32 @@ __global__ __launch_bounds__(256) void fix_runs_kernel(const int64_t* __restrict
     if (staged > (unsigned long long)capacity) return;
     const int64_t n = (int64_t)staged;
     for (int64_t t0 = (int64_t)blockIdx.x * kFixTile; t0 < n; t0 += (int64_t)gridDim.x * kFixTile) {
-        for (int x = threadIdx.x; x < kFixTile + kShortRun + 2; x += 256) {
+        // Every load of the tile in flight at once, then the LDS stores.
+        constexpr int kKeyIt = (kFixTile + kShortRun + 2 + 255) / 256;
+        constexpr int kValIt = (kFixTile + kShortRun + 255) / 256;
+        uint32_t kr[kKeyIt];
+        int32_t vr[kValIt];
+#pragma unroll
+        for (int m = 0; m < kKeyIt; ++m) {
+            const int x = threadIdx.x + m * 256;
             const int64_t p = t0 - 1 + x;
-            s_key[x] = p >= 0 && p < n ? (uint32_t)out_l[p] : kNoKey;
+            kr[m] = x < kFixTile + kShortRun + 2 && p >= 0 && p < n ? (uint32_t)out_l[p] : kNoKey;
         }
-        for (int x = threadIdx.x; x < kFixTile + kShortRun; x += 256) {
+#pragma unroll
+        for (int m = 0; m < kValIt; ++m) {
+            const int x = threadIdx.x + m * 256;
             const int64_t p = t0 + x;
-            s_val[x] = p < n ? (int32_t)out_r[p] : 0;
+            vr[m] = x < kFixTile + kShortRun && p < n ? (int32_t)out_r[p] : 0;
+        }
+#pragma unroll
+        for (int m = 0; m < kKeyIt; ++m) {
+            const int x = threadIdx.x + m * 256;
+            if (x < kFixTile + kShortRun + 2) s_key[x] = kr[m];
+        }
+#pragma unroll
+        for (int m = 0; m < kValIt; ++m) {
+            const int x = threadIdx.x + m * 256;
+            if (x < kFixTile + kShortRun) s_val[x] = vr[m];
         }
         __syncthreads();
         for (int x = threadIdx.x; x < kFixTile && t0 + x < n; x += 256) {
