@@ -210,10 +210,16 @@ __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const K* __restrict
     for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads) (&cnt[0][0])[i] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kRsTile;
-#pragma unroll 4
+    K kk[kRsItems];
+#pragma unroll
     for (int i = 0; i < kRsItems; ++i) {
         const int64_t idx = base + (int64_t)i * kRsThreads + threadIdx.x;
-        if (idx < n) atomicAdd(&cnt[w][(int)((keys[idx] >> shift) & dmask)], 1);
+        kk[i] = idx < n ? keys[idx] : (K)0;
+    }
+#pragma unroll
+    for (int i = 0; i < kRsItems; ++i) {
+        const int64_t idx = base + (int64_t)i * kRsThreads + threadIdx.x;
+        if (idx < n) atomicAdd(&cnt[w][(int)((kk[i] >> shift) & dmask)], 1);
     }
     __syncthreads();
     const int d = threadIdx.x;
@@ -249,12 +255,19 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(
     K k[kRsItems];
     int32_t v[kRsItems];
     int rank[kRsItems];
+    // every load of the tile in flight before the ranking (its wave fences
+    // would otherwise order each load after the previous item's ranking)
 #pragma unroll
     for (int i = 0; i < kRsItems; ++i) {
         const int64_t idx = base + (int64_t)i * 64 + lane;
         const bool valid = idx < n;
         k[i] = valid ? keys_in[idx] : (K)0;
         v[i] = valid ? (vals_in ? vals_in[idx] : (int32_t)idx) : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kRsItems; ++i) {
+        const int64_t idx = base + (int64_t)i * 64 + lane;
+        const bool valid = idx < n;
         const int d = (int)((k[i] >> shift) & dmask);
         uint64_t m = __ballot(valid);
 #pragma unroll
