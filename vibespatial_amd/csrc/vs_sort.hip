@@ -216,10 +216,23 @@ __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const K* __restrict
         const int64_t idx = base + (int64_t)i * kRsThreads + threadIdx.x;
         kk[i] = idx < n ? keys[idx] : (K)0;
     }
+    // A wave whose keys share one digit adds them with one LDS atomic (a
+    // join's staged keys arrive in long runs of one value, whose lanes would
+    // serialise on one bin); other waves add per lane.  Valid lanes are a
+    // prefix of the wave, so lane 0 is valid whenever any lane is.
 #pragma unroll
     for (int i = 0; i < kRsItems; ++i) {
         const int64_t idx = base + (int64_t)i * kRsThreads + threadIdx.x;
-        if (idx < n) atomicAdd(&cnt[w][(int)((kk[i] >> shift) & dmask)], 1);
+        const bool valid = idx < n;
+        const int d = (int)((kk[i] >> shift) & dmask);
+        const int d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint64_t vm = __ballot(valid);
+        const uint64_t same = __ballot(valid && d == d0);
+        if (same == vm) {
+            if ((threadIdx.x & 63) == 0 && vm) atomicAdd(&cnt[w][d0], __popcll(vm));
+        } else if (valid) {
+            atomicAdd(&cnt[w][d], 1);
+        }
     }
     __syncthreads();
     const int d = threadIdx.x;
