@@ -88,25 +88,57 @@ __global__ __launch_bounds__(1024) void scan_partials_kernel(long long* __restri
 }
 
 // Downsweep: thread t owns 16 consecutive elements of the tile.
+// A thread's kScanItems consecutive ints, loaded / stored as 16-byte
+// accesses where its run is whole and aligned (else element by element).
+__device__ __forceinline__ void scan_load(const int32_t* in, int64_t base, int64_t n, int (&v)[kScanItems]) {
+    if (base + kScanItems <= n && (reinterpret_cast<uintptr_t>(in + base) & 15) == 0) {
+#pragma unroll
+        for (int j = 0; j < kScanItems / 4; ++j) {
+            const int4 q = reinterpret_cast<const int4*>(in + base)[j];
+            v[4 * j] = q.x;
+            v[4 * j + 1] = q.y;
+            v[4 * j + 2] = q.z;
+            v[4 * j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) v[i] = (base + i < n) ? in[base + i] : 0;
+    }
+}
+
+// Writes the exclusive prefix of v starting at run.
+__device__ __forceinline__ void scan_store(int32_t* out, int64_t base, int64_t n, const int (&v)[kScanItems],
+                                           long long run) {
+    int o[kScanItems];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        o[i] = (int32_t)run;
+        run += v[i];
+    }
+    if (base + kScanItems <= n && (reinterpret_cast<uintptr_t>(out + base) & 15) == 0) {
+#pragma unroll
+        for (int j = 0; j < kScanItems / 4; ++j)
+            reinterpret_cast<int4*>(out + base)[j] = make_int4(o[4 * j], o[4 * j + 1], o[4 * j + 2], o[4 * j + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i)
+            if (base + i < n) out[base + i] = o[i];
+    }
+}
+
 __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const int32_t* __restrict__ in,
                                                                 int32_t* __restrict__ out, int64_t n,
                                                                 const long long* __restrict__ part) {
     __shared__ long long lds[4];
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     int v[kScanItems];
+    scan_load(in, base, n, v);
     long long s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        v[i] = (base + i < n) ? in[base + i] : 0;
-        s += v[i];
-    }
+    for (int i = 0; i < kScanItems; ++i) s += v[i];
     long long tot;
-    long long run = block_exclusive_scan_i64(s, lds, tot) + part[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < n) out[base + i] = (int32_t)run;
-        run += v[i];
-    }
+    const long long run = block_exclusive_scan_i64(s, lds, tot) + part[blockIdx.x];
+    scan_store(out, base, n, v, run);
 }
 
 // Downsweep that forms its own tile prefix from the tile sums (no separate
@@ -125,19 +157,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_fused_kernel(const int
     block_exclusive_scan_i64(p, lds, prefix);
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     int v[kScanItems];
+    scan_load(in, base, n, v);
     long long sum = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        v[i] = (base + i < n) ? in[base + i] : 0;
-        sum += v[i];
-    }
+    for (int i = 0; i < kScanItems; ++i) sum += v[i];
     long long tot;
-    long long run = block_exclusive_scan_i64(sum, lds, tot) + prefix;
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < n) out[base + i] = (int32_t)run;
-        run += v[i];
-    }
+    const long long run = block_exclusive_scan_i64(sum, lds, tot) + prefix;
+    scan_store(out, base, n, v, run);
     if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = prefix + tot;
 }
 
